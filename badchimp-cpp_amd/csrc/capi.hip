@@ -1,0 +1,646 @@
+// C ABI: context, topology upload (pull table), field transfer, steps.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/badchimp_amd.h"
+#include "context.hpp"
+#include "errors.hpp"
+#include "kernels.hpp"
+#include "kernels_f32.hpp"
+#include "lattice.hpp"
+
+using namespace lbamd;
+
+namespace {
+
+int fail(lb_ctx* c, int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  c->err = vformat(fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define LB_HIP(c, expr)                                                              \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return fail((c), LB_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));   \
+  } while (0)
+
+int rev_of(int lattice, int q) { return lattice == LB_D2Q9 ? D2Q9::rev(q) : D3Q19::rev(q); }
+double w_of(int lattice, int q) { return lattice == LB_D2Q9 ? D2Q9::w(q) : D3Q19::w(q); }
+
+size_t elem_size(int precision) {
+  return precision == LB_FP64 ? 8 : (precision == LB_FP32S ? 4 : 2);
+}
+
+template <typename T>
+int dev_alloc(lb_ctx* c, T** p, size_t count) {
+  if (*p) {
+    hipFree(*p);
+    *p = nullptr;
+  }
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+  if (e != hipSuccess) return fail(c, LB_ERR_NOMEM, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+  c->bytes += static_cast<int64_t>(count * sizeof(T));
+  return LB_OK;
+}
+
+void free_fields(lb_ctx* c) {
+  for (auto& p : c->d_f) {
+    if (p) hipFree(p);
+    p = nullptr;
+  }
+  c->n_fields = 0;
+}
+
+int alloc_fields(lb_ctx* c, int nf) {
+  if (c->n_fields == nf) return LB_OK;
+  free_fields(c);
+  const size_t bytes = static_cast<size_t>(nf) * c->Q * c->NP * elem_size(c->precision);
+  for (auto& p : c->d_f) {
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return fail(c, LB_ERR_NOMEM, "hipMalloc(%zu bytes) for populations: %s", bytes, hipGetErrorString(e));
+    c->bytes += static_cast<int64_t>(bytes);
+  }
+  c->n_fields = nf;
+  c->cur = 0;
+  c->post = false;
+  return LB_OK;
+}
+
+int check_ready(lb_ctx* c, int nf) {
+  if (!c->has_topo) return fail(c, LB_ERR_STATE, "no topology: call lb_set_topology first");
+  if (c->n_fields != nf)
+    return fail(c, LB_ERR_STATE, "populations hold %d field(s), %d needed: upload or initialise first", c->n_fields, nf);
+  return LB_OK;
+}
+
+// Time a sequence of launches on the context stream.
+struct Timer {
+  lb_ctx* c;
+  explicit Timer(lb_ctx* ctx) : c(ctx) { hipEventRecord(c->ev0, c->stream); }
+  int stop(int64_t n_launch) {
+    hipEventRecord(c->ev1, c->stream);
+    hipError_t e = hipEventSynchronize(c->ev1);
+    if (e != hipSuccess) return fail(c, LB_ERR_HIP, "kernel execution failed: %s", hipGetErrorString(e));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    c->kernel_ms += ms;
+    c->launches += n_launch;
+    return LB_OK;
+  }
+};
+
+// Ensure d_f[cur] holds post-collision values or pre-collision ones; for
+// transfers we need the pre-collision state in some buffer: returns it.
+int pre_state(lb_ctx* c, const void** out) {
+  if (!c->post) {
+    *out = c->d_f[c->cur];
+    return LB_OK;
+  }
+  void* dst = c->d_f[1 - c->cur];
+  hipError_t e;
+  if (c->precision == LB_FP64)
+    e = gather_f64(c->lattice, c->n_fields, static_cast<const double*>(c->d_f[c->cur]),
+                   static_cast<double*>(dst), c->d_link, c->n_bulk, c->NP, c->stream);
+  else
+    e = gather_lowp(c->precision, c->lattice, c->n_fields, c->d_f[c->cur], dst, c->d_link,
+                    c->n_bulk, c->NP, c->stream);
+  if (e != hipSuccess) return fail(c, LB_ERR_HIP, "gather: %s", hipGetErrorString(e));
+  *out = dst;
+  return LB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lb_version(void) { return "badchimp-amd 0.1 (gfx950)"; }
+
+const char* lb_last_error(const lb_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int lb_ctx_create(int lattice, int precision, int device, lb_ctx** out) {
+  if (!out) return fail_global(LB_ERR_ARG, "lb_ctx_create: null output");
+  if (lattice != LB_D2Q9 && lattice != LB_D3Q19) return fail_global(LB_ERR_ARG, "unknown lattice %d", lattice);
+  if (precision != LB_FP64 && precision != LB_FP32S && precision != LB_FP16S)
+    return fail_global(LB_ERR_ARG, "unknown precision %d", precision);
+  int n_dev = 0;
+  hipError_t e = hipGetDeviceCount(&n_dev);
+  if (e != hipSuccess || n_dev == 0) return fail_global(LB_ERR_HIP, "no HIP device available");
+  if (device < 0 || device >= n_dev) return fail_global(LB_ERR_ARG, "device %d out of range (%d devices)", device, n_dev);
+  e = hipSetDevice(device);
+  if (e != hipSuccess) return fail_global(LB_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(e));
+  lb_ctx* c = new (std::nothrow) lb_ctx();
+  if (!c) return fail_global(LB_ERR_NOMEM, "out of host memory");
+  c->lattice = lattice;
+  c->precision = precision;
+  c->device = device;
+  c->Q = lattice == LB_D2Q9 ? 9 : 19;
+  c->nD = lattice == LB_D2Q9 ? 2 : 3;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    delete c;
+    return fail_global(LB_ERR_HIP, "stream/event creation failed");
+  }
+  *out = c;
+  return LB_OK;
+}
+
+int lb_ctx_destroy(lb_ctx* c) {
+  if (!c) return LB_OK;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  free_fields(c);
+  for (void* p : {static_cast<void*>(c->d_link), static_cast<void*>(c->d_rho), static_cast<void*>(c->d_vel),
+                  static_cast<void*>(c->d_phi), static_cast<void*>(c->d_phi_wall), static_cast<void*>(c->d_partial),
+                  static_cast<void*>(c->d_scalar)})
+    if (p) hipFree(p);
+  hipEventDestroy(c->ev0);
+  hipEventDestroy(c->ev1);
+  hipStreamDestroy(c->stream);
+  delete c;
+  return LB_OK;
+}
+
+int lb_set_topology(lb_ctx* c, int32_t n_nodes, const int32_t* neighbor, const int16_t* node_type,
+                    const int32_t* bulk, int32_t n_bulk, int32_t flags) {
+  if (!c) return LB_ERR_ARG;
+  if (n_nodes < 1 || n_bulk < 0 || !neighbor || !node_type || (n_bulk > 0 && !bulk))
+    return fail(c, LB_ERR_ARG, "lb_set_topology: bad argument");
+  if (n_bulk >= static_cast<int32_t>(kLinkIndexMask)) return fail(c, LB_ERR_ARG, "too many bulk nodes");
+  hipSetDevice(c->device);
+  const int Q = c->Q;
+  std::vector<int32_t> compact(n_nodes, -1);
+  for (int32_t i = 0; i < n_bulk; ++i) {
+    const int32_t n = bulk[i];
+    if (n <= 0 || n >= n_nodes) return fail(c, LB_ERR_ARG, "bulk node %d out of range", n);
+    if (i > 0 && bulk[i - 1] >= n) return fail(c, LB_ERR_ARG, "bulk list must be strictly ascending");
+    if (node_type[n] < 2) return fail(c, LB_ERR_ARG, "bulk node %d is not fluid (type %d)", n, node_type[n]);
+    compact[n] = i;
+  }
+  // The push of the reference writes f_post(p, m) into slot (p, neighbor(p,m)).
+  // For node 0 the slot is shared: the last bulk node (ascending) wins.
+  std::vector<int32_t> winner(Q, -1);
+  for (int32_t i = 0; i < n_bulk; ++i) {
+    const int32_t* nb = neighbor + static_cast<size_t>(bulk[i]) * Q;
+    for (int p = 0; p < Q - 1; ++p)
+      if (nb[p] == 0) winner[p] = i;
+  }
+  const int64_t NP = ((static_cast<int64_t>(n_bulk) + kBlock - 1) / kBlock) * kBlock + (n_bulk == 0 ? kBlock : 0);
+  std::vector<uint32_t> link(static_cast<size_t>(Q - 1) * NP, 0u);
+  std::vector<int32_t> wall_slot(n_nodes, -1);
+  std::vector<int32_t> walls;
+  for (int32_t i = 0; i < n_bulk; ++i) {
+    const int32_t n = bulk[i];
+    const int32_t* nb = neighbor + static_cast<size_t>(n) * Q;
+    for (int q = 0; q < Q - 1; ++q) {
+      const int32_t s = nb[rev_of(c->lattice, q)];
+      if (s < 0 || s >= n_nodes) return fail(c, LB_ERR_ARG, "neighbour of node %d out of range", n);
+      uint32_t l;
+      if (compact[s] >= 0) {
+        l = (kLinkStream << 30) | static_cast<uint32_t>(compact[s]);
+      } else if (node_type[s] > 1) {
+        return fail(c, LB_ERR_UNSUPPORTED,
+                    "bulk node %d has a fluid neighbour %d on another rank: halo exchange is not available on one GPU", n, s);
+      } else if (s == 0) {
+        if (!(flags & LB_TOPO_ALLOW_GHOST0))
+          return fail(c, LB_ERR_GHOST0_LINK,
+                      "bulk node %d links to the ghost node 0 (direction %d); make the geometry periodic or walled, "
+                      "or pass LB_TOPO_ALLOW_GHOST0 to reproduce the reference's shared-slot result", n, q);
+        l = (kLinkGhost0 << 30) | static_cast<uint32_t>(winner[rev_of(c->lattice, q)]);
+      } else {
+        if (wall_slot[s] < 0) {
+          wall_slot[s] = static_cast<int32_t>(walls.size());
+          walls.push_back(s);
+        }
+        l = (kLinkBounce << 30) | static_cast<uint32_t>(wall_slot[s]);
+      }
+      link[static_cast<size_t>(q) * NP + i] = l;
+    }
+  }
+  // device allocation
+  hipStreamSynchronize(c->stream);
+  free_fields(c);
+  c->bytes = 0;
+  int rc;
+  if ((rc = dev_alloc(c, &c->d_link, link.size())) != LB_OK) return rc;
+  LB_HIP(c, hipMemcpy(c->d_link, link.data(), link.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  c->n_blocks = (n_bulk + kBlock - 1) / kBlock;
+  if (c->n_blocks == 0) c->n_blocks = 1;
+  if ((rc = dev_alloc(c, &c->d_rho, 2 * NP)) != LB_OK) return rc;
+  if ((rc = dev_alloc(c, &c->d_vel, static_cast<size_t>(c->nD) * NP)) != LB_OK) return rc;
+  if ((rc = dev_alloc(c, &c->d_phi, NP)) != LB_OK) return rc;
+  if ((rc = dev_alloc(c, &c->d_phi_wall, walls.size() + 1)) != LB_OK) return rc;
+  if ((rc = dev_alloc(c, &c->d_partial, 3 * c->n_blocks)) != LB_OK) return rc;
+  if ((rc = dev_alloc(c, &c->d_scalar, 8)) != LB_OK) return rc;
+  LB_HIP(c, hipMemset(c->d_phi_wall, 0, (walls.size() + 1) * sizeof(double)));
+  LB_HIP(c, hipMemset(c->d_rho, 0, 2 * NP * sizeof(double)));
+  LB_HIP(c, hipMemset(c->d_vel, 0, c->nD * NP * sizeof(double)));
+  LB_HIP(c, hipMemset(c->d_phi, 0, NP * sizeof(double)));
+  LB_HIP(c, hipMemset(c->d_scalar, 0, 8 * sizeof(double)));
+  c->n_nodes = n_nodes;
+  c->n_bulk = n_bulk;
+  c->NP = NP;
+  c->n_global = n_bulk;
+  c->bulk.assign(bulk, bulk + n_bulk);
+  c->wall_nodes = walls;
+  c->node_type.assign(node_type, node_type + n_nodes);
+  c->has_topo = true;
+  c->have_macro = false;
+  return LB_OK;
+}
+
+int lb_set_global_bulk_count(lb_ctx* c, int64_t n) {
+  if (!c) return LB_ERR_ARG;
+  if (n <= 0) return fail(c, LB_ERR_ARG, "global bulk count must be positive");
+  c->n_global = n;
+  return LB_OK;
+}
+
+// ------------------------------------------------------------ fields
+int lb_upload_f(lb_ctx* c, int32_t nf, const double* f) {
+  if (!c) return LB_ERR_ARG;
+  if (!c->has_topo) return fail(c, LB_ERR_STATE, "no topology");
+  if ((nf != 1 && nf != 2) || !f) return fail(c, LB_ERR_ARG, "lb_upload_f: bad argument");
+  hipSetDevice(c->device);
+  int rc = alloc_fields(c, nf);
+  if (rc != LB_OK) return rc;
+  const int Q = c->Q;
+  const int64_t NP = c->NP;
+  const size_t count = static_cast<size_t>(nf) * Q * NP;
+  if (c->precision == LB_FP64) {
+    std::vector<double> h(count, 0.0);
+    for (int32_t i = 0; i < c->n_bulk; ++i) {
+      const double* src = f + static_cast<size_t>(c->bulk[i]) * nf * Q;
+      for (int k = 0; k < nf; ++k)
+        for (int q = 0; q < Q; ++q) h[(static_cast<size_t>(k) * Q + q) * NP + i] = src[k * Q + q];
+    }
+    LB_HIP(c, hipMemcpy(c->d_f[0], h.data(), count * sizeof(double), hipMemcpyHostToDevice));
+  } else {
+    std::vector<double> h(count, 0.0);
+    for (int32_t i = 0; i < c->n_bulk; ++i) {
+      const double* src = f + static_cast<size_t>(c->bulk[i]) * nf * Q;
+      for (int k = 0; k < nf; ++k)
+        for (int q = 0; q < Q; ++q) h[(static_cast<size_t>(k) * Q + q) * NP + i] = src[k * Q + q];
+    }
+    double* tmp = nullptr;
+    LB_HIP(c, hipMalloc(&tmp, count * sizeof(double)));
+    hipError_t e = hipMemcpy(tmp, h.data(), count * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = store_lowp(c->precision, c->lattice, nf, tmp, c->d_f[0], c->n_bulk, NP, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(tmp);
+    if (e != hipSuccess) return fail(c, LB_ERR_HIP, "upload conversion: %s", hipGetErrorString(e));
+  }
+  c->cur = 0;
+  c->post = false;
+  c->have_macro = false;
+  return LB_OK;
+}
+
+int lb_download_f(lb_ctx* c, int32_t nf, double* f) {
+  if (!c) return LB_ERR_ARG;
+  if (!f) return fail(c, LB_ERR_ARG, "null output");
+  int rc = check_ready(c, nf);
+  if (rc != LB_OK) return rc;
+  hipSetDevice(c->device);
+  const void* src = nullptr;
+  if ((rc = pre_state(c, &src)) != LB_OK) return rc;
+  const int Q = c->Q;
+  const int64_t NP = c->NP;
+  const size_t count = static_cast<size_t>(nf) * Q * NP;
+  std::vector<double> h(count);
+  if (c->precision == LB_FP64) {
+    LB_HIP(c, hipMemcpyAsync(h.data(), src, count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    LB_HIP(c, hipStreamSynchronize(c->stream));
+  } else {
+    double* tmp = nullptr;
+    LB_HIP(c, hipMalloc(&tmp, count * sizeof(double)));
+    hipError_t e = load_lowp(c->precision, c->lattice, nf, src, tmp, c->n_bulk, NP, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), tmp, count * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(tmp);
+    if (e != hipSuccess) return fail(c, LB_ERR_HIP, "download conversion: %s", hipGetErrorString(e));
+  }
+  for (int32_t i = 0; i < c->n_bulk; ++i) {
+    double* dst = f + static_cast<size_t>(c->bulk[i]) * nf * Q;
+    for (int k = 0; k < nf; ++k)
+      for (int q = 0; q < Q; ++q) dst[k * Q + q] = h[(static_cast<size_t>(k) * Q + q) * NP + i];
+  }
+  return LB_OK;
+}
+
+int lb_init_equilibrium(lb_ctx* c, int32_t n_fields, int32_t field, int32_t rho_nfields,
+                        int32_t rho_field, const double* rho, const double* vel) {
+  if (!c) return LB_ERR_ARG;
+  if (!c->has_topo) return fail(c, LB_ERR_STATE, "no topology");
+  if ((n_fields != 1 && n_fields != 2) || field < 0 || field >= n_fields || rho_nfields < 1 ||
+      rho_field < 0 || rho_field >= rho_nfields || !rho || !vel)
+    return fail(c, LB_ERR_ARG, "lb_init_equilibrium: bad argument");
+  hipSetDevice(c->device);
+  if (c->n_fields != n_fields || c->post) {
+    // (re)start from an unset state: all fields are to be initialised
+    free_fields(c);
+    int rc = alloc_fields(c, n_fields);
+    if (rc != LB_OK) return rc;
+  }
+  const int64_t NP = c->NP;
+  std::vector<double> hr(NP, 0.0), hv(static_cast<size_t>(c->nD) * NP, 0.0);
+  for (int32_t i = 0; i < c->n_bulk; ++i) {
+    const int32_t n = c->bulk[i];
+    hr[i] = rho[static_cast<size_t>(n) * rho_nfields + rho_field];
+    for (int d = 0; d < c->nD; ++d) hv[d * NP + i] = vel[static_cast<size_t>(n) * c->nD + d];
+  }
+  double *dr = nullptr, *dv = nullptr;
+  LB_HIP(c, hipMalloc(&dr, NP * sizeof(double)));
+  LB_HIP(c, hipMalloc(&dv, c->nD * NP * sizeof(double)));
+  hipError_t e = hipMemcpy(dr, hr.data(), NP * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dv, hv.data(), c->nD * NP * sizeof(double), hipMemcpyHostToDevice);
+  if (c->precision == LB_FP64) {
+    if (e == hipSuccess) e = init_eq_f64(c->lattice, static_cast<double*>(c->d_f[c->cur]), field, c->n_bulk, NP, dr, dv, c->stream);
+  } else {
+    double* tmp = nullptr;
+    const size_t cnt = static_cast<size_t>(c->n_fields) * c->Q * NP;
+    if (e == hipSuccess) e = hipMalloc(&tmp, cnt * sizeof(double));
+    if (e == hipSuccess) e = load_lowp(c->precision, c->lattice, c->n_fields, c->d_f[c->cur], tmp, c->n_bulk, NP, c->stream);
+    if (e == hipSuccess) e = init_eq_f64(c->lattice, tmp, field, c->n_bulk, NP, dr, dv, c->stream);
+    if (e == hipSuccess) e = store_lowp(c->precision, c->lattice, c->n_fields, tmp, c->d_f[c->cur], c->n_bulk, NP, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (tmp) hipFree(tmp);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  hipFree(dr);
+  hipFree(dv);
+  if (e != hipSuccess) return fail(c, LB_ERR_HIP, "init_equilibrium: %s", hipGetErrorString(e));
+  c->have_macro = false;
+  return LB_OK;
+}
+
+int lb_download_scalar(lb_ctx* c, int32_t which, int32_t nf, double* s) {
+  if (!c) return LB_ERR_ARG;
+  if (!c->has_topo) return fail(c, LB_ERR_STATE, "no topology");
+  if (!s) return fail(c, LB_ERR_ARG, "null output");
+  if (!c->have_macro) return fail(c, LB_ERR_STATE, "no step has stored macroscopic fields yet");
+  hipSetDevice(c->device);
+  const int64_t NP = c->NP;
+  if (which == LB_RHO) {
+    if (nf != c->n_fields) return fail(c, LB_ERR_ARG, "rho has %d field(s)", c->n_fields);
+    std::vector<double> h(2 * NP);
+    LB_HIP(c, hipMemcpy(h.data(), c->d_rho, 2 * NP * sizeof(double), hipMemcpyDeviceToHost));
+    for (int32_t i = 0; i < c->n_bulk; ++i)
+      for (int k = 0; k < nf; ++k) s[static_cast<size_t>(c->bulk[i]) * nf + k] = h[k * NP + i];
+  } else if (which == LB_PHI) {
+    if (nf != 1) return fail(c, LB_ERR_ARG, "phi has one field");
+    if (c->n_fields != 2) return fail(c, LB_ERR_STATE, "phi exists only for two-phase fields");
+    std::vector<double> h(NP);
+    LB_HIP(c, hipMemcpy(h.data(), c->d_phi, NP * sizeof(double), hipMemcpyDeviceToHost));
+    for (int32_t i = 0; i < c->n_bulk; ++i) s[c->bulk[i]] = h[i];
+  } else {
+    return fail(c, LB_ERR_ARG, "unknown scalar %d", which);
+  }
+  return LB_OK;
+}
+
+int lb_download_vector(lb_ctx* c, int32_t which, int32_t nf, double* v) {
+  if (!c) return LB_ERR_ARG;
+  if (!c->has_topo) return fail(c, LB_ERR_STATE, "no topology");
+  if (!v || which != LB_VEL || nf != 1) return fail(c, LB_ERR_ARG, "lb_download_vector: bad argument");
+  if (!c->have_macro) return fail(c, LB_ERR_STATE, "no step has stored macroscopic fields yet");
+  hipSetDevice(c->device);
+  const int64_t NP = c->NP;
+  std::vector<double> h(static_cast<size_t>(c->nD) * NP);
+  LB_HIP(c, hipMemcpy(h.data(), c->d_vel, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int32_t i = 0; i < c->n_bulk; ++i)
+    for (int d = 0; d < c->nD; ++d) v[static_cast<size_t>(c->bulk[i]) * c->nD + d] = h[d * NP + i];
+  return LB_OK;
+}
+
+int lb_set_wall_densities(lb_ctx* c, const double* rho2) {
+  if (!c) return LB_ERR_ARG;
+  if (!c->has_topo) return fail(c, LB_ERR_STATE, "no topology");
+  if (!rho2) return fail(c, LB_ERR_ARG, "null argument");
+  hipSetDevice(c->device);
+  std::vector<double> h(c->wall_nodes.size() + 1, 0.0);
+  for (size_t k = 0; k < c->wall_nodes.size(); ++k) {
+    const int32_t n = c->wall_nodes[k];
+    // cgField is written only for solid-boundary nodes (main_TWOPHASE.cpp:280-284)
+    if (c->node_type[n] == 1) {
+      const double r0 = rho2[2 * static_cast<size_t>(n)];
+      const double r1 = rho2[2 * static_cast<size_t>(n) + 1];
+      h[k] = (r0 - r1) / (r0 + r1);
+    }
+  }
+  LB_HIP(c, hipMemcpy(c->d_phi_wall, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  return LB_OK;
+}
+
+// ------------------------------------------------------------ steps
+int lb_step_bgk(lb_ctx* c, int32_t nsteps, double tau, const double* force) {
+  if (!c) return LB_ERR_ARG;
+  int rc = check_ready(c, 1);
+  if (rc != LB_OK) return rc;
+  if (nsteps < 0 || !force || !(tau > 0.0)) return fail(c, LB_ERR_ARG, "lb_step_bgk: bad argument");
+  if (nsteps == 0) return LB_OK;
+  hipSetDevice(c->device);
+  BgkArgs a{};
+  a.tau_inv = 1.0 / tau;
+  a.tau_factor = 1 - 0.5 / tau;
+  for (int d = 0; d < 3; ++d) a.F[d] = d < c->nD ? force[d] : 0.0;
+  Timer t(c);
+  for (int32_t s = 0; s < nsteps; ++s) {
+    const bool last = s == nsteps - 1;
+    hipError_t e;
+    if (c->precision == LB_FP64)
+      e = bgk_f64(c->lattice, !c->post, last, false, static_cast<const double*>(c->d_f[c->cur]),
+                  static_cast<double*>(c->d_f[1 - c->cur]), c->d_link, c->n_bulk, c->NP, a, c->d_rho,
+                  c->d_vel, c->d_partial, c->stream);
+    else
+      e = bgk_lowp(c->precision, c->lattice, !c->post, last, false, c->d_f[c->cur], c->d_f[1 - c->cur],
+                   c->d_link, c->n_bulk, c->NP, a, c->d_rho, c->d_vel, c->d_partial, c->stream);
+    if (e != hipSuccess) return fail(c, LB_ERR_HIP, "bgk launch: %s", hipGetErrorString(e));
+    c->cur = 1 - c->cur;
+    c->post = true;
+  }
+  rc = t.stop(nsteps);
+  if (rc != LB_OK) return rc;
+  c->have_macro = true;
+  return LB_OK;
+}
+
+int lb_step_cg(lb_ctx* c, int32_t nsteps, double tau0, double tau1, double sigma, double beta,
+               int32_t force_mode, const double* force) {
+  if (!c) return LB_ERR_ARG;
+  int rc = check_ready(c, 2);
+  if (rc != LB_OK) return rc;
+  if (nsteps < 0 || !force || !(tau0 > 0.5) || !(tau1 > 0.5) ||
+      (force_mode != LB_FORCE_FIXED && force_mode != LB_FORCE_MOMX))
+    return fail(c, LB_ERR_ARG, "lb_step_cg: bad argument");
+  if (c->precision != LB_FP64) return fail(c, LB_ERR_UNSUPPORTED, "two-phase steps run in FP64 only in this build");
+  if (nsteps == 0) return LB_OK;
+  hipSetDevice(c->device);
+  CgArgs a{};
+  const double c2 = c->lattice == LB_D2Q9 ? D2Q9::c2 : D3Q19::c2;
+  a.nu0Inv = 1.0 / (c2 * (tau0 - 0.5));
+  a.nu1Inv = 1.0 / (c2 * (tau1 - 0.5));
+  a.sigma = sigma;
+  a.beta = beta;
+  a.q0 = 0.0;
+  a.q1 = 0.0;
+  for (int d = 0; d < 3; ++d) a.F[d] = d < c->nD ? force[d] : 0.0;
+  a.momx_mode = force_mode == LB_FORCE_MOMX;
+  a.momx = force[0];
+  const bool momx = a.momx_mode != 0;
+  Timer t(c);
+  int64_t launches = 0;
+  for (int32_t s = 0; s < nsteps; ++s) {
+    const bool last = s == nsteps - 1;
+    const double* fin = static_cast<const double*>(c->d_f[c->cur]);
+    double* fout = static_cast<double*>(c->d_f[1 - c->cur]);
+    hipError_t e = cg_phi_f64(c->lattice, !c->post, momx, fin, c->d_link, c->n_bulk, c->NP, c->d_phi,
+                              c->d_partial, c->stream);
+    if (e == hipSuccess && momx)
+      e = finish_momx(c->d_partial, c->n_blocks, a.momx, c->n_global, c->d_scalar, c->stream);
+    if (e == hipSuccess)
+      e = cg_f64(c->lattice, !c->post, last, fin, fout, c->d_link, c->n_bulk, c->NP, a, c->d_phi,
+                 c->d_phi_wall, c->d_scalar, c->d_rho, c->d_vel, c->stream);
+    if (e != hipSuccess) return fail(c, LB_ERR_HIP, "two-phase launch: %s", hipGetErrorString(e));
+    launches += momx ? 3 : 2;
+    c->cur = 1 - c->cur;
+    c->post = true;
+  }
+  rc = t.stop(launches);
+  if (rc != LB_OK) return rc;
+  if (momx) {
+    double h[2];
+    LB_HIP(c, hipMemcpy(h, c->d_scalar, 2 * sizeof(double), hipMemcpyDeviceToHost));
+    c->last_force[0] = h[0];
+  } else {
+    c->last_force[0] = a.F[0];
+  }
+  c->last_force[1] = a.F[1];
+  c->last_force[2] = a.F[2];
+  c->have_macro = true;
+  return LB_OK;
+}
+
+int lb_last_force(lb_ctx* c, double* force) {
+  if (!c) return LB_ERR_ARG;
+  if (!force) return fail(c, LB_ERR_ARG, "null output");
+  for (int d = 0; d < c->nD; ++d) force[d] = c->last_force[d];
+  return LB_OK;
+}
+
+int lb_reduce_flux(lb_ctx* c, int32_t dir, double* sum_fc, double* sum_phi0_fc, double* sum_phi1_fc) {
+  if (!c) return LB_ERR_ARG;
+  if (!c->has_topo || c->n_fields == 0) return fail(c, LB_ERR_STATE, "no populations");
+  if (dir < 0 || dir >= c->nD) return fail(c, LB_ERR_ARG, "dir out of range");
+  hipSetDevice(c->device);
+  hipError_t e;
+  if (c->precision == LB_FP64)
+    e = flux_f64(c->lattice, c->n_fields, !c->post, static_cast<const double*>(c->d_f[c->cur]), c->d_link,
+                 c->n_bulk, c->NP, dir, c->d_partial, c->stream);
+  else
+    e = flux_lowp(c->precision, c->lattice, c->n_fields, !c->post, c->d_f[c->cur], c->d_link, c->n_bulk,
+                  c->NP, dir, c->d_partial, c->stream);
+  if (e == hipSuccess) e = sum_partials(c->d_partial, c->n_blocks, 3, c->d_scalar + 4, c->stream);
+  double h[3] = {0, 0, 0};
+  if (e == hipSuccess) e = hipMemcpyAsync(h, c->d_scalar + 4, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return fail(c, LB_ERR_HIP, "flux reduction: %s", hipGetErrorString(e));
+  if (sum_fc) *sum_fc = h[0];
+  if (sum_phi0_fc) *sum_phi0_fc = h[1];
+  if (sum_phi1_fc) *sum_phi1_fc = h[2];
+  return LB_OK;
+}
+
+int lb_solve_permeability(lb_ctx* c, double tau, const double* force, int32_t check_every, double tol,
+                          int64_t max_steps, double n_box, double* k_out, int64_t* steps_out) {
+  if (!c) return LB_ERR_ARG;
+  int rc = check_ready(c, 1);
+  if (rc != LB_OK) return rc;
+  if (!force || !(tau > 0.5) || check_every < 1 || !(tol > 0.0) || max_steps < 1 || !(n_box > 0.0) ||
+      force[0] == 0.0)
+    return fail(c, LB_ERR_ARG, "lb_solve_permeability: bad argument");
+  hipSetDevice(c->device);
+  BgkArgs a{};
+  a.tau_inv = 1.0 / tau;
+  a.tau_factor = 1 - 0.5 / tau;
+  for (int d = 0; d < 3; ++d) a.F[d] = d < c->nD ? force[d] : 0.0;
+  const double nu = (tau - 0.5) / 3.0;
+  double k_prev = 0.0, k = 0.0;
+  int64_t steps = 0;
+  bool have_prev = false;
+  Timer t(c);
+  int64_t launches = 0;
+  while (steps < max_steps) {
+    const int64_t chunk = std::min<int64_t>(check_every, max_steps - steps);
+    for (int64_t s = 0; s < chunk; ++s) {
+      const bool last = s == chunk - 1;
+      hipError_t e;
+      if (c->precision == LB_FP64)
+        e = bgk_f64(c->lattice, !c->post, last, last, static_cast<const double*>(c->d_f[c->cur]),
+                    static_cast<double*>(c->d_f[1 - c->cur]), c->d_link, c->n_bulk, c->NP, a, c->d_rho,
+                    c->d_vel, c->d_partial, c->stream);
+      else
+        e = bgk_lowp(c->precision, c->lattice, !c->post, last, last, c->d_f[c->cur], c->d_f[1 - c->cur],
+                     c->d_link, c->n_bulk, c->NP, a, c->d_rho, c->d_vel, c->d_partial, c->stream);
+      if (e != hipSuccess) return fail(c, LB_ERR_HIP, "bgk launch: %s", hipGetErrorString(e));
+      c->cur = 1 - c->cur;
+      c->post = true;
+      ++launches;
+    }
+    steps += chunk;
+    hipError_t e = sum_partials(c->d_partial, c->n_blocks, 1, c->d_scalar + 4, c->stream);
+    double jx = 0.0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&jx, c->d_scalar + 4, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(c, LB_ERR_HIP, "permeability reduction: %s", hipGetErrorString(e));
+    // j_x summed over the state the last step started from; k = nu <j_x> / F_x
+    k = nu * (jx / n_box) / a.F[0];
+    if (!std::isfinite(k)) return fail(c, LB_ERR_STATE, "permeability is not finite after %lld steps", (long long)steps);
+    if (have_prev && std::fabs(k - k_prev) <= tol * std::fabs(k)) break;
+    k_prev = k;
+    have_prev = true;
+  }
+  rc = t.stop(launches);
+  if (rc != LB_OK) return rc;
+  c->have_macro = true;
+  if (k_out) *k_out = k;
+  if (steps_out) *steps_out = steps;
+  return LB_OK;
+}
+
+// ------------------------------------------------------------ timing
+int lb_sync(lb_ctx* c) {
+  if (!c) return LB_ERR_ARG;
+  hipSetDevice(c->device);
+  LB_HIP(c, hipStreamSynchronize(c->stream));
+  return LB_OK;
+}
+
+int lb_kernel_time(lb_ctx* c, double* ms, int64_t* launches) {
+  if (!c) return LB_ERR_ARG;
+  if (ms) *ms = c->kernel_ms;
+  if (launches) *launches = c->launches;
+  return LB_OK;
+}
+
+int lb_kernel_time_reset(lb_ctx* c) {
+  if (!c) return LB_ERR_ARG;
+  c->kernel_ms = 0.0;
+  c->launches = 0;
+  return LB_OK;
+}
+
+int64_t lb_device_bytes(const lb_ctx* c) { return c ? c->bytes : -1; }
+
+}  // extern "C"

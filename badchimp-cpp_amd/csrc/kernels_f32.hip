@@ -1,0 +1,281 @@
+// Reduced-precision BGK + Guo collide-stream on shifted populations.
+//
+// With g_q = f_q - w_q (reference density 1) the moments are
+//   rho = 1 + sum_q g_q,   j = sum_q g_q c_q   (sum_q w_q c_q = 0)
+// and the BGK relaxation of the reference (LBcollision.h:41-44) becomes
+//   g_q' = g_q - (g_q - geq_q)/tau + dF_q,
+//   geq_q = f_eq,q - w_q = w_q (drho + rho (3 cu + 4.5 cu^2 - 1.5 u^2)),
+// with the Guo term dF_q of LBcollision.h:206-211 unchanged.  Subtracting
+// w_q before rounding keeps the float (or half) mantissa for the deviation
+// from rest, which is what the flow carries.  Streaming and bounce-back use
+// the same pull table as the FP64 path.
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "kernels_f32.hpp"
+#include "lattice.hpp"
+
+namespace lbamd {
+namespace {
+
+template <typename S>
+struct Store;
+template <>
+struct Store<float> {
+  static __device__ __forceinline__ float get(float v) { return v; }
+  static __device__ __forceinline__ float put(float v) { return v; }
+  static constexpr float scale = 1.0f;
+};
+// half values travel as their 16-bit patterns
+template <>
+struct Store<uint16_t> {
+  static __device__ __forceinline__ float get(uint16_t v) {
+    return __half2float(__ushort_as_half(v)) * (1.0f / kHalfScale);
+  }
+  static __device__ __forceinline__ uint16_t put(float v) {
+    return __half_as_ushort(__float2half_rn(v * kHalfScale));
+  }
+  static constexpr float scale = kHalfScale;
+};
+
+template <class L>
+__device__ __forceinline__ int64_t pull_index(const uint32_t* __restrict__ link, int q, int32_t i,
+                                              int64_t NP) {
+  const uint32_t l = __builtin_nontemporal_load(&link[q * NP + i]);
+  const uint32_t kind = l >> 30;
+  const int64_t j = static_cast<int64_t>(l & kLinkIndexMask);
+  const int rq = L::rev(q);
+  return kind == kLinkStream ? q * NP + j : (kind == kLinkBounce ? rq * NP + i : rq * NP + j);
+}
+
+__device__ __forceinline__ double block_sum(double v) {
+  __shared__ double red[kBlock / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) r += red[w];
+  }
+  __syncthreads();
+  return r;
+}
+
+template <class L, typename S, bool FIRST, bool STORE, bool REDUCE>
+__global__ __launch_bounds__(kBlock) void k_bgk_lp(const S* __restrict__ gin, S* __restrict__ gout,
+                                                   const uint32_t* __restrict__ link, int32_t n,
+                                                   int64_t NP, float tau_inv, float tau_factor,
+                                                   float F0, float F1, float F2,
+                                                   double* __restrict__ rho_o,
+                                                   double* __restrict__ vel_o,
+                                                   double* __restrict__ partial) {
+  constexpr int Q = L::nQ, D = L::nD;
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  double jx = 0.0;
+  if (i < n) {
+    const float F[3] = {F0, F1, F2};
+    float g[Q];
+#pragma unroll
+    for (int q = 0; q < Q - 1; ++q) {
+      const int64_t idx = FIRST ? q * NP + i : pull_index<L>(link, q, i, NP);
+      g[q] = Store<S>::get(gin[idx]);
+    }
+    g[Q - 1] = Store<S>::get(gin[(Q - 1) * NP + i]);
+    float drho = 0.0f;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) drho += g[q];
+    const float rho = 1.0f + drho;
+    const float rinv = 1.0f / rho;
+    float u[D], jd[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      jd[d] = qsumc<L>(g, d) + 0.5f * F[d];
+      u[d] = jd[d] * rinv;
+    }
+    if (STORE) {
+      rho_o[i] = static_cast<double>(rho);
+#pragma unroll
+      for (int d = 0; d < D; ++d) vel_o[d * NP + i] = static_cast<double>(u[d]);
+    }
+    if (REDUCE) jx = static_cast<double>(jd[0]);
+    const float u2 = dot<L>(u, u);
+    const float uF = dot<L>(u, F);
+    const float c2u2 = 1.5f * u2;
+    const float c2uF = float(L::c2) * uF;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const float w = float(L::w(q));
+      const float cu = cdot<L>(q, u);
+      const float cF = cdot<L>(q, F);
+      const float geq = w * (drho + rho * (3.0f * cu + 4.5f * cu * cu - c2u2));
+      const float dF = (w * tau_factor) * (3.0f * cF + 9.0f * (cF * cu - c2uF));
+      const float gp = g[q] - tau_inv * (g[q] - geq) + dF;
+      __builtin_nontemporal_store(Store<S>::put(gp), &gout[q * NP + i]);
+    }
+  }
+  if (REDUCE) {
+    const double s = block_sum(jx);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  }
+}
+
+template <class L, typename S>
+__global__ __launch_bounds__(kBlock) void k_store(const double* __restrict__ f, S* __restrict__ g,
+                                                  int32_t n, int64_t NP, int nf) {
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  for (int k = 0; k < nf; ++k)
+#pragma unroll
+    for (int q = 0; q < L::nQ; ++q) {
+      const int64_t o = (static_cast<int64_t>(k) * L::nQ + q) * NP + i;
+      g[o] = Store<S>::put(static_cast<float>(f[o] - L::w(q)));
+    }
+}
+
+template <class L, typename S>
+__global__ __launch_bounds__(kBlock) void k_load(const S* __restrict__ g, double* __restrict__ f,
+                                                 int32_t n, int64_t NP, int nf) {
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  for (int k = 0; k < nf; ++k)
+#pragma unroll
+    for (int q = 0; q < L::nQ; ++q) {
+      const int64_t o = (static_cast<int64_t>(k) * L::nQ + q) * NP + i;
+      f[o] = static_cast<double>(Store<S>::get(g[o])) + L::w(q);
+    }
+}
+
+template <class L, typename S>
+__global__ __launch_bounds__(kBlock) void k_gather_lp(const S* __restrict__ gin, S* __restrict__ gout,
+                                                      const uint32_t* __restrict__ link, int32_t n,
+                                                      int64_t NP, int nf) {
+  constexpr int Q = L::nQ;
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int64_t fs = static_cast<int64_t>(Q) * NP;
+  for (int k = 0; k < nf; ++k) {
+#pragma unroll
+    for (int q = 0; q < Q - 1; ++q) gout[k * fs + q * NP + i] = gin[k * fs + pull_index<L>(link, q, i, NP)];
+    gout[k * fs + (Q - 1) * NP + i] = gin[k * fs + (Q - 1) * NP + i];
+  }
+}
+
+template <class L, typename S, bool FIRST>
+__global__ __launch_bounds__(kBlock) void k_flux_lp(const S* __restrict__ gin,
+                                                    const uint32_t* __restrict__ link, int32_t n,
+                                                    int64_t NP, int dir, double* __restrict__ partial,
+                                                    int64_t nblocks) {
+  constexpr int Q = L::nQ;
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  double s0 = 0.0;
+  if (i < n) {
+    float g[Q];
+#pragma unroll
+    for (int q = 0; q < Q - 1; ++q) {
+      const int64_t idx = FIRST ? q * NP + i : pull_index<L>(link, q, i, NP);
+      g[q] = Store<S>::get(gin[idx]);
+    }
+    g[Q - 1] = Store<S>::get(gin[(Q - 1) * NP + i]);
+    s0 = static_cast<double>(qsumc<L>(g, dir));
+  }
+  s0 = block_sum(s0);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = s0;
+    partial[nblocks + blockIdx.x] = 0.0;
+    partial[2 * nblocks + blockIdx.x] = 0.0;
+  }
+}
+
+inline dim3 grid_for(int32_t n) { return dim3(static_cast<unsigned>((n + kBlock - 1) / kBlock)); }
+
+template <typename F>
+hipError_t dispatch(int precision, int lattice, F&& f) {
+  if (lattice == 0) {
+    if (precision == 1) f(D2Q9{}, float{});
+    else f(D2Q9{}, uint16_t{});
+  } else {
+    if (precision == 1) f(D3Q19{}, float{});
+    else f(D3Q19{}, uint16_t{});
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t store_lowp(int precision, int lattice, int nf, const double* f, void* g, int32_t n,
+                      int64_t NP, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  return dispatch(precision, lattice, [&](auto l, auto st) {
+    using L = decltype(l);
+    using S = decltype(st);
+    k_store<L, S><<<grid_for(n), dim3(kBlock), 0, s>>>(f, static_cast<S*>(g), n, NP, nf);
+  });
+}
+
+hipError_t load_lowp(int precision, int lattice, int nf, const void* g, double* f, int32_t n,
+                     int64_t NP, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  return dispatch(precision, lattice, [&](auto l, auto st) {
+    using L = decltype(l);
+    using S = decltype(st);
+    k_load<L, S><<<grid_for(n), dim3(kBlock), 0, s>>>(static_cast<const S*>(g), f, n, NP, nf);
+  });
+}
+
+hipError_t gather_lowp(int precision, int lattice, int nf, const void* gin, void* gout,
+                       const uint32_t* link, int32_t n, int64_t NP, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  return dispatch(precision, lattice, [&](auto l, auto st) {
+    using L = decltype(l);
+    using S = decltype(st);
+    k_gather_lp<L, S><<<grid_for(n), dim3(kBlock), 0, s>>>(static_cast<const S*>(gin), static_cast<S*>(gout),
+                                                          link, n, NP, nf);
+  });
+}
+
+hipError_t bgk_lowp(int precision, int lattice, bool first, bool store, bool reduce,
+                    const void* gin, void* gout, const uint32_t* link, int32_t n, int64_t NP,
+                    const BgkArgs& a, double* rho, double* vel, double* partial, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const float ti = static_cast<float>(a.tau_inv), tf = static_cast<float>(a.tau_factor);
+  const float F0 = static_cast<float>(a.F[0]), F1 = static_cast<float>(a.F[1]), F2 = static_cast<float>(a.F[2]);
+  const int key = (first ? 4 : 0) | (store ? 2 : 0) | (reduce ? 1 : 0);
+  return dispatch(precision, lattice, [&](auto l, auto st) {
+    using L = decltype(l);
+    using S = decltype(st);
+    const S* gi = static_cast<const S*>(gin);
+    S* go = static_cast<S*>(gout);
+    const dim3 g = grid_for(n), b(kBlock);
+#define LB_K(F_, S_, R_) k_bgk_lp<L, S, F_, S_, R_><<<g, b, 0, s>>>(gi, go, link, n, NP, ti, tf, F0, F1, F2, rho, vel, partial)
+    switch (key) {
+      case 0: LB_K(false, false, false); break;
+      case 1: LB_K(false, false, true); break;
+      case 2: LB_K(false, true, false); break;
+      case 3: LB_K(false, true, true); break;
+      case 4: LB_K(true, false, false); break;
+      case 5: LB_K(true, false, true); break;
+      case 6: LB_K(true, true, false); break;
+      default: LB_K(true, true, true); break;
+    }
+#undef LB_K
+  });
+}
+
+hipError_t flux_lowp(int precision, int lattice, int nf, bool first, const void* gin,
+                     const uint32_t* link, int32_t n, int64_t NP, int dir, double* partial,
+                     hipStream_t s) {
+  if (nf != 1) return hipErrorInvalidValue;
+  const dim3 g = grid_for(n > 0 ? n : 1), b(kBlock);
+  const int64_t nb = g.x;
+  return dispatch(precision, lattice, [&](auto l, auto st) {
+    using L = decltype(l);
+    using S = decltype(st);
+    if (first) k_flux_lp<L, S, true><<<g, b, 0, s>>>(static_cast<const S*>(gin), link, n, NP, dir, partial, nb);
+    else k_flux_lp<L, S, false><<<g, b, 0, s>>>(static_cast<const S*>(gin), link, n, NP, dir, partial, nb);
+  });
+}
+
+}  // namespace lbamd

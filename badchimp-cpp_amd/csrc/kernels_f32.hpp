@@ -1,0 +1,32 @@
+// Launchers of the reduced-precision kernels (shifted populations).
+//
+// Storage holds g_q = (f_q - w_q) * s in float (FP32S, s = 1) or half
+// (FP16S, s = kHalfScale); the arithmetic is float; the flux and
+// permeability partial sums are accumulated in double.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "context.hpp"
+
+namespace lbamd {
+
+// Scale of the FP16 storage: deviations f - w of order 1e-7..1e-2 are kept in
+// the normal half range (6.1e-5 .. 65504) after multiplying by 2^12.
+constexpr float kHalfScale = 4096.0f;
+
+hipError_t store_lowp(int precision, int lattice, int nf, const double* f, void* g, int32_t n,
+                      int64_t NP, hipStream_t s);
+hipError_t load_lowp(int precision, int lattice, int nf, const void* g, double* f, int32_t n,
+                     int64_t NP, hipStream_t s);
+hipError_t gather_lowp(int precision, int lattice, int nf, const void* gin, void* gout,
+                       const uint32_t* link, int32_t n, int64_t NP, hipStream_t s);
+hipError_t bgk_lowp(int precision, int lattice, bool first, bool store, bool reduce,
+                    const void* gin, void* gout, const uint32_t* link, int32_t n, int64_t NP,
+                    const BgkArgs& a, double* rho, double* vel, double* partial, hipStream_t s);
+hipError_t flux_lowp(int precision, int lattice, int nf, bool first, const void* gin,
+                     const uint32_t* link, int32_t n, int64_t NP, int dir, double* partial,
+                     hipStream_t s);
+
+}  // namespace lbamd

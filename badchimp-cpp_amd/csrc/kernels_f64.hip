@@ -1,0 +1,447 @@
+// FP64 kernels: the reference's arithmetic, one IEEE operation at a time.
+//
+// This file is compiled with -ffp-contract=off: no multiply-add fusion, so
+// every product and sum rounds exactly where the reference's x86-64 build
+// (SSE2, no FMA) rounds it, and the results are bit-identical.
+//
+// Streaming is in pull form.  The reference pushes the post-collision values
+// to neighbor(q,n) (LBfield.h:350-357) and then copies back the values that a
+// node pushed into a solid neighbour (LBhalfwaybb.h:37-63).  Reading instead
+// f_post(q, neighbor(rev q, n)) for a fluid source, and f_post(rev q, n) for a
+// solid one, yields the same f_{t+1}(q,n) for every bulk node; the buffer keeps
+// post-collision values and each step starts with the gather.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "lattice.hpp"
+
+namespace lbamd {
+namespace {
+
+template <class L>
+__device__ __forceinline__ int64_t pull_index(const uint32_t* __restrict__ link, int q,
+                                              int32_t i, int64_t NP) {
+  const uint32_t l = __builtin_nontemporal_load(&link[q * NP + i]);
+  const uint32_t kind = l >> 30;
+  const int64_t j = static_cast<int64_t>(l & kLinkIndexMask);
+  const int rq = L::rev(q);
+  return kind == kLinkStream ? q * NP + j : (kind == kLinkBounce ? rq * NP + i : rq * NP + j);
+}
+
+// Load the pre-collision populations of node i for nf fields.
+template <class L, int NF, bool FIRST>
+__device__ __forceinline__ void load_node(const double* __restrict__ fin,
+                                          const uint32_t* __restrict__ link, int32_t i,
+                                          int64_t NP, double (&f)[NF][L::nQ]) {
+  const int64_t fstride = static_cast<int64_t>(L::nQ) * NP;
+#pragma unroll
+  for (int q = 0; q < L::nQ - 1; ++q) {
+    const int64_t idx = FIRST ? q * NP + i : pull_index<L>(link, q, i, NP);
+#pragma unroll
+    for (int k = 0; k < NF; ++k) f[k][q] = fin[k * fstride + idx];
+  }
+#pragma unroll
+  for (int k = 0; k < NF; ++k) f[k][L::nQ - 1] = fin[k * fstride + (L::nQ - 1) * NP + i];
+}
+
+// Deterministic block sum of 256 values; result valid in thread 0.
+__device__ __forceinline__ double block_sum(double v) {
+  __shared__ double red[kBlock / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) r += red[w];
+  }
+  __syncthreads();
+  return r;
+}
+
+// ------------------------------------------------------------------ BGK
+template <class L, bool FIRST, bool STORE, bool REDUCE>
+__global__ __launch_bounds__(kBlock) void k_bgk(const double* __restrict__ fin,
+                                                double* __restrict__ fout,
+                                                const uint32_t* __restrict__ link, int32_t n,
+                                                int64_t NP, BgkArgs a, double* __restrict__ rho_o,
+                                                double* __restrict__ vel_o,
+                                                double* __restrict__ partial) {
+  constexpr int Q = L::nQ, D = L::nD;
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  double jx = 0.0;
+  if (i < n) {
+    double f[1][Q];
+    load_node<L, 1, FIRST>(fin, link, i, NP, f);
+    // calcRho, calcVel (LBmacroscopic.h:11-45)
+    const double rho = qsum<L>(f[0]);
+    double u[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) u[d] = (qsumc<L>(f[0], d) + 0.5 * a.F[d]) / rho;
+    if (STORE) {
+      rho_o[i] = rho;
+#pragma unroll
+      for (int d = 0; d < D; ++d) vel_o[d * NP + i] = u[d];
+    }
+    if (REDUCE) jx = qsumc<L>(f[0], 0) + 0.5 * a.F[0];
+    const double u2 = dot<L>(u, u);
+    const double uF = dot<L>(u, a.F);
+    const double c2u2 = L::c2 * u2;
+    const double c2uF = L::c2 * uF;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const double cu = cdot<L>(q, u);
+      const double cF = cdot<L>(q, a.F);
+      // calcOmegaBGK (LBcollision.h:41-44)
+      const double feq = (rho * L::w(q)) * ((1.0 + L::c2Inv * cu) + L::c4Inv0_5 * (cu * cu - c2u2));
+      const double om = (-a.tau_inv) * (f[0][q] - feq);
+      // calcDeltaOmegaF (LBcollision.h:206-211)
+      const double dF = (L::w(q) * a.tau_factor) * (L::c2Inv * cF + L::c4Inv * (cF * cu - c2uF));
+      __builtin_nontemporal_store((f[0][q] + om) + dF, &fout[q * NP + i]);
+    }
+  }
+  if (REDUCE) {
+    const double s = block_sum(jx);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  }
+}
+
+// ------------------------------------------------------------------ two-phase pass 1
+template <class L, bool FIRST, bool MOMX>
+__global__ __launch_bounds__(kBlock) void k_cg_phi(const double* __restrict__ fin,
+                                                   const uint32_t* __restrict__ link, int32_t n,
+                                                   int64_t NP, double* __restrict__ phi,
+                                                   double* __restrict__ partial) {
+  constexpr int Q = L::nQ;
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  double mx = 0.0;
+  if (i < n) {
+    double f[2][Q];
+    load_node<L, 2, FIRST>(fin, link, i, NP, f);
+    const double r0 = qsum<L>(f[0]);
+    const double r1 = qsum<L>(f[1]);
+    phi[i] = (r0 - r1) / (r0 + r1);
+    if (MOMX) {
+      double ft[Q];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) ft[q] = f[0][q] + f[1][q];
+      mx = qsumc<L>(ft, 0);
+    }
+  }
+  if (MOMX) {
+    const double s = block_sum(mx);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  }
+}
+
+// ------------------------------------------------------------------ two-phase pass 3
+template <class L, bool FIRST, bool STORE>
+__global__ __launch_bounds__(kBlock) void k_cg(const double* __restrict__ fin,
+                                               double* __restrict__ fout,
+                                               const uint32_t* __restrict__ link, int32_t n,
+                                               int64_t NP, CgArgs a,
+                                               const double* __restrict__ phi,
+                                               const double* __restrict__ phi_wall,
+                                               const double* __restrict__ dscalar,
+                                               double* __restrict__ rho_o,
+                                               double* __restrict__ vel_o) {
+  constexpr int Q = L::nQ, D = L::nD;
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double f[2][Q];
+  load_node<L, 2, FIRST>(fin, link, i, NP, f);
+  double F[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) F[d] = a.F[d];
+  if (a.momx_mode) F[0] = dscalar[0];
+  double r0 = qsum<L>(f[0]);
+  double r1 = qsum<L>(f[1]);
+  double ft[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) ft[q] = f[0][q] + f[1][q];
+  const double rho = r0 + r1;
+  double u[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) u[d] = (qsumc<L>(ft, d) + 0.5 * F[d]) / rho;
+  // mass-source correction of the densities (main_TWOPHASE.cpp:332-335);
+  // the stored rho fields carry the corrected values
+  r0 = r0 + 0.5 * a.q0;
+  r1 = r1 + 0.5 * a.q1;
+  if (STORE) {
+    rho_o[i] = r0;
+    rho_o[NP + i] = r1;
+#pragma unroll
+    for (int d = 0; d < D; ++d) vel_o[d * NP + i] = u[d];
+  }
+  // viscosity mix (main_TWOPHASE.cpp:340)
+  const double tau = (L::c2Inv * rho) / (r0 * a.nu0Inv + r1 * a.nu1Inv) + 0.5;
+  const double tau_inv = 1.0 / tau;
+  const double tau_factor = 1.0 - 0.5 / tau;
+  const double uu = dot<L>(u, u);
+  const double uF = dot<L>(u, F);
+  const double c2uu = L::c2 * uu;
+  const double c2uF = L::c2 * uF;
+  // colour gradient from the neighbours' phi (LButilities.h:12-22, :62-66)
+  double pn[Q];
+#pragma unroll
+  for (int qq = 0; qq < Q - 1; ++qq) {
+    // neighbor(qq, n) is the stream source of direction rev(qq)
+    const int q = L::rev(qq);
+    const uint32_t l = link[q * NP + i];
+    const uint32_t kind = l >> 30;
+    const uint32_t j = l & kLinkIndexMask;
+    pn[qq] = kind == kLinkStream ? phi[j] : (kind == kLinkBounce ? phi_wall[j] : 0.0);
+  }
+  pn[Q - 1] = 0.0;
+  double g[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) g[d] = grad<L>(pn, d);
+  const double cgn = sqrt(dot<L>(g, g));
+  const double ginv = 1.0 / (cgn + (cgn < 2.220446049250313080847e-16 ? 1.0 : 0.0));
+#pragma unroll
+  for (int d = 0; d < D; ++d) g[d] = g[d] * ginv;
+  // calcDeltaOmegaST / RC prefactors (LBcollision2phase.h:12,75)
+  const double af = ((1.125 * cgn) * a.sigma) / tau;
+  const double rfb = ((a.beta * r0) * r1) / rho;
+  const double c0 = r0 / rho;
+  const double c1 = r1 / rho;
+  const int64_t fstride = static_cast<int64_t>(Q) * NP;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const double cu = cdot<L>(q, u);
+    const double cF = cdot<L>(q, F);
+    const double poly = (1.0 + L::c2Inv * cu) + L::c4Inv0_5 * (cu * cu - c2uu);
+    const double om = (-tau_inv) * (ft[q] - (rho * L::w(q)) * poly);
+    const double dF = (L::w(q) * tau_factor) * (L::c2Inv * cF + L::c4Inv * (cF * cu - c2uF));
+    const double dQ0 = ((tau_factor * a.q0) * L::w(q)) * poly;
+    const double dQ1 = ((tau_factor * a.q1) * L::w(q)) * poly;
+    const double cg = cdot<L>(q, g);
+    double st, rc;
+    if (q < Q - 1) {
+      st = af * ((L::w(q) * cg) * cg - L::B(q));
+      rc = ((rfb * L::w(q)) * cg) / L::cNorm(q);
+    } else {
+      st = (-af) * L::B(q);
+      rc = 0.0;
+    }
+    const double base = ((ft[q] + om) + dF) + st;
+    __builtin_nontemporal_store(((c0 * base) + rc) + dQ0, &fout[q * NP + i]);
+    __builtin_nontemporal_store(((c1 * base) - rc) + dQ1, &fout[fstride + q * NP + i]);
+  }
+}
+
+// ------------------------------------------------------------------ utilities
+template <class L>
+__global__ __launch_bounds__(kBlock) void k_gather(const double* __restrict__ fin,
+                                                   double* __restrict__ fout,
+                                                   const uint32_t* __restrict__ link, int32_t n,
+                                                   int64_t NP, int nf) {
+  constexpr int Q = L::nQ;
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int64_t fstride = static_cast<int64_t>(Q) * NP;
+  for (int k = 0; k < nf; ++k) {
+#pragma unroll
+    for (int q = 0; q < Q - 1; ++q)
+      fout[k * fstride + q * NP + i] = fin[k * fstride + pull_index<L>(link, q, i, NP)];
+    fout[k * fstride + (Q - 1) * NP + i] = fin[k * fstride + (Q - 1) * NP + i];
+  }
+}
+
+template <class L, int NF, bool FIRST>
+__global__ __launch_bounds__(kBlock) void k_flux(const double* __restrict__ fin,
+                                                 const uint32_t* __restrict__ link, int32_t n,
+                                                 int64_t NP, int dir,
+                                                 double* __restrict__ partial, int64_t nblocks) {
+  constexpr int Q = L::nQ;
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  if (i < n) {
+    double f[NF][Q];
+    load_node<L, NF, FIRST>(fin, link, i, NP, f);
+    double ft[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) ft[q] = NF == 2 ? f[0][q] + f[NF - 1][q] : f[0][q];
+    const double fc = qsumc<L>(ft, dir);
+    s0 = fc;
+    if (NF == 2) {
+      const double r0 = qsum<L>(f[0]);
+      const double r1 = qsum<L>(f[NF - 1]);
+      const double rt = r0 + r1;
+      s1 = (r0 / rt) * fc;
+      s2 = (r1 / rt) * fc;
+    }
+  }
+  s0 = block_sum(s0);
+  s1 = block_sum(s1);
+  s2 = block_sum(s2);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = s0;
+    partial[nblocks + blockIdx.x] = s1;
+    partial[2 * nblocks + blockIdx.x] = s2;
+  }
+}
+
+template <class L>
+__global__ __launch_bounds__(kBlock) void k_init_eq(double* __restrict__ f, int field,
+                                                    int32_t n, int64_t NP,
+                                                    const double* __restrict__ rho,
+                                                    const double* __restrict__ vel) {
+  constexpr int Q = L::nQ, D = L::nD;
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double u[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) u[d] = vel[d * NP + i];
+  const double uu = dot<L>(u, u);
+  const double r = rho[i];
+  const int64_t base = static_cast<int64_t>(field) * Q * NP;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const double cu = cdot<L>(q, u);
+    f[base + q * NP + i] = (L::w(q) * r) * ((1.0 + L::c2Inv * cu) + L::c4Inv0_5 * (cu * cu - L::c2 * uu));
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sum_partials(const double* __restrict__ partial,
+                                                         int64_t nblocks, int ncomp,
+                                                         double* __restrict__ out) {
+  for (int c = 0; c < ncomp; ++c) {
+    double s = 0.0;
+    for (int64_t b = threadIdx.x; b < nblocks; b += kBlock) s += partial[c * nblocks + b];
+    s = block_sum(s);
+    if (threadIdx.x == 0) out[c] = s;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_finish_momx(const double* __restrict__ partial,
+                                                        int64_t nblocks, double momx,
+                                                        int64_t n_global,
+                                                        double* __restrict__ dscalar) {
+  double s = 0.0;
+  for (int64_t b = threadIdx.x; b < nblocks; b += kBlock) s += partial[b];
+  s = block_sum(s);
+  if (threadIdx.x == 0) {
+    const double mean = s / static_cast<double>(n_global);
+    dscalar[0] = 2.0 * (momx - mean);
+    dscalar[1] = s;
+  }
+}
+
+inline dim3 grid_for(int32_t n) { return dim3(static_cast<unsigned>((n + kBlock - 1) / kBlock)); }
+
+}  // namespace
+
+// ------------------------------------------------------------------ launchers
+#define LB_DISPATCH_LATTICE(lat, L, ...)     \
+  do {                                       \
+    if ((lat) == 0) {                        \
+      using L = D2Q9;                        \
+      __VA_ARGS__;                           \
+    } else {                                 \
+      using L = D3Q19;                       \
+      __VA_ARGS__;                           \
+    }                                        \
+  } while (0)
+
+hipError_t bgk_f64(int lattice, bool first, bool store, bool reduce, const double* fin,
+                   double* fout, const uint32_t* link, int32_t n, int64_t NP,
+                   const BgkArgs& a, double* rho, double* vel, double* partial,
+                   hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const dim3 g = grid_for(n), b(kBlock);
+  LB_DISPATCH_LATTICE(lattice, L, {
+    const int key = (first ? 4 : 0) | (store ? 2 : 0) | (reduce ? 1 : 0);
+    switch (key) {
+      case 0: k_bgk<L, false, false, false><<<g, b, 0, s>>>(fin, fout, link, n, NP, a, rho, vel, partial); break;
+      case 1: k_bgk<L, false, false, true><<<g, b, 0, s>>>(fin, fout, link, n, NP, a, rho, vel, partial); break;
+      case 2: k_bgk<L, false, true, false><<<g, b, 0, s>>>(fin, fout, link, n, NP, a, rho, vel, partial); break;
+      case 3: k_bgk<L, false, true, true><<<g, b, 0, s>>>(fin, fout, link, n, NP, a, rho, vel, partial); break;
+      case 4: k_bgk<L, true, false, false><<<g, b, 0, s>>>(fin, fout, link, n, NP, a, rho, vel, partial); break;
+      case 5: k_bgk<L, true, false, true><<<g, b, 0, s>>>(fin, fout, link, n, NP, a, rho, vel, partial); break;
+      case 6: k_bgk<L, true, true, false><<<g, b, 0, s>>>(fin, fout, link, n, NP, a, rho, vel, partial); break;
+      default: k_bgk<L, true, true, true><<<g, b, 0, s>>>(fin, fout, link, n, NP, a, rho, vel, partial); break;
+    }
+  });
+  return hipGetLastError();
+}
+
+hipError_t cg_phi_f64(int lattice, bool first, bool momx, const double* fin,
+                      const uint32_t* link, int32_t n, int64_t NP, double* phi,
+                      double* partial, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const dim3 g = grid_for(n), b(kBlock);
+  LB_DISPATCH_LATTICE(lattice, L, {
+    if (first && momx) k_cg_phi<L, true, true><<<g, b, 0, s>>>(fin, link, n, NP, phi, partial);
+    else if (first) k_cg_phi<L, true, false><<<g, b, 0, s>>>(fin, link, n, NP, phi, partial);
+    else if (momx) k_cg_phi<L, false, true><<<g, b, 0, s>>>(fin, link, n, NP, phi, partial);
+    else k_cg_phi<L, false, false><<<g, b, 0, s>>>(fin, link, n, NP, phi, partial);
+  });
+  return hipGetLastError();
+}
+
+hipError_t cg_f64(int lattice, bool first, bool store, const double* fin, double* fout,
+                  const uint32_t* link, int32_t n, int64_t NP, const CgArgs& a,
+                  const double* phi, const double* phi_wall, const double* dscalar,
+                  double* rho, double* vel, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const dim3 g = grid_for(n), b(kBlock);
+  LB_DISPATCH_LATTICE(lattice, L, {
+    if (first && store) k_cg<L, true, true><<<g, b, 0, s>>>(fin, fout, link, n, NP, a, phi, phi_wall, dscalar, rho, vel);
+    else if (first) k_cg<L, true, false><<<g, b, 0, s>>>(fin, fout, link, n, NP, a, phi, phi_wall, dscalar, rho, vel);
+    else if (store) k_cg<L, false, true><<<g, b, 0, s>>>(fin, fout, link, n, NP, a, phi, phi_wall, dscalar, rho, vel);
+    else k_cg<L, false, false><<<g, b, 0, s>>>(fin, fout, link, n, NP, a, phi, phi_wall, dscalar, rho, vel);
+  });
+  return hipGetLastError();
+}
+
+hipError_t gather_f64(int lattice, int nfields, const double* fin, double* fout,
+                      const uint32_t* link, int32_t n, int64_t NP, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  LB_DISPATCH_LATTICE(lattice, L, {
+    k_gather<L><<<grid_for(n), dim3(kBlock), 0, s>>>(fin, fout, link, n, NP, nfields);
+  });
+  return hipGetLastError();
+}
+
+hipError_t flux_f64(int lattice, int nfields, bool first, const double* fin,
+                    const uint32_t* link, int32_t n, int64_t NP, int dir, double* partial,
+                    hipStream_t s) {
+  const dim3 g = grid_for(n > 0 ? n : 1), b(kBlock);
+  const int64_t nb = g.x;
+  LB_DISPATCH_LATTICE(lattice, L, {
+    if (nfields == 2) {
+      if (first) k_flux<L, 2, true><<<g, b, 0, s>>>(fin, link, n, NP, dir, partial, nb);
+      else k_flux<L, 2, false><<<g, b, 0, s>>>(fin, link, n, NP, dir, partial, nb);
+    } else {
+      if (first) k_flux<L, 1, true><<<g, b, 0, s>>>(fin, link, n, NP, dir, partial, nb);
+      else k_flux<L, 1, false><<<g, b, 0, s>>>(fin, link, n, NP, dir, partial, nb);
+    }
+  });
+  return hipGetLastError();
+}
+
+hipError_t init_eq_f64(int lattice, double* f, int field, int32_t n, int64_t NP,
+                       const double* rho, const double* vel, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  LB_DISPATCH_LATTICE(lattice, L, {
+    k_init_eq<L><<<grid_for(n), dim3(kBlock), 0, s>>>(f, field, n, NP, rho, vel);
+  });
+  return hipGetLastError();
+}
+
+hipError_t sum_partials(const double* partial, int64_t nblocks, int ncomp, double* out,
+                        hipStream_t s) {
+  k_sum_partials<<<dim3(1), dim3(kBlock), 0, s>>>(partial, nblocks, ncomp, out);
+  return hipGetLastError();
+}
+
+hipError_t finish_momx(const double* partial, int64_t nblocks, double momx,
+                       int64_t n_global, double* dscalar, hipStream_t s) {
+  k_finish_momx<<<dim3(1), dim3(kBlock), 0, s>>>(partial, nblocks, momx, n_global, dscalar);
+  return hipGetLastError();
+}
+
+}  // namespace lbamd

@@ -1,0 +1,351 @@
+"""ctypes binding of the C ABI in include/badchimp_amd.h.
+
+The shared library is badchimp-cpp_amd/lib/libbadchimp_amd.so, built by
+badchimp-cpp_amd/build.py.  Loading fails loudly when it is missing: there
+is no fallback implementation.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libbadchimp_amd.so")
+
+D2Q9, D3Q19 = 0, 1
+FP64, FP32S, FP16S = 0, 1, 2
+RHO, PHI = 0, 1
+VEL = 0
+FORCE_FIXED, FORCE_MOMX = 0, 1
+TOPO_ALLOW_GHOST0 = 1
+Q_OF = {D2Q9: 9, D3Q19: 19}
+ND_OF = {D2Q9: 2, D3Q19: 3}
+LATTICE_NAME = {"D2Q9": D2Q9, "D3Q19": D3Q19}
+PERIODIC_BITS = {"x": 1, "y": 2, "z": 4}
+
+_lib = None
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i16p = np.ctypeslib.ndpointer(dtype=np.int16, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+
+
+class LBError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("[%d] %s" % (code, msg))
+        self.code = code
+
+
+SIGNATURES = {
+    "lb_last_error": (C.c_char_p, [_vp]),
+    "lb_global_error": (C.c_char_p, []),
+    "lb_version": (C.c_char_p, []),
+    "lb_ctx_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
+    "lb_ctx_destroy": (C.c_int, [_vp]),
+    "lb_set_topology": (C.c_int, [_vp, C.c_int32, _i32p, _i16p, _i32p, C.c_int32, C.c_int32]),
+    "lb_set_global_bulk_count": (C.c_int, [_vp, C.c_int64]),
+    "lb_upload_f": (C.c_int, [_vp, C.c_int32, _f64p]),
+    "lb_download_f": (C.c_int, [_vp, C.c_int32, _f64p]),
+    "lb_init_equilibrium": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _f64p, _f64p]),
+    "lb_download_scalar": (C.c_int, [_vp, C.c_int32, C.c_int32, _f64p]),
+    "lb_download_vector": (C.c_int, [_vp, C.c_int32, C.c_int32, _f64p]),
+    "lb_set_wall_densities": (C.c_int, [_vp, _f64p]),
+    "lb_step_bgk": (C.c_int, [_vp, C.c_int32, C.c_double, _f64p]),
+    "lb_step_cg": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double,
+                             C.c_int32, _f64p]),
+    "lb_last_force": (C.c_int, [_vp, _f64p]),
+    "lb_reduce_flux": (C.c_int, [_vp, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double)]),
+    "lb_solve_permeability": (C.c_int, [_vp, C.c_double, _f64p, C.c_int32, C.c_double, C.c_int64,
+                                        C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "lb_sync": (C.c_int, [_vp]),
+    "lb_kernel_time": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "lb_kernel_time_reset": (C.c_int, [_vp]),
+    "lb_device_bytes": (C.c_int64, [_vp]),
+    "lb_topo_read_vtklb": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(_vp)]),
+    "lb_topo_from_geometry": (C.c_int, [C.c_int, C.c_int32, _i32p, _i32p, C.c_int32, C.c_int32,
+                                        C.POINTER(_vp)]),
+    "lb_topo_destroy": (C.c_int, [_vp]),
+    "lb_topo_num_nodes": (C.c_int32, [_vp]),
+    "lb_topo_num_bulk": (C.c_int32, [_vp]),
+    "lb_topo_num_dims": (C.c_int32, [_vp]),
+    "lb_topo_rank": (C.c_int32, [_vp]),
+    "lb_topo_global_dims": (C.c_int, [_vp, _i32p]),
+    "lb_topo_neighbors": (C.c_int, [_vp, _i32p]),
+    "lb_topo_positions": (C.c_int, [_vp, _i32p]),
+    "lb_topo_node_types": (C.c_int, [_vp, _i16p]),
+    "lb_topo_node_ranks": (C.c_int, [_vp, _i32p]),
+    "lb_topo_bulk": (C.c_int, [_vp, _i32p]),
+    "lb_topo_attribute": (C.c_int, [_vp, C.c_char_p, C.c_int32, _f64p]),
+    "lb_topo_write_vtklb": (C.c_int, [_vp, C.c_char_p, C.c_int32, C.POINTER(C.c_char_p),
+                                      C.POINTER(C.POINTER(C.c_double))]),
+    "lb_sphere_pack": (C.c_int, [C.c_int32, _i32p, C.c_double, C.c_double, C.c_uint64, C.c_int32,
+                                 C.c_int32, C.c_int32, C.c_double, _i32p, _f64p]),
+}
+
+
+def lib():
+    """Load the native library (once)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("native library missing: %s (run badchimp-cpp_amd/build.py)" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check_global(rc):
+    if rc != 0:
+        raise LBError(rc, lib().lb_global_error().decode())
+
+
+def periodic_mask(periodic):
+    m = 0
+    for ch in periodic.lower():
+        m |= PERIODIC_BITS.get(ch, 0)
+    return m
+
+
+def sphere_pack(shape, radius, porosity, seed, periodic="x", walls=True, max_fail=2000,
+                relax=0.97):
+    """Seeded sphere pack (see lb_sphere_pack).  Returns (geo int32 array, info dict)."""
+    dims = np.ascontiguousarray(shape, dtype=np.int32)
+    geo = np.empty(int(np.prod(dims)), dtype=np.int32)
+    info = np.zeros(4)
+    _check_global(lib().lb_sphere_pack(len(dims), dims, float(radius), float(porosity),
+                                       int(seed) & 0xFFFFFFFFFFFFFFFF, periodic_mask(periodic),
+                                       1 if walls else 0, int(max_fail), float(relax), geo, info))
+    d = {"porosity": float(info[0]), "n_spheres": int(info[1]),
+         "min_center_distance": float(info[2]), "trials": int(info[3]), "radius": float(radius),
+         "overlap_allowed": bool(info[2] < 2 * radius - 1e-12)}
+    return geo.reshape(tuple(int(x) for x in dims)), d
+
+
+class Topology:
+    """Node list of one rank (lb_topo)."""
+
+    def __init__(self, handle, lattice):
+        self._h = handle
+        self.lattice = lattice
+        self.Q = Q_OF[lattice]
+        self.nD = ND_OF[lattice]
+
+    @classmethod
+    def from_vtklb(cls, path, lattice):
+        h = _vp()
+        _check_global(lib().lb_topo_read_vtklb(path.encode(), lattice, C.byref(h)))
+        return cls(h, lattice)
+
+    @classmethod
+    def from_geometry(cls, geo, lattice, periodic="", rank=1):
+        geo = np.ascontiguousarray(geo, dtype=np.int32)
+        dims = np.ascontiguousarray(geo.shape, dtype=np.int32)
+        h = _vp()
+        _check_global(lib().lb_topo_from_geometry(lattice, geo.ndim, dims, geo.ravel(),
+                                                  periodic_mask(periodic), rank, C.byref(h)))
+        return cls(h, lattice)
+
+    def close(self):
+        if self._h:
+            lib().lb_topo_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def n_nodes(self):
+        return lib().lb_topo_num_nodes(self._h)
+
+    @property
+    def n_bulk(self):
+        return lib().lb_topo_num_bulk(self._h)
+
+    @property
+    def rank(self):
+        return lib().lb_topo_rank(self._h)
+
+    def global_dims(self):
+        out = np.zeros(self.nD, dtype=np.int32)
+        _check_global(lib().lb_topo_global_dims(self._h, out))
+        return out
+
+    def neighbors(self):
+        out = np.zeros(self.n_nodes * self.Q, dtype=np.int32)
+        _check_global(lib().lb_topo_neighbors(self._h, out))
+        return out.reshape(self.n_nodes, self.Q)
+
+    def positions(self):
+        out = np.zeros(self.n_nodes * self.nD, dtype=np.int32)
+        _check_global(lib().lb_topo_positions(self._h, out))
+        return out.reshape(self.n_nodes, self.nD)
+
+    def node_types(self):
+        out = np.zeros(self.n_nodes, dtype=np.int16)
+        _check_global(lib().lb_topo_node_types(self._h, out))
+        return out
+
+    def node_ranks(self):
+        out = np.zeros(self.n_nodes, dtype=np.int32)
+        _check_global(lib().lb_topo_node_ranks(self._h, out))
+        return out
+
+    def bulk(self):
+        out = np.zeros(max(self.n_bulk, 1), dtype=np.int32)
+        _check_global(lib().lb_topo_bulk(self._h, out))
+        return out[: self.n_bulk]
+
+    def attribute(self, name, as_float=False):
+        out = np.zeros(self.n_nodes)
+        _check_global(lib().lb_topo_attribute(self._h, name.encode(), 1 if as_float else 0, out))
+        return out
+
+    def write_vtklb(self, path, attributes=()):
+        names = (C.c_char_p * max(len(attributes), 1))()
+        vals = (C.POINTER(C.c_double) * max(len(attributes), 1))()
+        keep = []
+        for k, (name, v) in enumerate(attributes):
+            arr = np.ascontiguousarray(v, dtype=np.float64)
+            assert arr.shape == (self.n_nodes,)
+            keep.append(arr)
+            names[k] = name.encode()
+            vals[k] = arr.ctypes.data_as(C.POINTER(C.c_double))
+        _check_global(lib().lb_topo_write_vtklb(self._h, path.encode(), len(attributes), names, vals))
+
+
+class Context:
+    """One rank's subdomain on one GPU (lb_ctx)."""
+
+    def __init__(self, lattice, precision=FP64, device=0):
+        h = _vp()
+        _check_global(lib().lb_ctx_create(lattice, precision, device, C.byref(h)))
+        self._h = h
+        self.lattice = lattice
+        self.precision = precision
+        self.Q = Q_OF[lattice]
+        self.nD = ND_OF[lattice]
+        self.n_nodes = 0
+
+    def _check(self, rc):
+        if rc != 0:
+            raise LBError(rc, lib().lb_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            lib().lb_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # topology
+    def set_topology(self, neighbor, node_type, bulk, allow_ghost0=False):
+        neighbor = np.ascontiguousarray(neighbor, dtype=np.int32)
+        node_type = np.ascontiguousarray(node_type, dtype=np.int16)
+        bulk = np.ascontiguousarray(bulk, dtype=np.int32)
+        n_nodes = node_type.shape[0]
+        assert neighbor.size == n_nodes * self.Q
+        self._check(lib().lb_set_topology(self._h, n_nodes, neighbor.ravel(), node_type,
+                                          bulk if bulk.size else np.zeros(1, np.int32),
+                                          bulk.size, TOPO_ALLOW_GHOST0 if allow_ghost0 else 0))
+        self.n_nodes = n_nodes
+
+    def set_topology_from(self, topo, allow_ghost0=False):
+        self.set_topology(topo.neighbors(), topo.node_types(), topo.bulk(), allow_ghost0)
+
+    def set_global_bulk_count(self, n):
+        self._check(lib().lb_set_global_bulk_count(self._h, int(n)))
+
+    # fields
+    def upload_f(self, f):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        nf = f.size // (self.n_nodes * self.Q)
+        self._check(lib().lb_upload_f(self._h, nf, f.ravel()))
+
+    def download_f(self, nf=1, out=None):
+        if out is None:
+            out = np.zeros(self.n_nodes * nf * self.Q)
+        self._check(lib().lb_download_f(self._h, nf, out))
+        return out.reshape(self.n_nodes, nf, self.Q)
+
+    def init_equilibrium(self, n_fields, field, rho, vel, rho_field=0):
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        vel = np.ascontiguousarray(vel, dtype=np.float64)
+        nfr = rho.size // self.n_nodes
+        self._check(lib().lb_init_equilibrium(self._h, n_fields, field, nfr, rho_field, rho.ravel(),
+                                              vel.ravel()))
+
+    def download_scalar(self, which=RHO, nf=1):
+        out = np.zeros(self.n_nodes * nf)
+        self._check(lib().lb_download_scalar(self._h, which, nf, out))
+        return out.reshape(self.n_nodes, nf) if nf > 1 else out
+
+    def download_vel(self):
+        out = np.zeros(self.n_nodes * self.nD)
+        self._check(lib().lb_download_vector(self._h, VEL, 1, out))
+        return out.reshape(self.n_nodes, self.nD)
+
+    def set_wall_densities(self, rho2):
+        self._check(lib().lb_set_wall_densities(self._h, np.ascontiguousarray(rho2, dtype=np.float64).ravel()))
+
+    # steps
+    def step_bgk(self, nsteps, tau, force):
+        F = np.zeros(3)
+        F[: self.nD] = force[: self.nD]
+        self._check(lib().lb_step_bgk(self._h, int(nsteps), float(tau), F))
+
+    def step_cg(self, nsteps, tau0, tau1, sigma, beta, force_mode, force):
+        F = np.zeros(3)
+        F[: len(force)] = force
+        self._check(lib().lb_step_cg(self._h, int(nsteps), float(tau0), float(tau1), float(sigma),
+                                     float(beta), int(force_mode), F))
+
+    def last_force(self):
+        out = np.zeros(3)
+        self._check(lib().lb_last_force(self._h, out))
+        return out[: self.nD]
+
+    def reduce_flux(self, direction=0):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        self._check(lib().lb_reduce_flux(self._h, direction, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def solve_permeability(self, tau, force, check_every, tol, max_steps, n_box):
+        F = np.zeros(3)
+        F[: self.nD] = force[: self.nD]
+        k, s = C.c_double(), C.c_int64()
+        self._check(lib().lb_solve_permeability(self._h, float(tau), F, int(check_every), float(tol),
+                                                int(max_steps), float(n_box), C.byref(k), C.byref(s)))
+        return k.value, s.value
+
+    def sync(self):
+        self._check(lib().lb_sync(self._h))
+
+    def kernel_time(self):
+        ms, n = C.c_double(), C.c_int64()
+        self._check(lib().lb_kernel_time(self._h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def kernel_time_reset(self):
+        self._check(lib().lb_kernel_time_reset(self._h))
+
+    def device_bytes(self):
+        return lib().lb_device_bytes(self._h)

@@ -1,0 +1,184 @@
+"""Benchmark: D3Q19 BGK + Guo collide-stream on a 256^3 sphere pack (one GPU).
+
+Workload (the single-GPU configuration the metric is quoted on): D3Q19
+single-phase BGK, tau = 1, body force g_x = 1e-6, 256^3 box periodic in x
+with solid walls on the y and z faces, filled with a seeded sphere pack of
+radius 8 at porosity 0.40, solids as half-way bounce-back.
+
+value    = MLUPS of the FP32-shifted form (shifted populations in float,
+           float arithmetic, FP64 flux reductions), whole run, inputs resident.
+fp64     = the same for the plain FP64 form (the reference's arithmetic).
+roofline = the dominant kernel (the collide-stream step) against 8 TB/s,
+           with algorithmic bytes per lattice update = 2 Q s
+           (populations read once and written once; s = 4 B FP32S, 8 B FP64).
+permeability = k of both forms run to |dk|/k < 1e-7 (checked every 100
+           steps), and the relative error of FP32S against FP64.
+cpu_baseline = the oracle's CPU restatement of the reference loop on the same
+           node list, on the host's cores, for a bounded number of steps.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "badchimp-cpp_amd"))
+
+import lbamd  # noqa: E402
+
+METRIC = "D3Q19 MLUPS & %HBM roofline; batched perm solves/s + p99 at ≤1e-4 k err"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(size, seed=1):
+    geo, info = lbamd.sphere_pack((size, size, size), 8.0, 0.40, seed, periodic="x", walls=True)
+    topo = lbamd.Topology.from_geometry(geo, lbamd.D3Q19, "x")
+    return geo, info, topo
+
+
+def rest_state(n_nodes, bulk):
+    import oracle  # only for the weights table; the GPU legs do not use the oracle
+    return oracle.rest_state(1, n_nodes, bulk, np.ones(n_nodes))
+
+
+def gpu_leg(topo, nb, types, bulk, precision, steps, warmup, tau, force):
+    ctx = lbamd.Context(lbamd.D3Q19, precision, 0)
+    ctx.set_topology(nb, types, bulk)
+    w = np.array([2.0 / 36.0] * 3 + [1.0 / 36.0] * 6 + [2.0 / 36.0] * 3 + [1.0 / 36.0] * 6 + [12.0 / 36.0])
+    f0 = np.zeros((topo.n_nodes, 19))
+    f0[bulk] = w[None, :]
+    ctx.upload_f(f0)
+    if warmup:
+        ctx.step_bgk(warmup, tau, force)
+    ctx.sync()
+    ctx.kernel_time_reset()
+    t0 = time.perf_counter()
+    ctx.step_bgk(steps, tau, force)
+    ctx.sync()
+    t1 = time.perf_counter()
+    kms, launches = ctx.kernel_time()
+    ctx.close()
+    return t1 - t0, kms / max(launches, 1), launches
+
+
+def permeability(nb, types, bulk, n_nodes, precision, tau, force, n_box, tol=1e-7, max_steps=60000):
+    ctx = lbamd.Context(lbamd.D3Q19, precision, 0)
+    ctx.set_topology(nb, types, bulk)
+    w = np.array([2.0 / 36.0] * 3 + [1.0 / 36.0] * 6 + [2.0 / 36.0] * 3 + [1.0 / 36.0] * 6 + [12.0 / 36.0])
+    f0 = np.zeros((n_nodes, 19))
+    f0[bulk] = w[None, :]
+    ctx.upload_f(f0)
+    t0 = time.perf_counter()
+    k, steps = ctx.solve_permeability(tau, force, 100, tol, max_steps, n_box)
+    dt = time.perf_counter() - t0
+    ctx.close()
+    return k, steps, dt
+
+
+def cpu_baseline(nb, types, bulk, n_nodes, tau, force, budget_s=15.0):
+    import oracle
+    threads = min(16, os.cpu_count() or 1)
+    f0 = rest_state(n_nodes, bulk)
+    # one step to size the sample, then as many as fit the budget (at most 50)
+    t0 = time.perf_counter()
+    oracle.run_std_case(1, nb, types, bulk, f0, tau, force, 1, nthreads=threads)
+    one = time.perf_counter() - t0
+    steps = int(max(1, min(50, budget_s / max(one, 1e-6))))
+    t0 = time.perf_counter()
+    oracle.run_std_case(1, nb, types, bulk, f0, tau, force, steps, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": bulk.size * steps / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": "oracle std_case loop (push + bounce-back, FP64) on the same 256^3 node list, "
+                      "%d steps, %d OpenMP threads" % (steps, threads)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-perm", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every box here has one GPU and the engine runs one "
+              "subdomain per GPU; --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    tau, force = 1.0, np.array([1e-6, 0.0, 0.0])
+    t0 = time.perf_counter()
+    geo, info, topo = make_workload(args.size)
+    nb, types, bulk = topo.neighbors(), topo.node_types(), topo.bulk()
+    n_bulk = int(bulk.size)
+    log("workload: %d^3, %d bulk nodes, porosity %.4f, set-up %.1f s" % (args.size, n_bulk, info["porosity"], time.perf_counter() - t0))
+
+    t32, k32_ms, n32 = gpu_leg(topo, nb, types, bulk, lbamd.FP32S, args.steps, args.warmup, tau, force)
+    mlups32 = n_bulk * args.steps / t32 / 1e6
+    log("FP32S: %.1f MLUPS, kernel %.3f ms" % (mlups32, k32_ms))
+    t64, k64_ms, n64 = gpu_leg(topo, nb, types, bulk, lbamd.FP64, args.steps, args.warmup, tau, force)
+    mlups64 = n_bulk * args.steps / t64 / 1e6
+    log("FP64 : %.1f MLUPS, kernel %.3f ms" % (mlups64, k64_ms))
+
+    bytes32 = 2 * 19 * 4
+    bytes64 = 2 * 19 * 8
+    ach32 = n_bulk * bytes32 / (k32_ms * 1e-3) / 1e9
+    ach64 = n_bulk * bytes64 / (k64_ms * 1e-3) / 1e9
+
+    perm = None
+    if not args.no_perm:
+        n_box = float(geo.size)
+        k64, s64, d64 = permeability(nb, types, bulk, topo.n_nodes, lbamd.FP64, tau, force, n_box)
+        log("k FP64 = %.10e after %d steps (%.1f s)" % (k64, s64, d64))
+        k32, s32, d32 = permeability(nb, types, bulk, topo.n_nodes, lbamd.FP32S, tau, force, n_box)
+        log("k FP32S = %.10e after %d steps (%.1f s)" % (k32, s32, d32))
+        perm = {"k_fp64": k64, "steps_fp64": s64, "k_fp32s": k32, "steps_fp32s": s32,
+                "k_rel_err_fp32s": abs(k32 - k64) / abs(k64), "tol_dk_over_k": 1e-7,
+                "definition": "k = nu * sum_bulk(sum_q f c_x + F_x/2) / (N_box * F_x), nu = (tau-1/2)/3"}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(nb, types, bulk, topo.n_nodes, tau, force)
+        log("CPU baseline: %.2f MLUPS on %d threads" % (cpu["value"], cpu["cores"]))
+
+    line = {
+        "metric": METRIC,
+        "value": round(mlups32, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t32 / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (shifted populations, FP64 reductions)",
+        "data": "synthetic (seeded sphere pack)",
+        "config": {"workload": "D3Q19 BGK+Guo collide-stream, %d^3 sphere pack r=8 porosity %.3f, "
+                               "periodic x, walls y/z, tau=1, F=1e-6" % (args.size, info["porosity"]),
+                   "bulk_nodes": n_bulk, "form": "FP32-shifted", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_bgk_lp<D3Q19,float>", "avg_kernel_ms": k32_ms,
+                     "bytes_per_lup": bytes32},
+        "fp64": {"value": round(mlups64, 2), "unit": "MLUPS", "avg_kernel_ms": k64_ms,
+                 "achieved_GBs": round(ach64, 1), "frac": round(ach64 / HBM_PEAK_GBS, 4),
+                 "bytes_per_lup": bytes64},
+        "permeability": perm,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

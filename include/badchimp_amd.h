@@ -1,0 +1,219 @@
+/*
+ * badchimp_amd.h -- C ABI of the MI355X lattice-Boltzmann engine.
+ *
+ * One context is one rank's subdomain on one GPU: the labelled sparse node
+ * list with its neighbour index (as Grid/Nodes hold it), the population
+ * fields (as LbField holds them) and the macroscopic fields (as ScalarField /
+ * VectorField hold them), resident in HBM.  A BADChIMP case main keeps its
+ * set-up (LBvtk, Grid, Nodes, findBulkNodes) and replaces its per-node loop,
+ * LbField::swapData and HalfWayBounceBack::apply with one step call.
+ *
+ * Conventions
+ *   - Every call returns 0 on success or a negative LB_ERR_* code; the message
+ *     is available from lb_last_error(ctx) (or lb_global_error() for calls
+ *     without a context).  No call exits the process.
+ *   - Host arrays are copied in or out; the library keeps no host pointers.
+ *   - Node numbers are the reference's: node 0 is the ghost node, nodes
+ *     1..n_nodes-1 are the points of the .vtklb file.
+ *   - Arrays use the reference layouts:
+ *       neighbor  int32[n_nodes*Q]      neighbor[n*Q + q]   (Grid::neigList_, LBgrid.h:119,187-198)
+ *       node_type int16[n_nodes]        -1..4               (Nodes::nodeType_, LBnodes.h:47-54)
+ *       LbField   double[n_nodes*nF*Q]  f[(nF*Q)*n + Q*field + q]   (LBfield.h:298-305)
+ *       Scalar    double[n_nodes*nF]    s[nF*n + field]     (LBfield.h:92-100)
+ *       Vector    double[n_nodes*nF*nD] v[nF*nD*n + nD*field + d]   (LBfield.h:168-175)
+ *     Only bulk-node entries are read or written by the field calls.
+ *   - Calls are synchronous unless named *_async.  A context is not
+ *     thread-safe; use one context per rank or per host thread.
+ */
+#ifndef BADCHIMP_AMD_H
+#define BADCHIMP_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ constants */
+enum lb_lattice { LB_D2Q9 = 0, LB_D3Q19 = 1 };
+
+/* Population storage.  FP64 is the reference's arithmetic (lbBase_t = double,
+ * LBglobal.h:10).  FP32S / FP16S store the shifted populations f - w_q in
+ * float / half and compute in float; flux reductions stay in double. */
+enum lb_precision { LB_FP64 = 0, LB_FP32S = 1, LB_FP16S = 2 };
+
+enum lb_status {
+  LB_OK = 0,
+  LB_ERR_ARG = -1,           /* invalid argument */
+  LB_ERR_STATE = -2,         /* call out of order (e.g. step before topology) */
+  LB_ERR_GHOST0_LINK = -3,   /* a bulk node links to the ghost node 0 */
+  LB_ERR_UNSUPPORTED = -4,   /* e.g. other-rank fluid neighbours on one GPU */
+  LB_ERR_HIP = -5,           /* HIP runtime error */
+  LB_ERR_IO = -6,            /* file could not be read or parsed */
+  LB_ERR_NOMEM = -7          /* device or host allocation failed */
+};
+
+/* lb_set_topology flags */
+#define LB_TOPO_ALLOW_GHOST0 1 /* accept fluid links to node 0 and reproduce the
+                                  reference's shared-slot result exactly */
+
+/* field selectors */
+enum lb_scalar { LB_RHO = 0, LB_PHI = 1 };
+enum lb_vector { LB_VEL = 0 };
+
+/* two-phase body-force modes */
+enum lb_force_mode {
+  LB_FORCE_FIXED = 0, /* F given */
+  LB_FORCE_MOMX = 1   /* F_x = 2 (momx - <sum_q (f0+f1) c_x>), main_TWOPHASE.cpp:292-308 */
+};
+
+typedef struct lb_ctx lb_ctx;
+typedef struct lb_topo lb_topo;
+
+/* ------------------------------------------------------------ errors */
+const char* lb_last_error(const lb_ctx* ctx);
+const char* lb_global_error(void);
+const char* lb_version(void);
+
+/* ------------------------------------------------------------ context */
+/* Replaces the allocation of LbField f, fTmp / ScalarField / VectorField on
+ * the host (std_case/main.cpp:65-90, main_TWOPHASE.cpp:138-144). */
+int lb_ctx_create(int lattice, int precision, int device, lb_ctx** out);
+int lb_ctx_destroy(lb_ctx* ctx);
+
+/* Node list + neighbour index + node types + bulk list.  Replaces what the
+ * per-node loop reads from Grid (LBgrid.h:187-198), Nodes (LBnodes.h:72-83),
+ * findBulkNodes (LBgeometry.h:11-21) and the link classification of
+ * BoundaryHalwWayHelper (LBhalfwayhelperclass.h:110-161).  bulk must be
+ * ascending (findBulkNodes order); it is the node order of every sum. */
+int lb_set_topology(lb_ctx* ctx, int32_t n_nodes, const int32_t* neighbor,
+                    const int16_t* node_type, const int32_t* bulk,
+                    int32_t n_bulk, int32_t flags);
+
+/* Total bulk-node count over all ranks, the divisor of the flux mean
+ * (main_TWOPHASE.cpp:197-199,301).  Defaults to n_bulk. */
+int lb_set_global_bulk_count(lb_ctx* ctx, int64_t n);
+
+/* ------------------------------------------------------------ fields */
+/* Populations of n_fields fields (1: single phase, 2: two phase) in the
+ * LbField layout over n_nodes nodes (LBfield.h:285-305).  upload sets the
+ * pre-collision state f_t; download returns it. */
+int lb_upload_f(lb_ctx* ctx, int32_t n_fields, const double* f);
+int lb_download_f(lb_ctx* ctx, int32_t n_fields, double* f);
+
+/* Equilibrium initialisation f = w rho (1 + 3cu + 4.5(cu^2 - u^2/3)) of field
+ * `field` (of n_fields population fields) from ScalarField rho (field
+ * rho_field of rho_nfields) and VectorField vel (field 0), over bulk nodes
+ * (LBinitiatefield.h:33-57).  A call with a different n_fields, or after a
+ * step, starts a new state in which every field is to be initialised. */
+int lb_init_equilibrium(lb_ctx* ctx, int32_t n_fields, int32_t field,
+                        int32_t rho_nfields, int32_t rho_field,
+                        const double* rho, const double* vel);
+
+/* Macroscopic fields stored by the last step call: rho (1 field single
+ * phase, 2 fields two phase), phi (colour field, 1 field) and vel (1
+ * field).  They are the values of the pre-collision state of the last step,
+ * as std_case/main.cpp:119-120 and main_TWOPHASE.cpp:241-245,329 store them. */
+int lb_download_scalar(lb_ctx* ctx, int32_t which, int32_t n_fields, double* s);
+int lb_download_vector(lb_ctx* ctx, int32_t which, int32_t n_fields, double* v);
+
+/* Densities of the solid-boundary nodes for the two-phase wettability
+ * (main_TWOPHASE.cpp:171-178,280-284): ScalarField rho with 2 fields over
+ * n_nodes; only solid-boundary (type 1) entries are read. */
+int lb_set_wall_densities(lb_ctx* ctx, const double* rho2);
+
+/* ------------------------------------------------------------ steps */
+/* nsteps iterations of the std_case loop body (std_case/main.cpp:110-145):
+ * BGK + Guo forcing, streaming and half-way bounce-back, force uniform. */
+int lb_step_bgk(lb_ctx* ctx, int32_t nsteps, double tau, const double* force);
+
+/* nsteps iterations of the colour-gradient loop body
+ * (main_TWOPHASE.cpp:238-392).  force[nD]: in LB_FORCE_MOMX mode force[0]
+ * is the target momentum momx and force[1..] the fixed other components. */
+int lb_step_cg(lb_ctx* ctx, int32_t nsteps, double tau0, double tau1,
+               double sigma, double beta, int32_t force_mode,
+               const double* force);
+
+/* Body force used by the last two-phase step (force.dat of main_TWOPHASE). */
+int lb_last_force(lb_ctx* ctx, double* force);
+
+/* FP64 device reductions over bulk nodes of the pre-collision state
+ * (LBglobalforcing.h:8-33,61-78):
+ *   sum_fc      = sum_n sum_q f_q c_q,dir         (f = f0 + f1 for two phase)
+ *   sum_phi0_fc = sum_n (rho0/rho) sum_q f_q c_q,dir
+ *   sum_phi1_fc = sum_n (rho1/rho) sum_q f_q c_q,dir
+ * Any output pointer may be NULL. */
+int lb_reduce_flux(lb_ctx* ctx, int32_t dir, double* sum_fc,
+                   double* sum_phi0_fc, double* sum_phi1_fc);
+
+/* Single-phase permeability solve on the device: step BGK until
+ * |k_t - k_{t-M}| / |k_t| < tol, checked every M = check_every steps, or
+ * max_steps.  k = nu * (sum_n j_x) / (n_box * F_x) with j = sum_q f c + F/2
+ * and nu = (tau - 1/2)/3 (std_one_phase/main.cpp:603-624 uses the same
+ * relative-change test).  Returns k and the steps taken. */
+int lb_solve_permeability(lb_ctx* ctx, double tau, const double* force,
+                          int32_t check_every, double tol, int64_t max_steps,
+                          double n_box, double* k_out, int64_t* steps_out);
+
+/* ------------------------------------------------------------ timing */
+int lb_sync(lb_ctx* ctx);
+/* Device time (HIP events on the context's stream) summed over the step
+ * kernels launched since the last reset, and the number of launches. */
+int lb_kernel_time(lb_ctx* ctx, double* ms, int64_t* launches);
+int lb_kernel_time_reset(lb_ctx* ctx);
+/* Bytes of device memory the context holds. */
+int64_t lb_device_bytes(const lb_ctx* ctx);
+
+/* ------------------------------------------------------------ topology host side */
+/* .vtklb reader (LBvtk<DXQY>, LBvtk.h:221-614) followed by Grid
+ * (LBgrid.h:127-148), Nodes::setupNodeType (LBnodes.h:112-209) and
+ * findBulkNodes (LBgeometry.h:11-21). */
+int lb_topo_read_vtklb(const char* path, int lattice, lb_topo** out);
+
+/* Node list and neighbour index of one rank built natively from a voxel
+ * geometry with the reference writer's numbering (PythonScripts/vtklb.py:
+ * 6-44,70-124,182-218): geo[x][y][z] (C order, dims[0] slowest) is 0 for
+ * solid and the 1-based rank for fluid; periodic is a bit mask (1 = x,
+ * 2 = y, 4 = z).  Node types are set as Nodes does for that rank. */
+int lb_topo_from_geometry(int lattice, int32_t nd, const int32_t* dims,
+                          const int32_t* geo, int32_t periodic, int32_t rank,
+                          lb_topo** out);
+int lb_topo_destroy(lb_topo* t);
+int32_t lb_topo_num_nodes(const lb_topo* t);  /* incl. node 0 */
+int32_t lb_topo_num_bulk(const lb_topo* t);
+int32_t lb_topo_num_dims(const lb_topo* t);
+int32_t lb_topo_rank(const lb_topo* t);
+int lb_topo_global_dims(const lb_topo* t, int32_t* dims);  /* padded dims */
+int lb_topo_neighbors(const lb_topo* t, int32_t* neighbor);   /* [n_nodes*Q] */
+int lb_topo_positions(const lb_topo* t, int32_t* pos);        /* [n_nodes*nD], node 0 = -1 */
+int lb_topo_node_types(const lb_topo* t, int16_t* types);     /* [n_nodes] */
+int lb_topo_node_ranks(const lb_topo* t, int32_t* ranks);     /* [n_nodes] */
+int lb_topo_bulk(const lb_topo* t, int32_t* bulk);            /* [n_bulk] */
+/* Point attribute of a .vtklb file (SCALARS block), as double over n_nodes
+ * (node 0 = 0); as_float rounds each value through float first, the way
+ * main_TWOPHASE.cpp:158,166,173 reads rho0/rho1/wettability. */
+int lb_topo_attribute(const lb_topo* t, const char* name, int32_t as_float,
+                      double* out);
+/* Write the topology as a .vtklb file in the reference writer's format
+ * (PythonScripts/vtklb.py:222-295), with optional float attributes given
+ * over n_nodes (node 0 ignored). */
+int lb_topo_write_vtklb(const lb_topo* t, const char* path, int32_t n_attr,
+                        const char* const* names, const double* const* values);
+
+/* Seeded sphere pack (geometry generator, not in the reference's C++):
+ * spheres of `radius` placed by random sequential addition with centre
+ * distance >= 2r, relaxed by `relax` after `max_fail` consecutive failures,
+ * until the porosity of the non-wall region is <= porosity.  periodic: bit
+ * mask of axes along which spheres wrap; walls != 0 adds one solid layer on
+ * the faces of the other axes.  geo: int32[prod(dims)], 1 fluid / 0 solid.
+ * info[4] = {porosity, n_spheres, final min centre distance, trials}. */
+int lb_sphere_pack(int32_t nd, const int32_t* dims, double radius,
+                   double porosity, uint64_t seed, int32_t periodic,
+                   int32_t walls, int32_t max_fail, double relax,
+                   int32_t* geo, double* info);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BADCHIMP_AMD_H */

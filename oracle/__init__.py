@@ -1,0 +1,134 @@
+"""Oracle: CPU restatement of the reference's hot path (test infrastructure only).
+
+Only tests/, tests/golden/ scripts, ``__graft_entry__.smoke()`` and the
+CPU-baseline leg of ``bench.py`` import this package, and only as the checker.
+
+  oracle.vtklb_oracle   numpy restatement of PythonScripts/vtklb.py (node list)
+  oracle.oracle.c       C restatement of the std_case / twophase loops
+  oracle.run_*          ctypes wrappers of liboracle.so (built by oracle/Makefile)
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i16p = np.ctypeslib.ndpointer(dtype=np.int16, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_lib = None
+
+Q_OF = {0: 9, 1: 19}
+ND_OF = {0: 2, 1: 3}
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.orc_std_case.restype = C.c_int
+        L.orc_std_case.argtypes = [C.c_int, C.c_int32, _i32p, _i16p, _i32p, C.c_int32, _f64p, _f64p,
+                                   C.c_double, _f64p, C.c_int32, _f64p, _f64p, C.c_int]
+        L.orc_twophase.restype = C.c_int
+        L.orc_twophase.argtypes = [C.c_int, C.c_int32, _i32p, _i16p, _i32p, C.c_int32, _f64p, _f64p,
+                                   _f64p, _f64p, _f64p, C.c_double, C.c_double, C.c_double,
+                                   C.c_double, C.c_int, _f64p, C.c_int64, C.c_int32, _f64p]
+        L.orc_flux.restype = C.c_int
+        L.orc_flux.argtypes = [C.c_int, C.c_int32, _i32p, C.c_int32, _f64p, C.c_int, C.c_int, _f64p]
+        L.orc_node_types.restype = C.c_int
+        L.orc_node_types.argtypes = [C.c_int, C.c_int32, _i32p, _i32p, _i32p, C.c_int32, _i16p]
+        _lib = L
+    return _lib
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def run_std_case(lattice, neighbor, node_type, bulk, f, tau, force, nsteps, nthreads=1):
+    """nsteps iterations of the std_case body.  f: LbField (n_nodes, Q), updated
+    copy returned with the rho / vel of the last iteration."""
+    n_nodes = node_type.shape[0]
+    Q, D = Q_OF[lattice], ND_OF[lattice]
+    f = _c(f, np.float64).reshape(-1).copy()
+    ftmp = f.copy()
+    rho = np.zeros(n_nodes)
+    vel = np.zeros(n_nodes * D)
+    F = np.zeros(3)
+    F[:D] = force[:D]
+    bulk = _c(bulk, np.int32)
+    lib().orc_std_case(lattice, n_nodes, _c(neighbor, np.int32).ravel(), _c(node_type, np.int16),
+                       bulk if bulk.size else np.zeros(1, np.int32), bulk.size, f, ftmp,
+                       float(tau), F, int(nsteps), rho, vel, int(nthreads))
+    return f.reshape(n_nodes, Q), rho, vel.reshape(n_nodes, D)
+
+
+def run_twophase(lattice, neighbor, node_type, bulk, f, rho, tau0, tau1, sigma, beta,
+                 force_mode, force, nsteps, n_global=None):
+    """nsteps iterations of the twophase body.  f: LbField (n_nodes, 2, Q);
+    rho: ScalarField (n_nodes, 2) carrying the wall densities.  Returns
+    (f, rho, vel, cg, force_of_last_iteration)."""
+    n_nodes = node_type.shape[0]
+    Q, D = Q_OF[lattice], ND_OF[lattice]
+    f = _c(f, np.float64).reshape(-1).copy()
+    ftmp = f.copy()
+    rho = _c(rho, np.float64).reshape(-1).copy()
+    vel = np.zeros(n_nodes * D)
+    cg = np.zeros(n_nodes)
+    F = np.zeros(3)
+    F[: len(force)] = force
+    Fo = np.zeros(3)
+    bulk = _c(bulk, np.int32)
+    lib().orc_twophase(lattice, n_nodes, _c(neighbor, np.int32).ravel(), _c(node_type, np.int16),
+                       bulk, bulk.size, f, ftmp, rho, vel, cg, float(tau0), float(tau1),
+                       float(sigma), float(beta), int(force_mode), F,
+                       int(n_global if n_global is not None else bulk.size), int(nsteps), Fo)
+    return (f.reshape(n_nodes, 2, Q), rho.reshape(n_nodes, 2), vel.reshape(n_nodes, D), cg,
+            Fo[:D])
+
+
+def flux(lattice, bulk, f, n_fields, direction):
+    f = _c(f, np.float64).ravel()
+    out = np.zeros(3)
+    bulk = _c(bulk, np.int32)
+    n_nodes = f.size // (n_fields * Q_OF[lattice])
+    lib().orc_flux(lattice, n_nodes, bulk, bulk.size, f, n_fields, direction, out)
+    return out
+
+
+def node_types(lattice, neighbor, nodetype_attr, ranks, my_rank):
+    n_nodes = nodetype_attr.shape[0]
+    out = np.zeros(n_nodes, dtype=np.int16)
+    lib().orc_node_types(lattice, n_nodes, _c(neighbor, np.int32).ravel(), _c(nodetype_attr, np.int32),
+                         _c(ranks, np.int32), int(my_rank), out)
+    return out
+
+
+def bulk_nodes(node_type, ranks, my_rank):
+    """findBulkNodes (LBgeometry.h:11-21): fluid nodes of this rank, ascending."""
+    n = np.arange(node_type.shape[0])
+    return n[(n >= 1) & (node_type > 1) & (ranks == my_rank)].astype(np.int32)
+
+
+def rest_state(lattice, n_nodes, bulk, rho):
+    """f = w rho on bulk nodes (std_case/main.cpp:92-96), zeros elsewhere."""
+    w = WEIGHTS[lattice]
+    f = np.zeros((n_nodes, Q_OF[lattice]))
+    f[bulk] = w[None, :] * np.asarray(rho)[bulk, None]
+    return f
+
+
+WEIGHTS = {
+    0: np.array([4.0 / 36.0, 1.0 / 36.0] * 4 + [16.0 / 36.0]),
+    1: np.array([2.0 / 36.0] * 3 + [1.0 / 36.0] * 6 + [2.0 / 36.0] * 3 + [1.0 / 36.0] * 6
+                + [12.0 / 36.0]),
+}

@@ -1,0 +1,48 @@
+#!/usr/bin/env bash
+# Build the reference BADChIMP cases used as the parity checker, from the
+# sources where they lie under /root/reference, into oracle/_ref/ only.
+#
+#   std_case_d2q9   src/std_case/main.cpp as shipped (#define LT D2Q9)
+#   std_case_d3q19  the same file with its lattice line switched to D3Q19
+#                   (std_case/main.cpp:14-15 offers the switch); the edited
+#                   copy is written to oracle/_ref/ and never committed
+#   twophase        src/twophase/main_TWOPHASE.cpp as shipped (D3Q19)
+#
+# Flags follow the reference release build (CMakeLists.txt:71): -O3 -fopenmp
+# -std=c++17.  MPI is MPICH from /opt/conda (the reference links MPI via
+# find_package).  At run time only libmpi.so.12 goes on LD_LIBRARY_PATH
+# (oracle/_ref/mpilib), because the rest of /opt/conda/lib carries an old
+# libstdc++ (with the Fortran runtime libmpi itself needs).
+#
+# Needs /root/reference; exits 0 without building when it is absent.
+set -euo pipefail
+REF=/root/reference/src
+HERE="$(cd "$(dirname "$0")" && pwd)"
+OUT="$HERE/_ref"
+if [ ! -d "$REF" ]; then
+  echo "build_ref: /root/reference absent, nothing to build"
+  exit 0
+fi
+if [ ! -e /opt/conda/lib/libmpi.so ]; then
+  echo "build_ref: MPICH (/opt/conda/lib/libmpi.so) absent, reference unbuildable"
+  exit 0
+fi
+mkdir -p "$OUT/mpilib"
+for lib in libmpi.so.12 libgfortran.so.4 libquadmath.so.0; do
+  ln -sf "/opt/conda/lib/$lib" "$OUT/mpilib/$lib"
+done
+CXXFLAGS="-std=c++17 -O3 -fopenmp -w -I/opt/conda/include"
+LIBS="$REF/lbsolver/LBd2q9.cpp $REF/lbsolver/LBd3q19.cpp /opt/conda/lib/libmpi.so"
+
+sed 's/^#define LT D2Q9/#define LT D3Q19/' "$REF/std_case/main.cpp" > "$OUT/std_case_d3q19.cpp"
+
+build() {  # name, main source, extra include dir
+  local name=$1 src=$2 inc=$3
+  if [ ! -x "$OUT/$name" ] || [ "$src" -nt "$OUT/$name" ]; then
+    g++ $CXXFLAGS -I"$inc" "$src" $LIBS -o "$OUT/$name"
+    echo "build_ref: built $name"
+  fi
+}
+build std_case_d2q9 "$REF/std_case/main.cpp" "$REF/std_case"
+build std_case_d3q19 "$OUT/std_case_d3q19.cpp" "$REF/std_case"
+build twophase "$REF/twophase/main_TWOPHASE.cpp" "$REF/twophase"

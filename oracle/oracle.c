@@ -1,0 +1,424 @@
+/*
+ * oracle.c -- CPU restatement of the reference's hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * CPU-baseline leg of bench.py load this library, and only as the checker
+ * (or as the timed CPU baseline).  The product never calls it.
+ *
+ * It follows the reference step by step, in its data layout (AoS LbField,
+ * LBfield.h:298-305), with push streaming (LBfield.h:350-357), swap
+ * (LBfield.h:359) and half-way bounce-back after the push
+ * (LBhalfwayhelperclass.h:110-161, LBhalfwaybb.h:37-63).  The moment and
+ * collision formulas are written with the reference's explicit term order:
+ *   D2Q9  tables and helpers   LBd2q9.h:14-40, 128-177
+ *   D3Q19 tables and helpers   LBd3q19.h:14-40, 111-190
+ *   calcRho / calcVel          LBmacroscopic.h:11-45
+ *   calcOmegaBGK               LBcollision.h:27-46
+ *   calcDeltaOmegaQ            LBcollision.h:79-98
+ *   calcDeltaOmegaF            LBcollision.h:194-213
+ *   calcDeltaOmegaST / RC      LBcollision2phase.h:7-20, 70-86
+ *   grad / vecNorm             LButilities.h:12-22, 62-66
+ *   std_case loop              std_case/main.cpp:109-145
+ *   twophase loop              twophase/main_TWOPHASE.cpp:236-392
+ *   flux sums                  LBglobalforcing.h:8-33, 61-78
+ *
+ * Build: see oracle/Makefile (gcc -O2 -ffp-contract=off, SSE2 doubles as the
+ * reference's x86-64 build uses).
+ *
+ * Pinned by: the compiled reference's own outputs on the committed
+ * geometries (tests/golden/ref_*.npz) -- bit-exact -- and by the closed
+ * forms in tests/test_oracle.py (rest state, uniform-force growth, mass
+ * conservation, Poiseuille start-up series).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define SQRT2 1.4142135623730950488
+#define DBL_EPS_LB 2.220446049250313080847e-16
+
+/* ------------------------------------------------------------ lattices */
+typedef struct {
+  int nD, nQ, nDirPairs;
+  const int *c;      /* [nQ*nD] */
+  const double *w;   /* [nQ] */
+  const int *rev;    /* [nQ] */
+  const double *B;   /* [nQ] */
+  const double *cNorm;
+  double w1c2Inv, w2c2Inv;
+} lattice_t;
+
+static const int d2q9_c[18] = {1, 0, 1, 1, 0, 1, -1, 1, -1, 0, -1, -1, 0, -1, 1, -1, 0, 0};
+static const double d2q9_w[9] = {4.0 / 36.0, 1.0 / 36.0, 4.0 / 36.0, 1.0 / 36.0, 4.0 / 36.0,
+                                 1.0 / 36.0, 4.0 / 36.0, 1.0 / 36.0, 16.0 / 36.0};
+static const int d2q9_rev[9] = {4, 5, 6, 7, 0, 1, 2, 3, 8};
+static const double d2q9_B[9] = {8.0 / 108.0, 5.0 / 108.0, 8.0 / 108.0, 5.0 / 108.0, 8.0 / 108.0,
+                                 5.0 / 108.0, 8.0 / 108.0, 5.0 / 108.0, -16.0 / 108.0};
+static const double d2q9_cn[9] = {1.0, SQRT2, 1.0, SQRT2, 1.0, SQRT2, 1.0, SQRT2, 0.0};
+
+static const int d3q19_c[57] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 1, 1, 0, 1, -1, 0, 1, 0, 1, 1, 0,
+                                -1, 0, 1, 1, 0, 1, -1, -1, 0, 0, 0, -1, 0, 0, 0, -1, -1, -1, 0, -1,
+                                1, 0, -1, 0, -1, -1, 0, 1, 0, -1, -1, 0, -1, 1, 0, 0, 0};
+#define W1 (2.0 / 36.0)
+#define W2 (1.0 / 36.0)
+static const double d3q19_w[19] = {W1, W1, W1, W2, W2, W2, W2, W2, W2, W1,
+                                   W1, W1, W2, W2, W2, W2, W2, W2, 12.0 / 36.0};
+static const int d3q19_rev[19] = {9, 10, 11, 12, 13, 14, 15, 16, 17, 0, 1, 2, 3, 4, 5, 6, 7, 8, 18};
+#define B1 (1.0 / 54.0)
+#define B2 (2.0 / 54.0)
+static const double d3q19_B[19] = {B1, B1, B1, B2, B2, B2, B2, B2, B2, B1,
+                                   B1, B1, B2, B2, B2, B2, B2, B2, -12.0 / 54.0};
+static const double d3q19_cn[19] = {1.0, 1.0, 1.0, SQRT2, SQRT2, SQRT2, SQRT2, SQRT2, SQRT2, 1.0,
+                                    1.0, 1.0, SQRT2, SQRT2, SQRT2, SQRT2, SQRT2, SQRT2, 0.0};
+
+static lattice_t get_lattice(int id) {
+  lattice_t L;
+  if (id == 0) {
+    L.nD = 2; L.nQ = 9; L.nDirPairs = 4;
+    L.c = d2q9_c; L.w = d2q9_w; L.rev = d2q9_rev; L.B = d2q9_B; L.cNorm = d2q9_cn;
+    L.w1c2Inv = (4.0 / 36.0) * 3.0; L.w2c2Inv = (1.0 / 36.0) * 3.0;
+  } else {
+    L.nD = 3; L.nQ = 19; L.nDirPairs = 9;
+    L.c = d3q19_c; L.w = d3q19_w; L.rev = d3q19_rev; L.B = d3q19_B; L.cNorm = d3q19_cn;
+    L.w1c2Inv = W1 * 3.0; L.w2c2Inv = W2 * 3.0;
+  }
+  return L;
+}
+
+static const double c2Inv = 3.0, c4Inv = 9.0, c4Inv0_5 = 0.5 * 9.0;
+static const double c2 = 1.0 / 3.0;
+
+/* qSum: left fold from 0.0 (LBd3q19.h:173-180) */
+static double qsum(const lattice_t *L, const double *f) {
+  double r = 0.0;
+  for (int q = 0; q < L->nQ; ++q) r += f[q];
+  return r;
+}
+
+/* qSumC, written out as the generated headers do (LBd2q9.h:170-177, LBd3q19.h:182-190) */
+static void qsumc(const lattice_t *L, const double *d, double *r) {
+  if (L->nQ == 9) {
+    r[0] = + d[0] + d[1] - d[3] - d[4] - d[5] + d[7];
+    r[1] = + d[1] + d[2] + d[3] - d[5] - d[6] - d[7];
+  } else {
+    r[0] = + d[0] + d[3] + d[4] + d[5] + d[6] - d[9] - d[12] - d[13] - d[14] - d[15];
+    r[1] = + d[1] + d[3] - d[4] + d[7] + d[8] - d[10] - d[12] + d[13] - d[16] - d[17];
+    r[2] = + d[2] + d[5] - d[6] + d[7] - d[8] - d[11] - d[14] + d[15] - d[16] + d[17];
+  }
+}
+
+/* cDotAll (LBd2q9.h:128-142, LBd3q19.h:129-153) */
+static void cdotall(const lattice_t *L, const double *v, double *r) {
+  if (L->nQ == 9) {
+    r[0] = +v[0]; r[1] = +v[0] + v[1]; r[2] = +v[1]; r[3] = -v[0] + v[1];
+    r[4] = -v[0]; r[5] = -v[0] - v[1]; r[6] = -v[1]; r[7] = +v[0] - v[1]; r[8] = 0.0;
+  } else {
+    r[0] = +v[0]; r[1] = +v[1]; r[2] = +v[2];
+    r[3] = +v[0] + v[1]; r[4] = +v[0] - v[1]; r[5] = +v[0] + v[2]; r[6] = +v[0] - v[2];
+    r[7] = +v[1] + v[2]; r[8] = +v[1] - v[2];
+    r[9] = -v[0]; r[10] = -v[1]; r[11] = -v[2];
+    r[12] = -v[0] - v[1]; r[13] = -v[0] + v[1]; r[14] = -v[0] - v[2]; r[15] = -v[0] + v[2];
+    r[16] = -v[1] - v[2]; r[17] = -v[1] + v[2]; r[18] = 0.0;
+  }
+}
+
+static double dot(const lattice_t *L, const double *a, const double *b) {
+  return L->nD == 2 ? a[0] * b[0] + a[1] * b[1] : a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+/* grad (LBd2q9.h:144-151, LBd3q19.h:155-163) */
+static void lattice_grad(const lattice_t *L, const double *s, double *g) {
+  const double a = L->w1c2Inv, b = L->w2c2Inv;
+  if (L->nQ == 9) {
+    g[0] = + a * (+ s[0] - s[4]) + b * (+ s[1] - s[3] - s[5] + s[7]);
+    g[1] = + a * (+ s[2] - s[6]) + b * (+ s[1] + s[3] - s[5] - s[7]);
+  } else {
+    g[0] = + a * (+ s[0] - s[9]) + b * (+ s[3] + s[4] + s[5] + s[6] - s[12] - s[13] - s[14] - s[15]);
+    g[1] = + a * (+ s[1] - s[10]) + b * (+ s[3] - s[4] + s[7] + s[8] - s[12] + s[13] - s[16] - s[17]);
+    g[2] = + a * (+ s[2] - s[11]) + b * (+ s[5] - s[6] + s[7] - s[8] - s[14] + s[15] - s[16] + s[17]);
+  }
+}
+
+/* ------------------------------------------------------------ bounce-back */
+typedef struct {
+  int n;          /* boundary nodes */
+  int32_t *node;  /* [n] */
+  int32_t *nbeta, *ngamma, *ndelta;  /* [n] */
+  int32_t *links; /* [n*nDirPairs]: beta..., gamma..., delta... */
+} bb_t;
+
+static int is_fluid(const int16_t *type, int32_t n) { return type[n] > 1; }
+
+/* BoundaryHalwWayHelper ctor (LBhalfwayhelperclass.h:110-161) */
+static bb_t bb_setup(const lattice_t *L, const int32_t *nodes, int n, const int32_t *nb,
+                     const int16_t *type) {
+  bb_t b;
+  b.n = n;
+  b.node = malloc(sizeof(int32_t) * (n ? n : 1));
+  b.nbeta = malloc(sizeof(int32_t) * (n ? n : 1));
+  b.ngamma = malloc(sizeof(int32_t) * (n ? n : 1));
+  b.ndelta = malloc(sizeof(int32_t) * (n ? n : 1));
+  b.links = malloc(sizeof(int32_t) * (n ? n : 1) * L->nDirPairs);
+  for (int k = 0; k < n; ++k) {
+    const int32_t node = nodes[k];
+    int beta[9], gamma[9], delta[9], nbt = 0, ng = 0, nd = 0;
+    for (int q = 0; q < L->nDirPairs; ++q) {
+      const int32_t a = nb[(int64_t)node * L->nQ + q];
+      const int32_t r = nb[(int64_t)node * L->nQ + L->rev[q]];
+      if (is_fluid(type, a)) {
+        if (is_fluid(type, r)) gamma[ng++] = q;
+        else beta[nbt++] = q;
+      } else {
+        if (is_fluid(type, r)) beta[nbt++] = L->rev[q];
+        else delta[nd++] = q;
+      }
+    }
+    b.node[k] = node;
+    b.nbeta[k] = nbt; b.ngamma[k] = ng; b.ndelta[k] = nd;
+    int32_t *l = b.links + (int64_t)k * L->nDirPairs;
+    for (int i = 0; i < nbt; ++i) *l++ = beta[i];
+    for (int i = 0; i < ng; ++i) *l++ = gamma[i];
+    for (int i = 0; i < nd; ++i) *l++ = delta[i];
+  }
+  return b;
+}
+
+static void bb_free(bb_t *b) {
+  free(b->node); free(b->nbeta); free(b->ngamma); free(b->ndelta); free(b->links);
+}
+
+/* HalfWayBounceBack::apply (LBhalfwaybb.h:37-63) on field k of nF */
+static void bb_apply(const lattice_t *L, const bb_t *b, double *f, int nF, int k, const int32_t *nb) {
+  const int Q = L->nQ;
+  for (int i = 0; i < b->n; ++i) {
+    const int32_t node = b->node[i];
+    const int32_t *l = b->links + (int64_t)i * L->nDirPairs;
+    for (int m = 0; m < b->nbeta[i]; ++m) {
+      const int beta = l[m], br = L->rev[beta];
+      f[(int64_t)node * nF * Q + k * Q + beta] = f[(int64_t)nb[(int64_t)node * Q + br] * nF * Q + k * Q + br];
+    }
+    const int32_t *dl = l + b->nbeta[i] + b->ngamma[i];
+    for (int m = 0; m < b->ndelta[i]; ++m) {
+      const int d = dl[m], dr = L->rev[d];
+      f[(int64_t)node * nF * Q + k * Q + d] = f[(int64_t)nb[(int64_t)node * Q + dr] * nF * Q + k * Q + dr];
+      f[(int64_t)node * nF * Q + k * Q + dr] = f[(int64_t)nb[(int64_t)node * Q + d] * nF * Q + k * Q + d];
+    }
+  }
+}
+
+/* ------------------------------------------------------------ std_case */
+/* nsteps iterations of std_case/main.cpp:110-145.  f, ftmp: LbField (1 field)
+ * over n_nodes; rho, vel: ScalarField / VectorField of the last iteration. */
+int orc_std_case(int lattice, int32_t n_nodes, const int32_t *nb, const int16_t *type,
+                 const int32_t *bulk, int32_t n_bulk, double *f, double *ftmp, double tau,
+                 const double *F, int32_t nsteps, double *rho, double *vel, int nthreads) {
+  const lattice_t L = get_lattice(lattice);
+  const int Q = L.nQ, D = L.nD;
+  /* fluid boundary nodes of this rank (LBgeometry.h:48-56) */
+  int32_t *fb = malloc(sizeof(int32_t) * (n_nodes ? n_nodes : 1));
+  int nfb = 0;
+  for (int32_t n = 1; n < n_nodes; ++n)
+    if (type[n] == 2) fb[nfb++] = n;
+  bb_t bb = bb_setup(&L, fb, nfb, nb, type);
+  free(fb);
+  const double tau_inv = 1.0 / tau;
+  const double tau_factor = (1 - 0.5 / tau);
+  double cF[19];
+  cdotall(&L, F, cF);
+  double *a = f, *b = ftmp;
+  (void)nthreads;
+  for (int32_t it = 0; it < nsteps; ++it) {
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int32_t k = 0; k < n_bulk; ++k) {
+      const int32_t node = bulk[k];
+      const double *fn = a + (int64_t)node * Q;
+      const double rhoN = qsum(&L, fn);
+      double j[3], u[3], cu[19];
+      qsumc(&L, fn, j);
+      for (int d = 0; d < D; ++d) u[d] = (j[d] + 0.5 * F[d]) / rhoN;
+      rho[node] = rhoN;
+      for (int d = 0; d < D; ++d) vel[(int64_t)node * D + d] = u[d];
+      const double u2 = dot(&L, u, u);
+      cdotall(&L, u, cu);
+      const double uF = dot(&L, u, F);
+      for (int q = 0; q < Q; ++q) {
+        const double om = -tau_inv * (fn[q] - rhoN * L.w[q] * (1.0 + c2Inv * cu[q] + c4Inv0_5 * (cu[q] * cu[q] - c2 * u2)));
+        const double dF = L.w[q] * tau_factor * (c2Inv * cF[q] + c4Inv * (cF[q] * cu[q] - c2 * uF));
+        b[(int64_t)nb[(int64_t)node * Q + q] * Q + q] = fn[q] + om + dF;
+      }
+    }
+    double *t = a; a = b; b = t;
+    bb_apply(&L, &bb, a, 1, 0, nb);
+  }
+  if (a != f) memcpy(f, a, sizeof(double) * (size_t)n_nodes * Q);
+  bb_free(&bb);
+  return 0;
+}
+
+/* ------------------------------------------------------------ twophase */
+/* nsteps iterations of main_TWOPHASE.cpp:238-392.
+ *   f, ftmp : LbField with 2 fields over n_nodes
+ *   rho     : ScalarField with 2 fields; solid-boundary entries hold the
+ *             wettability densities on entry (main_TWOPHASE.cpp:171-178)
+ *   cg      : ScalarField (colour field) over n_nodes, zero-initialised
+ *   force_mode 0: F fixed; 1: F[0] = momx (flux controller :292-308)
+ *   force_out[nD] : body force of the last iteration */
+int orc_twophase(int lattice, int32_t n_nodes, const int32_t *nb, const int16_t *type,
+                 const int32_t *bulk, int32_t n_bulk, double *f, double *ftmp, double *rho,
+                 double *vel, double *cg, double tau0, double tau1, double sigma, double beta,
+                 int force_mode, const double *Fin, int64_t n_global, int32_t nsteps,
+                 double *force_out) {
+  const lattice_t L = get_lattice(lattice);
+  const int Q = L.nQ, D = L.nD, nF = 2;
+  bb_t bb = bb_setup(&L, bulk, n_bulk, nb, type);  /* built over all bulk nodes (:81) */
+  const double nu0Inv = 1.0 / (c2 * (tau0 - 0.5));
+  const double nu1Inv = 1.0 / (c2 * (tau1 - 0.5));
+  double F[3] = {Fin[0], Fin[1], D == 3 ? Fin[2] : 0.0};
+  const double momx = Fin[0];
+  const double q0Node = 0.0, q1Node = 0.0;  /* Q(0,n) = Q(1,n) = 0 (:132-136) */
+  double *a = f, *b = ftmp;
+  for (int32_t it = 0; it < nsteps; ++it) {
+    /* pass 1 */
+    for (int32_t k = 0; k < n_bulk; ++k) {
+      const int32_t n = bulk[k];
+      const double r0 = rho[(int64_t)n * 2 + 0] = qsum(&L, a + (int64_t)n * nF * Q);
+      const double r1 = rho[(int64_t)n * 2 + 1] = qsum(&L, a + (int64_t)n * nF * Q + Q);
+      cg[n] = (r0 - r1) / (r0 + r1);
+    }
+    for (int32_t n = 1; n < n_nodes; ++n)
+      if (type[n] == 1) {
+        const double r0 = rho[(int64_t)n * 2 + 0], r1 = rho[(int64_t)n * 2 + 1];
+        cg[n] = (r0 - r1) / (r0 + r1);
+      }
+    /* pass 2 */
+    if (force_mode == 1) {
+      double meanfcX = 0.0;
+      for (int32_t k = 0; k < n_bulk; ++k) {
+        const int32_t n = bulk[k];
+        double ft[19], sfc[3];
+        for (int q = 0; q < Q; ++q) ft[q] = a[(int64_t)n * nF * Q + q] + a[(int64_t)n * nF * Q + Q + q];
+        qsumc(&L, ft, sfc);
+        meanfcX += sfc[0];
+      }
+      double meanG = meanfcX;
+      meanG /= (double)n_global;
+      F[0] = 2 * (momx - meanG);
+    }
+    /* pass 3 */
+    double cF[19];
+    cdotall(&L, F, cF);
+    for (int32_t k = 0; k < n_bulk; ++k) {
+      const int32_t n = bulk[k];
+      const double *f0 = a + (int64_t)n * nF * Q, *f1 = f0 + Q;
+      double ft[19], j[3], u[3], cu[19];
+      for (int q = 0; q < Q; ++q) ft[q] = f0[q] + f1[q];
+      double r0 = rho[(int64_t)n * 2 + 0];
+      double r1 = rho[(int64_t)n * 2 + 1];
+      const double rhoN = r0 + r1;
+      qsumc(&L, ft, j);
+      for (int d = 0; d < D; ++d) u[d] = (j[d] + 0.5 * F[d]) / rhoN;
+      for (int d = 0; d < D; ++d) vel[(int64_t)n * D + d] = u[d];
+      rho[(int64_t)n * 2 + 0] = r0 += 0.5 * q0Node;
+      rho[(int64_t)n * 2 + 1] = r1 += 0.5 * q1Node;
+      const double tau = c2Inv * rhoN / (r0 * nu0Inv + r1 * nu1Inv) + 0.5;
+      const double uu = dot(&L, u, u);
+      cdotall(&L, u, cu);
+      const double tau_inv = 1.0 / tau;
+      const double tau_factor = (1 - 0.5 / tau);
+      const double uF = dot(&L, u, F);
+      /* grad of cg over the neighbours */
+      double s[19], g[3], cg_n[19];
+      for (int q = 0; q < Q; ++q) s[q] = cg[nb[(int64_t)n * Q + q]];
+      lattice_grad(&L, s, g);
+      const double CGNorm = sqrt(dot(&L, g, g));
+      const double inv = 1.0 / (CGNorm + (CGNorm < DBL_EPS_LB));
+      for (int d = 0; d < D; ++d) g[d] *= inv;
+      cdotall(&L, g, cg_n);
+      const double AF0_5 = 1.125 * CGNorm * sigma / tau;
+      const double rhoFacBeta = beta * r0 * r1 / rhoN;
+      const double c0 = (r0 / rhoN), c1 = (r1 / rhoN);
+      for (int q = 0; q < Q; ++q) {
+        const double om = -tau_inv * (ft[q] - rhoN * L.w[q] * (1.0 + c2Inv * cu[q] + c4Inv0_5 * (cu[q] * cu[q] - c2 * uu)));
+        const double dF = L.w[q] * tau_factor * (c2Inv * cF[q] + c4Inv * (cF[q] * cu[q] - c2 * uF));
+        const double dQ0 = tau_factor * q0Node * L.w[q] * (1.0 + c2Inv * cu[q] + c4Inv0_5 * (cu[q] * cu[q] - c2 * uu));
+        const double dQ1 = tau_factor * q1Node * L.w[q] * (1.0 + c2Inv * cu[q] + c4Inv0_5 * (cu[q] * cu[q] - c2 * uu));
+        double st, rc;
+        if (q < Q - 1) {
+          st = AF0_5 * (L.w[q] * cg_n[q] * cg_n[q] - L.B[q]);
+          rc = rhoFacBeta * L.w[q] * cg_n[q] / L.cNorm[q];
+        } else {
+          st = -AF0_5 * L.B[q];
+          rc = 0.0;
+        }
+        const int32_t to = nb[(int64_t)n * Q + q];
+        b[(int64_t)to * nF * Q + q] = c0 * (ft[q] + om + dF + st) + rc + dQ0;
+        b[(int64_t)to * nF * Q + Q + q] = c1 * (ft[q] + om + dF + st) - rc + dQ1;
+      }
+    }
+    double *t = a; a = b; b = t;
+    bb_apply(&L, &bb, a, nF, 0, nb);
+    bb_apply(&L, &bb, a, nF, 1, nb);
+  }
+  if (a != f) memcpy(f, a, sizeof(double) * (size_t)n_nodes * nF * Q);
+  if (force_out) for (int d = 0; d < D; ++d) force_out[d] = F[d];
+  bb_free(&bb);
+  return 0;
+}
+
+/* ------------------------------------------------------------ flux sums */
+/* LBglobalforcing.h:23-26 (sum of f c over bulk, field sum for nF = 2) and
+ * :61-78 (phi-weighted sums); sequential in bulk order.  out[3]. */
+int orc_flux(int lattice, int32_t n_nodes, const int32_t *bulk, int32_t n_bulk, const double *f,
+             int nF, int dir, double *out) {
+  const lattice_t L = get_lattice(lattice);
+  const int Q = L.nQ;
+  (void)n_nodes;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int32_t k = 0; k < n_bulk; ++k) {
+    const int32_t n = bulk[k];
+    const double *fn = f + (int64_t)n * nF * Q;
+    double ft[19], fc[3];
+    for (int q = 0; q < Q; ++q) ft[q] = nF == 2 ? fn[q] + fn[Q + q] : fn[q];
+    qsumc(&L, ft, fc);
+    s0 += fc[dir];
+    if (nF == 2) {
+      const double r0 = qsum(&L, fn), r1 = qsum(&L, fn + Q);
+      const double rt = r0 + r1;
+      s1 += (r0 / rt) * fc[dir];
+      s2 += (r1 / rt) * fc[dir];
+    }
+  }
+  out[0] = s0; out[1] = s1; out[2] = s2;
+  return 0;
+}
+
+/* ------------------------------------------------------------ node types */
+/* Nodes::setupNodeType (LBnodes.h:156-209); attr = 'nodetype' attribute
+ * (0 solid, nonzero fluid), rank[] per node, my_rank. */
+int orc_node_types(int lattice, int32_t n_nodes, const int32_t *nb, const int32_t *attr,
+                   const int32_t *rank, int32_t my_rank, int16_t *type) {
+  const lattice_t L = get_lattice(lattice);
+  const int Q = L.nQ;
+  type[0] = -1;
+  for (int32_t n = 1; n < n_nodes; ++n) type[n] = attr[n] == 0 ? 0 : 3;
+  for (int32_t n = 1; n < n_nodes; ++n) {
+    if (type[n] < 2) {
+      int fl = 0;
+      for (int q = 0; q < Q; ++q) fl |= type[nb[(int64_t)n * Q + q]] > 1;
+      type[n] = fl ? 1 : 0;
+    } else {
+      if (rank[n] != my_rank) {
+        type[n] = 4;
+      } else {
+        int so = 0;
+        for (int q = 0; q < Q; ++q) so |= type[nb[(int64_t)n * Q + q]] < 2;
+        type[n] = so ? 2 : 3;
+      }
+    }
+  }
+  return 0;
+}

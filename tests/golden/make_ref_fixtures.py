@@ -1,0 +1,197 @@
+"""Run the compiled reference on small geometries and store its fields as fixtures.
+
+Needs the reference binaries built by oracle/build_ref.sh (in the build
+container only; /root/reference is not available elsewhere).  For every case:
+
+  1. the geometry is written as .vtklb by the oracle's restatement of the
+     reference writer (itself pinned byte-for-byte against files written by
+     PythonScripts/vtklb.py, see make_vtklb_fixtures.py),
+  2. an input.dat with the case parameters is written,
+  3. the reference case (std_case D2Q9 / D3Q19 or twophase) runs with
+     iterations.write = iterations.max, so it writes VTK at i = 0 and i = max,
+  4. the VTU of i = max is parsed: rho (rho0, rho1) and vel on the bulk nodes,
+     in bulk order, i.e. the fields of the pre-collision state after max
+     steps (std_case/main.cpp:109-155, main_TWOPHASE.cpp:236-424),
+  5. inputs (geometry, parameters) and outputs are stored in ref_<case>.npz.
+
+Run:  python tests/golden/make_ref_fixtures.py [case ...]
+"""
+import glob
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "badchimp-cpp_amd"))
+
+from oracle.vtklb_oracle import Geometry  # noqa: E402
+
+REF = os.path.join(ROOT, "oracle", "_ref")
+VT = os.path.join(HERE, "vtklb")
+
+
+def parse_vtu(path):
+    """Cell data arrays of a VTU file written by the reference (VTK.h:1094-1180):
+    raw appended data, each array preceded by its byte count (uint32)."""
+    raw = open(path, "rb").read()
+    head, _, tail = raw.partition(b'<AppendedData encoding="raw">')
+    base = tail.index(b"_") + 1
+    out = {}
+    for m in re.finditer(rb'<DataArray Name="([^"]+)" type="(\w+)" NumberOfComponents="(\d+)" '
+                         rb'format="appended" offset="(\d+)"', head):
+        name, typ, ncomp, off = m.group(1).decode(), m.group(2).decode(), int(m.group(3)), int(m.group(4))
+        nbytes = int(np.frombuffer(tail, dtype=np.uint32, count=1, offset=base + off)[0])
+        dt = {"Float64": np.float64, "Float32": np.float32, "Int32": np.int32}[typ]
+        arr = np.frombuffer(tail, dtype=dt, count=nbytes // np.dtype(dt).itemsize,
+                            offset=base + off + 4).copy()
+        out[name] = arr.reshape(-1, ncomp) if ncomp > 1 else arr
+    return out
+
+
+def last_vtu(outdir, base):
+    files = sorted(glob.glob(os.path.join(outdir, "vtu", "0000_%s_*.vtu" % base)))
+    return files[-1]
+
+
+def input_dat(max_it, fluid_lines, extra=""):
+    return ("<iterations>\n    max %d\n    write %d\n<end>\n<fluid>\n%s<end>\n%s"
+            % (max_it, max_it, "".join("    %s\n" % l for l in fluid_lines), extra))
+
+
+def fmt(v):
+    return " ".join(repr(float(x)) for x in np.atleast_1d(v))
+
+
+def run_case(binary, geo, lattice, periodic, attrs, dat, twophase=False, outsub="tp/"):
+    tmp = tempfile.mkdtemp(prefix="lbref_")
+    try:
+        g = Geometry(geo, lattice, periodic)
+        if twophase:
+            root = os.path.join(tmp, "run")
+        else:
+            root = tmp
+        os.makedirs(os.path.join(root, "input", "mpi"))
+        os.makedirs(os.path.join(root, "output", outsub) if twophase else os.path.join(root, "output"))
+        with open(os.path.join(root, "input", "mpi", "tmp0.vtklb"), "w") as fh:
+            fh.write(g.vtklb_text(1, attrs))
+        with open(os.path.join(root, "input", "input.dat"), "w") as fh:
+            fh.write(dat)
+        cwd = root if twophase else os.path.join(tmp, "run")
+        os.makedirs(cwd, exist_ok=True)
+        env = dict(os.environ, LD_LIBRARY_PATH=os.path.join(REF, "mpilib"), OMP_NUM_THREADS="1")
+        res = subprocess.run([os.path.join(REF, binary)], cwd=cwd, env=env, capture_output=True,
+                             text=True, timeout=3600)
+        if res.returncode != 0:
+            raise RuntimeError("%s failed: %s %s" % (binary, res.stdout[-2000:], res.stderr[-2000:]))
+        if twophase:
+            od = os.path.join(root, "output", outsub)
+            data = parse_vtu(last_vtu(od, "fluid"))
+            force = np.loadtxt(os.path.join(od, "force.dat"), ndmin=2)
+            data["force_x"] = force
+        else:
+            data = parse_vtu(last_vtu(os.path.join(tmp, "output"), "lb_run"))
+        return data
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def load_geo(name):
+    return np.load(os.path.join(VT, name + "_geo.npy"))
+
+
+def sphere_geo(shape, radius, porosity, seed, periodic, walls):
+    import lbamd
+    geo, info = lbamd.sphere_pack(shape, radius, porosity, seed, periodic=periodic, walls=walls)
+    return geo.astype(np.int64), info
+
+
+def std_cases():
+    """(name, lattice, binary, geo, periodic, attrs, tau, force, max_it, profile_only)"""
+    g = load_geo("d2q9_channel_8x6_px")
+    yield "d2q9_channel_8x6", "D2Q9", g, "x", [("init_rho", np.ones(g.shape))], 1.0, [1e-6, 0.0], 20, False
+    g = load_geo("d2q9_random_7x5_pxy")
+    r = np.load(os.path.join(VT, "d2q9_random_7x5_pxy_attr_init_rho.npy"))
+    yield "d2q9_random_7x5", "D2Q9", g, "xy", [("init_rho", r)], 0.8, [1e-5, -3e-6], 50, False
+    g = np.ones((64, 32), dtype=np.int64)
+    g[:, 0] = 0
+    g[:, -1] = 0
+    yield "d2q9_channel_64x32", "D2Q9", g, "x", [("init_rho", np.ones(g.shape))], 0.7, [1e-5, 0.0], 500, False
+    g = np.ones((256, 128), dtype=np.int64)
+    g[:, 0] = 0
+    g[:, -1] = 0
+    yield "d2q9_channel_256x128", "D2Q9", g, "x", [("init_rho", np.ones(g.shape))], 1.0, [1e-6, 0.0], 5000, True
+    g = load_geo("d3q19_pack_6x5x4_pxyz")
+    yield "d3q19_pack_6x5x4_pxyz", "D3Q19", g, "xyz", [("init_rho", np.ones(g.shape))], 0.9, [1e-5, 2e-6, -1e-6], 30, False
+    g = load_geo("d3q19_pack_6x5x4_px_walls")
+    yield "d3q19_pack_6x5x4_px_walls", "D3Q19", g, "x", [("init_rho", np.ones(g.shape))], 1.0, [1e-5, 0.0, 0.0], 30, False
+    g = load_geo("d3q19_open_4x4x3")
+    yield "d3q19_open_4x4x3", "D3Q19", g, "", [("init_rho", np.ones(g.shape))], 0.8, [1e-5, 1e-6, 0.0], 10, False
+    g, _ = sphere_geo((24, 24, 24), 4.0, 0.5, 7, "x", True)
+    yield "d3q19_pack_24", "D3Q19", g, "x", [("init_rho", np.ones(g.shape))], 1.0, [1e-5, 0.0, 0.0], 200, False
+
+
+def tp_cases():
+    """(name, geo, periodic, rho0, wett, tau0, tau1, sigma, beta, momx, force, max_it)"""
+    rng = np.random.default_rng(99)
+    g = load_geo("d3q19_pack_6x5x4_pxyz")
+    r0 = np.load(os.path.join(VT, "d3q19_pack_6x5x4_pxyz_attr_rho0.npy"))
+    yield "tp_pack_6x5x4", g, "xyz", r0, np.ones(g.shape), 1.0, 1.0, 0.01, 0.7, 1e-4, [0.0, 0.0, 0.0], 30
+    g, _ = sphere_geo((16, 16, 16), 3.0, 0.5, 11, "xyz", False)
+    r0 = rng.choice([0.0, 0.25, 0.6, 1.0], size=g.shape)
+    yield "tp_pack_16", g, "xyz", r0, np.full(g.shape, 0.8), 0.8, 1.2, 0.01, 0.7, 1e-4, [0.0, 1e-6, -2e-6], 100
+    g, _ = sphere_geo((12, 14, 10), 3.0, 0.55, 5, "x", True)
+    r0 = (rng.random(g.shape) > 0.5).astype(float)
+    yield "tp_walls_12x14x10", g, "x", r0, np.full(g.shape, 1.0), 1.0, 1.0, 0.01, 0.7, 5e-5, [0.0, 0.0, 0.0], 60
+
+
+def main(selected):
+    for name, lat, g, per, attrs, tau, force, max_it, prof in std_cases():
+        if selected and name not in selected:
+            continue
+        binary = "std_case_d2q9" if lat == "D2Q9" else "std_case_d3q19"
+        dat = input_dat(max_it, ["tau %s" % fmt(tau), "bodyforce %s" % fmt(force)])
+        data = run_case(binary, g, lat, per, attrs, dat)
+        params = dict(case="std_case", lattice=lat, periodic=per, tau=tau, force=force, steps=max_it)
+        out = dict(geo=g.astype(np.int8), params=json.dumps(params),
+                   init_rho=attrs[0][1].astype(np.float64))
+        rho, vel = data["rho"], data["vel"]
+        if prof:
+            # the channel is x-invariant: keep one column (y = 1..ny-2) and the spread over x
+            ny = g.shape[1] - 2
+            out["rho_profile"] = rho.reshape(g.shape[0], ny)[0]
+            out["velx_profile"] = vel[:, 0].reshape(g.shape[0], ny)[0]
+            out["vely_profile"] = vel[:, 1].reshape(g.shape[0], ny)[0]
+            out["x_spread"] = np.array([np.ptp(rho.reshape(g.shape[0], ny), axis=0).max(),
+                                        np.ptp(vel[:, 0].reshape(g.shape[0], ny), axis=0).max()])
+        else:
+            out["rho"] = rho
+            out["vel"] = vel
+        np.savez_compressed(os.path.join(HERE, "ref_%s.npz" % name), **out)
+        print("wrote ref_%s.npz" % name, rho.shape)
+    for name, g, per, r0, wett, tau0, tau1, sigma, beta, momx, force, max_it in tp_cases():
+        if selected and name not in selected:
+            continue
+        attrs = [("rho0", r0), ("rho1", 1.0 - r0), ("wettability", wett),
+                 ("source", np.zeros(g.shape, dtype=np.int64))]
+        dat = input_dat(max_it, ["tau %s" % fmt([tau0, tau1]), "bodyforce %s" % fmt(force),
+                                 "sigma %s" % fmt(sigma), "beta %s" % fmt(beta), "momx %s" % fmt(momx)],
+                        extra="outdir tp/\n")
+        data = run_case("twophase", g, "D3Q19", per, attrs, dat, twophase=True)
+        params = dict(case="twophase", lattice="D3Q19", periodic=per, tau0=tau0, tau1=tau1, sigma=sigma,
+                      beta=beta, momx=momx, force=force, steps=max_it)
+        out = dict(geo=g.astype(np.int8), params=json.dumps(params), rho0_attr=r0, wett_attr=wett,
+                   rho0=data["rho0"], rho1=data["rho1"], vel=data["vel"], force_x=data["force_x"])
+        np.savez_compressed(os.path.join(HERE, "ref_%s.npz" % name), **out)
+        print("wrote ref_%s.npz" % name, data["rho0"].shape, data["force_x"][-1])
+
+
+if __name__ == "__main__":
+    main(set(sys.argv[1:]))

@@ -1,0 +1,64 @@
+"""Shared helpers of the test suite: topology through the oracle, fixture loading."""
+import json
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+for p in (ROOT, os.path.join(ROOT, "badchimp-cpp_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import oracle  # noqa: E402
+from oracle.vtklb_oracle import Geometry  # noqa: E402
+
+LAT = {"D2Q9": 0, "D3Q19": 1}
+
+
+class OracleTopo:
+    """Node list of rank 1 as the oracle restates it: neighbour index with the
+    node-0 row, node types (Nodes::setupNodeType) and the bulk list."""
+
+    def __init__(self, geo, lattice, periodic, rank=1):
+        self.lattice = LAT[lattice] if isinstance(lattice, str) else lattice
+        g = Geometry(geo, "D2Q9" if self.lattice == 0 else "D3Q19", periodic)
+        t = g.rank_topology(rank)
+        n = t.n_points + 1
+        self.geometry = g
+        self.rank_topology = t
+        self.n_nodes = n
+        self.neighbors = np.zeros((n, g.basis.shape[0]), dtype=np.int32)
+        self.neighbors[1:] = t.neighbors
+        attr = np.zeros(n, dtype=np.int32)
+        attr[1:] = t.nodetype
+        ranks = np.full(n, rank - 1, dtype=np.int32)
+        ranks[0] = -1
+        for r0, pairs in t.processors:
+            ranks[pairs[:, 0]] = r0
+        self.ranks = ranks
+        self.node_types = oracle.node_types(self.lattice, self.neighbors, attr, ranks, rank - 1)
+        self.bulk = oracle.bulk_nodes(self.node_types, ranks, rank - 1)
+
+    def attribute(self, val):
+        """Per-node values of a geometry-shaped attribute (node 0 = 0)."""
+        out = np.zeros(self.n_nodes)
+        out[1:] = self.geometry.padded_attribute(val)[self.rank_topology.ind_local]
+        return out
+
+
+def load_ref(name):
+    d = dict(np.load(os.path.join(GOLDEN, "ref_%s.npz" % name)))
+    d["params"] = json.loads(str(d["params"]))
+    return d
+
+
+def ref_names(prefix=""):
+    return sorted(f[4:-4] for f in os.listdir(GOLDEN)
+                  if f.startswith("ref_" + prefix) and f.endswith(".npz"))
+
+
+def float_round(a):
+    """Values as the reference reads them with `ifs >> float` (via the 18-digit text)."""
+    return np.array(["%.18e" % v for v in np.ravel(a)], dtype=np.float32).astype(np.float64).reshape(np.shape(a))
