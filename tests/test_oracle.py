@@ -1,0 +1,192 @@
+"""Pinning the oracle (CPU restatement of the reference loops).
+
+1. Against the compiled reference's own outputs (tests/golden/ref_*.npz,
+   written by make_ref_fixtures.py from oracle/_ref builds): bit-exact.
+2. Against what the mathematics fixes: rest state, Guo momentum growth in a
+   periodic box, mass conservation, the Poiseuille start-up series, phase
+   symmetry and the one-phase limit of the colour-gradient model.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import OracleTopo, float_round, load_ref, ref_names
+
+
+def ref_std_run(name):
+    d = load_ref(name)
+    p = d["params"]
+    T = OracleTopo(d["geo"].astype(np.int64), p["lattice"], p["periodic"])
+    f0 = oracle.rest_state(T.lattice, T.n_nodes, T.bulk, T.attribute(d["init_rho"]))
+    f, rho, vel = oracle.run_std_case(T.lattice, T.neighbors, T.node_types, T.bulk, f0, p["tau"],
+                                      np.array(p["force"]), p["steps"] + 1)
+    return d, T, f, rho, vel
+
+
+@pytest.mark.parametrize("name", [n for n in ref_names() if not n.startswith("tp_")])
+def test_std_case_bitexact_vs_compiled_reference(name):
+    d, T, f, rho, vel = ref_std_run(name)
+    D = vel.shape[1]
+    if "rho" in d:
+        assert np.array_equal(rho[T.bulk], d["rho"])
+        assert np.array_equal(vel[T.bulk], d["vel"][:, :D])
+    else:
+        nx, ny = d["geo"].shape[0], d["geo"].shape[1] - 2
+        r = rho[T.bulk].reshape(nx, ny)
+        u = vel[T.bulk, 0].reshape(nx, ny)
+        assert np.array_equal(r[0], d["rho_profile"])
+        assert np.array_equal(u[0], d["velx_profile"])
+        # x-invariance (bit-identical columns), as in the reference run
+        assert np.ptp(r, axis=0).max() == d["x_spread"][0] == 0.0
+        assert np.ptp(u, axis=0).max() == d["x_spread"][1] == 0.0
+
+
+def tp_start(d, T):
+    r0 = float_round(T.attribute(d["rho0_attr"]))
+    r1 = float_round(T.attribute(1.0 - d["rho0_attr"]))
+    w = float_round(T.attribute(d["wett_attr"]))
+    sb = T.node_types == 1
+    r0[sb] = w[sb]
+    r1[sb] = 1 - w[sb]
+    f0 = np.zeros((T.n_nodes, 2, 19))
+    W = oracle.WEIGHTS[1]
+    f0[T.bulk, 0] = W[None] * r0[T.bulk, None]
+    f0[T.bulk, 1] = W[None] * r1[T.bulk, None]
+    return f0, np.stack([r0, r1], 1)
+
+
+@pytest.mark.parametrize("name", ref_names("tp_"))
+def test_twophase_bitexact_vs_compiled_reference(name):
+    d = load_ref(name)
+    p = d["params"]
+    T = OracleTopo(d["geo"].astype(np.int64), p["lattice"], p["periodic"])
+    f0, rho = tp_start(d, T)
+    f, r, v, cg, F = oracle.run_twophase(1, T.neighbors, T.node_types, T.bulk, f0, rho, p["tau0"], p["tau1"],
+                                         p["sigma"], p["beta"], 1, [p["momx"]] + p["force"][1:], p["steps"] + 1)
+    assert np.array_equal(r[T.bulk, 0], d["rho0"])
+    assert np.array_equal(r[T.bulk, 1], d["rho1"])
+    assert np.array_equal(v[T.bulk], d["vel"])
+    assert F[0] == d["force_x"][-1, 1]
+
+
+def periodic_box(lattice, shape):
+    return OracleTopo(np.ones(shape, dtype=np.int64), lattice, "xyz"[: len(shape)])
+
+
+@pytest.mark.parametrize("lattice,shape", [("D2Q9", (7, 5)), ("D3Q19", (5, 4, 3))])
+def test_rest_state_is_exact(lattice, shape):
+    T = periodic_box(lattice, shape)
+    f0 = oracle.rest_state(T.lattice, T.n_nodes, T.bulk, np.full(T.n_nodes, 1.25))
+    f, rho, vel = oracle.run_std_case(T.lattice, T.neighbors, T.node_types, T.bulk, f0, 0.77,
+                                      np.zeros(3), 25)
+    # sum_q w_q rho and sum_q w_q c_q round to a few ulp: the rest state stays
+    # put to that accuracy and does not drift
+    assert np.abs(f[T.bulk] - f0[T.bulk]).max() <= 4e-16
+    assert np.abs(vel[T.bulk]).max() <= 1e-16
+
+
+@pytest.mark.parametrize("lattice,shape", [("D2Q9", (6, 5)), ("D3Q19", (4, 5, 3))])
+def test_guo_forcing_momentum_growth(lattice, shape):
+    """Uniform force in a periodic box: after t steps j = t F and the stored
+    velocity of step t is (t + 1/2) F / rho."""
+    T = periodic_box(lattice, shape)
+    D = len(shape)
+    F = np.array([2e-5, -1e-5, 3e-6])[:D]
+    f0 = oracle.rest_state(T.lattice, T.n_nodes, T.bulk, np.ones(T.n_nodes))
+    for t in (1, 10, 40):
+        f, rho, vel = oracle.run_std_case(T.lattice, T.neighbors, T.node_types, T.bulk, f0, 0.9, F, t + 1)
+        assert np.allclose(vel[T.bulk], (t + 0.5) * F[None, :], rtol=1e-12, atol=0)
+        assert np.allclose(rho[T.bulk], 1.0, rtol=0, atol=1e-14)
+
+
+def test_mass_conservation_with_walls():
+    rng = np.random.default_rng(3)
+    geo = (rng.random((12, 10, 8)) > 0.3).astype(np.int64)
+    geo[:, 0] = geo[:, -1] = 0
+    geo[:, :, 0] = geo[:, :, -1] = 0
+    T = OracleTopo(geo, "D3Q19", "x")
+    f0 = oracle.rest_state(1, T.n_nodes, T.bulk, 1.0 + 0.01 * rng.random(T.n_nodes))
+    m0 = f0[T.bulk].sum()
+    f, rho, vel = oracle.run_std_case(1, T.neighbors, T.node_types, T.bulk, f0, 0.7,
+                                      np.array([1e-4, 2e-5, 0.0]), 60)
+    assert abs(f[T.bulk].sum() - m0) <= 1e-12 * m0
+
+
+def poiseuille_startup(y, t, F, nu, H, nterms=2001):
+    """u(y,t) of a channel started from rest: steady parabola minus the odd
+    sine series (yh = y - 1/2 measured from the half-way wall)."""
+    yh = y - 0.5
+    u = F / (2 * nu) * yh * (H - yh)
+    for n in range(1, nterms, 2):
+        u -= (4 * F * H ** 2 / (nu * math.pi ** 3 * n ** 3)) * np.sin(n * math.pi * yh / H) * \
+            math.exp(-(n * math.pi) ** 2 * nu * t / H ** 2)
+    return u
+
+
+def test_channel_matches_poiseuille_startup_series():
+    """C1: D2Q9 256x128 channel, tau=1, F=1e-6, 5000 steps, from the reference
+    run; half-way walls at y=1/2 and y=126.5 (H=126), nu=1/6.  BGK + half-way
+    bounce-back carries a tau-dependent slip, so the bar is 1e-3 of the peak."""
+    d = load_ref("d2q9_channel_256x128")
+    p = d["params"]
+    y = np.arange(1, 127)
+    u = poiseuille_startup(y, p["steps"] + 0.5, p["force"][0], 1.0 / 6.0, 126.0)
+    err = np.abs(d["velx_profile"] - u).max() / u.max()
+    assert err < 1e-3
+    # the flow is far from steady at 5000 steps (about 39% of the steady peak)
+    steady = p["force"][0] / (2 / 6.0) * 63 * 63
+    assert 0.3 < d["velx_profile"].max() / steady < 0.5
+
+
+def test_twophase_phase_swap_symmetry():
+    d = load_ref("tp_pack_16")
+    p = d["params"]
+    T = OracleTopo(d["geo"].astype(np.int64), p["lattice"], p["periodic"])
+    f0, rho = tp_start(d, T)
+    args = (p["tau0"], p["tau0"], p["sigma"], p["beta"], 0, [1e-6, 0.0, 0.0], 30)
+    fa, ra, va, ca, _ = oracle.run_twophase(1, T.neighbors, T.node_types, T.bulk, f0, rho, *args)
+    fb, rb, vb, cb, _ = oracle.run_twophase(1, T.neighbors, T.node_types, T.bulk, f0[:, ::-1].copy(),
+                                            rho[:, ::-1].copy(), *args)
+    b = T.bulk
+    assert np.allclose(fa[b, 0], fb[b, 1], rtol=0, atol=1e-14)
+    assert np.allclose(fa[b, 1], fb[b, 0], rtol=0, atol=1e-14)
+    assert np.allclose(ca[b], -cb[b], rtol=0, atol=1e-12)
+    assert np.abs(va[b] - vb[b]).max() <= 1e-12 * np.abs(va[b]).max()
+
+
+def test_twophase_one_phase_limit_is_bgk():
+    """rho1 = 0 everywhere and fully wetting walls: phi = 1, no colour
+    gradient, no recolouring; the model is BGK + Guo with tau = tau0."""
+    geo = OracleTopo(np.ones((6, 5, 4), dtype=np.int64), "D3Q19", "xyz")
+    rng = np.random.default_rng(1)
+    T = geo
+    r0 = np.ones(T.n_nodes) + 0.001 * rng.random(T.n_nodes)
+    f0 = np.zeros((T.n_nodes, 2, 19))
+    W = oracle.WEIGHTS[1]
+    f0[T.bulk, 0] = W[None] * r0[T.bulk, None]
+    rho = np.stack([r0, np.zeros(T.n_nodes)], 1)
+    F = [1e-5, 0.0, 2e-6]
+    f2, *_ = oracle.run_twophase(1, T.neighbors, T.node_types, T.bulk, f0, rho, 0.8, 1.3, 0.01, 0.7, 0, F, 20)
+    f1, _, _ = oracle.run_std_case(1, T.neighbors, T.node_types, T.bulk, f0[:, 0], 0.8, np.array(F), 20)
+    assert np.allclose(f2[T.bulk, 0], f1[T.bulk], rtol=0, atol=1e-15)
+    assert np.all(f2[T.bulk, 1] == 0.0)
+
+
+def test_flux_sums_match_numpy():
+    d = load_ref("tp_pack_6x5x4")
+    p = d["params"]
+    T = OracleTopo(d["geo"].astype(np.int64), p["lattice"], p["periodic"])
+    f0, rho = tp_start(d, T)
+    f, *_ = oracle.run_twophase(1, T.neighbors, T.node_types, T.bulk, f0, rho, 1.0, 1.0, 0.01, 0.7, 0,
+                                [1e-5, 0, 0], 5)
+    from oracle.vtklb_oracle import BASIS
+    c = BASIS["D3Q19"][:, 0]
+    ft = f[T.bulk, 0] + f[T.bulk, 1]
+    fc = ft @ c
+    r0, r1 = f[T.bulk, 0].sum(1), f[T.bulk, 1].sum(1)
+    s = oracle.flux(1, T.bulk, f, 2, 0)
+    assert np.isclose(s[0], fc.sum(), rtol=1e-13, atol=1e-18)
+    assert np.isclose(s[1], (r0 / (r0 + r1) * fc).sum(), rtol=1e-13, atol=1e-18)
+    assert np.isclose(s[2], (r1 / (r0 + r1) * fc).sum(), rtol=1e-13, atol=1e-18)
