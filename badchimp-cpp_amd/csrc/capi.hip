@@ -10,7 +10,7 @@
 #include <vector>
 
 #include "../../include/badchimp_amd.h"
-#include "context.hpp"
+#include "ctx.hpp"
 #include "errors.hpp"
 #include "kernels.hpp"
 #include "kernels_f32.hpp"
@@ -112,12 +112,256 @@ int pre_state(lb_ctx* c, const void** out) {
   hipError_t e;
   if (c->precision == LB_FP64)
     e = gather_f64(c->lattice, c->n_fields, static_cast<const double*>(c->d_f[c->cur]),
-                   static_cast<double*>(dst), c->d_link, c->n_bulk, c->NP, c->stream);
+                   static_cast<double*>(dst), c->addr, c->n_bulk, c->NP, c->stream);
   else
-    e = gather_lowp(c->precision, c->lattice, c->n_fields, c->d_f[c->cur], dst, c->d_link,
+    e = gather_lowp(c->precision, c->lattice, c->n_fields, c->d_f[c->cur], dst, c->addr,
                     c->n_bulk, c->NP, c->stream);
   if (e != hipSuccess) return fail(c, LB_ERR_HIP, "gather: %s", hipGetErrorString(e));
   *out = dst;
+  return LB_OK;
+}
+
+// Cartesian row layout of the bulk nodes (see addressing.hpp).  Returns false
+// with a reason when the node list is not a set of cells of one box that the
+// layout can address.
+struct RowBuild {
+  RowGeom g{};
+  std::vector<int32_t> order;   // compact index -> bulk index
+  std::vector<uint32_t> coord;  // compact index -> packed cell
+  std::vector<uint2> rows;
+};
+
+bool build_rows(int lattice, int32_t n_nodes, const int32_t* nb, const int32_t* bulk, int32_t n_bulk,
+                const std::vector<int32_t>& bidx, const int32_t* pos, RowBuild& rb, std::string& why) {
+  const int nD = lattice == LB_D2Q9 ? 2 : 3, Q = lattice == LB_D2Q9 ? 9 : 19;
+  (void)n_nodes;
+  auto P = [&](int32_t n, int a) -> int32_t {
+    if (nD == 3) return pos[static_cast<size_t>(n) * 3 + a];
+    return a == 0 ? pos[static_cast<size_t>(n) * 2] : (a == 1 ? 0 : pos[static_cast<size_t>(n) * 2 + 1]);
+  };
+  auto C = [&](int q, int a) -> int {
+    const int d = nD == 3 ? a : (a == 0 ? 0 : (a == 2 ? 1 : -1));
+    if (d < 0) return 0;
+    return lattice == LB_D2Q9 ? D2Q9::c(q, d) : D3Q19::c(q, d);
+  };
+  if (n_bulk == 0) { why = "no bulk nodes"; return false; }
+  int32_t lo[3], hi[3], period[3] = {0, 0, 0};
+  for (int a = 0; a < 3; ++a) { lo[a] = P(bulk[0], a); hi[a] = lo[a]; }
+  for (int32_t k = 0; k < n_bulk; ++k)
+    for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], P(bulk[k], a)); hi[a] = std::max(hi[a], P(bulk[k], a)); }
+  // periods from the fluid links that wrap
+  for (int32_t k = 0; k < n_bulk; ++k) {
+    const int32_t n = bulk[k];
+    for (int q = 0; q < Q - 1; ++q) {
+      const int32_t m = nb[static_cast<size_t>(n) * Q + q];
+      if (m <= 0 || bidx[m] < 0) continue;
+      for (int a = 0; a < 3; ++a) {
+        const int32_t d = std::abs(P(m, a) - P(n, a) - C(q, a));
+        if (d == 0) continue;
+        if (period[a] == 0) period[a] = d;
+        else if (period[a] != d) { why = "inconsistent periodic wrap"; return false; }
+      }
+    }
+  }
+  int32_t E[3];
+  for (int a = 0; a < 3; ++a) {
+    E[a] = period[a] ? period[a] : hi[a] - lo[a] + 1;
+    if (hi[a] - lo[a] + 1 > E[a]) { why = "bulk nodes wider than the period"; return false; }
+  }
+  if (E[0] > 1024 || E[1] > 1024 || E[2] > 4096) { why = "box too large for packed coordinates"; return false; }
+  RowGeom& g = rb.g;
+  for (int a = 0; a < 3; ++a) g.E[a] = E[a];
+  g.periodic = (period[0] ? 1 : 0) | (period[1] ? 2 : 0) | (period[2] ? 4 : 0);
+  g.lx = E[0] >= 8 ? 3 : 0;
+  g.ly = E[1] >= 8 ? 3 : 0;
+  const int32_t nTx = (E[0] + (1 << g.lx) - 1) >> g.lx;
+  g.nTy = (E[1] + (1 << g.ly) - 1) >> g.ly;
+  g.W = (E[2] + 31) / 32;
+  const int64_t R = static_cast<int64_t>(nTx) * g.nTy << (g.lx + g.ly);
+  auto cell = [&](int32_t n, int32_t* X) {
+    for (int a = 0; a < 3; ++a) {
+      int32_t v = P(n, a) - lo[a];
+      v %= E[a];
+      if (v < 0) v += E[a];
+      X[a] = v;
+    }
+  };
+  std::vector<int64_t> key(n_bulk);
+  for (int32_t k = 0; k < n_bulk; ++k) {
+    int32_t X[3];
+    cell(bulk[k], X);
+    key[k] = static_cast<int64_t>(g.row(X[0], X[1])) * (static_cast<int64_t>(g.W) * 32) + X[2];
+  }
+  rb.order.resize(n_bulk);
+  for (int32_t k = 0; k < n_bulk; ++k) rb.order[k] = k;
+  std::stable_sort(rb.order.begin(), rb.order.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+  for (int32_t i = 1; i < n_bulk; ++i)
+    if (key[rb.order[i]] == key[rb.order[i - 1]]) { why = "two bulk nodes share a cell"; return false; }
+  rb.coord.resize(n_bulk);
+  rb.rows.assign(static_cast<size_t>(R) * g.W, uint2{0u, 0u});
+  for (int32_t i = 0; i < n_bulk; ++i) {
+    int32_t X[3];
+    cell(bulk[rb.order[i]], X);
+    rb.coord[i] = (static_cast<uint32_t>(X[0]) << 22) | (static_cast<uint32_t>(X[1]) << 12) | static_cast<uint32_t>(X[2]);
+    uint2& e = rb.rows[static_cast<size_t>(g.row(X[0], X[1])) * g.W + (X[2] >> 5)];
+    e.y |= 1u << (X[2] & 31);
+  }
+  uint32_t run = 0;
+  for (auto& e : rb.rows) {
+    e.x = run;
+    run += static_cast<uint32_t>(__builtin_popcount(e.y));
+  }
+  return true;
+}
+
+// Source of direction q for compact node i under the row layout, as the
+// kernels compute it: {stream, j} or {bounce, i}.
+std::pair<bool, int64_t> row_source(int lattice, const RowBuild& rb, int q, int32_t i) {
+  const int nD = lattice == LB_D2Q9 ? 2 : 3;
+  auto C = [&](int a) -> int {
+    const int d = nD == 3 ? a : (a == 0 ? 0 : (a == 2 ? 1 : -1));
+    if (d < 0) return 0;
+    return lattice == LB_D2Q9 ? D2Q9::c(q, d) : D3Q19::c(q, d);
+  };
+  const uint32_t cc = rb.coord[i];
+  int32_t X[3] = {static_cast<int32_t>(cc >> 22), static_cast<int32_t>((cc >> 12) & 1023u), static_cast<int32_t>(cc & 4095u)};
+  for (int a = 0; a < 3; ++a) {
+    if (C(a) == 0) continue;
+    X[a] -= C(a);
+    if (X[a] < 0 || X[a] >= rb.g.E[a]) {
+      if (rb.g.periodic & (1 << a)) X[a] += X[a] < 0 ? rb.g.E[a] : -rb.g.E[a];
+      else return {false, i};
+    }
+  }
+  const uint2 e = rb.rows[static_cast<size_t>(rb.g.row(X[0], X[1])) * rb.g.W + (X[2] >> 5)];
+  const uint32_t bit = 1u << (X[2] & 31);
+  if (e.y & bit) return {true, static_cast<int64_t>(e.x + __builtin_popcount(e.y & (bit - 1u)))};
+  return {false, i};
+}
+
+int setup_topology(lb_ctx* c, int32_t n_nodes, const int32_t* neighbor, const int16_t* node_type,
+                   const int32_t* bulk, int32_t n_bulk, const int32_t* pos, int32_t flags) {
+  if (n_nodes < 1 || n_bulk < 0 || !neighbor || !node_type || (n_bulk > 0 && !bulk))
+    return fail(c, LB_ERR_ARG, "lb_set_topology: bad argument");
+  if (n_bulk >= static_cast<int32_t>(kLinkIndexMask)) return fail(c, LB_ERR_ARG, "too many bulk nodes");
+  hipSetDevice(c->device);
+  const int Q = c->Q;
+  std::vector<int32_t> bidx(n_nodes, -1);  // node -> bulk index
+  for (int32_t k = 0; k < n_bulk; ++k) {
+    const int32_t n = bulk[k];
+    if (n <= 0 || n >= n_nodes) return fail(c, LB_ERR_ARG, "bulk node %d out of range", n);
+    if (k > 0 && bulk[k - 1] >= n) return fail(c, LB_ERR_ARG, "bulk list must be strictly ascending");
+    if (node_type[n] < 2) return fail(c, LB_ERR_ARG, "bulk node %d is not fluid (type %d)", n, node_type[n]);
+    bidx[n] = k;
+  }
+  for (size_t k = 0; k < static_cast<size_t>(n_nodes) * Q; ++k)
+    if (neighbor[k] < 0 || neighbor[k] >= n_nodes) return fail(c, LB_ERR_ARG, "neighbour index out of range");
+  // compact order: the bulk order, or the tiled row order of the row layout
+  RowBuild rb;
+  std::string why = "no positions given";
+  bool rows_ok = pos && !(flags & LB_TOPO_TABLE_ONLY) &&
+                 build_rows(c->lattice, n_nodes, neighbor, bulk, n_bulk, bidx, pos, rb, why);
+  std::vector<int32_t> order(n_bulk);
+  for (int32_t k = 0; k < n_bulk; ++k) order[k] = rows_ok ? rb.order[k] : k;
+  std::vector<int32_t> compact(n_nodes, -1);
+  for (int32_t i = 0; i < n_bulk; ++i) compact[bulk[order[i]]] = i;
+  // The push of the reference writes f_post(p, m) into slot (p, neighbor(p,m)).
+  // For node 0 the slot is shared: the last bulk node (ascending) wins.
+  std::vector<int32_t> winner(Q, -1);
+  for (int32_t k = 0; k < n_bulk; ++k) {
+    const int32_t* nb = neighbor + static_cast<size_t>(bulk[k]) * Q;
+    for (int p = 0; p < Q - 1; ++p)
+      if (nb[p] == 0) winner[p] = compact[bulk[k]];
+  }
+  const int64_t NP = ((static_cast<int64_t>(n_bulk) + kBlock - 1) / kBlock) * kBlock + (n_bulk == 0 ? kBlock : 0);
+  std::vector<uint32_t> link(static_cast<size_t>(Q - 1) * NP, 0u);
+  std::vector<int32_t> wall_slot(n_nodes, -1);
+  std::vector<int32_t> walls;
+  for (int32_t i = 0; i < n_bulk; ++i) {
+    const int32_t n = bulk[order[i]];
+    const int32_t* nb = neighbor + static_cast<size_t>(n) * Q;
+    for (int q = 0; q < Q - 1; ++q) {
+      const int32_t s = nb[rev_of(c->lattice, q)];
+      uint32_t l;
+      if (compact[s] >= 0) {
+        l = (kLinkStream << 30) | static_cast<uint32_t>(compact[s]);
+      } else if (node_type[s] > 1) {
+        return fail(c, LB_ERR_UNSUPPORTED,
+                    "bulk node %d has a fluid neighbour %d on another rank: halo exchange is not available on one GPU", n, s);
+      } else if (s == 0) {
+        if (!(flags & LB_TOPO_ALLOW_GHOST0))
+          return fail(c, LB_ERR_GHOST0_LINK,
+                      "bulk node %d links to the ghost node 0 (direction %d); make the geometry periodic or walled, "
+                      "or pass LB_TOPO_ALLOW_GHOST0 to reproduce the reference's shared-slot result", n, q);
+        l = (kLinkGhost0 << 30) | static_cast<uint32_t>(winner[rev_of(c->lattice, q)]);
+        rows_ok = false;
+        why = "fluid links to the ghost node 0";
+      } else {
+        if (wall_slot[s] < 0) {
+          wall_slot[s] = static_cast<int32_t>(walls.size());
+          walls.push_back(s);
+        }
+        l = (kLinkBounce << 30) | static_cast<uint32_t>(wall_slot[s]);
+      }
+      link[static_cast<size_t>(q) * NP + i] = l;
+      if (rows_ok) {
+        const auto r = row_source(c->lattice, rb, q, i);
+        const bool stream = (l >> 30) == kLinkStream;
+        if (r.first != stream || (stream && r.second != static_cast<int64_t>(l & kLinkIndexMask))) {
+          rows_ok = false;
+          why = "row layout disagrees with the neighbour index";
+        }
+      }
+    }
+  }
+  // device allocation
+  hipStreamSynchronize(c->stream);
+  free_fields(c);
+  for (void* p : {static_cast<void*>(c->d_coord), static_cast<void*>(c->d_rows)})
+    if (p) hipFree(p);
+  c->d_coord = nullptr;
+  c->d_rows = nullptr;
+  c->bytes = 0;
+  int rc;
+  if ((rc = dev_alloc(c, &c->d_link, link.size())) != LB_OK) return rc;
+  LB_HIP(c, hipMemcpy(c->d_link, link.data(), link.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  c->addr = Addr{};
+  c->addr.layout = 0;
+  c->addr.table = TableAddr{c->d_link, NP};
+  if (rows_ok) {
+    std::vector<uint32_t> coord(NP, 0u);
+    std::copy(rb.coord.begin(), rb.coord.end(), coord.begin());
+    if ((rc = dev_alloc(c, &c->d_coord, coord.size())) != LB_OK) return rc;
+    if ((rc = dev_alloc(c, &c->d_rows, rb.rows.size())) != LB_OK) return rc;
+    LB_HIP(c, hipMemcpy(c->d_coord, coord.data(), coord.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    LB_HIP(c, hipMemcpy(c->d_rows, rb.rows.data(), rb.rows.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    c->addr.layout = 1;
+    c->addr.rows = RowAddr{c->d_coord, c->d_rows, NP, rb.g};
+  }
+  c->layout_note = rows_ok ? "rows" : why;
+  c->n_blocks = (n_bulk + kBlock - 1) / kBlock;
+  if (c->n_blocks == 0) c->n_blocks = 1;
+  if ((rc = dev_alloc(c, &c->d_rho, 2 * NP)) != LB_OK) return rc;
+  if ((rc = dev_alloc(c, &c->d_vel, static_cast<size_t>(c->nD) * NP)) != LB_OK) return rc;
+  if ((rc = dev_alloc(c, &c->d_phi, NP)) != LB_OK) return rc;
+  if ((rc = dev_alloc(c, &c->d_phi_wall, walls.size() + 1)) != LB_OK) return rc;
+  if ((rc = dev_alloc(c, &c->d_partial, 3 * c->n_blocks)) != LB_OK) return rc;
+  if ((rc = dev_alloc(c, &c->d_scalar, 8)) != LB_OK) return rc;
+  LB_HIP(c, hipMemset(c->d_phi_wall, 0, (walls.size() + 1) * sizeof(double)));
+  LB_HIP(c, hipMemset(c->d_rho, 0, 2 * NP * sizeof(double)));
+  LB_HIP(c, hipMemset(c->d_vel, 0, c->nD * NP * sizeof(double)));
+  LB_HIP(c, hipMemset(c->d_phi, 0, NP * sizeof(double)));
+  LB_HIP(c, hipMemset(c->d_scalar, 0, 8 * sizeof(double)));
+  c->n_nodes = n_nodes;
+  c->n_bulk = n_bulk;
+  c->NP = NP;
+  c->n_global = n_bulk;
+  c->node_of.resize(n_bulk);
+  for (int32_t i = 0; i < n_bulk; ++i) c->node_of[i] = bulk[order[i]];
+  c->wall_nodes = walls;
+  c->node_type.assign(node_type, node_type + n_nodes);
+  c->has_topo = true;
+  c->have_macro = false;
   return LB_OK;
 }
 
@@ -163,7 +407,8 @@ int lb_ctx_destroy(lb_ctx* c) {
   free_fields(c);
   for (void* p : {static_cast<void*>(c->d_link), static_cast<void*>(c->d_rho), static_cast<void*>(c->d_vel),
                   static_cast<void*>(c->d_phi), static_cast<void*>(c->d_phi_wall), static_cast<void*>(c->d_partial),
-                  static_cast<void*>(c->d_scalar)})
+                  static_cast<void*>(c->d_scalar), static_cast<void*>(c->d_coord),
+                  static_cast<void*>(c->d_rows)})
     if (p) hipFree(p);
   hipEventDestroy(c->ev0);
   hipEventDestroy(c->ev1);
@@ -175,90 +420,19 @@ int lb_ctx_destroy(lb_ctx* c) {
 int lb_set_topology(lb_ctx* c, int32_t n_nodes, const int32_t* neighbor, const int16_t* node_type,
                     const int32_t* bulk, int32_t n_bulk, int32_t flags) {
   if (!c) return LB_ERR_ARG;
-  if (n_nodes < 1 || n_bulk < 0 || !neighbor || !node_type || (n_bulk > 0 && !bulk))
-    return fail(c, LB_ERR_ARG, "lb_set_topology: bad argument");
-  if (n_bulk >= static_cast<int32_t>(kLinkIndexMask)) return fail(c, LB_ERR_ARG, "too many bulk nodes");
-  hipSetDevice(c->device);
-  const int Q = c->Q;
-  std::vector<int32_t> compact(n_nodes, -1);
-  for (int32_t i = 0; i < n_bulk; ++i) {
-    const int32_t n = bulk[i];
-    if (n <= 0 || n >= n_nodes) return fail(c, LB_ERR_ARG, "bulk node %d out of range", n);
-    if (i > 0 && bulk[i - 1] >= n) return fail(c, LB_ERR_ARG, "bulk list must be strictly ascending");
-    if (node_type[n] < 2) return fail(c, LB_ERR_ARG, "bulk node %d is not fluid (type %d)", n, node_type[n]);
-    compact[n] = i;
-  }
-  // The push of the reference writes f_post(p, m) into slot (p, neighbor(p,m)).
-  // For node 0 the slot is shared: the last bulk node (ascending) wins.
-  std::vector<int32_t> winner(Q, -1);
-  for (int32_t i = 0; i < n_bulk; ++i) {
-    const int32_t* nb = neighbor + static_cast<size_t>(bulk[i]) * Q;
-    for (int p = 0; p < Q - 1; ++p)
-      if (nb[p] == 0) winner[p] = i;
-  }
-  const int64_t NP = ((static_cast<int64_t>(n_bulk) + kBlock - 1) / kBlock) * kBlock + (n_bulk == 0 ? kBlock : 0);
-  std::vector<uint32_t> link(static_cast<size_t>(Q - 1) * NP, 0u);
-  std::vector<int32_t> wall_slot(n_nodes, -1);
-  std::vector<int32_t> walls;
-  for (int32_t i = 0; i < n_bulk; ++i) {
-    const int32_t n = bulk[i];
-    const int32_t* nb = neighbor + static_cast<size_t>(n) * Q;
-    for (int q = 0; q < Q - 1; ++q) {
-      const int32_t s = nb[rev_of(c->lattice, q)];
-      if (s < 0 || s >= n_nodes) return fail(c, LB_ERR_ARG, "neighbour of node %d out of range", n);
-      uint32_t l;
-      if (compact[s] >= 0) {
-        l = (kLinkStream << 30) | static_cast<uint32_t>(compact[s]);
-      } else if (node_type[s] > 1) {
-        return fail(c, LB_ERR_UNSUPPORTED,
-                    "bulk node %d has a fluid neighbour %d on another rank: halo exchange is not available on one GPU", n, s);
-      } else if (s == 0) {
-        if (!(flags & LB_TOPO_ALLOW_GHOST0))
-          return fail(c, LB_ERR_GHOST0_LINK,
-                      "bulk node %d links to the ghost node 0 (direction %d); make the geometry periodic or walled, "
-                      "or pass LB_TOPO_ALLOW_GHOST0 to reproduce the reference's shared-slot result", n, q);
-        l = (kLinkGhost0 << 30) | static_cast<uint32_t>(winner[rev_of(c->lattice, q)]);
-      } else {
-        if (wall_slot[s] < 0) {
-          wall_slot[s] = static_cast<int32_t>(walls.size());
-          walls.push_back(s);
-        }
-        l = (kLinkBounce << 30) | static_cast<uint32_t>(wall_slot[s]);
-      }
-      link[static_cast<size_t>(q) * NP + i] = l;
-    }
-  }
-  // device allocation
-  hipStreamSynchronize(c->stream);
-  free_fields(c);
-  c->bytes = 0;
-  int rc;
-  if ((rc = dev_alloc(c, &c->d_link, link.size())) != LB_OK) return rc;
-  LB_HIP(c, hipMemcpy(c->d_link, link.data(), link.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-  c->n_blocks = (n_bulk + kBlock - 1) / kBlock;
-  if (c->n_blocks == 0) c->n_blocks = 1;
-  if ((rc = dev_alloc(c, &c->d_rho, 2 * NP)) != LB_OK) return rc;
-  if ((rc = dev_alloc(c, &c->d_vel, static_cast<size_t>(c->nD) * NP)) != LB_OK) return rc;
-  if ((rc = dev_alloc(c, &c->d_phi, NP)) != LB_OK) return rc;
-  if ((rc = dev_alloc(c, &c->d_phi_wall, walls.size() + 1)) != LB_OK) return rc;
-  if ((rc = dev_alloc(c, &c->d_partial, 3 * c->n_blocks)) != LB_OK) return rc;
-  if ((rc = dev_alloc(c, &c->d_scalar, 8)) != LB_OK) return rc;
-  LB_HIP(c, hipMemset(c->d_phi_wall, 0, (walls.size() + 1) * sizeof(double)));
-  LB_HIP(c, hipMemset(c->d_rho, 0, 2 * NP * sizeof(double)));
-  LB_HIP(c, hipMemset(c->d_vel, 0, c->nD * NP * sizeof(double)));
-  LB_HIP(c, hipMemset(c->d_phi, 0, NP * sizeof(double)));
-  LB_HIP(c, hipMemset(c->d_scalar, 0, 8 * sizeof(double)));
-  c->n_nodes = n_nodes;
-  c->n_bulk = n_bulk;
-  c->NP = NP;
-  c->n_global = n_bulk;
-  c->bulk.assign(bulk, bulk + n_bulk);
-  c->wall_nodes = walls;
-  c->node_type.assign(node_type, node_type + n_nodes);
-  c->has_topo = true;
-  c->have_macro = false;
-  return LB_OK;
+  return setup_topology(c, n_nodes, neighbor, node_type, bulk, n_bulk, nullptr, flags);
 }
+
+int lb_set_topology_grid(lb_ctx* c, int32_t n_nodes, const int32_t* neighbor, const int16_t* node_type,
+                         const int32_t* bulk, int32_t n_bulk, const int32_t* pos, int32_t flags) {
+  if (!c) return LB_ERR_ARG;
+  if (!pos) return fail(c, LB_ERR_ARG, "lb_set_topology_grid: null positions");
+  return setup_topology(c, n_nodes, neighbor, node_type, bulk, n_bulk, pos, flags);
+}
+
+int32_t lb_layout(const lb_ctx* c) { return c && c->has_topo ? c->addr.layout : -1; }
+
+const char* lb_layout_note(const lb_ctx* c) { return c ? c->layout_note.c_str() : "null context"; }
 
 int lb_set_global_bulk_count(lb_ctx* c, int64_t n) {
   if (!c) return LB_ERR_ARG;
@@ -281,7 +455,7 @@ int lb_upload_f(lb_ctx* c, int32_t nf, const double* f) {
   if (c->precision == LB_FP64) {
     std::vector<double> h(count, 0.0);
     for (int32_t i = 0; i < c->n_bulk; ++i) {
-      const double* src = f + static_cast<size_t>(c->bulk[i]) * nf * Q;
+      const double* src = f + static_cast<size_t>(c->node_of[i]) * nf * Q;
       for (int k = 0; k < nf; ++k)
         for (int q = 0; q < Q; ++q) h[(static_cast<size_t>(k) * Q + q) * NP + i] = src[k * Q + q];
     }
@@ -289,7 +463,7 @@ int lb_upload_f(lb_ctx* c, int32_t nf, const double* f) {
   } else {
     std::vector<double> h(count, 0.0);
     for (int32_t i = 0; i < c->n_bulk; ++i) {
-      const double* src = f + static_cast<size_t>(c->bulk[i]) * nf * Q;
+      const double* src = f + static_cast<size_t>(c->node_of[i]) * nf * Q;
       for (int k = 0; k < nf; ++k)
         for (int q = 0; q < Q; ++q) h[(static_cast<size_t>(k) * Q + q) * NP + i] = src[k * Q + q];
     }
@@ -332,7 +506,7 @@ int lb_download_f(lb_ctx* c, int32_t nf, double* f) {
     if (e != hipSuccess) return fail(c, LB_ERR_HIP, "download conversion: %s", hipGetErrorString(e));
   }
   for (int32_t i = 0; i < c->n_bulk; ++i) {
-    double* dst = f + static_cast<size_t>(c->bulk[i]) * nf * Q;
+    double* dst = f + static_cast<size_t>(c->node_of[i]) * nf * Q;
     for (int k = 0; k < nf; ++k)
       for (int q = 0; q < Q; ++q) dst[k * Q + q] = h[(static_cast<size_t>(k) * Q + q) * NP + i];
   }
@@ -356,7 +530,7 @@ int lb_init_equilibrium(lb_ctx* c, int32_t n_fields, int32_t field, int32_t rho_
   const int64_t NP = c->NP;
   std::vector<double> hr(NP, 0.0), hv(static_cast<size_t>(c->nD) * NP, 0.0);
   for (int32_t i = 0; i < c->n_bulk; ++i) {
-    const int32_t n = c->bulk[i];
+    const int32_t n = c->node_of[i];
     hr[i] = rho[static_cast<size_t>(n) * rho_nfields + rho_field];
     for (int d = 0; d < c->nD; ++d) hv[d * NP + i] = vel[static_cast<size_t>(n) * c->nD + d];
   }
@@ -397,13 +571,13 @@ int lb_download_scalar(lb_ctx* c, int32_t which, int32_t nf, double* s) {
     std::vector<double> h(2 * NP);
     LB_HIP(c, hipMemcpy(h.data(), c->d_rho, 2 * NP * sizeof(double), hipMemcpyDeviceToHost));
     for (int32_t i = 0; i < c->n_bulk; ++i)
-      for (int k = 0; k < nf; ++k) s[static_cast<size_t>(c->bulk[i]) * nf + k] = h[k * NP + i];
+      for (int k = 0; k < nf; ++k) s[static_cast<size_t>(c->node_of[i]) * nf + k] = h[k * NP + i];
   } else if (which == LB_PHI) {
     if (nf != 1) return fail(c, LB_ERR_ARG, "phi has one field");
     if (c->n_fields != 2) return fail(c, LB_ERR_STATE, "phi exists only for two-phase fields");
     std::vector<double> h(NP);
     LB_HIP(c, hipMemcpy(h.data(), c->d_phi, NP * sizeof(double), hipMemcpyDeviceToHost));
-    for (int32_t i = 0; i < c->n_bulk; ++i) s[c->bulk[i]] = h[i];
+    for (int32_t i = 0; i < c->n_bulk; ++i) s[c->node_of[i]] = h[i];
   } else {
     return fail(c, LB_ERR_ARG, "unknown scalar %d", which);
   }
@@ -420,7 +594,7 @@ int lb_download_vector(lb_ctx* c, int32_t which, int32_t nf, double* v) {
   std::vector<double> h(static_cast<size_t>(c->nD) * NP);
   LB_HIP(c, hipMemcpy(h.data(), c->d_vel, h.size() * sizeof(double), hipMemcpyDeviceToHost));
   for (int32_t i = 0; i < c->n_bulk; ++i)
-    for (int d = 0; d < c->nD; ++d) v[static_cast<size_t>(c->bulk[i]) * c->nD + d] = h[d * NP + i];
+    for (int d = 0; d < c->nD; ++d) v[static_cast<size_t>(c->node_of[i]) * c->nD + d] = h[d * NP + i];
   return LB_OK;
 }
 
@@ -461,11 +635,11 @@ int lb_step_bgk(lb_ctx* c, int32_t nsteps, double tau, const double* force) {
     hipError_t e;
     if (c->precision == LB_FP64)
       e = bgk_f64(c->lattice, !c->post, last, false, static_cast<const double*>(c->d_f[c->cur]),
-                  static_cast<double*>(c->d_f[1 - c->cur]), c->d_link, c->n_bulk, c->NP, a, c->d_rho,
+                  static_cast<double*>(c->d_f[1 - c->cur]), c->addr, c->n_bulk, c->NP, a, c->d_rho,
                   c->d_vel, c->d_partial, c->stream);
     else
       e = bgk_lowp(c->precision, c->lattice, !c->post, last, false, c->d_f[c->cur], c->d_f[1 - c->cur],
-                   c->d_link, c->n_bulk, c->NP, a, c->d_rho, c->d_vel, c->d_partial, c->stream);
+                   c->addr, c->n_bulk, c->NP, a, c->d_rho, c->d_vel, c->d_partial, c->stream);
     if (e != hipSuccess) return fail(c, LB_ERR_HIP, "bgk launch: %s", hipGetErrorString(e));
     c->cur = 1 - c->cur;
     c->post = true;
@@ -546,10 +720,10 @@ int lb_reduce_flux(lb_ctx* c, int32_t dir, double* sum_fc, double* sum_phi0_fc, 
   hipSetDevice(c->device);
   hipError_t e;
   if (c->precision == LB_FP64)
-    e = flux_f64(c->lattice, c->n_fields, !c->post, static_cast<const double*>(c->d_f[c->cur]), c->d_link,
+    e = flux_f64(c->lattice, c->n_fields, !c->post, static_cast<const double*>(c->d_f[c->cur]), c->addr,
                  c->n_bulk, c->NP, dir, c->d_partial, c->stream);
   else
-    e = flux_lowp(c->precision, c->lattice, c->n_fields, !c->post, c->d_f[c->cur], c->d_link, c->n_bulk,
+    e = flux_lowp(c->precision, c->lattice, c->n_fields, !c->post, c->d_f[c->cur], c->addr, c->n_bulk,
                   c->NP, dir, c->d_partial, c->stream);
   if (e == hipSuccess) e = sum_partials(c->d_partial, c->n_blocks, 3, c->d_scalar + 4, c->stream);
   double h[3] = {0, 0, 0};
@@ -588,11 +762,11 @@ int lb_solve_permeability(lb_ctx* c, double tau, const double* force, int32_t ch
       hipError_t e;
       if (c->precision == LB_FP64)
         e = bgk_f64(c->lattice, !c->post, last, last, static_cast<const double*>(c->d_f[c->cur]),
-                    static_cast<double*>(c->d_f[1 - c->cur]), c->d_link, c->n_bulk, c->NP, a, c->d_rho,
+                    static_cast<double*>(c->d_f[1 - c->cur]), c->addr, c->n_bulk, c->NP, a, c->d_rho,
                     c->d_vel, c->d_partial, c->stream);
       else
         e = bgk_lowp(c->precision, c->lattice, !c->post, last, last, c->d_f[c->cur], c->d_f[1 - c->cur],
-                     c->d_link, c->n_bulk, c->NP, a, c->d_rho, c->d_vel, c->d_partial, c->stream);
+                     c->addr, c->n_bulk, c->NP, a, c->d_rho, c->d_vel, c->d_partial, c->stream);
       if (e != hipSuccess) return fail(c, LB_ERR_HIP, "bgk launch: %s", hipGetErrorString(e));
       c->cur = 1 - c->cur;
       c->post = true;

@@ -43,37 +43,3 @@ struct CgArgs {
 };
 
 }  // namespace lbamd
-
-struct lb_ctx {
-  int lattice = 0, precision = 0, device = 0;
-  int Q = 0, nD = 0;
-  std::string err;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  double kernel_ms = 0.0;
-  int64_t launches = 0;
-
-  bool has_topo = false;
-  int32_t n_nodes = 0, n_bulk = 0;
-  int64_t NP = 0;
-  int64_t n_global = 0;
-  std::vector<int32_t> bulk;        // compact -> node number
-  std::vector<int32_t> wall_nodes;  // wall slot -> node number
-  std::vector<int16_t> node_type;
-  uint32_t* d_link = nullptr;
-
-  int n_fields = 0;  // populations allocated for this many fields
-  void* d_f[2] = {nullptr, nullptr};
-  int cur = 0;
-  bool post = false;  // buffer cur holds post-collision values (needs a gather)
-  double* d_rho = nullptr;   // [2][NP]
-  double* d_vel = nullptr;   // [nD][NP]
-  double* d_phi = nullptr;   // [NP]
-  double* d_phi_wall = nullptr;  // [n_wall + 1]
-  double* d_partial = nullptr;   // [3][n_blocks]
-  double* d_scalar = nullptr;    // [8]: 0 force x, 1..3 flux sums
-  int64_t n_blocks = 0;
-  bool have_macro = false;
-  double last_force[3] = {0, 0, 0};
-  int64_t bytes = 0;
-};

@@ -1,0 +1,47 @@
+// The context object behind the C ABI handle: one rank's subdomain on one GPU.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+struct lb_ctx {
+  int lattice = 0, precision = 0, device = 0;
+  int Q = 0, nD = 0;
+  std::string err;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double kernel_ms = 0.0;
+  int64_t launches = 0;
+
+  bool has_topo = false;
+  int32_t n_nodes = 0, n_bulk = 0;
+  int64_t NP = 0;
+  int64_t n_global = 0;
+  std::vector<int32_t> node_of;     // compact index -> reference node number
+  std::vector<int32_t> wall_nodes;  // wall slot -> node number
+  std::vector<int16_t> node_type;
+  uint32_t* d_link = nullptr;       // pull table in compact order (both layouts)
+  uint32_t* d_coord = nullptr;      // row layout: packed cell coordinates
+  uint2* d_rows = nullptr;          // row layout: {base, mask} per (row, word)
+  lbamd::Addr addr;                 // layout used by the single-phase kernels
+  std::string layout_note;          // "rows", or why the table layout is used
+
+  int n_fields = 0;  // populations allocated for this many fields
+  void* d_f[2] = {nullptr, nullptr};
+  int cur = 0;
+  bool post = false;  // buffer cur holds post-collision values (needs a gather)
+  double* d_rho = nullptr;       // [2][NP]
+  double* d_vel = nullptr;       // [nD][NP]
+  double* d_phi = nullptr;       // [NP]
+  double* d_phi_wall = nullptr;  // [n_wall + 1]
+  double* d_partial = nullptr;   // [3][n_blocks]
+  double* d_scalar = nullptr;    // [8]: 0 force x, 1 flux sum, 4..6 reductions
+  int64_t n_blocks = 0;
+  bool have_macro = false;
+  double last_force[3] = {0, 0, 0};
+  int64_t bytes = 0;
+};

@@ -4,37 +4,43 @@
 
 #include <cstdint>
 
+#include "addressing.hpp"
 #include "context.hpp"
 
 namespace lbamd {
+
+// Address layout of a launch (see addressing.hpp).
+struct Addr {
+  int layout = 0;  // 0 table, 1 rows
+  TableAddr table{};
+  RowAddr rows{};
+};
 
 // ---- FP64 (reference arithmetic, -ffp-contract=off) ----------------------
 // One BGK + Guo collide-stream step in pull form (std_case/main.cpp:110-145).
 // first: buffer holds pre-collision f (identity read); store: write rho/vel;
 // reduce: write per-block partial sums of j_x = sum_q f c_x + F_x/2.
 hipError_t bgk_f64(int lattice, bool first, bool store, bool reduce, const double* fin,
-                   double* fout, const uint32_t* link, int32_t n, int64_t NP,
-                   const BgkArgs& a, double* rho, double* vel, double* partial,
-                   hipStream_t s);
+                   double* fout, const Addr& ad, int32_t n, int64_t NP, const BgkArgs& a,
+                   double* rho, double* vel, double* partial, hipStream_t s);
 
 // Two-phase pass 1 (+ pass 2 partial sums): phi = (rho0-rho1)/(rho0+rho1)
-// and sum_q (f0+f1) c_x per block (main_TWOPHASE.cpp:238-298).
+// and sum_q (f0+f1) c_x per block (main_TWOPHASE.cpp:238-298).  Table layout.
 hipError_t cg_phi_f64(int lattice, bool first, bool momx, const double* fin,
                       const uint32_t* link, int32_t n, int64_t NP, double* phi,
                       double* partial, hipStream_t s);
-// Two-phase pass 3 (main_TWOPHASE.cpp:312-374) and bounce-back.
+// Two-phase pass 3 (main_TWOPHASE.cpp:312-374) and bounce-back.  Table layout.
 hipError_t cg_f64(int lattice, bool first, bool store, const double* fin, double* fout,
                   const uint32_t* link, int32_t n, int64_t NP, const CgArgs& a,
                   const double* phi, const double* phi_wall, const double* dscalar,
                   double* rho, double* vel, hipStream_t s);
 // Streaming only: fout = pre-collision state of the post-collision buffer fin.
 hipError_t gather_f64(int lattice, int nfields, const double* fin, double* fout,
-                      const uint32_t* link, int32_t n, int64_t NP, hipStream_t s);
+                      const Addr& ad, int32_t n, int64_t NP, hipStream_t s);
 // Flux partial sums over the pre-collision state (LBglobalforcing.h:8-33,61-78):
 // three per-block partials (sum fc, sum phi0 fc, sum phi1 fc).
-hipError_t flux_f64(int lattice, int nfields, bool first, const double* fin,
-                    const uint32_t* link, int32_t n, int64_t NP, int dir, double* partial,
-                    hipStream_t s);
+hipError_t flux_f64(int lattice, int nfields, bool first, const double* fin, const Addr& ad,
+                    int32_t n, int64_t NP, int dir, double* partial, hipStream_t s);
 // Equilibrium initialisation on bulk nodes (LBinitiatefield.h:50-56).
 hipError_t init_eq_f64(int lattice, double* f, int field, int32_t n, int64_t NP,
                        const double* rho, const double* vel, hipStream_t s);

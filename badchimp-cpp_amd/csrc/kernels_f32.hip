@@ -8,7 +8,7 @@
 // with the Guo term dF_q of LBcollision.h:206-211 unchanged.  Subtracting
 // w_q before rounding keeps the float (or half) mantissa for the deviation
 // from rest, which is what the flow carries.  Streaming and bounce-back use
-// the same pull table as the FP64 path.
+// the same address layouts as the FP64 path.
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
@@ -24,7 +24,6 @@ template <>
 struct Store<float> {
   static __device__ __forceinline__ float get(float v) { return v; }
   static __device__ __forceinline__ float put(float v) { return v; }
-  static constexpr float scale = 1.0f;
 };
 // half values travel as their 16-bit patterns
 template <>
@@ -35,18 +34,7 @@ struct Store<uint16_t> {
   static __device__ __forceinline__ uint16_t put(float v) {
     return __half_as_ushort(__float2half_rn(v * kHalfScale));
   }
-  static constexpr float scale = kHalfScale;
 };
-
-template <class L>
-__device__ __forceinline__ int64_t pull_index(const uint32_t* __restrict__ link, int q, int32_t i,
-                                              int64_t NP) {
-  const uint32_t l = __builtin_nontemporal_load(&link[q * NP + i]);
-  const uint32_t kind = l >> 30;
-  const int64_t j = static_cast<int64_t>(l & kLinkIndexMask);
-  const int rq = L::rev(q);
-  return kind == kLinkStream ? q * NP + j : (kind == kLinkBounce ? rq * NP + i : rq * NP + j);
-}
 
 __device__ __forceinline__ double block_sum(double v) {
   __shared__ double red[kBlock / 64];
@@ -64,26 +52,39 @@ __device__ __forceinline__ double block_sum(double v) {
   return r;
 }
 
-template <class L, typename S, bool FIRST, bool STORE, bool REDUCE>
-__global__ __launch_bounds__(kBlock) void k_bgk_lp(const S* __restrict__ gin, S* __restrict__ gout,
-                                                   const uint32_t* __restrict__ link, int32_t n,
-                                                   int64_t NP, float tau_inv, float tau_factor,
+template <class L, typename S, bool FIRST, class A>
+__device__ __forceinline__ void load_g(const S* __restrict__ gin, const A& ad, int32_t i, int64_t NP,
+                                       float (&g)[L::nQ]) {
+  if (FIRST) {
+#pragma unroll
+    for (int q = 0; q < L::nQ - 1; ++q) g[q] = Store<S>::get(gin[q * NP + i]);
+  } else {
+    // phase 1: all addressing loads; phase 2: all gathers
+    const typename A::Node nd = ad.node(i);
+    typename A::Pre p[L::nQ - 1];
+#pragma unroll
+    for (int q = 0; q < L::nQ - 1; ++q) p[q] = ad.template pre<L>(q, i, nd);
+#pragma unroll
+    for (int q = 0; q < L::nQ - 1; ++q) g[q] = Store<S>::get(gin[ad.template src<L>(q, i, nd, p[q])]);
+  }
+  g[L::nQ - 1] = Store<S>::get(gin[(L::nQ - 1) * NP + i]);
+}
+
+template <class L, typename S, class A, bool FIRST, bool STORE, bool REDUCE>
+__global__ __launch_bounds__(kBlock) void k_bgk_lp(const S* __restrict__ gin, S* __restrict__ gout, A ad,
+                                                   int32_t n, int64_t NP, float tau_inv, float tau_factor,
                                                    float F0, float F1, float F2,
                                                    double* __restrict__ rho_o,
                                                    double* __restrict__ vel_o,
                                                    double* __restrict__ partial) {
   constexpr int Q = L::nQ, D = L::nD;
-  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const int32_t b = xcd_block(blockIdx.x, gridDim.x);
+  const int32_t i = b * kBlock + threadIdx.x;
   double jx = 0.0;
   if (i < n) {
     const float F[3] = {F0, F1, F2};
     float g[Q];
-#pragma unroll
-    for (int q = 0; q < Q - 1; ++q) {
-      const int64_t idx = FIRST ? q * NP + i : pull_index<L>(link, q, i, NP);
-      g[q] = Store<S>::get(gin[idx]);
-    }
-    g[Q - 1] = Store<S>::get(gin[(Q - 1) * NP + i]);
+    load_g<L, S, FIRST>(gin, ad, i, NP, g);
     float drho = 0.0f;
 #pragma unroll
     for (int q = 0; q < Q; ++q) drho += g[q];
@@ -118,7 +119,7 @@ __global__ __launch_bounds__(kBlock) void k_bgk_lp(const S* __restrict__ gin, S*
   }
   if (REDUCE) {
     const double s = block_sum(jx);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    if (threadIdx.x == 0) partial[b] = s;
   }
 }
 
@@ -148,24 +149,23 @@ __global__ __launch_bounds__(kBlock) void k_load(const S* __restrict__ g, double
     }
 }
 
-template <class L, typename S>
+template <class L, typename S, class A>
 __global__ __launch_bounds__(kBlock) void k_gather_lp(const S* __restrict__ gin, S* __restrict__ gout,
-                                                      const uint32_t* __restrict__ link, int32_t n,
-                                                      int64_t NP, int nf) {
+                                                      A ad, int32_t n, int64_t NP, int nf) {
   constexpr int Q = L::nQ;
   const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const int64_t fs = static_cast<int64_t>(Q) * NP;
+  const typename A::Node nd = ad.node(i);
   for (int k = 0; k < nf; ++k) {
 #pragma unroll
-    for (int q = 0; q < Q - 1; ++q) gout[k * fs + q * NP + i] = gin[k * fs + pull_index<L>(link, q, i, NP)];
+    for (int q = 0; q < Q - 1; ++q) gout[k * fs + q * NP + i] = gin[k * fs + ad.template src<L>(q, i, nd)];
     gout[k * fs + (Q - 1) * NP + i] = gin[k * fs + (Q - 1) * NP + i];
   }
 }
 
-template <class L, typename S, bool FIRST>
-__global__ __launch_bounds__(kBlock) void k_flux_lp(const S* __restrict__ gin,
-                                                    const uint32_t* __restrict__ link, int32_t n,
+template <class L, typename S, class A, bool FIRST>
+__global__ __launch_bounds__(kBlock) void k_flux_lp(const S* __restrict__ gin, A ad, int32_t n,
                                                     int64_t NP, int dir, double* __restrict__ partial,
                                                     int64_t nblocks) {
   constexpr int Q = L::nQ;
@@ -173,12 +173,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_lp(const S* __restrict__ gin,
   double s0 = 0.0;
   if (i < n) {
     float g[Q];
-#pragma unroll
-    for (int q = 0; q < Q - 1; ++q) {
-      const int64_t idx = FIRST ? q * NP + i : pull_index<L>(link, q, i, NP);
-      g[q] = Store<S>::get(gin[idx]);
-    }
-    g[Q - 1] = Store<S>::get(gin[(Q - 1) * NP + i]);
+    load_g<L, S, FIRST>(gin, ad, i, NP, g);
     s0 = static_cast<double>(qsumc<L>(g, dir));
   }
   s0 = block_sum(s0);
@@ -191,14 +186,19 @@ __global__ __launch_bounds__(kBlock) void k_flux_lp(const S* __restrict__ gin,
 
 inline dim3 grid_for(int32_t n) { return dim3(static_cast<unsigned>((n + kBlock - 1) / kBlock)); }
 
+// f(lattice tag, storage tag, address layout) for the runtime choices
 template <typename F>
-hipError_t dispatch(int precision, int lattice, F&& f) {
+hipError_t dispatch(int precision, int lattice, const Addr& ad, F&& f) {
+  auto by_addr = [&](auto l, auto st) {
+    if (ad.layout == 1) f(l, st, ad.rows);
+    else f(l, st, ad.table);
+  };
   if (lattice == 0) {
-    if (precision == 1) f(D2Q9{}, float{});
-    else f(D2Q9{}, uint16_t{});
+    if (precision == 1) by_addr(D2Q9{}, float{});
+    else by_addr(D2Q9{}, uint16_t{});
   } else {
-    if (precision == 1) f(D3Q19{}, float{});
-    else f(D3Q19{}, uint16_t{});
+    if (precision == 1) by_addr(D3Q19{}, float{});
+    else by_addr(D3Q19{}, uint16_t{});
   }
   return hipGetLastError();
 }
@@ -208,7 +208,7 @@ hipError_t dispatch(int precision, int lattice, F&& f) {
 hipError_t store_lowp(int precision, int lattice, int nf, const double* f, void* g, int32_t n,
                       int64_t NP, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  return dispatch(precision, lattice, [&](auto l, auto st) {
+  return dispatch(precision, lattice, Addr{}, [&](auto l, auto st, auto) {
     using L = decltype(l);
     using S = decltype(st);
     k_store<L, S><<<grid_for(n), dim3(kBlock), 0, s>>>(f, static_cast<S*>(g), n, NP, nf);
@@ -218,7 +218,7 @@ hipError_t store_lowp(int precision, int lattice, int nf, const double* f, void*
 hipError_t load_lowp(int precision, int lattice, int nf, const void* g, double* f, int32_t n,
                      int64_t NP, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  return dispatch(precision, lattice, [&](auto l, auto st) {
+  return dispatch(precision, lattice, Addr{}, [&](auto l, auto st, auto) {
     using L = decltype(l);
     using S = decltype(st);
     k_load<L, S><<<grid_for(n), dim3(kBlock), 0, s>>>(static_cast<const S*>(g), f, n, NP, nf);
@@ -226,30 +226,33 @@ hipError_t load_lowp(int precision, int lattice, int nf, const void* g, double* 
 }
 
 hipError_t gather_lowp(int precision, int lattice, int nf, const void* gin, void* gout,
-                       const uint32_t* link, int32_t n, int64_t NP, hipStream_t s) {
+                       const Addr& ad, int32_t n, int64_t NP, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  return dispatch(precision, lattice, [&](auto l, auto st) {
+  return dispatch(precision, lattice, ad, [&](auto l, auto st, auto a) {
     using L = decltype(l);
     using S = decltype(st);
-    k_gather_lp<L, S><<<grid_for(n), dim3(kBlock), 0, s>>>(static_cast<const S*>(gin), static_cast<S*>(gout),
-                                                          link, n, NP, nf);
+    using A = decltype(a);
+    k_gather_lp<L, S, A><<<grid_for(n), dim3(kBlock), 0, s>>>(static_cast<const S*>(gin), static_cast<S*>(gout),
+                                                             a, n, NP, nf);
   });
 }
 
 hipError_t bgk_lowp(int precision, int lattice, bool first, bool store, bool reduce,
-                    const void* gin, void* gout, const uint32_t* link, int32_t n, int64_t NP,
+                    const void* gin, void* gout, const Addr& ad, int32_t n, int64_t NP,
                     const BgkArgs& a, double* rho, double* vel, double* partial, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const float ti = static_cast<float>(a.tau_inv), tf = static_cast<float>(a.tau_factor);
   const float F0 = static_cast<float>(a.F[0]), F1 = static_cast<float>(a.F[1]), F2 = static_cast<float>(a.F[2]);
   const int key = (first ? 4 : 0) | (store ? 2 : 0) | (reduce ? 1 : 0);
-  return dispatch(precision, lattice, [&](auto l, auto st) {
+  return dispatch(precision, lattice, ad, [&](auto l, auto st, auto addr) {
     using L = decltype(l);
     using S = decltype(st);
+    using A = decltype(addr);
     const S* gi = static_cast<const S*>(gin);
     S* go = static_cast<S*>(gout);
     const dim3 g = grid_for(n), b(kBlock);
-#define LB_K(F_, S_, R_) k_bgk_lp<L, S, F_, S_, R_><<<g, b, 0, s>>>(gi, go, link, n, NP, ti, tf, F0, F1, F2, rho, vel, partial)
+#define LB_K(F_, S_, R_) \
+  k_bgk_lp<L, S, A, F_, S_, R_><<<g, b, 0, s>>>(gi, go, addr, n, NP, ti, tf, F0, F1, F2, rho, vel, partial)
     switch (key) {
       case 0: LB_K(false, false, false); break;
       case 1: LB_K(false, false, true); break;
@@ -265,16 +268,17 @@ hipError_t bgk_lowp(int precision, int lattice, bool first, bool store, bool red
 }
 
 hipError_t flux_lowp(int precision, int lattice, int nf, bool first, const void* gin,
-                     const uint32_t* link, int32_t n, int64_t NP, int dir, double* partial,
+                     const Addr& ad, int32_t n, int64_t NP, int dir, double* partial,
                      hipStream_t s) {
   if (nf != 1) return hipErrorInvalidValue;
   const dim3 g = grid_for(n > 0 ? n : 1), b(kBlock);
   const int64_t nb = g.x;
-  return dispatch(precision, lattice, [&](auto l, auto st) {
+  return dispatch(precision, lattice, ad, [&](auto l, auto st, auto addr) {
     using L = decltype(l);
     using S = decltype(st);
-    if (first) k_flux_lp<L, S, true><<<g, b, 0, s>>>(static_cast<const S*>(gin), link, n, NP, dir, partial, nb);
-    else k_flux_lp<L, S, false><<<g, b, 0, s>>>(static_cast<const S*>(gin), link, n, NP, dir, partial, nb);
+    using A = decltype(addr);
+    if (first) k_flux_lp<L, S, A, true><<<g, b, 0, s>>>(static_cast<const S*>(gin), addr, n, NP, dir, partial, nb);
+    else k_flux_lp<L, S, A, false><<<g, b, 0, s>>>(static_cast<const S*>(gin), addr, n, NP, dir, partial, nb);
   });
 }
 

@@ -9,6 +9,7 @@
 #include <cstdint>
 
 #include "context.hpp"
+#include "kernels.hpp"
 
 namespace lbamd {
 
@@ -21,12 +22,12 @@ hipError_t store_lowp(int precision, int lattice, int nf, const double* f, void*
 hipError_t load_lowp(int precision, int lattice, int nf, const void* g, double* f, int32_t n,
                      int64_t NP, hipStream_t s);
 hipError_t gather_lowp(int precision, int lattice, int nf, const void* gin, void* gout,
-                       const uint32_t* link, int32_t n, int64_t NP, hipStream_t s);
+                       const Addr& ad, int32_t n, int64_t NP, hipStream_t s);
 hipError_t bgk_lowp(int precision, int lattice, bool first, bool store, bool reduce,
-                    const void* gin, void* gout, const uint32_t* link, int32_t n, int64_t NP,
+                    const void* gin, void* gout, const Addr& ad, int32_t n, int64_t NP,
                     const BgkArgs& a, double* rho, double* vel, double* partial, hipStream_t s);
 hipError_t flux_lowp(int precision, int lattice, int nf, bool first, const void* gin,
-                     const uint32_t* link, int32_t n, int64_t NP, int dir, double* partial,
+                     const Addr& ad, int32_t n, int64_t NP, int dir, double* partial,
                      hipStream_t s);
 
 }  // namespace lbamd

@@ -18,27 +18,28 @@
 namespace lbamd {
 namespace {
 
-template <class L>
-__device__ __forceinline__ int64_t pull_index(const uint32_t* __restrict__ link, int q,
-                                              int32_t i, int64_t NP) {
-  const uint32_t l = __builtin_nontemporal_load(&link[q * NP + i]);
-  const uint32_t kind = l >> 30;
-  const int64_t j = static_cast<int64_t>(l & kLinkIndexMask);
-  const int rq = L::rev(q);
-  return kind == kLinkStream ? q * NP + j : (kind == kLinkBounce ? rq * NP + i : rq * NP + j);
-}
-
-// Load the pre-collision populations of node i for nf fields.
-template <class L, int NF, bool FIRST>
-__device__ __forceinline__ void load_node(const double* __restrict__ fin,
-                                          const uint32_t* __restrict__ link, int32_t i,
+// Load the pre-collision populations of node i for NF fields.
+template <class L, int NF, bool FIRST, class A>
+__device__ __forceinline__ void load_node(const double* __restrict__ fin, const A& ad, int32_t i,
                                           int64_t NP, double (&f)[NF][L::nQ]) {
   const int64_t fstride = static_cast<int64_t>(L::nQ) * NP;
+  if (FIRST) {
 #pragma unroll
-  for (int q = 0; q < L::nQ - 1; ++q) {
-    const int64_t idx = FIRST ? q * NP + i : pull_index<L>(link, q, i, NP);
+    for (int q = 0; q < L::nQ - 1; ++q)
 #pragma unroll
-    for (int k = 0; k < NF; ++k) f[k][q] = fin[k * fstride + idx];
+      for (int k = 0; k < NF; ++k) f[k][q] = fin[k * fstride + q * NP + i];
+  } else {
+    // phase 1: all addressing loads; phase 2: all gathers
+    const typename A::Node nd = ad.node(i);
+    typename A::Pre p[L::nQ - 1];
+#pragma unroll
+    for (int q = 0; q < L::nQ - 1; ++q) p[q] = ad.template pre<L>(q, i, nd);
+#pragma unroll
+    for (int q = 0; q < L::nQ - 1; ++q) {
+      const int64_t idx = ad.template src<L>(q, i, nd, p[q]);
+#pragma unroll
+      for (int k = 0; k < NF; ++k) f[k][q] = fin[k * fstride + idx];
+    }
   }
 #pragma unroll
   for (int k = 0; k < NF; ++k) f[k][L::nQ - 1] = fin[k * fstride + (L::nQ - 1) * NP + i];
@@ -62,19 +63,19 @@ __device__ __forceinline__ double block_sum(double v) {
 }
 
 // ------------------------------------------------------------------ BGK
-template <class L, bool FIRST, bool STORE, bool REDUCE>
+template <class L, class A, bool FIRST, bool STORE, bool REDUCE>
 __global__ __launch_bounds__(kBlock) void k_bgk(const double* __restrict__ fin,
-                                                double* __restrict__ fout,
-                                                const uint32_t* __restrict__ link, int32_t n,
+                                                double* __restrict__ fout, A ad, int32_t n,
                                                 int64_t NP, BgkArgs a, double* __restrict__ rho_o,
                                                 double* __restrict__ vel_o,
                                                 double* __restrict__ partial) {
   constexpr int Q = L::nQ, D = L::nD;
-  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const int32_t b = xcd_block(blockIdx.x, gridDim.x);
+  const int32_t i = b * kBlock + threadIdx.x;
   double jx = 0.0;
   if (i < n) {
     double f[1][Q];
-    load_node<L, 1, FIRST>(fin, link, i, NP, f);
+    load_node<L, 1, FIRST>(fin, ad, i, NP, f);
     // calcRho, calcVel (LBmacroscopic.h:11-45)
     const double rho = qsum<L>(f[0]);
     double u[D];
@@ -104,22 +105,22 @@ __global__ __launch_bounds__(kBlock) void k_bgk(const double* __restrict__ fin,
   }
   if (REDUCE) {
     const double s = block_sum(jx);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    if (threadIdx.x == 0) partial[b] = s;
   }
 }
 
 // ------------------------------------------------------------------ two-phase pass 1
 template <class L, bool FIRST, bool MOMX>
-__global__ __launch_bounds__(kBlock) void k_cg_phi(const double* __restrict__ fin,
-                                                   const uint32_t* __restrict__ link, int32_t n,
-                                                   int64_t NP, double* __restrict__ phi,
+__global__ __launch_bounds__(kBlock) void k_cg_phi(const double* __restrict__ fin, TableAddr ad,
+                                                   int32_t n, int64_t NP, double* __restrict__ phi,
                                                    double* __restrict__ partial) {
   constexpr int Q = L::nQ;
-  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const int32_t b = xcd_block(blockIdx.x, gridDim.x);
+  const int32_t i = b * kBlock + threadIdx.x;
   double mx = 0.0;
   if (i < n) {
     double f[2][Q];
-    load_node<L, 2, FIRST>(fin, link, i, NP, f);
+    load_node<L, 2, FIRST>(fin, ad, i, NP, f);
     const double r0 = qsum<L>(f[0]);
     const double r1 = qsum<L>(f[1]);
     phi[i] = (r0 - r1) / (r0 + r1);
@@ -132,15 +133,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_phi(const double* __restrict__ fi
   }
   if (MOMX) {
     const double s = block_sum(mx);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    if (threadIdx.x == 0) partial[b] = s;
   }
 }
 
 // ------------------------------------------------------------------ two-phase pass 3
 template <class L, bool FIRST, bool STORE>
 __global__ __launch_bounds__(kBlock) void k_cg(const double* __restrict__ fin,
-                                               double* __restrict__ fout,
-                                               const uint32_t* __restrict__ link, int32_t n,
+                                               double* __restrict__ fout, TableAddr ad, int32_t n,
                                                int64_t NP, CgArgs a,
                                                const double* __restrict__ phi,
                                                const double* __restrict__ phi_wall,
@@ -148,10 +148,11 @@ __global__ __launch_bounds__(kBlock) void k_cg(const double* __restrict__ fin,
                                                double* __restrict__ rho_o,
                                                double* __restrict__ vel_o) {
   constexpr int Q = L::nQ, D = L::nD;
-  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const int32_t b = xcd_block(blockIdx.x, gridDim.x);
+  const int32_t i = b * kBlock + threadIdx.x;
   if (i >= n) return;
   double f[2][Q];
-  load_node<L, 2, FIRST>(fin, link, i, NP, f);
+  load_node<L, 2, FIRST>(fin, ad, i, NP, f);
   double F[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) F[d] = a.F[d];
@@ -189,7 +190,7 @@ __global__ __launch_bounds__(kBlock) void k_cg(const double* __restrict__ fin,
   for (int qq = 0; qq < Q - 1; ++qq) {
     // neighbor(qq, n) is the stream source of direction rev(qq)
     const int q = L::rev(qq);
-    const uint32_t l = link[q * NP + i];
+    const uint32_t l = ad.link[q * NP + i];
     const uint32_t kind = l >> 30;
     const uint32_t j = l & kLinkIndexMask;
     pn[qq] = kind == kLinkStream ? phi[j] : (kind == kLinkBounce ? phi_wall[j] : 0.0);
@@ -233,34 +234,33 @@ __global__ __launch_bounds__(kBlock) void k_cg(const double* __restrict__ fin,
 }
 
 // ------------------------------------------------------------------ utilities
-template <class L>
+template <class L, class A>
 __global__ __launch_bounds__(kBlock) void k_gather(const double* __restrict__ fin,
-                                                   double* __restrict__ fout,
-                                                   const uint32_t* __restrict__ link, int32_t n,
+                                                   double* __restrict__ fout, A ad, int32_t n,
                                                    int64_t NP, int nf) {
   constexpr int Q = L::nQ;
   const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const int64_t fstride = static_cast<int64_t>(Q) * NP;
+  const typename A::Node nd = ad.node(i);
   for (int k = 0; k < nf; ++k) {
 #pragma unroll
     for (int q = 0; q < Q - 1; ++q)
-      fout[k * fstride + q * NP + i] = fin[k * fstride + pull_index<L>(link, q, i, NP)];
+      fout[k * fstride + q * NP + i] = fin[k * fstride + ad.template src<L>(q, i, nd)];
     fout[k * fstride + (Q - 1) * NP + i] = fin[k * fstride + (Q - 1) * NP + i];
   }
 }
 
-template <class L, int NF, bool FIRST>
-__global__ __launch_bounds__(kBlock) void k_flux(const double* __restrict__ fin,
-                                                 const uint32_t* __restrict__ link, int32_t n,
-                                                 int64_t NP, int dir,
-                                                 double* __restrict__ partial, int64_t nblocks) {
+template <class L, class A, int NF, bool FIRST>
+__global__ __launch_bounds__(kBlock) void k_flux(const double* __restrict__ fin, A ad, int32_t n,
+                                                 int64_t NP, int dir, double* __restrict__ partial,
+                                                 int64_t nblocks) {
   constexpr int Q = L::nQ;
   const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
   if (i < n) {
     double f[NF][Q];
-    load_node<L, NF, FIRST>(fin, link, i, NP, f);
+    load_node<L, NF, FIRST>(fin, ad, i, NP, f);
     double ft[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) ft[q] = NF == 2 ? f[0][q] + f[NF - 1][q] : f[0][q];
@@ -335,35 +335,45 @@ inline dim3 grid_for(int32_t n) { return dim3(static_cast<unsigned>((n + kBlock 
 }  // namespace
 
 // ------------------------------------------------------------------ launchers
-#define LB_DISPATCH_LATTICE(lat, L, ...)     \
-  do {                                       \
-    if ((lat) == 0) {                        \
-      using L = D2Q9;                        \
-      __VA_ARGS__;                           \
-    } else {                                 \
-      using L = D3Q19;                       \
-      __VA_ARGS__;                           \
-    }                                        \
+#define LB_DISPATCH_LATTICE(lat, L, ...) \
+  do {                                   \
+    if ((lat) == 0) {                    \
+      using L = D2Q9;                    \
+      __VA_ARGS__;                       \
+    } else {                             \
+      using L = D3Q19;                   \
+      __VA_ARGS__;                       \
+    }                                    \
   } while (0)
 
+template <class L, class A>
+static void launch_bgk(bool first, bool store, bool reduce, dim3 g, hipStream_t s, const double* fin,
+                       double* fout, const A& ad, int32_t n, int64_t NP, const BgkArgs& a, double* rho,
+                       double* vel, double* partial) {
+  const dim3 b(kBlock);
+  const int key = (first ? 4 : 0) | (store ? 2 : 0) | (reduce ? 1 : 0);
+#define LB_K(F_, S_, R_) k_bgk<L, A, F_, S_, R_><<<g, b, 0, s>>>(fin, fout, ad, n, NP, a, rho, vel, partial)
+  switch (key) {
+    case 0: LB_K(false, false, false); break;
+    case 1: LB_K(false, false, true); break;
+    case 2: LB_K(false, true, false); break;
+    case 3: LB_K(false, true, true); break;
+    case 4: LB_K(true, false, false); break;
+    case 5: LB_K(true, false, true); break;
+    case 6: LB_K(true, true, false); break;
+    default: LB_K(true, true, true); break;
+  }
+#undef LB_K
+}
+
 hipError_t bgk_f64(int lattice, bool first, bool store, bool reduce, const double* fin,
-                   double* fout, const uint32_t* link, int32_t n, int64_t NP,
-                   const BgkArgs& a, double* rho, double* vel, double* partial,
-                   hipStream_t s) {
+                   double* fout, const Addr& ad, int32_t n, int64_t NP, const BgkArgs& a,
+                   double* rho, double* vel, double* partial, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  const dim3 g = grid_for(n), b(kBlock);
+  const dim3 g = grid_for(n);
   LB_DISPATCH_LATTICE(lattice, L, {
-    const int key = (first ? 4 : 0) | (store ? 2 : 0) | (reduce ? 1 : 0);
-    switch (key) {
-      case 0: k_bgk<L, false, false, false><<<g, b, 0, s>>>(fin, fout, link, n, NP, a, rho, vel, partial); break;
-      case 1: k_bgk<L, false, false, true><<<g, b, 0, s>>>(fin, fout, link, n, NP, a, rho, vel, partial); break;
-      case 2: k_bgk<L, false, true, false><<<g, b, 0, s>>>(fin, fout, link, n, NP, a, rho, vel, partial); break;
-      case 3: k_bgk<L, false, true, true><<<g, b, 0, s>>>(fin, fout, link, n, NP, a, rho, vel, partial); break;
-      case 4: k_bgk<L, true, false, false><<<g, b, 0, s>>>(fin, fout, link, n, NP, a, rho, vel, partial); break;
-      case 5: k_bgk<L, true, false, true><<<g, b, 0, s>>>(fin, fout, link, n, NP, a, rho, vel, partial); break;
-      case 6: k_bgk<L, true, true, false><<<g, b, 0, s>>>(fin, fout, link, n, NP, a, rho, vel, partial); break;
-      default: k_bgk<L, true, true, true><<<g, b, 0, s>>>(fin, fout, link, n, NP, a, rho, vel, partial); break;
-    }
+    if (ad.layout == 1) launch_bgk<L>(first, store, reduce, g, s, fin, fout, ad.rows, n, NP, a, rho, vel, partial);
+    else launch_bgk<L>(first, store, reduce, g, s, fin, fout, ad.table, n, NP, a, rho, vel, partial);
   });
   return hipGetLastError();
 }
@@ -373,11 +383,12 @@ hipError_t cg_phi_f64(int lattice, bool first, bool momx, const double* fin,
                       double* partial, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const dim3 g = grid_for(n), b(kBlock);
+  const TableAddr ad{link, NP};
   LB_DISPATCH_LATTICE(lattice, L, {
-    if (first && momx) k_cg_phi<L, true, true><<<g, b, 0, s>>>(fin, link, n, NP, phi, partial);
-    else if (first) k_cg_phi<L, true, false><<<g, b, 0, s>>>(fin, link, n, NP, phi, partial);
-    else if (momx) k_cg_phi<L, false, true><<<g, b, 0, s>>>(fin, link, n, NP, phi, partial);
-    else k_cg_phi<L, false, false><<<g, b, 0, s>>>(fin, link, n, NP, phi, partial);
+    if (first && momx) k_cg_phi<L, true, true><<<g, b, 0, s>>>(fin, ad, n, NP, phi, partial);
+    else if (first) k_cg_phi<L, true, false><<<g, b, 0, s>>>(fin, ad, n, NP, phi, partial);
+    else if (momx) k_cg_phi<L, false, true><<<g, b, 0, s>>>(fin, ad, n, NP, phi, partial);
+    else k_cg_phi<L, false, false><<<g, b, 0, s>>>(fin, ad, n, NP, phi, partial);
   });
   return hipGetLastError();
 }
@@ -388,37 +399,46 @@ hipError_t cg_f64(int lattice, bool first, bool store, const double* fin, double
                   double* rho, double* vel, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const dim3 g = grid_for(n), b(kBlock);
+  const TableAddr ad{link, NP};
   LB_DISPATCH_LATTICE(lattice, L, {
-    if (first && store) k_cg<L, true, true><<<g, b, 0, s>>>(fin, fout, link, n, NP, a, phi, phi_wall, dscalar, rho, vel);
-    else if (first) k_cg<L, true, false><<<g, b, 0, s>>>(fin, fout, link, n, NP, a, phi, phi_wall, dscalar, rho, vel);
-    else if (store) k_cg<L, false, true><<<g, b, 0, s>>>(fin, fout, link, n, NP, a, phi, phi_wall, dscalar, rho, vel);
-    else k_cg<L, false, false><<<g, b, 0, s>>>(fin, fout, link, n, NP, a, phi, phi_wall, dscalar, rho, vel);
+    if (first && store) k_cg<L, true, true><<<g, b, 0, s>>>(fin, fout, ad, n, NP, a, phi, phi_wall, dscalar, rho, vel);
+    else if (first) k_cg<L, true, false><<<g, b, 0, s>>>(fin, fout, ad, n, NP, a, phi, phi_wall, dscalar, rho, vel);
+    else if (store) k_cg<L, false, true><<<g, b, 0, s>>>(fin, fout, ad, n, NP, a, phi, phi_wall, dscalar, rho, vel);
+    else k_cg<L, false, false><<<g, b, 0, s>>>(fin, fout, ad, n, NP, a, phi, phi_wall, dscalar, rho, vel);
   });
   return hipGetLastError();
 }
 
 hipError_t gather_f64(int lattice, int nfields, const double* fin, double* fout,
-                      const uint32_t* link, int32_t n, int64_t NP, hipStream_t s) {
+                      const Addr& ad, int32_t n, int64_t NP, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   LB_DISPATCH_LATTICE(lattice, L, {
-    k_gather<L><<<grid_for(n), dim3(kBlock), 0, s>>>(fin, fout, link, n, NP, nfields);
+    if (ad.layout == 1) k_gather<L, RowAddr><<<grid_for(n), dim3(kBlock), 0, s>>>(fin, fout, ad.rows, n, NP, nfields);
+    else k_gather<L, TableAddr><<<grid_for(n), dim3(kBlock), 0, s>>>(fin, fout, ad.table, n, NP, nfields);
   });
   return hipGetLastError();
 }
 
-hipError_t flux_f64(int lattice, int nfields, bool first, const double* fin,
-                    const uint32_t* link, int32_t n, int64_t NP, int dir, double* partial,
-                    hipStream_t s) {
-  const dim3 g = grid_for(n > 0 ? n : 1), b(kBlock);
+template <class L, class A>
+static void launch_flux(int nfields, bool first, dim3 g, hipStream_t s, const double* fin, const A& ad,
+                        int32_t n, int64_t NP, int dir, double* partial, int64_t nb) {
+  const dim3 b(kBlock);
+  if (nfields == 2) {
+    if (first) k_flux<L, A, 2, true><<<g, b, 0, s>>>(fin, ad, n, NP, dir, partial, nb);
+    else k_flux<L, A, 2, false><<<g, b, 0, s>>>(fin, ad, n, NP, dir, partial, nb);
+  } else {
+    if (first) k_flux<L, A, 1, true><<<g, b, 0, s>>>(fin, ad, n, NP, dir, partial, nb);
+    else k_flux<L, A, 1, false><<<g, b, 0, s>>>(fin, ad, n, NP, dir, partial, nb);
+  }
+}
+
+hipError_t flux_f64(int lattice, int nfields, bool first, const double* fin, const Addr& ad,
+                    int32_t n, int64_t NP, int dir, double* partial, hipStream_t s) {
+  const dim3 g = grid_for(n > 0 ? n : 1);
   const int64_t nb = g.x;
   LB_DISPATCH_LATTICE(lattice, L, {
-    if (nfields == 2) {
-      if (first) k_flux<L, 2, true><<<g, b, 0, s>>>(fin, link, n, NP, dir, partial, nb);
-      else k_flux<L, 2, false><<<g, b, 0, s>>>(fin, link, n, NP, dir, partial, nb);
-    } else {
-      if (first) k_flux<L, 1, true><<<g, b, 0, s>>>(fin, link, n, NP, dir, partial, nb);
-      else k_flux<L, 1, false><<<g, b, 0, s>>>(fin, link, n, NP, dir, partial, nb);
-    }
+    if (ad.layout == 1) launch_flux<L>(nfields, first, g, s, fin, ad.rows, n, NP, dir, partial, nb);
+    else launch_flux<L>(nfields, first, g, s, fin, ad.table, n, NP, dir, partial, nb);
   });
   return hipGetLastError();
 }

@@ -18,6 +18,7 @@ RHO, PHI = 0, 1
 VEL = 0
 FORCE_FIXED, FORCE_MOMX = 0, 1
 TOPO_ALLOW_GHOST0 = 1
+TOPO_TABLE_ONLY = 2
 Q_OF = {D2Q9: 9, D3Q19: 19}
 ND_OF = {D2Q9: 2, D3Q19: 3}
 LATTICE_NAME = {"D2Q9": D2Q9, "D3Q19": D3Q19}
@@ -44,6 +45,10 @@ SIGNATURES = {
     "lb_ctx_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
     "lb_ctx_destroy": (C.c_int, [_vp]),
     "lb_set_topology": (C.c_int, [_vp, C.c_int32, _i32p, _i16p, _i32p, C.c_int32, C.c_int32]),
+    "lb_set_topology_grid": (C.c_int, [_vp, C.c_int32, _i32p, _i16p, _i32p, C.c_int32, _i32p,
+                                       C.c_int32]),
+    "lb_layout": (C.c_int32, [_vp]),
+    "lb_layout_note": (C.c_char_p, [_vp]),
     "lb_set_global_bulk_count": (C.c_int, [_vp, C.c_int64]),
     "lb_upload_f": (C.c_int, [_vp, C.c_int32, _f64p]),
     "lb_download_f": (C.c_int, [_vp, C.c_int32, _f64p]),
@@ -268,8 +273,30 @@ class Context:
                                           bulk.size, TOPO_ALLOW_GHOST0 if allow_ghost0 else 0))
         self.n_nodes = n_nodes
 
-    def set_topology_from(self, topo, allow_ghost0=False):
-        self.set_topology(topo.neighbors(), topo.node_types(), topo.bulk(), allow_ghost0)
+    def set_topology_grid(self, neighbor, node_type, bulk, pos, allow_ghost0=False, table_only=False):
+        neighbor = np.ascontiguousarray(neighbor, dtype=np.int32)
+        node_type = np.ascontiguousarray(node_type, dtype=np.int16)
+        bulk = np.ascontiguousarray(bulk, dtype=np.int32)
+        pos = np.ascontiguousarray(pos, dtype=np.int32)
+        n_nodes = node_type.shape[0]
+        assert neighbor.size == n_nodes * self.Q and pos.size == n_nodes * self.nD
+        flags = (TOPO_ALLOW_GHOST0 if allow_ghost0 else 0) | (TOPO_TABLE_ONLY if table_only else 0)
+        self._check(lib().lb_set_topology_grid(self._h, n_nodes, neighbor.ravel(), node_type,
+                                               bulk if bulk.size else np.zeros(1, np.int32),
+                                               bulk.size, pos.ravel(), flags))
+        self.n_nodes = n_nodes
+
+    def set_topology_from(self, topo, allow_ghost0=False, table_only=False):
+        self.set_topology_grid(topo.neighbors(), topo.node_types(), topo.bulk(), topo.positions(),
+                               allow_ghost0, table_only)
+
+    @property
+    def layout(self):
+        return lib().lb_layout(self._h)
+
+    @property
+    def layout_note(self):
+        return lib().lb_layout_note(self._h).decode()
 
     def set_global_bulk_count(self, n):
         self._check(lib().lb_set_global_bulk_count(self._h, int(n)))

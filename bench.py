@@ -51,9 +51,15 @@ def rest_state(n_nodes, bulk):
     return oracle.rest_state(1, n_nodes, bulk, np.ones(n_nodes))
 
 
-def gpu_leg(topo, nb, types, bulk, precision, steps, warmup, tau, force):
+def make_ctx(precision, nb, types, bulk, pos, layout):
     ctx = lbamd.Context(lbamd.D3Q19, precision, 0)
-    ctx.set_topology(nb, types, bulk)
+    ctx.set_topology_grid(nb, types, bulk, pos, table_only=(layout == "table"))
+    return ctx
+
+
+def gpu_leg(topo, nb, types, bulk, pos, layout, precision, steps, warmup, tau, force):
+    ctx = make_ctx(precision, nb, types, bulk, pos, layout)
+    got_layout = "rows" if ctx.layout == 1 else "table (%s)" % ctx.layout_note
     w = np.array([2.0 / 36.0] * 3 + [1.0 / 36.0] * 6 + [2.0 / 36.0] * 3 + [1.0 / 36.0] * 6 + [12.0 / 36.0])
     f0 = np.zeros((topo.n_nodes, 19))
     f0[bulk] = w[None, :]
@@ -68,12 +74,12 @@ def gpu_leg(topo, nb, types, bulk, precision, steps, warmup, tau, force):
     t1 = time.perf_counter()
     kms, launches = ctx.kernel_time()
     ctx.close()
-    return t1 - t0, kms / max(launches, 1), launches
+    return t1 - t0, kms / max(launches, 1), got_layout
 
 
-def permeability(nb, types, bulk, n_nodes, precision, tau, force, n_box, tol=1e-7, max_steps=60000):
-    ctx = lbamd.Context(lbamd.D3Q19, precision, 0)
-    ctx.set_topology(nb, types, bulk)
+def permeability(nb, types, bulk, pos, layout, n_nodes, precision, tau, force, n_box, tol=1e-7,
+                 max_steps=60000):
+    ctx = make_ctx(precision, nb, types, bulk, pos, layout)
     w = np.array([2.0 / 36.0] * 3 + [1.0 / 36.0] * 6 + [2.0 / 36.0] * 3 + [1.0 / 36.0] * 6 + [12.0 / 36.0])
     f0 = np.zeros((n_nodes, 19))
     f0[bulk] = w[None, :]
@@ -108,6 +114,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--layout", choices=["rows", "table"], default="rows")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-perm", action="store_true")
     args = ap.parse_args()
@@ -119,16 +126,16 @@ def main():
     tau, force = 1.0, np.array([1e-6, 0.0, 0.0])
     t0 = time.perf_counter()
     geo, info, topo = make_workload(args.size)
-    nb, types, bulk = topo.neighbors(), topo.node_types(), topo.bulk()
+    nb, types, bulk, pos = topo.neighbors(), topo.node_types(), topo.bulk(), topo.positions()
     n_bulk = int(bulk.size)
     log("workload: %d^3, %d bulk nodes, porosity %.4f, set-up %.1f s" % (args.size, n_bulk, info["porosity"], time.perf_counter() - t0))
 
-    t32, k32_ms, n32 = gpu_leg(topo, nb, types, bulk, lbamd.FP32S, args.steps, args.warmup, tau, force)
+    t32, k32_ms, lay32 = gpu_leg(topo, nb, types, bulk, pos, args.layout, lbamd.FP32S, args.steps, args.warmup, tau, force)
     mlups32 = n_bulk * args.steps / t32 / 1e6
-    log("FP32S: %.1f MLUPS, kernel %.3f ms" % (mlups32, k32_ms))
-    t64, k64_ms, n64 = gpu_leg(topo, nb, types, bulk, lbamd.FP64, args.steps, args.warmup, tau, force)
+    log("FP32S: %.1f MLUPS, kernel %.3f ms, layout %s" % (mlups32, k32_ms, lay32))
+    t64, k64_ms, lay64 = gpu_leg(topo, nb, types, bulk, pos, args.layout, lbamd.FP64, args.steps, args.warmup, tau, force)
     mlups64 = n_bulk * args.steps / t64 / 1e6
-    log("FP64 : %.1f MLUPS, kernel %.3f ms" % (mlups64, k64_ms))
+    log("FP64 : %.1f MLUPS, kernel %.3f ms, layout %s" % (mlups64, k64_ms, lay64))
 
     bytes32 = 2 * 19 * 4
     bytes64 = 2 * 19 * 8
@@ -138,9 +145,9 @@ def main():
     perm = None
     if not args.no_perm:
         n_box = float(geo.size)
-        k64, s64, d64 = permeability(nb, types, bulk, topo.n_nodes, lbamd.FP64, tau, force, n_box)
+        k64, s64, d64 = permeability(nb, types, bulk, pos, args.layout, topo.n_nodes, lbamd.FP64, tau, force, n_box)
         log("k FP64 = %.10e after %d steps (%.1f s)" % (k64, s64, d64))
-        k32, s32, d32 = permeability(nb, types, bulk, topo.n_nodes, lbamd.FP32S, tau, force, n_box)
+        k32, s32, d32 = permeability(nb, types, bulk, pos, args.layout, topo.n_nodes, lbamd.FP32S, tau, force, n_box)
         log("k FP32S = %.10e after %d steps (%.1f s)" % (k32, s32, d32))
         perm = {"k_fp64": k64, "steps_fp64": s64, "k_fp32s": k32, "steps_fp32s": s32,
                 "k_rel_err_fp32s": abs(k32 - k64) / abs(k64), "tol_dk_over_k": 1e-7,
@@ -166,7 +173,8 @@ def main():
         "data": "synthetic (seeded sphere pack)",
         "config": {"workload": "D3Q19 BGK+Guo collide-stream, %d^3 sphere pack r=8 porosity %.3f, "
                                "periodic x, walls y/z, tau=1, F=1e-6" % (args.size, info["porosity"]),
-                   "bulk_nodes": n_bulk, "form": "FP32-shifted", "parallelism": "single GPU"},
+                   "bulk_nodes": n_bulk, "form": "FP32-shifted", "layout": lay32,
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_bgk_lp<D3Q19,float>", "avg_kernel_ms": k32_ms,

@@ -56,6 +56,8 @@ enum lb_status {
 /* lb_set_topology flags */
 #define LB_TOPO_ALLOW_GHOST0 1 /* accept fluid links to node 0 and reproduce the
                                   reference's shared-slot result exactly */
+#define LB_TOPO_TABLE_ONLY 2   /* keep the per-link pull table even when the
+                                  row layout would apply */
 
 /* field selectors */
 enum lb_scalar { LB_RHO = 0, LB_PHI = 1 };
@@ -89,6 +91,21 @@ int lb_ctx_destroy(lb_ctx* ctx);
 int lb_set_topology(lb_ctx* ctx, int32_t n_nodes, const int32_t* neighbor,
                     const int16_t* node_type, const int32_t* bulk,
                     int32_t n_bulk, int32_t flags);
+
+/* The same, with the node positions (Grid::pos_, int32[n_nodes*nD],
+ * LBgrid.h:120,215-258).  When the bulk nodes are cells of one box (periodic
+ * along any axes) the single-phase kernels switch to the row layout: a 4-byte
+ * cell coordinate per node and a cached per-row bit mask instead of one
+ * 4-byte link per direction.  The layout is used only after every link it
+ * yields has been checked against the neighbour index; otherwise the table
+ * layout stays (lb_layout_note says why).  Results are identical in both. */
+int lb_set_topology_grid(lb_ctx* ctx, int32_t n_nodes, const int32_t* neighbor,
+                         const int16_t* node_type, const int32_t* bulk,
+                         int32_t n_bulk, const int32_t* pos, int32_t flags);
+
+/* 0 = per-link table, 1 = row layout, -1 = no topology. */
+int32_t lb_layout(const lb_ctx* ctx);
+const char* lb_layout_note(const lb_ctx* ctx);
 
 /* Total bulk-node count over all ranks, the divisor of the flux mean
  * (main_TWOPHASE.cpp:197-199,301).  Defaults to n_bulk. */

@@ -38,6 +38,8 @@ class OracleTopo:
         for r0, pairs in t.processors:
             ranks[pairs[:, 0]] = r0
         self.ranks = ranks
+        self.positions = np.full((n, g.nd), -1, dtype=np.int32)
+        self.positions[1:] = t.points
         self.node_types = oracle.node_types(self.lattice, self.neighbors, attr, ranks, rank - 1)
         self.bulk = oracle.bulk_nodes(self.node_types, ranks, rank - 1)
 

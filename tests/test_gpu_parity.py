@@ -15,9 +15,16 @@ from helpers import OracleTopo, float_round, load_ref, ref_names
 pytestmark = pytest.mark.gpu
 
 
-def gpu_std_case(T, f0, tau, force, nsteps, precision=lbamd.FP64, allow_ghost0=False, chunks=None):
+def gpu_std_case(T, f0, tau, force, nsteps, precision=lbamd.FP64, allow_ghost0=False, chunks=None,
+                 layout="table"):
     ctx = lbamd.Context(T.lattice, precision)
-    ctx.set_topology(T.neighbors, T.node_types, T.bulk, allow_ghost0=allow_ghost0)
+    if layout == "rows":
+        ctx.set_topology_grid(T.neighbors, T.node_types, T.bulk, T.positions, allow_ghost0=allow_ghost0)
+        # the row layout applies unless the node list links to the ghost node
+        assert ctx.layout == (0 if allow_ghost0 else 1), ctx.layout_note
+    else:
+        ctx.set_topology(T.neighbors, T.node_types, T.bulk, allow_ghost0=allow_ghost0)
+        assert ctx.layout == 0
     ctx.upload_f(f0)
     for n in (chunks or [nsteps]):
         ctx.step_bgk(n, tau, np.asarray(force, dtype=float))
@@ -32,14 +39,15 @@ def rest(T, rho_nodes):
     return oracle.rest_state(T.lattice, T.n_nodes, T.bulk, rho_nodes)
 
 
+@pytest.mark.parametrize("layout", ["table", "rows"])
 @pytest.mark.parametrize("name", [n for n in ref_names() if not n.startswith("tp_")])
-def test_std_case_vs_reference_fixture(gpu_available, name):
+def test_std_case_vs_reference_fixture(gpu_available, name, layout):
     d = load_ref(name)
     p = d["params"]
     T = OracleTopo(d["geo"].astype(np.int64), p["lattice"], p["periodic"])
     f0 = rest(T, T.attribute(d["init_rho"]))
     ghost0 = bool((T.neighbors[T.bulk] == 0).any())
-    f, rho, vel = gpu_std_case(T, f0, p["tau"], p["force"], p["steps"] + 1, allow_ghost0=ghost0)
+    f, rho, vel = gpu_std_case(T, f0, p["tau"], p["force"], p["steps"] + 1, allow_ghost0=ghost0, layout=layout)
     D = vel.shape[1]
     if "rho" in d:
         assert np.array_equal(rho[T.bulk], d["rho"])
@@ -51,21 +59,24 @@ def test_std_case_vs_reference_fixture(gpu_available, name):
         assert np.array_equal(vel[T.bulk, 0].reshape(nx, ny)[0], d["velx_profile"])
 
 
+@pytest.mark.parametrize("layout", ["table", "rows"])
 @pytest.mark.parametrize("lattice,shape,periodic,seed", [
     ("D2Q9", (40, 23), "xy", 1),
     ("D2Q9", (33, 17), "x", 2),
     ("D3Q19", (17, 12, 9), "xyz", 3),
     ("D3Q19", (20, 16, 13), "x", 4),
+    ("D3Q19", (40, 9, 70), "xz", 5),
 ])
-def test_std_case_vs_oracle_random(gpu_available, lattice, shape, periodic, seed):
+def test_std_case_vs_oracle_random(gpu_available, lattice, shape, periodic, seed, layout):
     rng = np.random.default_rng(seed)
     geo = (rng.random(shape) > 0.3).astype(np.int64)
-    if periodic == "x":  # walls on the other faces keep the ghost node out
+    # walls on the faces of non-periodic axes keep the ghost node out
+    if "y" not in periodic:
         geo[:, 0] = 0
         geo[:, -1] = 0
-        if len(shape) == 3:
-            geo[:, :, 0] = 0
-            geo[:, :, -1] = 0
+    if len(shape) == 3 and "z" not in periodic:
+        geo[:, :, 0] = 0
+        geo[:, :, -1] = 0
     T = OracleTopo(geo, lattice, periodic)
     rho0 = np.ones(T.n_nodes) + 0.01 * rng.random(T.n_nodes)
     f0 = rest(T, rho0)
@@ -73,7 +84,7 @@ def test_std_case_vs_oracle_random(gpu_available, lattice, shape, periodic, seed
     force = [2e-5, -1e-5, 5e-6][: len(shape)]
     nsteps = 37
     fo, ro, vo = oracle.run_std_case(T.lattice, T.neighbors, T.node_types, T.bulk, f0, 0.83, np.array(force), nsteps)
-    fg, rg, vg = gpu_std_case(T, f0, 0.83, force, nsteps, chunks=[1, 10, 26])
+    fg, rg, vg = gpu_std_case(T, f0, 0.83, force, nsteps, chunks=[1, 10, 26], layout=layout)
     assert np.array_equal(fg[T.bulk], fo[T.bulk])
     assert np.array_equal(rg[T.bulk], ro[T.bulk])
     assert np.array_equal(vg[T.bulk], vo[T.bulk])
