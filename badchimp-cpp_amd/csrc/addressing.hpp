@@ -1,24 +1,41 @@
-// Where a bulk node pulls each population from: two address layouts.
+// Where a bulk node pulls each population from: three address layouts.
+//
+// Every layout provides sources<L>(i, src): the population element index
+// (pidx, context.hpp) of the post-collision value that becomes f_q of node i,
+// for q = 0..Q-2 (the rest direction always reads itself).  A stream link
+// gives (q, j) with j the compact index of the upwind neighbour; a link from
+// a solid gives (rev q, i), the half-way bounce-back.  Each layout first
+// issues all its addressing loads and then does the arithmetic, so that the
+// loads are in flight together.
 //
 // TableAddr  one 32-bit link per (direction, node): kind + index (see
 //            context.hpp).  Works for any node list, including fluid links to
-//            the ghost node 0.  Costs 4 B per link per step.
+//            the ghost node 0.  4 B per link per step.
 //
-// RowAddr    the Cartesian box of the bulk nodes is cut into rows along the
-//            last axis (z in 3-D, y in 2-D).  Per row and 32-cell word it
+// RowAddr    the bulk nodes are cells of a Cartesian box, cut into rows along
+//            the last axis (z in 3-D, y in 2-D).  Per row and 32-cell word it
 //            keeps {base, mask}: the fluid bit mask and the compact index of
-//            the word's first fluid cell.  A node stores only its packed
-//            cell coordinate (4 B); the source of direction q is the cell
-//            r - c_q (periodic axes wrap): if its bit is set the source is
-//            compact index base + popcount(mask below the bit) (stream),
-//            otherwise the node's own reverse population (bounce-back).  The
-//            row table is a few MB and stays in L2 / MALL, so the index
-//            traffic per node and step drops from 72 B (D3Q19) to 4 B.
-//            Rows are ordered in tiles of TX x TY rows, which keeps the 18
-//            neighbour rows of a tile close in the compact order.
+//            the word's first fluid cell.  A node stores only its packed cell
+//            coordinate (4 B); the source cell r - c_q (periodic axes wrap) is
+//            fluid if its bit is set, and then its compact index is base +
+//            popcount(mask below the bit).  The table is a few MB and stays in
+//            L2 / MALL.  One lookup per direction.
 //
-// Both are validated on the host against the neighbour index: RowAddr is
-// switched on only when every link it computes equals the table's.
+// RowWin     as RowAddr for boxes that are not periodic along the row axis:
+//            per row and word w it keeps a 64-bit window of the row's fluid
+//            bits over cells [32w-16, 32w+48) and the compact index of the
+//            window's first fluid cell.  The node's own word then covers the
+//            cells z-1, z, z+1 of every neighbouring row, so the 9 (D3Q19) or
+//            3 (D2Q9) neighbouring rows are one 16-byte load each, and the
+//            three sources in a row follow from one popcount: with b0 the
+//            index of the first fluid cell at or after z, the source at z is
+//            b0, at z-1 it is b0-1, at z+1 it is b0 + bit(z).  The window
+//            table has a halo of rows, so no wrap logic runs per node.
+//
+// Rows are ordered in tiles of 2^lx x 2^ly rows, which keeps the neighbour
+// rows of a tile close in the compact order.  The row layouts are validated on
+// the host against the neighbour index and switched on only when every link
+// they compute equals the table's.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -37,30 +54,25 @@ __device__ __forceinline__ int32_t xcd_block(int32_t b, int32_t nblocks) {
   return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
 }
 
-// Each layout splits the address computation in two phases, so that a
-// kernel first issues all Q-1 loads of addressing data (pre) and only then
-// the Q-1 population gathers (src): both batches are in flight together
-// instead of one dependent pair at a time.
+// Element indices are 32-bit: a context keeps Q * NP * fields below 2^32.
+// kOff32: the layout is only used for population buffers below 4 GiB, so the
+// kernels may address them with 32-bit byte offsets from the buffer base.
 struct TableAddr {
+  static constexpr bool kOff32 = false;
   const uint32_t* __restrict__ link;
-  int64_t NP;
-  struct Node {};
-  using Pre = uint32_t;
-  __device__ __forceinline__ Node node(int32_t) const { return {}; }
+  uint32_t NP;
   template <class L>
-  __device__ __forceinline__ Pre pre(int q, int32_t i, Node) const {
-    return link[q * NP + i];
-  }
-  template <class L>
-  __device__ __forceinline__ int64_t src(int q, int32_t i, Node, Pre l) const {
-    const uint32_t kind = l >> 30;
-    const int64_t j = static_cast<int64_t>(l & kLinkIndexMask);
-    const int rq = L::rev(q);
-    return kind == kLinkStream ? q * NP + j : (kind == kLinkBounce ? rq * NP + i : rq * NP + j);
-  }
-  template <class L>
-  __device__ __forceinline__ int64_t src(int q, int32_t i, Node n) const {
-    return src<L>(q, i, n, pre<L>(q, i, n));
+  __device__ __forceinline__ void sources(int32_t i, uint32_t (&src)[L::nQ - 1]) const {
+    uint32_t l[L::nQ - 1];
+#pragma unroll
+    for (int q = 0; q < L::nQ - 1; ++q) l[q] = link[static_cast<uint64_t>(q) * NP + i];
+#pragma unroll
+    for (int q = 0; q < L::nQ - 1; ++q) {
+      const uint32_t kind = l[q] >> 30;
+      const uint32_t j = l[q] & kLinkIndexMask;
+      const uint32_t rq = L::rev(q);
+      src[q] = kind == kLinkStream ? pidx<L::nQ>(q, j) : (kind == kLinkBounce ? pidx<L::nQ>(rq, i) : pidx<L::nQ>(rq, j));
+    }
   }
 };
 
@@ -74,59 +86,116 @@ struct RowGeom {
     return (((X >> lx) * nTy + (Y >> ly)) << (lx + ly)) | ((X & ((1 << lx) - 1)) << ly) |
            (Y & ((1 << ly) - 1));
   }
-};
-
-struct RowAddr {
-  const uint32_t* __restrict__ coord;  // X << 22 | Y << 12 | Z
-  const uint2* __restrict__ rows;      // {base, mask} per (row, word)
-  int64_t NP;
-  RowGeom g;
-  struct Node {
-    int32_t X, Y, Z;
-  };
-  __device__ __forceinline__ Node node(int32_t i) const {
-    const uint32_t c = coord[i];
-    return {static_cast<int32_t>(c >> 22), static_cast<int32_t>((c >> 12) & 1023u),
-            static_cast<int32_t>(c & 4095u)};
-  }
-  // lattice direction components mapped to (X, Y, Z)
-  template <class L>
-  static __device__ __forceinline__ int cx(int q) { return L::c(q, 0); }
-  template <class L>
-  static __device__ __forceinline__ int cy(int q) { return L::nD == 3 ? L::c(q, 1) : 0; }
-  template <class L>
-  static __device__ __forceinline__ int cz(int q) { return L::nD == 3 ? L::c(q, 2) : L::c(q, 1); }
-
-  // wrap one coordinate; false when it leaves a non-periodic box
-  static __device__ __forceinline__ bool wrap(int32_t& v, int32_t E, bool periodic) {
+  // wrap one coordinate into [0, E); false when it leaves a non-periodic box
+  __host__ __device__ static __forceinline__ bool wrap(int32_t& v, int32_t E, bool periodic) {
     if (v >= 0 && v < E) return true;
     v += v < 0 ? E : -E;
     return periodic;
   }
-  // pre: {base, mask restricted to the source bit} of the source cell's word
-  // (mask 0 when the cell is solid or outside the box)
-  using Pre = uint2;
+};
+
+// lattice direction components mapped to (X, Y, Z)
+template <class L>
+__host__ __device__ __forceinline__ constexpr int cx(int q) { return L::c(q, 0); }
+template <class L>
+__host__ __device__ __forceinline__ constexpr int cy(int q) { return L::nD == 3 ? L::c(q, 1) : 0; }
+template <class L>
+__host__ __device__ __forceinline__ constexpr int cz(int q) { return L::nD == 3 ? L::c(q, 2) : L::c(q, 1); }
+
+struct RowAddr {
+  static constexpr bool kOff32 = false;
+  const uint32_t* __restrict__ coord;  // X << 22 | Y << 12 | Z
+  const uint2* __restrict__ rows;      // {base, mask} per (row, word)
+  uint32_t NP;
+  RowGeom g;
   template <class L>
-  __device__ __forceinline__ Pre pre(int q, int32_t, Node n) const {
-    int32_t X = n.X - cx<L>(q), Y = n.Y - cy<L>(q), Z = n.Z - cz<L>(q);
-    bool inside = true;
-    if (cx<L>(q) != 0) inside = wrap(X, g.E[0], g.periodic & 1) && inside;
-    if (cy<L>(q) != 0) inside = wrap(Y, g.E[1], g.periodic & 2) && inside;
-    if (cz<L>(q) != 0) inside = wrap(Z, g.E[2], g.periodic & 4) && inside;
-    const int64_t slot = inside ? static_cast<int64_t>(g.row(X, Y)) * g.W + (Z >> 5) : 0;
-    const uint2 e = rows[slot];
-    const uint32_t bit = inside ? (1u << (Z & 31)) : 0u;
-    // {compact index if the cell is fluid, fluid flag}
-    return uint2{e.x + __popc(e.y & (bit - 1u)), (e.y & bit) ? 1u : 0u};
-  }
-  template <class L>
-  __device__ __forceinline__ int64_t src(int q, int32_t i, Node, Pre p) const {
-    return p.y ? q * NP + static_cast<int64_t>(p.x) : L::rev(q) * NP + i;
-  }
-  template <class L>
-  __device__ __forceinline__ int64_t src(int q, int32_t i, Node n) const {
-    return src<L>(q, i, n, pre<L>(q, i, n));
+  __device__ __forceinline__ void sources(int32_t i, uint32_t (&src)[L::nQ - 1]) const {
+    const uint32_t c = coord[i];
+    const int32_t X0 = static_cast<int32_t>(c >> 22), Y0 = static_cast<int32_t>((c >> 12) & 1023u),
+                  Z0 = static_cast<int32_t>(c & 4095u);
+    uint2 e[L::nQ - 1];
+    uint32_t bit[L::nQ - 1];
+#pragma unroll
+    for (int q = 0; q < L::nQ - 1; ++q) {
+      int32_t X = X0 - cx<L>(q), Y = Y0 - cy<L>(q), Z = Z0 - cz<L>(q);
+      bool inside = true;
+      if (cx<L>(q) != 0) inside = RowGeom::wrap(X, g.E[0], g.periodic & 1) && inside;
+      if (cy<L>(q) != 0) inside = RowGeom::wrap(Y, g.E[1], g.periodic & 2) && inside;
+      if (cz<L>(q) != 0) inside = RowGeom::wrap(Z, g.E[2], g.periodic & 4) && inside;
+      const int64_t slot = inside ? static_cast<int64_t>(g.row(X, Y)) * g.W + (Z >> 5) : 0;
+      e[q] = rows[slot];
+      bit[q] = inside ? (1u << (Z & 31)) : 0u;
+    }
+#pragma unroll
+    for (int q = 0; q < L::nQ - 1; ++q) {
+      const bool fluid = (e[q].y & bit[q]) != 0u;
+      const uint32_t j = e[q].x + __popc(e[q].y & (bit[q] - 1u));
+      src[q] = fluid ? pidx<L::nQ>(q, j) : pidx<L::nQ>(L::rev(q), i);
+    }
   }
 };
+
+struct RowWin {
+  static constexpr bool kOff32 = true;
+  const uint32_t* __restrict__ coord;  // X << 22 | Y << 12 | Z
+  const uint4* __restrict__ win;       // {base, mask lo, mask hi, 0} per (padded row, word)
+  uint32_t NP;
+  int32_t PY;  // padded rows along Y: E_Y + 2
+  int32_t W;   // words per row
+
+  // The window table is indexed by the halo-padded row (X+1, Y+1): halo rows
+  // of a periodic axis repeat the rows they wrap to, those of a closed axis
+  // are empty (all solid).  The 9 (D3Q19) or 3 (D2Q9) neighbouring rows of a
+  // node are then one vector offset plus a per-direction uniform base.
+  template <class L>
+  __device__ __forceinline__ void sources(int32_t i, uint32_t (&src)[L::nQ - 1]) const {
+    const uint32_t c = coord[i];
+    const uint32_t X = c >> 22, Y = (c >> 12) & 1023u, Z = c & 4095u;
+    const uint32_t w = Z >> 5;
+    const uint32_t t = Z - 32u * w + 16u;  // bit of z in the window, 16..47
+    const uint32_t base = ((X + 1u) * static_cast<uint32_t>(PY) + (Y + 1u)) * static_cast<uint32_t>(W) + w;
+    constexpr int NY = L::nD == 3 ? 3 : 1;
+    uint4 e[3 * NY];
+#pragma unroll
+    for (int k = 0; k < 3 * NY; ++k) {
+      const int dx = k / NY - 1, dy = NY > 1 ? k % NY - 1 : 0;
+      const uint4* __restrict__ wk = win + (dx * PY + dy) * W;
+      e[k] = *reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(wk) + base * 16u);
+    }
+    // per row: index of the first fluid cell at or after z, fluid bits of z-1, z, z+1
+    uint32_t b0[3 * NY];
+    uint32_t bm[3 * NY];
+    const uint64_t below = (1ull << t) - 1ull;
+#pragma unroll
+    for (int k = 0; k < 3 * NY; ++k) {
+      const uint64_t m = (static_cast<uint64_t>(e[k].z) << 32) | e[k].y;
+      b0[k] = e[k].x + static_cast<uint32_t>(__popcll(m & below));
+      bm[k] = static_cast<uint32_t>(m >> (t - 1u)) & 7u;  // bit0: z-1, bit1: z, bit2: z+1
+    }
+#pragma unroll
+    for (int q = 0; q < L::nQ - 1; ++q) {
+      // source cell r - c_q
+      const int k = (1 - cx<L>(q)) * NY + (NY > 1 ? 1 - cy<L>(q) : 0);
+      const int dz = -cz<L>(q);
+      const bool fluid = (bm[k] >> (dz + 1)) & 1u;
+      const uint32_t j = dz == 0 ? b0[k] : (dz < 0 ? b0[k] - 1u : b0[k] + ((bm[k] >> 1) & 1u));
+      src[q] = fluid ? pidx<L::nQ>(q, j) : pidx<L::nQ>(L::rev(q), i);
+    }
+  }
+};
+
+// Element access of a population buffer: 32-bit byte offsets from the base
+// when the layout guarantees the buffer is below 4 GiB (global loads with a
+// scalar base and a 32-bit vector offset), 64-bit addresses otherwise.
+template <bool OFF32, typename T>
+__device__ __forceinline__ T ld_elem(const T* __restrict__ base, uint32_t e) {
+  if (OFF32) return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + e * static_cast<uint32_t>(sizeof(T)));
+  return base[e];
+}
+template <bool OFF32, typename T>
+__device__ __forceinline__ T* elem_ptr(T* __restrict__ base, uint32_t e) {
+  if (OFF32) return reinterpret_cast<T*>(reinterpret_cast<char*>(base) + e * static_cast<uint32_t>(sizeof(T)));
+  return base + e;
+}
 
 }  // namespace lbamd

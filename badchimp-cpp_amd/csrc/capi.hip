@@ -239,6 +239,80 @@ std::pair<bool, int64_t> row_source(int lattice, const RowBuild& rb, int q, int3
   return {false, i};
 }
 
+// Row windows (addressing.hpp, RowWin): for the halo-padded row (X+1, Y+1)
+// and word w, the fluid bits of cells [32w-16, 32w+48) and the compact index
+// of the first fluid cell at or after 32w-16.  Halo rows of a periodic axis
+// repeat the rows they wrap to; those of a closed axis stay empty.  Needs a
+// box that is not periodic along the row axis.
+bool build_windows(const RowBuild& rb, std::vector<uint4>& win, int32_t& PY, std::string& why) {
+  const RowGeom& g = rb.g;
+  if (g.periodic & 4) {
+    why = "periodic along the row axis";
+    return false;
+  }
+  const int32_t PX = g.E[0] + 2;
+  PY = g.E[1] + 2;
+  win.assign(static_cast<size_t>(PX) * PY * g.W, uint4{0u, 0u, 0u, 0u});
+  auto fluid = [&](size_t r, int32_t z) -> bool {
+    if (z < 0 || z >= g.E[2]) return false;
+    return (rb.rows[r * g.W + (z >> 5)].y >> (z & 31)) & 1u;
+  };
+  auto first_at_or_after = [&](size_t r, int32_t z) -> uint32_t {
+    // compact index of the first fluid cell of row r at or after z (or the
+    // index one past the row's last fluid cell)
+    const int32_t zc = std::max(z, 0);
+    if (zc >= 32 * g.W) {
+      const uint2 e = rb.rows[r * g.W + g.W - 1];
+      return e.x + static_cast<uint32_t>(__builtin_popcount(e.y));
+    }
+    const uint2 e = rb.rows[r * g.W + (zc >> 5)];
+    return e.x + static_cast<uint32_t>(__builtin_popcount(e.y & ((1u << (zc & 31)) - 1u)));
+  };
+  for (int32_t px = 0; px < PX; ++px)
+    for (int32_t py = 0; py < PY; ++py) {
+      int32_t X = px - 1, Y = py - 1;
+      bool inside = RowGeom::wrap(X, g.E[0], g.periodic & 1);
+      inside = RowGeom::wrap(Y, g.E[1], g.periodic & 2) && inside;
+      if (!inside) continue;
+      const size_t r = static_cast<size_t>(g.row(X, Y));
+      for (int32_t w = 0; w < g.W; ++w) {
+        const int32_t s0 = 32 * w - 16;
+        uint64_t m = 0;
+        for (int k = 0; k < 64; ++k)
+          if (fluid(r, s0 + k)) m |= 1ull << k;
+        win[(static_cast<size_t>(px) * PY + py) * g.W + w] =
+            uint4{first_at_or_after(r, s0), static_cast<uint32_t>(m), static_cast<uint32_t>(m >> 32), 0u};
+      }
+    }
+  return true;
+}
+
+// Source of direction q for compact node i under the row windows, as RowWin
+// computes it on the device.
+std::pair<bool, int64_t> win_source(int lattice, const RowBuild& rb, const std::vector<uint4>& win, int32_t PY,
+                                    int q, int32_t i) {
+  const int nD = lattice == LB_D2Q9 ? 2 : 3;
+  auto C = [&](int a) -> int {
+    const int d = nD == 3 ? a : (a == 0 ? 0 : (a == 2 ? 1 : -1));
+    if (d < 0) return 0;
+    return lattice == LB_D2Q9 ? D2Q9::c(q, d) : D3Q19::c(q, d);
+  };
+  const RowGeom& g = rb.g;
+  const uint32_t cc = rb.coord[i];
+  const int32_t X = static_cast<int32_t>(cc >> 22), Y = static_cast<int32_t>((cc >> 12) & 1023u),
+                Z = static_cast<int32_t>(cc & 4095u);
+  const int32_t w = Z >> 5;
+  const uint32_t t = static_cast<uint32_t>(Z - 32 * w + 16);
+  const size_t slot = (static_cast<size_t>(X + 1 - C(0)) * PY + (Y + 1 - C(1))) * g.W + w;
+  const uint4 e = win[slot];
+  const uint64_t m = (static_cast<uint64_t>(e.z) << 32) | e.y;
+  const int64_t b0 = static_cast<int64_t>(e.x) + __builtin_popcountll(m & ((1ull << t) - 1ull));
+  const uint32_t bm = static_cast<uint32_t>(m >> (t - 1)) & 7u;
+  const int dz = -C(2);
+  if (!((bm >> (dz + 1)) & 1u)) return {false, i};
+  return {true, dz == 0 ? b0 : (dz < 0 ? b0 - 1 : b0 + ((bm >> 1) & 1u))};
+}
+
 int setup_topology(lb_ctx* c, int32_t n_nodes, const int32_t* neighbor, const int16_t* node_type,
                    const int32_t* bulk, int32_t n_bulk, const int32_t* pos, int32_t flags) {
   if (n_nodes < 1 || n_bulk < 0 || !neighbor || !node_type || (n_bulk > 0 && !bulk))
@@ -274,6 +348,14 @@ int setup_topology(lb_ctx* c, int32_t n_nodes, const int32_t* neighbor, const in
       if (nb[p] == 0) winner[p] = compact[bulk[k]];
   }
   const int64_t NP = ((static_cast<int64_t>(n_bulk) + kBlock - 1) / kBlock) * kBlock + (n_bulk == 0 ? kBlock : 0);
+  // element indices of both fields of a buffer are 32-bit
+  if (static_cast<uint64_t>(Q) * static_cast<uint64_t>(NP) * 2u >= (1ull << 32))
+    return fail(c, LB_ERR_UNSUPPORTED, "%d bulk nodes exceed one context (limit %llu); split the domain", n_bulk,
+                static_cast<unsigned long long>((1ull << 32) / (2u * Q)));
+  // the row windows address one population buffer with 32-bit byte offsets
+  if (rows_ok && static_cast<uint64_t>(Q) * static_cast<uint64_t>(NP) * elem_size(c->precision) >= (1ull << 32))
+    why = "population buffer above 4 GiB";
+  const bool win_allowed = static_cast<uint64_t>(Q) * static_cast<uint64_t>(NP) * elem_size(c->precision) < (1ull << 32);
   std::vector<uint32_t> link(static_cast<size_t>(Q - 1) * NP, 0u);
   std::vector<int32_t> wall_slot(n_nodes, -1);
   std::vector<int32_t> walls;
@@ -314,20 +396,36 @@ int setup_topology(lb_ctx* c, int32_t n_nodes, const int32_t* neighbor, const in
       }
     }
   }
+  // row windows when the row axis is not periodic, checked link by link
+  std::vector<uint4> win;
+  int32_t PY = 0;
+  bool win_ok = rows_ok && win_allowed && build_windows(rb, win, PY, why);
+  for (int32_t i = 0; win_ok && i < n_bulk; ++i)
+    for (int q = 0; q < Q - 1; ++q) {
+      const uint32_t l = link[static_cast<size_t>(q) * NP + i];
+      const auto r = win_source(c->lattice, rb, win, PY, q, i);
+      const bool stream = (l >> 30) == kLinkStream;
+      if (r.first != stream || (stream && r.second != static_cast<int64_t>(l & kLinkIndexMask))) {
+        win_ok = false;
+        why = "row windows disagree with the neighbour index";
+        break;
+      }
+    }
   // device allocation
   hipStreamSynchronize(c->stream);
   free_fields(c);
-  for (void* p : {static_cast<void*>(c->d_coord), static_cast<void*>(c->d_rows)})
+  for (void* p : {static_cast<void*>(c->d_coord), static_cast<void*>(c->d_rows), static_cast<void*>(c->d_win)})
     if (p) hipFree(p);
   c->d_coord = nullptr;
   c->d_rows = nullptr;
+  c->d_win = nullptr;
   c->bytes = 0;
   int rc;
   if ((rc = dev_alloc(c, &c->d_link, link.size())) != LB_OK) return rc;
   LB_HIP(c, hipMemcpy(c->d_link, link.data(), link.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
   c->addr = Addr{};
   c->addr.layout = 0;
-  c->addr.table = TableAddr{c->d_link, NP};
+  c->addr.table = TableAddr{c->d_link, static_cast<uint32_t>(NP)};
   if (rows_ok) {
     std::vector<uint32_t> coord(NP, 0u);
     std::copy(rb.coord.begin(), rb.coord.end(), coord.begin());
@@ -336,9 +434,15 @@ int setup_topology(lb_ctx* c, int32_t n_nodes, const int32_t* neighbor, const in
     LB_HIP(c, hipMemcpy(c->d_coord, coord.data(), coord.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     LB_HIP(c, hipMemcpy(c->d_rows, rb.rows.data(), rb.rows.size() * sizeof(uint2), hipMemcpyHostToDevice));
     c->addr.layout = 1;
-    c->addr.rows = RowAddr{c->d_coord, c->d_rows, NP, rb.g};
+    c->addr.rows = RowAddr{c->d_coord, c->d_rows, static_cast<uint32_t>(NP), rb.g};
+    if (win_ok) {
+      if ((rc = dev_alloc(c, &c->d_win, win.size())) != LB_OK) return rc;
+      LB_HIP(c, hipMemcpy(c->d_win, win.data(), win.size() * sizeof(uint4), hipMemcpyHostToDevice));
+      c->addr.layout = 2;
+      c->addr.win = RowWin{c->d_coord, c->d_win, static_cast<uint32_t>(NP), PY, rb.g.W};
+    }
   }
-  c->layout_note = rows_ok ? "rows" : why;
+  c->layout_note = win_ok ? "row windows" : (rows_ok ? "rows (" + why + ")" : why);
   c->n_blocks = (n_bulk + kBlock - 1) / kBlock;
   if (c->n_blocks == 0) c->n_blocks = 1;
   if ((rc = dev_alloc(c, &c->d_rho, 2 * NP)) != LB_OK) return rc;
@@ -408,7 +512,7 @@ int lb_ctx_destroy(lb_ctx* c) {
   for (void* p : {static_cast<void*>(c->d_link), static_cast<void*>(c->d_rho), static_cast<void*>(c->d_vel),
                   static_cast<void*>(c->d_phi), static_cast<void*>(c->d_phi_wall), static_cast<void*>(c->d_partial),
                   static_cast<void*>(c->d_scalar), static_cast<void*>(c->d_coord),
-                  static_cast<void*>(c->d_rows)})
+                  static_cast<void*>(c->d_rows), static_cast<void*>(c->d_win)})
     if (p) hipFree(p);
   hipEventDestroy(c->ev0);
   hipEventDestroy(c->ev1);
@@ -457,7 +561,7 @@ int lb_upload_f(lb_ctx* c, int32_t nf, const double* f) {
     for (int32_t i = 0; i < c->n_bulk; ++i) {
       const double* src = f + static_cast<size_t>(c->node_of[i]) * nf * Q;
       for (int k = 0; k < nf; ++k)
-        for (int q = 0; q < Q; ++q) h[(static_cast<size_t>(k) * Q + q) * NP + i] = src[k * Q + q];
+        for (int q = 0; q < Q; ++q) h[static_cast<size_t>(k) * Q * NP + pidx_host(Q, q, i)] = src[k * Q + q];
     }
     LB_HIP(c, hipMemcpy(c->d_f[0], h.data(), count * sizeof(double), hipMemcpyHostToDevice));
   } else {
@@ -465,7 +569,7 @@ int lb_upload_f(lb_ctx* c, int32_t nf, const double* f) {
     for (int32_t i = 0; i < c->n_bulk; ++i) {
       const double* src = f + static_cast<size_t>(c->node_of[i]) * nf * Q;
       for (int k = 0; k < nf; ++k)
-        for (int q = 0; q < Q; ++q) h[(static_cast<size_t>(k) * Q + q) * NP + i] = src[k * Q + q];
+        for (int q = 0; q < Q; ++q) h[static_cast<size_t>(k) * Q * NP + pidx_host(Q, q, i)] = src[k * Q + q];
     }
     double* tmp = nullptr;
     LB_HIP(c, hipMalloc(&tmp, count * sizeof(double)));
@@ -508,7 +612,7 @@ int lb_download_f(lb_ctx* c, int32_t nf, double* f) {
   for (int32_t i = 0; i < c->n_bulk; ++i) {
     double* dst = f + static_cast<size_t>(c->node_of[i]) * nf * Q;
     for (int k = 0; k < nf; ++k)
-      for (int q = 0; q < Q; ++q) dst[k * Q + q] = h[(static_cast<size_t>(k) * Q + q) * NP + i];
+      for (int q = 0; q < Q; ++q) dst[k * Q + q] = h[static_cast<size_t>(k) * Q * NP + pidx_host(Q, q, i)];
   }
   return LB_OK;
 }

@@ -27,6 +27,21 @@ constexpr uint32_t kLinkGhost0 = 2u;
 constexpr uint32_t kLinkIndexMask = (1u << 30) - 1u;
 constexpr int kBlock = 256;
 
+// Population element (direction q, node i) of one field: blocks of kPB
+// consecutive nodes keep their Q directions one after another (AoSoA), so a
+// wavefront's loads and stores of one node block fall into one DRAM region
+// instead of Q separate planes.  Field k starts at k * Q * NP (NP is a
+// multiple of kPB).
+constexpr int kPLog = 6;
+constexpr uint32_t kPB = 1u << kPLog;
+template <int Q>
+__host__ __device__ __forceinline__ constexpr uint32_t pidx(uint32_t q, uint32_t i) {
+  return (((i >> kPLog) * static_cast<uint32_t>(Q) + q) << kPLog) | (i & (kPB - 1u));
+}
+inline uint32_t pidx_host(int Q, uint32_t q, uint32_t i) {
+  return (((i >> kPLog) * static_cast<uint32_t>(Q) + q) << kPLog) | (i & (kPB - 1u));
+}
+
 struct BgkArgs {
   double tau_inv;     // 1.0 / tau                (LBcollision.h:40)
   double tau_factor;  // 1 - 0.5 / tau            (LBcollision.h:206)

@@ -27,6 +27,7 @@ struct lb_ctx {
   uint32_t* d_link = nullptr;       // pull table in compact order (both layouts)
   uint32_t* d_coord = nullptr;      // row layout: packed cell coordinates
   uint2* d_rows = nullptr;          // row layout: {base, mask} per (row, word)
+  uint4* d_win = nullptr;           // row windows: {base, mask lo, mask hi, 0} per (row, word)
   lbamd::Addr addr;                 // layout used by the single-phase kernels
   std::string layout_note;          // "rows", or why the table layout is used
 

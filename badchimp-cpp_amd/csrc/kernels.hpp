@@ -11,9 +11,10 @@ namespace lbamd {
 
 // Address layout of a launch (see addressing.hpp).
 struct Addr {
-  int layout = 0;  // 0 table, 1 rows
+  int layout = 0;  // 0 table, 1 rows, 2 row windows
   TableAddr table{};
   RowAddr rows{};
+  RowWin win{};
 };
 
 // ---- FP64 (reference arithmetic, -ffp-contract=off) ----------------------

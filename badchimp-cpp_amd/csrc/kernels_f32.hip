@@ -55,19 +55,18 @@ __device__ __forceinline__ double block_sum(double v) {
 template <class L, typename S, bool FIRST, class A>
 __device__ __forceinline__ void load_g(const S* __restrict__ gin, const A& ad, int32_t i, int64_t NP,
                                        float (&g)[L::nQ]) {
+  constexpr bool O = A::kOff32;
+  const uint32_t np = static_cast<uint32_t>(NP);
   if (FIRST) {
 #pragma unroll
-    for (int q = 0; q < L::nQ - 1; ++q) g[q] = Store<S>::get(gin[q * NP + i]);
+    for (int q = 0; q < L::nQ - 1; ++q) g[q] = Store<S>::get(ld_elem<O>(gin, pidx<L::nQ>(q, i)));
   } else {
-    // phase 1: all addressing loads; phase 2: all gathers
-    const typename A::Node nd = ad.node(i);
-    typename A::Pre p[L::nQ - 1];
+    uint32_t src[L::nQ - 1];
+    ad.template sources<L>(i, src);
 #pragma unroll
-    for (int q = 0; q < L::nQ - 1; ++q) p[q] = ad.template pre<L>(q, i, nd);
-#pragma unroll
-    for (int q = 0; q < L::nQ - 1; ++q) g[q] = Store<S>::get(gin[ad.template src<L>(q, i, nd, p[q])]);
+    for (int q = 0; q < L::nQ - 1; ++q) g[q] = Store<S>::get(ld_elem<O>(gin, src[q]));
   }
-  g[L::nQ - 1] = Store<S>::get(gin[(L::nQ - 1) * NP + i]);
+  g[L::nQ - 1] = Store<S>::get(ld_elem<O>(gin, pidx<L::nQ>(L::nQ - 1, i)));
 }
 
 template <class L, typename S, class A, bool FIRST, bool STORE, bool REDUCE>
@@ -102,19 +101,39 @@ __global__ __launch_bounds__(kBlock) void k_bgk_lp(const S* __restrict__ gin, S*
       for (int d = 0; d < D; ++d) vel_o[d * NP + i] = static_cast<double>(u[d]);
     }
     if (REDUCE) jx = static_cast<double>(jd[0]);
-    const float u2 = dot<L>(u, u);
-    const float uF = dot<L>(u, F);
-    const float c2u2 = 1.5f * u2;
-    const float c2uF = float(L::c2) * uF;
+    // Directions q and rev q share |c_q . u| and |c_q . F|: split geq and the
+    // Guo term into the parts even and odd in c_q and form both at once.
+    //   geq  = w (a0 + 4.5 rho cu^2 +- 3 rho cu),  a0 = drho - 1.5 rho u^2
+    //   dF   = w tf (9 cF cu - 3 uF +- 3 cF)
+    //   g'   = (1 - 1/tau) g + geq/tau + dF
+    const float a0 = drho - 1.5f * rho * dot<L>(u, u);
+    const float m3uF = -3.0f * dot<L>(u, F);
+    const float omt = 1.0f - tau_inv;
+    const float r45 = 4.5f * rho, r3 = 3.0f * rho;
+    const uint32_t np = static_cast<uint32_t>(NP);
+    constexpr bool O = A::kOff32;
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
+    for (int q = 0; q < Q - 1; ++q) {
+      const int rq = L::rev(q);
+      if (rq < q) continue;  // each pair once
       const float w = float(L::w(q));
+      const float wt = w * tau_inv, wf = w * tau_factor;
       const float cu = cdot<L>(q, u);
       const float cF = cdot<L>(q, F);
-      const float geq = w * (drho + rho * (3.0f * cu + 4.5f * cu * cu - c2u2));
-      const float dF = (w * tau_factor) * (3.0f * cF + 9.0f * (cF * cu - c2uF));
-      const float gp = g[q] - tau_inv * (g[q] - geq) + dF;
-      __builtin_nontemporal_store(Store<S>::put(gp), &gout[q * NP + i]);
+      const float ev = a0 + r45 * cu * cu;
+      const float od = r3 * cu;
+      const float fev = 9.0f * cF * cu + m3uF;
+      const float fod = 3.0f * cF;
+      const float gq = omt * g[q] + wt * (ev + od) + wf * (fev + fod);
+      const float gr = omt * g[rq] + wt * (ev - od) + wf * (fev - fod);
+      __builtin_nontemporal_store(Store<S>::put(gq), elem_ptr<O>(gout, pidx<Q>(q, i)));
+      __builtin_nontemporal_store(Store<S>::put(gr), elem_ptr<O>(gout, pidx<Q>(rq, i)));
+    }
+    {
+      constexpr int q = Q - 1;  // rest: c = 0
+      const float w = float(L::w(q));
+      const float gq = omt * g[q] + (w * tau_inv) * a0 + (w * tau_factor) * m3uF;
+      __builtin_nontemporal_store(Store<S>::put(gq), elem_ptr<O>(gout, pidx<Q>(q, i)));
     }
   }
   if (REDUCE) {
@@ -131,7 +150,7 @@ __global__ __launch_bounds__(kBlock) void k_store(const double* __restrict__ f, 
   for (int k = 0; k < nf; ++k)
 #pragma unroll
     for (int q = 0; q < L::nQ; ++q) {
-      const int64_t o = (static_cast<int64_t>(k) * L::nQ + q) * NP + i;
+      const int64_t o = static_cast<int64_t>(k) * L::nQ * NP + pidx<L::nQ>(q, i);
       g[o] = Store<S>::put(static_cast<float>(f[o] - L::w(q)));
     }
 }
@@ -144,7 +163,7 @@ __global__ __launch_bounds__(kBlock) void k_load(const S* __restrict__ g, double
   for (int k = 0; k < nf; ++k)
 #pragma unroll
     for (int q = 0; q < L::nQ; ++q) {
-      const int64_t o = (static_cast<int64_t>(k) * L::nQ + q) * NP + i;
+      const int64_t o = static_cast<int64_t>(k) * L::nQ * NP + pidx<L::nQ>(q, i);
       f[o] = static_cast<double>(Store<S>::get(g[o])) + L::w(q);
     }
 }
@@ -156,11 +175,12 @@ __global__ __launch_bounds__(kBlock) void k_gather_lp(const S* __restrict__ gin,
   const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const int64_t fs = static_cast<int64_t>(Q) * NP;
-  const typename A::Node nd = ad.node(i);
+  uint32_t src[Q - 1];
+  ad.template sources<L>(i, src);
   for (int k = 0; k < nf; ++k) {
 #pragma unroll
-    for (int q = 0; q < Q - 1; ++q) gout[k * fs + q * NP + i] = gin[k * fs + ad.template src<L>(q, i, nd)];
-    gout[k * fs + (Q - 1) * NP + i] = gin[k * fs + (Q - 1) * NP + i];
+    for (int q = 0; q < Q - 1; ++q) gout[k * fs + pidx<Q>(q, i)] = gin[k * fs + src[q]];
+    gout[k * fs + pidx<Q>(Q - 1, i)] = gin[k * fs + pidx<Q>(Q - 1, i)];
   }
 }
 
@@ -190,7 +210,8 @@ inline dim3 grid_for(int32_t n) { return dim3(static_cast<unsigned>((n + kBlock 
 template <typename F>
 hipError_t dispatch(int precision, int lattice, const Addr& ad, F&& f) {
   auto by_addr = [&](auto l, auto st) {
-    if (ad.layout == 1) f(l, st, ad.rows);
+    if (ad.layout == 2) f(l, st, ad.win);
+    else if (ad.layout == 1) f(l, st, ad.rows);
     else f(l, st, ad.table);
   };
   if (lattice == 0) {

@@ -22,27 +22,24 @@ namespace {
 template <class L, int NF, bool FIRST, class A>
 __device__ __forceinline__ void load_node(const double* __restrict__ fin, const A& ad, int32_t i,
                                           int64_t NP, double (&f)[NF][L::nQ]) {
-  const int64_t fstride = static_cast<int64_t>(L::nQ) * NP;
+  constexpr bool O = A::kOff32;
+  const uint32_t np = static_cast<uint32_t>(NP);
+  const uint32_t fstride = static_cast<uint32_t>(L::nQ) * np;
   if (FIRST) {
 #pragma unroll
     for (int q = 0; q < L::nQ - 1; ++q)
 #pragma unroll
-      for (int k = 0; k < NF; ++k) f[k][q] = fin[k * fstride + q * NP + i];
+      for (int k = 0; k < NF; ++k) f[k][q] = ld_elem<O>(fin, k * fstride + pidx<L::nQ>(q, i));
   } else {
-    // phase 1: all addressing loads; phase 2: all gathers
-    const typename A::Node nd = ad.node(i);
-    typename A::Pre p[L::nQ - 1];
+    uint32_t src[L::nQ - 1];
+    ad.template sources<L>(i, src);
 #pragma unroll
-    for (int q = 0; q < L::nQ - 1; ++q) p[q] = ad.template pre<L>(q, i, nd);
+    for (int q = 0; q < L::nQ - 1; ++q)
 #pragma unroll
-    for (int q = 0; q < L::nQ - 1; ++q) {
-      const int64_t idx = ad.template src<L>(q, i, nd, p[q]);
-#pragma unroll
-      for (int k = 0; k < NF; ++k) f[k][q] = fin[k * fstride + idx];
-    }
+      for (int k = 0; k < NF; ++k) f[k][q] = ld_elem<O>(fin, k * fstride + src[q]);
   }
 #pragma unroll
-  for (int k = 0; k < NF; ++k) f[k][L::nQ - 1] = fin[k * fstride + (L::nQ - 1) * NP + i];
+  for (int k = 0; k < NF; ++k) f[k][L::nQ - 1] = ld_elem<O>(fin, k * fstride + pidx<L::nQ>(L::nQ - 1, i));
 }
 
 // Deterministic block sum of 256 values; result valid in thread 0.
@@ -100,7 +97,8 @@ __global__ __launch_bounds__(kBlock) void k_bgk(const double* __restrict__ fin,
       const double om = (-a.tau_inv) * (f[0][q] - feq);
       // calcDeltaOmegaF (LBcollision.h:206-211)
       const double dF = (L::w(q) * a.tau_factor) * (L::c2Inv * cF + L::c4Inv * (cF * cu - c2uF));
-      __builtin_nontemporal_store((f[0][q] + om) + dF, &fout[q * NP + i]);
+      __builtin_nontemporal_store((f[0][q] + om) + dF,
+                                  elem_ptr<A::kOff32>(fout, pidx<Q>(q, i)));
     }
   }
   if (REDUCE) {
@@ -228,8 +226,8 @@ __global__ __launch_bounds__(kBlock) void k_cg(const double* __restrict__ fin,
       rc = 0.0;
     }
     const double base = ((ft[q] + om) + dF) + st;
-    __builtin_nontemporal_store(((c0 * base) + rc) + dQ0, &fout[q * NP + i]);
-    __builtin_nontemporal_store(((c1 * base) - rc) + dQ1, &fout[fstride + q * NP + i]);
+    __builtin_nontemporal_store(((c0 * base) + rc) + dQ0, &fout[pidx<Q>(q, i)]);
+    __builtin_nontemporal_store(((c1 * base) - rc) + dQ1, &fout[fstride + pidx<Q>(q, i)]);
   }
 }
 
@@ -242,12 +240,12 @@ __global__ __launch_bounds__(kBlock) void k_gather(const double* __restrict__ fi
   const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const int64_t fstride = static_cast<int64_t>(Q) * NP;
-  const typename A::Node nd = ad.node(i);
+  uint32_t src[Q - 1];
+  ad.template sources<L>(i, src);
   for (int k = 0; k < nf; ++k) {
 #pragma unroll
-    for (int q = 0; q < Q - 1; ++q)
-      fout[k * fstride + q * NP + i] = fin[k * fstride + ad.template src<L>(q, i, nd)];
-    fout[k * fstride + (Q - 1) * NP + i] = fin[k * fstride + (Q - 1) * NP + i];
+    for (int q = 0; q < Q - 1; ++q) fout[k * fstride + pidx<Q>(q, i)] = fin[k * fstride + src[q]];
+    fout[k * fstride + pidx<Q>(Q - 1, i)] = fin[k * fstride + pidx<Q>(Q - 1, i)];
   }
 }
 
@@ -301,7 +299,7 @@ __global__ __launch_bounds__(kBlock) void k_init_eq(double* __restrict__ f, int 
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const double cu = cdot<L>(q, u);
-    f[base + q * NP + i] = (L::w(q) * r) * ((1.0 + L::c2Inv * cu) + L::c4Inv0_5 * (cu * cu - L::c2 * uu));
+    f[base + pidx<Q>(q, i)] = (L::w(q) * r) * ((1.0 + L::c2Inv * cu) + L::c4Inv0_5 * (cu * cu - L::c2 * uu));
   }
 }
 
@@ -372,7 +370,8 @@ hipError_t bgk_f64(int lattice, bool first, bool store, bool reduce, const doubl
   if (n <= 0) return hipSuccess;
   const dim3 g = grid_for(n);
   LB_DISPATCH_LATTICE(lattice, L, {
-    if (ad.layout == 1) launch_bgk<L>(first, store, reduce, g, s, fin, fout, ad.rows, n, NP, a, rho, vel, partial);
+    if (ad.layout == 2) launch_bgk<L>(first, store, reduce, g, s, fin, fout, ad.win, n, NP, a, rho, vel, partial);
+    else if (ad.layout == 1) launch_bgk<L>(first, store, reduce, g, s, fin, fout, ad.rows, n, NP, a, rho, vel, partial);
     else launch_bgk<L>(first, store, reduce, g, s, fin, fout, ad.table, n, NP, a, rho, vel, partial);
   });
   return hipGetLastError();
@@ -383,7 +382,7 @@ hipError_t cg_phi_f64(int lattice, bool first, bool momx, const double* fin,
                       double* partial, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const dim3 g = grid_for(n), b(kBlock);
-  const TableAddr ad{link, NP};
+  const TableAddr ad{link, static_cast<uint32_t>(NP)};
   LB_DISPATCH_LATTICE(lattice, L, {
     if (first && momx) k_cg_phi<L, true, true><<<g, b, 0, s>>>(fin, ad, n, NP, phi, partial);
     else if (first) k_cg_phi<L, true, false><<<g, b, 0, s>>>(fin, ad, n, NP, phi, partial);
@@ -399,7 +398,7 @@ hipError_t cg_f64(int lattice, bool first, bool store, const double* fin, double
                   double* rho, double* vel, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const dim3 g = grid_for(n), b(kBlock);
-  const TableAddr ad{link, NP};
+  const TableAddr ad{link, static_cast<uint32_t>(NP)};
   LB_DISPATCH_LATTICE(lattice, L, {
     if (first && store) k_cg<L, true, true><<<g, b, 0, s>>>(fin, fout, ad, n, NP, a, phi, phi_wall, dscalar, rho, vel);
     else if (first) k_cg<L, true, false><<<g, b, 0, s>>>(fin, fout, ad, n, NP, a, phi, phi_wall, dscalar, rho, vel);
@@ -413,7 +412,8 @@ hipError_t gather_f64(int lattice, int nfields, const double* fin, double* fout,
                       const Addr& ad, int32_t n, int64_t NP, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   LB_DISPATCH_LATTICE(lattice, L, {
-    if (ad.layout == 1) k_gather<L, RowAddr><<<grid_for(n), dim3(kBlock), 0, s>>>(fin, fout, ad.rows, n, NP, nfields);
+    if (ad.layout == 2) k_gather<L, RowWin><<<grid_for(n), dim3(kBlock), 0, s>>>(fin, fout, ad.win, n, NP, nfields);
+    else if (ad.layout == 1) k_gather<L, RowAddr><<<grid_for(n), dim3(kBlock), 0, s>>>(fin, fout, ad.rows, n, NP, nfields);
     else k_gather<L, TableAddr><<<grid_for(n), dim3(kBlock), 0, s>>>(fin, fout, ad.table, n, NP, nfields);
   });
   return hipGetLastError();
@@ -437,7 +437,8 @@ hipError_t flux_f64(int lattice, int nfields, bool first, const double* fin, con
   const dim3 g = grid_for(n > 0 ? n : 1);
   const int64_t nb = g.x;
   LB_DISPATCH_LATTICE(lattice, L, {
-    if (ad.layout == 1) launch_flux<L>(nfields, first, g, s, fin, ad.rows, n, NP, dir, partial, nb);
+    if (ad.layout == 2) launch_flux<L>(nfields, first, g, s, fin, ad.win, n, NP, dir, partial, nb);
+    else if (ad.layout == 1) launch_flux<L>(nfields, first, g, s, fin, ad.rows, n, NP, dir, partial, nb);
     else launch_flux<L>(nfields, first, g, s, fin, ad.table, n, NP, dir, partial, nb);
   });
   return hipGetLastError();

@@ -68,6 +68,7 @@ SIGNATURES = {
     "lb_kernel_time": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "lb_kernel_time_reset": (C.c_int, [_vp]),
     "lb_device_bytes": (C.c_int64, [_vp]),
+    "lb_bench_copy": (C.c_int, [C.c_int, C.c_int64, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "lb_topo_read_vtklb": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(_vp)]),
     "lb_topo_from_geometry": (C.c_int, [C.c_int, C.c_int32, _i32p, _i32p, C.c_int32, C.c_int32,
                                         C.POINTER(_vp)]),
@@ -130,6 +131,13 @@ def sphere_pack(shape, radius, porosity, seed, periodic="x", walls=True, max_fai
          "min_center_distance": float(info[2]), "trials": int(info[3]), "radius": float(radius),
          "overlap_allowed": bool(info[2] < 2 * radius - 1e-12)}
     return geo.reshape(tuple(int(x) for x in dims)), d
+
+
+def bench_copy(nbytes, iters=20, device=0):
+    """Plain device copy: (read+write GB/s, ms per copy)."""
+    g, m = C.c_double(), C.c_double()
+    _check_global(lib().lb_bench_copy(device, int(nbytes), int(iters), C.byref(g), C.byref(m)))
+    return g.value, m.value
 
 
 class Topology:

@@ -59,7 +59,9 @@ def make_ctx(precision, nb, types, bulk, pos, layout):
 
 def gpu_leg(topo, nb, types, bulk, pos, layout, precision, steps, warmup, tau, force):
     ctx = make_ctx(precision, nb, types, bulk, pos, layout)
-    got_layout = "rows" if ctx.layout == 1 else "table (%s)" % ctx.layout_note
+    got_layout = {2: "row windows", 1: "rows"}.get(ctx.layout, "table")
+    if ctx.layout_note and ctx.layout_note != got_layout:
+        got_layout += " (%s)" % ctx.layout_note
     w = np.array([2.0 / 36.0] * 3 + [1.0 / 36.0] * 6 + [2.0 / 36.0] * 3 + [1.0 / 36.0] * 6 + [12.0 / 36.0])
     f0 = np.zeros((topo.n_nodes, 19))
     f0[bulk] = w[None, :]
@@ -137,6 +139,8 @@ def main():
     mlups64 = n_bulk * args.steps / t64 / 1e6
     log("FP64 : %.1f MLUPS, kernel %.3f ms, layout %s" % (mlups64, k64_ms, lay64))
 
+    copy_gbs, _ = lbamd.bench_copy(1 << 30, 20)
+    log("plain copy: %.1f GB/s (read + write)" % copy_gbs)
     bytes32 = 2 * 19 * 4
     bytes64 = 2 * 19 * 8
     ach32 = n_bulk * bytes32 / (k32_ms * 1e-3) / 1e9
@@ -178,7 +182,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_bgk_lp<D3Q19,float>", "avg_kernel_ms": k32_ms,
-                     "bytes_per_lup": bytes32},
+                     "bytes_per_lup": bytes32, "plain_copy_GBs": round(copy_gbs, 1)},
         "fp64": {"value": round(mlups64, 2), "unit": "MLUPS", "avg_kernel_ms": k64_ms,
                  "achieved_GBs": round(ach64, 1), "frac": round(ach64 / HBM_PEAK_GBS, 4),
                  "bytes_per_lup": bytes64},

@@ -180,6 +180,10 @@ int lb_kernel_time(lb_ctx* ctx, double* ms, int64_t* launches);
 int lb_kernel_time_reset(lb_ctx* ctx);
 /* Bytes of device memory the context holds. */
 int64_t lb_device_bytes(const lb_ctx* ctx);
+/* Calibration (no counterpart in the reference): a plain device copy of
+ * `bytes` bytes repeated `iters` times; returns read+write GB/s and the time
+ * per copy.  Used to scale bandwidth figures and HBM counter readings. */
+int lb_bench_copy(int device, int64_t bytes, int32_t iters, double* gbs, double* ms_per_iter);
 
 /* ------------------------------------------------------------ topology host side */
 /* .vtklb reader (LBvtk<DXQY>, LBvtk.h:221-614) followed by Grid

@@ -20,8 +20,13 @@ def gpu_std_case(T, f0, tau, force, nsteps, precision=lbamd.FP64, allow_ghost0=F
     ctx = lbamd.Context(T.lattice, precision)
     if layout == "rows":
         ctx.set_topology_grid(T.neighbors, T.node_types, T.bulk, T.positions, allow_ghost0=allow_ghost0)
-        # the row layout applies unless the node list links to the ghost node
-        assert ctx.layout == (0 if allow_ghost0 else 1), ctx.layout_note
+        # a row layout applies unless the node list links to the ghost node:
+        # row windows (2) when the row axis is not periodic, else rows (1)
+        if allow_ghost0:
+            assert ctx.layout == 0
+        else:
+            assert ctx.layout == (1 if T.geometry.periodic.endswith("z") or
+                                  (T.lattice == 0 and "y" in T.geometry.periodic) else 2), ctx.layout_note
     else:
         ctx.set_topology(T.neighbors, T.node_types, T.bulk, allow_ghost0=allow_ghost0)
         assert ctx.layout == 0
