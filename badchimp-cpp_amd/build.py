@@ -5,7 +5,11 @@
 Produces badchimp-cpp_amd/lib/libbadchimp_amd.so from csrc/.  The FP64
 kernels are compiled with -ffp-contract=off (no multiply-add fusion) so that
 their arithmetic matches the reference's double arithmetic operation by
-operation; every HIP file is compiled without packed FP32 instructions.
+operation.  The reduced-precision kernels fuse multiply-adds only within one
+expression (-ffp-contract=on), so that every instantiation of a kernel (with
+or without the optional stores and reductions) rounds the same way and a run
+does not depend on how its steps are split into calls.  Every HIP file is
+compiled without packed FP32 instructions.
 """
 import concurrent.futures as cf
 import os
@@ -24,7 +28,7 @@ HIP = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-pack
        "-munsafe-fp-atomics"]
 SOURCES = {
     "kernels_f64.hip": HIP + ["-O3", "-ffp-contract=off"],
-    "kernels_f32.hip": HIP + ["-O3"],
+    "kernels_f32.hip": HIP + ["-O3", "-ffp-contract=on"],
     "capi.hip": HIP + ["-O2", "-ffp-contract=off"],
     "calibrate.hip": HIP + ["-O3"],
     "topology.cpp": ["-O2", "-ffp-contract=off", "-x", "c++"],

@@ -681,7 +681,13 @@ int lb_download_scalar(lb_ctx* c, int32_t which, int32_t nf, double* s) {
     if (c->n_fields != 2) return fail(c, LB_ERR_STATE, "phi exists only for two-phase fields");
     std::vector<double> h(NP);
     LB_HIP(c, hipMemcpy(h.data(), c->d_phi, NP * sizeof(double), hipMemcpyDeviceToHost));
-    for (int32_t i = 0; i < c->n_bulk; ++i) s[c->node_of[i]] = h[i];
+    if (c->precision == LB_FP64) {
+      for (int32_t i = 0; i < c->n_bulk; ++i) s[c->node_of[i]] = h[i];
+    } else {
+      // float phi values in the reduced-precision forms
+      const float* hf = reinterpret_cast<const float*>(h.data());
+      for (int32_t i = 0; i < c->n_bulk; ++i) s[c->node_of[i]] = static_cast<double>(hf[i]);
+    }
   } else {
     return fail(c, LB_ERR_ARG, "unknown scalar %d", which);
   }
@@ -762,7 +768,6 @@ int lb_step_cg(lb_ctx* c, int32_t nsteps, double tau0, double tau1, double sigma
   if (nsteps < 0 || !force || !(tau0 > 0.5) || !(tau1 > 0.5) ||
       (force_mode != LB_FORCE_FIXED && force_mode != LB_FORCE_MOMX))
     return fail(c, LB_ERR_ARG, "lb_step_cg: bad argument");
-  if (c->precision != LB_FP64) return fail(c, LB_ERR_UNSUPPORTED, "two-phase steps run in FP64 only in this build");
   if (nsteps == 0) return LB_OK;
   hipSetDevice(c->device);
   CgArgs a{};
@@ -781,15 +786,28 @@ int lb_step_cg(lb_ctx* c, int32_t nsteps, double tau0, double tau1, double sigma
   int64_t launches = 0;
   for (int32_t s = 0; s < nsteps; ++s) {
     const bool last = s == nsteps - 1;
-    const double* fin = static_cast<const double*>(c->d_f[c->cur]);
-    double* fout = static_cast<double*>(c->d_f[1 - c->cur]);
-    hipError_t e = cg_phi_f64(c->lattice, !c->post, momx, fin, c->d_link, c->n_bulk, c->NP, c->d_phi,
-                              c->d_partial, c->stream);
-    if (e == hipSuccess && momx)
-      e = finish_momx(c->d_partial, c->n_blocks, a.momx, c->n_global, c->d_scalar, c->stream);
-    if (e == hipSuccess)
-      e = cg_f64(c->lattice, !c->post, last, fin, fout, c->d_link, c->n_bulk, c->NP, a, c->d_phi,
-                 c->d_phi_wall, c->d_scalar, c->d_rho, c->d_vel, c->stream);
+    hipError_t e;
+    if (c->precision == LB_FP64) {
+      const double* fin = static_cast<const double*>(c->d_f[c->cur]);
+      double* fout = static_cast<double*>(c->d_f[1 - c->cur]);
+      e = cg_phi_f64(c->lattice, !c->post, momx, fin, c->d_link, c->n_bulk, c->NP, c->d_phi, c->d_partial,
+                     c->stream);
+      if (e == hipSuccess && momx)
+        e = finish_momx(c->d_partial, c->n_blocks, a.momx, c->n_global, c->d_scalar, c->stream);
+      if (e == hipSuccess)
+        e = cg_f64(c->lattice, !c->post, last, fin, fout, c->d_link, c->n_bulk, c->NP, a, c->d_phi,
+                   c->d_phi_wall, c->d_scalar, c->d_rho, c->d_vel, c->stream);
+    } else {
+      // the phi buffer holds float values in the reduced-precision forms
+      float* phi = reinterpret_cast<float*>(c->d_phi);
+      e = cg_phi_lowp(c->precision, c->lattice, !c->post, momx, c->d_f[c->cur], c->d_link, c->n_bulk, c->NP,
+                      phi, c->d_partial, c->stream);
+      if (e == hipSuccess && momx)
+        e = finish_momx(c->d_partial, c->n_blocks, a.momx, c->n_global, c->d_scalar, c->stream);
+      if (e == hipSuccess)
+        e = cg_lowp(c->precision, c->lattice, !c->post, last, c->d_f[c->cur], c->d_f[1 - c->cur], c->d_link,
+                    c->n_bulk, c->NP, a, phi, c->d_phi_wall, c->d_scalar, c->d_rho, c->d_vel, c->stream);
+    }
     if (e != hipSuccess) return fail(c, LB_ERR_HIP, "two-phase launch: %s", hipGetErrorString(e));
     launches += momx ? 3 : 2;
     c->cur = 1 - c->cur;

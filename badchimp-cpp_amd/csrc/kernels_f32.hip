@@ -142,17 +142,28 @@ __global__ __launch_bounds__(kBlock) void k_bgk_lp(const S* __restrict__ gin, S*
   }
 }
 
+// Conversion from and to FP64 populations.  One field: g = f - w.  Two fields
+// (the two-phase populations f0, f1) are kept as the shifted total and the
+// colour difference,
+//   t = f0 + f1 - w,   d = f0 - f1,
+// so that the flow, which lives in the total, keeps the shifted precision and
+// the colour keeps the full relative precision of d.
 template <class L, typename S>
 __global__ __launch_bounds__(kBlock) void k_store(const double* __restrict__ f, S* __restrict__ g,
                                                   int32_t n, int64_t NP, int nf) {
   const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  for (int k = 0; k < nf; ++k)
+  const int64_t fs = static_cast<int64_t>(L::nQ) * NP;
 #pragma unroll
-    for (int q = 0; q < L::nQ; ++q) {
-      const int64_t o = static_cast<int64_t>(k) * L::nQ * NP + pidx<L::nQ>(q, i);
+  for (int q = 0; q < L::nQ; ++q) {
+    const int64_t o = pidx<L::nQ>(q, i);
+    if (nf == 2) {
+      g[o] = Store<S>::put(static_cast<float>((f[o] + f[fs + o]) - L::w(q)));
+      g[fs + o] = Store<S>::put(static_cast<float>(f[o] - f[fs + o]));
+    } else {
       g[o] = Store<S>::put(static_cast<float>(f[o] - L::w(q)));
     }
+  }
 }
 
 template <class L, typename S>
@@ -160,12 +171,163 @@ __global__ __launch_bounds__(kBlock) void k_load(const S* __restrict__ g, double
                                                  int32_t n, int64_t NP, int nf) {
   const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  for (int k = 0; k < nf; ++k)
+  const int64_t fs = static_cast<int64_t>(L::nQ) * NP;
 #pragma unroll
-    for (int q = 0; q < L::nQ; ++q) {
-      const int64_t o = static_cast<int64_t>(k) * L::nQ * NP + pidx<L::nQ>(q, i);
-      f[o] = static_cast<double>(Store<S>::get(g[o])) + L::w(q);
+  for (int q = 0; q < L::nQ; ++q) {
+    const int64_t o = pidx<L::nQ>(q, i);
+    const double t = static_cast<double>(Store<S>::get(g[o])) + L::w(q);
+    if (nf == 2) {
+      const double d = static_cast<double>(Store<S>::get(g[fs + o]));
+      f[o] = 0.5 * (t + d);
+      f[fs + o] = 0.5 * (t - d);
+    } else {
+      f[o] = t;
     }
+  }
+}
+
+// Both planes (t, d) of a two-phase node, one address computation.
+template <class L, typename S, bool FIRST>
+__device__ __forceinline__ void load_td(const S* __restrict__ gin, uint32_t fs, const TableAddr& ad, int32_t i,
+                                        float (&t)[L::nQ], float (&d)[L::nQ]) {
+  if (FIRST) {
+#pragma unroll
+    for (int q = 0; q < L::nQ - 1; ++q) {
+      t[q] = Store<S>::get(gin[pidx<L::nQ>(q, i)]);
+      d[q] = Store<S>::get(gin[fs + pidx<L::nQ>(q, i)]);
+    }
+  } else {
+    uint32_t src[L::nQ - 1];
+    ad.template sources<L>(i, src);
+#pragma unroll
+    for (int q = 0; q < L::nQ - 1; ++q) {
+      t[q] = Store<S>::get(gin[src[q]]);
+      d[q] = Store<S>::get(gin[fs + src[q]]);
+    }
+  }
+  t[L::nQ - 1] = Store<S>::get(gin[pidx<L::nQ>(L::nQ - 1, i)]);
+  d[L::nQ - 1] = Store<S>::get(gin[fs + pidx<L::nQ>(L::nQ - 1, i)]);
+}
+
+// Two-phase pass 1 on (t, d): phi = (r0 - r1) / (r0 + r1) = sum d / (1 + sum t)
+// (main_TWOPHASE.cpp:246-262), and in the momentum-controlled mode the block
+// sums of sum_q (f0 + f1) c_x = sum_q t c_x in FP64 (main_TWOPHASE.cpp:292-308).
+template <class L, typename S, bool FIRST, bool MOMX>
+__global__ __launch_bounds__(kBlock) void k_cg_phi_lp(const S* __restrict__ gin, TableAddr ad, int32_t n,
+                                                      int64_t NP, float* __restrict__ phi,
+                                                      double* __restrict__ partial) {
+  constexpr int Q = L::nQ;
+  const int32_t b = xcd_block(blockIdx.x, gridDim.x);
+  const int32_t i = b * kBlock + threadIdx.x;
+  double mx = 0.0;
+  if (i < n) {
+    float t[Q], d[Q];
+    load_td<L, S, FIRST>(gin, static_cast<uint32_t>(Q * NP), ad, i, t, d);
+    float st = 0.0f, sd = 0.0f;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      st += t[q];
+      sd += d[q];
+    }
+    phi[i] = sd / (1.0f + st);
+    if (MOMX) mx = static_cast<double>(qsumc<L>(t, 0));
+  }
+  if (MOMX) {
+    const double s = block_sum(mx);
+    if (threadIdx.x == 0) partial[b] = s;
+  }
+}
+
+// Two-phase pass 3 on (t, d) (main_TWOPHASE.cpp:318-392): BGK with the mixed
+// relaxation time and the Guo force on the total, the surface-tension
+// perturbation (LBcollision2phase.h:7-20) and the recolouring
+// (LBcollision2phase.h:70-86).  With base_q = f_tot,q + Omega_q + dOmegaF_q +
+// dOmegaST_q and c_k = rho_k / rho,
+//   f0' = c0 base + RC,  f1' = c1 base - RC
+// give t' = base - w and d' = phi (base) + 2 RC with phi = c0 - c1.
+template <class L, typename S, bool FIRST, bool STORE>
+__global__ __launch_bounds__(kBlock) void k_cg_lp(const S* __restrict__ gin, S* __restrict__ gout, TableAddr ad,
+                                                  int32_t n, int64_t NP, float nu0Inv, float nu1Inv,
+                                                  float sigma, float beta, float F0, float F1, float F2,
+                                                  int32_t momx_mode, const float* __restrict__ phi,
+                                                  const double* __restrict__ phi_wall,
+                                                  const double* __restrict__ dscalar,
+                                                  double* __restrict__ rho_o, double* __restrict__ vel_o) {
+  constexpr int Q = L::nQ, D = L::nD;
+  const int32_t b = xcd_block(blockIdx.x, gridDim.x);
+  const int32_t i = b * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t fs = static_cast<uint32_t>(Q * NP);
+  float t[Q], dd[Q];
+  load_td<L, S, FIRST>(gin, fs, ad, i, t, dd);
+  float F[3] = {F0, F1, F2};
+  if (momx_mode) F[0] = static_cast<float>(dscalar[0]);
+  float drho = 0.0f, sd = 0.0f;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    drho += t[q];
+    sd += dd[q];
+  }
+  const float rho = 1.0f + drho;
+  const float rinv = 1.0f / rho;
+  const float r0 = 0.5f * (rho + sd), r1 = 0.5f * (rho - sd);
+  const float phi_n = sd * rinv;
+  float u[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) u[d] = (qsumc<L>(t, d) + 0.5f * F[d]) * rinv;
+  if (STORE) {
+    rho_o[i] = static_cast<double>(r0);
+    rho_o[NP + i] = static_cast<double>(r1);
+#pragma unroll
+    for (int d = 0; d < D; ++d) vel_o[d * NP + i] = static_cast<double>(u[d]);
+  }
+  const float tau = (3.0f * rho) / (r0 * nu0Inv + r1 * nu1Inv) + 0.5f;
+  const float ti = 1.0f / tau;
+  const float tf = 1.0f - 0.5f * ti;
+  // colour gradient from the neighbours' phi (LButilities.h:12-22)
+  float pn[Q];
+#pragma unroll
+  for (int qq = 0; qq < Q - 1; ++qq) {
+    const int q = L::rev(qq);  // neighbor(qq, n) is the stream source of rev(qq)
+    const uint32_t l = ad.link[static_cast<uint32_t>(q) * ad.NP + i];
+    const uint32_t kind = l >> 30;
+    const uint32_t j = l & kLinkIndexMask;
+    pn[qq] = kind == kLinkStream ? phi[j] : (kind == kLinkBounce ? static_cast<float>(phi_wall[j]) : 0.0f);
+  }
+  pn[Q - 1] = 0.0f;
+  float g[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) g[d] = grad<L>(pn, d);
+  const float cgn = sqrtf(dot<L>(g, g));
+  const float ginv = 1.0f / (cgn + (cgn < 2.220446049250313e-16f ? 1.0f : 0.0f));
+#pragma unroll
+  for (int d = 0; d < D; ++d) g[d] = g[d] * ginv;
+  const float af = 1.125f * cgn * sigma * ti;
+  const float rfb = beta * r0 * r1 * rinv;
+  const float a0 = drho - 1.5f * rho * dot<L>(u, u);
+  const float m3uF = -3.0f * dot<L>(u, F);
+  const float omt = 1.0f - ti;
+  const float r45 = 4.5f * rho, r3 = 3.0f * rho;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const float w = float(L::w(q));
+    const float cu = cdot<L>(q, u);
+    const float cF = cdot<L>(q, F);
+    const float cg = cdot<L>(q, g);
+    const float geq = w * (a0 + r45 * cu * cu + r3 * cu);
+    const float dF = (w * tf) * (9.0f * cF * cu + m3uF + 3.0f * cF);
+    float st, rc;
+    if (q < Q - 1) {
+      st = af * (w * cg * cg - float(L::B(q)));
+      rc = rfb * w * cg * float(1.0 / L::cNorm(q));
+    } else {
+      st = -af * float(L::B(q));
+      rc = 0.0f;
+    }
+    const float bs = omt * t[q] + ti * geq + dF + st;
+    __builtin_nontemporal_store(Store<S>::put(bs), &gout[pidx<Q>(q, i)]);
+    __builtin_nontemporal_store(Store<S>::put(phi_n * (w + bs) + 2.0f * rc), &gout[fs + pidx<Q>(q, i)]);
+  }
 }
 
 template <class L, typename S, class A>
@@ -201,6 +363,40 @@ __global__ __launch_bounds__(kBlock) void k_flux_lp(const S* __restrict__ gin, A
     partial[blockIdx.x] = s0;
     partial[nblocks + blockIdx.x] = 0.0;
     partial[2 * nblocks + blockIdx.x] = 0.0;
+  }
+}
+
+// Two-phase flux sums on (t, d): sum_q (f0 + f1) c = sum_q t c, and the phase
+// shares r_k / rho of it (LBglobalforcing.h:61-78).
+template <class L, typename S, bool FIRST>
+__global__ __launch_bounds__(kBlock) void k_flux_td(const S* __restrict__ gin, TableAddr ad, int32_t n,
+                                                    int64_t NP, int dir, double* __restrict__ partial,
+                                                    int64_t nblocks) {
+  constexpr int Q = L::nQ;
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  if (i < n) {
+    float t[Q], d[Q];
+    load_td<L, S, FIRST>(gin, static_cast<uint32_t>(Q * NP), ad, i, t, d);
+    float st = 0.0f, sd = 0.0f;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      st += t[q];
+      sd += d[q];
+    }
+    const float rho = 1.0f + st;
+    const double fc = static_cast<double>(qsumc<L>(t, dir));
+    s0 = fc;
+    s1 = static_cast<double>(0.5f * (rho + sd) / rho) * fc;
+    s2 = static_cast<double>(0.5f * (rho - sd) / rho) * fc;
+  }
+  s0 = block_sum(s0);
+  s1 = block_sum(s1);
+  s2 = block_sum(s2);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = s0;
+    partial[nblocks + blockIdx.x] = s1;
+    partial[2 * nblocks + blockIdx.x] = s2;
   }
 }
 
@@ -291,15 +487,74 @@ hipError_t bgk_lowp(int precision, int lattice, bool first, bool store, bool red
 hipError_t flux_lowp(int precision, int lattice, int nf, bool first, const void* gin,
                      const Addr& ad, int32_t n, int64_t NP, int dir, double* partial,
                      hipStream_t s) {
-  if (nf != 1) return hipErrorInvalidValue;
+  if (nf != 1 && nf != 2) return hipErrorInvalidValue;
   const dim3 g = grid_for(n > 0 ? n : 1), b(kBlock);
   const int64_t nb = g.x;
+  if (nf == 2) {
+    return dispatch(precision, lattice, Addr{}, [&](auto l, auto st, auto) {
+      using L = decltype(l);
+      using S = decltype(st);
+      const S* gi = static_cast<const S*>(gin);
+      if (first) k_flux_td<L, S, true><<<g, b, 0, s>>>(gi, ad.table, n, NP, dir, partial, nb);
+      else k_flux_td<L, S, false><<<g, b, 0, s>>>(gi, ad.table, n, NP, dir, partial, nb);
+    });
+  }
   return dispatch(precision, lattice, ad, [&](auto l, auto st, auto addr) {
     using L = decltype(l);
     using S = decltype(st);
     using A = decltype(addr);
     if (first) k_flux_lp<L, S, A, true><<<g, b, 0, s>>>(static_cast<const S*>(gin), addr, n, NP, dir, partial, nb);
     else k_flux_lp<L, S, A, false><<<g, b, 0, s>>>(static_cast<const S*>(gin), addr, n, NP, dir, partial, nb);
+  });
+}
+
+hipError_t cg_phi_lowp(int precision, int lattice, bool first, bool momx, const void* gin,
+                       const uint32_t* link, int32_t n, int64_t NP, float* phi, double* partial,
+                       hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const TableAddr ta{link, static_cast<uint32_t>(NP)};
+  return dispatch(precision, lattice, Addr{}, [&](auto l, auto st, auto) {
+    using L = decltype(l);
+    using S = decltype(st);
+    const S* gi = static_cast<const S*>(gin);
+    const dim3 g = grid_for(n), b(kBlock);
+    if (first) {
+      if (momx) k_cg_phi_lp<L, S, true, true><<<g, b, 0, s>>>(gi, ta, n, NP, phi, partial);
+      else k_cg_phi_lp<L, S, true, false><<<g, b, 0, s>>>(gi, ta, n, NP, phi, partial);
+    } else {
+      if (momx) k_cg_phi_lp<L, S, false, true><<<g, b, 0, s>>>(gi, ta, n, NP, phi, partial);
+      else k_cg_phi_lp<L, S, false, false><<<g, b, 0, s>>>(gi, ta, n, NP, phi, partial);
+    }
+  });
+}
+
+hipError_t cg_lowp(int precision, int lattice, bool first, bool store, const void* gin, void* gout,
+                   const uint32_t* link, int32_t n, int64_t NP, const CgArgs& a, const float* phi,
+                   const double* phi_wall, const double* dscalar, double* rho, double* vel,
+                   hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (a.q0 != 0.0 || a.q1 != 0.0) return hipErrorInvalidValue;  // no mass sources in this form
+  const TableAddr ta{link, static_cast<uint32_t>(NP)};
+  const float n0 = static_cast<float>(a.nu0Inv), n1 = static_cast<float>(a.nu1Inv);
+  const float sg = static_cast<float>(a.sigma), be = static_cast<float>(a.beta);
+  const float F0 = static_cast<float>(a.F[0]), F1 = static_cast<float>(a.F[1]), F2 = static_cast<float>(a.F[2]);
+  return dispatch(precision, lattice, Addr{}, [&](auto l, auto st, auto) {
+    using L = decltype(l);
+    using S = decltype(st);
+    const S* gi = static_cast<const S*>(gin);
+    S* go = static_cast<S*>(gout);
+    const dim3 g = grid_for(n), b(kBlock);
+#define LB_K(F_, S_)                                                                                       \
+  k_cg_lp<L, S, F_, S_><<<g, b, 0, s>>>(gi, go, ta, n, NP, n0, n1, sg, be, F0, F1, F2, a.momx_mode, phi, \
+                                        phi_wall, dscalar, rho, vel)
+    if (first) {
+      if (store) LB_K(true, true);
+      else LB_K(true, false);
+    } else {
+      if (store) LB_K(false, true);
+      else LB_K(false, false);
+    }
+#undef LB_K
   });
 }
 

@@ -29,5 +29,14 @@ hipError_t bgk_lowp(int precision, int lattice, bool first, bool store, bool red
 hipError_t flux_lowp(int precision, int lattice, int nf, bool first, const void* gin,
                      const Addr& ad, int32_t n, int64_t NP, int dir, double* partial,
                      hipStream_t s);
+// Two-phase populations are stored as (t, d) = (f0 + f1 - w, f0 - f1) and use
+// the link table.
+hipError_t cg_phi_lowp(int precision, int lattice, bool first, bool momx, const void* gin,
+                       const uint32_t* link, int32_t n, int64_t NP, float* phi, double* partial,
+                       hipStream_t s);
+hipError_t cg_lowp(int precision, int lattice, bool first, bool store, const void* gin, void* gout,
+                   const uint32_t* link, int32_t n, int64_t NP, const CgArgs& a, const float* phi,
+                   const double* phi_wall, const double* dscalar, double* rho, double* vel,
+                   hipStream_t s);
 
 }  // namespace lbamd

@@ -36,8 +36,24 @@ METRIC = "D3Q19 MLUPS & %HBM roofline; batched perm solves/s + p99 at ≤1e-4 k 
 HBM_PEAK_GBS = 8000.0
 
 
+PROFILE = os.path.join(ROOT, "profiles", "k_bgk_d3q19_256_pack.md")
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def profiled_traffic(form):
+    """HBM bytes per collide-stream launch from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs (copy-calibrated) of this workload."""
+    try:
+        for line in open(PROFILE):
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == form:
+                return int(parts[2])
+    except OSError:
+        pass
+    return None
 
 
 def make_workload(size, seed=1):
@@ -146,6 +162,10 @@ def main():
     ach32 = n_bulk * bytes32 / (k32_ms * 1e-3) / 1e9
     ach64 = n_bulk * bytes64 / (k64_ms * 1e-3) / 1e9
 
+    # counters were taken on the 256^3 workload only
+    traffic32 = profiled_traffic("FP32S") if args.size == 256 else None
+    traffic64 = profiled_traffic("FP64") if args.size == 256 else None
+
     perm = None
     if not args.no_perm:
         n_box = float(geo.size)
@@ -180,12 +200,12 @@ def main():
                    "bulk_nodes": n_bulk, "form": "FP32-shifted", "layout": lay32,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": traffic32,
                      "kernel": "k_bgk_lp<D3Q19,float>", "avg_kernel_ms": k32_ms,
                      "bytes_per_lup": bytes32, "plain_copy_GBs": round(copy_gbs, 1)},
         "fp64": {"value": round(mlups64, 2), "unit": "MLUPS", "avg_kernel_ms": k64_ms,
                  "achieved_GBs": round(ach64, 1), "frac": round(ach64 / HBM_PEAK_GBS, 4),
-                 "bytes_per_lup": bytes64},
+                 "bytes_per_lup": bytes64, "traffic": traffic64},
         "permeability": perm,
         "cpu_baseline": cpu,
     }

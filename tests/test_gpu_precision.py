@@ -4,6 +4,9 @@ float arithmetic, FP64 reductions) against the FP64 oracle.
 Tolerances (stated per quantity, norm-wise: max |a - b| / max |b|):
   FP32S  velocity 1e-4, density deviation 1e-3 (absolute 1e-9), permeability 1e-4
   FP16S  velocity 2e-2, permeability 2e-2
+two-phase (phase field absolute, velocity norm-wise, fluxes relative):
+  FP32S  phi 1e-4, velocity 1e-4, fluxes 1e-5
+  FP16S  phi 1e-2, velocity 2e-2, fluxes 2e-3
 and, for every layout, the same precision gives bit-identical results
 (the per-node arithmetic does not depend on how sources are addressed).
 """
@@ -100,3 +103,91 @@ def test_permeability_matches_oracle_flux(gpu_available):
     jx = (ro[T.bulk] * vo[T.bulk, 0]).sum()
     k_ref = (0.5 / 3.0) * jx / geo.size / F[0]
     assert abs(k - k_ref) <= 1e-12 * abs(k_ref)
+
+
+def tp_case(shape, seed, sat=0.5, wett=1.0):
+    """Sphere pack with phases drawn per pore node (Bernoulli(sat)), solid
+    boundary wettability `wett` (rho0 = wett, rho1 = 1 - wett on type-1 nodes)."""
+    geo, T = pack(shape, 3.0, 0.5, seed)
+    rng = np.random.default_rng(seed)
+    r0 = np.zeros(T.n_nodes)
+    r0[T.bulk] = (rng.random(T.bulk.size) < sat).astype(float)
+    r1 = np.zeros(T.n_nodes)
+    r1[T.bulk] = 1.0 - r0[T.bulk]
+    sb = T.node_types == 1
+    r0[sb] = wett
+    r1[sb] = 1.0 - wett
+    W = oracle.WEIGHTS[1]
+    f0 = np.zeros((T.n_nodes, 2, 19))
+    f0[T.bulk, 0] = W[None] * r0[T.bulk, None]
+    f0[T.bulk, 1] = W[None] * r1[T.bulk, None]
+    return T, f0, np.stack([r0, r1], 1)
+
+
+@pytest.mark.parametrize("precision,ptol,vtol,qtol", [(lbamd.FP32S, 1e-4, 1e-4, 1e-5),
+                                                      (lbamd.FP16S, 1e-2, 2e-2, 2e-3)])
+def test_twophase_reduced_precision_vs_oracle(gpu_available, precision, ptol, vtol, qtol):
+    """Colour-gradient steps on (f0 + f1 - w, f0 - f1) in reduced precision
+    against the FP64 oracle: phase field (absolute), velocity (norm-wise) and
+    the total and per-phase fluxes (relative)."""
+    T, f0, rho = tp_case((24, 20, 18), 13)
+    tau0, tau1, sigma, beta, force, steps = 0.8, 1.2, 0.01, 0.7, [1e-5, 0.0, 0.0], 200
+    fo, ro, vo, cgo, Fo = oracle.run_twophase(1, T.neighbors, T.node_types, T.bulk, f0, rho, tau0, tau1,
+                                              sigma, beta, 0, force, steps)
+    ctx = lbamd.Context(T.lattice, precision)
+    ctx.set_topology(T.neighbors, T.node_types, T.bulk)
+    ctx.upload_f(f0)
+    ctx.set_wall_densities(rho)
+    ctx.step_cg(steps, tau0, tau1, sigma, beta, lbamd.FORCE_FIXED, force)
+    phi, v = ctx.download_scalar(lbamd.PHI), ctx.download_vel()
+    q = np.array(ctx.reduce_flux(0))
+    ctx.close()
+    qo = oracle.flux(1, T.bulk, fo, 2, 0)
+    b = T.bulk
+    ep, ev, eq = np.abs(phi[b] - cgo[b]).max(), normerr(v[b], vo[b]), np.abs(q - qo).max() / np.abs(qo).max()
+    print("phi %.3e  vel %.3e  flux %.3e" % (ep, ev, eq))
+    assert ep < ptol and ev < vtol and eq < qtol
+
+
+@pytest.mark.parametrize("precision", [lbamd.FP32S, lbamd.FP16S])
+def test_twophase_roundtrip_and_chunking(gpu_available, precision):
+    """Upload/download of (f0, f1) through the (t, d) storage is exact to the
+    storage precision, and stepping in chunks equals stepping at once."""
+    T, f0, rho = tp_case((16, 14, 12), 3)
+    ctx = lbamd.Context(T.lattice, precision)
+    ctx.set_topology(T.neighbors, T.node_types, T.bulk)
+    ctx.upload_f(f0)
+    back = ctx.download_f(2)
+    tol = 1e-7 if precision == lbamd.FP32S else 2e-4
+    assert np.abs(back[T.bulk] - f0[T.bulk]).max() < tol
+    ctx.set_wall_densities(rho)
+    outs = []
+    for chunks in ([30], [1, 9, 20]):
+        ctx.upload_f(f0)
+        for n in chunks:
+            ctx.step_cg(n, 1.0, 1.0, 0.01, 0.7, lbamd.FORCE_MOMX, [1e-5, 0.0, 0.0])
+        outs.append((ctx.download_f(2), ctx.download_scalar(lbamd.PHI), ctx.last_force()))
+    ctx.close()
+    assert np.array_equal(outs[0][0][T.bulk], outs[1][0][T.bulk])
+    assert np.array_equal(outs[0][1][T.bulk], outs[1][1][T.bulk])
+    assert np.array_equal(outs[0][2], outs[1][2])
+
+
+@pytest.mark.parametrize("precision", [lbamd.FP32S, lbamd.FP16S])
+def test_bgk_chunking_is_exact(gpu_available, precision):
+    """Reduced-precision single-phase steps give the same bits however the
+    steps are split into calls (the stored-moment variant of the kernel
+    rounds like the plain one)."""
+    geo, T = pack((20, 18, 16), 3.0, 0.5, 8)
+    f0 = oracle.rest_state(1, T.n_nodes, T.bulk, np.ones(T.n_nodes))
+    outs = []
+    for chunks in ([40], [1, 19, 20]):
+        ctx = lbamd.Context(T.lattice, precision)
+        ctx.set_topology_grid(T.neighbors, T.node_types, T.bulk, T.positions)
+        ctx.upload_f(f0)
+        for n in chunks:
+            ctx.step_bgk(n, 0.9, np.array([2e-5, 1e-6, 0.0]))
+        outs.append((ctx.download_f(1), ctx.download_vel()))
+        ctx.close()
+    assert np.array_equal(outs[0][0][T.bulk], outs[1][0][T.bulk])
+    assert np.array_equal(outs[0][1][T.bulk], outs[1][1][T.bulk])
