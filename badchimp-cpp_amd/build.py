@@ -30,6 +30,7 @@ SOURCES = {
     "kernels_f64.hip": HIP + ["-O3", "-ffp-contract=off"],
     "kernels_f32.hip": HIP + ["-O3", "-ffp-contract=on"],
     "capi.hip": HIP + ["-O2", "-ffp-contract=off"],
+    "batch.hip": HIP + ["-O2", "-ffp-contract=off"],
     "calibrate.hip": HIP + ["-O3"],
     "topology.cpp": ["-O2", "-ffp-contract=off", "-x", "c++"],
 }

@@ -1,0 +1,634 @@
+// Batched and serving form: many independent permeability solves resident in
+// one device pool, advanced together.
+//
+// A solve is a geometry (cells > 0 fluid), a box with periodic axes, and the
+// std_case parameters (tau, body force).  It is admitted when its footprint
+// fits into the pool: two population buffers, coordinates, the row-window
+// table and the block partials, carved from one pre-allocated device region by
+// a first-fit allocator.  Every step is one launch over all resident solves
+// (k_batch_bgk / k_batch_bgk_lp); every check_every steps the last launch also
+// writes block partials of j_x, one small launch sums them per solve, the host
+// reads one double per solve, evaluates k = nu <j_x> / F_x and retires the
+// solves whose |dk| <= tol |k| (the stop of lb_solve_permeability).  Retired
+// solves free their memory; waiting requests can then be admitted, at a check
+// boundary, so every solve sees whole check intervals.
+//
+// The step launches of one check interval are captured in a HIP graph and
+// replayed while the resident set is unchanged.  The launch shape (workgroups
+// per CU of the persistent launch, or one workgroup per 256-node block) is
+// chosen by timing the candidates on live steps after each change of the
+// resident set; the shape does not change any result, so tuning costs nothing
+// but time.
+//
+// Each solve's per-node arithmetic, block partition and reduction order are
+// those of a single context with row windows (lb_solve_permeability), so a
+// solve gives the same k and step count alone, in any mix, and through the
+// single-solve path.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/badchimp_amd.h"
+#include "errors.hpp"
+#include "kernels.hpp"
+#include "kernels_f32.hpp"
+#include "lattice.hpp"
+#include "rowbuild.hpp"
+
+using namespace lbamd;
+
+namespace {
+
+constexpr int kMaxSlots = 1 << (32 - kItemShift);  // 16384
+constexpr int64_t kAlign = 256;
+
+int64_t align_up(int64_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+size_t elem_bytes(int precision) { return precision == LB_FP64 ? 8 : (precision == LB_FP32S ? 4 : 2); }
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// First-fit allocator over [0, size) in kAlign units, coalescing on free.
+struct RangeAlloc {
+  std::map<int64_t, int64_t> free_;  // offset -> size
+  int64_t total = 0, used = 0;
+  void init(int64_t size) {
+    free_.clear();
+    total = size;
+    used = 0;
+    if (size > 0) free_[0] = size;
+  }
+  int64_t alloc(int64_t size) {
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < size) continue;
+      const int64_t off = it->first, rest = it->second - size;
+      free_.erase(it);
+      if (rest > 0) free_[off + size] = rest;
+      used += size;
+      return off;
+    }
+    return -1;
+  }
+  void release(int64_t off, int64_t size) {
+    used -= size;
+    auto it = free_.emplace(off, size).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  int64_t largest() const {
+    int64_t m = 0;
+    for (auto& kv : free_) m = std::max(m, kv.second);
+    return m;
+  }
+};
+
+// Sizes of one solve's device region.
+struct Footprint {
+  int64_t n = 0, NP = 0, nblk = 0, nwin = 0;
+  int64_t off_f0 = 0, off_f1 = 0, off_coord = 0, off_win = 0, off_part = 0, bytes = 0;
+};
+
+Footprint footprint_of(int Q, size_t esz, int64_t n, int64_t nwin) {
+  Footprint fp;
+  fp.n = n;
+  fp.NP = std::max<int64_t>((n + kBlock - 1) / kBlock, 1) * kBlock;
+  fp.nblk = std::max<int64_t>((n + kBlock - 1) / kBlock, 1);
+  fp.nwin = nwin;
+  const int64_t fb = align_up(static_cast<int64_t>(Q) * fp.NP * static_cast<int64_t>(esz));
+  fp.off_f0 = 0;
+  fp.off_f1 = fb;
+  fp.off_coord = 2 * fb;
+  fp.off_win = fp.off_coord + align_up(fp.NP * 4);
+  fp.off_part = fp.off_win + align_up(nwin * 16);
+  fp.bytes = fp.off_part + align_up(fp.nblk * 8);
+  return fp;
+}
+
+struct Slot {
+  bool used = false;
+  int64_t tag = 0;
+  int64_t off = 0;
+  Footprint fp;
+  double tau = 1.0, F0 = 0.0, nu = 0.0, n_box = 1.0, tol = 0.0;
+  int64_t max_steps = 0, steps = 0;
+  double k = 0.0, k_prev = 0.0;
+  bool have_prev = false;
+  double t_admit = 0.0;
+};
+
+struct Result {
+  int64_t tag, steps;
+  double k, seconds;
+  int32_t status;  // 0 converged, 1 max_steps reached
+};
+
+}  // namespace
+
+struct lb_batch {
+  int lattice = 0, precision = 0, device = 0, Q = 0, nD = 0;
+  size_t esz = 0;
+  int32_t check_every = 100;
+  std::string err;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int n_cu = 256;
+  // pool
+  char* pool = nullptr;
+  RangeAlloc alloc;
+  // solves
+  std::vector<Slot> slots;
+  BatchSolve* d_solves = nullptr;
+  uint32_t* d_items = nullptr;
+  int64_t items_cap = 0;
+  int32_t* d_slots = nullptr;
+  double* d_out = nullptr;
+  double* h_out = nullptr;  // pinned
+  std::vector<uint32_t> items;
+  std::vector<int32_t> active;
+  bool dirty = true;
+  int64_t global_step = 0;
+  int64_t resident_nodes = 0;
+  // launch shape: wg_per_cu > 0 persistent, 0 one workgroup per item
+  int32_t shape_mode = -1;  // -1 autotune, else fixed
+  int32_t shape = 4;
+  int64_t tuned_nodes = -1;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  // results and counters
+  std::vector<Result> done;
+  double kernel_ms = 0.0;
+  int64_t launches = 0, lups = 0;
+};
+
+namespace {
+
+int bfail(lb_batch* b, int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  b->err = vformat(fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define LB_BHIP(b, call)                                                         \
+  do {                                                                           \
+    hipError_t e_ = (call);                                                      \
+    if (e_ != hipSuccess) return bfail(b, LB_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+void drop_graphs(lb_batch* b) {
+  for (auto& g : b->graph)
+    if (g) {
+      hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+// grid and items per workgroup of a shape
+void shape_dims(const lb_batch* b, int32_t shape, int32_t n_items, int32_t* grid, int32_t* per) {
+  if (shape <= 0 || n_items <= b->n_cu * shape) {
+    *grid = n_items;
+    *per = 1;
+    return;
+  }
+  const int32_t g = b->n_cu * shape;
+  *per = (n_items + g - 1) / g;
+  *grid = (n_items + *per - 1) / *per;
+}
+
+hipError_t launch_step(lb_batch* b, bool reduce, int32_t shape) {
+  const int32_t n = static_cast<int32_t>(b->items.size());
+  int32_t grid, per;
+  shape_dims(b, shape, n, &grid, &per);
+  const int32_t parity = static_cast<int32_t>(b->global_step & 1);
+  if (b->precision == LB_FP64)
+    return batch_bgk_f64(b->lattice, reduce, b->d_solves, b->d_items, n, grid, per, parity, b->stream);
+  return batch_bgk_lowp(b->precision, b->lattice, reduce, b->d_solves, b->d_items, n, grid, per, parity,
+                        b->stream);
+}
+
+// rebuild the item and slot lists after admissions / retirements
+int refresh(lb_batch* b) {
+  b->items.clear();
+  b->active.clear();
+  b->resident_nodes = 0;
+  for (int32_t s = 0; s < static_cast<int32_t>(b->slots.size()); ++s) {
+    const Slot& sl = b->slots[s];
+    if (!sl.used) continue;
+    b->active.push_back(s);
+    b->resident_nodes += sl.fp.n;
+    for (int64_t k = 0; k < sl.fp.nblk; ++k)
+      b->items.push_back((static_cast<uint32_t>(s) << kItemShift) | static_cast<uint32_t>(k));
+  }
+  if (static_cast<int64_t>(b->items.size()) > b->items_cap) {
+    if (b->d_items) hipFree(b->d_items);
+    b->d_items = nullptr;
+    b->items_cap = std::max<int64_t>(static_cast<int64_t>(b->items.size()) * 2, 4096);
+    LB_BHIP(b, hipMalloc(&b->d_items, b->items_cap * sizeof(uint32_t)));
+  }
+  if (!b->items.empty())
+    LB_BHIP(b, hipMemcpyAsync(b->d_items, b->items.data(), b->items.size() * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, b->stream));
+  if (!b->active.empty())
+    LB_BHIP(b, hipMemcpyAsync(b->d_slots, b->active.data(), b->active.size() * sizeof(int32_t),
+                              hipMemcpyHostToDevice, b->stream));
+  LB_BHIP(b, hipStreamSynchronize(b->stream));
+  drop_graphs(b);
+  b->dirty = false;
+  return LB_OK;
+}
+
+// one check interval of steps (graph when available), then the per-solve sums
+int run_interval(lb_batch* b, bool tune) {
+  const int M = b->check_every;
+  const int32_t cand[] = {0, 1, 2, 4, 8};
+  constexpr int kCand = 5;
+  const int start_parity = static_cast<int>(b->global_step & 1);
+  if (tune) {
+    // time each candidate shape on live steps; results do not depend on the shape
+    std::vector<hipEvent_t> ev(2 * M);
+    for (auto& e : ev) LB_BHIP(b, hipEventCreate(&e));
+    for (int s = 0; s < M; ++s) {
+      hipEventRecord(ev[2 * s], b->stream);
+      hipError_t e = launch_step(b, s == M - 1, cand[s % kCand]);
+      hipEventRecord(ev[2 * s + 1], b->stream);
+      if (e != hipSuccess) return bfail(b, LB_ERR_HIP, "batch step: %s", hipGetErrorString(e));
+      ++b->global_step;
+    }
+    LB_BHIP(b, hipStreamSynchronize(b->stream));
+    double t[kCand] = {0, 0, 0, 0, 0};
+    int cnt[kCand] = {0, 0, 0, 0, 0};
+    for (int s = 0; s < M; ++s) {
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, ev[2 * s], ev[2 * s + 1]);
+      // the first launch of each shape warms up; skip it when there are others
+      if (s >= kCand || M <= kCand) {
+        t[s % kCand] += ms;
+        ++cnt[s % kCand];
+      }
+      b->kernel_ms += ms;
+    }
+    for (auto& e : ev) hipEventDestroy(e);
+    int best = 3;
+    double bt = 1e300;
+    for (int c = 0; c < kCand; ++c)
+      if (cnt[c] > 0 && t[c] / cnt[c] < bt) {
+        bt = t[c] / cnt[c];
+        best = c;
+      }
+    b->shape = cand[best];
+    b->tuned_nodes = b->resident_nodes;
+    hipEventRecord(b->ev0, b->stream);
+  } else {
+    hipEventRecord(b->ev0, b->stream);
+    hipGraphExec_t& g = b->graph[start_parity];
+    if (!g) {
+      hipGraph_t graph = nullptr;
+      LB_BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      hipError_t e = hipSuccess;
+      const int64_t gs = b->global_step;
+      for (int s = 0; s < M && e == hipSuccess; ++s) {
+        e = launch_step(b, s == M - 1, b->shape);
+        ++b->global_step;
+      }
+      b->global_step = gs;
+      hipError_t e2 = hipStreamEndCapture(b->stream, &graph);
+      if (e != hipSuccess || e2 != hipSuccess) {
+        if (graph) hipGraphDestroy(graph);
+        return bfail(b, LB_ERR_HIP, "graph capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+      }
+      e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+      hipGraphDestroy(graph);
+      if (e != hipSuccess) {
+        g = nullptr;
+        return bfail(b, LB_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+      }
+    }
+    LB_BHIP(b, hipGraphLaunch(g, b->stream));
+    b->global_step += M;
+  }
+  hipError_t e = batch_sum(b->d_solves, b->d_slots, static_cast<int32_t>(b->active.size()), b->d_out, b->stream);
+  if (e != hipSuccess) return bfail(b, LB_ERR_HIP, "batch sum: %s", hipGetErrorString(e));
+  hipEventRecord(b->ev1, b->stream);
+  LB_BHIP(b, hipMemcpyAsync(b->h_out, b->d_out, b->slots.size() * sizeof(double), hipMemcpyDeviceToHost,
+                            b->stream));
+  LB_BHIP(b, hipStreamSynchronize(b->stream));
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, b->ev0, b->ev1);
+  if (!tune) b->kernel_ms += ms;
+  b->launches += M + 1;
+  b->lups += b->resident_nodes * M;
+  return LB_OK;
+}
+
+void retire(lb_batch* b, int32_t s, int32_t status) {
+  Slot& sl = b->slots[s];
+  b->done.push_back(Result{sl.tag, sl.steps, sl.k, now_s() - sl.t_admit, status});
+  b->alloc.release(sl.off, sl.fp.bytes);
+  sl.used = false;
+  b->dirty = true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lb_batch_create(int lattice, int precision, int device, int64_t pool_bytes, int32_t check_every,
+                    lb_batch** out) {
+  if (!out) return fail_global(LB_ERR_ARG, "lb_batch_create: null output");
+  *out = nullptr;
+  if ((lattice != LB_D2Q9 && lattice != LB_D3Q19) ||
+      (precision != LB_FP64 && precision != LB_FP32S && precision != LB_FP16S) || pool_bytes < kAlign ||
+      check_every < 1)
+    return fail_global(LB_ERR_ARG, "lb_batch_create: bad argument");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail_global(LB_ERR_HIP, "lb_batch_create: no HIP device available");
+  if (device < 0 || device >= ndev) return fail_global(LB_ERR_ARG, "lb_batch_create: device %d out of range", device);
+  lb_batch* b = new (std::nothrow) lb_batch;
+  if (!b) return fail_global(LB_ERR_NOMEM, "lb_batch_create: out of host memory");
+  b->lattice = lattice;
+  b->precision = precision;
+  b->device = device;
+  b->Q = lattice == LB_D2Q9 ? 9 : 19;
+  b->nD = lattice == LB_D2Q9 ? 2 : 3;
+  b->esz = elem_bytes(precision);
+  b->check_every = check_every;
+  hipSetDevice(device);
+  hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, device);
+  if (b->n_cu <= 0) b->n_cu = 256;
+  hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+  if (e == hipSuccess) e = hipMalloc(&b->pool, static_cast<size_t>(pool_bytes));
+  if (e == hipSuccess) e = hipMalloc(&b->d_solves, kMaxSlots * sizeof(BatchSolve));
+  if (e == hipSuccess) e = hipMalloc(&b->d_slots, kMaxSlots * sizeof(int32_t));
+  if (e == hipSuccess) e = hipMalloc(&b->d_out, kMaxSlots * sizeof(double));
+  if (e == hipSuccess) e = hipHostMalloc(&b->h_out, kMaxSlots * sizeof(double), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    const int rc = fail_global(e == hipErrorOutOfMemory ? LB_ERR_NOMEM : LB_ERR_HIP, "lb_batch_create: %s",
+                               hipGetErrorString(e));
+    lb_batch_destroy(b);
+    return rc;
+  }
+  b->alloc.init(pool_bytes / kAlign * kAlign);
+  *out = b;
+  return LB_OK;
+}
+
+int lb_batch_destroy(lb_batch* b) {
+  if (!b) return LB_OK;
+  hipSetDevice(b->device);
+  if (b->stream) hipStreamSynchronize(b->stream);
+  drop_graphs(b);
+  if (b->pool) hipFree(b->pool);
+  if (b->d_solves) hipFree(b->d_solves);
+  if (b->d_items) hipFree(b->d_items);
+  if (b->d_slots) hipFree(b->d_slots);
+  if (b->d_out) hipFree(b->d_out);
+  if (b->h_out) hipHostFree(b->h_out);
+  if (b->ev0) hipEventDestroy(b->ev0);
+  if (b->ev1) hipEventDestroy(b->ev1);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+  return LB_OK;
+}
+
+const char* lb_batch_last_error(const lb_batch* b) { return b ? b->err.c_str() : lb_global_error(); }
+
+int lb_batch_footprint(int lattice, int precision, int32_t nd, const int32_t* dims, const int32_t* geo,
+                       int32_t periodic_mask, int64_t* bytes, int64_t* n_fluid) {
+  const int want = lattice == LB_D2Q9 ? 2 : 3;
+  if ((lattice != LB_D2Q9 && lattice != LB_D3Q19) || nd != want || !dims || !geo)
+    return fail_global(LB_ERR_ARG, "lb_batch_footprint: bad argument");
+  if (precision != LB_FP64 && precision != LB_FP32S && precision != LB_FP16S)
+    return fail_global(LB_ERR_ARG, "lb_batch_footprint: bad precision");
+  RowBuild rb;
+  std::string why;
+  if (!rows_from_geometry(lattice, dims, geo, periodic_mask, rb, why))
+    return fail_global(LB_ERR_UNSUPPORTED, "lb_batch_footprint: %s", why.c_str());
+  const int64_t nwin = static_cast<int64_t>(rb.g.E[0] + 2) * (rb.g.E[1] + 2) * rb.g.W;
+  const Footprint fp = footprint_of(lattice == LB_D2Q9 ? 9 : 19, elem_bytes(precision),
+                                    static_cast<int64_t>(rb.coord.size()), nwin);
+  if (bytes) *bytes = fp.bytes;
+  if (n_fluid) *n_fluid = fp.n;
+  return LB_OK;
+}
+
+int lb_batch_add(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* geo, int32_t periodic_mask,
+                 double tau, const double* force, double tol, int64_t max_steps, int64_t tag) {
+  if (!b) return LB_ERR_ARG;
+  if (nd != b->nD || !dims || !geo || !force || !(tau > 0.5) || !(tol > 0.0) || max_steps < 1 || force[0] == 0.0)
+    return bfail(b, LB_ERR_ARG, "lb_batch_add: bad argument");
+  if (b->dirty == false && b->active.size() >= static_cast<size_t>(kMaxSlots))
+    return bfail(b, LB_ERR_NOMEM, "lb_batch_add: all %d slots in use", kMaxSlots);
+  hipSetDevice(b->device);
+  RowBuild rb;
+  std::string why;
+  if (!rows_from_geometry(b->lattice, dims, geo, periodic_mask, rb, why))
+    return bfail(b, LB_ERR_UNSUPPORTED, "lb_batch_add: %s", why.c_str());
+  std::vector<uint4> win;
+  int32_t PY = 0;
+  if (!build_windows(rb, win, PY, why)) return bfail(b, LB_ERR_UNSUPPORTED, "lb_batch_add: %s", why.c_str());
+  const int64_t n = static_cast<int64_t>(rb.coord.size());
+  // every node's own window must name it (a cheap check of the construction)
+  for (int64_t i = 0; i < n; ++i) {
+    const uint32_t c = rb.coord[i];
+    const uint32_t X = c >> 22, Y = (c >> 12) & 1023u, Z = c & 4095u, w = Z >> 5, t = Z - 32 * w + 16;
+    const uint4 e = win[(static_cast<size_t>(X + 1) * PY + (Y + 1)) * rb.g.W + w];
+    const uint64_t m = (static_cast<uint64_t>(e.z) << 32) | e.y;
+    if (!((m >> t) & 1u) || e.x + static_cast<uint32_t>(__builtin_popcountll(m & ((1ull << t) - 1ull))) != i)
+      return bfail(b, LB_ERR_STATE, "lb_batch_add: row windows inconsistent at node %lld", (long long)i);
+  }
+  const Footprint fp = footprint_of(b->Q, b->esz, n, static_cast<int64_t>(win.size()));
+  if (static_cast<uint64_t>(b->Q) * static_cast<uint64_t>(fp.NP) * b->esz >= (1ull << 32) ||
+      fp.nblk > static_cast<int64_t>(kItemMask) + 1)
+    return bfail(b, LB_ERR_UNSUPPORTED, "lb_batch_add: %lld fluid nodes exceed one solve", (long long)n);
+  int32_t slot = -1;
+  for (int32_t s = 0; s < static_cast<int32_t>(b->slots.size()); ++s)
+    if (!b->slots[s].used) {
+      slot = s;
+      break;
+    }
+  if (slot < 0) {
+    if (b->slots.size() >= static_cast<size_t>(kMaxSlots))
+      return bfail(b, LB_ERR_NOMEM, "lb_batch_add: all %d slots in use", kMaxSlots);
+    b->slots.emplace_back();
+    slot = static_cast<int32_t>(b->slots.size()) - 1;
+  }
+  const int64_t off = b->alloc.alloc(fp.bytes);
+  if (off < 0)
+    return bfail(b, LB_ERR_NOMEM, "lb_batch_add: %lld bytes do not fit (largest free range %lld of %lld free)",
+                 (long long)fp.bytes, (long long)b->alloc.largest(), (long long)(b->alloc.total - b->alloc.used));
+  char* base = b->pool + off;
+  BatchSolve d{};
+  d.coord = reinterpret_cast<const uint32_t*>(base + fp.off_coord);
+  d.win = reinterpret_cast<const uint4*>(base + fp.off_win);
+  d.f[0] = base + fp.off_f0;
+  d.f[1] = base + fp.off_f1;
+  d.partial = reinterpret_cast<double*>(base + fp.off_part);
+  d.NP = static_cast<uint32_t>(fp.NP);
+  d.PY = PY;
+  d.W = rb.g.W;
+  d.n = static_cast<int32_t>(n);
+  d.parity = static_cast<int32_t>(b->global_step & 1);  // f[0] holds the state at the next step
+  d.a.tau_inv = 1.0 / tau;
+  d.a.tau_factor = 1 - 0.5 / tau;
+  for (int k = 0; k < 3; ++k) d.a.F[k] = k < b->nD ? force[k] : 0.0;
+  hipError_t e = hipMemcpyAsync(base + fp.off_coord, rb.coord.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                b->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(base + fp.off_win, win.data(), win.size() * sizeof(uint4), hipMemcpyHostToDevice, b->stream);
+  // rest state f = w: zero in the shifted forms
+  if (e == hipSuccess) {
+    if (b->precision == LB_FP64) e = batch_rest_f64(b->lattice, reinterpret_cast<double*>(base + fp.off_f0), fp.NP, b->stream);
+    else e = hipMemsetAsync(base + fp.off_f0, 0, static_cast<size_t>(b->Q) * fp.NP * b->esz, b->stream);
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(b->d_solves + slot, &d, sizeof(BatchSolve), hipMemcpyHostToDevice, b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  if (e != hipSuccess) {
+    b->alloc.release(off, fp.bytes);
+    return bfail(b, LB_ERR_HIP, "lb_batch_add: %s", hipGetErrorString(e));
+  }
+  Slot& sl = b->slots[slot];
+  sl = Slot{};
+  sl.used = true;
+  sl.tag = tag;
+  sl.off = off;
+  sl.fp = fp;
+  sl.tau = tau;
+  sl.F0 = force[0];
+  sl.nu = (tau - 0.5) / 3.0;
+  int64_t nbox = 1;
+  for (int k = 0; k < nd; ++k) nbox *= dims[k];
+  sl.n_box = static_cast<double>(nbox);
+  sl.tol = tol;
+  sl.max_steps = max_steps;
+  sl.t_admit = now_s();
+  b->dirty = true;
+  return LB_OK;
+}
+
+int lb_batch_advance(lb_batch* b, int32_t intervals, int32_t* n_finished) {
+  if (!b) return LB_ERR_ARG;
+  if (intervals < 0) return bfail(b, LB_ERR_ARG, "lb_batch_advance: bad argument");
+  hipSetDevice(b->device);
+  int32_t fin = 0;
+  for (int32_t it = 0; it < intervals; ++it) {
+    if (b->dirty) {
+      const int rc = refresh(b);
+      if (rc != LB_OK) return rc;
+    }
+    if (b->active.empty()) break;
+    const bool tune = b->shape_mode < 0 &&
+                      (b->tuned_nodes < 0 || std::llabs(b->resident_nodes - b->tuned_nodes) * 5 > b->tuned_nodes);
+    if (b->shape_mode >= 0) b->shape = b->shape_mode;
+    int rc = run_interval(b, tune);
+    if (rc != LB_OK) return rc;
+    for (int32_t s : b->active) {
+      Slot& sl = b->slots[s];
+      sl.steps += b->check_every;
+      const double jx = b->h_out[s];
+      // k = nu <j_x> / F_x with j_x summed over the state the last step started from
+      sl.k = sl.nu * (jx / sl.n_box) / sl.F0;
+      if (!std::isfinite(sl.k)) {
+        retire(b, s, 2);
+        ++fin;
+        continue;
+      }
+      if (sl.have_prev && std::fabs(sl.k - sl.k_prev) <= sl.tol * std::fabs(sl.k)) {
+        retire(b, s, 0);
+        ++fin;
+        continue;
+      }
+      sl.k_prev = sl.k;
+      sl.have_prev = true;
+      if (sl.steps >= sl.max_steps) {
+        retire(b, s, 1);
+        ++fin;
+      }
+    }
+  }
+  if (n_finished) *n_finished = fin;
+  return LB_OK;
+}
+
+int lb_batch_collect(lb_batch* b, int32_t max_n, int64_t* tags, double* k, int64_t* steps, double* seconds,
+                     int32_t* status, int32_t* n_out) {
+  if (!b) return LB_ERR_ARG;
+  if (max_n < 0 || !n_out) return bfail(b, LB_ERR_ARG, "lb_batch_collect: bad argument");
+  const int32_t n = std::min<int32_t>(max_n, static_cast<int32_t>(b->done.size()));
+  for (int32_t i = 0; i < n; ++i) {
+    const Result& r = b->done[i];
+    if (tags) tags[i] = r.tag;
+    if (k) k[i] = r.k;
+    if (steps) steps[i] = r.steps;
+    if (seconds) seconds[i] = r.seconds;
+    if (status) status[i] = r.status;
+  }
+  b->done.erase(b->done.begin(), b->done.begin() + n);
+  *n_out = n;
+  return LB_OK;
+}
+
+int lb_batch_status(const lb_batch* b, int32_t* n_resident, int64_t* resident_nodes, int64_t* bytes_used,
+                    int64_t* bytes_free, int64_t* largest_free) {
+  if (!b) return LB_ERR_ARG;
+  int32_t nr = 0;
+  int64_t nodes = 0;
+  for (const Slot& s : b->slots)
+    if (s.used) {
+      ++nr;
+      nodes += s.fp.n;
+    }
+  if (n_resident) *n_resident = nr;
+  if (resident_nodes) *resident_nodes = nodes;
+  if (bytes_used) *bytes_used = b->alloc.used;
+  if (bytes_free) *bytes_free = b->alloc.total - b->alloc.used;
+  if (largest_free) *largest_free = b->alloc.largest();
+  return LB_OK;
+}
+
+int lb_batch_set_shape(lb_batch* b, int32_t wg_per_cu) {
+  if (!b) return LB_ERR_ARG;
+  if (wg_per_cu < -1 || wg_per_cu > 64) return bfail(b, LB_ERR_ARG, "lb_batch_set_shape: bad argument");
+  b->shape_mode = wg_per_cu;
+  if (wg_per_cu >= 0) b->shape = wg_per_cu;
+  b->tuned_nodes = -1;
+  drop_graphs(b);
+  return LB_OK;
+}
+
+int lb_batch_shape(const lb_batch* b, int32_t* wg_per_cu) {
+  if (!b || !wg_per_cu) return LB_ERR_ARG;
+  *wg_per_cu = b->shape;
+  return LB_OK;
+}
+
+int lb_batch_kernel_time(const lb_batch* b, double* ms, int64_t* launches, int64_t* lups) {
+  if (!b) return LB_ERR_ARG;
+  if (ms) *ms = b->kernel_ms;
+  if (launches) *launches = b->launches;
+  if (lups) *lups = b->lups;
+  return LB_OK;
+}
+
+}  // extern "C"

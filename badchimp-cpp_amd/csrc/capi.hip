@@ -15,6 +15,7 @@
 #include "kernels.hpp"
 #include "kernels_f32.hpp"
 #include "lattice.hpp"
+#include "rowbuild.hpp"
 
 using namespace lbamd;
 
@@ -121,16 +122,9 @@ int pre_state(lb_ctx* c, const void** out) {
   return LB_OK;
 }
 
-// Cartesian row layout of the bulk nodes (see addressing.hpp).  Returns false
+// Row layout of a node list (positions and neighbour index).  Returns false
 // with a reason when the node list is not a set of cells of one box that the
 // layout can address.
-struct RowBuild {
-  RowGeom g{};
-  std::vector<int32_t> order;   // compact index -> bulk index
-  std::vector<uint32_t> coord;  // compact index -> packed cell
-  std::vector<uint2> rows;
-};
-
 bool build_rows(int lattice, int32_t n_nodes, const int32_t* nb, const int32_t* bulk, int32_t n_bulk,
                 const std::vector<int32_t>& bidx, const int32_t* pos, RowBuild& rb, std::string& why) {
   const int nD = lattice == LB_D2Q9 ? 2 : 3, Q = lattice == LB_D2Q9 ? 9 : 19;
@@ -237,80 +231,6 @@ std::pair<bool, int64_t> row_source(int lattice, const RowBuild& rb, int q, int3
   const uint32_t bit = 1u << (X[2] & 31);
   if (e.y & bit) return {true, static_cast<int64_t>(e.x + __builtin_popcount(e.y & (bit - 1u)))};
   return {false, i};
-}
-
-// Row windows (addressing.hpp, RowWin): for the halo-padded row (X+1, Y+1)
-// and word w, the fluid bits of cells [32w-16, 32w+48) and the compact index
-// of the first fluid cell at or after 32w-16.  Halo rows of a periodic axis
-// repeat the rows they wrap to; those of a closed axis stay empty.  Needs a
-// box that is not periodic along the row axis.
-bool build_windows(const RowBuild& rb, std::vector<uint4>& win, int32_t& PY, std::string& why) {
-  const RowGeom& g = rb.g;
-  if (g.periodic & 4) {
-    why = "periodic along the row axis";
-    return false;
-  }
-  const int32_t PX = g.E[0] + 2;
-  PY = g.E[1] + 2;
-  win.assign(static_cast<size_t>(PX) * PY * g.W, uint4{0u, 0u, 0u, 0u});
-  auto fluid = [&](size_t r, int32_t z) -> bool {
-    if (z < 0 || z >= g.E[2]) return false;
-    return (rb.rows[r * g.W + (z >> 5)].y >> (z & 31)) & 1u;
-  };
-  auto first_at_or_after = [&](size_t r, int32_t z) -> uint32_t {
-    // compact index of the first fluid cell of row r at or after z (or the
-    // index one past the row's last fluid cell)
-    const int32_t zc = std::max(z, 0);
-    if (zc >= 32 * g.W) {
-      const uint2 e = rb.rows[r * g.W + g.W - 1];
-      return e.x + static_cast<uint32_t>(__builtin_popcount(e.y));
-    }
-    const uint2 e = rb.rows[r * g.W + (zc >> 5)];
-    return e.x + static_cast<uint32_t>(__builtin_popcount(e.y & ((1u << (zc & 31)) - 1u)));
-  };
-  for (int32_t px = 0; px < PX; ++px)
-    for (int32_t py = 0; py < PY; ++py) {
-      int32_t X = px - 1, Y = py - 1;
-      bool inside = RowGeom::wrap(X, g.E[0], g.periodic & 1);
-      inside = RowGeom::wrap(Y, g.E[1], g.periodic & 2) && inside;
-      if (!inside) continue;
-      const size_t r = static_cast<size_t>(g.row(X, Y));
-      for (int32_t w = 0; w < g.W; ++w) {
-        const int32_t s0 = 32 * w - 16;
-        uint64_t m = 0;
-        for (int k = 0; k < 64; ++k)
-          if (fluid(r, s0 + k)) m |= 1ull << k;
-        win[(static_cast<size_t>(px) * PY + py) * g.W + w] =
-            uint4{first_at_or_after(r, s0), static_cast<uint32_t>(m), static_cast<uint32_t>(m >> 32), 0u};
-      }
-    }
-  return true;
-}
-
-// Source of direction q for compact node i under the row windows, as RowWin
-// computes it on the device.
-std::pair<bool, int64_t> win_source(int lattice, const RowBuild& rb, const std::vector<uint4>& win, int32_t PY,
-                                    int q, int32_t i) {
-  const int nD = lattice == LB_D2Q9 ? 2 : 3;
-  auto C = [&](int a) -> int {
-    const int d = nD == 3 ? a : (a == 0 ? 0 : (a == 2 ? 1 : -1));
-    if (d < 0) return 0;
-    return lattice == LB_D2Q9 ? D2Q9::c(q, d) : D3Q19::c(q, d);
-  };
-  const RowGeom& g = rb.g;
-  const uint32_t cc = rb.coord[i];
-  const int32_t X = static_cast<int32_t>(cc >> 22), Y = static_cast<int32_t>((cc >> 12) & 1023u),
-                Z = static_cast<int32_t>(cc & 4095u);
-  const int32_t w = Z >> 5;
-  const uint32_t t = static_cast<uint32_t>(Z - 32 * w + 16);
-  const size_t slot = (static_cast<size_t>(X + 1 - C(0)) * PY + (Y + 1 - C(1))) * g.W + w;
-  const uint4 e = win[slot];
-  const uint64_t m = (static_cast<uint64_t>(e.z) << 32) | e.y;
-  const int64_t b0 = static_cast<int64_t>(e.x) + __builtin_popcountll(m & ((1ull << t) - 1ull));
-  const uint32_t bm = static_cast<uint32_t>(m >> (t - 1)) & 7u;
-  const int dz = -C(2);
-  if (!((bm >> (dz + 1)) & 1u)) return {false, i};
-  return {true, dz == 0 ? b0 : (dz < 0 ? b0 - 1 : b0 + ((bm >> 1) & 1u))};
 }
 
 int setup_topology(lb_ctx* c, int32_t n_nodes, const int32_t* neighbor, const int16_t* node_type,

@@ -1,8 +1,10 @@
 // Device context of one rank's subdomain and the shared device-side layout.
 //
-// Layout in HBM (SoA over the compact bulk order i = 0..n_bulk-1, which is
-// the reference's findBulkNodes order, padded to NP = multiple of 256):
-//   f[k][q][i]    populations of field k, storage type per precision,
+// Layout in HBM (over the compact bulk order i = 0..n_bulk-1: the
+// reference's findBulkNodes order, or the tiled row order of the row
+// layouts; padded to NP = multiple of 256):
+//   f[k](q,i)     populations of field k at pidx(q, i) (blocks of 64 nodes,
+//                 directions one after another), storage type per precision,
 //                 two buffers (ping-pong)
 //   link[q][i]    q = 0..Q-2 (the rest direction Q-1 always reads itself):
 //                 bits 30-31 kind, bits 0-29 index
@@ -47,6 +49,25 @@ struct BgkArgs {
   double tau_factor;  // 1 - 0.5 / tau            (LBcollision.h:206)
   double F[3];        // body force
 };
+
+// One solve of the batched form: its two population buffers (each Q * NP
+// elements, AoSoA as above), its row-window address table, its block
+// partials and its parameters.  parity: which buffer holds the current state
+// at even global steps.
+struct BatchSolve {
+  const uint32_t* coord;
+  const uint4* win;
+  void* f[2];
+  double* partial;
+  uint32_t NP;
+  int32_t PY, W, n;
+  int32_t parity;
+  int32_t pad_;
+  BgkArgs a;
+};
+// Work item of a batched launch: slot << kItemShift | 256-node block of the solve.
+constexpr int kItemShift = 18;
+constexpr uint32_t kItemMask = (1u << kItemShift) - 1u;
 
 struct CgArgs {
   double nu0Inv, nu1Inv;  // 1.0 / (c2 * (tau_k - 0.5))  (main_TWOPHASE.cpp:128-129)

@@ -46,6 +46,18 @@ hipError_t flux_f64(int lattice, int nfields, bool first, const double* fin, con
 hipError_t init_eq_f64(int lattice, double* f, int field, int32_t n, int64_t NP,
                        const double* rho, const double* vel, hipStream_t s);
 
+// ---- batched form -----------------------------------------------------------
+// One FP64 step of every solve listed in items (see k_batch_bgk): grid
+// workgroups of 256 threads, each over a run of per_wg items; reduce writes
+// each solve's block partials of j_x.
+hipError_t batch_bgk_f64(int lattice, bool reduce, const BatchSolve* solves, const uint32_t* items,
+                         int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, hipStream_t s);
+// out[slot] = sum of the solve's block partials, for each slot listed.
+hipError_t batch_sum(const BatchSolve* solves, const int32_t* slots, int32_t n_slots, double* out,
+                     hipStream_t s);
+// Rest state (f = w) of one FP64 buffer of Q * NP elements.
+hipError_t batch_rest_f64(int lattice, double* f, int64_t NP, hipStream_t s);
+
 // ---- shared ---------------------------------------------------------------
 // out[c] = sum over blocks of partial[c*nblocks + b], c < ncomp (fixed order).
 hipError_t sum_partials(const double* partial, int64_t nblocks, int ncomp, double* out,

@@ -69,6 +69,68 @@ __device__ __forceinline__ void load_g(const S* __restrict__ gin, const A& ad, i
   g[L::nQ - 1] = Store<S>::get(ld_elem<O>(gin, pidx<L::nQ>(L::nQ - 1, i)));
 }
 
+// One node of the BGK + Guo step on shifted populations; returns j_x (FP64)
+// for the permeability sum when REDUCE.
+template <class L, typename S, class A, bool FIRST, bool STORE, bool REDUCE>
+__device__ __forceinline__ double bgk_node_lp(const S* __restrict__ gin, S* __restrict__ gout, const A& ad,
+                                              int32_t i, int64_t NP, float tau_inv, float tau_factor,
+                                              const float (&F)[3], double* __restrict__ rho_o,
+                                              double* __restrict__ vel_o) {
+  constexpr int Q = L::nQ, D = L::nD;
+  float g[Q];
+  load_g<L, S, FIRST>(gin, ad, i, NP, g);
+  float drho = 0.0f;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) drho += g[q];
+  const float rho = 1.0f + drho;
+  const float rinv = 1.0f / rho;
+  float u[D], jd[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    jd[d] = qsumc<L>(g, d) + 0.5f * F[d];
+    u[d] = jd[d] * rinv;
+  }
+  if (STORE) {
+    rho_o[i] = static_cast<double>(rho);
+#pragma unroll
+    for (int d = 0; d < D; ++d) vel_o[d * NP + i] = static_cast<double>(u[d]);
+  }
+  // Directions q and rev q share |c_q . u| and |c_q . F|: split geq and the
+  // Guo term into the parts even and odd in c_q and form both at once.
+  //   geq  = w (a0 + 4.5 rho cu^2 +- 3 rho cu),  a0 = drho - 1.5 rho u^2
+  //   dF   = w tf (9 cF cu - 3 uF +- 3 cF)
+  //   g'   = (1 - 1/tau) g + geq/tau + dF
+  const float a0 = drho - 1.5f * rho * dot<L>(u, u);
+  const float m3uF = -3.0f * dot<L>(u, F);
+  const float omt = 1.0f - tau_inv;
+  const float r45 = 4.5f * rho, r3 = 3.0f * rho;
+  constexpr bool O = A::kOff32;
+#pragma unroll
+  for (int q = 0; q < Q - 1; ++q) {
+    const int rq = L::rev(q);
+    if (rq < q) continue;  // each pair once
+    const float w = float(L::w(q));
+    const float wt = w * tau_inv, wf = w * tau_factor;
+    const float cu = cdot<L>(q, u);
+    const float cF = cdot<L>(q, F);
+    const float ev = a0 + r45 * cu * cu;
+    const float od = r3 * cu;
+    const float fev = 9.0f * cF * cu + m3uF;
+    const float fod = 3.0f * cF;
+    const float gq = omt * g[q] + wt * (ev + od) + wf * (fev + fod);
+    const float gr = omt * g[rq] + wt * (ev - od) + wf * (fev - fod);
+    __builtin_nontemporal_store(Store<S>::put(gq), elem_ptr<O>(gout, pidx<Q>(q, i)));
+    __builtin_nontemporal_store(Store<S>::put(gr), elem_ptr<O>(gout, pidx<Q>(rq, i)));
+  }
+  {
+    constexpr int q = Q - 1;  // rest: c = 0
+    const float w = float(L::w(q));
+    const float gq = omt * g[q] + (w * tau_inv) * a0 + (w * tau_factor) * m3uF;
+    __builtin_nontemporal_store(Store<S>::put(gq), elem_ptr<O>(gout, pidx<Q>(q, i)));
+  }
+  return REDUCE ? static_cast<double>(jd[0]) : 0.0;
+}
+
 template <class L, typename S, class A, bool FIRST, bool STORE, bool REDUCE>
 __global__ __launch_bounds__(kBlock) void k_bgk_lp(const S* __restrict__ gin, S* __restrict__ gout, A ad,
                                                    int32_t n, int64_t NP, float tau_inv, float tau_factor,
@@ -76,69 +138,46 @@ __global__ __launch_bounds__(kBlock) void k_bgk_lp(const S* __restrict__ gin, S*
                                                    double* __restrict__ rho_o,
                                                    double* __restrict__ vel_o,
                                                    double* __restrict__ partial) {
-  constexpr int Q = L::nQ, D = L::nD;
   const int32_t b = xcd_block(blockIdx.x, gridDim.x);
   const int32_t i = b * kBlock + threadIdx.x;
   double jx = 0.0;
   if (i < n) {
     const float F[3] = {F0, F1, F2};
-    float g[Q];
-    load_g<L, S, FIRST>(gin, ad, i, NP, g);
-    float drho = 0.0f;
-#pragma unroll
-    for (int q = 0; q < Q; ++q) drho += g[q];
-    const float rho = 1.0f + drho;
-    const float rinv = 1.0f / rho;
-    float u[D], jd[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      jd[d] = qsumc<L>(g, d) + 0.5f * F[d];
-      u[d] = jd[d] * rinv;
-    }
-    if (STORE) {
-      rho_o[i] = static_cast<double>(rho);
-#pragma unroll
-      for (int d = 0; d < D; ++d) vel_o[d * NP + i] = static_cast<double>(u[d]);
-    }
-    if (REDUCE) jx = static_cast<double>(jd[0]);
-    // Directions q and rev q share |c_q . u| and |c_q . F|: split geq and the
-    // Guo term into the parts even and odd in c_q and form both at once.
-    //   geq  = w (a0 + 4.5 rho cu^2 +- 3 rho cu),  a0 = drho - 1.5 rho u^2
-    //   dF   = w tf (9 cF cu - 3 uF +- 3 cF)
-    //   g'   = (1 - 1/tau) g + geq/tau + dF
-    const float a0 = drho - 1.5f * rho * dot<L>(u, u);
-    const float m3uF = -3.0f * dot<L>(u, F);
-    const float omt = 1.0f - tau_inv;
-    const float r45 = 4.5f * rho, r3 = 3.0f * rho;
-    const uint32_t np = static_cast<uint32_t>(NP);
-    constexpr bool O = A::kOff32;
-#pragma unroll
-    for (int q = 0; q < Q - 1; ++q) {
-      const int rq = L::rev(q);
-      if (rq < q) continue;  // each pair once
-      const float w = float(L::w(q));
-      const float wt = w * tau_inv, wf = w * tau_factor;
-      const float cu = cdot<L>(q, u);
-      const float cF = cdot<L>(q, F);
-      const float ev = a0 + r45 * cu * cu;
-      const float od = r3 * cu;
-      const float fev = 9.0f * cF * cu + m3uF;
-      const float fod = 3.0f * cF;
-      const float gq = omt * g[q] + wt * (ev + od) + wf * (fev + fod);
-      const float gr = omt * g[rq] + wt * (ev - od) + wf * (fev - fod);
-      __builtin_nontemporal_store(Store<S>::put(gq), elem_ptr<O>(gout, pidx<Q>(q, i)));
-      __builtin_nontemporal_store(Store<S>::put(gr), elem_ptr<O>(gout, pidx<Q>(rq, i)));
-    }
-    {
-      constexpr int q = Q - 1;  // rest: c = 0
-      const float w = float(L::w(q));
-      const float gq = omt * g[q] + (w * tau_inv) * a0 + (w * tau_factor) * m3uF;
-      __builtin_nontemporal_store(Store<S>::put(gq), elem_ptr<O>(gout, pidx<Q>(q, i)));
-    }
+    jx = bgk_node_lp<L, S, A, FIRST, STORE, REDUCE>(gin, gout, ad, i, NP, tau_inv, tau_factor, F, rho_o, vel_o);
   }
   if (REDUCE) {
     const double s = block_sum(jx);
     if (threadIdx.x == 0) partial[b] = s;
+  }
+}
+
+// Batched form (see k_batch_bgk in the FP64 kernels): the parameters are
+// rounded to float exactly as the single-solve launcher rounds them.
+template <class L, typename S, bool REDUCE>
+__global__ __launch_bounds__(kBlock) void k_batch_bgk_lp(const BatchSolve* __restrict__ solves,
+                                                         const uint32_t* __restrict__ items, int32_t n_items,
+                                                         int32_t per_wg, int32_t parity) {
+  const int32_t g = xcd_block(blockIdx.x, gridDim.x);
+  const int32_t beg = g * per_wg;
+  const int32_t end = beg + per_wg < n_items ? beg + per_wg : n_items;
+  for (int32_t it = beg; it < end; ++it) {
+    const uint32_t item = items[it];
+    const BatchSolve& s = solves[item >> kItemShift];
+    const int32_t lb = static_cast<int32_t>(item & kItemMask);
+    const int p = (parity ^ s.parity) & 1;
+    const RowWin ad{s.coord, s.win, s.NP, s.PY, s.W};
+    const int32_t i = lb * kBlock + static_cast<int32_t>(threadIdx.x);
+    double jx = 0.0;
+    if (i < s.n) {
+      const float F[3] = {static_cast<float>(s.a.F[0]), static_cast<float>(s.a.F[1]), static_cast<float>(s.a.F[2])};
+      jx = bgk_node_lp<L, S, RowWin, false, false, REDUCE>(
+          static_cast<const S*>(s.f[p]), static_cast<S*>(s.f[p ^ 1]), ad, i, s.NP, static_cast<float>(s.a.tau_inv),
+          static_cast<float>(s.a.tau_factor), F, nullptr, nullptr);
+    }
+    if (REDUCE) {
+      const double t = block_sum(jx);
+      if (threadIdx.x == 0) s.partial[lb] = t;
+    }
   }
 }
 
@@ -505,6 +544,18 @@ hipError_t flux_lowp(int precision, int lattice, int nf, bool first, const void*
     using A = decltype(addr);
     if (first) k_flux_lp<L, S, A, true><<<g, b, 0, s>>>(static_cast<const S*>(gin), addr, n, NP, dir, partial, nb);
     else k_flux_lp<L, S, A, false><<<g, b, 0, s>>>(static_cast<const S*>(gin), addr, n, NP, dir, partial, nb);
+  });
+}
+
+hipError_t batch_bgk_lowp(int precision, int lattice, bool reduce, const BatchSolve* solves, const uint32_t* items,
+                          int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, hipStream_t s) {
+  return dispatch(precision, lattice, Addr{}, [&](auto l, auto st, auto) {
+    using L = decltype(l);
+    using S = decltype(st);
+    if (reduce)
+      k_batch_bgk_lp<L, S, true><<<dim3(grid), dim3(kBlock), 0, s>>>(solves, items, n_items, per_wg, parity);
+    else
+      k_batch_bgk_lp<L, S, false><<<dim3(grid), dim3(kBlock), 0, s>>>(solves, items, n_items, per_wg, parity);
   });
 }
 

@@ -29,6 +29,9 @@ hipError_t bgk_lowp(int precision, int lattice, bool first, bool store, bool red
 hipError_t flux_lowp(int precision, int lattice, int nf, bool first, const void* gin,
                      const Addr& ad, int32_t n, int64_t NP, int dir, double* partial,
                      hipStream_t s);
+// Batched form in reduced precision (see batch_bgk_f64).
+hipError_t batch_bgk_lowp(int precision, int lattice, bool reduce, const BatchSolve* solves, const uint32_t* items,
+                          int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, hipStream_t s);
 // Two-phase populations are stored as (t, d) = (f0 + f1 - w, f0 - f1) and use
 // the link table.
 hipError_t cg_phi_lowp(int precision, int lattice, bool first, bool momx, const void* gin,

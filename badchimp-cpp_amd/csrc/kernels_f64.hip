@@ -60,51 +60,114 @@ __device__ __forceinline__ double block_sum(double v) {
 }
 
 // ------------------------------------------------------------------ BGK
+// One node of the std_case loop body (std_case/main.cpp:110-145); returns the
+// node's j_x = sum_q f c_x + F_x/2 for the permeability sum when REDUCE.
+template <class L, class A, bool FIRST, bool STORE, bool REDUCE>
+__device__ __forceinline__ double bgk_node(const double* __restrict__ fin, double* __restrict__ fout, const A& ad,
+                                           int32_t i, int64_t NP, const BgkArgs& a, double* __restrict__ rho_o,
+                                           double* __restrict__ vel_o) {
+  constexpr int Q = L::nQ, D = L::nD;
+  double jx = 0.0;
+  double f[1][Q];
+  load_node<L, 1, FIRST>(fin, ad, i, NP, f);
+  // calcRho, calcVel (LBmacroscopic.h:11-45)
+  const double rho = qsum<L>(f[0]);
+  double u[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) u[d] = (qsumc<L>(f[0], d) + 0.5 * a.F[d]) / rho;
+  if (STORE) {
+    rho_o[i] = rho;
+#pragma unroll
+    for (int d = 0; d < D; ++d) vel_o[d * NP + i] = u[d];
+  }
+  if (REDUCE) jx = qsumc<L>(f[0], 0) + 0.5 * a.F[0];
+  const double u2 = dot<L>(u, u);
+  const double uF = dot<L>(u, a.F);
+  const double c2u2 = L::c2 * u2;
+  const double c2uF = L::c2 * uF;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const double cu = cdot<L>(q, u);
+    const double cF = cdot<L>(q, a.F);
+    // calcOmegaBGK (LBcollision.h:41-44)
+    const double feq = (rho * L::w(q)) * ((1.0 + L::c2Inv * cu) + L::c4Inv0_5 * (cu * cu - c2u2));
+    const double om = (-a.tau_inv) * (f[0][q] - feq);
+    // calcDeltaOmegaF (LBcollision.h:206-211)
+    const double dF = (L::w(q) * a.tau_factor) * (L::c2Inv * cF + L::c4Inv * (cF * cu - c2uF));
+    __builtin_nontemporal_store((f[0][q] + om) + dF, elem_ptr<A::kOff32>(fout, pidx<Q>(q, i)));
+  }
+  return jx;
+}
+
 template <class L, class A, bool FIRST, bool STORE, bool REDUCE>
 __global__ __launch_bounds__(kBlock) void k_bgk(const double* __restrict__ fin,
                                                 double* __restrict__ fout, A ad, int32_t n,
                                                 int64_t NP, BgkArgs a, double* __restrict__ rho_o,
                                                 double* __restrict__ vel_o,
                                                 double* __restrict__ partial) {
-  constexpr int Q = L::nQ, D = L::nD;
   const int32_t b = xcd_block(blockIdx.x, gridDim.x);
   const int32_t i = b * kBlock + threadIdx.x;
   double jx = 0.0;
-  if (i < n) {
-    double f[1][Q];
-    load_node<L, 1, FIRST>(fin, ad, i, NP, f);
-    // calcRho, calcVel (LBmacroscopic.h:11-45)
-    const double rho = qsum<L>(f[0]);
-    double u[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) u[d] = (qsumc<L>(f[0], d) + 0.5 * a.F[d]) / rho;
-    if (STORE) {
-      rho_o[i] = rho;
-#pragma unroll
-      for (int d = 0; d < D; ++d) vel_o[d * NP + i] = u[d];
-    }
-    if (REDUCE) jx = qsumc<L>(f[0], 0) + 0.5 * a.F[0];
-    const double u2 = dot<L>(u, u);
-    const double uF = dot<L>(u, a.F);
-    const double c2u2 = L::c2 * u2;
-    const double c2uF = L::c2 * uF;
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const double cu = cdot<L>(q, u);
-      const double cF = cdot<L>(q, a.F);
-      // calcOmegaBGK (LBcollision.h:41-44)
-      const double feq = (rho * L::w(q)) * ((1.0 + L::c2Inv * cu) + L::c4Inv0_5 * (cu * cu - c2u2));
-      const double om = (-a.tau_inv) * (f[0][q] - feq);
-      // calcDeltaOmegaF (LBcollision.h:206-211)
-      const double dF = (L::w(q) * a.tau_factor) * (L::c2Inv * cF + L::c4Inv * (cF * cu - c2uF));
-      __builtin_nontemporal_store((f[0][q] + om) + dF,
-                                  elem_ptr<A::kOff32>(fout, pidx<Q>(q, i)));
-    }
-  }
+  if (i < n) jx = bgk_node<L, A, FIRST, STORE, REDUCE>(fin, fout, ad, i, NP, a, rho_o, vel_o);
   if (REDUCE) {
     const double s = block_sum(jx);
     if (threadIdx.x == 0) partial[b] = s;
   }
+}
+
+// Batched form: one launch advances every resident solve one step.  Work
+// items are 256-node blocks of the solves (item = slot << kItemShift | block);
+// each workgroup takes a contiguous run of per_wg items, and the logical
+// workgroup order is XCD-grouped, so neighbouring blocks of one solve share an
+// L2.  A solve's buffers, address table and parameters come from its
+// descriptor (uniform per item), and the per-node arithmetic is the single
+// solve's (bgk_node), so a solve gets the same bits alone or in any mix.
+template <class L, bool REDUCE>
+__global__ __launch_bounds__(kBlock) void k_batch_bgk(const BatchSolve* __restrict__ solves,
+                                                      const uint32_t* __restrict__ items, int32_t n_items,
+                                                      int32_t per_wg, int32_t parity) {
+  const int32_t g = xcd_block(blockIdx.x, gridDim.x);
+  const int32_t beg = g * per_wg;
+  const int32_t end = beg + per_wg < n_items ? beg + per_wg : n_items;
+  for (int32_t it = beg; it < end; ++it) {
+    const uint32_t item = items[it];
+    const BatchSolve& s = solves[item >> kItemShift];
+    const int32_t lb = static_cast<int32_t>(item & kItemMask);
+    const int p = (parity ^ s.parity) & 1;
+    const RowWin ad{s.coord, s.win, s.NP, s.PY, s.W};
+    const int32_t i = lb * kBlock + static_cast<int32_t>(threadIdx.x);
+    double jx = 0.0;
+    if (i < s.n)
+      jx = bgk_node<L, RowWin, false, false, REDUCE>(static_cast<const double*>(s.f[p]),
+                                                     static_cast<double*>(s.f[p ^ 1]), ad, i, s.NP, s.a,
+                                                     nullptr, nullptr);
+    if (REDUCE) {
+      const double t = block_sum(jx);
+      if (threadIdx.x == 0) s.partial[lb] = t;
+    }
+  }
+}
+
+// Per-solve sum of the block partials, in the order k_sum_partials uses.
+__global__ __launch_bounds__(kBlock) void k_batch_sum(const BatchSolve* __restrict__ solves,
+                                                      const int32_t* __restrict__ slots,
+                                                      double* __restrict__ out) {
+  const int32_t slot = slots[blockIdx.x];
+  const BatchSolve& s = solves[slot];
+  const int64_t nb = s.n > 0 ? (s.n + kBlock - 1) / kBlock : 1;
+  double v = 0.0;
+  for (int64_t b = threadIdx.x; b < nb; b += kBlock) v += s.partial[b];
+  v = block_sum(v);
+  if (threadIdx.x == 0) out[slot] = v;
+}
+
+// Rest state f = w of a solve's pre-collision buffer (FP64 form).
+template <class L>
+__global__ __launch_bounds__(kBlock) void k_batch_rest(double* __restrict__ f, int64_t NP) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (e >= static_cast<int64_t>(L::nQ) * NP) return;
+  const int q = static_cast<int>((e >> kPLog) % L::nQ);
+  f[e] = L::w(q);
 }
 
 // ------------------------------------------------------------------ two-phase pass 1
@@ -449,6 +512,30 @@ hipError_t init_eq_f64(int lattice, double* f, int field, int32_t n, int64_t NP,
   if (n <= 0) return hipSuccess;
   LB_DISPATCH_LATTICE(lattice, L, {
     k_init_eq<L><<<grid_for(n), dim3(kBlock), 0, s>>>(f, field, n, NP, rho, vel);
+  });
+  return hipGetLastError();
+}
+
+hipError_t batch_bgk_f64(int lattice, bool reduce, const BatchSolve* solves, const uint32_t* items,
+                         int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, hipStream_t s) {
+  LB_DISPATCH_LATTICE(lattice, L, {
+    if (reduce) k_batch_bgk<L, true><<<dim3(grid), dim3(kBlock), 0, s>>>(solves, items, n_items, per_wg, parity);
+    else k_batch_bgk<L, false><<<dim3(grid), dim3(kBlock), 0, s>>>(solves, items, n_items, per_wg, parity);
+  });
+  return hipGetLastError();
+}
+
+hipError_t batch_sum(const BatchSolve* solves, const int32_t* slots, int32_t n_slots, double* out,
+                     hipStream_t s) {
+  if (n_slots <= 0) return hipSuccess;
+  k_batch_sum<<<dim3(n_slots), dim3(kBlock), 0, s>>>(solves, slots, out);
+  return hipGetLastError();
+}
+
+hipError_t batch_rest_f64(int lattice, double* f, int64_t NP, hipStream_t s) {
+  LB_DISPATCH_LATTICE(lattice, L, {
+    const int64_t n = static_cast<int64_t>(L::nQ) * NP;
+    k_batch_rest<L><<<dim3(static_cast<unsigned>((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s>>>(f, NP);
   });
   return hipGetLastError();
 }

@@ -88,6 +88,22 @@ SIGNATURES = {
                                       C.POINTER(C.POINTER(C.c_double))]),
     "lb_sphere_pack": (C.c_int, [C.c_int32, _i32p, C.c_double, C.c_double, C.c_uint64, C.c_int32,
                                  C.c_int32, C.c_int32, C.c_double, _i32p, _f64p]),
+    "lb_batch_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int32, C.POINTER(_vp)]),
+    "lb_batch_destroy": (C.c_int, [_vp]),
+    "lb_batch_last_error": (C.c_char_p, [_vp]),
+    "lb_batch_footprint": (C.c_int, [C.c_int, C.c_int, C.c_int32, _i32p, _i32p, C.c_int32,
+                                     C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "lb_batch_add": (C.c_int, [_vp, C.c_int32, _i32p, _i32p, C.c_int32, C.c_double, _f64p, C.c_double,
+                               C.c_int64, C.c_int64]),
+    "lb_batch_advance": (C.c_int, [_vp, C.c_int32, C.POINTER(C.c_int32)]),
+    "lb_batch_collect": (C.c_int, [_vp, C.c_int32, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
+                                   _f64p, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"), _f64p,
+                                   _i32p, C.POINTER(C.c_int32)]),
+    "lb_batch_status": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                  C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "lb_batch_set_shape": (C.c_int, [_vp, C.c_int32]),
+    "lb_batch_shape": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
+    "lb_batch_kernel_time": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
 }
 
 
@@ -384,3 +400,91 @@ class Context:
 
     def device_bytes(self):
         return lib().lb_device_bytes(self._h)
+
+
+def batch_footprint(lattice, precision, geo, periodic="x"):
+    """(bytes, fluid nodes) a solve of this geometry takes in a Batch pool."""
+    g = np.ascontiguousarray(geo, dtype=np.int32)
+    dims = np.ascontiguousarray(g.shape, dtype=np.int32)
+    nbytes, nf = C.c_int64(), C.c_int64()
+    _check_global(lib().lb_batch_footprint(lattice, precision, len(dims), dims, g.ravel(), periodic_mask(periodic),
+                                           C.byref(nbytes), C.byref(nf)))
+    return nbytes.value, nf.value
+
+
+class Batch:
+    """Many permeability solves resident in one device pool (lb_batch)."""
+
+    def __init__(self, lattice, precision=FP32S, pool_bytes=1 << 30, check_every=100, device=0):
+        h = _vp()
+        _check_global(lib().lb_batch_create(lattice, precision, device, int(pool_bytes), int(check_every),
+                                            C.byref(h)))
+        self._h = h
+        self.lattice = lattice
+        self.precision = precision
+        self.nD = ND_OF[lattice]
+        self.check_every = int(check_every)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise LBError(rc, lib().lb_batch_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            lib().lb_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, geo, periodic, tau, force, tol, max_steps, tag):
+        """Admit a solve; False when it does not fit into the pool now."""
+        g = np.ascontiguousarray(geo, dtype=np.int32)
+        dims = np.ascontiguousarray(g.shape, dtype=np.int32)
+        F = np.zeros(3)
+        F[: self.nD] = np.asarray(force, dtype=float)[: self.nD]
+        rc = lib().lb_batch_add(self._h, len(dims), dims, g.ravel(), periodic_mask(periodic), float(tau), F,
+                                float(tol), int(max_steps), int(tag))
+        if rc == -7:
+            return False
+        self._check(rc)
+        return True
+
+    def advance(self, intervals=1):
+        n = C.c_int32()
+        self._check(lib().lb_batch_advance(self._h, int(intervals), C.byref(n)))
+        return n.value
+
+    def collect(self, max_n=1 << 16):
+        tags = np.zeros(max_n, dtype=np.int64)
+        k = np.zeros(max_n)
+        steps = np.zeros(max_n, dtype=np.int64)
+        sec = np.zeros(max_n)
+        st = np.zeros(max_n, dtype=np.int32)
+        n = C.c_int32()
+        self._check(lib().lb_batch_collect(self._h, int(max_n), tags, k, steps, sec, st, C.byref(n)))
+        m = n.value
+        return [{"tag": int(tags[i]), "k": float(k[i]), "steps": int(steps[i]), "seconds": float(sec[i]),
+                 "status": int(st[i])} for i in range(m)]
+
+    def status(self):
+        a, b, c, d, e = C.c_int32(), C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        self._check(lib().lb_batch_status(self._h, C.byref(a), C.byref(b), C.byref(c), C.byref(d), C.byref(e)))
+        return {"resident": a.value, "resident_nodes": b.value, "bytes_used": c.value, "bytes_free": d.value,
+                "largest_free": e.value}
+
+    def set_shape(self, wg_per_cu):
+        self._check(lib().lb_batch_set_shape(self._h, int(wg_per_cu)))
+
+    def shape(self):
+        v = C.c_int32()
+        self._check(lib().lb_batch_shape(self._h, C.byref(v)))
+        return v.value
+
+    def kernel_time(self):
+        ms, n, l = C.c_double(), C.c_int64(), C.c_int64()
+        self._check(lib().lb_batch_kernel_time(self._h, C.byref(ms), C.byref(n), C.byref(l)))
+        return ms.value, n.value, l.value

@@ -1,0 +1,112 @@
+"""Serving form: continuous batching of permeability requests on one GPU.
+
+A Server owns one Batch pool (lb_batch).  Requests wait in arrival order;
+between check intervals every waiting request whose footprint fits into the
+pool's free memory is admitted (first fit over the queue, so one large
+request does not hold back smaller ones behind it), all resident solves are
+advanced together, and finished solves are returned with their latency
+(completion time - arrival time).  State stays resident until convergence.
+"""
+import time
+from collections import deque
+
+from .ffi import D3Q19, FP32S, Batch, batch_footprint
+
+
+class Request:
+    __slots__ = ("tag", "geo", "periodic", "tau", "force", "tol", "max_steps", "t_arrival", "bytes", "kind")
+
+    def __init__(self, tag, geo, periodic="x", tau=1.0, force=(1e-6, 0.0, 0.0), tol=1e-6, max_steps=100000,
+                 t_arrival=None, kind="perm"):
+        self.tag = tag
+        self.geo = geo
+        self.periodic = periodic
+        self.tau = tau
+        self.force = force
+        self.tol = tol
+        self.max_steps = max_steps
+        self.t_arrival = t_arrival
+        self.bytes = None
+        self.kind = kind
+
+
+class Server:
+    def __init__(self, pool_bytes, lattice=D3Q19, precision=FP32S, check_every=100, device=0, lookahead=64):
+        self.batch = Batch(lattice, precision, pool_bytes, check_every, device)
+        self.lattice, self.precision = lattice, precision
+        self.pool_bytes = pool_bytes
+        self.lookahead = lookahead
+        self.waiting = deque()
+        self.resident = {}
+        self.done = []
+        self.clock = time.perf_counter
+
+    def submit(self, req):
+        if req.t_arrival is None:
+            req.t_arrival = self.clock()
+        if req.bytes is None:
+            req.bytes = batch_footprint(self.lattice, self.precision, req.geo, req.periodic)[0]
+        if req.bytes > self.pool_bytes:
+            raise ValueError("request %r needs %d bytes, more than the whole pool" % (req.tag, req.bytes))
+        self.waiting.append(req)
+
+    def admit(self):
+        """Admit waiting requests that fit, in arrival order, looking past
+        ones that do not fit yet (at most `lookahead` deep)."""
+        free = self.batch.status()["largest_free"]
+        kept = deque()
+        n = 0
+        while self.waiting:
+            r = self.waiting.popleft()
+            if n < self.lookahead and r.bytes <= free and self.batch.add(
+                    r.geo, r.periodic, r.tau, r.force, r.tol, r.max_steps, r.tag):
+                r.geo = None  # the device holds the solve now
+                self.resident[r.tag] = r
+                free = self.batch.status()["largest_free"]
+            else:
+                kept.append(r)
+            n += 1
+        self.waiting = kept
+
+    def step(self, intervals=1):
+        """Admit, advance all resident solves, and collect finished ones."""
+        self.admit()
+        if not self.resident:
+            return []
+        self.batch.advance(intervals)
+        now = self.clock()
+        out = []
+        for r in self.batch.collect():
+            req = self.resident.pop(r["tag"])
+            r["latency"] = now - req.t_arrival
+            r["kind"] = req.kind
+            out.append(r)
+        self.done.extend(out)
+        return out
+
+    def idle(self):
+        return not self.waiting and not self.resident
+
+    def close(self):
+        self.batch.close()
+
+
+def run_stream(server, requests, poll_sleep=1e-3):
+    """Replay requests (with t_arrival offsets in seconds from the start) in
+    real time through the server; returns (results, wall seconds)."""
+    pending = deque(sorted(requests, key=lambda r: r.t_arrival))
+    t0 = server.clock()
+    offsets = {}
+    while pending or not server.idle():
+        now = server.clock() - t0
+        while pending and pending[0].t_arrival <= now:
+            r = pending.popleft()
+            offsets[r.tag] = r.t_arrival
+            r.t_arrival = t0 + r.t_arrival
+            server.submit(r)
+        if server.idle():
+            if pending:
+                time.sleep(min(poll_sleep, max(0.0, pending[0].t_arrival - (server.clock() - t0))))
+            continue
+        server.step(1)
+    return server.done, server.clock() - t0

@@ -15,6 +15,15 @@ permeability = k of both forms run to |dk|/k < 1e-7 (checked every 100
            steps), and the relative error of FP32S against FP64.
 cpu_baseline = the oracle's CPU restatement of the reference loop on the same
            node list, on the host's cores, for a bounded number of steps.
+batched  = the ensemble of independent sphere-pack permeability solves (cubic
+           sizes log-uniform in 32..128, porosity U(0.30, 0.50), radius
+           U(3, 8), one seed; |dk|/k < 1e-6 checked every 100 steps), all
+           resident in one pool and advanced together in FP32S: solves/s over
+           the whole ensemble (wall clock from the first admission to the last
+           result, host set-up of each solve included), against the plain FP64
+           path run one solve at a time (lb_solve_permeability on its own
+           context), timed on a size-stratified subset (every m-th solve by
+           fluid count) and scaled by m; k error of FP32S vs FP64 on that subset.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256]
 """
@@ -126,6 +135,84 @@ def cpu_baseline(nb, types, bulk, n_nodes, tau, force, budget_s=15.0):
                       "%d steps, %d OpenMP threads" % (steps, threads)}
 
 
+def ensemble(n, seed=2024):
+    """Parameters and geometries of the batched ensemble."""
+    from concurrent.futures import ThreadPoolExecutor
+    rng = np.random.default_rng(seed)
+    L = np.rint(np.exp(rng.uniform(np.log(32), np.log(128), n))).astype(int)
+    phi = rng.uniform(0.30, 0.50, n)
+    rad = rng.uniform(3.0, 8.0, n)
+    seeds = rng.integers(1, 2**62, n)
+
+    def one(i):
+        geo, info = lbamd.sphere_pack((L[i],) * 3, rad[i], phi[i], int(seeds[i]), periodic="x", walls=True)
+        return geo
+
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        geos = list(ex.map(one, range(n)))
+    return geos, {"L": L, "porosity": phi, "radius": rad}
+
+
+def batched_leg(n, every, tau, force):
+    t0 = time.perf_counter()
+    geos, par = ensemble(n)
+    log("ensemble: %d samples, %.1f M cells, generated in %.1f s" % (n, sum(g.size for g in geos) / 1e6,
+                                                                       time.perf_counter() - t0))
+    fps = [lbamd.batch_footprint(lbamd.D3Q19, lbamd.FP32S, g, "x") for g in geos]
+    pool = int(sum(f[0] for f in fps) * 1.01) + (64 << 20)
+    nfluid = np.array([f[1] for f in fps])
+    b = lbamd.Batch(lbamd.D3Q19, lbamd.FP32S, pool, 100)
+    t0 = time.perf_counter()
+    for i, g in enumerate(geos):
+        assert b.add(g, "x", tau, force, 1e-6, 200000, i)
+    t_adm = time.perf_counter() - t0
+    res = {}
+    while len(res) < n:
+        b.advance(1)
+        for r in b.collect():
+            res[r["tag"]] = r
+    wall = time.perf_counter() - t0
+    kms, launches, lups = b.kernel_time()
+    shape = b.shape()
+    b.close()
+    log("batched FP32S: %d solves in %.2f s (admission %.2f s), %.1f GLUPS device, shape %d" %
+        (n, wall, t_adm, lups / (kms * 1e-3) / 1e9, shape))
+    # plain FP64 one at a time on a size-stratified subset
+    order = np.argsort(nfluid, kind="stable")
+    subset = order[every // 2::every]
+    t_one = 0.0
+    errs = []
+    for i in subset:
+        g = geos[i]
+        topo = lbamd.Topology.from_geometry(g, lbamd.D3Q19, "x")
+        nb, types, bulk, pos = topo.neighbors(), topo.node_types(), topo.bulk(), topo.positions()
+        ctx = make_ctx(lbamd.FP64, nb, types, bulk, pos, "rows")
+        w = np.array([2.0] * 3 + [1.0] * 6 + [2.0] * 3 + [1.0] * 6 + [12.0]) / 36.0
+        f0 = np.zeros((topo.n_nodes, 19))
+        f0[bulk] = w[None, :]
+        ctx.upload_f(f0)
+        ctx.sync()
+        t1 = time.perf_counter()
+        k64, s64 = ctx.solve_permeability(tau, force, 100, 1e-6, 200000, float(g.size))
+        t_one += time.perf_counter() - t1
+        ctx.close()
+        errs.append(abs(res[int(i)]["k"] - k64) / abs(k64))
+    est_one = t_one * n / len(subset)
+    steps = np.array([res[i]["steps"] for i in range(n)])
+    out = {"n_solves": n, "solves_per_s": n / wall, "wall_s": wall, "admission_s": t_adm,
+           "device_GLUPS": lups / (kms * 1e-3) / 1e9, "lups": int(lups), "launch_shape_wg_per_cu": shape,
+           "steps_median": int(np.median(steps)), "steps_max": int(steps.max()),
+           "fluid_nodes": int(nfluid.sum()), "pool_GB": pool / 1e9,
+           "fp64_one_at_a_time_solves_per_s": n / est_one, "fp64_subset": int(len(subset)),
+           "speedup_vs_fp64_one_at_a_time": est_one / wall,
+           "k_rel_err_fp32s_vs_fp64_max": float(max(errs)), "tol_dk_over_k": 1e-6,
+           "converged": int(sum(1 for r in res.values() if r["status"] == 0))}
+    log("FP64 one at a time (subset of %d, scaled): %.2f solves/s; batched %.2f solves/s (x%.2f); "
+        "max k err %.2e" % (len(subset), out["fp64_one_at_a_time_solves_per_s"], out["solves_per_s"],
+                            out["speedup_vs_fp64_one_at_a_time"], out["k_rel_err_fp32s_vs_fp64_max"]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +222,8 @@ def main():
     ap.add_argument("--layout", choices=["rows", "table"], default="rows")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-perm", action="store_true")
+    ap.add_argument("--batch", type=int, default=512, help="ensemble size (0: skip)")
+    ap.add_argument("--baseline-every", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box here has one GPU and the engine runs one "
@@ -177,6 +266,8 @@ def main():
                 "k_rel_err_fp32s": abs(k32 - k64) / abs(k64), "tol_dk_over_k": 1e-7,
                 "definition": "k = nu * sum_bulk(sum_q f c_x + F_x/2) / (N_box * F_x), nu = (tau-1/2)/3"}
 
+    batched = batched_leg(args.batch, args.baseline_every, tau, force) if args.batch > 0 else None
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(nb, types, bulk, topo.n_nodes, tau, force)
@@ -207,6 +298,7 @@ def main():
                  "achieved_GBs": round(ach64, 1), "frac": round(ach64 / HBM_PEAK_GBS, 4),
                  "bytes_per_lup": bytes64, "traffic": traffic64},
         "permeability": perm,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

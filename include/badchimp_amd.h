@@ -233,6 +233,52 @@ int lb_sphere_pack(int32_t nd, const int32_t* dims, double radius,
                    int32_t walls, int32_t max_fail, double relax,
                    int32_t* geo, double* info);
 
+/* ---- batched and serving form -------------------------------------------
+ * No counterpart in the reference: the reference runs one solve per process
+ * (std_case/main.cpp).  A batch keeps many independent single-phase
+ * permeability solves (the std_case step with lb_solve_permeability's stop)
+ * resident in one pre-allocated device pool of pool_bytes and advances all of
+ * them with one launch per step.  A solve is admitted when its footprint
+ * (populations x 2, coordinates, row-window table, block partials) fits into
+ * the pool; it retires when |dk| <= tol |k| between checks every check_every
+ * steps (or at max_steps) and frees its memory.  Each solve's per-node
+ * arithmetic and reduction order are those of a single context with row
+ * windows, so its k and step count are the same alone, in any mix, and
+ * through lb_solve_permeability. */
+typedef struct lb_batch lb_batch;
+int lb_batch_create(int lattice, int precision, int device, int64_t pool_bytes,
+                    int32_t check_every, lb_batch** out);
+int lb_batch_destroy(lb_batch* b);
+const char* lb_batch_last_error(const lb_batch* b);
+/* Bytes a solve of this geometry (int32 cells, > 0 fluid, C order over
+ * x, y[, z]) takes in a pool, and its fluid node count. */
+int lb_batch_footprint(int lattice, int precision, int32_t nd, const int32_t* dims,
+                       const int32_t* geo, int32_t periodic_mask, int64_t* bytes,
+                       int64_t* n_fluid);
+/* Admit a solve that starts at rest (f = w, rho = 1): LB_ERR_NOMEM when it
+ * does not fit now (retry after others retire).  n_box = prod(dims). */
+int lb_batch_add(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* geo,
+                 int32_t periodic_mask, double tau, const double* force, double tol,
+                 int64_t max_steps, int64_t tag);
+/* Advance every resident solve by `intervals` check intervals; solves that
+ * finish retire at the end of their interval.  Admissions take effect at the
+ * next interval. */
+int lb_batch_advance(lb_batch* b, int32_t intervals, int32_t* n_finished);
+/* Take up to max_n finished solves: tag, k, steps, seconds resident, status
+ * (0 converged, 1 max_steps reached, 2 k not finite). */
+int lb_batch_collect(lb_batch* b, int32_t max_n, int64_t* tags, double* k, int64_t* steps,
+                     double* seconds, int32_t* status, int32_t* n_out);
+int lb_batch_status(const lb_batch* b, int32_t* n_resident, int64_t* resident_nodes,
+                    int64_t* bytes_used, int64_t* bytes_free, int64_t* largest_free);
+/* Launch shape: -1 autotune on live steps after each change of the resident
+ * set (default), 0 one workgroup per 256-node block, k > 0 a persistent launch
+ * of k workgroups per CU. */
+int lb_batch_set_shape(lb_batch* b, int32_t wg_per_cu);
+int lb_batch_shape(const lb_batch* b, int32_t* wg_per_cu);
+/* Device time of the step launches and sums (HIP events), launches, and
+ * lattice updates done. */
+int lb_batch_kernel_time(const lb_batch* b, double* ms, int64_t* launches, int64_t* lups);
+
 #ifdef __cplusplus
 }
 #endif

@@ -79,3 +79,37 @@ def test_topology_roundtrip_through_writer(tmp_path):
         assert np.array_equal(getattr(t, m)(), getattr(r, m)())
     got = r.attribute("init_rho")
     assert np.array_equal(got[1:], rho[1:])
+
+
+def test_batch_footprint_without_gpu():
+    """The footprint of a batched solve is host arithmetic: fluid count and
+    the sum of 256-byte aligned regions (2 population buffers, coordinates,
+    row windows, block partials)."""
+    geo, _ = lbamd.sphere_pack((20, 18, 16), 3.0, 0.5, 4, periodic="x", walls=True)
+    nbytes, nf = lbamd.batch_footprint(lbamd.D3Q19, lbamd.FP32S, geo, "x")
+    assert nf == int((geo > 0).sum())
+    NP = -(-nf // 256) * 256
+    a = lambda v: -(-v // 256) * 256
+    E = [20, 16, 14]  # periodic x spans the box; y, z span the fluid cells inside the walls
+    nwin = (E[0] + 2) * (E[1] + 2) * (-(-E[2] // 32))
+    assert nbytes == 2 * a(19 * NP * 4) + a(NP * 4) + a(nwin * 16) + a(-(-nf // 256) * 8)
+    nb64, _ = lbamd.batch_footprint(lbamd.D3Q19, lbamd.FP64, geo, "x")
+    assert nb64 - nbytes == 2 * (a(19 * NP * 8) - a(19 * NP * 4))
+
+
+def test_batch_create_without_gpu_fails_loudly():
+    if lbamd.lib().lb_batch_create is None:
+        pytest.skip("symbol missing")
+    import ctypes as C
+    h = C.c_void_p()
+    rc = lbamd.lib().lb_batch_create(lbamd.D3Q19, lbamd.FP32S, 0, 1 << 20, 100, C.byref(h))
+    try:
+        import torch
+        has_gpu = torch.cuda.is_available()
+    except Exception:
+        has_gpu = False
+    if has_gpu:
+        assert rc == 0
+        lbamd.lib().lb_batch_destroy(h)
+    else:
+        assert rc != 0 and b"device" in lbamd.lib().lb_global_error()

@@ -1,0 +1,112 @@
+"""Batched form: many permeability solves resident in one pool.
+
+Bars: every solve's k and step count are bit-identical to the single-context
+path (lb_solve_permeability with row windows, same precision), alone in a
+batch and in any mix, whatever launch shape the batch uses; admission follows
+the pool's free memory.  FP64 batch results therefore also carry the
+single path's parity with the reference loop.
+"""
+import numpy as np
+import pytest
+
+import lbamd
+
+pytestmark = pytest.mark.gpu
+
+TAU, FORCE = 1.0, np.array([1e-6, 0.0, 0.0])
+
+
+def pack(shape, radius, porosity, seed):
+    geo, _ = lbamd.sphere_pack(shape, radius, porosity, seed, periodic="x", walls=True)
+    return geo
+
+
+def single(geo, precision, check_every, tol, max_steps):
+    topo = lbamd.Topology.from_geometry(geo, lbamd.D3Q19, "x")
+    nb, types, bulk, pos = topo.neighbors(), topo.node_types(), topo.bulk(), topo.positions()
+    ctx = lbamd.Context(lbamd.D3Q19, precision)
+    ctx.set_topology_grid(nb, types, bulk, pos)
+    assert ctx.layout == 2, ctx.layout_note
+    W = np.array([2.0] * 3 + [1.0] * 6 + [2.0] * 3 + [1.0] * 6 + [12.0]) / 36.0
+    f0 = np.zeros((topo.n_nodes, 19))
+    f0[bulk] = W[None, :]
+    ctx.upload_f(f0)
+    k, steps = ctx.solve_permeability(TAU, FORCE, check_every, tol, max_steps, float(geo.size))
+    ctx.close()
+    return k, steps
+
+
+def run_batch(geos, precision, check_every, tol, max_steps, shape=-1, pool=1 << 28, late=None):
+    """Admit geos (tag = index); `late` maps an interval number to indices that
+    are admitted only then.  Returns {tag: (k, steps)}."""
+    b = lbamd.Batch(lbamd.D3Q19, precision, pool, check_every)
+    b.set_shape(shape)
+    late = late or {}
+    held = {i for v in late.values() for i in v}
+    for i, g in enumerate(geos):
+        if i not in held:
+            assert b.add(g, "x", TAU, FORCE, tol, max_steps, i)
+    out = {}
+    interval = 0
+    while len(out) < len(geos):
+        for i in late.get(interval, []):
+            assert b.add(geos[i], "x", TAU, FORCE, tol, max_steps, i)
+        b.advance(1)
+        interval += 1
+        for r in b.collect():
+            out[r["tag"]] = (r["k"], r["steps"], r["status"])
+        assert interval < 10 * max_steps // check_every
+    b.close()
+    return out
+
+
+@pytest.fixture(scope="module")
+def geos():
+    return [pack((24, 20, 20), 3.0, 0.45, 1), pack((32, 28, 24), 4.0, 0.40, 2), pack((20, 24, 36), 3.0, 0.5, 3)]
+
+
+@pytest.mark.parametrize("precision", [lbamd.FP64, lbamd.FP32S])
+def test_batch_equals_single_context(gpu_available, geos, precision):
+    got = run_batch(geos, precision, 50, 1e-6, 20000)
+    for i, g in enumerate(geos):
+        k, steps = single(g, precision, 50, 1e-6, 20000)
+        assert got[i][2] == 0
+        assert got[i][0] == k and got[i][1] == steps, (i, got[i], k, steps)
+
+
+def test_batch_alone_equals_mix_and_shapes(gpu_available, geos):
+    extra = [pack((28, 28, 28), 3.5, 0.42, 9), pack((16, 16, 16), 2.5, 0.5, 10)]
+    alone = run_batch([geos[0]], lbamd.FP32S, 50, 1e-6, 20000)[0]
+    # in a mix, admitted later than the others, under each launch shape
+    for shape in (-1, 0, 1, 3):
+        mix = run_batch(extra + [geos[0], geos[1]], lbamd.FP32S, 50, 1e-6, 20000, shape=shape, late={3: [2]})
+        assert mix[2] == alone, (shape, mix[2], alone)
+
+
+def test_admission_by_footprint(gpu_available, geos):
+    sizes = [lbamd.batch_footprint(lbamd.D3Q19, lbamd.FP32S, g, "x")[0] for g in geos]
+    pool = sizes[0] + sizes[1] + 1024
+    b = lbamd.Batch(lbamd.D3Q19, lbamd.FP32S, pool, 50)
+    assert b.add(geos[0], "x", TAU, FORCE, 1e-6, 20000, 0)
+    assert b.add(geos[1], "x", TAU, FORCE, 1e-6, 20000, 1)
+    st = b.status()
+    assert st["resident"] == 2 and st["bytes_used"] == sizes[0] + sizes[1]
+    assert not b.add(geos[2], "x", TAU, FORCE, 1e-6, 20000, 2)
+    done = []
+    while not done:
+        b.advance(1)
+        done = b.collect()
+    # a retired solve frees its region; the waiting one fits if the freed
+    # memory suffices
+    freed = sum(sizes[r["tag"]] for r in done)
+    assert b.status()["bytes_used"] == sizes[0] + sizes[1] - freed
+    if freed + 1024 >= sizes[2]:
+        assert b.add(geos[2], "x", TAU, FORCE, 1e-6, 20000, 2)
+    b.close()
+
+
+def test_fp16s_batch_error(gpu_available, geos):
+    ref = run_batch(geos, lbamd.FP64, 50, 1e-6, 20000)
+    got = run_batch(geos, lbamd.FP16S, 50, 1e-6, 20000)
+    for i in ref:
+        assert abs(got[i][0] - ref[i][0]) / ref[i][0] < 2e-2
