@@ -184,6 +184,38 @@ struct RowWin {
   }
 };
 
+// RowWin::links: the source node of every direction: bit q of fluid is set
+// when the source cell of direction q is the fluid node j[q] (else a solid,
+// and j[q] is undefined).
+template <class L>
+__device__ __forceinline__ void rowwin_links(const RowWin& ad, int32_t i, uint32_t (&j)[L::nQ - 1], uint32_t& fluid) {
+  const uint32_t c = ad.coord[i];
+  const uint32_t X = c >> 22, Y = (c >> 12) & 1023u, Z = c & 4095u;
+  const uint32_t w = Z >> 5;
+  const uint32_t t = Z - 32u * w + 16u;
+  const uint32_t base = ((X + 1u) * static_cast<uint32_t>(ad.PY) + (Y + 1u)) * static_cast<uint32_t>(ad.W) + w;
+  constexpr int NY = L::nD == 3 ? 3 : 1;
+  uint4 e[3 * NY];
+#pragma unroll
+  for (int k = 0; k < 3 * NY; ++k) {
+    const int dx = k / NY - 1, dy = NY > 1 ? k % NY - 1 : 0;
+    const uint4* __restrict__ wk = ad.win + (dx * ad.PY + dy) * ad.W;
+    e[k] = *reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(wk) + base * 16u);
+  }
+  const uint64_t below = (1ull << t) - 1ull;
+  fluid = 0u;
+#pragma unroll
+  for (int q = 0; q < L::nQ - 1; ++q) {
+    const int k = (1 - cx<L>(q)) * NY + (NY > 1 ? 1 - cy<L>(q) : 0);
+    const int dz = -cz<L>(q);
+    const uint64_t m = (static_cast<uint64_t>(e[k].z) << 32) | e[k].y;
+    const uint32_t b0 = e[k].x + static_cast<uint32_t>(__popcll(m & below));
+    const uint32_t bm = static_cast<uint32_t>(m >> (t - 1u)) & 7u;
+    fluid |= ((bm >> (dz + 1)) & 1u) << q;
+    j[q] = dz == 0 ? b0 : (dz < 0 ? b0 - 1u : b0 + ((bm >> 1) & 1u));
+  }
+}
+
 // Element access of a population buffer: 32-bit byte offsets from the base
 // when the layout guarantees the buffer is below 4 GiB (global loads with a
 // scalar base and a 32-bit vector offset), 64-bit addresses otherwise.

@@ -100,30 +100,35 @@ struct RangeAlloc {
   }
 };
 
-// Sizes of one solve's device region.
+// Sizes of one solve's device region (nf = 1 single-phase, 2 two-phase).
 struct Footprint {
   int64_t n = 0, NP = 0, nblk = 0, nwin = 0;
-  int64_t off_f0 = 0, off_f1 = 0, off_coord = 0, off_win = 0, off_part = 0, bytes = 0;
+  int nf = 1;
+  int64_t off_f0 = 0, off_f1 = 0, off_coord = 0, off_win = 0, off_part = 0, off_phi = 0, bytes = 0;
 };
 
-Footprint footprint_of(int Q, size_t esz, int64_t n, int64_t nwin) {
+Footprint footprint_of(int Q, size_t esz, int64_t n, int64_t nwin, int nf) {
   Footprint fp;
   fp.n = n;
+  fp.nf = nf;
   fp.NP = std::max<int64_t>((n + kBlock - 1) / kBlock, 1) * kBlock;
   fp.nblk = std::max<int64_t>((n + kBlock - 1) / kBlock, 1);
   fp.nwin = nwin;
-  const int64_t fb = align_up(static_cast<int64_t>(Q) * fp.NP * static_cast<int64_t>(esz));
+  const int64_t fb = align_up(static_cast<int64_t>(nf) * Q * fp.NP * static_cast<int64_t>(esz));
   fp.off_f0 = 0;
   fp.off_f1 = fb;
   fp.off_coord = 2 * fb;
   fp.off_win = fp.off_coord + align_up(fp.NP * 4);
   fp.off_part = fp.off_win + align_up(nwin * 16);
-  fp.bytes = fp.off_part + align_up(fp.nblk * 8);
+  fp.off_phi = fp.off_part + align_up((nf == 2 ? 3 : 1) * fp.nblk * 8);
+  fp.bytes = fp.off_phi + (nf == 2 ? align_up(fp.NP * 4) : 0);
   return fp;
 }
 
 struct Slot {
   bool used = false;
+  int kind = 0;  // 0 single-phase, 1 two-phase
+  double nu0 = 0.0, nu1 = 0.0, k0 = 0.0, k1 = 0.0, k0_prev = 0.0, k1_prev = 0.0;
   int64_t tag = 0;
   int64_t off = 0;
   Footprint fp;
@@ -136,7 +141,7 @@ struct Slot {
 
 struct Result {
   int64_t tag, steps;
-  double k, seconds;
+  double k, k0, k1, seconds;
   int32_t status;  // 0 converged, 1 max_steps reached
 };
 
@@ -161,7 +166,8 @@ struct lb_batch {
   int32_t* d_slots = nullptr;
   double* d_out = nullptr;
   double* h_out = nullptr;  // pinned
-  std::vector<uint32_t> items;
+  std::vector<uint32_t> items;  // single-phase items, then two-phase items
+  int32_t n_items_bgk = 0, n_items_cg = 0;
   std::vector<int32_t> active;
   bool dirty = true;
   int64_t global_step = 0;
@@ -214,14 +220,25 @@ void shape_dims(const lb_batch* b, int32_t shape, int32_t n_items, int32_t* grid
 }
 
 hipError_t launch_step(lb_batch* b, bool reduce, int32_t shape) {
-  const int32_t n = static_cast<int32_t>(b->items.size());
-  int32_t grid, per;
-  shape_dims(b, shape, n, &grid, &per);
   const int32_t parity = static_cast<int32_t>(b->global_step & 1);
-  if (b->precision == LB_FP64)
-    return batch_bgk_f64(b->lattice, reduce, b->d_solves, b->d_items, n, grid, per, parity, b->stream);
-  return batch_bgk_lowp(b->precision, b->lattice, reduce, b->d_solves, b->d_items, n, grid, per, parity,
-                        b->stream);
+  int32_t grid, per;
+  hipError_t e = hipSuccess;
+  if (b->n_items_bgk > 0) {
+    const int32_t n = b->n_items_bgk;
+    shape_dims(b, shape, n, &grid, &per);
+    if (b->precision == LB_FP64)
+      e = batch_bgk_f64(b->lattice, reduce, b->d_solves, b->d_items, n, grid, per, parity, b->stream);
+    else
+      e = batch_bgk_lowp(b->precision, b->lattice, reduce, b->d_solves, b->d_items, n, grid, per, parity,
+                         b->stream);
+  }
+  if (e == hipSuccess && b->n_items_cg > 0) {
+    const int32_t n = b->n_items_cg;
+    shape_dims(b, shape, n, &grid, &per);
+    e = batch_cg_lowp(b->precision, b->lattice, reduce, b->d_solves, b->d_items + b->n_items_bgk, n, grid, per,
+                      parity, b->stream);
+  }
+  return e;
 }
 
 // rebuild the item and slot lists after admissions / retirements
@@ -229,14 +246,18 @@ int refresh(lb_batch* b) {
   b->items.clear();
   b->active.clear();
   b->resident_nodes = 0;
-  for (int32_t s = 0; s < static_cast<int32_t>(b->slots.size()); ++s) {
-    const Slot& sl = b->slots[s];
-    if (!sl.used) continue;
-    b->active.push_back(s);
-    b->resident_nodes += sl.fp.n;
-    for (int64_t k = 0; k < sl.fp.nblk; ++k)
-      b->items.push_back((static_cast<uint32_t>(s) << kItemShift) | static_cast<uint32_t>(k));
+  for (int kind = 0; kind < 2; ++kind) {
+    for (int32_t s = 0; s < static_cast<int32_t>(b->slots.size()); ++s) {
+      const Slot& sl = b->slots[s];
+      if (!sl.used || sl.kind != kind) continue;
+      b->active.push_back(s);
+      b->resident_nodes += sl.fp.n;
+      for (int64_t k = 0; k < sl.fp.nblk; ++k)
+        b->items.push_back((static_cast<uint32_t>(s) << kItemShift) | static_cast<uint32_t>(k));
+    }
+    if (kind == 0) b->n_items_bgk = static_cast<int32_t>(b->items.size());
   }
+  b->n_items_cg = static_cast<int32_t>(b->items.size()) - b->n_items_bgk;
   if (static_cast<int64_t>(b->items.size()) > b->items_cap) {
     if (b->d_items) hipFree(b->d_items);
     b->d_items = nullptr;
@@ -327,7 +348,7 @@ int run_interval(lb_batch* b, bool tune) {
   hipError_t e = batch_sum(b->d_solves, b->d_slots, static_cast<int32_t>(b->active.size()), b->d_out, b->stream);
   if (e != hipSuccess) return bfail(b, LB_ERR_HIP, "batch sum: %s", hipGetErrorString(e));
   hipEventRecord(b->ev1, b->stream);
-  LB_BHIP(b, hipMemcpyAsync(b->h_out, b->d_out, b->slots.size() * sizeof(double), hipMemcpyDeviceToHost,
+  LB_BHIP(b, hipMemcpyAsync(b->h_out, b->d_out, 3 * b->slots.size() * sizeof(double), hipMemcpyDeviceToHost,
                             b->stream));
   LB_BHIP(b, hipStreamSynchronize(b->stream));
   float ms = 0.f;
@@ -340,7 +361,7 @@ int run_interval(lb_batch* b, bool tune) {
 
 void retire(lb_batch* b, int32_t s, int32_t status) {
   Slot& sl = b->slots[s];
-  b->done.push_back(Result{sl.tag, sl.steps, sl.k, now_s() - sl.t_admit, status});
+  b->done.push_back(Result{sl.tag, sl.steps, sl.k, sl.k0, sl.k1, now_s() - sl.t_admit, status});
   b->alloc.release(sl.off, sl.fp.bytes);
   sl.used = false;
   b->dirty = true;
@@ -380,8 +401,8 @@ int lb_batch_create(int lattice, int precision, int device, int64_t pool_bytes, 
   if (e == hipSuccess) e = hipMalloc(&b->pool, static_cast<size_t>(pool_bytes));
   if (e == hipSuccess) e = hipMalloc(&b->d_solves, kMaxSlots * sizeof(BatchSolve));
   if (e == hipSuccess) e = hipMalloc(&b->d_slots, kMaxSlots * sizeof(int32_t));
-  if (e == hipSuccess) e = hipMalloc(&b->d_out, kMaxSlots * sizeof(double));
-  if (e == hipSuccess) e = hipHostMalloc(&b->h_out, kMaxSlots * sizeof(double), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc(&b->d_out, 3 * kMaxSlots * sizeof(double));
+  if (e == hipSuccess) e = hipHostMalloc(&b->h_out, 3 * kMaxSlots * sizeof(double), hipHostMallocDefault);
   if (e != hipSuccess) {
     const int rc = fail_global(e == hipErrorOutOfMemory ? LB_ERR_NOMEM : LB_ERR_HIP, "lb_batch_create: %s",
                                hipGetErrorString(e));
@@ -413,32 +434,44 @@ int lb_batch_destroy(lb_batch* b) {
 
 const char* lb_batch_last_error(const lb_batch* b) { return b ? b->err.c_str() : lb_global_error(); }
 
-int lb_batch_footprint(int lattice, int precision, int32_t nd, const int32_t* dims, const int32_t* geo,
-                       int32_t periodic_mask, int64_t* bytes, int64_t* n_fluid) {
+int lb_batch_footprint(int lattice, int precision, int32_t n_fields, int32_t nd, const int32_t* dims,
+                       const int32_t* geo, int32_t periodic_mask, int64_t* bytes, int64_t* n_fluid) {
   const int want = lattice == LB_D2Q9 ? 2 : 3;
   if ((lattice != LB_D2Q9 && lattice != LB_D3Q19) || nd != want || !dims || !geo)
     return fail_global(LB_ERR_ARG, "lb_batch_footprint: bad argument");
   if (precision != LB_FP64 && precision != LB_FP32S && precision != LB_FP16S)
     return fail_global(LB_ERR_ARG, "lb_batch_footprint: bad precision");
+  if (n_fields != 1 && n_fields != 2) return fail_global(LB_ERR_ARG, "lb_batch_footprint: n_fields must be 1 or 2");
   RowBuild rb;
   std::string why;
   if (!rows_from_geometry(lattice, dims, geo, periodic_mask, rb, why))
     return fail_global(LB_ERR_UNSUPPORTED, "lb_batch_footprint: %s", why.c_str());
   const int64_t nwin = static_cast<int64_t>(rb.g.E[0] + 2) * (rb.g.E[1] + 2) * rb.g.W;
   const Footprint fp = footprint_of(lattice == LB_D2Q9 ? 9 : 19, elem_bytes(precision),
-                                    static_cast<int64_t>(rb.coord.size()), nwin);
+                                    static_cast<int64_t>(rb.coord.size()), nwin, n_fields);
   if (bytes) *bytes = fp.bytes;
   if (n_fluid) *n_fluid = fp.n;
   return LB_OK;
 }
 
-int lb_batch_add(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* geo, int32_t periodic_mask,
-                 double tau, const double* force, double tol, int64_t max_steps, int64_t tag) {
-  if (!b) return LB_ERR_ARG;
-  if (nd != b->nD || !dims || !geo || !force || !(tau > 0.5) || !(tol > 0.0) || max_steps < 1 || force[0] == 0.0)
-    return bfail(b, LB_ERR_ARG, "lb_batch_add: bad argument");
-  if (b->dirty == false && b->active.size() >= static_cast<size_t>(kMaxSlots))
-    return bfail(b, LB_ERR_NOMEM, "lb_batch_add: all %d slots in use", kMaxSlots);
+}  // extern "C"
+
+namespace {
+
+// Parameters of a solve being admitted.
+struct AddArgs {
+  int kind = 0;                 // 0 single-phase, 1 two-phase
+  double tau = 1.0;             // single-phase
+  double tau0 = 1.0, tau1 = 1.0, sigma = 0.0, beta = 0.0, wett = 1.0;  // two-phase
+  const double* rho0 = nullptr;  // two-phase: phase-0 density per geometry cell
+  const double* force = nullptr;
+  double tol = 0.0;
+  int64_t max_steps = 0;
+  int64_t tag = 0;
+};
+
+int add_solve(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* geo, int32_t periodic_mask,
+              const AddArgs& p) {
   hipSetDevice(b->device);
   RowBuild rb;
   std::string why;
@@ -457,8 +490,9 @@ int lb_batch_add(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* ge
     if (!((m >> t) & 1u) || e.x + static_cast<uint32_t>(__builtin_popcountll(m & ((1ull << t) - 1ull))) != i)
       return bfail(b, LB_ERR_STATE, "lb_batch_add: row windows inconsistent at node %lld", (long long)i);
   }
-  const Footprint fp = footprint_of(b->Q, b->esz, n, static_cast<int64_t>(win.size()));
-  if (static_cast<uint64_t>(b->Q) * static_cast<uint64_t>(fp.NP) * b->esz >= (1ull << 32) ||
+  const int nf = p.kind == 1 ? 2 : 1;
+  const Footprint fp = footprint_of(b->Q, b->esz, n, static_cast<int64_t>(win.size()), nf);
+  if (static_cast<uint64_t>(nf) * b->Q * static_cast<uint64_t>(fp.NP) * b->esz >= (1ull << 32) ||
       fp.nblk > static_cast<int64_t>(kItemMask) + 1)
     return bfail(b, LB_ERR_UNSUPPORTED, "lb_batch_add: %lld fluid nodes exceed one solve", (long long)n);
   int32_t slot = -1;
@@ -484,26 +518,85 @@ int lb_batch_add(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* ge
   d.f[0] = base + fp.off_f0;
   d.f[1] = base + fp.off_f1;
   d.partial = reinterpret_cast<double*>(base + fp.off_part);
+  d.phi = nf == 2 ? reinterpret_cast<float*>(base + fp.off_phi) : nullptr;
   d.NP = static_cast<uint32_t>(fp.NP);
   d.PY = PY;
   d.W = rb.g.W;
   d.n = static_cast<int32_t>(n);
   d.parity = static_cast<int32_t>(b->global_step & 1);  // f[0] holds the state at the next step
+  d.kind = p.kind;
+  const double tau = p.kind == 1 ? p.tau0 : p.tau;
   d.a.tau_inv = 1.0 / tau;
   d.a.tau_factor = 1 - 0.5 / tau;
-  for (int k = 0; k < 3; ++k) d.a.F[k] = k < b->nD ? force[k] : 0.0;
+  for (int k = 0; k < 3; ++k) d.a.F[k] = k < b->nD ? p.force[k] : 0.0;
+  const double c2 = b->lattice == LB_D2Q9 ? D2Q9::c2 : D3Q19::c2;
+  if (p.kind == 1) {
+    // as lb_step_cg and lb_set_wall_densities compute them
+    d.nu0Inv = 1.0 / (c2 * (p.tau0 - 0.5));
+    d.nu1Inv = 1.0 / (c2 * (p.tau1 - 0.5));
+    d.sigma = p.sigma;
+    d.beta = p.beta;
+    const double r0 = p.wett, r1 = 1.0 - p.wett;
+    d.phi_w = (r0 - r1) / (r0 + r1);
+  }
   hipError_t e = hipMemcpyAsync(base + fp.off_coord, rb.coord.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice,
                                 b->stream);
   if (e == hipSuccess)
     e = hipMemcpyAsync(base + fp.off_win, win.data(), win.size() * sizeof(uint4), hipMemcpyHostToDevice, b->stream);
-  // rest state f = w: zero in the shifted forms
+  double* tmp = nullptr;
   if (e == hipSuccess) {
-    if (b->precision == LB_FP64) e = batch_rest_f64(b->lattice, reinterpret_cast<double*>(base + fp.off_f0), fp.NP, b->stream);
-    else e = hipMemsetAsync(base + fp.off_f0, 0, static_cast<size_t>(b->Q) * fp.NP * b->esz, b->stream);
+    if (p.kind == 0) {
+      // rest state f = w: zero in the shifted forms; a pull from it is the
+      // identity, so the first step needs no special read
+      if (b->precision == LB_FP64)
+        e = batch_rest_f64(b->lattice, reinterpret_cast<double*>(base + fp.off_f0), fp.NP, b->stream);
+      else
+        e = hipMemsetAsync(base + fp.off_f0, 0, static_cast<size_t>(b->Q) * fp.NP * b->esz, b->stream);
+    } else {
+      // f_k = w rho_k at rest, placed where the first pull reads it: element
+      // (q, i) goes to the source of direction q of node i (the pull is a
+      // permutation of the elements), so every step is the same kernel
+      const size_t fsz = static_cast<size_t>(b->Q) * fp.NP;
+      std::vector<double> h(2 * fsz, 0.0);
+      const int Q = b->Q;
+      for (int64_t i = 0; i < n; ++i) {
+        const uint32_t c = rb.coord[i];
+        const int32_t X = static_cast<int32_t>(c >> 22), Y = static_cast<int32_t>((c >> 12) & 1023u),
+                      Z = static_cast<int32_t>(c & 4095u);
+        // geometry cell of the compact node (inverse of rows_from_geometry)
+        int32_t P[3] = {X, Y, Z};
+        for (int a = 0; a < 3; ++a) {
+          const int32_t Dm = a == 1 && b->nD == 2 ? 1 : dims[b->nD == 3 ? a : (a == 0 ? 0 : 1)];
+          P[a] = (rb.g.periodic >> a) & 1 ? (P[a] + rb.lo[a]) % Dm : P[a] + rb.lo[a];
+        }
+        const int64_t cell = b->nD == 3 ? (static_cast<int64_t>(P[0]) * dims[1] + P[1]) * dims[2] + P[2]
+                                        : static_cast<int64_t>(P[0]) * dims[1] + P[2];
+        const double r0 = p.rho0[cell], r1 = 1.0 - p.rho0[cell];
+        for (int q = 0; q < Q; ++q) {
+          uint32_t el;
+          const double w = b->lattice == LB_D2Q9 ? D2Q9::w(q) : D3Q19::w(q);
+          if (q == Q - 1) {
+            el = pidx_host(Q, q, static_cast<uint32_t>(i));
+          } else {
+            const auto src = win_source(b->lattice, rb, win, PY, q, static_cast<int32_t>(i));
+            const int rq = b->lattice == LB_D2Q9 ? D2Q9::rev(q) : D3Q19::rev(q);
+            el = src.first ? pidx_host(Q, q, static_cast<uint32_t>(src.second))
+                           : pidx_host(Q, rq, static_cast<uint32_t>(i));
+          }
+          h[el] = w * r0;
+          h[fsz + el] = w * r1;
+        }
+      }
+      e = hipMalloc(&tmp, 2 * fsz * sizeof(double));
+      if (e == hipSuccess) e = hipMemcpyAsync(tmp, h.data(), 2 * fsz * sizeof(double), hipMemcpyHostToDevice, b->stream);
+      if (e == hipSuccess)
+        e = store_lowp(b->precision, b->lattice, 2, tmp, base + fp.off_f0, static_cast<int32_t>(n), fp.NP, b->stream);
+    }
   }
   if (e == hipSuccess)
     e = hipMemcpyAsync(b->d_solves + slot, &d, sizeof(BatchSolve), hipMemcpyHostToDevice, b->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  if (tmp) hipFree(tmp);
   if (e != hipSuccess) {
     b->alloc.release(off, fp.bytes);
     return bfail(b, LB_ERR_HIP, "lb_batch_add: %s", hipGetErrorString(e));
@@ -511,20 +604,66 @@ int lb_batch_add(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* ge
   Slot& sl = b->slots[slot];
   sl = Slot{};
   sl.used = true;
-  sl.tag = tag;
+  sl.kind = p.kind;
+  sl.tag = p.tag;
   sl.off = off;
   sl.fp = fp;
   sl.tau = tau;
-  sl.F0 = force[0];
+  sl.F0 = p.force[0];
   sl.nu = (tau - 0.5) / 3.0;
+  sl.nu0 = (p.tau0 - 0.5) / 3.0;
+  sl.nu1 = (p.tau1 - 0.5) / 3.0;
   int64_t nbox = 1;
   for (int k = 0; k < nd; ++k) nbox *= dims[k];
   sl.n_box = static_cast<double>(nbox);
-  sl.tol = tol;
-  sl.max_steps = max_steps;
+  sl.tol = p.tol;
+  sl.max_steps = p.max_steps;
   sl.t_admit = now_s();
   b->dirty = true;
   return LB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lb_batch_add(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* geo, int32_t periodic_mask,
+                 double tau, const double* force, double tol, int64_t max_steps, int64_t tag) {
+  if (!b) return LB_ERR_ARG;
+  if (nd != b->nD || !dims || !geo || !force || !(tau > 0.5) || !(tol > 0.0) || max_steps < 1 || force[0] == 0.0)
+    return bfail(b, LB_ERR_ARG, "lb_batch_add: bad argument");
+  AddArgs p;
+  p.kind = 0;
+  p.tau = tau;
+  p.force = force;
+  p.tol = tol;
+  p.max_steps = max_steps;
+  p.tag = tag;
+  return add_solve(b, nd, dims, geo, periodic_mask, p);
+}
+
+int lb_batch_add_twophase(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* geo, int32_t periodic_mask,
+                          const double* rho0, double wettability, double tau0, double tau1, double sigma,
+                          double beta, const double* force, double tol, int64_t max_steps, int64_t tag) {
+  if (!b) return LB_ERR_ARG;
+  if (nd != b->nD || !dims || !geo || !rho0 || !force || !(tau0 > 0.5) || !(tau1 > 0.5) || !(tol > 0.0) ||
+      max_steps < 1 || force[0] == 0.0 || !(wettability >= 0.0 && wettability <= 1.0))
+    return bfail(b, LB_ERR_ARG, "lb_batch_add_twophase: bad argument");
+  if (b->precision == LB_FP64)
+    return bfail(b, LB_ERR_UNSUPPORTED, "lb_batch_add_twophase: two-phase batches run in FP32S or FP16S");
+  AddArgs p;
+  p.kind = 1;
+  p.tau0 = tau0;
+  p.tau1 = tau1;
+  p.sigma = sigma;
+  p.beta = beta;
+  p.wett = wettability;
+  p.rho0 = rho0;
+  p.force = force;
+  p.tol = tol;
+  p.max_steps = max_steps;
+  p.tag = tag;
+  return add_solve(b, nd, dims, geo, periodic_mask, p);
 }
 
 int lb_batch_advance(lb_batch* b, int32_t intervals, int32_t* n_finished) {
@@ -546,20 +685,34 @@ int lb_batch_advance(lb_batch* b, int32_t intervals, int32_t* n_finished) {
     for (int32_t s : b->active) {
       Slot& sl = b->slots[s];
       sl.steps += b->check_every;
-      const double jx = b->h_out[s];
-      // k = nu <j_x> / F_x with j_x summed over the state the last step started from
-      sl.k = sl.nu * (jx / sl.n_box) / sl.F0;
+      const double* o = b->h_out + 3 * s;
+      bool conv;
+      if (sl.kind == 0) {
+        // k = nu <j_x> / F_x with j_x summed over the state the last step started from
+        sl.k = sl.nu * (o[0] / sl.n_box) / sl.F0;
+        conv = sl.have_prev && std::fabs(sl.k - sl.k_prev) <= sl.tol * std::fabs(sl.k);
+      } else {
+        // phase permeabilities k_i = nu_i <phi_i sum_q f c_x> / F_x (LBglobalforcing.h:61-78)
+        sl.k0 = sl.nu0 * (o[1] / sl.n_box) / sl.F0;
+        sl.k1 = sl.nu1 * (o[2] / sl.n_box) / sl.F0;
+        sl.k = sl.k0 + sl.k1;
+        const double scale = std::fabs(sl.k0) + std::fabs(sl.k1);
+        conv = sl.have_prev && std::fabs(sl.k0 - sl.k0_prev) <= sl.tol * scale &&
+               std::fabs(sl.k1 - sl.k1_prev) <= sl.tol * scale;
+      }
       if (!std::isfinite(sl.k)) {
         retire(b, s, 2);
         ++fin;
         continue;
       }
-      if (sl.have_prev && std::fabs(sl.k - sl.k_prev) <= sl.tol * std::fabs(sl.k)) {
+      if (conv) {
         retire(b, s, 0);
         ++fin;
         continue;
       }
       sl.k_prev = sl.k;
+      sl.k0_prev = sl.k0;
+      sl.k1_prev = sl.k1;
       sl.have_prev = true;
       if (sl.steps >= sl.max_steps) {
         retire(b, s, 1);
@@ -571,8 +724,8 @@ int lb_batch_advance(lb_batch* b, int32_t intervals, int32_t* n_finished) {
   return LB_OK;
 }
 
-int lb_batch_collect(lb_batch* b, int32_t max_n, int64_t* tags, double* k, int64_t* steps, double* seconds,
-                     int32_t* status, int32_t* n_out) {
+int lb_batch_collect(lb_batch* b, int32_t max_n, int64_t* tags, double* k, double* k_phase, int64_t* steps,
+                     double* seconds, int32_t* status, int32_t* n_out) {
   if (!b) return LB_ERR_ARG;
   if (max_n < 0 || !n_out) return bfail(b, LB_ERR_ARG, "lb_batch_collect: bad argument");
   const int32_t n = std::min<int32_t>(max_n, static_cast<int32_t>(b->done.size()));
@@ -580,6 +733,10 @@ int lb_batch_collect(lb_batch* b, int32_t max_n, int64_t* tags, double* k, int64
     const Result& r = b->done[i];
     if (tags) tags[i] = r.tag;
     if (k) k[i] = r.k;
+    if (k_phase) {
+      k_phase[2 * i] = r.k0;
+      k_phase[2 * i + 1] = r.k1;
+    }
     if (steps) steps[i] = r.steps;
     if (seconds) seconds[i] = r.seconds;
     if (status) status[i] = r.status;

@@ -58,12 +58,14 @@ struct BatchSolve {
   const uint32_t* coord;
   const uint4* win;
   void* f[2];
-  double* partial;
+  double* partial;  // [ncomp][blocks]: 1 component single-phase, 3 two-phase
+  float* phi;       // two-phase colour field (NP), null for single-phase
   uint32_t NP;
   int32_t PY, W, n;
   int32_t parity;
-  int32_t pad_;
-  BgkArgs a;
+  int32_t kind;     // 0 single-phase BGK, 1 colour-gradient two-phase
+  BgkArgs a;        // tau_inv, tau_factor (single-phase) and the body force
+  double nu0Inv, nu1Inv, sigma, beta, phi_w;  // two-phase parameters, wall colour
 };
 // Work item of a batched launch: slot << kItemShift | 256-node block of the solve.
 constexpr int kItemShift = 18;

@@ -226,55 +226,75 @@ __global__ __launch_bounds__(kBlock) void k_load(const S* __restrict__ g, double
 }
 
 // Both planes (t, d) of a two-phase node, one address computation.
-template <class L, typename S, bool FIRST>
-__device__ __forceinline__ void load_td(const S* __restrict__ gin, uint32_t fs, const TableAddr& ad, int32_t i,
+template <class L, typename S, bool FIRST, class A>
+__device__ __forceinline__ void load_td(const S* __restrict__ gin, uint32_t fs, const A& ad, int32_t i,
                                         float (&t)[L::nQ], float (&d)[L::nQ]) {
+  constexpr bool O = A::kOff32;
   if (FIRST) {
 #pragma unroll
     for (int q = 0; q < L::nQ - 1; ++q) {
-      t[q] = Store<S>::get(gin[pidx<L::nQ>(q, i)]);
-      d[q] = Store<S>::get(gin[fs + pidx<L::nQ>(q, i)]);
+      t[q] = Store<S>::get(ld_elem<O>(gin, pidx<L::nQ>(q, i)));
+      d[q] = Store<S>::get(ld_elem<O>(gin, fs + pidx<L::nQ>(q, i)));
     }
   } else {
     uint32_t src[L::nQ - 1];
     ad.template sources<L>(i, src);
 #pragma unroll
     for (int q = 0; q < L::nQ - 1; ++q) {
-      t[q] = Store<S>::get(gin[src[q]]);
-      d[q] = Store<S>::get(gin[fs + src[q]]);
+      t[q] = Store<S>::get(ld_elem<O>(gin, src[q]));
+      d[q] = Store<S>::get(ld_elem<O>(gin, fs + src[q]));
     }
   }
-  t[L::nQ - 1] = Store<S>::get(gin[pidx<L::nQ>(L::nQ - 1, i)]);
-  d[L::nQ - 1] = Store<S>::get(gin[fs + pidx<L::nQ>(L::nQ - 1, i)]);
+  t[L::nQ - 1] = Store<S>::get(ld_elem<O>(gin, pidx<L::nQ>(L::nQ - 1, i)));
+  d[L::nQ - 1] = Store<S>::get(ld_elem<O>(gin, fs + pidx<L::nQ>(L::nQ - 1, i)));
 }
 
-// Two-phase pass 1 on (t, d): phi = (r0 - r1) / (r0 + r1) = sum d / (1 + sum t)
-// (main_TWOPHASE.cpp:246-262), and in the momentum-controlled mode the block
-// sums of sum_q (f0 + f1) c_x = sum_q t c_x in FP64 (main_TWOPHASE.cpp:292-308).
-template <class L, typename S, bool FIRST, bool MOMX>
-__global__ __launch_bounds__(kBlock) void k_cg_phi_lp(const S* __restrict__ gin, TableAddr ad, int32_t n,
-                                                      int64_t NP, float* __restrict__ phi,
-                                                      double* __restrict__ partial) {
-  constexpr int Q = L::nQ;
-  const int32_t b = xcd_block(blockIdx.x, gridDim.x);
-  const int32_t i = b * kBlock + threadIdx.x;
-  double mx = 0.0;
-  if (i < n) {
-    float t[Q], d[Q];
-    load_td<L, S, FIRST>(gin, static_cast<uint32_t>(Q * NP), ad, i, t, d);
-    float st = 0.0f, sd = 0.0f;
+// (t, d) of a node from row-window links, and the links themselves for the
+// neighbour phi lookup: bit q of fluid set when the source of direction q is
+// the fluid node j[q], otherwise it is a solid (bounce-back).
+template <class L, typename S>
+__device__ __forceinline__ void load_td_links(const S* __restrict__ gin, uint32_t fs, const RowWin& ad, int32_t i,
+                                              float (&t)[L::nQ], float (&d)[L::nQ], uint32_t (&j)[L::nQ - 1],
+                                              uint32_t& fluid) {
+  rowwin_links<L>(ad, i, j, fluid);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      st += t[q];
-      sd += d[q];
-    }
-    phi[i] = sd / (1.0f + st);
-    if (MOMX) mx = static_cast<double>(qsumc<L>(t, 0));
+  for (int q = 0; q < L::nQ - 1; ++q) {
+    const uint32_t e = (fluid >> q) & 1u ? pidx<L::nQ>(q, j[q]) : pidx<L::nQ>(L::rev(q), i);
+    t[q] = Store<S>::get(ld_elem<true>(gin, e));
+    d[q] = Store<S>::get(ld_elem<true>(gin, fs + e));
   }
-  if (MOMX) {
-    const double s = block_sum(mx);
-    if (threadIdx.x == 0) partial[b] = s;
+  t[L::nQ - 1] = Store<S>::get(ld_elem<true>(gin, pidx<L::nQ>(L::nQ - 1, i)));
+  d[L::nQ - 1] = Store<S>::get(ld_elem<true>(gin, fs + pidx<L::nQ>(L::nQ - 1, i)));
+}
+
+// phi = (r0 - r1) / (r0 + r1) = sum d / (1 + sum t)  (main_TWOPHASE.cpp:246-262)
+template <class L>
+__device__ __forceinline__ float phi_td(const float (&t)[L::nQ], const float (&d)[L::nQ]) {
+  float st = 0.0f, sd = 0.0f;
+#pragma unroll
+  for (int q = 0; q < L::nQ; ++q) {
+    st += t[q];
+    sd += d[q];
   }
+  return sd / (1.0f + st);
+}
+
+// Two-phase flux sums of one node on (t, d): sum_q (f0 + f1) c = sum_q t c,
+// and the phase shares r_k / rho of it (LBglobalforcing.h:61-78).
+template <class L>
+__device__ __forceinline__ void flux_td(const float (&t)[L::nQ], const float (&d)[L::nQ], int dir, double& s0,
+                                        double& s1, double& s2) {
+  float st = 0.0f, sd = 0.0f;
+#pragma unroll
+  for (int q = 0; q < L::nQ; ++q) {
+    st += t[q];
+    sd += d[q];
+  }
+  const float rho = 1.0f + st;
+  const double fc = static_cast<double>(qsumc<L>(t, dir));
+  s0 = fc;
+  s1 = static_cast<double>(0.5f * (rho + sd) / rho) * fc;
+  s2 = static_cast<double>(0.5f * (rho - sd) / rho) * fc;
 }
 
 // Two-phase pass 3 on (t, d) (main_TWOPHASE.cpp:318-392): BGK with the mixed
@@ -284,23 +304,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_phi_lp(const S* __restrict__ gin,
 // dOmegaST_q and c_k = rho_k / rho,
 //   f0' = c0 base + RC,  f1' = c1 base - RC
 // give t' = base - w and d' = phi (base) + 2 RC with phi = c0 - c1.
-template <class L, typename S, bool FIRST, bool STORE>
-__global__ __launch_bounds__(kBlock) void k_cg_lp(const S* __restrict__ gin, S* __restrict__ gout, TableAddr ad,
-                                                  int32_t n, int64_t NP, float nu0Inv, float nu1Inv,
-                                                  float sigma, float beta, float F0, float F1, float F2,
-                                                  int32_t momx_mode, const float* __restrict__ phi,
-                                                  const double* __restrict__ phi_wall,
-                                                  const double* __restrict__ dscalar,
-                                                  double* __restrict__ rho_o, double* __restrict__ vel_o) {
+// pn[qq]: phi of neighbor(qq, n) (LButilities.h:12-22).
+template <class L, typename S, bool O, bool STORE>
+__device__ __forceinline__ void cg_collide(const float (&t)[L::nQ], const float (&dd)[L::nQ], float (&pn)[L::nQ],
+                                           const float (&F)[3], float nu0Inv, float nu1Inv, float sigma, float beta,
+                                           S* __restrict__ gout, uint32_t fs, int32_t i, int64_t NP,
+                                           double* __restrict__ rho_o, double* __restrict__ vel_o) {
   constexpr int Q = L::nQ, D = L::nD;
-  const int32_t b = xcd_block(blockIdx.x, gridDim.x);
-  const int32_t i = b * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t fs = static_cast<uint32_t>(Q * NP);
-  float t[Q], dd[Q];
-  load_td<L, S, FIRST>(gin, fs, ad, i, t, dd);
-  float F[3] = {F0, F1, F2};
-  if (momx_mode) F[0] = static_cast<float>(dscalar[0]);
   float drho = 0.0f, sd = 0.0f;
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
@@ -323,16 +333,6 @@ __global__ __launch_bounds__(kBlock) void k_cg_lp(const S* __restrict__ gin, S* 
   const float tau = (3.0f * rho) / (r0 * nu0Inv + r1 * nu1Inv) + 0.5f;
   const float ti = 1.0f / tau;
   const float tf = 1.0f - 0.5f * ti;
-  // colour gradient from the neighbours' phi (LButilities.h:12-22)
-  float pn[Q];
-#pragma unroll
-  for (int qq = 0; qq < Q - 1; ++qq) {
-    const int q = L::rev(qq);  // neighbor(qq, n) is the stream source of rev(qq)
-    const uint32_t l = ad.link[static_cast<uint32_t>(q) * ad.NP + i];
-    const uint32_t kind = l >> 30;
-    const uint32_t j = l & kLinkIndexMask;
-    pn[qq] = kind == kLinkStream ? phi[j] : (kind == kLinkBounce ? static_cast<float>(phi_wall[j]) : 0.0f);
-  }
   pn[Q - 1] = 0.0f;
   float g[D];
 #pragma unroll
@@ -364,8 +364,134 @@ __global__ __launch_bounds__(kBlock) void k_cg_lp(const S* __restrict__ gin, S* 
       rc = 0.0f;
     }
     const float bs = omt * t[q] + ti * geq + dF + st;
-    __builtin_nontemporal_store(Store<S>::put(bs), &gout[pidx<Q>(q, i)]);
-    __builtin_nontemporal_store(Store<S>::put(phi_n * (w + bs) + 2.0f * rc), &gout[fs + pidx<Q>(q, i)]);
+    __builtin_nontemporal_store(Store<S>::put(bs), elem_ptr<O>(gout, pidx<Q>(q, i)));
+    __builtin_nontemporal_store(Store<S>::put(phi_n * (w + bs) + 2.0f * rc), elem_ptr<O>(gout, fs + pidx<Q>(q, i)));
+  }
+}
+
+// Two-phase pass 1 (+ the momentum sums of the flux controller, in FP64:
+// sum_q (f0 + f1) c_x = sum_q t c_x, main_TWOPHASE.cpp:292-308).
+template <class L, typename S, bool FIRST, bool MOMX>
+__global__ __launch_bounds__(kBlock) void k_cg_phi_lp(const S* __restrict__ gin, TableAddr ad, int32_t n,
+                                                      int64_t NP, float* __restrict__ phi,
+                                                      double* __restrict__ partial) {
+  constexpr int Q = L::nQ;
+  const int32_t b = xcd_block(blockIdx.x, gridDim.x);
+  const int32_t i = b * kBlock + threadIdx.x;
+  double mx = 0.0;
+  if (i < n) {
+    float t[Q], d[Q];
+    load_td<L, S, FIRST>(gin, static_cast<uint32_t>(Q * NP), ad, i, t, d);
+    phi[i] = phi_td<L>(t, d);
+    if (MOMX) mx = static_cast<double>(qsumc<L>(t, 0));
+  }
+  if (MOMX) {
+    const double s = block_sum(mx);
+    if (threadIdx.x == 0) partial[b] = s;
+  }
+}
+
+// Two-phase pass 3 with the link table; solid neighbours take the wall
+// colour of their slot.
+template <class L, typename S, bool FIRST, bool STORE>
+__global__ __launch_bounds__(kBlock) void k_cg_lp(const S* __restrict__ gin, S* __restrict__ gout, TableAddr ad,
+                                                  int32_t n, int64_t NP, float nu0Inv, float nu1Inv,
+                                                  float sigma, float beta, float F0, float F1, float F2,
+                                                  int32_t momx_mode, const float* __restrict__ phi,
+                                                  const double* __restrict__ phi_wall,
+                                                  const double* __restrict__ dscalar,
+                                                  double* __restrict__ rho_o, double* __restrict__ vel_o) {
+  constexpr int Q = L::nQ;
+  const int32_t b = xcd_block(blockIdx.x, gridDim.x);
+  const int32_t i = b * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t fs = static_cast<uint32_t>(Q * NP);
+  float t[Q], dd[Q];
+  load_td<L, S, FIRST>(gin, fs, ad, i, t, dd);
+  float F[3] = {F0, F1, F2};
+  if (momx_mode) F[0] = static_cast<float>(dscalar[0]);
+  float pn[Q];
+#pragma unroll
+  for (int qq = 0; qq < Q - 1; ++qq) {
+    const int q = L::rev(qq);  // neighbor(qq, n) is the stream source of rev(qq)
+    const uint32_t l = ad.link[static_cast<uint32_t>(q) * ad.NP + i];
+    const uint32_t kind = l >> 30;
+    const uint32_t j = l & kLinkIndexMask;
+    pn[qq] = kind == kLinkStream ? phi[j] : (kind == kLinkBounce ? static_cast<float>(phi_wall[j]) : 0.0f);
+  }
+  cg_collide<L, S, false, STORE>(t, dd, pn, F, nu0Inv, nu1Inv, sigma, beta, gout, fs, i, NP, rho_o, vel_o);
+}
+
+// Batched two-phase passes (items as k_batch_bgk_lp; fixed force; every solid
+// neighbour carries the solve's wall colour phi_w).  Pass 1 also writes the
+// three flux partials per block on the interval's last step.
+template <class L, typename S, bool REDUCE>
+__global__ __launch_bounds__(kBlock) void k_batch_cg_phi_lp(const BatchSolve* __restrict__ solves,
+                                                            const uint32_t* __restrict__ items, int32_t n_items,
+                                                            int32_t per_wg, int32_t parity) {
+  constexpr int Q = L::nQ;
+  const int32_t g = xcd_block(blockIdx.x, gridDim.x);
+  const int32_t beg = g * per_wg;
+  const int32_t end = beg + per_wg < n_items ? beg + per_wg : n_items;
+  for (int32_t it = beg; it < end; ++it) {
+    const uint32_t item = items[it];
+    const BatchSolve& s = solves[item >> kItemShift];
+    const int32_t lb = static_cast<int32_t>(item & kItemMask);
+    const int p = (parity ^ s.parity) & 1;
+    const RowWin ad{s.coord, s.win, s.NP, s.PY, s.W};
+    const int32_t i = lb * kBlock + static_cast<int32_t>(threadIdx.x);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    if (i < s.n) {
+      float t[Q], d[Q];
+      load_td<L, S, false>(static_cast<const S*>(s.f[p]), Q * s.NP, ad, i, t, d);
+      s.phi[i] = phi_td<L>(t, d);
+      if (REDUCE) flux_td<L>(t, d, 0, s0, s1, s2);
+    }
+    if (REDUCE) {
+      const int64_t nb = (s.n + kBlock - 1) / kBlock;
+      s0 = block_sum(s0);
+      s1 = block_sum(s1);
+      s2 = block_sum(s2);
+      if (threadIdx.x == 0) {
+        s.partial[lb] = s0;
+        s.partial[nb + lb] = s1;
+        s.partial[2 * nb + lb] = s2;
+      }
+    }
+  }
+}
+
+template <class L, typename S>
+__global__ __launch_bounds__(kBlock) void k_batch_cg_lp(const BatchSolve* __restrict__ solves,
+                                                        const uint32_t* __restrict__ items, int32_t n_items,
+                                                        int32_t per_wg, int32_t parity) {
+  constexpr int Q = L::nQ;
+  const int32_t g = xcd_block(blockIdx.x, gridDim.x);
+  const int32_t beg = g * per_wg;
+  const int32_t end = beg + per_wg < n_items ? beg + per_wg : n_items;
+  for (int32_t it = beg; it < end; ++it) {
+    const uint32_t item = items[it];
+    const BatchSolve& s = solves[item >> kItemShift];
+    const int32_t lb = static_cast<int32_t>(item & kItemMask);
+    const int p = (parity ^ s.parity) & 1;
+    const RowWin ad{s.coord, s.win, s.NP, s.PY, s.W};
+    const int32_t i = lb * kBlock + static_cast<int32_t>(threadIdx.x);
+    if (i >= s.n) continue;
+    const uint32_t fs = Q * s.NP;
+    float t[Q], dd[Q];
+    uint32_t j[Q - 1], fluid;
+    load_td_links<L, S>(static_cast<const S*>(s.f[p]), fs, ad, i, t, dd, j, fluid);
+    const float pw = static_cast<float>(s.phi_w);
+    float pn[Q];
+#pragma unroll
+    for (int qq = 0; qq < Q - 1; ++qq) {
+      const int q = L::rev(qq);
+      pn[qq] = (fluid >> q) & 1u ? s.phi[j[q]] : pw;
+    }
+    const float F[3] = {static_cast<float>(s.a.F[0]), static_cast<float>(s.a.F[1]), static_cast<float>(s.a.F[2])};
+    cg_collide<L, S, true, false>(t, dd, pn, F, static_cast<float>(s.nu0Inv), static_cast<float>(s.nu1Inv),
+                                  static_cast<float>(s.sigma), static_cast<float>(s.beta),
+                                  static_cast<S*>(s.f[p ^ 1]), fs, i, s.NP, nullptr, nullptr);
   }
 }
 
@@ -405,8 +531,6 @@ __global__ __launch_bounds__(kBlock) void k_flux_lp(const S* __restrict__ gin, A
   }
 }
 
-// Two-phase flux sums on (t, d): sum_q (f0 + f1) c = sum_q t c, and the phase
-// shares r_k / rho of it (LBglobalforcing.h:61-78).
 template <class L, typename S, bool FIRST>
 __global__ __launch_bounds__(kBlock) void k_flux_td(const S* __restrict__ gin, TableAddr ad, int32_t n,
                                                     int64_t NP, int dir, double* __restrict__ partial,
@@ -417,17 +541,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_td(const S* __restrict__ gin, T
   if (i < n) {
     float t[Q], d[Q];
     load_td<L, S, FIRST>(gin, static_cast<uint32_t>(Q * NP), ad, i, t, d);
-    float st = 0.0f, sd = 0.0f;
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      st += t[q];
-      sd += d[q];
-    }
-    const float rho = 1.0f + st;
-    const double fc = static_cast<double>(qsumc<L>(t, dir));
-    s0 = fc;
-    s1 = static_cast<double>(0.5f * (rho + sd) / rho) * fc;
-    s2 = static_cast<double>(0.5f * (rho - sd) / rho) * fc;
+    flux_td<L>(t, d, dir, s0, s1, s2);
   }
   s0 = block_sum(s0);
   s1 = block_sum(s1);
@@ -556,6 +670,18 @@ hipError_t batch_bgk_lowp(int precision, int lattice, bool reduce, const BatchSo
       k_batch_bgk_lp<L, S, true><<<dim3(grid), dim3(kBlock), 0, s>>>(solves, items, n_items, per_wg, parity);
     else
       k_batch_bgk_lp<L, S, false><<<dim3(grid), dim3(kBlock), 0, s>>>(solves, items, n_items, per_wg, parity);
+  });
+}
+
+hipError_t batch_cg_lowp(int precision, int lattice, bool reduce, const BatchSolve* solves, const uint32_t* items,
+                         int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, hipStream_t s) {
+  return dispatch(precision, lattice, Addr{}, [&](auto l, auto st, auto) {
+    using L = decltype(l);
+    using S = decltype(st);
+    const dim3 g(grid), b(kBlock);
+    if (reduce) k_batch_cg_phi_lp<L, S, true><<<g, b, 0, s>>>(solves, items, n_items, per_wg, parity);
+    else k_batch_cg_phi_lp<L, S, false><<<g, b, 0, s>>>(solves, items, n_items, per_wg, parity);
+    k_batch_cg_lp<L, S><<<g, b, 0, s>>>(solves, items, n_items, per_wg, parity);
   });
 }
 

@@ -32,6 +32,9 @@ hipError_t flux_lowp(int precision, int lattice, int nf, bool first, const void*
 // Batched form in reduced precision (see batch_bgk_f64).
 hipError_t batch_bgk_lowp(int precision, int lattice, bool reduce, const BatchSolve* solves, const uint32_t* items,
                           int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, hipStream_t s);
+// Both passes of one batched two-phase step (see k_batch_cg_phi_lp).
+hipError_t batch_cg_lowp(int precision, int lattice, bool reduce, const BatchSolve* solves, const uint32_t* items,
+                         int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, hipStream_t s);
 // Two-phase populations are stored as (t, d) = (f0 + f1 - w, f0 - f1) and use
 // the link table.
 hipError_t cg_phi_lowp(int precision, int lattice, bool first, bool momx, const void* gin,

@@ -148,17 +148,21 @@ __global__ __launch_bounds__(kBlock) void k_batch_bgk(const BatchSolve* __restri
   }
 }
 
-// Per-solve sum of the block partials, in the order k_sum_partials uses.
+// Per-solve sums of the block partials (1 component single-phase, 3
+// two-phase), in the order k_sum_partials uses: out[3 slot + c].
 __global__ __launch_bounds__(kBlock) void k_batch_sum(const BatchSolve* __restrict__ solves,
                                                       const int32_t* __restrict__ slots,
                                                       double* __restrict__ out) {
   const int32_t slot = slots[blockIdx.x];
   const BatchSolve& s = solves[slot];
   const int64_t nb = s.n > 0 ? (s.n + kBlock - 1) / kBlock : 1;
-  double v = 0.0;
-  for (int64_t b = threadIdx.x; b < nb; b += kBlock) v += s.partial[b];
-  v = block_sum(v);
-  if (threadIdx.x == 0) out[slot] = v;
+  const int ncomp = s.kind == 1 ? 3 : 1;
+  for (int c = 0; c < ncomp; ++c) {
+    double v = 0.0;
+    for (int64_t b = threadIdx.x; b < nb; b += kBlock) v += s.partial[c * nb + b];
+    v = block_sum(v);
+    if (threadIdx.x == 0) out[3 * slot + c] = v;
+  }
 }
 
 // Rest state f = w of a solve's pre-collision buffer (FP64 form).

@@ -19,6 +19,7 @@ namespace lbamd {
 // {base, mask} per (row, 32-cell word).
 struct RowBuild {
   RowGeom g{};
+  int32_t lo[3] = {0, 0, 0};    // rows_from_geometry: geometry cell of box coordinate 0
   std::vector<int32_t> order;   // compact index -> bulk index
   std::vector<uint32_t> coord;  // compact index -> packed cell
   std::vector<uint2> rows;
@@ -62,6 +63,7 @@ inline bool rows_from_geometry(int lattice, const int32_t* dims, const int32_t* 
           }
         }
   if (nfluid == 0) { why = "no fluid cells"; return false; }
+  for (int a = 0; a < 3; ++a) rb.lo[a] = lo[a];
   RowGeom& g = rb.g;
   g.periodic = 0;
   for (int a = 0; a < 3; ++a) {

@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libbadchimp_amd.so")
+# LBAMD_LIB selects another build of the library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("LBAMD_LIB") or os.path.join(os.path.dirname(HERE), "lib", "libbadchimp_amd.so")
 
 D2Q9, D3Q19 = 0, 1
 FP64, FP32S, FP16S = 0, 1, 2
@@ -91,14 +92,17 @@ SIGNATURES = {
     "lb_batch_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int32, C.POINTER(_vp)]),
     "lb_batch_destroy": (C.c_int, [_vp]),
     "lb_batch_last_error": (C.c_char_p, [_vp]),
-    "lb_batch_footprint": (C.c_int, [C.c_int, C.c_int, C.c_int32, _i32p, _i32p, C.c_int32,
+    "lb_batch_footprint": (C.c_int, [C.c_int, C.c_int, C.c_int32, C.c_int32, _i32p, _i32p, C.c_int32,
                                      C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "lb_batch_add_twophase": (C.c_int, [_vp, C.c_int32, _i32p, _i32p, C.c_int32, _f64p, C.c_double, C.c_double,
+                                        C.c_double, C.c_double, C.c_double, _f64p, C.c_double, C.c_int64,
+                                        C.c_int64]),
     "lb_batch_add": (C.c_int, [_vp, C.c_int32, _i32p, _i32p, C.c_int32, C.c_double, _f64p, C.c_double,
                                C.c_int64, C.c_int64]),
     "lb_batch_advance": (C.c_int, [_vp, C.c_int32, C.POINTER(C.c_int32)]),
     "lb_batch_collect": (C.c_int, [_vp, C.c_int32, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
-                                   _f64p, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"), _f64p,
-                                   _i32p, C.POINTER(C.c_int32)]),
+                                   _f64p, _f64p, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
+                                   _f64p, _i32p, C.POINTER(C.c_int32)]),
     "lb_batch_status": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                   C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "lb_batch_set_shape": (C.c_int, [_vp, C.c_int32]),
@@ -402,13 +406,13 @@ class Context:
         return lib().lb_device_bytes(self._h)
 
 
-def batch_footprint(lattice, precision, geo, periodic="x"):
+def batch_footprint(lattice, precision, geo, periodic="x", n_fields=1):
     """(bytes, fluid nodes) a solve of this geometry takes in a Batch pool."""
     g = np.ascontiguousarray(geo, dtype=np.int32)
     dims = np.ascontiguousarray(g.shape, dtype=np.int32)
     nbytes, nf = C.c_int64(), C.c_int64()
-    _check_global(lib().lb_batch_footprint(lattice, precision, len(dims), dims, g.ravel(), periodic_mask(periodic),
-                                           C.byref(nbytes), C.byref(nf)))
+    _check_global(lib().lb_batch_footprint(lattice, precision, int(n_fields), len(dims), dims, g.ravel(),
+                                           periodic_mask(periodic), C.byref(nbytes), C.byref(nf)))
     return nbytes.value, nf.value
 
 
@@ -453,6 +457,24 @@ class Batch:
         self._check(rc)
         return True
 
+    def add_twophase(self, geo, periodic, rho0, wettability, tau0, tau1, sigma, beta, force, tol, max_steps, tag):
+        """Admit a colour-gradient solve (rho0 per geometry cell); False when
+        it does not fit into the pool now."""
+        g = np.ascontiguousarray(geo, dtype=np.int32)
+        r = np.ascontiguousarray(rho0, dtype=np.float64)
+        if r.shape != g.shape:
+            raise ValueError("rho0 must have the geometry's shape")
+        dims = np.ascontiguousarray(g.shape, dtype=np.int32)
+        F = np.zeros(3)
+        F[: self.nD] = np.asarray(force, dtype=float)[: self.nD]
+        rc = lib().lb_batch_add_twophase(self._h, len(dims), dims, g.ravel(), periodic_mask(periodic), r.ravel(),
+                                         float(wettability), float(tau0), float(tau1), float(sigma), float(beta),
+                                         F, float(tol), int(max_steps), int(tag))
+        if rc == -7:
+            return False
+        self._check(rc)
+        return True
+
     def advance(self, intervals=1):
         n = C.c_int32()
         self._check(lib().lb_batch_advance(self._h, int(intervals), C.byref(n)))
@@ -461,14 +483,15 @@ class Batch:
     def collect(self, max_n=1 << 16):
         tags = np.zeros(max_n, dtype=np.int64)
         k = np.zeros(max_n)
+        kp = np.zeros(2 * max_n)
         steps = np.zeros(max_n, dtype=np.int64)
         sec = np.zeros(max_n)
         st = np.zeros(max_n, dtype=np.int32)
         n = C.c_int32()
-        self._check(lib().lb_batch_collect(self._h, int(max_n), tags, k, steps, sec, st, C.byref(n)))
+        self._check(lib().lb_batch_collect(self._h, int(max_n), tags, k, kp, steps, sec, st, C.byref(n)))
         m = n.value
-        return [{"tag": int(tags[i]), "k": float(k[i]), "steps": int(steps[i]), "seconds": float(sec[i]),
-                 "status": int(st[i])} for i in range(m)]
+        return [{"tag": int(tags[i]), "k": float(k[i]), "k_phase": (float(kp[2 * i]), float(kp[2 * i + 1])),
+                 "steps": int(steps[i]), "seconds": float(sec[i]), "status": int(st[i])} for i in range(m)]
 
     def status(self):
         a, b, c, d, e = C.c_int32(), C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()

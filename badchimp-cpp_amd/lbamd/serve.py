@@ -14,10 +14,15 @@ from .ffi import D3Q19, FP32S, Batch, batch_footprint
 
 
 class Request:
-    __slots__ = ("tag", "geo", "periodic", "tau", "force", "tol", "max_steps", "t_arrival", "bytes", "kind")
+    """A permeability request (kind "perm": single-phase BGK, tau) or a
+    relative-permeability request (kind "relperm": colour-gradient two-phase
+    with the phase-0 density rho0 per cell, wettability, tau0, tau1, sigma,
+    beta)."""
+    __slots__ = ("tag", "geo", "periodic", "tau", "force", "tol", "max_steps", "t_arrival", "bytes", "kind",
+                 "rho0", "wett", "tau0", "tau1", "sigma", "beta")
 
     def __init__(self, tag, geo, periodic="x", tau=1.0, force=(1e-6, 0.0, 0.0), tol=1e-6, max_steps=100000,
-                 t_arrival=None, kind="perm"):
+                 t_arrival=None, kind="perm", rho0=None, wett=1.0, tau0=1.0, tau1=1.0, sigma=0.01, beta=0.7):
         self.tag = tag
         self.geo = geo
         self.periodic = periodic
@@ -28,11 +33,23 @@ class Request:
         self.t_arrival = t_arrival
         self.bytes = None
         self.kind = kind
+        self.rho0 = rho0
+        self.wett = wett
+        self.tau0, self.tau1, self.sigma, self.beta = tau0, tau1, sigma, beta
+
+    def add_to(self, batch):
+        if self.kind == "relperm":
+            return batch.add_twophase(self.geo, self.periodic, self.rho0, self.wett, self.tau0, self.tau1,
+                                      self.sigma, self.beta, self.force, self.tol, self.max_steps, self.tag)
+        return batch.add(self.geo, self.periodic, self.tau, self.force, self.tol, self.max_steps, self.tag)
 
 
 class Server:
-    def __init__(self, pool_bytes, lattice=D3Q19, precision=FP32S, check_every=100, device=0, lookahead=64):
-        self.batch = Batch(lattice, precision, pool_bytes, check_every, device)
+    def __init__(self, pool_bytes, lattice=D3Q19, precision=FP32S, check_every=100, device=0, lookahead=64,
+                 batch=None):
+        # batch: an object with the Batch interface (add, add_twophase,
+        # advance, collect, status, close); a device Batch by default
+        self.batch = batch if batch is not None else Batch(lattice, precision, pool_bytes, check_every, device)
         self.lattice, self.precision = lattice, precision
         self.pool_bytes = pool_bytes
         self.lookahead = lookahead
@@ -45,7 +62,8 @@ class Server:
         if req.t_arrival is None:
             req.t_arrival = self.clock()
         if req.bytes is None:
-            req.bytes = batch_footprint(self.lattice, self.precision, req.geo, req.periodic)[0]
+            nf = 2 if req.kind == "relperm" else 1
+            req.bytes = batch_footprint(self.lattice, self.precision, req.geo, req.periodic, nf)[0]
         if req.bytes > self.pool_bytes:
             raise ValueError("request %r needs %d bytes, more than the whole pool" % (req.tag, req.bytes))
         self.waiting.append(req)
@@ -58,9 +76,8 @@ class Server:
         n = 0
         while self.waiting:
             r = self.waiting.popleft()
-            if n < self.lookahead and r.bytes <= free and self.batch.add(
-                    r.geo, r.periodic, r.tau, r.force, r.tol, r.max_steps, r.tag):
-                r.geo = None  # the device holds the solve now
+            if n < self.lookahead and r.bytes <= free and r.add_to(self.batch):
+                r.geo = r.rho0 = None  # the device holds the solve now
                 self.resident[r.tag] = r
                 free = self.batch.status()["largest_free"]
             else:

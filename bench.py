@@ -24,6 +24,22 @@ batched  = the ensemble of independent sphere-pack permeability solves (cubic
            path run one solve at a time (lb_solve_permeability on its own
            context), timed on a size-stratified subset (every m-th solve by
            fluid count) and scaled by m; k error of FP32S vs FP64 on that subset.
+twophase = the colour-gradient case: 192^3 sphere pack (radius 8, porosity
+           0.38), phases drawn per pore node with saturation 0.5, solid
+           wettability 1, tau0 = tau1 = 1, sigma = 0.01, beta = 0.7, body force
+           1e-6 on the total, 50000 steps in FP64 and in FP32S: MLUPS of both,
+           the phase field difference and the phase permeabilities
+           k_i = nu_i <phi_i sum_q f c_x> / F_x of both.
+serving  = a stream of requests, 70 % permeability (single-phase) and 30 %
+           relative permeability (colour-gradient, saturation U(0.2, 0.8),
+           phases drawn per pore cell), cubic sizes U(48, 160), seeded sphere
+           packs; admitted by HBM footprint into one pool, advanced together
+           (continuous batching, FP32S), each resident until it converges.
+           First a burst (all requests at t = 0) gives the capacity in
+           solves/s; then Poisson arrivals at 0.7 x that capacity give p50/p99
+           latency.  The same stop rules on the plain FP64 path one request at
+           a time (a size-stratified subset, scaled) give the baseline
+           solves/s and the k error.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256]
 """
@@ -213,6 +229,204 @@ def batched_leg(n, every, tau, force):
     return out
 
 
+W19 = np.array([2.0] * 3 + [1.0] * 6 + [2.0] * 3 + [1.0] * 6 + [12.0]) / 36.0
+TP = {"tau0": 1.0, "tau1": 1.0, "sigma": 0.01, "beta": 0.7, "wett": 1.0}
+
+
+def request_mix(n, seed, t_rate=None):
+    """Serving requests: sizes U(48, 160), 30 % two-phase with S_w U(0.2, 0.8)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from lbamd.serve import Request
+    rng = np.random.default_rng(seed)
+    L = rng.integers(48, 161, n)
+    two = rng.random(n) < 0.3
+    phi = rng.uniform(0.30, 0.50, n)
+    rad = rng.uniform(3.0, 8.0, n)
+    sw = rng.uniform(0.2, 0.8, n)
+    seeds = rng.integers(1, 2**62, n)
+    arrivals = np.cumsum(rng.exponential(1.0 / t_rate, n)) if t_rate else np.zeros(n)
+
+    def one(i):
+        geo, _ = lbamd.sphere_pack((L[i],) * 3, rad[i], phi[i], int(seeds[i]), periodic="x", walls=True)
+        if not two[i]:
+            return Request(i, geo, "x", tau=1.0, force=(1e-6, 0.0, 0.0), tol=1e-6, max_steps=200000,
+                           t_arrival=float(arrivals[i]))
+        r = np.random.default_rng(int(seeds[i]) + 1)
+        rho0 = np.where(geo > 0, (r.random(geo.shape) < sw[i]).astype(float), 0.0)
+        return Request(i, geo, "x", force=(1e-6, 0.0, 0.0), tol=1e-4, max_steps=20000, t_arrival=float(arrivals[i]),
+                       kind="relperm", rho0=rho0, wett=TP["wett"], tau0=TP["tau0"], tau1=TP["tau1"],
+                       sigma=TP["sigma"], beta=TP["beta"])
+
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        return list(ex.map(one, range(n)))
+
+
+def fp64_one(req):
+    """The plain FP64 path for one request (own context, same stop rule);
+    returns (k or (k0, k1), steps, device seconds)."""
+    geo = req.geo
+    topo = lbamd.Topology.from_geometry(geo, lbamd.D3Q19, "x")
+    nb, types, bulk, pos = topo.neighbors(), topo.node_types(), topo.bulk(), topo.positions()
+    n = topo.n_nodes
+    nbox = float(geo.size)
+    if req.kind == "perm":
+        ctx = make_ctx(lbamd.FP64, nb, types, bulk, pos, "rows")
+        f0 = np.zeros((n, 19))
+        f0[bulk] = W19[None, :]
+        ctx.upload_f(f0)
+        ctx.sync()
+        t0 = time.perf_counter()
+        k, steps = ctx.solve_permeability(req.tau, np.asarray(req.force), 100, req.tol, req.max_steps, nbox)
+        dt = time.perf_counter() - t0
+        ctx.close()
+        return k, steps, dt
+    r0 = np.zeros(n)
+    r0[1:] = req.rho0[pos[1:, 0], pos[1:, 1], pos[1:, 2]]
+    r1 = 1.0 - r0
+    sb = types == 1
+    r0[sb], r1[sb] = req.wett, 1.0 - req.wett
+    f0 = np.zeros((n, 2, 19))
+    f0[bulk, 0] = W19[None] * r0[bulk, None]
+    f0[bulk, 1] = W19[None] * r1[bulk, None]
+    ctx = make_ctx(lbamd.FP64, nb, types, bulk, pos, "rows")
+    ctx.upload_f(f0)
+    ctx.set_wall_densities(np.stack([r0, r1], 1))
+    ctx.sync()
+    nu0, nu1 = (req.tau0 - 0.5) / 3.0, (req.tau1 - 0.5) / 3.0
+    F = np.asarray(req.force, dtype=float)
+    t0 = time.perf_counter()
+    prev, steps = None, 0
+    while True:
+        # flux of the state the interval's last step starts from, as the batch reports it
+        ctx.step_cg(99, req.tau0, req.tau1, req.sigma, req.beta, lbamd.FORCE_FIXED, F)
+        _, s1, s2 = ctx.reduce_flux(0)
+        ctx.step_cg(1, req.tau0, req.tau1, req.sigma, req.beta, lbamd.FORCE_FIXED, F)
+        steps += 100
+        k = (nu0 * (s1 / nbox) / F[0], nu1 * (s2 / nbox) / F[0])
+        scale = abs(k[0]) + abs(k[1])
+        if prev is not None and abs(k[0] - prev[0]) <= req.tol * scale and abs(k[1] - prev[1]) <= req.tol * scale:
+            break
+        prev = k
+        if steps >= req.max_steps:
+            break
+    dt = time.perf_counter() - t0
+    ctx.close()
+    return k, steps, dt
+
+
+def twophase_leg(size, steps, seed=3):
+    geo, info = lbamd.sphere_pack((size,) * 3, 8.0, 0.38, seed, periodic="x", walls=True)
+    topo = lbamd.Topology.from_geometry(geo, lbamd.D3Q19, "x")
+    nb, types, bulk, pos = topo.neighbors(), topo.node_types(), topo.bulk(), topo.positions()
+    n = topo.n_nodes
+    rng = np.random.default_rng(seed + 1)
+    r0 = np.zeros(n)
+    r0[bulk] = (rng.random(bulk.size) < 0.5).astype(float)
+    r1 = np.zeros(n)
+    r1[bulk] = 1.0 - r0[bulk]
+    sb = types == 1
+    r0[sb], r1[sb] = TP["wett"], 1.0 - TP["wett"]
+    f0 = np.zeros((n, 2, 19))
+    f0[bulk, 0] = W19[None] * r0[bulk, None]
+    f0[bulk, 1] = W19[None] * r1[bulk, None]
+    rho2 = np.stack([r0, r1], 1)
+    F = np.array([1e-6, 0.0, 0.0])
+    nbox = float(geo.size)
+    res = {}
+    for prec, name in ((lbamd.FP64, "fp64"), (lbamd.FP32S, "fp32s")):
+        ctx = lbamd.Context(lbamd.D3Q19, prec)
+        ctx.set_topology_grid(nb, types, bulk, pos)
+        ctx.upload_f(f0)
+        ctx.set_wall_densities(rho2)
+        ctx.sync()
+        ctx.kernel_time_reset()
+        t0 = time.perf_counter()
+        ctx.step_cg(steps, TP["tau0"], TP["tau1"], TP["sigma"], TP["beta"], lbamd.FORCE_FIXED, F)
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        kms, launches = ctx.kernel_time()
+        s0, s1, s2 = ctx.reduce_flux(0)
+        phi = ctx.download_scalar(lbamd.PHI)[bulk]
+        ctx.close()
+        k0 = (TP["tau0"] - 0.5) / 3.0 * (s1 / nbox) / F[0]
+        k1 = (TP["tau1"] - 0.5) / 3.0 * (s2 / nbox) / F[0]
+        res[name] = {"MLUPS": bulk.size * steps / dt / 1e6, "k0": k0, "k1": k1, "phi": phi,
+                     "ms_per_step": kms / steps}
+        log("two-phase %s: %.1f MLUPS, k0 %.6e k1 %.6e" % (name, res[name]["MLUPS"], k0, k1))
+    a, b = res["fp64"], res["fp32s"]
+    scale = abs(a["k0"]) + abs(a["k1"])
+    out = {"workload": "D3Q19 colour gradient, %d^3 sphere pack r=8 porosity %.3f, S_w 0.5, %d steps" %
+                       (size, info["porosity"], steps),
+           "bulk_nodes": int(bulk.size),
+           "fp64_MLUPS": a["MLUPS"], "fp32s_MLUPS": b["MLUPS"],
+           "fp64_ms_per_step": a["ms_per_step"], "fp32s_ms_per_step": b["ms_per_step"],
+           "k_phase_fp64": [a["k0"], a["k1"]], "k_phase_fp32s": [b["k0"], b["k1"]],
+           "k_phase_err_rel_to_sum": [abs(b["k0"] - a["k0"]) / scale, abs(b["k1"] - a["k1"]) / scale],
+           "phi_max_abs_diff": float(np.abs(a["phi"] - b["phi"]).max()),
+           "phi_mean_abs_diff": float(np.abs(a["phi"] - b["phi"]).mean()),
+           "phase_sign_flips": int(np.sum(np.sign(a["phi"]) != np.sign(b["phi"])))}
+    log("two-phase FP32S vs FP64: k err %s, phi max %.3e mean %.3e" %
+        (out["k_phase_err_rel_to_sum"], out["phi_max_abs_diff"], out["phi_mean_abs_diff"]))
+    return out
+
+
+def serving_leg(n_req, pool_gb, every):
+    from lbamd.serve import Server, run_stream
+    pool = int(pool_gb * 1e9)
+    # 1) burst: capacity
+    reqs = request_mix(n_req, 7)
+    keep = {r.tag: r for r in reqs}  # geometries for the baseline (the server drops its copies)
+    import copy
+    base_reqs = {r.tag: copy.copy(r) for r in reqs}
+    srv = Server(pool, precision=lbamd.FP32S, check_every=100)
+    done, wall = run_stream(srv, reqs)
+    srv.close()
+    cap = len(done) / wall
+    lat_b = np.array([d["latency"] for d in done])
+    log("serving burst: %d requests in %.1f s = %.2f solves/s (p50 %.2f s, p99 %.2f s)" %
+        (len(done), wall, cap, np.percentile(lat_b, 50), np.percentile(lat_b, 99)))
+    # 2) Poisson arrivals at 0.7 x capacity
+    rate = 0.7 * cap
+    reqs2 = request_mix(n_req, 8, t_rate=rate)
+    srv = Server(pool, precision=lbamd.FP32S, check_every=100)
+    done2, wall2 = run_stream(srv, reqs2)
+    srv.close()
+    lat = np.array([d["latency"] for d in done2])
+    log("serving Poisson %.2f/s: %d requests in %.1f s, p50 %.2f s, p99 %.2f s" %
+        (rate, len(done2), wall2, np.percentile(lat, 50), np.percentile(lat, 99)))
+    # 3) plain FP64 one request at a time on a size-stratified subset of the burst
+    byres = {d["tag"]: d for d in done}
+    cells = np.array([base_reqs[t].geo.size for t in range(n_req)])
+    subset = np.argsort(cells, kind="stable")[every // 2::every]
+    t_one, errs, errs_tp = 0.0, [], []
+    for t in subset:
+        k, steps, dt = fp64_one(base_reqs[int(t)])
+        t_one += dt
+        got = byres[int(t)]
+        if base_reqs[int(t)].kind == "perm":
+            errs.append(abs(got["k"] - k) / abs(k))
+        else:
+            scale = abs(k[0]) + abs(k[1])
+            errs_tp.append(max(abs(got["k_phase"][0] - k[0]), abs(got["k_phase"][1] - k[1])) / scale)
+    est = t_one * n_req / len(subset)
+    del keep
+    out = {"n_requests": n_req, "mix": "70% permeability / 30% relative permeability, L U(48,160)",
+           "pool_GB": pool / 1e9,
+           "burst_solves_per_s": cap, "burst_wall_s": wall,
+           "poisson_rate_per_s": rate, "poisson_wall_s": wall2, "poisson_solves_per_s": len(done2) / wall2,
+           "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+           "fp64_one_at_a_time_solves_per_s": n_req / est, "fp64_subset": int(len(subset)),
+           "speedup_vs_fp64_one_at_a_time": est / wall,
+           "k_rel_err_max_perm": float(max(errs)) if errs else None,
+           "k_rel_err_max_relperm": float(max(errs_tp)) if errs_tp else None,
+           "two_phase_stop": "|dk_i| <= 1e-4 (|k_0|+|k_1|) per 100 steps, at most 20000 steps",
+           "steps_median": int(np.median([d["steps"] for d in done]))}
+    log("FP64 one at a time (subset %d, scaled): %.3f solves/s -> x%.2f; k err perm %s relperm %s" %
+        (len(subset), out["fp64_one_at_a_time_solves_per_s"], out["speedup_vs_fp64_one_at_a_time"],
+         out["k_rel_err_max_perm"], out["k_rel_err_max_relperm"]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -224,6 +438,10 @@ def main():
     ap.add_argument("--no-perm", action="store_true")
     ap.add_argument("--batch", type=int, default=512, help="ensemble size (0: skip)")
     ap.add_argument("--baseline-every", type=int, default=16)
+    ap.add_argument("--serve", type=int, default=48, help="requests per serving run (0: skip)")
+    ap.add_argument("--tp-size", type=int, default=192)
+    ap.add_argument("--tp-steps", type=int, default=50000, help="two-phase steps (0: skip)")
+    ap.add_argument("--serve-pool-gb", type=float, default=16.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box here has one GPU and the engine runs one "
@@ -267,6 +485,8 @@ def main():
                 "definition": "k = nu * sum_bulk(sum_q f c_x + F_x/2) / (N_box * F_x), nu = (tau-1/2)/3"}
 
     batched = batched_leg(args.batch, args.baseline_every, tau, force) if args.batch > 0 else None
+    twophase = twophase_leg(args.tp_size, args.tp_steps) if args.tp_steps > 0 else None
+    serving = serving_leg(args.serve, args.serve_pool_gb, 8) if args.serve > 0 else None
 
     cpu = None
     if not args.no_cpu:
@@ -299,6 +519,8 @@ def main():
                  "bytes_per_lup": bytes64, "traffic": traffic64},
         "permeability": perm,
         "batched": batched,
+        "twophase": twophase,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

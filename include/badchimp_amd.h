@@ -235,8 +235,9 @@ int lb_sphere_pack(int32_t nd, const int32_t* dims, double radius,
 
 /* ---- batched and serving form -------------------------------------------
  * No counterpart in the reference: the reference runs one solve per process
- * (std_case/main.cpp).  A batch keeps many independent single-phase
- * permeability solves (the std_case step with lb_solve_permeability's stop)
+ * (std_case/main.cpp).  A batch keeps many independent permeability
+ * solves (the std_case step with lb_solve_permeability's stop, or the
+ * colour-gradient step of twophase with phase permeabilities)
  * resident in one pre-allocated device pool of pool_bytes and advances all of
  * them with one launch per step.  A solve is admitted when its footprint
  * (populations x 2, coordinates, row-window table, block partials) fits into
@@ -251,23 +252,37 @@ int lb_batch_create(int lattice, int precision, int device, int64_t pool_bytes,
 int lb_batch_destroy(lb_batch* b);
 const char* lb_batch_last_error(const lb_batch* b);
 /* Bytes a solve of this geometry (int32 cells, > 0 fluid, C order over
- * x, y[, z]) takes in a pool, and its fluid node count. */
-int lb_batch_footprint(int lattice, int precision, int32_t nd, const int32_t* dims,
-                       const int32_t* geo, int32_t periodic_mask, int64_t* bytes,
-                       int64_t* n_fluid);
+ * x, y[, z]) takes in a pool (n_fields 1 single-phase, 2 two-phase), and its
+ * fluid node count. */
+int lb_batch_footprint(int lattice, int precision, int32_t n_fields, int32_t nd,
+                       const int32_t* dims, const int32_t* geo, int32_t periodic_mask,
+                       int64_t* bytes, int64_t* n_fluid);
 /* Admit a solve that starts at rest (f = w, rho = 1): LB_ERR_NOMEM when it
  * does not fit now (retry after others retire).  n_box = prod(dims). */
 int lb_batch_add(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* geo,
                  int32_t periodic_mask, double tau, const double* force, double tol,
                  int64_t max_steps, int64_t tag);
+/* Admit a colour-gradient two-phase solve (FP32S / FP16S batches): rho0 per
+ * geometry cell (phase-0 density, rho1 = 1 - rho0, f_k = w rho_k at rest),
+ * the solid-boundary colour from wettability (rho0 = wettability on every
+ * solid neighbour, main_TWOPHASE.cpp:171-178,280-284), fixed body force on the
+ * total (the LB_FORCE_FIXED mode of lb_step_cg).  It retires when both phase
+ * permeabilities k_i = nu_i <phi_i sum_q f c_x> / F_x (nu_i = (tau_i - 1/2)/3,
+ * LBglobalforcing.h:61-78) change by at most tol (|k_0| + |k_1|) between
+ * checks. */
+int lb_batch_add_twophase(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* geo,
+                          int32_t periodic_mask, const double* rho0, double wettability,
+                          double tau0, double tau1, double sigma, double beta,
+                          const double* force, double tol, int64_t max_steps, int64_t tag);
 /* Advance every resident solve by `intervals` check intervals; solves that
  * finish retire at the end of their interval.  Admissions take effect at the
  * next interval. */
 int lb_batch_advance(lb_batch* b, int32_t intervals, int32_t* n_finished);
-/* Take up to max_n finished solves: tag, k, steps, seconds resident, status
+/* Take up to max_n finished solves: tag, k (two-phase: k_0 + k_1),
+ * k_phase[2] (two-phase k_0, k_1), steps, seconds resident, status
  * (0 converged, 1 max_steps reached, 2 k not finite). */
-int lb_batch_collect(lb_batch* b, int32_t max_n, int64_t* tags, double* k, int64_t* steps,
-                     double* seconds, int32_t* status, int32_t* n_out);
+int lb_batch_collect(lb_batch* b, int32_t max_n, int64_t* tags, double* k, double* k_phase,
+                     int64_t* steps, double* seconds, int32_t* status, int32_t* n_out);
 int lb_batch_status(const lb_batch* b, int32_t* n_resident, int64_t* resident_nodes,
                     int64_t* bytes_used, int64_t* bytes_free, int64_t* largest_free);
 /* Launch shape: -1 autotune on live steps after each change of the resident

@@ -110,3 +110,58 @@ def test_fp16s_batch_error(gpu_available, geos):
     got = run_batch(geos, lbamd.FP16S, 50, 1e-6, 20000)
     for i in ref:
         assert abs(got[i][0] - ref[i][0]) / ref[i][0] < 2e-2
+
+
+def tp_single(geo, rho0, wett, precision, tau0, tau1, sigma, beta, steps):
+    """Phase permeabilities of the single-context two-phase path (fixed force)
+    over the state after steps - 1 steps, as the batch reports them."""
+    topo = lbamd.Topology.from_geometry(geo, lbamd.D3Q19, "x")
+    nb, types, bulk, pos = topo.neighbors(), topo.node_types(), topo.bulk(), topo.positions()
+    n = topo.n_nodes
+    r0 = np.zeros(n)
+    r0[1:] = rho0[pos[1:, 0], pos[1:, 1], pos[1:, 2]]
+    r1 = 1.0 - r0
+    sb = types == 1
+    r0[sb], r1[sb] = wett, 1.0 - wett
+    W = np.array([2.0] * 3 + [1.0] * 6 + [2.0] * 3 + [1.0] * 6 + [12.0]) / 36.0
+    f0 = np.zeros((n, 2, 19))
+    f0[bulk, 0] = W[None] * r0[bulk, None]
+    f0[bulk, 1] = W[None] * r1[bulk, None]
+    ctx = lbamd.Context(lbamd.D3Q19, precision)
+    ctx.set_topology_grid(nb, types, bulk, pos)
+    ctx.upload_f(f0)
+    ctx.set_wall_densities(np.stack([r0, r1], 1))
+    ctx.step_cg(steps - 1, tau0, tau1, sigma, beta, lbamd.FORCE_FIXED, FORCE)
+    s0, s1, s2 = ctx.reduce_flux(0)
+    ctx.close()
+    nbox = float(geo.size)
+    return (tau0 - 0.5) / 3.0 * (s1 / nbox) / FORCE[0], (tau1 - 0.5) / 3.0 * (s2 / nbox) / FORCE[0]
+
+
+@pytest.mark.parametrize("precision", [lbamd.FP32S, lbamd.FP16S])
+def test_batch_twophase_equals_single_context(gpu_available, geos, precision):
+    """Two-phase solves in a batch (mixed with single-phase ones, admitted at
+    different times) give the phase permeabilities of the single-context
+    two-phase path bit for bit."""
+    rng = np.random.default_rng(5)
+    params = (1.0, 0.8, 0.01, 0.7)
+    steps = 200
+    rho0s = [np.where(g > 0, (rng.random(g.shape) < 0.5).astype(float), 0.0) for g in geos[:2]]
+    b = lbamd.Batch(lbamd.D3Q19, precision, 1 << 28, steps)
+    assert b.add(geos[2], "x", TAU, FORCE, 1e-6, 20000, 99)
+    assert b.add_twophase(geos[0], "x", rho0s[0], 1.0, *params, FORCE, 1e-12, steps, 0)
+    b.advance(1)
+    assert b.add_twophase(geos[1], "x", rho0s[1], 0.7, *params, FORCE, 1e-12, steps, 1)
+    got = {}
+    while len(got) < 3:
+        b.advance(1)
+        for r in b.collect():
+            got[r["tag"]] = r
+    b.close()
+    for i, wett in ((0, 1.0), (1, 0.7)):
+        k0, k1 = tp_single(geos[i], rho0s[i], wett, precision, *params, steps)
+        assert got[i]["status"] == 1 and got[i]["steps"] == steps
+        assert got[i]["k_phase"] == (k0, k1), (i, got[i]["k_phase"], (k0, k1))
+    # the single-phase solve is unaffected by its two-phase neighbours
+    alone = run_batch([geos[2]], precision, steps, 1e-6, 20000)[0]
+    assert (got[99]["k"], got[99]["steps"]) == alone[:2]
