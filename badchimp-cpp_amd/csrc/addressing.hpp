@@ -216,6 +216,43 @@ __device__ __forceinline__ void rowwin_links(const RowWin& ad, int32_t i, uint32
   }
 }
 
+// Colour of the neighbours neighbor(qq, n) for the colour gradient
+// (LButilities.h:12-22): a fluid neighbour's phi; a solid neighbour's wall
+// colour, per wall slot with the link table or one value for the whole node
+// list with the row windows; 0 for the ghost node 0.  pn[Q-1] = 0.
+struct WallPhi {
+  const double* slot;  // per wall slot (link table)
+  double uniform;      // every solid neighbour (row windows)
+};
+
+template <class L, typename T>
+__device__ __forceinline__ void neighbour_phi(const TableAddr& ad, int32_t i, const T* __restrict__ phi,
+                                              const WallPhi& wp, T (&pn)[L::nQ]) {
+#pragma unroll
+  for (int qq = 0; qq < L::nQ - 1; ++qq) {
+    const int q = L::rev(qq);  // neighbor(qq, n) is the stream source of rev(qq)
+    const uint32_t l = ad.link[static_cast<uint32_t>(q) * ad.NP + i];
+    const uint32_t kind = l >> 30;
+    const uint32_t j = l & kLinkIndexMask;
+    pn[qq] = kind == kLinkStream ? phi[j] : (kind == kLinkBounce ? static_cast<T>(wp.slot[j]) : static_cast<T>(0));
+  }
+  pn[L::nQ - 1] = static_cast<T>(0);
+}
+
+template <class L, typename T>
+__device__ __forceinline__ void neighbour_phi(const RowWin& ad, int32_t i, const T* __restrict__ phi,
+                                              const WallPhi& wp, T (&pn)[L::nQ]) {
+  uint32_t j[L::nQ - 1], fluid;
+  rowwin_links<L>(ad, i, j, fluid);
+  const T pw = static_cast<T>(wp.uniform);
+#pragma unroll
+  for (int qq = 0; qq < L::nQ - 1; ++qq) {
+    const int q = L::rev(qq);
+    pn[qq] = (fluid >> q) & 1u ? phi[j[q]] : pw;
+  }
+  pn[L::nQ - 1] = static_cast<T>(0);
+}
+
 // Element access of a population buffer: 32-bit byte offsets from the base
 // when the layout guarantees the buffer is below 4 GiB (global loads with a
 // scalar base and a 32-bit vector offset), 64-bit addresses otherwise.

@@ -372,6 +372,8 @@ int setup_topology(lb_ctx* c, int32_t n_nodes, const int32_t* neighbor, const in
   if ((rc = dev_alloc(c, &c->d_partial, 3 * c->n_blocks)) != LB_OK) return rc;
   if ((rc = dev_alloc(c, &c->d_scalar, 8)) != LB_OK) return rc;
   LB_HIP(c, hipMemset(c->d_phi_wall, 0, (walls.size() + 1) * sizeof(double)));
+  c->wall_uniform = true;
+  c->wall_phi = 0.0;
   LB_HIP(c, hipMemset(c->d_rho, 0, 2 * NP * sizeof(double)));
   LB_HIP(c, hipMemset(c->d_vel, 0, c->nD * NP * sizeof(double)));
   LB_HIP(c, hipMemset(c->d_phi, 0, NP * sizeof(double)));
@@ -644,6 +646,11 @@ int lb_set_wall_densities(lb_ctx* c, const double* rho2) {
     }
   }
   LB_HIP(c, hipMemcpy(c->d_phi_wall, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  // one colour on every wall slot lets the two-phase steps use the row windows
+  c->wall_uniform = true;
+  c->wall_phi = c->wall_nodes.empty() ? 0.0 : h[0];
+  for (size_t k = 0; k < c->wall_nodes.size(); ++k)
+    if (!(h[k] == c->wall_phi)) c->wall_uniform = false;
   return LB_OK;
 }
 
@@ -702,6 +709,11 @@ int lb_step_cg(lb_ctx* c, int32_t nsteps, double tau0, double tau1, double sigma
   a.momx_mode = force_mode == LB_FORCE_MOMX;
   a.momx = force[0];
   const bool momx = a.momx_mode != 0;
+  // row windows when every solid neighbour has one colour and both fields fit
+  // the 32-bit byte offsets of that layout; else the link table
+  const bool rowwin = c->addr.layout == 2 && c->wall_uniform &&
+                      2ull * c->Q * static_cast<uint64_t>(c->NP) * elem_size(c->precision) < (1ull << 32);
+  const WallPhi wp{c->d_phi_wall, c->wall_phi};
   Timer t(c);
   int64_t launches = 0;
   for (int32_t s = 0; s < nsteps; ++s) {
@@ -710,23 +722,23 @@ int lb_step_cg(lb_ctx* c, int32_t nsteps, double tau0, double tau1, double sigma
     if (c->precision == LB_FP64) {
       const double* fin = static_cast<const double*>(c->d_f[c->cur]);
       double* fout = static_cast<double*>(c->d_f[1 - c->cur]);
-      e = cg_phi_f64(c->lattice, !c->post, momx, fin, c->d_link, c->n_bulk, c->NP, c->d_phi, c->d_partial,
+      e = cg_phi_f64(c->lattice, !c->post, momx, fin, c->addr, rowwin, c->n_bulk, c->NP, c->d_phi, c->d_partial,
                      c->stream);
       if (e == hipSuccess && momx)
         e = finish_momx(c->d_partial, c->n_blocks, a.momx, c->n_global, c->d_scalar, c->stream);
       if (e == hipSuccess)
-        e = cg_f64(c->lattice, !c->post, last, fin, fout, c->d_link, c->n_bulk, c->NP, a, c->d_phi,
-                   c->d_phi_wall, c->d_scalar, c->d_rho, c->d_vel, c->stream);
+        e = cg_f64(c->lattice, !c->post, last, fin, fout, c->addr, rowwin, c->n_bulk, c->NP, a, c->d_phi, wp,
+                   c->d_scalar, c->d_rho, c->d_vel, c->stream);
     } else {
       // the phi buffer holds float values in the reduced-precision forms
       float* phi = reinterpret_cast<float*>(c->d_phi);
-      e = cg_phi_lowp(c->precision, c->lattice, !c->post, momx, c->d_f[c->cur], c->d_link, c->n_bulk, c->NP,
-                      phi, c->d_partial, c->stream);
+      e = cg_phi_lowp(c->precision, c->lattice, !c->post, momx, c->d_f[c->cur], c->addr, rowwin, c->n_bulk,
+                      c->NP, phi, c->d_partial, c->stream);
       if (e == hipSuccess && momx)
         e = finish_momx(c->d_partial, c->n_blocks, a.momx, c->n_global, c->d_scalar, c->stream);
       if (e == hipSuccess)
-        e = cg_lowp(c->precision, c->lattice, !c->post, last, c->d_f[c->cur], c->d_f[1 - c->cur], c->d_link,
-                    c->n_bulk, c->NP, a, phi, c->d_phi_wall, c->d_scalar, c->d_rho, c->d_vel, c->stream);
+        e = cg_lowp(c->precision, c->lattice, !c->post, last, c->d_f[c->cur], c->d_f[1 - c->cur], c->addr, rowwin,
+                    c->n_bulk, c->NP, a, phi, wp, c->d_scalar, c->d_rho, c->d_vel, c->stream);
     }
     if (e != hipSuccess) return fail(c, LB_ERR_HIP, "two-phase launch: %s", hipGetErrorString(e));
     launches += momx ? 3 : 2;

@@ -36,9 +36,13 @@ constexpr int kBlock = 256;
 // multiple of kPB).
 constexpr int kPLog = 6;
 constexpr uint32_t kPB = 1u << kPLog;
+// Written as i + 64 (Q - 1) (i >> 6) + 64 q (the same value), so that the
+// per-node part is one 24-bit multiply-add (i >> 6 < 2^24 for every element
+// index below 2^32, which a context guarantees) and the direction part is a
+// constant that folds into the memory instruction's offset.
 template <int Q>
 __host__ __device__ __forceinline__ constexpr uint32_t pidx(uint32_t q, uint32_t i) {
-  return (((i >> kPLog) * static_cast<uint32_t>(Q) + q) << kPLog) | (i & (kPB - 1u));
+  return ((i >> kPLog) & 0xFFFFFFu) * static_cast<uint32_t>((Q - 1) << kPLog) + i + (q << kPLog);
 }
 inline uint32_t pidx_host(int Q, uint32_t q, uint32_t i) {
   return (((i >> kPLog) * static_cast<uint32_t>(Q) + q) << kPLog) | (i & (kPB - 1u));

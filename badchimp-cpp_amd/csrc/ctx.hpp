@@ -39,6 +39,8 @@ struct lb_ctx {
   double* d_vel = nullptr;       // [nD][NP]
   double* d_phi = nullptr;       // [NP]
   double* d_phi_wall = nullptr;  // [n_wall + 1]
+  bool wall_uniform = true;      // every wall slot has the colour wall_phi
+  double wall_phi = 0.0;
   double* d_partial = nullptr;   // [3][n_blocks]
   double* d_scalar = nullptr;    // [8]: 0 force x, 1 flux sum, 4..6 reductions
   int64_t n_blocks = 0;

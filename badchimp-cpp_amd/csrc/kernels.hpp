@@ -26,15 +26,15 @@ hipError_t bgk_f64(int lattice, bool first, bool store, bool reduce, const doubl
                    double* rho, double* vel, double* partial, hipStream_t s);
 
 // Two-phase pass 1 (+ pass 2 partial sums): phi = (rho0-rho1)/(rho0+rho1)
-// and sum_q (f0+f1) c_x per block (main_TWOPHASE.cpp:238-298).  Table layout.
-hipError_t cg_phi_f64(int lattice, bool first, bool momx, const double* fin,
-                      const uint32_t* link, int32_t n, int64_t NP, double* phi,
-                      double* partial, hipStream_t s);
-// Two-phase pass 3 (main_TWOPHASE.cpp:312-374) and bounce-back.  Table layout.
-hipError_t cg_f64(int lattice, bool first, bool store, const double* fin, double* fout,
-                  const uint32_t* link, int32_t n, int64_t NP, const CgArgs& a,
-                  const double* phi, const double* phi_wall, const double* dscalar,
-                  double* rho, double* vel, hipStream_t s);
+// and sum_q (f0+f1) c_x per block (main_TWOPHASE.cpp:238-298).  rowwin: the
+// row windows of ad (every solid neighbour has the colour wp.uniform), else
+// the link table (wall colour per slot).
+hipError_t cg_phi_f64(int lattice, bool first, bool momx, const double* fin, const Addr& ad, bool rowwin,
+                      int32_t n, int64_t NP, double* phi, double* partial, hipStream_t s);
+// Two-phase pass 3 (main_TWOPHASE.cpp:312-374) and bounce-back.
+hipError_t cg_f64(int lattice, bool first, bool store, const double* fin, double* fout, const Addr& ad,
+                  bool rowwin, int32_t n, int64_t NP, const CgArgs& a, const double* phi, const WallPhi& wp,
+                  const double* dscalar, double* rho, double* vel, hipStream_t s);
 // Streaming only: fout = pre-collision state of the post-collision buffer fin.
 hipError_t gather_f64(int lattice, int nfields, const double* fin, double* fout,
                       const Addr& ad, int32_t n, int64_t NP, hipStream_t s);

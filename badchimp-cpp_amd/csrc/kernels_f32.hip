@@ -12,6 +12,8 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "kernels_f32.hpp"
 #include "lattice.hpp"
 
@@ -104,7 +106,8 @@ __device__ __forceinline__ double bgk_node_lp(const S* __restrict__ gin, S* __re
   const float m3uF = -3.0f * dot<L>(u, F);
   const float omt = 1.0f - tau_inv;
   const float r45 = 4.5f * rho, r3 = 3.0f * rho;
-  constexpr bool O = A::kOff32;
+  // one 64-bit address per node; the direction offsets fold into the stores
+  S* __restrict__ out = elem_ptr<A::kOff32>(gout, pidx<Q>(0, i));
 #pragma unroll
   for (int q = 0; q < Q - 1; ++q) {
     const int rq = L::rev(q);
@@ -119,14 +122,14 @@ __device__ __forceinline__ double bgk_node_lp(const S* __restrict__ gin, S* __re
     const float fod = 3.0f * cF;
     const float gq = omt * g[q] + wt * (ev + od) + wf * (fev + fod);
     const float gr = omt * g[rq] + wt * (ev - od) + wf * (fev - fod);
-    __builtin_nontemporal_store(Store<S>::put(gq), elem_ptr<O>(gout, pidx<Q>(q, i)));
-    __builtin_nontemporal_store(Store<S>::put(gr), elem_ptr<O>(gout, pidx<Q>(rq, i)));
+    __builtin_nontemporal_store(Store<S>::put(gq), out + (q << kPLog));
+    __builtin_nontemporal_store(Store<S>::put(gr), out + (rq << kPLog));
   }
   {
     constexpr int q = Q - 1;  // rest: c = 0
     const float w = float(L::w(q));
     const float gq = omt * g[q] + (w * tau_inv) * a0 + (w * tau_factor) * m3uF;
-    __builtin_nontemporal_store(Store<S>::put(gq), elem_ptr<O>(gout, pidx<Q>(q, i)));
+    __builtin_nontemporal_store(Store<S>::put(gq), out + (q << kPLog));
   }
   return REDUCE ? static_cast<double>(jd[0]) : 0.0;
 }
@@ -347,6 +350,8 @@ __device__ __forceinline__ void cg_collide(const float (&t)[L::nQ], const float 
   const float m3uF = -3.0f * dot<L>(u, F);
   const float omt = 1.0f - ti;
   const float r45 = 4.5f * rho, r3 = 3.0f * rho;
+  S* __restrict__ out = elem_ptr<O>(gout, pidx<Q>(0, i));
+  S* __restrict__ outd = elem_ptr<O>(gout, fs + pidx<Q>(0, i));
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const float w = float(L::w(q));
@@ -364,15 +369,15 @@ __device__ __forceinline__ void cg_collide(const float (&t)[L::nQ], const float 
       rc = 0.0f;
     }
     const float bs = omt * t[q] + ti * geq + dF + st;
-    __builtin_nontemporal_store(Store<S>::put(bs), elem_ptr<O>(gout, pidx<Q>(q, i)));
-    __builtin_nontemporal_store(Store<S>::put(phi_n * (w + bs) + 2.0f * rc), elem_ptr<O>(gout, fs + pidx<Q>(q, i)));
+    __builtin_nontemporal_store(Store<S>::put(bs), out + (q << kPLog));
+    __builtin_nontemporal_store(Store<S>::put(phi_n * (w + bs) + 2.0f * rc), outd + (q << kPLog));
   }
 }
 
 // Two-phase pass 1 (+ the momentum sums of the flux controller, in FP64:
 // sum_q (f0 + f1) c_x = sum_q t c_x, main_TWOPHASE.cpp:292-308).
-template <class L, typename S, bool FIRST, bool MOMX>
-__global__ __launch_bounds__(kBlock) void k_cg_phi_lp(const S* __restrict__ gin, TableAddr ad, int32_t n,
+template <class L, typename S, bool FIRST, bool MOMX, class A>
+__global__ __launch_bounds__(kBlock) void k_cg_phi_lp(const S* __restrict__ gin, A ad, int32_t n,
                                                       int64_t NP, float* __restrict__ phi,
                                                       double* __restrict__ partial) {
   constexpr int Q = L::nQ;
@@ -391,14 +396,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_phi_lp(const S* __restrict__ gin,
   }
 }
 
-// Two-phase pass 3 with the link table; solid neighbours take the wall
-// colour of their slot.
-template <class L, typename S, bool FIRST, bool STORE>
-__global__ __launch_bounds__(kBlock) void k_cg_lp(const S* __restrict__ gin, S* __restrict__ gout, TableAddr ad,
+// Two-phase pass 3; solid neighbours take their wall colour (neighbour_phi).
+template <class L, typename S, bool FIRST, bool STORE, class A>
+__global__ __launch_bounds__(kBlock) void k_cg_lp(const S* __restrict__ gin, S* __restrict__ gout, A ad,
                                                   int32_t n, int64_t NP, float nu0Inv, float nu1Inv,
                                                   float sigma, float beta, float F0, float F1, float F2,
-                                                  int32_t momx_mode, const float* __restrict__ phi,
-                                                  const double* __restrict__ phi_wall,
+                                                  int32_t momx_mode, const float* __restrict__ phi, WallPhi wp,
                                                   const double* __restrict__ dscalar,
                                                   double* __restrict__ rho_o, double* __restrict__ vel_o) {
   constexpr int Q = L::nQ;
@@ -411,15 +414,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_lp(const S* __restrict__ gin, S* 
   float F[3] = {F0, F1, F2};
   if (momx_mode) F[0] = static_cast<float>(dscalar[0]);
   float pn[Q];
-#pragma unroll
-  for (int qq = 0; qq < Q - 1; ++qq) {
-    const int q = L::rev(qq);  // neighbor(qq, n) is the stream source of rev(qq)
-    const uint32_t l = ad.link[static_cast<uint32_t>(q) * ad.NP + i];
-    const uint32_t kind = l >> 30;
-    const uint32_t j = l & kLinkIndexMask;
-    pn[qq] = kind == kLinkStream ? phi[j] : (kind == kLinkBounce ? static_cast<float>(phi_wall[j]) : 0.0f);
-  }
-  cg_collide<L, S, false, STORE>(t, dd, pn, F, nu0Inv, nu1Inv, sigma, beta, gout, fs, i, NP, rho_o, vel_o);
+  neighbour_phi<L>(ad, i, phi, wp, pn);
+  cg_collide<L, S, A::kOff32, STORE>(t, dd, pn, F, nu0Inv, nu1Inv, sigma, beta, gout, fs, i, NP, rho_o, vel_o);
 }
 
 // Batched two-phase passes (items as k_batch_bgk_lp; fixed force; every solid
@@ -685,53 +681,63 @@ hipError_t batch_cg_lowp(int precision, int lattice, bool reduce, const BatchSol
   });
 }
 
-hipError_t cg_phi_lowp(int precision, int lattice, bool first, bool momx, const void* gin,
-                       const uint32_t* link, int32_t n, int64_t NP, float* phi, double* partial,
-                       hipStream_t s) {
+hipError_t cg_phi_lowp(int precision, int lattice, bool first, bool momx, const void* gin, const Addr& ad,
+                       bool rowwin, int32_t n, int64_t NP, float* phi, double* partial, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  const TableAddr ta{link, static_cast<uint32_t>(NP)};
-  return dispatch(precision, lattice, Addr{}, [&](auto l, auto st, auto) {
+  Addr sel = ad;
+  sel.layout = rowwin ? 2 : 0;
+  return dispatch(precision, lattice, sel, [&](auto l, auto st, auto addr) {
     using L = decltype(l);
     using S = decltype(st);
-    const S* gi = static_cast<const S*>(gin);
-    const dim3 g = grid_for(n), b(kBlock);
-    if (first) {
-      if (momx) k_cg_phi_lp<L, S, true, true><<<g, b, 0, s>>>(gi, ta, n, NP, phi, partial);
-      else k_cg_phi_lp<L, S, true, false><<<g, b, 0, s>>>(gi, ta, n, NP, phi, partial);
+    using A = decltype(addr);
+    if constexpr (std::is_same<A, RowAddr>::value) {
+      return;  // not used for two-phase
     } else {
-      if (momx) k_cg_phi_lp<L, S, false, true><<<g, b, 0, s>>>(gi, ta, n, NP, phi, partial);
-      else k_cg_phi_lp<L, S, false, false><<<g, b, 0, s>>>(gi, ta, n, NP, phi, partial);
+      const S* gi = static_cast<const S*>(gin);
+      const dim3 g = grid_for(n), b(kBlock);
+      if (first) {
+        if (momx) k_cg_phi_lp<L, S, true, true, A><<<g, b, 0, s>>>(gi, addr, n, NP, phi, partial);
+        else k_cg_phi_lp<L, S, true, false, A><<<g, b, 0, s>>>(gi, addr, n, NP, phi, partial);
+      } else {
+        if (momx) k_cg_phi_lp<L, S, false, true, A><<<g, b, 0, s>>>(gi, addr, n, NP, phi, partial);
+        else k_cg_phi_lp<L, S, false, false, A><<<g, b, 0, s>>>(gi, addr, n, NP, phi, partial);
+      }
     }
   });
 }
 
-hipError_t cg_lowp(int precision, int lattice, bool first, bool store, const void* gin, void* gout,
-                   const uint32_t* link, int32_t n, int64_t NP, const CgArgs& a, const float* phi,
-                   const double* phi_wall, const double* dscalar, double* rho, double* vel,
-                   hipStream_t s) {
+hipError_t cg_lowp(int precision, int lattice, bool first, bool store, const void* gin, void* gout, const Addr& ad,
+                   bool rowwin, int32_t n, int64_t NP, const CgArgs& a, const float* phi, const WallPhi& wp,
+                   const double* dscalar, double* rho, double* vel, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   if (a.q0 != 0.0 || a.q1 != 0.0) return hipErrorInvalidValue;  // no mass sources in this form
-  const TableAddr ta{link, static_cast<uint32_t>(NP)};
   const float n0 = static_cast<float>(a.nu0Inv), n1 = static_cast<float>(a.nu1Inv);
   const float sg = static_cast<float>(a.sigma), be = static_cast<float>(a.beta);
   const float F0 = static_cast<float>(a.F[0]), F1 = static_cast<float>(a.F[1]), F2 = static_cast<float>(a.F[2]);
-  return dispatch(precision, lattice, Addr{}, [&](auto l, auto st, auto) {
+  Addr sel = ad;
+  sel.layout = rowwin ? 2 : 0;
+  return dispatch(precision, lattice, sel, [&](auto l, auto st, auto addr) {
     using L = decltype(l);
     using S = decltype(st);
-    const S* gi = static_cast<const S*>(gin);
-    S* go = static_cast<S*>(gout);
-    const dim3 g = grid_for(n), b(kBlock);
-#define LB_K(F_, S_)                                                                                       \
-  k_cg_lp<L, S, F_, S_><<<g, b, 0, s>>>(gi, go, ta, n, NP, n0, n1, sg, be, F0, F1, F2, a.momx_mode, phi, \
-                                        phi_wall, dscalar, rho, vel)
-    if (first) {
-      if (store) LB_K(true, true);
-      else LB_K(true, false);
+    using A = decltype(addr);
+    if constexpr (std::is_same<A, RowAddr>::value) {
+      return;  // not used for two-phase
     } else {
-      if (store) LB_K(false, true);
-      else LB_K(false, false);
-    }
+      const S* gi = static_cast<const S*>(gin);
+      S* go = static_cast<S*>(gout);
+      const dim3 g = grid_for(n), b(kBlock);
+#define LB_K(F_, S_)                                                                                        \
+  k_cg_lp<L, S, F_, S_, A><<<g, b, 0, s>>>(gi, go, addr, n, NP, n0, n1, sg, be, F0, F1, F2, a.momx_mode, phi, \
+                                           wp, dscalar, rho, vel)
+      if (first) {
+        if (store) LB_K(true, true);
+        else LB_K(true, false);
+      } else {
+        if (store) LB_K(false, true);
+        else LB_K(false, false);
+      }
 #undef LB_K
+    }
   });
 }
 

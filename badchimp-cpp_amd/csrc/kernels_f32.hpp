@@ -35,14 +35,12 @@ hipError_t batch_bgk_lowp(int precision, int lattice, bool reduce, const BatchSo
 // Both passes of one batched two-phase step (see k_batch_cg_phi_lp).
 hipError_t batch_cg_lowp(int precision, int lattice, bool reduce, const BatchSolve* solves, const uint32_t* items,
                          int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, hipStream_t s);
-// Two-phase populations are stored as (t, d) = (f0 + f1 - w, f0 - f1) and use
-// the link table.
-hipError_t cg_phi_lowp(int precision, int lattice, bool first, bool momx, const void* gin,
-                       const uint32_t* link, int32_t n, int64_t NP, float* phi, double* partial,
-                       hipStream_t s);
-hipError_t cg_lowp(int precision, int lattice, bool first, bool store, const void* gin, void* gout,
-                   const uint32_t* link, int32_t n, int64_t NP, const CgArgs& a, const float* phi,
-                   const double* phi_wall, const double* dscalar, double* rho, double* vel,
-                   hipStream_t s);
+// Two-phase populations are stored as (t, d) = (f0 + f1 - w, f0 - f1).
+// rowwin: the row windows of ad with one wall colour, else the link table.
+hipError_t cg_phi_lowp(int precision, int lattice, bool first, bool momx, const void* gin, const Addr& ad,
+                       bool rowwin, int32_t n, int64_t NP, float* phi, double* partial, hipStream_t s);
+hipError_t cg_lowp(int precision, int lattice, bool first, bool store, const void* gin, void* gout, const Addr& ad,
+                   bool rowwin, int32_t n, int64_t NP, const CgArgs& a, const float* phi, const WallPhi& wp,
+                   const double* dscalar, double* rho, double* vel, hipStream_t s);
 
 }  // namespace lbamd

@@ -85,6 +85,7 @@ __device__ __forceinline__ double bgk_node(const double* __restrict__ fin, doubl
   const double uF = dot<L>(u, a.F);
   const double c2u2 = L::c2 * u2;
   const double c2uF = L::c2 * uF;
+  double* __restrict__ out = elem_ptr<A::kOff32>(fout, pidx<Q>(0, i));
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const double cu = cdot<L>(q, u);
@@ -94,7 +95,7 @@ __device__ __forceinline__ double bgk_node(const double* __restrict__ fin, doubl
     const double om = (-a.tau_inv) * (f[0][q] - feq);
     // calcDeltaOmegaF (LBcollision.h:206-211)
     const double dF = (L::w(q) * a.tau_factor) * (L::c2Inv * cF + L::c4Inv * (cF * cu - c2uF));
-    __builtin_nontemporal_store((f[0][q] + om) + dF, elem_ptr<A::kOff32>(fout, pidx<Q>(q, i)));
+    __builtin_nontemporal_store((f[0][q] + om) + dF, out + (q << kPLog));
   }
   return jx;
 }
@@ -175,8 +176,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_rest(double* __restrict__ f, i
 }
 
 // ------------------------------------------------------------------ two-phase pass 1
-template <class L, bool FIRST, bool MOMX>
-__global__ __launch_bounds__(kBlock) void k_cg_phi(const double* __restrict__ fin, TableAddr ad,
+template <class L, bool FIRST, bool MOMX, class A>
+__global__ __launch_bounds__(kBlock) void k_cg_phi(const double* __restrict__ fin, A ad,
                                                    int32_t n, int64_t NP, double* __restrict__ phi,
                                                    double* __restrict__ partial) {
   constexpr int Q = L::nQ;
@@ -203,12 +204,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_phi(const double* __restrict__ fi
 }
 
 // ------------------------------------------------------------------ two-phase pass 3
-template <class L, bool FIRST, bool STORE>
+template <class L, bool FIRST, bool STORE, class A>
 __global__ __launch_bounds__(kBlock) void k_cg(const double* __restrict__ fin,
-                                               double* __restrict__ fout, TableAddr ad, int32_t n,
+                                               double* __restrict__ fout, A ad, int32_t n,
                                                int64_t NP, CgArgs a,
-                                               const double* __restrict__ phi,
-                                               const double* __restrict__ phi_wall,
+                                               const double* __restrict__ phi, WallPhi wp,
                                                const double* __restrict__ dscalar,
                                                double* __restrict__ rho_o,
                                                double* __restrict__ vel_o) {
@@ -251,16 +251,7 @@ __global__ __launch_bounds__(kBlock) void k_cg(const double* __restrict__ fin,
   const double c2uF = L::c2 * uF;
   // colour gradient from the neighbours' phi (LButilities.h:12-22, :62-66)
   double pn[Q];
-#pragma unroll
-  for (int qq = 0; qq < Q - 1; ++qq) {
-    // neighbor(qq, n) is the stream source of direction rev(qq)
-    const int q = L::rev(qq);
-    const uint32_t l = ad.link[q * NP + i];
-    const uint32_t kind = l >> 30;
-    const uint32_t j = l & kLinkIndexMask;
-    pn[qq] = kind == kLinkStream ? phi[j] : (kind == kLinkBounce ? phi_wall[j] : 0.0);
-  }
-  pn[Q - 1] = 0.0;
+  neighbour_phi<L>(ad, i, phi, wp, pn);
   double g[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) g[d] = grad<L>(pn, d);
@@ -274,6 +265,8 @@ __global__ __launch_bounds__(kBlock) void k_cg(const double* __restrict__ fin,
   const double c0 = r0 / rho;
   const double c1 = r1 / rho;
   const int64_t fstride = static_cast<int64_t>(Q) * NP;
+  double* __restrict__ out0 = elem_ptr<A::kOff32>(fout, pidx<Q>(0, i));
+  double* __restrict__ out1 = elem_ptr<A::kOff32>(fout, static_cast<uint32_t>(fstride) + pidx<Q>(0, i));
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const double cu = cdot<L>(q, u);
@@ -293,8 +286,8 @@ __global__ __launch_bounds__(kBlock) void k_cg(const double* __restrict__ fin,
       rc = 0.0;
     }
     const double base = ((ft[q] + om) + dF) + st;
-    __builtin_nontemporal_store(((c0 * base) + rc) + dQ0, &fout[pidx<Q>(q, i)]);
-    __builtin_nontemporal_store(((c1 * base) - rc) + dQ1, &fout[fstride + pidx<Q>(q, i)]);
+    __builtin_nontemporal_store(((c0 * base) + rc) + dQ0, out0 + (q << kPLog));
+    __builtin_nontemporal_store(((c1 * base) - rc) + dQ1, out1 + (q << kPLog));
   }
 }
 
@@ -444,34 +437,40 @@ hipError_t bgk_f64(int lattice, bool first, bool store, bool reduce, const doubl
   return hipGetLastError();
 }
 
-hipError_t cg_phi_f64(int lattice, bool first, bool momx, const double* fin,
-                      const uint32_t* link, int32_t n, int64_t NP, double* phi,
-                      double* partial, hipStream_t s) {
+hipError_t cg_phi_f64(int lattice, bool first, bool momx, const double* fin, const Addr& ad, bool rowwin,
+                      int32_t n, int64_t NP, double* phi, double* partial, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const dim3 g = grid_for(n), b(kBlock);
-  const TableAddr ad{link, static_cast<uint32_t>(NP)};
-  LB_DISPATCH_LATTICE(lattice, L, {
-    if (first && momx) k_cg_phi<L, true, true><<<g, b, 0, s>>>(fin, ad, n, NP, phi, partial);
-    else if (first) k_cg_phi<L, true, false><<<g, b, 0, s>>>(fin, ad, n, NP, phi, partial);
-    else if (momx) k_cg_phi<L, false, true><<<g, b, 0, s>>>(fin, ad, n, NP, phi, partial);
-    else k_cg_phi<L, false, false><<<g, b, 0, s>>>(fin, ad, n, NP, phi, partial);
-  });
+  auto go = [&](auto addr) {
+    using A = decltype(addr);
+    LB_DISPATCH_LATTICE(lattice, L, {
+      if (first && momx) k_cg_phi<L, true, true, A><<<g, b, 0, s>>>(fin, addr, n, NP, phi, partial);
+      else if (first) k_cg_phi<L, true, false, A><<<g, b, 0, s>>>(fin, addr, n, NP, phi, partial);
+      else if (momx) k_cg_phi<L, false, true, A><<<g, b, 0, s>>>(fin, addr, n, NP, phi, partial);
+      else k_cg_phi<L, false, false, A><<<g, b, 0, s>>>(fin, addr, n, NP, phi, partial);
+    });
+  };
+  if (rowwin) go(ad.win);
+  else go(ad.table);
   return hipGetLastError();
 }
 
-hipError_t cg_f64(int lattice, bool first, bool store, const double* fin, double* fout,
-                  const uint32_t* link, int32_t n, int64_t NP, const CgArgs& a,
-                  const double* phi, const double* phi_wall, const double* dscalar,
-                  double* rho, double* vel, hipStream_t s) {
+hipError_t cg_f64(int lattice, bool first, bool store, const double* fin, double* fout, const Addr& ad,
+                  bool rowwin, int32_t n, int64_t NP, const CgArgs& a, const double* phi, const WallPhi& wp,
+                  const double* dscalar, double* rho, double* vel, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const dim3 g = grid_for(n), b(kBlock);
-  const TableAddr ad{link, static_cast<uint32_t>(NP)};
-  LB_DISPATCH_LATTICE(lattice, L, {
-    if (first && store) k_cg<L, true, true><<<g, b, 0, s>>>(fin, fout, ad, n, NP, a, phi, phi_wall, dscalar, rho, vel);
-    else if (first) k_cg<L, true, false><<<g, b, 0, s>>>(fin, fout, ad, n, NP, a, phi, phi_wall, dscalar, rho, vel);
-    else if (store) k_cg<L, false, true><<<g, b, 0, s>>>(fin, fout, ad, n, NP, a, phi, phi_wall, dscalar, rho, vel);
-    else k_cg<L, false, false><<<g, b, 0, s>>>(fin, fout, ad, n, NP, a, phi, phi_wall, dscalar, rho, vel);
-  });
+  auto go = [&](auto addr) {
+    using A = decltype(addr);
+    LB_DISPATCH_LATTICE(lattice, L, {
+      if (first && store) k_cg<L, true, true, A><<<g, b, 0, s>>>(fin, fout, addr, n, NP, a, phi, wp, dscalar, rho, vel);
+      else if (first) k_cg<L, true, false, A><<<g, b, 0, s>>>(fin, fout, addr, n, NP, a, phi, wp, dscalar, rho, vel);
+      else if (store) k_cg<L, false, true, A><<<g, b, 0, s>>>(fin, fout, addr, n, NP, a, phi, wp, dscalar, rho, vel);
+      else k_cg<L, false, false, A><<<g, b, 0, s>>>(fin, fout, addr, n, NP, a, phi, wp, dscalar, rho, vel);
+    });
+  };
+  if (rowwin) go(ad.win);
+  else go(ad.table);
   return hipGetLastError();
 }
 

@@ -281,7 +281,7 @@ def fp64_one(req):
         ctx.close()
         return k, steps, dt
     r0 = np.zeros(n)
-    r0[1:] = req.rho0[pos[1:, 0], pos[1:, 1], pos[1:, 2]]
+    r0[bulk] = req.rho0[pos[bulk, 0], pos[bulk, 1], pos[bulk, 2]]
     r1 = 1.0 - r0
     sb = types == 1
     r0[sb], r1[sb] = req.wett, 1.0 - req.wett
@@ -375,9 +375,8 @@ def serving_leg(n_req, pool_gb, every):
     pool = int(pool_gb * 1e9)
     # 1) burst: capacity
     reqs = request_mix(n_req, 7)
-    keep = {r.tag: r for r in reqs}  # geometries for the baseline (the server drops its copies)
     import copy
-    base_reqs = {r.tag: copy.copy(r) for r in reqs}
+    base_reqs = {r.tag: copy.copy(r) for r in reqs}  # the server drops its copies' geometries
     srv = Server(pool, precision=lbamd.FP32S, check_every=100)
     done, wall = run_stream(srv, reqs)
     srv.close()
@@ -394,22 +393,38 @@ def serving_leg(n_req, pool_gb, every):
     lat = np.array([d["latency"] for d in done2])
     log("serving Poisson %.2f/s: %d requests in %.1f s, p50 %.2f s, p99 %.2f s" %
         (rate, len(done2), wall2, np.percentile(lat, 50), np.percentile(lat, 99)))
-    # 3) plain FP64 one request at a time on a size-stratified subset of the burst
+    # 3) plain FP64 one request at a time on a subset of the burst stratified by
+    #    kind and size (every m-th request of each kind by cell count), each
+    #    kind's time scaled by its own count
     byres = {d["tag"]: d for d in done}
-    cells = np.array([base_reqs[t].geo.size for t in range(n_req)])
-    subset = np.argsort(cells, kind="stable")[every // 2::every]
-    t_one, errs, errs_tp = 0.0, [], []
-    for t in subset:
-        k, steps, dt = fp64_one(base_reqs[int(t)])
-        t_one += dt
-        got = byres[int(t)]
-        if base_reqs[int(t)].kind == "perm":
-            errs.append(abs(got["k"] - k) / abs(k))
-        else:
-            scale = abs(k[0]) + abs(k[1])
-            errs_tp.append(max(abs(got["k_phase"][0] - k[0]), abs(got["k_phase"][1] - k[1])) / scale)
-    est = t_one * n_req / len(subset)
-    del keep
+    est, errs, errs_tp, n_sub = 0.0, [], [], 0
+    for kind in ("perm", "relperm"):
+        tags = [t for t in range(n_req) if base_reqs[t].kind == kind]
+        if not tags:
+            continue
+        tags.sort(key=lambda t: base_reqs[t].geo.size)
+        m = max(1, min(every, len(tags)))
+        sub = tags[m // 2::m] or tags[:1]
+        t_kind = 0.0
+        for t in sub:
+            k, steps, dt = fp64_one(base_reqs[t])
+            t_kind += dt
+            got = byres[t]
+            if kind == "perm":
+                errs.append(abs(got["k"] - k) / abs(k))
+            else:
+                scale = abs(k[0]) + abs(k[1])
+                errs_tp.append(max(abs(got["k_phase"][0] - k[0]), abs(got["k_phase"][1] - k[1])) / scale)
+        est += t_kind * len(tags) / len(sub)
+        n_sub += len(sub)
+    per_kind = {}
+    for kind in ("perm", "relperm"):
+        ds = [d for d in done if d["kind"] == kind]
+        if ds:
+            per_kind[kind] = {"n": len(ds), "steps_median": int(np.median([d["steps"] for d in ds])),
+                              "max_steps_reached": int(sum(1 for d in ds if d["status"] == 1)),
+                              "latency_p50_s": float(np.percentile([d["latency"] for d in ds], 50))}
+    subset = range(n_sub)
     out = {"n_requests": n_req, "mix": "70% permeability / 30% relative permeability, L U(48,160)",
            "pool_GB": pool / 1e9,
            "burst_solves_per_s": cap, "burst_wall_s": wall,
@@ -420,7 +435,7 @@ def serving_leg(n_req, pool_gb, every):
            "k_rel_err_max_perm": float(max(errs)) if errs else None,
            "k_rel_err_max_relperm": float(max(errs_tp)) if errs_tp else None,
            "two_phase_stop": "|dk_i| <= 1e-4 (|k_0|+|k_1|) per 100 steps, at most 20000 steps",
-           "steps_median": int(np.median([d["steps"] for d in done]))}
+           "steps_median": int(np.median([d["steps"] for d in done])), "per_kind_burst": per_kind}
     log("FP64 one at a time (subset %d, scaled): %.3f solves/s -> x%.2f; k err perm %s relperm %s" %
         (len(subset), out["fp64_one_at_a_time_solves_per_s"], out["speedup_vs_fp64_one_at_a_time"],
          out["k_rel_err_max_perm"], out["k_rel_err_max_relperm"]))

@@ -119,7 +119,7 @@ def tp_single(geo, rho0, wett, precision, tau0, tau1, sigma, beta, steps):
     nb, types, bulk, pos = topo.neighbors(), topo.node_types(), topo.bulk(), topo.positions()
     n = topo.n_nodes
     r0 = np.zeros(n)
-    r0[1:] = rho0[pos[1:, 0], pos[1:, 1], pos[1:, 2]]
+    r0[bulk] = rho0[pos[bulk, 0], pos[bulk, 1], pos[bulk, 2]]
     r1 = 1.0 - r0
     sb = types == 1
     r0[sb], r1[sb] = wett, 1.0 - wett

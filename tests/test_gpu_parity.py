@@ -120,9 +120,12 @@ def tp_initial(T, d):
     return f0, np.stack([r0, r1], 1)
 
 
-def gpu_twophase(T, f0, rho, p, mode, force, nsteps, chunks=None):
-    ctx = lbamd.Context(T.lattice)
-    ctx.set_topology(T.neighbors, T.node_types, T.bulk)
+def gpu_twophase(T, f0, rho, p, mode, force, nsteps, chunks=None, layout="table", precision=lbamd.FP64):
+    ctx = lbamd.Context(T.lattice, precision)
+    if layout == "table":
+        ctx.set_topology(T.neighbors, T.node_types, T.bulk)
+    else:
+        ctx.set_topology_grid(T.neighbors, T.node_types, T.bulk, T.positions)
     ctx.upload_f(f0)
     ctx.set_wall_densities(rho)
     for n in (chunks or [nsteps]):
@@ -149,15 +152,19 @@ def test_twophase_vs_reference_fixture(gpu_available, name):
     assert abs(F[0] - d["force_x"][-1, 1]) <= 1e-12 * abs(d["force_x"][-1, 1])
 
 
-def test_twophase_fixed_force_bitexact(gpu_available):
-    d = load_ref("tp_pack_16")
+@pytest.mark.parametrize("name,layout", [("tp_pack_16", "table"), ("tp_walls_12x14x10", "table"),
+                                         ("tp_walls_12x14x10", "grid")])
+def test_twophase_fixed_force_bitexact(gpu_available, name, layout):
+    """Fixed-force colour gradient bit for bit against the oracle, with the
+    link table and (uniform wall colour) with the row windows."""
+    d = load_ref(name)
     p = d["params"]
     T = OracleTopo(d["geo"].astype(np.int64), p["lattice"], p["periodic"])
     f0, rho = tp_initial(T, d)
     force = [3e-6, 1e-6, -2e-6]
     fo, ro, vo, cgo, Fo = oracle.run_twophase(1, T.neighbors, T.node_types, T.bulk, f0, rho, p["tau0"], p["tau1"],
                                               p["sigma"], p["beta"], 0, force, 25)
-    f, r, v, phi, F = gpu_twophase(T, f0, rho, p, lbamd.FORCE_FIXED, force, 25, chunks=[1, 24])
+    f, r, v, phi, F = gpu_twophase(T, f0, rho, p, lbamd.FORCE_FIXED, force, 25, chunks=[1, 24], layout=layout)
     b = T.bulk
     assert np.array_equal(f[b], fo[b])
     assert np.array_equal(r[b], ro[b])
@@ -183,3 +190,21 @@ def test_flux_reduction(gpu_available):
         ref = oracle.flux(1, T.bulk, fo, 2, direction)
         assert np.all(np.abs(got - ref) <= 1e-12 * np.abs(ref).max())
     ctx.close()
+
+
+@pytest.mark.parametrize("precision", [lbamd.FP64, lbamd.FP32S, lbamd.FP16S])
+def test_twophase_layouts_identical(gpu_available, precision):
+    """The row-window two-phase path (uniform wall colour) gives the same bits
+    as the link-table path, in the momentum-controlled mode too."""
+    d = load_ref("tp_walls_12x14x10")
+    p = d["params"]
+    T = OracleTopo(d["geo"].astype(np.int64), p["lattice"], p["periodic"])
+    f0, rho = tp_initial(T, d)
+    outs = []
+    for layout in ("table", "grid"):
+        outs.append(gpu_twophase(T, f0, rho, p, lbamd.FORCE_MOMX, [p["momx"], 0.0, 0.0], 30, chunks=[1, 29],
+                                 layout=layout, precision=precision))
+    b = T.bulk
+    for x, y in zip(outs[0][:4], outs[1][:4]):
+        assert np.array_equal(x[b], y[b])
+    assert np.array_equal(outs[0][4], outs[1][4])
