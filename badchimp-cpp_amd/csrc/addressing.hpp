@@ -45,14 +45,11 @@
 
 namespace lbamd {
 
-// Logical block of a launch, so that each XCD (blocks are dispatched
-// round-robin over the 8 XCDs) works on one contiguous range of nodes and its
-// L2 sees the neighbours' rows.
-__device__ __forceinline__ int32_t xcd_block(int32_t b, int32_t nblocks) {
-  const int32_t q = nblocks >> 3, r = nblocks & 7;
-  const int32_t xcd = b & 7, slot = b >> 3;
-  return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
-}
+// Block order: consecutive blocks go to the XCDs round-robin (the hardware's
+// dispatch order), so all CUs work on one compact front of rows whose
+// neighbours stay in the shared MALL.  Remapping blocks so that each XCD owns
+// one contiguous range (eight fronts, one per L2) measured 3 % slower on the
+// 256^3 pack.
 
 // Element indices are 32-bit: a context keeps Q * NP * fields below 2^32.
 // kOff32: the layout is only used for population buffers below 4 GiB, so the

@@ -166,8 +166,8 @@ bool build_rows(int lattice, int32_t n_nodes, const int32_t* nb, const int32_t* 
   RowGeom& g = rb.g;
   for (int a = 0; a < 3; ++a) g.E[a] = E[a];
   g.periodic = (period[0] ? 1 : 0) | (period[1] ? 2 : 0) | (period[2] ? 4 : 0);
-  g.lx = E[0] >= 8 ? 3 : 0;
-  g.ly = E[1] >= 8 ? 3 : 0;
+  g.lx = kTileLogX;
+  g.ly = E[1] >= (1 << kTileLogY) ? kTileLogY : 0;
   const int32_t nTx = (E[0] + (1 << g.lx) - 1) >> g.lx;
   g.nTy = (E[1] + (1 << g.ly) - 1) >> g.ly;
   g.W = (E[2] + 31) / 32;

@@ -141,7 +141,7 @@ __global__ __launch_bounds__(kBlock) void k_bgk_lp(const S* __restrict__ gin, S*
                                                    double* __restrict__ rho_o,
                                                    double* __restrict__ vel_o,
                                                    double* __restrict__ partial) {
-  const int32_t b = xcd_block(blockIdx.x, gridDim.x);
+  const int32_t b = static_cast<int32_t>(blockIdx.x);
   const int32_t i = b * kBlock + threadIdx.x;
   double jx = 0.0;
   if (i < n) {
@@ -160,7 +160,7 @@ template <class L, typename S, bool REDUCE>
 __global__ __launch_bounds__(kBlock) void k_batch_bgk_lp(const BatchSolve* __restrict__ solves,
                                                          const uint32_t* __restrict__ items, int32_t n_items,
                                                          int32_t per_wg, int32_t parity) {
-  const int32_t g = xcd_block(blockIdx.x, gridDim.x);
+  const int32_t g = static_cast<int32_t>(blockIdx.x);
   const int32_t beg = g * per_wg;
   const int32_t end = beg + per_wg < n_items ? beg + per_wg : n_items;
   for (int32_t it = beg; it < end; ++it) {
@@ -381,7 +381,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_phi_lp(const S* __restrict__ gin,
                                                       int64_t NP, float* __restrict__ phi,
                                                       double* __restrict__ partial) {
   constexpr int Q = L::nQ;
-  const int32_t b = xcd_block(blockIdx.x, gridDim.x);
+  const int32_t b = static_cast<int32_t>(blockIdx.x);
   const int32_t i = b * kBlock + threadIdx.x;
   double mx = 0.0;
   if (i < n) {
@@ -405,7 +405,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_lp(const S* __restrict__ gin, S* 
                                                   const double* __restrict__ dscalar,
                                                   double* __restrict__ rho_o, double* __restrict__ vel_o) {
   constexpr int Q = L::nQ;
-  const int32_t b = xcd_block(blockIdx.x, gridDim.x);
+  const int32_t b = static_cast<int32_t>(blockIdx.x);
   const int32_t i = b * kBlock + threadIdx.x;
   if (i >= n) return;
   const uint32_t fs = static_cast<uint32_t>(Q * NP);
@@ -426,7 +426,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_cg_phi_lp(const BatchSolve* __
                                                             const uint32_t* __restrict__ items, int32_t n_items,
                                                             int32_t per_wg, int32_t parity) {
   constexpr int Q = L::nQ;
-  const int32_t g = xcd_block(blockIdx.x, gridDim.x);
+  const int32_t g = static_cast<int32_t>(blockIdx.x);
   const int32_t beg = g * per_wg;
   const int32_t end = beg + per_wg < n_items ? beg + per_wg : n_items;
   for (int32_t it = beg; it < end; ++it) {
@@ -462,7 +462,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_cg_lp(const BatchSolve* __rest
                                                         const uint32_t* __restrict__ items, int32_t n_items,
                                                         int32_t per_wg, int32_t parity) {
   constexpr int Q = L::nQ;
-  const int32_t g = xcd_block(blockIdx.x, gridDim.x);
+  const int32_t g = static_cast<int32_t>(blockIdx.x);
   const int32_t beg = g * per_wg;
   const int32_t end = beg + per_wg < n_items ? beg + per_wg : n_items;
   for (int32_t it = beg; it < end; ++it) {

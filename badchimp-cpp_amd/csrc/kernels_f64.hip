@@ -106,7 +106,7 @@ __global__ __launch_bounds__(kBlock) void k_bgk(const double* __restrict__ fin,
                                                 int64_t NP, BgkArgs a, double* __restrict__ rho_o,
                                                 double* __restrict__ vel_o,
                                                 double* __restrict__ partial) {
-  const int32_t b = xcd_block(blockIdx.x, gridDim.x);
+  const int32_t b = static_cast<int32_t>(blockIdx.x);
   const int32_t i = b * kBlock + threadIdx.x;
   double jx = 0.0;
   if (i < n) jx = bgk_node<L, A, FIRST, STORE, REDUCE>(fin, fout, ad, i, NP, a, rho_o, vel_o);
@@ -118,16 +118,15 @@ __global__ __launch_bounds__(kBlock) void k_bgk(const double* __restrict__ fin,
 
 // Batched form: one launch advances every resident solve one step.  Work
 // items are 256-node blocks of the solves (item = slot << kItemShift | block);
-// each workgroup takes a contiguous run of per_wg items, and the logical
-// workgroup order is XCD-grouped, so neighbouring blocks of one solve share an
-// L2.  A solve's buffers, address table and parameters come from its
+// each workgroup takes a contiguous run of per_wg items, so the resident
+// workgroups sweep the items as one compact front.  A solve's buffers, address table and parameters come from its
 // descriptor (uniform per item), and the per-node arithmetic is the single
 // solve's (bgk_node), so a solve gets the same bits alone or in any mix.
 template <class L, bool REDUCE>
 __global__ __launch_bounds__(kBlock) void k_batch_bgk(const BatchSolve* __restrict__ solves,
                                                       const uint32_t* __restrict__ items, int32_t n_items,
                                                       int32_t per_wg, int32_t parity) {
-  const int32_t g = xcd_block(blockIdx.x, gridDim.x);
+  const int32_t g = static_cast<int32_t>(blockIdx.x);
   const int32_t beg = g * per_wg;
   const int32_t end = beg + per_wg < n_items ? beg + per_wg : n_items;
   for (int32_t it = beg; it < end; ++it) {
@@ -181,7 +180,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_phi(const double* __restrict__ fi
                                                    int32_t n, int64_t NP, double* __restrict__ phi,
                                                    double* __restrict__ partial) {
   constexpr int Q = L::nQ;
-  const int32_t b = xcd_block(blockIdx.x, gridDim.x);
+  const int32_t b = static_cast<int32_t>(blockIdx.x);
   const int32_t i = b * kBlock + threadIdx.x;
   double mx = 0.0;
   if (i < n) {
@@ -213,7 +212,7 @@ __global__ __launch_bounds__(kBlock) void k_cg(const double* __restrict__ fin,
                                                double* __restrict__ rho_o,
                                                double* __restrict__ vel_o) {
   constexpr int Q = L::nQ, D = L::nD;
-  const int32_t b = xcd_block(blockIdx.x, gridDim.x);
+  const int32_t b = static_cast<int32_t>(blockIdx.x);
   const int32_t i = b * kBlock + threadIdx.x;
   if (i >= n) return;
   double f[2][Q];
