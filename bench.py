@@ -379,6 +379,7 @@ def serving_leg(n_req, pool_gb, every):
     base_reqs = {r.tag: copy.copy(r) for r in reqs}  # the server drops its copies' geometries
     srv = Server(pool, precision=lbamd.FP32S, check_every=100)
     done, wall = run_stream(srv, reqs)
+    kms, _, lups = srv.batch.kernel_time()
     srv.close()
     cap = len(done) / wall
     lat_b = np.array([d["latency"] for d in done])
@@ -428,6 +429,8 @@ def serving_leg(n_req, pool_gb, every):
     out = {"n_requests": n_req, "mix": "70% permeability / 30% relative permeability, L U(48,160)",
            "pool_GB": pool / 1e9,
            "burst_solves_per_s": cap, "burst_wall_s": wall,
+           "burst_device_GLUPS": lups / (kms * 1e-3) / 1e9 if kms > 0 else None,
+           "burst_device_busy_frac": kms * 1e-3 / wall,
            "poisson_rate_per_s": rate, "poisson_wall_s": wall2, "poisson_solves_per_s": len(done2) / wall2,
            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
            "fp64_one_at_a_time_solves_per_s": n_req / est, "fp64_subset": int(len(subset)),
