@@ -24,6 +24,9 @@ batched  = the ensemble of independent sphere-pack permeability solves (cubic
            path run one solve at a time (lb_solve_permeability on its own
            context), timed on a size-stratified subset (every m-th solve by
            fluid count) and scaled by m; k error of FP32S vs FP64 on that subset.
+channel  = the D2Q9 256x128 channel (solid rows y = 0, 127, periodic x, tau 1,
+           F 1e-6, 5000 steps from rest) in FP64: MLUPS and the velocity
+           profile against the Poiseuille start-up series.
 twophase = the colour-gradient case: 192^3 sphere pack (radius 8, porosity
            0.38), phases drawn per pore node with saturation 0.5, solid
            wettability 1, tau0 = tau1 = 1, sigma = 0.01, beta = 0.7, body force
@@ -314,6 +317,37 @@ def fp64_one(req):
     return k, steps, dt
 
 
+def channel_leg(steps=5000):
+    from lbamd.geometry import channel_2d
+    geo = channel_2d(256, 128)
+    topo = lbamd.Topology.from_geometry(geo, lbamd.D2Q9, "x")
+    nb, types, bulk, pos = topo.neighbors(), topo.node_types(), topo.bulk(), topo.positions()
+    # rest state f = w in the D2Q9 direction order (axis, diagonal alternating, rest last)
+    w9 = np.array([4.0, 1.0, 4.0, 1.0, 4.0, 1.0, 4.0, 1.0, 16.0]) / 36.0
+    f0 = np.zeros((topo.n_nodes, 9))
+    f0[bulk] = w9[None, :]
+    ctx = lbamd.Context(lbamd.D2Q9, lbamd.FP64)
+    ctx.set_topology_grid(nb, types, bulk, pos)
+    ctx.upload_f(f0)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step_bgk(steps + 1, 1.0, np.array([1e-6, 0.0]))
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    vel = ctx.download_vel()
+    ctx.close()
+    prof = vel[bulk, 0].reshape(256, 126)[0]
+    # start-up series of plane Poiseuille flow between half-way walls (H = 126, nu = 1/6)
+    H, nu, F, t = 126.0, 1.0 / 6.0, 1e-6, steps + 0.5
+    yh = np.arange(1, 127) - 0.5
+    u = F / (2 * nu) * yh * (H - yh)
+    for n in range(1, 400, 2):
+        u -= 4 * F * H * H / (nu * np.pi ** 3 * n ** 3) * np.sin(n * np.pi * yh / H) * np.exp(-n * n * np.pi ** 2 * nu * t / (H * H))
+    return {"workload": "D2Q9 256x128 channel, tau 1, F 1e-6, %d steps, FP64" % steps,
+            "MLUPS": bulk.size * (steps + 1) / dt / 1e6, "wall_s": dt,
+            "profile_err_vs_startup_series": float(np.abs(prof - u).max() / u.max())}
+
+
 def twophase_leg(size, steps, seed=3):
     geo, info = lbamd.sphere_pack((size,) * 3, 8.0, 0.38, seed, periodic="x", walls=True)
     topo = lbamd.Topology.from_geometry(geo, lbamd.D3Q19, "x")
@@ -457,6 +491,7 @@ def main():
     ap.add_argument("--batch", type=int, default=512, help="ensemble size (0: skip)")
     ap.add_argument("--baseline-every", type=int, default=16)
     ap.add_argument("--serve", type=int, default=48, help="requests per serving run (0: skip)")
+    ap.add_argument("--no-channel", action="store_true")
     ap.add_argument("--tp-size", type=int, default=192)
     ap.add_argument("--tp-steps", type=int, default=50000, help="two-phase steps (0: skip)")
     ap.add_argument("--serve-pool-gb", type=float, default=16.0)
@@ -503,6 +538,10 @@ def main():
                 "definition": "k = nu * sum_bulk(sum_q f c_x + F_x/2) / (N_box * F_x), nu = (tau-1/2)/3"}
 
     batched = batched_leg(args.batch, args.baseline_every, tau, force) if args.batch > 0 else None
+    channel = None if args.no_channel else channel_leg()
+    if channel:
+        log("channel: %.1f MLUPS, profile error vs start-up series %.2e" % (channel["MLUPS"],
+                                                                          channel["profile_err_vs_startup_series"]))
     twophase = twophase_leg(args.tp_size, args.tp_steps) if args.tp_steps > 0 else None
     serving = serving_leg(args.serve, args.serve_pool_gb, 8) if args.serve > 0 else None
 
@@ -537,6 +576,7 @@ def main():
                  "bytes_per_lup": bytes64, "traffic": traffic64},
         "permeability": perm,
         "batched": batched,
+        "channel": channel,
         "twophase": twophase,
         "serving": serving,
         "cpu_baseline": cpu,

@@ -7,6 +7,10 @@
 #                   (std_case/main.cpp:14-15 offers the switch); the edited
 #                   copy is written to oracle/_ref/ and never committed
 #   twophase        src/twophase/main_TWOPHASE.cpp as shipped (D3Q19)
+#   std_case_gpu_d2q9, std_case_gpu_d3q19
+#                   tests/integration/std_case_gpu.cpp: the reference's
+#                   set-up classes driving this repository's library through
+#                   the C++ shim (needs badchimp-cpp_amd/lib/libbadchimp_amd.so)
 #
 # Flags follow the reference release build (CMakeLists.txt:71): -O3 -fopenmp
 # -std=c++17.  MPI is MPICH from /opt/conda (the reference links MPI via
@@ -34,7 +38,12 @@ done
 CXXFLAGS="-std=c++17 -O3 -fopenmp -w -I/opt/conda/include"
 LIBS="$REF/lbsolver/LBd2q9.cpp $REF/lbsolver/LBd3q19.cpp /opt/conda/lib/libmpi.so"
 
-sed 's/^#define LT D2Q9/#define LT D3Q19/' "$REF/std_case/main.cpp" > "$OUT/std_case_d3q19.cpp"
+sed 's/^#define LT D2Q9/#define LT D3Q19/' "$REF/std_case/main.cpp" > "$OUT/std_case_d3q19.cpp.new"
+if cmp -s "$OUT/std_case_d3q19.cpp.new" "$OUT/std_case_d3q19.cpp"; then
+  rm "$OUT/std_case_d3q19.cpp.new"
+else
+  mv "$OUT/std_case_d3q19.cpp.new" "$OUT/std_case_d3q19.cpp"
+fi
 
 build() {  # name, main source, extra include dir
   local name=$1 src=$2 inc=$3
@@ -46,3 +55,20 @@ build() {  # name, main source, extra include dir
 build std_case_d2q9 "$REF/std_case/main.cpp" "$REF/std_case"
 build std_case_d3q19 "$OUT/std_case_d3q19.cpp" "$REF/std_case"
 build twophase "$REF/twophase/main_TWOPHASE.cpp" "$REF/twophase"
+
+# the drop-in demonstration (test infrastructure): reference set-up + shim
+REPO="$(cd "$HERE/.." && pwd)"
+LIBDIR="$REPO/badchimp-cpp_amd/lib"
+DEMO="$REPO/tests/integration/std_case_gpu.cpp"
+SHIM="$REPO/badchimp-cpp_amd/include/lbamd/lbsolver_gpu.hpp"
+if [ -e "$LIBDIR/libbadchimp_amd.so" ]; then
+  for lat in D2Q9 D3Q19; do
+    name=std_case_gpu_$(echo $lat | tr 'A-Z' 'a-z')
+    if [ ! -x "$OUT/$name" ] || [ "$DEMO" -nt "$OUT/$name" ] || [ "$SHIM" -nt "$OUT/$name" ]; then
+      g++ -std=c++17 -O2 -w -I/opt/conda/include -I"$REF" -I"$REPO/badchimp-cpp_amd/include" -DLB_LATTICE=$lat \
+        "$DEMO" "$REF/lbsolver/LBd2q9.cpp" "$REF/lbsolver/LBd3q19.cpp" /opt/conda/lib/libmpi.so \
+        -L"$LIBDIR" -lbadchimp_amd -Wl,-rpath,'$ORIGIN/../../badchimp-cpp_amd/lib' -o "$OUT/$name"
+      echo "build_ref: built $name"
+    fi
+  done
+fi
