@@ -180,27 +180,33 @@ def batched_leg(n, every, tau, force):
     fps = [lbamd.batch_footprint(lbamd.D3Q19, lbamd.FP32S, g, "x") for g in geos]
     pool = int(sum(f[0] for f in fps) * 1.01) + (64 << 20)
     nfluid = np.array([f[1] for f in fps])
-    b = lbamd.Batch(lbamd.D3Q19, lbamd.FP32S, pool, 100)
-    t0 = time.perf_counter()
-    for i, g in enumerate(geos):
-        assert b.add(g, "x", tau, force, 1e-6, 200000, i)
-    t_adm = time.perf_counter() - t0
-    res = {}
-    while len(res) < n:
-        b.advance(1)
-        for r in b.collect():
-            res[r["tag"]] = r
-    wall = time.perf_counter() - t0
-    kms, launches, lups = b.kernel_time()
-    shape = b.shape()
-    b.close()
+    def run(precision):
+        b = lbamd.Batch(lbamd.D3Q19, precision, pool, 100)
+        t0 = time.perf_counter()
+        for i, g in enumerate(geos):
+            assert b.add(g, "x", tau, force, 1e-6, 200000, i)
+        t_adm = time.perf_counter() - t0
+        res = {}
+        while len(res) < n:
+            b.advance(1)
+            for r in b.collect():
+                res[r["tag"]] = r
+        wall = time.perf_counter() - t0
+        kms, launches, lups = b.kernel_time()
+        shape = b.shape()
+        b.close()
+        return res, wall, t_adm, kms, lups, shape
+
+    res, wall, t_adm, kms, lups, shape = run(lbamd.FP32S)
+    res16, wall16, _, kms16, lups16, _ = run(lbamd.FP16S)
+    log("batched FP16S: %d solves in %.2f s, %.1f GLUPS device" % (n, wall16, lups16 / (kms16 * 1e-3) / 1e9))
     log("batched FP32S: %d solves in %.2f s (admission %.2f s), %.1f GLUPS device, shape %d" %
         (n, wall, t_adm, lups / (kms * 1e-3) / 1e9, shape))
     # plain FP64 one at a time on a size-stratified subset
     order = np.argsort(nfluid, kind="stable")
     subset = order[every // 2::every]
     t_one = 0.0
-    errs = []
+    errs, errs16 = [], []
     for i in subset:
         g = geos[i]
         topo = lbamd.Topology.from_geometry(g, lbamd.D3Q19, "x")
@@ -216,6 +222,7 @@ def batched_leg(n, every, tau, force):
         t_one += time.perf_counter() - t1
         ctx.close()
         errs.append(abs(res[int(i)]["k"] - k64) / abs(k64))
+        errs16.append(abs(res16[int(i)]["k"] - k64) / abs(k64))
     est_one = t_one * n / len(subset)
     steps = np.array([res[i]["steps"] for i in range(n)])
     out = {"n_solves": n, "solves_per_s": n / wall, "wall_s": wall, "admission_s": t_adm,
@@ -225,6 +232,9 @@ def batched_leg(n, every, tau, force):
            "fp64_one_at_a_time_solves_per_s": n / est_one, "fp64_subset": int(len(subset)),
            "speedup_vs_fp64_one_at_a_time": est_one / wall,
            "k_rel_err_fp32s_vs_fp64_max": float(max(errs)), "tol_dk_over_k": 1e-6,
+           "fp16s_solves_per_s": n / wall16, "fp16s_device_GLUPS": lups16 / (kms16 * 1e-3) / 1e9,
+           "fp16s_speedup_vs_fp64_one_at_a_time": est_one / wall16,
+           "k_rel_err_fp16s_vs_fp64_max": float(max(errs16)),
            "converged": int(sum(1 for r in res.values() if r["status"] == 0))}
     log("FP64 one at a time (subset of %d, scaled): %.2f solves/s; batched %.2f solves/s (x%.2f); "
         "max k err %.2e" % (len(subset), out["fp64_one_at_a_time_solves_per_s"], out["solves_per_s"],
@@ -367,7 +377,7 @@ def twophase_leg(size, steps, seed=3):
     F = np.array([1e-6, 0.0, 0.0])
     nbox = float(geo.size)
     res = {}
-    for prec, name in ((lbamd.FP64, "fp64"), (lbamd.FP32S, "fp32s")):
+    for prec, name in ((lbamd.FP64, "fp64"), (lbamd.FP32S, "fp32s"), (lbamd.FP16S, "fp16s")):
         ctx = lbamd.Context(lbamd.D3Q19, prec)
         ctx.set_topology_grid(nb, types, bulk, pos)
         ctx.upload_f(f0)
@@ -387,7 +397,7 @@ def twophase_leg(size, steps, seed=3):
         res[name] = {"MLUPS": bulk.size * steps / dt / 1e6, "k0": k0, "k1": k1, "phi": phi,
                      "ms_per_step": kms / steps}
         log("two-phase %s: %.1f MLUPS, k0 %.6e k1 %.6e" % (name, res[name]["MLUPS"], k0, k1))
-    a, b = res["fp64"], res["fp32s"]
+    a, b, h = res["fp64"], res["fp32s"], res["fp16s"]
     scale = abs(a["k0"]) + abs(a["k1"])
     out = {"workload": "D3Q19 colour gradient, %d^3 sphere pack r=8 porosity %.3f, S_w 0.5, %d steps" %
                        (size, info["porosity"], steps),
@@ -398,7 +408,10 @@ def twophase_leg(size, steps, seed=3):
            "k_phase_err_rel_to_sum": [abs(b["k0"] - a["k0"]) / scale, abs(b["k1"] - a["k1"]) / scale],
            "phi_max_abs_diff": float(np.abs(a["phi"] - b["phi"]).max()),
            "phi_mean_abs_diff": float(np.abs(a["phi"] - b["phi"]).mean()),
-           "phase_sign_flips": int(np.sum(np.sign(a["phi"]) != np.sign(b["phi"])))}
+           "phase_sign_flips": int(np.sum(np.sign(a["phi"]) != np.sign(b["phi"]))),
+           "fp16s_MLUPS": h["MLUPS"], "k_phase_fp16s": [h["k0"], h["k1"]],
+           "k_phase_err_rel_to_sum_fp16s": [abs(h["k0"] - a["k0"]) / scale, abs(h["k1"] - a["k1"]) / scale],
+           "phi_mean_abs_diff_fp16s": float(np.abs(a["phi"] - h["phi"]).mean())}
     log("two-phase FP32S vs FP64: k err %s, phi max %.3e mean %.3e" %
         (out["k_phase_err_rel_to_sum"], out["phi_max_abs_diff"], out["phi_mean_abs_diff"]))
     return out
@@ -514,6 +527,9 @@ def main():
     t64, k64_ms, lay64 = gpu_leg(topo, nb, types, bulk, pos, args.layout, lbamd.FP64, args.steps, args.warmup, tau, force)
     mlups64 = n_bulk * args.steps / t64 / 1e6
     log("FP64 : %.1f MLUPS, kernel %.3f ms, layout %s" % (mlups64, k64_ms, lay64))
+    t16, k16_ms, _ = gpu_leg(topo, nb, types, bulk, pos, args.layout, lbamd.FP16S, args.steps, args.warmup, tau, force)
+    mlups16 = n_bulk * args.steps / t16 / 1e6
+    log("FP16S: %.1f MLUPS, kernel %.3f ms" % (mlups16, k16_ms))
 
     copy_gbs, _ = lbamd.bench_copy(1 << 30, 20)
     log("plain copy: %.1f GB/s (read + write)" % copy_gbs)
@@ -533,8 +549,11 @@ def main():
         log("k FP64 = %.10e after %d steps (%.1f s)" % (k64, s64, d64))
         k32, s32, d32 = permeability(nb, types, bulk, pos, args.layout, topo.n_nodes, lbamd.FP32S, tau, force, n_box)
         log("k FP32S = %.10e after %d steps (%.1f s)" % (k32, s32, d32))
+        k16, s16, d16 = permeability(nb, types, bulk, pos, args.layout, topo.n_nodes, lbamd.FP16S, tau, force, n_box)
+        log("k FP16S = %.10e after %d steps (%.1f s)" % (k16, s16, d16))
         perm = {"k_fp64": k64, "steps_fp64": s64, "k_fp32s": k32, "steps_fp32s": s32,
-                "k_rel_err_fp32s": abs(k32 - k64) / abs(k64), "tol_dk_over_k": 1e-7,
+                "k_rel_err_fp32s": abs(k32 - k64) / abs(k64), "k_fp16s": k16, "steps_fp16s": s16,
+                "k_rel_err_fp16s": abs(k16 - k64) / abs(k64), "tol_dk_over_k": 1e-7,
                 "definition": "k = nu * sum_bulk(sum_q f c_x + F_x/2) / (N_box * F_x), nu = (tau-1/2)/3"}
 
     batched = batched_leg(args.batch, args.baseline_every, tau, force) if args.batch > 0 else None
@@ -571,6 +590,9 @@ def main():
                      "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": traffic32,
                      "kernel": "k_bgk_lp<D3Q19,float>", "avg_kernel_ms": k32_ms,
                      "bytes_per_lup": bytes32, "plain_copy_GBs": round(copy_gbs, 1)},
+        "fp16s": {"value": round(mlups16, 2), "unit": "MLUPS", "avg_kernel_ms": k16_ms,
+                  "achieved_GBs": round(n_bulk * 76 / (k16_ms * 1e-3) / 1e9, 1),
+                  "frac": round(n_bulk * 76 / (k16_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "bytes_per_lup": 76},
         "fp64": {"value": round(mlups64, 2), "unit": "MLUPS", "avg_kernel_ms": k64_ms,
                  "achieved_GBs": round(ach64, 1), "frac": round(ach64 / HBM_PEAK_GBS, 4),
                  "bytes_per_lup": bytes64, "traffic": traffic64},
