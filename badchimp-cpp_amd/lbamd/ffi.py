@@ -70,6 +70,8 @@ SIGNATURES = {
     "lb_kernel_time_reset": (C.c_int, [_vp]),
     "lb_device_bytes": (C.c_int64, [_vp]),
     "lb_bench_copy": (C.c_int, [C.c_int, C.c_int64, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "lb_bench_stream": (C.c_int, [C.c_int, C.c_int32, C.c_int64, C.c_int32, C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double)]),
     "lb_topo_read_vtklb": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(_vp)]),
     "lb_topo_from_geometry": (C.c_int, [C.c_int, C.c_int32, _i32p, _i32p, C.c_int32, C.c_int32,
                                         C.POINTER(_vp)]),
@@ -157,6 +159,13 @@ def bench_copy(nbytes, iters=20, device=0):
     """Plain device copy: (read+write GB/s, ms per copy)."""
     g, m = C.c_double(), C.c_double()
     _check_global(lib().lb_bench_copy(device, int(nbytes), int(iters), C.byref(g), C.byref(m)))
+    return g.value, m.value
+
+
+def bench_stream(n_nodes, lattice=D3Q19, iters=20, device=0):
+    """Identity AoSoA streaming of float populations: (read+write GB/s, ms)."""
+    g, m = C.c_double(), C.c_double()
+    _check_global(lib().lb_bench_stream(device, lattice, int(n_nodes), int(iters), C.byref(g), C.byref(m)))
     return g.value, m.value
 
 

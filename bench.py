@@ -533,6 +533,8 @@ def main():
 
     copy_gbs, _ = lbamd.bench_copy(1 << 30, 20)
     log("plain copy: %.1f GB/s (read + write)" % copy_gbs)
+    stream_gbs, _ = lbamd.bench_stream(n_bulk, lbamd.D3Q19, 20)
+    log("AoSoA identity stream (same node count): %.1f GB/s (read + write)" % stream_gbs)
     bytes32 = 2 * 19 * 4
     bytes64 = 2 * 19 * 8
     ach32 = n_bulk * bytes32 / (k32_ms * 1e-3) / 1e9
@@ -589,7 +591,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": traffic32,
                      "kernel": "k_bgk_lp<D3Q19,float>", "avg_kernel_ms": k32_ms,
-                     "bytes_per_lup": bytes32, "plain_copy_GBs": round(copy_gbs, 1)},
+                     "bytes_per_lup": bytes32, "plain_copy_GBs": round(copy_gbs, 1),
+                     "aosoa_stream_GBs": round(stream_gbs, 1)},
         "fp16s": {"value": round(mlups16, 2), "unit": "MLUPS", "avg_kernel_ms": k16_ms,
                   "achieved_GBs": round(n_bulk * 76 / (k16_ms * 1e-3) / 1e9, 1),
                   "frac": round(n_bulk * 76 / (k16_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "bytes_per_lup": 76},

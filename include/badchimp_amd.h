@@ -184,6 +184,11 @@ int64_t lb_device_bytes(const lb_ctx* ctx);
  * `bytes` bytes repeated `iters` times; returns read+write GB/s and the time
  * per copy.  Used to scale bandwidth figures and HBM counter readings. */
 int lb_bench_copy(int device, int64_t bytes, int32_t iters, double* gbs, double* ms_per_iter);
+/* Identity streaming of n_nodes float populations in the AoSoA layout (Q
+ * loads and Q stores per node, no gathers, no collision): the bandwidth
+ * ceiling of the population layout itself (read + write GB/s). */
+int lb_bench_stream(int device, int32_t lattice, int64_t n_nodes, int32_t iters, double* gbs,
+                    double* ms_per_iter);
 
 /* ------------------------------------------------------------ topology host side */
 /* .vtklb reader (LBvtk<DXQY>, LBvtk.h:221-614) followed by Grid
