@@ -7,10 +7,11 @@
 #                   (std_case/main.cpp:14-15 offers the switch); the edited
 #                   copy is written to oracle/_ref/ and never committed
 #   twophase        src/twophase/main_TWOPHASE.cpp as shipped (D3Q19)
-#   std_case_gpu_d2q9, std_case_gpu_d3q19
-#                   tests/integration/std_case_gpu.cpp: the reference's
-#                   set-up classes driving this repository's library through
-#                   the C++ shim (needs badchimp-cpp_amd/lib/libbadchimp_amd.so)
+#   std_case_gpu_d2q9, std_case_gpu_d3q19, twophase_gpu
+#                   tests/integration/{std_case,twophase}_gpu.cpp: the
+#                   reference's set-up classes driving this repository's
+#                   library through the C++ shim (needs
+#                   badchimp-cpp_amd/lib/libbadchimp_amd.so)
 #
 # Flags follow the reference release build (CMakeLists.txt:71): -O3 -fopenmp
 # -std=c++17.  MPI is MPICH from /opt/conda (the reference links MPI via
@@ -71,4 +72,11 @@ if [ -e "$LIBDIR/libbadchimp_amd.so" ]; then
       echo "build_ref: built $name"
     fi
   done
+  TPDEMO="$REPO/tests/integration/twophase_gpu.cpp"
+  if [ ! -x "$OUT/twophase_gpu" ] || [ "$TPDEMO" -nt "$OUT/twophase_gpu" ] || [ "$SHIM" -nt "$OUT/twophase_gpu" ]; then
+    g++ -std=c++17 -O2 -w -I/opt/conda/include -I"$REF" -I"$REPO/badchimp-cpp_amd/include" \
+      "$TPDEMO" "$REF/lbsolver/LBd2q9.cpp" "$REF/lbsolver/LBd3q19.cpp" /opt/conda/lib/libmpi.so \
+      -L"$LIBDIR" -lbadchimp_amd -Wl,-rpath,'$ORIGIN/../../badchimp-cpp_amd/lib' -o "$OUT/twophase_gpu"
+    echo "build_ref: built twophase_gpu"
+  fi
 fi
