@@ -67,8 +67,11 @@ struct TableAddr {
     for (int q = 0; q < L::nQ - 1; ++q) {
       const uint32_t kind = l[q] >> 30;
       const uint32_t j = l[q] & kLinkIndexMask;
-      const uint32_t rq = L::rev(q);
-      src[q] = kind == kLinkStream ? pidx<L::nQ>(q, j) : (kind == kLinkBounce ? pidx<L::nQ>(rq, i) : pidx<L::nQ>(rq, j));
+      // stream: (q, j); bounce: (rev q, i); ghost 0: (rev q, j) -- one index
+      // computation on selected operands, no branch
+      const uint32_t jj = kind == kLinkBounce ? static_cast<uint32_t>(i) : j;
+      const uint32_t qq = kind == kLinkStream ? static_cast<uint32_t>(q) : static_cast<uint32_t>(L::rev(q));
+      src[q] = pidx<L::nQ>(qq, jj);
     }
   }
 };
@@ -125,9 +128,9 @@ struct RowAddr {
     }
 #pragma unroll
     for (int q = 0; q < L::nQ - 1; ++q) {
-      const bool fluid = (e[q].y & bit[q]) != 0u;
+      const uint32_t m = 0u - static_cast<uint32_t>((e[q].y & bit[q]) != 0u);
       const uint32_t j = e[q].x + __popc(e[q].y & (bit[q] - 1u));
-      src[q] = fluid ? pidx<L::nQ>(q, j) : pidx<L::nQ>(L::rev(q), i);
+      src[q] = (pidx<L::nQ>(q, j) & m) | (pidx<L::nQ>(L::rev(q), i) & ~m);
     }
   }
 };
@@ -174,9 +177,11 @@ struct RowWin {
       // source cell r - c_q
       const int k = (1 - cx<L>(q)) * NY + (NY > 1 ? 1 - cy<L>(q) : 0);
       const int dz = -cz<L>(q);
-      const bool fluid = (bm[k] >> (dz + 1)) & 1u;
+      const uint32_t fluid = (bm[k] >> (dz + 1)) & 1u;
       const uint32_t j = dz == 0 ? b0[k] : (dz < 0 ? b0[k] - 1u : b0[k] + ((bm[k] >> 1) & 1u));
-      src[q] = fluid ? pidx<L::nQ>(q, j) : pidx<L::nQ>(L::rev(q), i);
+      // both candidates, combined by a mask: a per-lane select, not a branch
+      const uint32_t m = 0u - fluid;
+      src[q] = (pidx<L::nQ>(q, j) & m) | (pidx<L::nQ>(L::rev(q), i) & ~m);
     }
   }
 };
@@ -245,7 +250,11 @@ __device__ __forceinline__ void neighbour_phi(const RowWin& ad, int32_t i, const
 #pragma unroll
   for (int qq = 0; qq < L::nQ - 1; ++qq) {
     const int q = L::rev(qq);
-    pn[qq] = (fluid >> q) & 1u ? phi[j[q]] : pw;
+    // load from the node itself where the neighbour is solid (always in
+    // bounds), then select: no branch
+    const bool f = (fluid >> q) & 1u;
+    const T v = phi[f ? j[q] : static_cast<uint32_t>(i)];
+    pn[qq] = f ? v : pw;
   }
   pn[L::nQ - 1] = static_cast<T>(0);
 }

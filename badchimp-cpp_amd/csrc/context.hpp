@@ -41,8 +41,8 @@ constexpr uint32_t kPB = 1u << kPLog;
 // index below 2^32, which a context guarantees) and the direction part is a
 // constant that folds into the memory instruction's offset.
 template <int Q>
-__host__ __device__ __forceinline__ constexpr uint32_t pidx(uint32_t q, uint32_t i) {
-  return ((i >> kPLog) & 0xFFFFFFu) * static_cast<uint32_t>((Q - 1) << kPLog) + i + (q << kPLog);
+__device__ __forceinline__ uint32_t pidx(uint32_t q, uint32_t i) {
+  return __umul24(i >> kPLog, static_cast<uint32_t>((Q - 1) << kPLog)) + i + (q << kPLog);
 }
 inline uint32_t pidx_host(int Q, uint32_t q, uint32_t i) {
   return (((i >> kPLog) * static_cast<uint32_t>(Q) + q) << kPLog) | (i & (kPB - 1u));

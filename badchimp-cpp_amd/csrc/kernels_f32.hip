@@ -262,7 +262,8 @@ __device__ __forceinline__ void load_td_links(const S* __restrict__ gin, uint32_
   rowwin_links<L>(ad, i, j, fluid);
 #pragma unroll
   for (int q = 0; q < L::nQ - 1; ++q) {
-    const uint32_t e = (fluid >> q) & 1u ? pidx<L::nQ>(q, j[q]) : pidx<L::nQ>(L::rev(q), i);
+    const uint32_t m = 0u - ((fluid >> q) & 1u);
+    const uint32_t e = (pidx<L::nQ>(q, j[q]) & m) | (pidx<L::nQ>(L::rev(q), i) & ~m);
     t[q] = Store<S>::get(ld_elem<true>(gin, e));
     d[q] = Store<S>::get(ld_elem<true>(gin, fs + e));
   }
@@ -482,7 +483,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_cg_lp(const BatchSolve* __rest
 #pragma unroll
     for (int qq = 0; qq < Q - 1; ++qq) {
       const int q = L::rev(qq);
-      pn[qq] = (fluid >> q) & 1u ? s.phi[j[q]] : pw;
+      const bool f = (fluid >> q) & 1u;
+      const float v = s.phi[f ? j[q] : static_cast<uint32_t>(i)];
+      pn[qq] = f ? v : pw;
     }
     const float F[3] = {static_cast<float>(s.a.F[0]), static_cast<float>(s.a.F[1]), static_cast<float>(s.a.F[2])};
     cg_collide<L, S, true, false>(t, dd, pn, F, static_cast<float>(s.nu0Inv), static_cast<float>(s.nu1Inv),
