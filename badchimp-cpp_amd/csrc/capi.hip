@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "../../include/badchimp_amd.h"
@@ -870,5 +871,80 @@ int lb_kernel_time_reset(lb_ctx* c) {
 }
 
 int64_t lb_device_bytes(const lb_ctx* c) { return c ? c->bytes : -1; }
+
+// ------------------------------------------------------------ field files
+// The reference's binary field files (LBfield.h:102-138, 233-276, 378-421):
+// int32 header, then the doubles in the field's host layout over all nodes
+// of the node list.  Non-bulk nodes are written as 0 (their values are not
+// part of the state between steps).
+namespace {
+
+int write_file(lb_ctx* c, const std::string& path, const std::vector<int32_t>& header, const std::vector<double>& v) {
+  FILE* fh = std::fopen(path.c_str(), "wb");
+  if (!fh) return fail(c, LB_ERR_IO, "cannot open %s for writing", path.c_str());
+  bool ok = std::fwrite(header.data(), sizeof(int32_t), header.size(), fh) == header.size() &&
+            std::fwrite(v.data(), sizeof(double), v.size(), fh) == v.size();
+  ok = (std::fclose(fh) == 0) && ok;
+  if (!ok) return fail(c, LB_ERR_IO, "short write to %s", path.c_str());
+  return LB_OK;
+}
+
+}  // namespace
+
+int lb_write_lbfield(lb_ctx* c, const char* prefix) {
+  if (!c) return LB_ERR_ARG;
+  if (!prefix) return fail(c, LB_ERR_ARG, "null path");
+  if (!c->has_topo || c->n_fields == 0) return fail(c, LB_ERR_STATE, "no populations");
+  const int nf = c->n_fields;
+  std::vector<double> f(static_cast<size_t>(c->n_nodes) * nf * c->Q, 0.0);
+  int rc = lb_download_f(c, nf, f.data());
+  if (rc != LB_OK) return rc;
+  return write_file(c, std::string(prefix) + ".lblbf", {nf, c->Q, c->n_nodes}, f);
+}
+
+int lb_read_lbfield(lb_ctx* c, const char* prefix) {
+  if (!c) return LB_ERR_ARG;
+  if (!prefix) return fail(c, LB_ERR_ARG, "null path");
+  if (!c->has_topo) return fail(c, LB_ERR_STATE, "no topology");
+  const std::string path = std::string(prefix) + ".lblbf";
+  FILE* fh = std::fopen(path.c_str(), "rb");
+  if (!fh) return fail(c, LB_ERR_IO, "cannot open %s", path.c_str());
+  int32_t hdr[3] = {0, 0, 0};
+  if (std::fread(hdr, sizeof(int32_t), 3, fh) != 3) {
+    std::fclose(fh);
+    return fail(c, LB_ERR_IO, "%s: short header", path.c_str());
+  }
+  if ((hdr[0] != 1 && hdr[0] != 2) || hdr[1] != c->Q || hdr[2] != c->n_nodes) {
+    std::fclose(fh);
+    return fail(c, LB_ERR_ARG, "%s holds %d field(s), %d directions, %d nodes; the context has %d directions, %d nodes",
+                path.c_str(), hdr[0], hdr[1], hdr[2], c->Q, c->n_nodes);
+  }
+  std::vector<double> f(static_cast<size_t>(hdr[0]) * hdr[1] * hdr[2]);
+  const size_t got = std::fread(f.data(), sizeof(double), f.size(), fh);
+  std::fclose(fh);
+  if (got != f.size()) return fail(c, LB_ERR_IO, "%s: %zu of %zu values", path.c_str(), got, f.size());
+  return lb_upload_f(c, hdr[0], f.data());
+}
+
+int lb_write_scalar(lb_ctx* c, int32_t which, int32_t n_fields, const char* prefix) {
+  if (!c) return LB_ERR_ARG;
+  if (!prefix) return fail(c, LB_ERR_ARG, "null path");
+  if (!c->has_topo) return fail(c, LB_ERR_STATE, "no topology");
+  if (n_fields < 1 || n_fields > 2) return fail(c, LB_ERR_ARG, "bad field count");
+  std::vector<double> s(static_cast<size_t>(c->n_nodes) * n_fields, 0.0);
+  int rc = lb_download_scalar(c, which, n_fields, s.data());
+  if (rc != LB_OK) return rc;
+  return write_file(c, std::string(prefix) + ".lbsca", {n_fields, c->n_nodes}, s);
+}
+
+int lb_write_vector(lb_ctx* c, int32_t which, const char* prefix) {
+  if (!c) return LB_ERR_ARG;
+  if (!prefix) return fail(c, LB_ERR_ARG, "null path");
+  if (!c->has_topo) return fail(c, LB_ERR_STATE, "no topology");
+  std::vector<double> v(static_cast<size_t>(c->n_nodes) * c->nD, 0.0);
+  int rc = lb_download_vector(c, which, 1, v.data());
+  if (rc != LB_OK) return rc;
+  return write_file(c, std::string(prefix) + ".lbvec", {1, c->nD, c->n_nodes}, v);
+}
 
 }  // extern "C"

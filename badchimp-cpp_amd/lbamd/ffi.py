@@ -91,6 +91,10 @@ SIGNATURES = {
                                       C.POINTER(C.POINTER(C.c_double))]),
     "lb_sphere_pack": (C.c_int, [C.c_int32, _i32p, C.c_double, C.c_double, C.c_uint64, C.c_int32,
                                  C.c_int32, C.c_int32, C.c_double, _i32p, _f64p]),
+    "lb_write_lbfield": (C.c_int, [_vp, C.c_char_p]),
+    "lb_read_lbfield": (C.c_int, [_vp, C.c_char_p]),
+    "lb_write_scalar": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_char_p]),
+    "lb_write_vector": (C.c_int, [_vp, C.c_int32, C.c_char_p]),
     "lb_batch_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int32, C.POINTER(_vp)]),
     "lb_batch_destroy": (C.c_int, [_vp]),
     "lb_batch_last_error": (C.c_char_p, [_vp]),
@@ -399,6 +403,19 @@ class Context:
         self._check(lib().lb_solve_permeability(self._h, float(tau), F, int(check_every), float(tol),
                                                 int(max_steps), float(n_box), C.byref(k), C.byref(s)))
         return k.value, s.value
+
+    def write_lbfield(self, prefix):
+        """Population checkpoint in the reference's .lblbf format."""
+        self._check(lib().lb_write_lbfield(self._h, os.fsencode(prefix)))
+
+    def read_lbfield(self, prefix):
+        self._check(lib().lb_read_lbfield(self._h, os.fsencode(prefix)))
+
+    def write_scalar(self, prefix, which=RHO, nf=1):
+        self._check(lib().lb_write_scalar(self._h, which, nf, os.fsencode(prefix)))
+
+    def write_vector(self, prefix, which=VEL):
+        self._check(lib().lb_write_vector(self._h, which, os.fsencode(prefix)))
 
     def sync(self):
         self._check(lib().lb_sync(self._h))

@@ -139,6 +139,18 @@ int lb_download_vector(lb_ctx* ctx, int32_t which, int32_t n_fields, double* v);
  * n_nodes; only solid-boundary (type 1) entries are read. */
 int lb_set_wall_densities(lb_ctx* ctx, const double* rho2);
 
+/* Field files in the reference's binary formats, "<prefix>.lblbf" etc.
+ * (LbField/ScalarField/VectorField::writeToFile, LBfield.h:102-138,233-276,
+ * 378-421): int32 header {nFields, Q, nNodes} / {nFields, nNodes} /
+ * {nFields, nD, nNodes}, then the doubles over all nodes in the host layout.
+ * The population file holds the pre-collision state (a checkpoint: read it
+ * back into a context with the same node list to resume); non-bulk nodes are
+ * written as 0. */
+int lb_write_lbfield(lb_ctx* ctx, const char* prefix);
+int lb_read_lbfield(lb_ctx* ctx, const char* prefix);
+int lb_write_scalar(lb_ctx* ctx, int32_t which, int32_t n_fields, const char* prefix);
+int lb_write_vector(lb_ctx* ctx, int32_t which, const char* prefix);
+
 /* ------------------------------------------------------------ steps */
 /* nsteps iterations of the std_case loop body (std_case/main.cpp:110-145):
  * BGK + Guo forcing, streaming and half-way bounce-back, force uniform. */

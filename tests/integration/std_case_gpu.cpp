@@ -3,11 +3,15 @@
 // the bulk list and the fields with the reference's lbsolver headers, then
 // hands the step loop to the GPU through lbamd::GpuSubdomain.
 //
-//   std_case_gpu <case.vtklb> <tau> <iterations> <precision 0|1|2> <out.bin> <F_x> <F_y> [<F_z>]
+//   std_case_gpu <case.vtklb> <tau> <iterations> <precision 0|1|2> <out.bin> <F_x> <F_y> [<F_z>] [<ckpt>]
 //
 // Runs iterations + 1 steps (the loop of std_case/main.cpp:109 runs
 // i = 0..max) and writes, for each bulk node in findBulkNodes order, rho and
-// the nD velocity components as doubles (the fields the loop stores).
+// the nD velocity components as doubles (the fields the loop stores).  With
+// <ckpt>, the field files are also cross-checked: the library's .lblbf is
+// read by the reference's LbField::readFromFile, and the reference's
+// writeToFile output is read back by the library; it prints the number of
+// bulk entries that differ.
 // Built by oracle/build_ref.sh where the reference tree is present.
 #include <cstdio>
 #include <cstdlib>
@@ -49,11 +53,29 @@ int main(int argc, char** argv) {
   for (int n : bulk)
     for (int q = 0; q < LT::nQ; ++q) f(0, q, n) = LT::w[q] * rho(0, n);
 
+  const bool ckpt = argc > 6 + LT::nD;
   try {
     lbamd::GpuSubdomain<LT> gpu(grid, nodes, bulk, precision);
     gpu.upload(f);
     gpu.stepBGK(iterations + 1, tau, force);
     gpu.downloadMacro(rho, vel, bulk);
+    if (ckpt) {
+      const std::string prefix = argv[6 + LT::nD];
+      LbField<LT> now(1, grid.size()), theirs(1, grid.size()), back(1, grid.size());
+      gpu.download(now, bulk);
+      // library writer -> reference reader
+      lbamd::check(lb_write_lbfield(gpu.handle(), (prefix + "_lib").c_str()), gpu.handle());
+      theirs.readFromFile(prefix + "_lib");
+      // reference writer -> library reader
+      now.writeToFile(prefix + "_ref");
+      lbamd::check(lb_read_lbfield(gpu.handle(), (prefix + "_ref").c_str()), gpu.handle());
+      gpu.download(back, bulk);
+      long bad = 0;
+      for (int n : bulk)
+        for (int q = 0; q < LT::nQ; ++q)
+          bad += (theirs(0, q, n) != now(0, q, n)) + (back(0, q, n) != now(0, q, n));
+      std::printf("checkpoint mismatches %ld\n", bad);
+    }
   } catch (const lbamd::GpuError& e) {
     std::fprintf(stderr, "GPU error %d: %s\n", e.code, e.what());
     return 1;

@@ -89,3 +89,26 @@ def test_shim_twophase_drop_in_matches_reference(gpu_available, name):
     for col, ref in ((got[:, 0], d["rho0"]), (got[:, 1], d["rho1"]), (got[:, 2:], d["vel"])):
         assert np.abs(col - ref).max() <= 1e-12 * np.abs(ref).max()
     assert abs(fx - d["force_x"][-1, 1]) <= 1e-12 * abs(d["force_x"][-1, 1])
+
+
+@pytest.mark.parametrize("name", ["d3q19_pack_24", "d2q9_random_7x5"])
+def test_field_files_interchange_with_reference(gpu_available, name):
+    """.lblbf files written by the library are read by the reference's
+    LbField::readFromFile, and the reference's writeToFile output is read back
+    by the library, with every bulk entry unchanged."""
+    d = load_ref(name)
+    p = d["params"]
+    exe = demo(p["lattice"])
+    if exe is None:
+        pytest.skip("drop-in demo not built (needs the reference tree at build time)")
+    g = Geometry(d["geo"].astype(np.int64), p["lattice"], p["periodic"])
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "case.vtklb")
+        with open(path, "w") as fh:
+            fh.write(g.vtklb_text(1, [("init_rho", d["init_rho"])]))
+        F = [repr(float(x)) for x in p["force"]]
+        env = dict(os.environ, LD_LIBRARY_PATH=os.path.join(REF, "mpilib"))
+        res = subprocess.run([exe, path, repr(float(p["tau"])), str(p["steps"]), "0", os.path.join(tmp, "o.bin")]
+                             + F + [os.path.join(tmp, "ck")], env=env, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    assert "checkpoint mismatches 0" in res.stdout, res.stdout
