@@ -165,3 +165,51 @@ def test_batch_twophase_equals_single_context(gpu_available, geos, precision):
     # the single-phase solve is unaffected by its two-phase neighbours
     alone = run_batch([geos[2]], precision, steps, 1e-6, 20000)[0]
     assert (got[99]["k"], got[99]["steps"]) == alone[:2]
+
+
+def test_batch_d2q9_equals_single_context(gpu_available):
+    """The batch runs D2Q9 solves too: channels of different widths, each
+    bit-identical to its single-context solve."""
+    from lbamd.geometry import channel_2d
+    W9 = np.array([4.0, 1.0, 4.0, 1.0, 4.0, 1.0, 4.0, 1.0, 16.0]) / 36.0
+    F = np.array([1e-6, 0.0])
+    geos = [channel_2d(32, 18), channel_2d(48, 26), channel_2d(20, 12)]
+    b = lbamd.Batch(lbamd.D2Q9, lbamd.FP32S, 1 << 24, 50)
+    for i, g in enumerate(geos):
+        assert b.add(g, "x", 1.0, F, 1e-6, 50000, i)
+    got = {}
+    while len(got) < len(geos):
+        b.advance(1)
+        for r in b.collect():
+            got[r["tag"]] = (r["k"], r["steps"])
+    b.close()
+    for i, g in enumerate(geos):
+        topo = lbamd.Topology.from_geometry(g, lbamd.D2Q9, "x")
+        nb, types, bulk, pos = topo.neighbors(), topo.node_types(), topo.bulk(), topo.positions()
+        ctx = lbamd.Context(lbamd.D2Q9, lbamd.FP32S)
+        ctx.set_topology_grid(nb, types, bulk, pos)
+        assert ctx.layout == 2, ctx.layout_note
+        f0 = np.zeros((topo.n_nodes, 9))
+        f0[bulk] = W9[None, :]
+        ctx.upload_f(f0)
+        k, steps = ctx.solve_permeability(1.0, F, 50, 1e-6, 50000, float(g.size))
+        ctx.close()
+        assert got[i] == (k, steps), (i, got[i], k, steps)
+
+
+def test_batch_rejections(gpu_available, geos):
+    """Two-phase solves need a reduced-precision batch; geometries the row
+    windows cannot address and bad arguments are refused with a message."""
+    b = lbamd.Batch(lbamd.D3Q19, lbamd.FP64, 1 << 26, 50)
+    rho0 = (geos[0] > 0).astype(float)
+    with pytest.raises(lbamd.LBError) as e:
+        b.add_twophase(geos[0], "x", rho0, 1.0, 1.0, 1.0, 0.01, 0.7, FORCE, 1e-4, 1000, 0)
+    assert e.value.code == -4
+    with pytest.raises(lbamd.LBError) as e:
+        b.add(geos[0], "xyz", TAU, FORCE, 1e-6, 1000, 1)  # periodic along the row axis
+    assert e.value.code == -4
+    with pytest.raises(lbamd.LBError) as e:
+        b.add(geos[0], "x", 0.4, FORCE, 1e-6, 1000, 2)  # tau <= 1/2
+    assert e.value.code == -1
+    assert b.status()["resident"] == 0
+    b.close()
