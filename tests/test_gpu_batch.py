@@ -213,3 +213,37 @@ def test_batch_rejections(gpu_available, geos):
     assert e.value.code == -1
     assert b.status()["resident"] == 0
     b.close()
+
+
+def test_serving_results_equal_alone(gpu_available, geos):
+    """Continuous batching: a stream of permeability and relative-permeability
+    requests arriving over time through lbamd.serve gives, for every request,
+    the bits of the same request served alone."""
+    from lbamd.serve import Request, Server, run_stream
+    extra = [pack((26, 22, 24), 3.0, 0.45, 21), pack((18, 20, 16), 2.5, 0.5, 22)]
+    allg = geos + extra
+
+    def requests():
+        out = []
+        for i, g in enumerate(allg):
+            t = 0.002 * i
+            if i % 2:
+                rho0 = np.where(g > 0, (np.random.default_rng(i).random(g.shape) < 0.5).astype(float), 0.0)
+                out.append(Request(i, g, "x", force=FORCE, tol=1e-4, max_steps=1500, t_arrival=t, kind="relperm",
+                                   rho0=rho0, wett=1.0, tau0=1.0, tau1=0.9, sigma=0.01, beta=0.7))
+            else:
+                out.append(Request(i, g, "x", tau=TAU, force=FORCE, tol=1e-6, max_steps=20000, t_arrival=t))
+        return out
+
+    srv = Server(1 << 28, precision=lbamd.FP32S, check_every=50)
+    mix, _ = run_stream(srv, requests())
+    srv.close()
+    mix = {d["tag"]: d for d in mix}
+    assert sorted(mix) == list(range(len(allg)))
+    for r in requests():
+        solo = Server(1 << 28, precision=lbamd.FP32S, check_every=50)
+        r.t_arrival = 0.0
+        (d,), _ = run_stream(solo, [r])
+        solo.close()
+        m = mix[r.tag]
+        assert (d["k"], d["k_phase"], d["steps"], d["status"]) == (m["k"], m["k_phase"], m["steps"], m["status"]), r.tag
