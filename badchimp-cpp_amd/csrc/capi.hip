@@ -379,6 +379,8 @@ int setup_topology(lb_ctx* c, int32_t n_nodes, const int32_t* neighbor, const in
   LB_HIP(c, hipMemset(c->d_vel, 0, c->nD * NP * sizeof(double)));
   LB_HIP(c, hipMemset(c->d_phi, 0, NP * sizeof(double)));
   LB_HIP(c, hipMemset(c->d_scalar, 0, 8 * sizeof(double)));
+  // partials stay 0 where no block writes them (an empty bulk list)
+  LB_HIP(c, hipMemset(c->d_partial, 0, 3 * c->n_blocks * sizeof(double)));
   c->n_nodes = n_nodes;
   c->n_bulk = n_bulk;
   c->NP = NP;
