@@ -658,17 +658,27 @@ int lb_set_wall_densities(lb_ctx* c, const double* rho2) {
 }
 
 // ------------------------------------------------------------ steps
-int lb_step_bgk(lb_ctx* c, int32_t nsteps, double tau, const double* force) {
-  if (!c) return LB_ERR_ARG;
-  int rc = check_ready(c, 1);
-  if (rc != LB_OK) return rc;
-  if (nsteps < 0 || !force || !(tau > 0.0)) return fail(c, LB_ERR_ARG, "lb_step_bgk: bad argument");
-  if (nsteps == 0) return LB_OK;
-  hipSetDevice(c->device);
+}  // extern "C"
+
+namespace {
+// Step parameters of the single-phase loop body: BGK (tau_anti <= 0) or the
+// two-relaxation-time operator, with the reference's constants
+// (LBcollision.h:40, 62-63, 206, 223-224).
+BgkArgs single_phase_args(const lb_ctx* c, double tau, double tau_anti, const double* force) {
   BgkArgs a{};
   a.tau_inv = 1.0 / tau;
   a.tau_factor = 1 - 0.5 / tau;
   for (int d = 0; d < 3; ++d) a.F[d] = d < c->nD ? force[d] : 0.0;
+  if (tau_anti > 0.0) {
+    a.trt = 1;
+    a.tauA_inv = 1.0 / tau_anti;
+    a.tauA_factor = 1 - 0.5 / tau_anti;
+  }
+  return a;
+}
+
+int step_single(lb_ctx* c, int32_t nsteps, const BgkArgs& a) {
+  hipSetDevice(c->device);
   Timer t(c);
   for (int32_t s = 0; s < nsteps; ++s) {
     const bool last = s == nsteps - 1;
@@ -684,10 +694,32 @@ int lb_step_bgk(lb_ctx* c, int32_t nsteps, double tau, const double* force) {
     c->cur = 1 - c->cur;
     c->post = true;
   }
-  rc = t.stop(nsteps);
+  const int rc = t.stop(nsteps);
   if (rc != LB_OK) return rc;
   c->have_macro = true;
   return LB_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int lb_step_bgk(lb_ctx* c, int32_t nsteps, double tau, const double* force) {
+  if (!c) return LB_ERR_ARG;
+  int rc = check_ready(c, 1);
+  if (rc != LB_OK) return rc;
+  if (nsteps < 0 || !force || !(tau > 0.0)) return fail(c, LB_ERR_ARG, "lb_step_bgk: bad argument");
+  if (nsteps == 0) return LB_OK;
+  return step_single(c, nsteps, single_phase_args(c, tau, 0.0, force));
+}
+
+int lb_step_trt(lb_ctx* c, int32_t nsteps, double tau_sym, double tau_anti, const double* force) {
+  if (!c) return LB_ERR_ARG;
+  int rc = check_ready(c, 1);
+  if (rc != LB_OK) return rc;
+  if (nsteps < 0 || !force || !(tau_sym > 0.5) || !(tau_anti > 0.5))
+    return fail(c, LB_ERR_ARG, "lb_step_trt: bad argument (tau_sym and tau_anti must exceed 1/2)");
+  if (nsteps == 0) return LB_OK;
+  return step_single(c, nsteps, single_phase_args(c, tau_sym, tau_anti, force));
 }
 
 int lb_step_cg(lb_ctx* c, int32_t nsteps, double tau0, double tau1, double sigma, double beta,
@@ -793,19 +825,19 @@ int lb_reduce_flux(lb_ctx* c, int32_t dir, double* sum_fc, double* sum_phi0_fc, 
   return LB_OK;
 }
 
-int lb_solve_permeability(lb_ctx* c, double tau, const double* force, int32_t check_every, double tol,
-                          int64_t max_steps, double n_box, double* k_out, int64_t* steps_out) {
-  if (!c) return LB_ERR_ARG;
+}  // extern "C"
+
+namespace {
+int solve_permeability(lb_ctx* c, const char* name, double tau, double tau_anti, const double* force,
+                       int32_t check_every, double tol, int64_t max_steps, double n_box, double* k_out,
+                       int64_t* steps_out) {
   int rc = check_ready(c, 1);
   if (rc != LB_OK) return rc;
   if (!force || !(tau > 0.5) || check_every < 1 || !(tol > 0.0) || max_steps < 1 || !(n_box > 0.0) ||
       force[0] == 0.0)
-    return fail(c, LB_ERR_ARG, "lb_solve_permeability: bad argument");
+    return fail(c, LB_ERR_ARG, "%s: bad argument", name);
   hipSetDevice(c->device);
-  BgkArgs a{};
-  a.tau_inv = 1.0 / tau;
-  a.tau_factor = 1 - 0.5 / tau;
-  for (int d = 0; d < 3; ++d) a.F[d] = d < c->nD ? force[d] : 0.0;
+  const BgkArgs a = single_phase_args(c, tau, tau_anti, force);
   const double nu = (tau - 0.5) / 3.0;
   double k_prev = 0.0, k = 0.0;
   int64_t steps = 0;
@@ -848,6 +880,25 @@ int lb_solve_permeability(lb_ctx* c, double tau, const double* force, int32_t ch
   if (k_out) *k_out = k;
   if (steps_out) *steps_out = steps;
   return LB_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int lb_solve_permeability(lb_ctx* c, double tau, const double* force, int32_t check_every, double tol,
+                          int64_t max_steps, double n_box, double* k_out, int64_t* steps_out) {
+  if (!c) return LB_ERR_ARG;
+  return solve_permeability(c, "lb_solve_permeability", tau, 0.0, force, check_every, tol, max_steps, n_box,
+                            k_out, steps_out);
+}
+
+int lb_solve_permeability_trt(lb_ctx* c, double tau_sym, double tau_anti, const double* force,
+                              int32_t check_every, double tol, int64_t max_steps, double n_box, double* k_out,
+                              int64_t* steps_out) {
+  if (!c) return LB_ERR_ARG;
+  if (!(tau_anti > 0.5)) return fail(c, LB_ERR_ARG, "lb_solve_permeability_trt: tau_anti must exceed 1/2");
+  return solve_permeability(c, "lb_solve_permeability_trt", tau_sym, tau_anti, force, check_every, tol,
+                            max_steps, n_box, k_out, steps_out);
 }
 
 // ------------------------------------------------------------ timing

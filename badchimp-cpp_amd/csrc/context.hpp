@@ -49,9 +49,14 @@ inline uint32_t pidx_host(int Q, uint32_t q, uint32_t i) {
 }
 
 struct BgkArgs {
-  double tau_inv;     // 1.0 / tau                (LBcollision.h:40)
-  double tau_factor;  // 1 - 0.5 / tau            (LBcollision.h:206)
+  double tau_inv;     // 1.0 / tau                (LBcollision.h:40); 1 / tauSym for TRT (:62)
+  double tau_factor;  // 1 - 0.5 / tau            (LBcollision.h:206); tauSym_factor for TRT (:223)
   double F[3];        // body force
+  // two-relaxation-time operator (calcOmegaBGKTRT / calcDeltaOmegaFTRT with
+  // phi = 1, LBcollision.h:50-75, 214-233); used when trt != 0
+  int32_t trt;
+  double tauA_inv;     // 1.0 / tauAnti                  (LBcollision.h:63)
+  double tauA_factor;  // 1 - 0.5 / tauAnti              (LBcollision.h:224)
 };
 
 // One solve of the batched form: its two population buffers (each Q * NP

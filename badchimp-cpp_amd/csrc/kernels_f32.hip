@@ -72,12 +72,15 @@ __device__ __forceinline__ void load_g(const S* __restrict__ gin, const A& ad, i
 }
 
 // One node of the BGK + Guo step on shifted populations; returns j_x (FP64)
-// for the permeability sum when REDUCE.
-template <class L, typename S, class A, bool FIRST, bool STORE, bool REDUCE>
+// for the permeability sum when REDUCE.  TRT: the two-relaxation-time
+// operator (LBcollision.h:50-75, 214-233) with 1/tauAnti = tauA_inv and
+// tauAnti_factor = tauA_factor; tau_inv, tau_factor are then the symmetric ones.
+template <class L, typename S, class A, bool FIRST, bool STORE, bool REDUCE, bool TRT = false>
 __device__ __forceinline__ double bgk_node_lp(const S* __restrict__ gin, S* __restrict__ gout, const A& ad,
                                               int32_t i, int64_t NP, float tau_inv, float tau_factor,
                                               const float (&F)[3], double* __restrict__ rho_o,
-                                              double* __restrict__ vel_o) {
+                                              double* __restrict__ vel_o, float tauA_inv = 0.0f,
+                                              float tauA_factor = 0.0f) {
   constexpr int Q = L::nQ, D = L::nD;
   float g[Q];
   load_g<L, S, FIRST>(gin, ad, i, NP, g);
@@ -108,6 +111,33 @@ __device__ __forceinline__ double bgk_node_lp(const S* __restrict__ gin, S* __re
   const float r45 = 4.5f * rho, r3 = 3.0f * rho;
   // one 64-bit address per node; the direction offsets fold into the stores
   S* __restrict__ out = elem_ptr<A::kOff32>(gout, pidx<Q>(0, i));
+  if constexpr (TRT) {
+    // Each pair relaxes its even part (gs = (g_q + g_r)/2 = f_sym - w, towards
+    // w(a0 + 4.5 rho cu^2)) with 1/tauSym and its odd part (ga = (g_q - g_r)/2,
+    // towards 3 w rho cu) with 1/tauAnti; the Guo term splits the same way.
+    const float oma = 1.0f - tauA_inv;
+#pragma unroll
+    for (int q = 0; q < Q - 1; ++q) {
+      const int rq = L::rev(q);
+      if (rq < q) continue;
+      const float w = float(L::w(q));
+      const float cu = cdot<L>(q, u);
+      const float cF = cdot<L>(q, F);
+      const float gs = 0.5f * (g[q] + g[rq]);
+      const float ga = 0.5f * (g[q] - g[rq]);
+      const float sym = omt * gs + (w * tau_inv) * (a0 + r45 * cu * cu) + (w * tau_factor) * (9.0f * cF * cu + m3uF);
+      const float anti = oma * ga + (w * tauA_inv) * (r3 * cu) + (w * tauA_factor) * (3.0f * cF);
+      __builtin_nontemporal_store(Store<S>::put(sym + anti), out + (q << kPLog));
+      __builtin_nontemporal_store(Store<S>::put(sym - anti), out + (rq << kPLog));
+    }
+    {
+      constexpr int q = Q - 1;  // rest: even only
+      const float w = float(L::w(q));
+      const float gq = omt * g[q] + (w * tau_inv) * a0 + (w * tau_factor) * m3uF;
+      __builtin_nontemporal_store(Store<S>::put(gq), out + (q << kPLog));
+    }
+    return REDUCE ? static_cast<double>(jd[0]) : 0.0;
+  }
 #pragma unroll
   for (int q = 0; q < Q - 1; ++q) {
     const int rq = L::rev(q);
@@ -134,19 +164,21 @@ __device__ __forceinline__ double bgk_node_lp(const S* __restrict__ gin, S* __re
   return REDUCE ? static_cast<double>(jd[0]) : 0.0;
 }
 
-template <class L, typename S, class A, bool FIRST, bool STORE, bool REDUCE>
+template <class L, typename S, class A, bool FIRST, bool STORE, bool REDUCE, bool TRT>
 __global__ __launch_bounds__(kBlock) void k_bgk_lp(const S* __restrict__ gin, S* __restrict__ gout, A ad,
                                                    int32_t n, int64_t NP, float tau_inv, float tau_factor,
                                                    float F0, float F1, float F2,
                                                    double* __restrict__ rho_o,
                                                    double* __restrict__ vel_o,
-                                                   double* __restrict__ partial) {
+                                                   double* __restrict__ partial, float tauA_inv,
+                                                   float tauA_factor) {
   const int32_t b = static_cast<int32_t>(blockIdx.x);
   const int32_t i = b * kBlock + threadIdx.x;
   double jx = 0.0;
   if (i < n) {
     const float F[3] = {F0, F1, F2};
-    jx = bgk_node_lp<L, S, A, FIRST, STORE, REDUCE>(gin, gout, ad, i, NP, tau_inv, tau_factor, F, rho_o, vel_o);
+    jx = bgk_node_lp<L, S, A, FIRST, STORE, REDUCE, TRT>(gin, gout, ad, i, NP, tau_inv, tau_factor, F, rho_o,
+                                                         vel_o, tauA_inv, tauA_factor);
   }
   if (REDUCE) {
     const double s = block_sum(jx);
@@ -610,8 +642,11 @@ hipError_t bgk_lowp(int precision, int lattice, bool first, bool store, bool red
                     const void* gin, void* gout, const Addr& ad, int32_t n, int64_t NP,
                     const BgkArgs& a, double* rho, double* vel, double* partial, hipStream_t s) {
   if (n <= 0) return hipSuccess;
+  if (reduce && !store) return hipErrorInvalidValue;
   const float ti = static_cast<float>(a.tau_inv), tf = static_cast<float>(a.tau_factor);
   const float F0 = static_cast<float>(a.F[0]), F1 = static_cast<float>(a.F[1]), F2 = static_cast<float>(a.F[2]);
+  const float tai = static_cast<float>(a.tauA_inv), taf = static_cast<float>(a.tauA_factor);  // TRT only
+  // (first, store, reduce); reduce comes with store
   const int key = (first ? 4 : 0) | (store ? 2 : 0) | (reduce ? 1 : 0);
   return dispatch(precision, lattice, ad, [&](auto l, auto st, auto addr) {
     using L = decltype(l);
@@ -620,18 +655,24 @@ hipError_t bgk_lowp(int precision, int lattice, bool first, bool store, bool red
     const S* gi = static_cast<const S*>(gin);
     S* go = static_cast<S*>(gout);
     const dim3 g = grid_for(n), b(kBlock);
-#define LB_K(F_, S_, R_) \
-  k_bgk_lp<L, S, A, F_, S_, R_><<<g, b, 0, s>>>(gi, go, addr, n, NP, ti, tf, F0, F1, F2, rho, vel, partial)
-    switch (key) {
-      case 0: LB_K(false, false, false); break;
-      case 1: LB_K(false, false, true); break;
-      case 2: LB_K(false, true, false); break;
-      case 3: LB_K(false, true, true); break;
-      case 4: LB_K(true, false, false); break;
-      case 5: LB_K(true, false, true); break;
-      case 6: LB_K(true, true, false); break;
-      default: LB_K(true, true, true); break;
+#define LB_K(F_, S_, R_, T_)                                                                             \
+  k_bgk_lp<L, S, A, F_, S_, R_, T_><<<g, b, 0, s>>>(gi, go, addr, n, NP, ti, tf, F0, F1, F2, rho, vel, partial, \
+                                                    tai, taf)
+#define LB_KEYS(T_)                                 \
+  switch (key) {                                    \
+    case 0: LB_K(false, false, false, T_); break;   \
+    case 2: LB_K(false, true, false, T_); break;    \
+    case 3: LB_K(false, true, true, T_); break;     \
+    case 4: LB_K(true, false, false, T_); break;    \
+    case 6: LB_K(true, true, false, T_); break;     \
+    default: LB_K(true, true, true, T_); break;     \
+  }
+    if (a.trt) {
+      LB_KEYS(true)
+    } else {
+      LB_KEYS(false)
     }
+#undef LB_KEYS
 #undef LB_K
   });
 }

@@ -62,7 +62,9 @@ __device__ __forceinline__ double block_sum(double v) {
 // ------------------------------------------------------------------ BGK
 // One node of the std_case loop body (std_case/main.cpp:110-145); returns the
 // node's j_x = sum_q f c_x + F_x/2 for the permeability sum when REDUCE.
-template <class L, class A, bool FIRST, bool STORE, bool REDUCE>
+// TRT: the loop body with the two-relaxation-time operator in place of
+// calcOmegaBGK / calcDeltaOmegaF (LBcollision.h:50-75, 214-233).
+template <class L, class A, bool FIRST, bool STORE, bool REDUCE, bool TRT = false>
 __device__ __forceinline__ double bgk_node(const double* __restrict__ fin, double* __restrict__ fout, const A& ad,
                                            int32_t i, int64_t NP, const BgkArgs& a, double* __restrict__ rho_o,
                                            double* __restrict__ vel_o) {
@@ -86,6 +88,24 @@ __device__ __forceinline__ double bgk_node(const double* __restrict__ fin, doubl
   const double c2u2 = L::c2 * u2;
   const double c2uF = L::c2 * uF;
   double* __restrict__ out = elem_ptr<A::kOff32>(fout, pidx<Q>(0, i));
+  if constexpr (TRT) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int r = L::rev(q);
+      const double cu = cdot<L>(q, u);
+      const double cF = cdot<L>(q, a.F);
+      // calcOmegaBGKTRT (LBcollision.h:66-70)
+      const double fqSym = 0.5 * (f[0][q] + f[0][r]);
+      const double fqAnti = 0.5 * (f[0][q] - f[0][r]);
+      const double rw = rho * L::w(q);
+      const double om = (-a.tau_inv) * (fqSym - rw * (1.0 + L::c4Inv0_5 * (cu * cu - c2u2))) -
+                        a.tauA_inv * (fqAnti - (rw * L::c2Inv) * cu);
+      // calcDeltaOmegaFTRT, phi = 1 (LBcollision.h:229): w * 1.0 == w
+      const double dF = L::w(q) * ((a.tauA_factor * L::c2Inv) * cF + (a.tau_factor * L::c4Inv) * (cF * cu - c2uF));
+      __builtin_nontemporal_store((f[0][q] + om) + dF, out + (q << kPLog));
+    }
+    return jx;
+  }
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const double cu = cdot<L>(q, u);
@@ -100,7 +120,7 @@ __device__ __forceinline__ double bgk_node(const double* __restrict__ fin, doubl
   return jx;
 }
 
-template <class L, class A, bool FIRST, bool STORE, bool REDUCE>
+template <class L, class A, bool FIRST, bool STORE, bool REDUCE, bool TRT>
 __global__ __launch_bounds__(kBlock) void k_bgk(const double* __restrict__ fin,
                                                 double* __restrict__ fout, A ad, int32_t n,
                                                 int64_t NP, BgkArgs a, double* __restrict__ rho_o,
@@ -109,7 +129,7 @@ __global__ __launch_bounds__(kBlock) void k_bgk(const double* __restrict__ fin,
   const int32_t b = static_cast<int32_t>(blockIdx.x);
   const int32_t i = b * kBlock + threadIdx.x;
   double jx = 0.0;
-  if (i < n) jx = bgk_node<L, A, FIRST, STORE, REDUCE>(fin, fout, ad, i, NP, a, rho_o, vel_o);
+  if (i < n) jx = bgk_node<L, A, FIRST, STORE, REDUCE, TRT>(fin, fout, ad, i, NP, a, rho_o, vel_o);
   if (REDUCE) {
     const double s = block_sum(jx);
     if (threadIdx.x == 0) partial[b] = s;
@@ -408,18 +428,26 @@ static void launch_bgk(bool first, bool store, bool reduce, dim3 g, hipStream_t 
                        double* fout, const A& ad, int32_t n, int64_t NP, const BgkArgs& a, double* rho,
                        double* vel, double* partial) {
   const dim3 b(kBlock);
+  // (first, store, reduce): reduce comes with store (the last step of a
+  // check interval), so (x, false, true) is not instantiated
   const int key = (first ? 4 : 0) | (store ? 2 : 0) | (reduce ? 1 : 0);
-#define LB_K(F_, S_, R_) k_bgk<L, A, F_, S_, R_><<<g, b, 0, s>>>(fin, fout, ad, n, NP, a, rho, vel, partial)
-  switch (key) {
-    case 0: LB_K(false, false, false); break;
-    case 1: LB_K(false, false, true); break;
-    case 2: LB_K(false, true, false); break;
-    case 3: LB_K(false, true, true); break;
-    case 4: LB_K(true, false, false); break;
-    case 5: LB_K(true, false, true); break;
-    case 6: LB_K(true, true, false); break;
-    default: LB_K(true, true, true); break;
+#define LB_K(F_, S_, R_, T_) \
+  k_bgk<L, A, F_, S_, R_, T_><<<g, b, 0, s>>>(fin, fout, ad, n, NP, a, rho, vel, partial)
+#define LB_KEYS(T_)                            \
+  switch (key) {                               \
+    case 0: LB_K(false, false, false, T_); break; \
+    case 2: LB_K(false, true, false, T_); break;  \
+    case 3: LB_K(false, true, true, T_); break;   \
+    case 4: LB_K(true, false, false, T_); break;  \
+    case 6: LB_K(true, true, false, T_); break;   \
+    default: LB_K(true, true, true, T_); break;   \
   }
+  if (a.trt) {
+    LB_KEYS(true)
+  } else {
+    LB_KEYS(false)
+  }
+#undef LB_KEYS
 #undef LB_K
 }
 
@@ -427,6 +455,7 @@ hipError_t bgk_f64(int lattice, bool first, bool store, bool reduce, const doubl
                    double* fout, const Addr& ad, int32_t n, int64_t NP, const BgkArgs& a,
                    double* rho, double* vel, double* partial, hipStream_t s) {
   if (n <= 0) return hipSuccess;
+  if (reduce && !store) return hipErrorInvalidValue;
   const dim3 g = grid_for(n);
   LB_DISPATCH_LATTICE(lattice, L, {
     if (ad.layout == 2) launch_bgk<L>(first, store, reduce, g, s, fin, fout, ad.win, n, NP, a, rho, vel, partial);

@@ -6,6 +6,7 @@
 // half-way bounce-back with one call:
 //
 //   std_case/main.cpp:109-145       ->  gpu.stepBGK(1, tau, bodyForce(0, 0));
+//     (with calcOmegaBGKTRT / calcDeltaOmegaFTRT)  gpu.stepTRT(1, tauSym, tauAnti, F);
 //   main_TWOPHASE.cpp:238-392       ->  gpu.stepCG(1, tau0, tau1, sigma, beta, momx, Fyz);
 //
 // The wrapper reads the reference's objects only through their public
@@ -134,6 +135,14 @@ class GpuSubdomain {
     double F[3] = {0, 0, 0};
     for (int d = 0; d < DXQY::nD; ++d) F[d] = force[d];
     check(lb_step_bgk(ctx_, nsteps, tau, F), ctx_);
+  }
+
+  // the same loop body with calcOmegaBGKTRT / calcDeltaOmegaFTRT (phi = 1)
+  template <typename VEC>
+  void stepTRT(int nsteps, double tauSym, double tauAnti, const VEC& force) {
+    double F[3] = {0, 0, 0};
+    for (int d = 0; d < DXQY::nD; ++d) F[d] = force[d];
+    check(lb_step_trt(ctx_, nsteps, tauSym, tauAnti, F), ctx_);
   }
 
   // nsteps of the colour-gradient loop body with the momx flux controller

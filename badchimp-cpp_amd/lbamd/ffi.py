@@ -58,6 +58,7 @@ SIGNATURES = {
     "lb_download_vector": (C.c_int, [_vp, C.c_int32, C.c_int32, _f64p]),
     "lb_set_wall_densities": (C.c_int, [_vp, _f64p]),
     "lb_step_bgk": (C.c_int, [_vp, C.c_int32, C.c_double, _f64p]),
+    "lb_step_trt": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_double, _f64p]),
     "lb_step_cg": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double,
                              C.c_int32, _f64p]),
     "lb_last_force": (C.c_int, [_vp, _f64p]),
@@ -65,6 +66,9 @@ SIGNATURES = {
                                  C.POINTER(C.c_double)]),
     "lb_solve_permeability": (C.c_int, [_vp, C.c_double, _f64p, C.c_int32, C.c_double, C.c_int64,
                                         C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "lb_solve_permeability_trt": (C.c_int, [_vp, C.c_double, C.c_double, _f64p, C.c_int32, C.c_double,
+                                            C.c_int64, C.c_double, C.POINTER(C.c_double),
+                                            C.POINTER(C.c_int64)]),
     "lb_sync": (C.c_int, [_vp]),
     "lb_kernel_time": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "lb_kernel_time_reset": (C.c_int, [_vp]),
@@ -380,6 +384,13 @@ class Context:
         F[: self.nD] = force[: self.nD]
         self._check(lib().lb_step_bgk(self._h, int(nsteps), float(tau), F))
 
+    def step_trt(self, nsteps, tau_sym, tau_anti, force):
+        """std_case steps with the two-relaxation-time operator
+        (calcOmegaBGKTRT / calcDeltaOmegaFTRT, LBcollision.h:50-75, 214-233)."""
+        F = np.zeros(3)
+        F[: self.nD] = force[: self.nD]
+        self._check(lib().lb_step_trt(self._h, int(nsteps), float(tau_sym), float(tau_anti), F))
+
     def step_cg(self, nsteps, tau0, tau1, sigma, beta, force_mode, force):
         F = np.zeros(3)
         F[: len(force)] = force
@@ -402,6 +413,15 @@ class Context:
         k, s = C.c_double(), C.c_int64()
         self._check(lib().lb_solve_permeability(self._h, float(tau), F, int(check_every), float(tol),
                                                 int(max_steps), float(n_box), C.byref(k), C.byref(s)))
+        return k.value, s.value
+
+    def solve_permeability_trt(self, tau_sym, tau_anti, force, check_every, tol, max_steps, n_box):
+        F = np.zeros(3)
+        F[: self.nD] = force[: self.nD]
+        k, s = C.c_double(), C.c_int64()
+        self._check(lib().lb_solve_permeability_trt(self._h, float(tau_sym), float(tau_anti), F,
+                                                    int(check_every), float(tol), int(max_steps), float(n_box),
+                                                    C.byref(k), C.byref(s)))
         return k.value, s.value
 
     def write_lbfield(self, prefix):

@@ -101,7 +101,7 @@ def make_ctx(precision, nb, types, bulk, pos, layout):
     return ctx
 
 
-def gpu_leg(topo, nb, types, bulk, pos, layout, precision, steps, warmup, tau, force):
+def gpu_leg(topo, nb, types, bulk, pos, layout, precision, steps, warmup, tau, force, tau_anti=None):
     ctx = make_ctx(precision, nb, types, bulk, pos, layout)
     got_layout = {2: "row windows", 1: "rows"}.get(ctx.layout, "table")
     if ctx.layout_note and ctx.layout_note != got_layout:
@@ -110,12 +110,18 @@ def gpu_leg(topo, nb, types, bulk, pos, layout, precision, steps, warmup, tau, f
     f0 = np.zeros((topo.n_nodes, 19))
     f0[bulk] = w[None, :]
     ctx.upload_f(f0)
+
+    def step(n):
+        if tau_anti is None:
+            ctx.step_bgk(n, tau, force)
+        else:
+            ctx.step_trt(n, tau, tau_anti, force)
     if warmup:
-        ctx.step_bgk(warmup, tau, force)
+        step(warmup)
     ctx.sync()
     ctx.kernel_time_reset()
     t0 = time.perf_counter()
-    ctx.step_bgk(steps, tau, force)
+    step(steps)
     ctx.sync()
     t1 = time.perf_counter()
     kms, launches = ctx.kernel_time()
@@ -124,17 +130,46 @@ def gpu_leg(topo, nb, types, bulk, pos, layout, precision, steps, warmup, tau, f
 
 
 def permeability(nb, types, bulk, pos, layout, n_nodes, precision, tau, force, n_box, tol=1e-7,
-                 max_steps=60000):
+                 max_steps=60000, tau_anti=None):
     ctx = make_ctx(precision, nb, types, bulk, pos, layout)
     w = np.array([2.0 / 36.0] * 3 + [1.0 / 36.0] * 6 + [2.0 / 36.0] * 3 + [1.0 / 36.0] * 6 + [12.0 / 36.0])
     f0 = np.zeros((n_nodes, 19))
     f0[bulk] = w[None, :]
     ctx.upload_f(f0)
     t0 = time.perf_counter()
-    k, steps = ctx.solve_permeability(tau, force, 100, tol, max_steps, n_box)
+    if tau_anti is None:
+        k, steps = ctx.solve_permeability(tau, force, 100, tol, max_steps, n_box)
+    else:
+        k, steps = ctx.solve_permeability_trt(tau, tau_anti, force, 100, tol, max_steps, n_box)
     dt = time.perf_counter() - t0
     ctx.close()
     return k, steps, dt
+
+
+def trt_leg(args, geo, topo, nb, types, bulk, pos, force, n_bulk, k_bgk):
+    """Two-relaxation-time step on the same pack: tau_sym = 1, tau_anti =
+    7/8 ((tau_sym - 1/2)(tau_anti - 1/2) = 3/16), timed in FP32S and FP64, and
+    its permeability in FP64 and FP32S beside BGK's at tau = 1."""
+    ts, ta = 1.0, 0.875
+    out = {"tau_sym": ts, "tau_anti": ta}
+    for name, prec, bpl in (("fp32s", lbamd.FP32S, 2 * 19 * 4), ("fp64", lbamd.FP64, 2 * 19 * 8)):
+        t, kms, _ = gpu_leg(topo, nb, types, bulk, pos, args.layout, prec, args.steps, args.warmup, ts, force,
+                            tau_anti=ta)
+        gbs = n_bulk * bpl / (kms * 1e-3) / 1e9
+        out[name] = {"value": round(n_bulk * args.steps / t / 1e6, 2), "unit": "MLUPS", "avg_kernel_ms": kms,
+                     "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        log("TRT %s: %.1f MLUPS, kernel %.3f ms" % (name, out[name]["value"], kms))
+    if not args.no_perm:
+        n_box = float(geo.size)
+        k64, s64, _ = permeability(nb, types, bulk, pos, args.layout, topo.n_nodes, lbamd.FP64, ts, force, n_box,
+                                   tau_anti=ta)
+        k32, s32, _ = permeability(nb, types, bulk, pos, args.layout, topo.n_nodes, lbamd.FP32S, ts, force, n_box,
+                                   tau_anti=ta)
+        out.update({"k_fp64": k64, "steps_fp64": s64, "k_fp32s": k32, "steps_fp32s": s32,
+                    "k_rel_err_fp32s": abs(k32 - k64) / abs(k64),
+                    "k_bgk_tau1_over_k_trt": (k_bgk / k64) if k_bgk else None})
+        log("TRT k FP64 = %.10e (%d steps), FP32S rel err %.2e" % (k64, s64, out["k_rel_err_fp32s"]))
+    return out
 
 
 def cpu_baseline(nb, types, bulk, n_nodes, tau, force, budget_s=15.0):
@@ -505,6 +540,7 @@ def main():
     ap.add_argument("--baseline-every", type=int, default=16)
     ap.add_argument("--serve", type=int, default=48, help="requests per serving run (0: skip)")
     ap.add_argument("--no-channel", action="store_true")
+    ap.add_argument("--no-trt", action="store_true")
     ap.add_argument("--tp-size", type=int, default=192)
     ap.add_argument("--tp-steps", type=int, default=50000, help="two-phase steps (0: skip)")
     ap.add_argument("--serve-pool-gb", type=float, default=16.0)
@@ -558,6 +594,8 @@ def main():
                 "k_rel_err_fp16s": abs(k16 - k64) / abs(k64), "tol_dk_over_k": 1e-7,
                 "definition": "k = nu * sum_bulk(sum_q f c_x + F_x/2) / (N_box * F_x), nu = (tau-1/2)/3"}
 
+    trt = None if args.no_trt else trt_leg(args, geo, topo, nb, types, bulk, pos, force, n_bulk,
+                                            perm["k_fp64"] if perm else None)
     batched = batched_leg(args.batch, args.baseline_every, tau, force) if args.batch > 0 else None
     channel = None if args.no_channel else channel_leg()
     if channel:
@@ -600,6 +638,7 @@ def main():
                  "achieved_GBs": round(ach64, 1), "frac": round(ach64 / HBM_PEAK_GBS, 4),
                  "bytes_per_lup": bytes64, "traffic": traffic64},
         "permeability": perm,
+        "trt": trt,
         "batched": batched,
         "channel": channel,
         "twophase": twophase,

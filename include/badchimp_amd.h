@@ -156,6 +156,14 @@ int lb_write_vector(lb_ctx* ctx, int32_t which, const char* prefix);
  * BGK + Guo forcing, streaming and half-way bounce-back, force uniform. */
 int lb_step_bgk(lb_ctx* ctx, int32_t nsteps, double tau, const double* force);
 
+/* The same loop body with the two-relaxation-time operator in place of
+ * calcOmegaBGK / calcDeltaOmegaF: calcOmegaBGKTRT(f, tau_sym, tau_anti, ...)
+ * and calcDeltaOmegaFTRT(tau_sym, tau_anti, 1.0, ...) (LBcollision.h:50-75,
+ * 214-233).  The even part of each direction pair relaxes with tau_sym (the
+ * viscosity, nu = (tau_sym - 1/2)/3), the odd part with tau_anti.  Both > 1/2. */
+int lb_step_trt(lb_ctx* ctx, int32_t nsteps, double tau_sym, double tau_anti,
+                const double* force);
+
 /* nsteps iterations of the colour-gradient loop body
  * (main_TWOPHASE.cpp:238-392).  force[nD]: in LB_FORCE_MOMX mode force[0]
  * is the target momentum momx and force[1..] the fixed other components. */
@@ -183,6 +191,13 @@ int lb_reduce_flux(lb_ctx* ctx, int32_t dir, double* sum_fc,
 int lb_solve_permeability(lb_ctx* ctx, double tau, const double* force,
                           int32_t check_every, double tol, int64_t max_steps,
                           double n_box, double* k_out, int64_t* steps_out);
+/* The same solve stepping lb_step_trt; nu = (tau_sym - 1/2)/3.  With
+ * (tau_sym - 1/2)(tau_anti - 1/2) fixed (3/16 puts a half-way wall exactly
+ * half-way for Poiseuille flow) k does not depend on the viscosity. */
+int lb_solve_permeability_trt(lb_ctx* ctx, double tau_sym, double tau_anti,
+                              const double* force, int32_t check_every,
+                              double tol, int64_t max_steps, double n_box,
+                              double* k_out, int64_t* steps_out);
 
 /* ------------------------------------------------------------ timing */
 int lb_sync(lb_ctx* ctx);

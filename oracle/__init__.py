@@ -38,6 +38,9 @@ def lib():
         L.orc_std_case.restype = C.c_int
         L.orc_std_case.argtypes = [C.c_int, C.c_int32, _i32p, _i16p, _i32p, C.c_int32, _f64p, _f64p,
                                    C.c_double, _f64p, C.c_int32, _f64p, _f64p, C.c_int]
+        L.orc_std_case_trt.restype = C.c_int
+        L.orc_std_case_trt.argtypes = [C.c_int, C.c_int32, _i32p, _i16p, _i32p, C.c_int32, _f64p, _f64p,
+                                       C.c_double, C.c_double, _f64p, C.c_int32, _f64p, _f64p, C.c_int]
         L.orc_twophase.restype = C.c_int
         L.orc_twophase.argtypes = [C.c_int, C.c_int32, _i32p, _i16p, _i32p, C.c_int32, _f64p, _f64p,
                                    _f64p, _f64p, _f64p, C.c_double, C.c_double, C.c_double,
@@ -56,7 +59,9 @@ def _c(a, dt):
 
 def run_std_case(lattice, neighbor, node_type, bulk, f, tau, force, nsteps, nthreads=1):
     """nsteps iterations of the std_case body.  f: LbField (n_nodes, Q), updated
-    copy returned with the rho / vel of the last iteration."""
+    copy returned with the rho / vel of the last iteration.  tau: the BGK
+    relaxation time, or a pair (tau_sym, tau_anti) for the two-relaxation-time
+    operator (calcOmegaBGKTRT / calcDeltaOmegaFTRT with phi = 1)."""
     n_nodes = node_type.shape[0]
     Q, D = Q_OF[lattice], ND_OF[lattice]
     f = _c(f, np.float64).reshape(-1).copy()
@@ -66,9 +71,13 @@ def run_std_case(lattice, neighbor, node_type, bulk, f, tau, force, nsteps, nthr
     F = np.zeros(3)
     F[:D] = force[:D]
     bulk = _c(bulk, np.int32)
-    lib().orc_std_case(lattice, n_nodes, _c(neighbor, np.int32).ravel(), _c(node_type, np.int16),
-                       bulk if bulk.size else np.zeros(1, np.int32), bulk.size, f, ftmp,
-                       float(tau), F, int(nsteps), rho, vel, int(nthreads))
+    args = (lattice, n_nodes, _c(neighbor, np.int32).ravel(), _c(node_type, np.int16),
+            bulk if bulk.size else np.zeros(1, np.int32), bulk.size, f, ftmp)
+    tail = (F, int(nsteps), rho, vel, int(nthreads))
+    if np.ndim(tau):
+        lib().orc_std_case_trt(*args, float(tau[0]), float(tau[1]), *tail)
+    else:
+        lib().orc_std_case(*args, float(tau), *tail)
     return f.reshape(n_nodes, Q), rho, vel.reshape(n_nodes, D)
 
 

@@ -7,6 +7,12 @@
 #                   (std_case/main.cpp:14-15 offers the switch); the edited
 #                   copy is written to oracle/_ref/ and never committed
 #   twophase        src/twophase/main_TWOPHASE.cpp as shipped (D3Q19)
+#   std_case_trt_d2q9, std_case_trt_d3q19
+#                   std_case/main.cpp with its collision switched to the
+#                   reference's two-relaxation-time operator: calcOmegaBGK ->
+#                   calcOmegaBGKTRT (LBcollision.h:50-75) and calcDeltaOmegaF
+#                   -> calcDeltaOmegaFTRT with phi = 1 (LBcollision.h:214-233);
+#                   tau = fluid.tau[0] (symmetric), tauAnti = fluid.tau[1]
 #   std_case_gpu_d2q9, std_case_gpu_d3q19, twophase_gpu
 #                   tests/integration/{std_case,twophase}_gpu.cpp: the
 #                   reference's set-up classes driving this repository's
@@ -46,6 +52,24 @@ else
   mv "$OUT/std_case_d3q19.cpp.new" "$OUT/std_case_d3q19.cpp"
 fi
 
+# the two-relaxation-time variants (edited copies, never committed)
+trt_edit() {  # lattice line
+  sed -e "s/^#define LT D2Q9/#define LT $1/" \
+      -e 's/lbBase_t tau = input\["fluid"\]\["tau"\];/lbBase_t tau = input["fluid"]["tau"][0]; lbBase_t tauAnti = input["fluid"]["tau"][1];/' \
+      -e 's/calcOmegaBGK<LT>(fNode, tau, rhoNode, u2, cu)/calcOmegaBGKTRT<LT>(fNode, tau, tauAnti, rhoNode, u2, cu)/' \
+      -e 's/calcDeltaOmegaF<LT>(tau, cu, uF, cF)/calcDeltaOmegaFTRT<LT>(tau, tauAnti, 1.0, cu, uF, cF)/' \
+      "$REF/std_case/main.cpp"
+}
+for lat in D2Q9 D3Q19; do
+  f="$OUT/std_case_trt_$(echo $lat | tr 'A-Z' 'a-z').cpp"
+  trt_edit $lat > "$f.new"
+  if ! grep -q calcOmegaBGKTRT "$f.new" || ! grep -q calcDeltaOmegaFTRT "$f.new" || ! grep -q tauAnti "$f.new"; then
+    echo "build_ref: the TRT switch did not apply to std_case/main.cpp" >&2
+    exit 1
+  fi
+  if cmp -s "$f.new" "$f"; then rm "$f.new"; else mv "$f.new" "$f"; fi
+done
+
 build() {  # name, main source, extra include dir
   local name=$1 src=$2 inc=$3
   if [ ! -x "$OUT/$name" ] || [ "$src" -nt "$OUT/$name" ]; then
@@ -56,6 +80,8 @@ build() {  # name, main source, extra include dir
 build std_case_d2q9 "$REF/std_case/main.cpp" "$REF/std_case"
 build std_case_d3q19 "$OUT/std_case_d3q19.cpp" "$REF/std_case"
 build twophase "$REF/twophase/main_TWOPHASE.cpp" "$REF/twophase"
+build std_case_trt_d2q9 "$OUT/std_case_trt_d2q9.cpp" "$REF/std_case"
+build std_case_trt_d3q19 "$OUT/std_case_trt_d3q19.cpp" "$REF/std_case"
 
 # the drop-in demonstration (test infrastructure): reference set-up + shim
 REPO="$(cd "$HERE/.." && pwd)"

@@ -14,8 +14,10 @@
  *   D3Q19 tables and helpers   LBd3q19.h:14-40, 111-190
  *   calcRho / calcVel          LBmacroscopic.h:11-45
  *   calcOmegaBGK               LBcollision.h:27-46
+ *   calcOmegaBGKTRT            LBcollision.h:50-75
  *   calcDeltaOmegaQ            LBcollision.h:79-98
  *   calcDeltaOmegaF            LBcollision.h:194-213
+ *   calcDeltaOmegaFTRT         LBcollision.h:214-233
  *   calcDeltaOmegaST / RC      LBcollision2phase.h:7-20, 70-86
  *   grad / vecNorm             LButilities.h:12-22, 62-66
  *   std_case loop              std_case/main.cpp:109-145
@@ -213,9 +215,10 @@ static void bb_apply(const lattice_t *L, const bb_t *b, double *f, int nF, int k
 /* ------------------------------------------------------------ std_case */
 /* nsteps iterations of std_case/main.cpp:110-145.  f, ftmp: LbField (1 field)
  * over n_nodes; rho, vel: ScalarField / VectorField of the last iteration. */
-int orc_std_case(int lattice, int32_t n_nodes, const int32_t *nb, const int16_t *type,
-                 const int32_t *bulk, int32_t n_bulk, double *f, double *ftmp, double tau,
-                 const double *F, int32_t nsteps, double *rho, double *vel, int nthreads) {
+static int std_case_run(int lattice, int32_t n_nodes, const int32_t *nb, const int16_t *type,
+                        const int32_t *bulk, int32_t n_bulk, double *f, double *ftmp, double tau,
+                        int trt, double tauAnti, const double *F, int32_t nsteps, double *rho,
+                        double *vel, int nthreads) {
   const lattice_t L = get_lattice(lattice);
   const int Q = L.nQ, D = L.nD;
   /* fluid boundary nodes of this rank (LBgeometry.h:48-56) */
@@ -227,6 +230,9 @@ int orc_std_case(int lattice, int32_t n_nodes, const int32_t *nb, const int16_t 
   free(fb);
   const double tau_inv = 1.0 / tau;
   const double tau_factor = (1 - 0.5 / tau);
+  /* two-relaxation-time constants (LBcollision.h:62-63, 223-224) */
+  const double tauSym_inv = 1.0 / tau, tauAnti_inv = 1.0 / tauAnti;
+  const double tauSym_factor = (1 - 0.5 / tau), tauAnti_factor = (1 - 0.5 / tauAnti);
   double cF[19];
   cdotall(&L, F, cF);
   double *a = f, *b = ftmp;
@@ -246,8 +252,19 @@ int orc_std_case(int lattice, int32_t n_nodes, const int32_t *nb, const int16_t 
       cdotall(&L, u, cu);
       const double uF = dot(&L, u, F);
       for (int q = 0; q < Q; ++q) {
-        const double om = -tau_inv * (fn[q] - rhoN * L.w[q] * (1.0 + c2Inv * cu[q] + c4Inv0_5 * (cu[q] * cu[q] - c2 * u2)));
-        const double dF = L.w[q] * tau_factor * (c2Inv * cF[q] + c4Inv * (cF[q] * cu[q] - c2 * uF));
+        double om, dF;
+        if (!trt) {
+          om = -tau_inv * (fn[q] - rhoN * L.w[q] * (1.0 + c2Inv * cu[q] + c4Inv0_5 * (cu[q] * cu[q] - c2 * u2)));
+          dF = L.w[q] * tau_factor * (c2Inv * cF[q] + c4Inv * (cF[q] * cu[q] - c2 * uF));
+        } else {
+          /* calcOmegaBGKTRT (LBcollision.h:64-71), calcDeltaOmegaFTRT with
+           * phi = 1 (LBcollision.h:226-229; w*1.0 == w exactly) */
+          const double fqSym = 0.5 * (fn[q] + fn[L.rev[q]]);
+          const double fqAnti = 0.5 * (fn[q] - fn[L.rev[q]]);
+          om = -tauSym_inv * (fqSym - rhoN * L.w[q] * (1.0 + c4Inv0_5 * (cu[q] * cu[q] - c2 * u2)))
+               - tauAnti_inv * (fqAnti - rhoN * L.w[q] * c2Inv * cu[q]);
+          dF = L.w[q] * 1.0 * (tauAnti_factor * c2Inv * cF[q] + tauSym_factor * c4Inv * (cF[q] * cu[q] - c2 * uF));
+        }
         b[(int64_t)nb[(int64_t)node * Q + q] * Q + q] = fn[q] + om + dF;
       }
     }
@@ -257,6 +274,24 @@ int orc_std_case(int lattice, int32_t n_nodes, const int32_t *nb, const int16_t 
   if (a != f) memcpy(f, a, sizeof(double) * (size_t)n_nodes * Q);
   bb_free(&bb);
   return 0;
+}
+
+int orc_std_case(int lattice, int32_t n_nodes, const int32_t *nb, const int16_t *type,
+                 const int32_t *bulk, int32_t n_bulk, double *f, double *ftmp, double tau,
+                 const double *F, int32_t nsteps, double *rho, double *vel, int nthreads) {
+  return std_case_run(lattice, n_nodes, nb, type, bulk, n_bulk, f, ftmp, tau, 0, tau, F, nsteps, rho,
+                      vel, nthreads);
+}
+
+/* std_case with the two-relaxation-time operator: calcOmegaBGK ->
+ * calcOmegaBGKTRT(f, tauSym, tauAnti, ...) and calcDeltaOmegaF ->
+ * calcDeltaOmegaFTRT(tauSym, tauAnti, 1.0, ...), LBcollision.h:50-75, 214-233. */
+int orc_std_case_trt(int lattice, int32_t n_nodes, const int32_t *nb, const int16_t *type,
+                     const int32_t *bulk, int32_t n_bulk, double *f, double *ftmp, double tau_sym,
+                     double tau_anti, const double *F, int32_t nsteps, double *rho, double *vel,
+                     int nthreads) {
+  return std_case_run(lattice, n_nodes, nb, type, bulk, n_bulk, f, ftmp, tau_sym, 1, tau_anti, F, nsteps,
+                      rho, vel, nthreads);
 }
 
 /* ------------------------------------------------------------ twophase */

@@ -7,7 +7,8 @@ container only; /root/reference is not available elsewhere).  For every case:
      reference writer (itself pinned byte-for-byte against files written by
      PythonScripts/vtklb.py, see make_vtklb_fixtures.py),
   2. an input.dat with the case parameters is written,
-  3. the reference case (std_case D2Q9 / D3Q19 or twophase) runs with
+  3. the reference case (std_case D2Q9 / D3Q19, its two-relaxation-time
+     variant, or twophase) runs with
      iterations.write = iterations.max, so it writes VTK at i = 0 and i = max,
   4. the VTU of i = max is parsed: rho (rho0, rho1) and vel on the bulk nodes,
      in bulk order, i.e. the fields of the pre-collision state after max
@@ -138,6 +139,28 @@ def std_cases():
     yield "d3q19_pack_24", "D3Q19", g, "x", [("init_rho", np.ones(g.shape))], 1.0, [1e-5, 0.0, 0.0], 200, False
 
 
+def trt_cases():
+    """Two-relaxation-time std_case (oracle/_ref/std_case_trt_*):
+    (name, lattice, geo, periodic, attrs, (tau_sym, tau_anti), force, max_it)"""
+    g = load_geo("d2q9_random_7x5_pxy")
+    r = np.load(os.path.join(VT, "d2q9_random_7x5_pxy_attr_init_rho.npy"))
+    yield "trt_d2q9_random_7x5", "D2Q9", g, "xy", [("init_rho", r)], (0.8, 1.3), [1e-5, -3e-6], 50
+    g = np.ones((64, 32), dtype=np.int64)
+    g[:, 0] = 0
+    g[:, -1] = 0
+    # (tau_sym - 1/2)(tau_anti - 1/2) = 3/16: the wall sits exactly half-way for Poiseuille flow
+    yield "trt_d2q9_channel_64x32", "D2Q9", g, "x", [("init_rho", np.ones(g.shape))], (1.0, 0.875), [1e-5, 0.0], 500
+    g = load_geo("d3q19_pack_6x5x4_px_walls")
+    yield "trt_d3q19_pack_6x5x4_px_walls", "D3Q19", g, "x", [("init_rho", np.ones(g.shape))], (1.0, 0.6), \
+        [1e-5, 0.0, 0.0], 30
+    g = load_geo("d3q19_pack_6x5x4_pxyz")
+    yield "trt_d3q19_pack_6x5x4_pxyz", "D3Q19", g, "xyz", [("init_rho", np.ones(g.shape))], (0.9, 1.7), \
+        [1e-5, 2e-6, -1e-6], 30
+    g, _ = sphere_geo((24, 24, 24), 4.0, 0.5, 7, "x", True)
+    yield "trt_d3q19_pack_24", "D3Q19", g, "x", [("init_rho", np.ones(g.shape))], (0.7, 1.2), \
+        [1e-5, 0.0, 0.0], 200
+
+
 def tp_cases():
     """(name, geo, periodic, rho0, wett, tau0, tau1, sigma, beta, momx, force, max_it)"""
     rng = np.random.default_rng(99)
@@ -176,6 +199,17 @@ def main(selected):
             out["vel"] = vel
         np.savez_compressed(os.path.join(HERE, "ref_%s.npz" % name), **out)
         print("wrote ref_%s.npz" % name, rho.shape)
+    for name, lat, g, per, attrs, taus, force, max_it in trt_cases():
+        if selected and name not in selected:
+            continue
+        binary = "std_case_trt_d2q9" if lat == "D2Q9" else "std_case_trt_d3q19"
+        dat = input_dat(max_it, ["tau %s" % fmt(list(taus)), "bodyforce %s" % fmt(force)])
+        data = run_case(binary, g, lat, per, attrs, dat)
+        params = dict(case="std_case_trt", lattice=lat, periodic=per, tau=list(taus), force=force, steps=max_it)
+        np.savez_compressed(os.path.join(HERE, "ref_%s.npz" % name), geo=g.astype(np.int8),
+                            params=json.dumps(params), init_rho=attrs[0][1].astype(np.float64),
+                            rho=data["rho"], vel=data["vel"])
+        print("wrote ref_%s.npz" % name, data["rho"].shape)
     for name, g, per, r0, wett, tau0, tau1, sigma, beta, momx, force, max_it in tp_cases():
         if selected and name not in selected:
             continue

@@ -61,6 +61,11 @@ def ref_names(prefix=""):
                   if f.startswith("ref_" + prefix) and f.endswith(".npz"))
 
 
+def std_ref_names():
+    """The BGK std_case fixtures (not two-phase, not two-relaxation-time)."""
+    return [n for n in ref_names() if not n.startswith(("tp_", "trt_"))]
+
+
 def float_round(a):
     """Values as the reference reads them with `ifs >> float` (via the 18-digit text)."""
     return np.array(["%.18e" % v for v in np.ravel(a)], dtype=np.float32).astype(np.float64).reshape(np.shape(a))

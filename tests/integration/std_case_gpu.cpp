@@ -3,7 +3,7 @@
 // the bulk list and the fields with the reference's lbsolver headers, then
 // hands the step loop to the GPU through lbamd::GpuSubdomain.
 //
-//   std_case_gpu <case.vtklb> <tau> <iterations> <precision 0|1|2> <out.bin> <F_x> <F_y> [<F_z>] [<ckpt>]
+//   std_case_gpu <case.vtklb> <tau | tauSym,tauAnti> <iterations> <precision 0|1|2> <out.bin> <F_x> <F_y> [<F_z>] [<ckpt>]
 //
 // Runs iterations + 1 steps (the loop of std_case/main.cpp:109 runs
 // i = 0..max) and writes, for each bulk node in findBulkNodes order, rho and
@@ -14,6 +14,7 @@
 // bulk entries that differ.
 // Built by oracle/build_ref.sh where the reference tree is present.
 #include <cstdio>
+#include <cstring>
 #include <cstdlib>
 #include <fstream>
 #include <string>
@@ -33,7 +34,10 @@ int main(int argc, char** argv) {
     return 2;
   }
   const std::string path = argv[1];
+  // "tau" (BGK) or "tauSym,tauAnti" (the loop with calcOmegaBGKTRT / calcDeltaOmegaFTRT)
   const double tau = std::atof(argv[2]);
+  const char* comma = std::strchr(argv[2], ',');
+  const double tauAnti = comma ? std::atof(comma + 1) : 0.0;
   const int iterations = std::atoi(argv[3]);
   const int precision = std::atoi(argv[4]);
   const std::string out = argv[5];
@@ -57,7 +61,10 @@ int main(int argc, char** argv) {
   try {
     lbamd::GpuSubdomain<LT> gpu(grid, nodes, bulk, precision);
     gpu.upload(f);
-    gpu.stepBGK(iterations + 1, tau, force);
+    if (comma)
+      gpu.stepTRT(iterations + 1, tau, tauAnti, force);
+    else
+      gpu.stepBGK(iterations + 1, tau, force);
     gpu.downloadMacro(rho, vel, bulk);
     if (ckpt) {
       const std::string prefix = argv[6 + LT::nD];

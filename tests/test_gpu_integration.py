@@ -2,8 +2,9 @@
 Grid, Nodes, findBulkNodes, LbField) drive this library via
 lbamd::GpuSubdomain (tests/integration/std_case_gpu.cpp, built by
 oracle/build_ref.sh where the reference tree is present).  Its rho and
-velocity after the std_case loop must equal the compiled reference's
-fixtures bit for bit.  Skipped where the demo was not built."""
+velocity after the std_case loop (BGK, or the two-relaxation-time loop for
+the trt_ fixtures) must equal the compiled reference's fixtures bit for bit.
+Skipped where the demo was not built."""
 import os
 import subprocess
 import tempfile
@@ -11,7 +12,7 @@ import tempfile
 import numpy as np
 import pytest
 
-from helpers import load_ref, ref_names
+from helpers import load_ref, ref_names, std_ref_names
 from oracle.vtklb_oracle import Geometry
 
 pytestmark = pytest.mark.gpu
@@ -25,7 +26,7 @@ def demo(lattice):
     return p if os.access(p, os.X_OK) else None
 
 
-CASES = [n for n in ref_names() if not n.startswith("tp_")]
+CASES = std_ref_names() + ref_names("trt_")
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -45,7 +46,8 @@ def test_shim_drop_in_matches_reference(gpu_available, name):
         out = os.path.join(tmp, "out.bin")
         F = [repr(float(x)) for x in p["force"]]
         env = dict(os.environ, LD_LIBRARY_PATH=os.path.join(REF, "mpilib"))
-        res = subprocess.run([exe, path, repr(float(p["tau"])), str(p["steps"]), "0", out] + F, env=env,
+        tau = ",".join(repr(float(t)) for t in np.atleast_1d(p["tau"]))  # "tauSym,tauAnti" for TRT
+        res = subprocess.run([exe, path, tau, str(p["steps"]), "0", out] + F, env=env,
                              capture_output=True, text=True, timeout=120)
         if res.returncode != 0:
             # the shim refuses fluid links to the ghost node 0 unless asked

@@ -10,7 +10,7 @@ import pytest
 
 import lbamd
 import oracle
-from helpers import OracleTopo, float_round, load_ref, ref_names
+from helpers import OracleTopo, float_round, load_ref, ref_names, std_ref_names
 
 pytestmark = pytest.mark.gpu
 
@@ -45,7 +45,7 @@ def rest(T, rho_nodes):
 
 
 @pytest.mark.parametrize("layout", ["table", "rows"])
-@pytest.mark.parametrize("name", [n for n in ref_names() if not n.startswith("tp_")])
+@pytest.mark.parametrize("name", std_ref_names())
 def test_std_case_vs_reference_fixture(gpu_available, name, layout):
     d = load_ref(name)
     p = d["params"]

@@ -1,0 +1,190 @@
+"""Two-relaxation-time operator on the device (lb_step_trt,
+lb_solve_permeability_trt) against the compiled reference's TRT std_case and
+the oracle.
+
+Bar: FP64 bit-exact (array_equal) for populations, rho and vel; the
+permeability sum 1e-12 relative.  Reduced precision as for BGK, norm-wise:
+FP32S velocity 1e-4, FP16S 2e-2.
+"""
+import numpy as np
+import pytest
+
+import lbamd
+import oracle
+from helpers import OracleTopo, load_ref, ref_names
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_trt(T, f0, taus, force, chunks, precision=lbamd.FP64, layout="table", allow_ghost0=False):
+    ctx = lbamd.Context(T.lattice, precision)
+    if layout == "rows":
+        ctx.set_topology_grid(T.neighbors, T.node_types, T.bulk, T.positions, allow_ghost0=allow_ghost0)
+        assert ctx.layout != 0 or allow_ghost0
+    else:
+        ctx.set_topology(T.neighbors, T.node_types, T.bulk, allow_ghost0=allow_ghost0)
+    ctx.upload_f(f0)
+    for n in chunks:
+        ctx.step_trt(n, taus[0], taus[1], np.asarray(force, dtype=float))
+    out = ctx.download_f(1)[:, 0, :], ctx.download_scalar(lbamd.RHO), ctx.download_vel()
+    ctx.close()
+    return out
+
+
+@pytest.mark.parametrize("layout", ["table", "rows"])
+@pytest.mark.parametrize("name", ref_names("trt_"))
+def test_trt_vs_reference_fixture(gpu_available, name, layout):
+    d = load_ref(name)
+    p = d["params"]
+    T = OracleTopo(d["geo"].astype(np.int64), p["lattice"], p["periodic"])
+    f0 = oracle.rest_state(T.lattice, T.n_nodes, T.bulk, T.attribute(d["init_rho"]))
+    ghost0 = bool((T.neighbors[T.bulk] == 0).any())
+    f, rho, vel = gpu_trt(T, f0, p["tau"], p["force"], [p["steps"] + 1], layout=layout, allow_ghost0=ghost0)
+    D = vel.shape[1]
+    assert np.array_equal(rho[T.bulk], d["rho"])
+    assert np.array_equal(vel[T.bulk], d["vel"][:, :D])
+
+
+def random_case(lattice, shape, periodic, seed):
+    rng = np.random.default_rng(seed)
+    geo = (rng.random(shape) > 0.3).astype(np.int64)
+    if "y" not in periodic:
+        geo[:, 0] = 0
+        geo[:, -1] = 0
+    if len(shape) == 3 and "z" not in periodic:
+        geo[:, :, 0] = 0
+        geo[:, :, -1] = 0
+    T = OracleTopo(geo, lattice, periodic)
+    f0 = oracle.rest_state(T.lattice, T.n_nodes, T.bulk, 1.0 + 0.01 * rng.random(T.n_nodes))
+    f0[T.bulk] += 1e-4 * rng.standard_normal((T.bulk.size, f0.shape[1]))
+    return T, f0, [2e-5, -1e-5, 5e-6][: len(shape)]
+
+
+@pytest.mark.parametrize("layout", ["table", "rows"])
+@pytest.mark.parametrize("lattice,shape,periodic,seed,taus", [
+    ("D2Q9", (40, 23), "xy", 1, (0.83, 1.4)),
+    ("D2Q9", (33, 17), "x", 2, (1.0, 0.875)),
+    ("D3Q19", (17, 12, 9), "xyz", 3, (0.6, 2.375)),
+    ("D3Q19", (20, 16, 13), "x", 4, (1.3, 0.7)),
+])
+def test_trt_vs_oracle_random(gpu_available, lattice, shape, periodic, seed, taus, layout):
+    T, f0, force = random_case(lattice, shape, periodic, seed)
+    nsteps = 37
+    fo, ro, vo = oracle.run_std_case(T.lattice, T.neighbors, T.node_types, T.bulk, f0, taus, np.array(force), nsteps)
+    fg, rg, vg = gpu_trt(T, f0, taus, force, [1, 10, 26], layout=layout)
+    b = T.bulk
+    assert np.array_equal(fg[b], fo[b])
+    assert np.array_equal(rg[b], ro[b])
+    assert np.array_equal(vg[b], vo[b])
+
+
+def pack(shape, radius, porosity, seed, periodic="x"):
+    geo, _ = lbamd.sphere_pack(shape, radius, porosity, seed, periodic=periodic, walls=True)
+    return geo.astype(np.int64), OracleTopo(geo.astype(np.int64), "D3Q19", periodic)
+
+
+@pytest.mark.parametrize("precision,vtol", [(lbamd.FP32S, 1e-4), (lbamd.FP16S, 2e-2)])
+def test_trt_reduced_precision_vs_oracle(gpu_available, precision, vtol):
+    geo, T = pack((28, 24, 20), 4.0, 0.5, 21)
+    f0 = oracle.rest_state(1, T.n_nodes, T.bulk, np.ones(T.n_nodes))
+    force = np.array([2e-5, 0.0, 0.0])
+    taus = (0.9, 0.5 + 0.1875 / 0.4)
+    fo, ro, vo = oracle.run_std_case(1, T.neighbors, T.node_types, T.bulk, f0, taus, force, 300)
+    f, r, v = gpu_trt(T, f0, taus, force, [300], precision=precision, layout="rows")
+    b = T.bulk
+    assert np.abs(v[b] - vo[b]).max() / np.abs(vo[b]).max() < vtol
+    if precision == lbamd.FP32S:
+        assert np.abs(r[b] - ro[b]).max() < max(1e-3 * np.abs(ro[b] - 1).max(), 1e-9)
+
+
+@pytest.mark.parametrize("precision", [lbamd.FP64, lbamd.FP32S, lbamd.FP16S])
+def test_trt_layouts_and_chunks_identical(gpu_available, precision):
+    geo, T = pack((24, 20, 18), 3.0, 0.5, 5)
+    rng = np.random.default_rng(2)
+    f0 = oracle.rest_state(1, T.n_nodes, T.bulk, np.ones(T.n_nodes))
+    f0[T.bulk] += 1e-4 * rng.standard_normal((T.bulk.size, 19))
+    taus, force = (0.8, 1.15), [1e-5, 2e-6, 0.0]
+    a = gpu_trt(T, f0, taus, force, [40], precision=precision, layout="table")
+    b = gpu_trt(T, f0, taus, force, [13, 1, 26], precision=precision, layout="rows")
+    for x, y in zip(a, b):
+        assert np.array_equal(x[T.bulk], y[T.bulk])
+
+
+def test_trt_equal_times_close_to_bgk(gpu_available):
+    """tau_sym = tau_anti is BGK up to rounding (the FP64 operators differ
+    only in the order of their terms)."""
+    T, f0, force = random_case("D3Q19", (17, 12, 9), "x", 7)
+    ft, rt, vt = gpu_trt(T, f0, (0.9, 0.9), force, [30])
+    ctx = lbamd.Context(T.lattice, lbamd.FP64)
+    ctx.set_topology(T.neighbors, T.node_types, T.bulk)
+    ctx.upload_f(f0)
+    ctx.step_bgk(30, 0.9, np.asarray(force, dtype=float))
+    fb = ctx.download_f(1)[:, 0, :]
+    ctx.close()
+    assert np.abs(ft[T.bulk] - fb[T.bulk]).max() <= 1e-14
+
+
+def test_trt_permeability_matches_oracle_flux(gpu_available):
+    geo, T = pack((20, 16, 16), 3.0, 0.5, 4)
+    f0 = oracle.rest_state(1, T.n_nodes, T.bulk, np.ones(T.n_nodes))
+    F = np.array([1e-6, 0.0, 0.0])
+    taus = (0.8, 0.5 + 0.1875 / 0.3)
+    ctx = lbamd.Context(lbamd.D3Q19, lbamd.FP64)
+    ctx.set_topology_grid(T.neighbors, T.node_types, T.bulk, T.positions)
+    ctx.upload_f(f0)
+    k, steps = ctx.solve_permeability_trt(taus[0], taus[1], F, 50, 1e-30, 200, float(geo.size))
+    ctx.close()
+    assert steps == 200
+    fo, ro, vo = oracle.run_std_case(1, T.neighbors, T.node_types, T.bulk, f0, taus, F, 200)
+    jx = (ro[T.bulk] * vo[T.bulk, 0]).sum()
+    k_ref = ((taus[0] - 0.5) / 3.0) * jx / geo.size / F[0]
+    assert abs(k - k_ref) <= 1e-12 * abs(k_ref)
+
+
+def test_trt_permeability_independent_of_viscosity(gpu_available):
+    """With (tau_sym - 1/2)(tau_anti - 1/2) = 3/16 held, TRT's k barely moves
+    with the viscosity while BGK's moves by tens of percent (its half-way walls
+    move with tau).  On D3Q19 packs a residual of a few 1e-4 remains between
+    tau_sym = 0.8 and 1.4 (the FP64 oracle gives 6.8e-4 on a 16^3 periodic
+    pack at steady state; on a D2Q9 cylinder array and in a channel it is
+    below 1e-7), so the bar is 2e-3 and a factor 50 under BGK."""
+    geo, T = pack((32, 24, 24), 4.0, 0.5, 17)
+    f0 = oracle.rest_state(1, T.n_nodes, T.bulk, np.ones(T.n_nodes))
+    F = np.array([1e-6, 0.0, 0.0])
+    ks = {}
+    for kind, ts in [("trt", 0.8), ("trt", 1.4), ("bgk", 0.8), ("bgk", 1.4)]:
+        ctx = lbamd.Context(lbamd.D3Q19, lbamd.FP64)
+        ctx.set_topology_grid(T.neighbors, T.node_types, T.bulk, T.positions)
+        ctx.upload_f(f0)
+        if kind == "trt":
+            ks[kind, ts] = ctx.solve_permeability_trt(ts, 0.5 + 0.1875 / (ts - 0.5), F, 100, 1e-9, 100000,
+                                                      float(geo.size))[0]
+        else:
+            ks[kind, ts] = ctx.solve_permeability(ts, F, 100, 1e-9, 100000, float(geo.size))[0]
+        ctx.close()
+    trt = abs(ks["trt", 0.8] - ks["trt", 1.4]) / ks["trt", 0.8]
+    bgk = abs(ks["bgk", 0.8] - ks["bgk", 1.4]) / ks["bgk", 0.8]
+    assert trt < 2e-3, ks
+    assert bgk > 50 * trt, ks
+
+
+def test_trt_rejects_bad_arguments(gpu_available):
+    T, f0, force = random_case("D2Q9", (12, 9), "x", 9)
+    ctx = lbamd.Context(T.lattice, lbamd.FP64)
+    ctx.set_topology(T.neighbors, T.node_types, T.bulk)
+    ctx.upload_f(f0)
+    for taus in [(0.5, 1.0), (1.0, 0.5), (1.0, -1.0)]:
+        with pytest.raises(lbamd.LBError) as e:
+            ctx.step_trt(1, taus[0], taus[1], np.zeros(2))
+        assert e.value.code == -1  # LB_ERR_ARG
+    with pytest.raises(lbamd.LBError):
+        ctx.solve_permeability_trt(1.0, 0.5, np.array([1e-6, 0.0]), 10, 1e-6, 100, 1.0)
+    ctx.close()
+    ctx = lbamd.Context(lbamd.D3Q19, lbamd.FP64)
+    T2 = OracleTopo(np.ones((4, 4, 4), dtype=np.int64), "D3Q19", "xyz")
+    ctx.set_topology(T2.neighbors, T2.node_types, T2.bulk)
+    ctx.upload_f(np.zeros((T2.n_nodes, 2, 19)))
+    with pytest.raises(lbamd.LBError) as e:  # two-field context: the single-phase step is refused
+        ctx.step_trt(1, 1.0, 1.0, np.zeros(3))
+    assert e.value.code == -2  # LB_ERR_STATE
+    ctx.close()

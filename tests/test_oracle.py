@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import OracleTopo, float_round, load_ref, ref_names
+from helpers import OracleTopo, float_round, load_ref, ref_names, std_ref_names
 
 
 def ref_std_run(name):
@@ -25,7 +25,7 @@ def ref_std_run(name):
     return d, T, f, rho, vel
 
 
-@pytest.mark.parametrize("name", [n for n in ref_names() if not n.startswith("tp_")])
+@pytest.mark.parametrize("name", std_ref_names())
 def test_std_case_bitexact_vs_compiled_reference(name):
     d, T, f, rho, vel = ref_std_run(name)
     D = vel.shape[1]
@@ -41,6 +41,59 @@ def test_std_case_bitexact_vs_compiled_reference(name):
         # x-invariance (bit-identical columns), as in the reference run
         assert np.ptp(r, axis=0).max() == d["x_spread"][0] == 0.0
         assert np.ptp(u, axis=0).max() == d["x_spread"][1] == 0.0
+
+
+@pytest.mark.parametrize("name", ref_names("trt_"))
+def test_trt_bitexact_vs_compiled_reference(name):
+    """std_case with calcOmegaBGKTRT / calcDeltaOmegaFTRT (phi = 1), built from
+    the reference's own main.cpp by oracle/build_ref.sh."""
+    d, T, f, rho, vel = ref_std_run(name)
+    assert len(d["params"]["tau"]) == 2
+    D = vel.shape[1]
+    assert np.array_equal(rho[T.bulk], d["rho"])
+    assert np.array_equal(vel[T.bulk], d["vel"][:, :D])
+
+
+@pytest.mark.parametrize("lattice,shape,periodic", [("D2Q9", (9, 7), "xy"), ("D3Q19", (6, 5, 7), "x")])
+def test_trt_with_equal_times_is_bgk(lattice, shape, periodic):
+    """tau_sym = tau_anti: the even and odd relaxations recombine into BGK and
+    both Guo terms into calcDeltaOmegaF (equal up to rounding)."""
+    rng = np.random.default_rng(8)
+    geo = (rng.random(shape) > 0.25).astype(np.int64)
+    T = OracleTopo(geo, lattice, periodic)
+    f0 = oracle.rest_state(T.lattice, T.n_nodes, T.bulk, 1.0 + 0.01 * rng.random(T.n_nodes))
+    F = np.array([2e-5, -1e-5, 5e-6])
+    fb, rb, vb = oracle.run_std_case(T.lattice, T.neighbors, T.node_types, T.bulk, f0, 0.7, F, 40)
+    ft, rt, vt = oracle.run_std_case(T.lattice, T.neighbors, T.node_types, T.bulk, f0, (0.7, 0.7), F, 40)
+    b = T.bulk
+    # populations are O(w); j = sum_q f c cancels to O(1e-5), so both carry
+    # absolute rounding of a few 1e-16
+    assert np.abs(ft[b] - fb[b]).max() <= 1e-14
+    assert np.abs(vt[b] - vb[b]).max() <= 1e-14
+    # and a distinct tau_anti changes the flow
+    fa, ra, va = oracle.run_std_case(T.lattice, T.neighbors, T.node_types, T.bulk, f0, (0.7, 1.1), F, 40)
+    assert np.abs(va[b] - vb[b]).max() > 1e-3 * np.abs(vb[b]).max()
+
+
+@pytest.mark.parametrize("tau_sym,tau_anti", [(1.0, 0.875), (0.875, 1.0), (0.6, 0.5 + 0.1875 / 0.1)])
+def test_trt_magic_parameter_gives_exact_poiseuille(tau_sym, tau_anti):
+    """(tau_sym - 1/2)(tau_anti - 1/2) = 3/16: the steady channel flow with
+    half-way bounce-back is the parabola u = F/(2 nu) (y+1/2)(H-1/2-y) with
+    nu = (tau_sym - 1/2)/3, walls exactly half-way (BGK at tau = 1 misses it
+    by 5e-3).  A swap of the two times changes nu and fails this."""
+    H = 8
+    geo = np.ones((4, H + 2), dtype=np.int64)
+    geo[:, 0] = geo[:, -1] = 0
+    T = OracleTopo(geo, "D2Q9", "x")
+    f0 = oracle.rest_state(0, T.n_nodes, T.bulk, np.ones(T.n_nodes))
+    F = 1e-5
+    f, rho, vel = oracle.run_std_case(0, T.neighbors, T.node_types, T.bulk, f0, (tau_sym, tau_anti),
+                                      np.array([F, 0.0]), 20000)
+    y = T.positions[T.bulk, 1] - 1
+    nu = (tau_sym - 0.5) / 3
+    exact = F / (2 * nu) * (y + 0.5) * (H - 0.5 - y)
+    assert np.abs(vel[T.bulk, 0] - exact).max() <= 1e-10 * exact.max()
+    assert np.abs(vel[T.bulk, 1]).max() <= 1e-12 * exact.max()
 
 
 def tp_start(d, T):
