@@ -32,6 +32,7 @@ SOURCES = {
     "capi.hip": HIP + ["-O2", "-ffp-contract=off"],
     "batch.hip": HIP + ["-O2", "-ffp-contract=off"],
     "calibrate.hip": HIP + ["-O3"],
+    "topo_device.hip": HIP + ["-O3"],
     "topology.cpp": ["-O2", "-ffp-contract=off", "-x", "c++"],
 }
 

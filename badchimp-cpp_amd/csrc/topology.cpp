@@ -481,6 +481,129 @@ int lb_topo_from_geometry(int lattice, int32_t nd, const int32_t* dims, const in
   return LB_OK;
 }
 
+// ---- binary twin of .vtklb ------------------------------------------------
+// Layout (little-endian): "LBVTKB01", int32 {lattice, nD, Q, rank, zero_ghost,
+// n_nodes}, int32 global_dims[nD], int32 pos[n_nodes*nD], int32
+// neighbor[n_nodes*Q], int32 nodetype_attr[n_nodes], int32 node_rank[n_nodes],
+// int32 n_blocks, per block {int32 rank, int32 count, int32 pairs[2*count]},
+// int32 n_attr, per attribute {int32 name length, name, int32 type length,
+// type, int64 count, double values[count], float values[count]}.
+int lb_topo_write_binary(const lb_topo* t, const char* path) {
+  if (!t || !path) return fail_global(LB_ERR_ARG, "null argument");
+  std::unique_ptr<FILE, int (*)(FILE*)> fh(std::fopen(path, "wb"), std::fclose);
+  if (!fh) return fail_global(LB_ERR_IO, "could not open %s for writing", path);
+  FILE* f = fh.get();
+  auto put = [&](const void* p, size_t bytes) { return bytes == 0 || std::fwrite(p, 1, bytes, f) == bytes; };
+  auto put32 = [&](int32_t v) { return put(&v, 4); };
+  bool ok = put("LBVTKB01", 8);
+  const int32_t head[6] = {t->lattice, t->nD, t->Q, t->rank, t->zero_ghost ? 1 : 0, t->n_nodes};
+  ok = ok && put(head, sizeof(head));
+  ok = ok && put(t->global_dims.data(), 4 * t->global_dims.size());
+  ok = ok && put(t->pos.data(), 4 * t->pos.size());
+  ok = ok && put(t->neighbor.data(), 4 * t->neighbor.size());
+  ok = ok && put(t->nodetype_attr.data(), 4 * t->nodetype_attr.size());
+  ok = ok && put(t->node_rank.data(), 4 * t->node_rank.size());
+  ok = ok && put32(static_cast<int32_t>(t->processors.size()));
+  for (auto& blk : t->processors) {
+    ok = ok && put32(blk.first) && put32(static_cast<int32_t>(blk.second.size()));
+    for (auto& pr : blk.second) ok = ok && put32(pr.first) && put32(pr.second);
+  }
+  ok = ok && put32(static_cast<int32_t>(t->attributes.size() + t->attr_values.size()));
+  auto put_attr = [&](const std::string& name, const std::string& type, const std::vector<double>& dv,
+                      const std::vector<float>& fv) {
+    const int64_t n = static_cast<int64_t>(dv.size());
+    return put32(static_cast<int32_t>(name.size())) && put(name.data(), name.size()) &&
+           put32(static_cast<int32_t>(type.size())) && put(type.data(), type.size()) && put(&n, 8) &&
+           put(dv.data(), 8 * dv.size()) && put(fv.data(), 4 * fv.size());
+  };
+  for (auto& a : t->attributes) {
+    const auto& toks = a.second.second;
+    std::vector<double> dv(toks.size());
+    std::vector<float> fv(toks.size());
+    for (size_t i = 0; i < toks.size(); ++i) {
+      // `ifs >> double` and `ifs >> float` each round the decimal text once
+      dv[i] = std::strtod(toks[i].c_str(), nullptr);
+      fv[i] = std::strtof(toks[i].c_str(), nullptr);
+    }
+    ok = ok && put_attr(a.first, a.second.first, dv, fv);
+  }
+  for (auto& a : t->attr_values) ok = ok && put_attr(a.first, a.second.type, a.second.d, a.second.f);
+  ok = ok && std::fflush(f) == 0;
+  return ok ? LB_OK : fail_global(LB_ERR_IO, "write error on %s", path);
+}
+
+int lb_topo_read_binary(const char* path, int lattice, lb_topo** out) {
+  if (!path || !out || (lattice != LB_D2Q9 && lattice != LB_D3Q19))
+    return fail_global(LB_ERR_ARG, "lb_topo_read_binary: bad argument");
+  std::unique_ptr<FILE, int (*)(FILE*)> fh(std::fopen(path, "rb"), std::fclose);
+  if (!fh) return fail_global(LB_ERR_IO, "could not open file %s", path);
+  FILE* f = fh.get();
+  auto get = [&](void* p, size_t bytes) { return bytes == 0 || std::fread(p, 1, bytes, f) == bytes; };
+  auto bad = [&](const char* what) { return fail_global(LB_ERR_IO, "%s: %s", path, what); };
+  char magic[8];
+  int32_t head[6];
+  if (!get(magic, 8) || std::memcmp(magic, "LBVTKB01", 8) != 0) return bad("not a binary vtklb file");
+  if (!get(head, sizeof(head))) return bad("truncated header");
+  const int Q = lattice == LB_D2Q9 ? 9 : 19, nD = lattice == LB_D2Q9 ? 2 : 3;
+  if (head[0] != lattice || head[1] != nD || head[2] != Q)
+    return fail_global(LB_ERR_ARG, "%s holds lattice %d (nD %d, Q %d), %d was asked for", path, head[0], head[1],
+                       head[2], lattice);
+  if (head[5] < 1) return bad("bad node count");
+  try {
+    auto t = std::make_unique<lb_topo>();
+    t->lattice = lattice;
+    t->nD = nD;
+    t->Q = Q;
+    t->rank = head[3];
+    t->zero_ghost = head[4] != 0;
+    t->n_nodes = head[5];
+    const size_t N = static_cast<size_t>(t->n_nodes);
+    t->global_dims.resize(nD);
+    t->pos.resize(N * nD);
+    t->neighbor.resize(N * Q);
+    t->nodetype_attr.resize(N);
+    t->node_rank.resize(N);
+    if (!get(t->global_dims.data(), 4 * nD) || !get(t->pos.data(), 4 * t->pos.size()) ||
+        !get(t->neighbor.data(), 4 * t->neighbor.size()) || !get(t->nodetype_attr.data(), 4 * N) ||
+        !get(t->node_rank.data(), 4 * N))
+      return bad("truncated arrays");
+    for (int32_t v : t->neighbor)
+      if (v < 0 || v >= t->n_nodes) return bad("neighbour index out of range");
+    int32_t nblk = 0;
+    if (!get(&nblk, 4) || nblk < 0) return bad("bad PROCESSOR block count");
+    for (int32_t b = 0; b < nblk; ++b) {
+      int32_t r = 0, cnt = 0;
+      if (!get(&r, 4) || !get(&cnt, 4) || cnt < 0) return bad("bad PROCESSOR block");
+      std::vector<std::pair<int32_t, int32_t>> pairs(cnt);
+      for (auto& pr : pairs)
+        if (!get(&pr.first, 4) || !get(&pr.second, 4)) return bad("truncated PROCESSOR block");
+      t->processors.emplace_back(r, std::move(pairs));
+    }
+    int32_t nattr = 0;
+    if (!get(&nattr, 4) || nattr < 0) return bad("bad attribute count");
+    for (int32_t a = 0; a < nattr; ++a) {
+      int32_t ln = 0, lt = 0;
+      int64_t n = 0;
+      std::string name, type;
+      if (!get(&ln, 4) || ln < 0 || ln > 4096) return bad("bad attribute name");
+      name.resize(ln);
+      if (!get(&name[0], ln) || !get(&lt, 4) || lt < 0 || lt > 4096) return bad("bad attribute type");
+      type.resize(lt);
+      if (!get(&type[0], lt) || !get(&n, 8) || n < 0 || n > t->n_nodes) return bad("bad attribute size");
+      std::vector<double> dv(n);
+      std::vector<float> fv(n);
+      if (!get(dv.data(), 8 * n) || !get(fv.data(), 4 * n)) return bad("truncated attribute");
+      t->attr_values.emplace(name, lb_topo::AttrValues{type, std::move(dv), std::move(fv)});
+    }
+    // node types and bulk list exactly as after the text reader
+    lbamd::finish_topology(*t);
+    *out = t.release();
+  } catch (const std::bad_alloc&) {
+    return fail_global(LB_ERR_NOMEM, "lb_topo_read_binary: out of host memory");
+  }
+  return LB_OK;
+}
+
 int lb_topo_destroy(lb_topo* t) {
   delete t;
   return LB_OK;
@@ -523,11 +646,19 @@ int lb_topo_bulk(const lb_topo* t, int32_t* out) {
 
 int lb_topo_attribute(const lb_topo* t, const char* name, int32_t as_float, double* out) {
   if (!t || !name || !out) return fail_global(LB_ERR_ARG, "null argument");
-  auto it = t->attributes.find(name);
-  if (it == t->attributes.end())
-    return fail_global(LB_ERR_IO, "Data attribute with name %s do not exist.", name);
-  const auto& vals = it->second.second;
   const int begin = t->zero_ghost ? 1 : 0;
+  auto it = t->attributes.find(name);
+  if (it == t->attributes.end()) {
+    auto jt = t->attr_values.find(name);
+    if (jt == t->attr_values.end())
+      return fail_global(LB_ERR_IO, "Data attribute with name %s do not exist.", name);
+    if (begin) out[0] = 0.0;
+    const size_t n = jt->second.d.size();
+    for (size_t i = 0; i < n && static_cast<int64_t>(i) + begin < t->n_nodes; ++i)
+      out[i + begin] = as_float ? static_cast<double>(jt->second.f[i]) : jt->second.d[i];
+    return LB_OK;
+  }
+  const auto& vals = it->second.second;
   if (begin) out[0] = 0.0;
   for (size_t i = 0; i < vals.size() && static_cast<int64_t>(i) + begin < t->n_nodes; ++i) {
     // `ifs >> float` / `ifs >> double` both round the decimal text directly

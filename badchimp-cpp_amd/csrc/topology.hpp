@@ -24,6 +24,14 @@ struct lb_topo {
   std::vector<std::pair<int, std::vector<std::pair<int32_t, int32_t>>>> processors;
   // SCALARS blocks: name -> (type, raw tokens, one per point in file order)
   std::map<std::string, std::pair<std::string, std::vector<std::string>>> attributes;
+  // SCALARS blocks read from the binary twin: name -> type and the values
+  // read as double and as float, one per point in file order
+  struct AttrValues {
+    std::string type;
+    std::vector<double> d;
+    std::vector<float> f;
+  };
+  std::map<std::string, AttrValues> attr_values;
 };
 
 namespace lbamd {

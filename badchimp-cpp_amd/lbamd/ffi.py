@@ -79,6 +79,10 @@ SIGNATURES = {
     "lb_topo_read_vtklb": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(_vp)]),
     "lb_topo_from_geometry": (C.c_int, [C.c_int, C.c_int32, _i32p, _i32p, C.c_int32, C.c_int32,
                                         C.POINTER(_vp)]),
+    "lb_topo_from_geometry_device": (C.c_int, [C.c_int, C.c_int, C.c_int32, _i32p, _i32p, C.c_int32,
+                                               C.POINTER(_vp)]),
+    "lb_topo_write_binary": (C.c_int, [_vp, C.c_char_p]),
+    "lb_topo_read_binary": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(_vp)]),
     "lb_topo_destroy": (C.c_int, [_vp]),
     "lb_topo_num_nodes": (C.c_int32, [_vp]),
     "lb_topo_num_bulk": (C.c_int32, [_vp]),
@@ -200,6 +204,26 @@ class Topology:
         _check_global(lib().lb_topo_from_geometry(lattice, geo.ndim, dims, geo.ravel(),
                                                   periodic_mask(periodic), rank, C.byref(h)))
         return cls(h, lattice)
+
+    @classmethod
+    def from_geometry_device(cls, geo, lattice, periodic="", device=0):
+        """The node list of from_geometry (rank 1), built on the GPU."""
+        geo = np.ascontiguousarray(geo, dtype=np.int32)
+        dims = np.ascontiguousarray(geo.shape, dtype=np.int32)
+        h = _vp()
+        _check_global(lib().lb_topo_from_geometry_device(device, lattice, geo.ndim, dims, geo.ravel(),
+                                                         periodic_mask(periodic), C.byref(h)))
+        return cls(h, lattice)
+
+    @classmethod
+    def from_binary(cls, path, lattice):
+        """Read the binary twin of a .vtklb file (write_binary)."""
+        h = _vp()
+        _check_global(lib().lb_topo_read_binary(os.fsencode(path), lattice, C.byref(h)))
+        return cls(h, lattice)
+
+    def write_binary(self, path):
+        _check_global(lib().lb_topo_write_binary(self._h, os.fsencode(path)))
 
     def close(self):
         if self._h:

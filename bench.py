@@ -86,8 +86,56 @@ def profiled_traffic(form):
 
 def make_workload(size, seed=1):
     geo, info = lbamd.sphere_pack((size, size, size), 8.0, 0.40, seed, periodic="x", walls=True)
+    t0 = time.perf_counter()
     topo = lbamd.Topology.from_geometry(geo, lbamd.D3Q19, "x")
+    info["host_builder_s"] = time.perf_counter() - t0
     return geo, info, topo
+
+
+def setup_leg(geo, info, topo, text_size=128):
+    """Geometry -> node list on the GPU against the host builder (same
+    arrays), and the binary .vtklb twin against the text format."""
+    import tempfile
+    lbamd.Topology.from_geometry_device(np.ones((8, 8, 8), np.int32), lbamd.D3Q19, "x").close()  # HIP init
+    t0 = time.perf_counter()
+    dev = lbamd.Topology.from_geometry_device(geo, lbamd.D3Q19, "x")
+    t_dev = time.perf_counter() - t0
+    same = all(np.array_equal(getattr(topo, m)(), getattr(dev, m)())
+               for m in ("neighbors", "positions", "node_types", "bulk"))
+    dev.close()
+    out = {"nodes": int(topo.n_nodes), "host_builder_s": info.get("host_builder_s"), "device_builder_s": t_dev,
+           "device_equals_host": bool(same)}
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "t.vtklbb")
+        t0 = time.perf_counter()
+        topo.write_binary(path)
+        out["binary_write_s"] = time.perf_counter() - t0
+        out["binary_bytes"] = os.path.getsize(path)
+        t0 = time.perf_counter()
+        b = lbamd.Topology.from_binary(path, lbamd.D3Q19)
+        out["binary_read_s"] = time.perf_counter() - t0
+        b.close()
+        os.remove(path)
+        # the text format on a smaller pack (its reader is ~100x slower)
+        g2, _ = lbamd.sphere_pack((text_size,) * 3, 8.0, 0.40, 1, periodic="x", walls=True)
+        t2 = lbamd.Topology.from_geometry(g2, lbamd.D3Q19, "x")
+        tp, bp = os.path.join(tmp, "t.vtklb"), os.path.join(tmp, "t2.vtklbb")
+        t2.write_vtklb(tp)
+        t2.write_binary(bp)
+        t0 = time.perf_counter()
+        a = lbamd.Topology.from_vtklb(tp, lbamd.D3Q19)
+        t_text = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        b = lbamd.Topology.from_binary(bp, lbamd.D3Q19)
+        t_bin = time.perf_counter() - t0
+        out.update({"text_size": text_size, "text_nodes": int(t2.n_nodes), "text_read_s": t_text,
+                    "text_bytes": os.path.getsize(tp), "binary_read_same_case_s": t_bin,
+                    "text_equals_binary": bool(all(np.array_equal(getattr(a, m)(), getattr(b, m)())
+                                                   for m in ("neighbors", "positions", "node_types", "bulk")))})
+    log("set-up: host builder %.2f s, device builder %.3f s (same: %s); binary twin write %.2f s, read %.2f s; "
+        "text %d^3 read %.2f s vs binary %.3f s" % (out["host_builder_s"], t_dev, same, out["binary_write_s"],
+                                                    out["binary_read_s"], text_size, t_text, t_bin))
+    return out
 
 
 def rest_state(n_nodes, bulk):
@@ -541,6 +589,7 @@ def main():
     ap.add_argument("--serve", type=int, default=48, help="requests per serving run (0: skip)")
     ap.add_argument("--no-channel", action="store_true")
     ap.add_argument("--no-trt", action="store_true")
+    ap.add_argument("--no-setup", action="store_true")
     ap.add_argument("--tp-size", type=int, default=192)
     ap.add_argument("--tp-steps", type=int, default=50000, help="two-phase steps (0: skip)")
     ap.add_argument("--serve-pool-gb", type=float, default=16.0)
@@ -594,6 +643,7 @@ def main():
                 "k_rel_err_fp16s": abs(k16 - k64) / abs(k64), "tol_dk_over_k": 1e-7,
                 "definition": "k = nu * sum_bulk(sum_q f c_x + F_x/2) / (N_box * F_x), nu = (tau-1/2)/3"}
 
+    setup = None if args.no_setup else setup_leg(geo, info, topo)
     trt = None if args.no_trt else trt_leg(args, geo, topo, nb, types, bulk, pos, force, n_bulk,
                                             perm["k_fp64"] if perm else None)
     batched = batched_leg(args.batch, args.baseline_every, tau, force) if args.batch > 0 else None
@@ -638,6 +688,7 @@ def main():
                  "achieved_GBs": round(ach64, 1), "frac": round(ach64 / HBM_PEAK_GBS, 4),
                  "bytes_per_lup": bytes64, "traffic": traffic64},
         "permeability": perm,
+        "setup": setup,
         "trt": trt,
         "batched": batched,
         "channel": channel,

@@ -231,6 +231,22 @@ int lb_topo_read_vtklb(const char* path, int lattice, lb_topo** out);
 int lb_topo_from_geometry(int lattice, int32_t nd, const int32_t* dims,
                           const int32_t* geo, int32_t periodic, int32_t rank,
                           lb_topo** out);
+/* The same node list, neighbour index, positions, node types and bulk list
+ * as lb_topo_from_geometry for rank 1, built on GPU `device` (padding and
+ * rim copies, fluid/wall flags, a tiled scan for the numbering, neighbour
+ * index and node types in kernels).  Single-rank geometries only: values 0
+ * (solid) and 1 (fluid); other positive values give LB_ERR_UNSUPPORTED. */
+int lb_topo_from_geometry_device(int device, int lattice, int32_t nd, const int32_t* dims,
+                                 const int32_t* geo, int32_t periodic, lb_topo** out);
+/* Binary twin of a .vtklb file (no counterpart in the reference: a format of
+ * this library).  It holds what the text file holds -- global dimensions,
+ * positions, neighbour index, nodetype attribute, node ranks, PROCESSOR
+ * blocks and the SCALARS attributes (each as double and as float, the two
+ * ways the reference reads them) -- as raw little-endian arrays.  Reading it
+ * runs the same node typing and bulk list as lb_topo_read_vtklb, so both give
+ * the same topology. */
+int lb_topo_write_binary(const lb_topo* t, const char* path);
+int lb_topo_read_binary(const char* path, int lattice, lb_topo** out);
 int lb_topo_destroy(lb_topo* t);
 int32_t lb_topo_num_nodes(const lb_topo* t);  /* incl. node 0 */
 int32_t lb_topo_num_bulk(const lb_topo* t);

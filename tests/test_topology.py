@@ -193,3 +193,51 @@ def test_oracle_node_types_known_case():
     solids = np.nonzero(attr == 0)[0][1:]
     assert np.all(T.node_types[solids] == 1)
     assert np.array_equal(oracle.bulk_nodes(T.node_types, T.ranks, 0), np.arange(1, 9))
+
+
+TOPO_ARRAYS = ("neighbors", "positions", "node_types", "bulk", "node_ranks", "global_dims")
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_binary_twin_round_trip_of_reference_files(case, tmp_path):
+    """A .vtklb written by the reference writer, read as text, written as the
+    binary twin and read back: the same topology, and every attribute gives
+    the same doubles and the same float roundings as from the text."""
+    name, _, lattice, periodic, attrs = case
+    geo = np.load(os.path.join(VT, name + "_geo.npy"))
+    L = lbamd.LATTICE_NAME[lattice]
+    for r in range(1, int(geo.max()) + 1):
+        text = lbamd.Topology.from_vtklb(os.path.join(VT, "%s%d.vtklb" % (name, r - 1)), L)
+        path = str(tmp_path / "t.vtklbb")
+        text.write_binary(path)
+        binr = lbamd.Topology.from_binary(path, L)
+        for m in TOPO_ARRAYS:
+            assert np.array_equal(getattr(text, m)(), getattr(binr, m)()), m
+        assert text.rank == binr.rank
+        for k in ["nodetype"] + sorted(attrs):
+            for as_float in (False, True):
+                assert np.array_equal(text.attribute(k, as_float), binr.attribute(k, as_float)), (k, as_float)
+        # and a second generation (binary -> binary) is byte-identical
+        path2 = str(tmp_path / "t2.vtklbb")
+        binr.write_binary(path2)
+        assert open(path, "rb").read() == open(path2, "rb").read()
+
+
+def test_binary_twin_of_built_topology_and_rejections(tmp_path):
+    geo, _ = lbamd.sphere_pack((20, 18, 16), 3.0, 0.5, 3, periodic="xz", walls=True)
+    t = lbamd.Topology.from_geometry(geo, lbamd.D3Q19, "xz")
+    path = str(tmp_path / "p.vtklbb")
+    t.write_binary(path)
+    b = lbamd.Topology.from_binary(path, lbamd.D3Q19)
+    for m in TOPO_ARRAYS:
+        assert np.array_equal(getattr(t, m)(), getattr(b, m)()), m
+    with pytest.raises(lbamd.LBError):  # wrong lattice
+        lbamd.Topology.from_binary(path, lbamd.D2Q9)
+    raw = open(path, "rb").read()
+    bad = str(tmp_path / "bad.vtklbb")
+    open(bad, "wb").write(raw[: len(raw) // 2])
+    with pytest.raises(lbamd.LBError):  # truncated
+        lbamd.Topology.from_binary(bad, lbamd.D3Q19)
+    open(bad, "wb").write(b"XXXXXXXX" + raw[8:])
+    with pytest.raises(lbamd.LBError):  # not a binary twin
+        lbamd.Topology.from_binary(bad, lbamd.D3Q19)
