@@ -34,6 +34,7 @@ SOURCES = {
     "calibrate.hip": HIP + ["-O3"],
     "topo_device.hip": HIP + ["-O3"],
     "topology.cpp": ["-O2", "-ffp-contract=off", "-x", "c++"],
+    "vtu.cpp": ["-O2", "-x", "c++"],
 }
 
 

@@ -84,6 +84,13 @@ SIGNATURES = {
     "lb_topo_write_binary": (C.c_int, [_vp, C.c_char_p]),
     "lb_topo_read_binary": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(_vp)]),
     "lb_topo_destroy": (C.c_int, [_vp]),
+    "lb_vtu_create": (C.c_int, [C.c_int32, _i32p, C.c_int32, C.c_char_p, C.c_int32, C.c_int32, C.c_int32,
+                                C.POINTER(_vp)]),
+    "lb_vtu_add_file": (C.c_int, [_vp, C.c_char_p]),
+    "lb_vtu_add_variable": (C.c_int, [_vp, C.c_char_p, C.c_int32, _vp]),
+    "lb_vtu_write": (C.c_int, [_vp, C.c_double]),
+    "lb_vtu_num_points": (C.c_int32, [_vp]),
+    "lb_vtu_destroy": (C.c_int, [_vp]),
     "lb_topo_num_nodes": (C.c_int32, [_vp]),
     "lb_topo_num_bulk": (C.c_int32, [_vp]),
     "lb_topo_num_dims": (C.c_int32, [_vp]),
@@ -474,6 +481,56 @@ class Context:
 
     def device_bytes(self):
         return lib().lb_device_bytes(self._h)
+
+
+VTU_DTYPES = {np.dtype(np.float64): 0, np.dtype(np.float32): 1, np.dtype(np.int32): 2}
+
+
+class VtuOutput:
+    """VTU/PVTU writer in the reference's format (io/Output.h, io/VTK.h).
+
+    pos: (n_cells, nd) cell positions (Grid::pos of the output nodes).
+    Variables are arrays of `dtype` with shape (n_cells,) or (n_cells, nd);
+    the writer reads them at every write(), so update them in place."""
+
+    def __init__(self, pos, directory, rank=0, nprocs=1, dtype=np.float64):
+        pos = np.ascontiguousarray(pos, dtype=np.int32)
+        self.n_cells, self.nd = pos.shape
+        self.dtype = np.dtype(dtype)
+        self._vars = []
+        self._h = _vp()
+        _check_global(lib().lb_vtu_create(self.nd, pos.ravel() if pos.size else np.zeros(1, np.int32),
+                                          self.n_cells, os.fsencode(directory), rank, nprocs,
+                                          VTU_DTYPES[self.dtype], C.byref(self._h)))
+
+    def add_file(self, name):
+        _check_global(lib().lb_vtu_add_file(self._h, name.encode()))
+
+    def add_variable(self, name, array):
+        if array.dtype != self.dtype or not array.flags.c_contiguous or array.shape[0] != self.n_cells:
+            raise ValueError("variable %r must be a C-contiguous %s array over %d cells" % (name, self.dtype,
+                                                                                          self.n_cells))
+        dim = 1 if array.ndim == 1 else array.shape[1]
+        self._vars.append(array)  # the writer reads it at every write
+        _check_global(lib().lb_vtu_add_variable(self._h, name.encode(), dim, array.ctypes.data_as(_vp)))
+
+    def write(self, time=0.0):
+        _check_global(lib().lb_vtu_write(self._h, float(time)))
+
+    @property
+    def n_points(self):
+        return lib().lb_vtu_num_points(self._h)
+
+    def close(self):
+        if self._h:
+            lib().lb_vtu_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def batch_footprint(lattice, precision, geo, periodic="x", n_fields=1):

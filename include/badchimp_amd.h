@@ -281,6 +281,30 @@ int lb_sphere_pack(int32_t nd, const int32_t* dims, double radius,
                    int32_t walls, int32_t max_fail, double relax,
                    int32_t* geo, double* info);
 
+/* ------------------------------------------------------------ VTU output */
+/* VTU/PVTU files in the reference's format (io/Output.h:25-175,
+ * io/VTK.h:316-560, 854-1355), the output std_case and twophase write:
+ * one voxel (3D) or pixel (2D) cell centred on each position, shared corner
+ * points numbered in order of first use, data appended raw.  lb_vtu_create
+ * takes the positions of the cells (Grid::pos(bulkNodes): n_cells*nd
+ * ints), the output directory (files go to dir/vtu/RRRR_name_NNNNNNN.vtu and
+ * dir/name_NNNNNNN.pvtu), the rank and number of ranks, and the data type
+ * of every variable.  A variable is a host array of n_cells*dim values of
+ * that type, cell-major; it is read at every lb_vtu_write, like the
+ * reference's Output reads its fields (dim 2 is written with a zero third
+ * component).  Each rank writes its own piece line of the .pvtu at its
+ * slot; rank 0 writes the header and the last rank the footer, so with
+ * several ranks the caller orders the calls (rank 0 first). */
+enum lb_vtu_dtype { LB_VTU_F64 = 0, LB_VTU_F32 = 1, LB_VTU_I32 = 2 };
+typedef struct lb_vtu lb_vtu;
+int lb_vtu_create(int32_t nd, const int32_t* pos, int32_t n_cells, const char* dir,
+                  int32_t rank, int32_t nprocs, int32_t dtype, lb_vtu** out);
+int lb_vtu_add_file(lb_vtu* v, const char* name);                     /* Output::add_file */
+int lb_vtu_add_variable(lb_vtu* v, const char* name, int32_t dim, const void* data);
+int lb_vtu_write(lb_vtu* v, double time);                             /* Output::write */
+int32_t lb_vtu_num_points(const lb_vtu* v);
+int lb_vtu_destroy(lb_vtu* v);
+
 /* ---- batched and serving form -------------------------------------------
  * No counterpart in the reference: the reference runs one solve per process
  * (std_case/main.cpp).  A batch keeps many independent permeability

@@ -13,7 +13,9 @@ container only; /root/reference is not available elsewhere).  For every case:
   4. the VTU of i = max is parsed: rho (rho0, rho1) and vel on the bulk nodes,
      in bulk order, i.e. the fields of the pre-collision state after max
      steps (std_case/main.cpp:109-155, main_TWOPHASE.cpp:236-424),
-  5. inputs (geometry, parameters) and outputs are stored in ref_<case>.npz.
+  5. inputs (geometry, parameters) and outputs are stored in ref_<case>.npz;
+     for a few small cases the VTU and PVTU files themselves are kept in
+     tests/golden/vtu/ as fixtures of the VTU writer.
 
 Run:  python tests/golden/make_ref_fixtures.py [case ...]
 """
@@ -71,7 +73,18 @@ def fmt(v):
     return " ".join(repr(float(x)) for x in np.atleast_1d(v))
 
 
-def run_case(binary, geo, lattice, periodic, attrs, dat, twophase=False, outsub="tp/"):
+def keep_output(outdir, base, name):
+    """Copy the reference's last VTU file and its PVTU into tests/golden/vtu/
+    (the byte-level fixtures of the VTU writer)."""
+    dst = os.path.join(HERE, "vtu")
+    os.makedirs(dst, exist_ok=True)
+    vtu = last_vtu(outdir, base)
+    shutil.copyfile(vtu, os.path.join(dst, name + ".vtu"))
+    n = os.path.basename(vtu).split("_")[-1]  # NNNNNNN.vtu
+    shutil.copyfile(os.path.join(outdir, "%s_%s.pvtu" % (base, n[:-4])), os.path.join(dst, name + ".pvtu"))
+
+
+def run_case(binary, geo, lattice, periodic, attrs, dat, twophase=False, outsub="tp/", keep=None):
     tmp = tempfile.mkdtemp(prefix="lbref_")
     try:
         g = Geometry(geo, lattice, periodic)
@@ -97,8 +110,12 @@ def run_case(binary, geo, lattice, periodic, attrs, dat, twophase=False, outsub=
             data = parse_vtu(last_vtu(od, "fluid"))
             force = np.loadtxt(os.path.join(od, "force.dat"), ndmin=2)
             data["force_x"] = force
+            if keep:
+                keep_output(od, "fluid", keep)
         else:
             data = parse_vtu(last_vtu(os.path.join(tmp, "output"), "lb_run"))
+            if keep:
+                keep_output(os.path.join(tmp, "output"), "lb_run", keep)
         return data
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
@@ -175,13 +192,16 @@ def tp_cases():
     yield "tp_walls_12x14x10", g, "x", r0, np.full(g.shape, 1.0), 1.0, 1.0, 0.01, 0.7, 5e-5, [0.0, 0.0, 0.0], 60
 
 
+VTU_KEEP = {"d2q9_random_7x5", "d3q19_pack_6x5x4_pxyz", "trt_d3q19_pack_6x5x4_px_walls", "tp_pack_6x5x4"}
+
+
 def main(selected):
     for name, lat, g, per, attrs, tau, force, max_it, prof in std_cases():
         if selected and name not in selected:
             continue
         binary = "std_case_d2q9" if lat == "D2Q9" else "std_case_d3q19"
         dat = input_dat(max_it, ["tau %s" % fmt(tau), "bodyforce %s" % fmt(force)])
-        data = run_case(binary, g, lat, per, attrs, dat)
+        data = run_case(binary, g, lat, per, attrs, dat, keep=name if name in VTU_KEEP else None)
         params = dict(case="std_case", lattice=lat, periodic=per, tau=tau, force=force, steps=max_it)
         out = dict(geo=g.astype(np.int8), params=json.dumps(params),
                    init_rho=attrs[0][1].astype(np.float64))
@@ -204,7 +224,7 @@ def main(selected):
             continue
         binary = "std_case_trt_d2q9" if lat == "D2Q9" else "std_case_trt_d3q19"
         dat = input_dat(max_it, ["tau %s" % fmt(list(taus)), "bodyforce %s" % fmt(force)])
-        data = run_case(binary, g, lat, per, attrs, dat)
+        data = run_case(binary, g, lat, per, attrs, dat, keep=name if name in VTU_KEEP else None)
         params = dict(case="std_case_trt", lattice=lat, periodic=per, tau=list(taus), force=force, steps=max_it)
         np.savez_compressed(os.path.join(HERE, "ref_%s.npz" % name), geo=g.astype(np.int8),
                             params=json.dumps(params), init_rho=attrs[0][1].astype(np.float64),
@@ -218,7 +238,8 @@ def main(selected):
         dat = input_dat(max_it, ["tau %s" % fmt([tau0, tau1]), "bodyforce %s" % fmt(force),
                                  "sigma %s" % fmt(sigma), "beta %s" % fmt(beta), "momx %s" % fmt(momx)],
                         extra="outdir tp/\n")
-        data = run_case("twophase", g, "D3Q19", per, attrs, dat, twophase=True)
+        data = run_case("twophase", g, "D3Q19", per, attrs, dat, twophase=True,
+                        keep=name if name in VTU_KEEP else None)
         params = dict(case="twophase", lattice="D3Q19", periodic=per, tau0=tau0, tau1=tau1, sigma=sigma,
                       beta=beta, momx=momx, force=force, steps=max_it)
         out = dict(geo=g.astype(np.int8), params=json.dumps(params), rho0_attr=r0, wett_attr=wett,
