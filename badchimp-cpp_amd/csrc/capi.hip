@@ -437,7 +437,7 @@ int lb_ctx_destroy(lb_ctx* c) {
   for (void* p : {static_cast<void*>(c->d_link), static_cast<void*>(c->d_rho), static_cast<void*>(c->d_vel),
                   static_cast<void*>(c->d_phi), static_cast<void*>(c->d_phi_wall), static_cast<void*>(c->d_partial),
                   static_cast<void*>(c->d_scalar), static_cast<void*>(c->d_coord),
-                  static_cast<void*>(c->d_rows), static_cast<void*>(c->d_win)})
+                  static_cast<void*>(c->d_rows), static_cast<void*>(c->d_win), static_cast<void*>(c->d_table)})
     if (p) hipFree(p);
   hipEventDestroy(c->ev0);
   hipEventDestroy(c->ev1);
@@ -613,6 +613,12 @@ int lb_download_scalar(lb_ctx* c, int32_t which, int32_t nf, double* s) {
       const float* hf = reinterpret_cast<const float*>(h.data());
       for (int32_t i = 0; i < c->n_bulk; ++i) s[c->node_of[i]] = static_cast<double>(hf[i]);
     }
+  } else if (which == LB_VISC) {
+    if (nf != 1) return fail(c, LB_ERR_ARG, "the viscosity has one field");
+    if (!c->have_visc) return fail(c, LB_ERR_STATE, "the last stored step was not a rheology step");
+    std::vector<double> h(NP);
+    LB_HIP(c, hipMemcpy(h.data(), c->d_rho + NP, NP * sizeof(double), hipMemcpyDeviceToHost));
+    for (int32_t i = 0; i < c->n_bulk; ++i) s[c->node_of[i]] = h[i];
   } else {
     return fail(c, LB_ERR_ARG, "unknown scalar %d", which);
   }
@@ -670,7 +676,7 @@ BgkArgs single_phase_args(const lb_ctx* c, double tau, double tau_anti, const do
   a.tau_factor = 1 - 0.5 / tau;
   for (int d = 0; d < 3; ++d) a.F[d] = d < c->nD ? force[d] : 0.0;
   if (tau_anti > 0.0) {
-    a.trt = 1;
+    a.mode = kTrt;
     a.tauA_inv = 1.0 / tau_anti;
     a.tauA_factor = 1 - 0.5 / tau_anti;
   }
@@ -697,6 +703,7 @@ int step_single(lb_ctx* c, int32_t nsteps, const BgkArgs& a) {
   const int rc = t.stop(nsteps);
   if (rc != LB_OK) return rc;
   c->have_macro = true;
+  c->have_visc = a.mode == kRheology;
   return LB_OK;
 }
 }  // namespace
@@ -710,6 +717,58 @@ int lb_step_bgk(lb_ctx* c, int32_t nsteps, double tau, const double* force) {
   if (nsteps < 0 || !force || !(tau > 0.0)) return fail(c, LB_ERR_ARG, "lb_step_bgk: bad argument");
   if (nsteps == 0) return LB_OK;
   return step_single(c, nsteps, single_phase_args(c, tau, 0.0, force));
+}
+
+int lb_set_viscosity_table(lb_ctx* c, int32_t n, const double* strain_rate_sq, const double* viscosity) {
+  if (!c) return LB_ERR_ARG;
+  if (n < 1 || !strain_rate_sq || !viscosity) return fail(c, LB_ERR_ARG, "lb_set_viscosity_table: bad argument");
+  for (int32_t k = 0; k < n; ++k) {
+    if (!std::isfinite(strain_rate_sq[k]) || !std::isfinite(viscosity[k]))
+      return fail(c, LB_ERR_ARG, "viscosity table row %d is not finite", k);
+    if (k > 0 && !(strain_rate_sq[k] > strain_rate_sq[k - 1]))
+      return fail(c, LB_ERR_ARG, "viscosity table: strain-rate squares must increase (row %d)", k);
+  }
+  // GeneralizedNewtonian's table with its constant extension (LBrheology.h:370-384)
+  const int32_t m = n + 2;
+  std::vector<double> h(2 * static_cast<size_t>(m));
+  double* sr = h.data();
+  double* nu = h.data() + m;
+  for (int32_t k = 0; k < n; ++k) {
+    sr[k + 1] = strain_rate_sq[k];
+    nu[k + 1] = viscosity[k];
+  }
+  sr[0] = sr[1] - 1;
+  nu[0] = nu[1];
+  sr[n + 1] = sr[n] + 1;
+  nu[n + 1] = nu[n];
+  std::vector<float> hf(h.begin(), h.end());
+  hipSetDevice(c->device);
+  if (c->d_table) hipFree(c->d_table);
+  c->d_table = nullptr;
+  c->tab_n = 0;
+  const size_t bytes = h.size() * sizeof(double) + hf.size() * sizeof(float);
+  LB_HIP(c, hipMalloc(reinterpret_cast<void**>(&c->d_table), bytes));
+  LB_HIP(c, hipMemcpy(c->d_table, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  LB_HIP(c, hipMemcpy(c->d_table + h.size(), hf.data(), hf.size() * sizeof(float), hipMemcpyHostToDevice));
+  c->tab_n = m;
+  return LB_OK;
+}
+
+int lb_step_rheology(lb_ctx* c, int32_t nsteps, const double* force) {
+  if (!c) return LB_ERR_ARG;
+  int rc = check_ready(c, 1);
+  if (rc != LB_OK) return rc;
+  if (nsteps < 0 || !force) return fail(c, LB_ERR_ARG, "lb_step_rheology: bad argument");
+  if (!c->d_table) return fail(c, LB_ERR_STATE, "lb_step_rheology: no viscosity table (lb_set_viscosity_table)");
+  if (nsteps == 0) return LB_OK;
+  BgkArgs a = single_phase_args(c, 1.0, 0.0, force);
+  a.mode = kRheology;
+  a.tab_n = c->tab_n;
+  a.tab_sr = c->d_table;
+  a.tab_nu = c->d_table + c->tab_n;
+  a.tab_srf = reinterpret_cast<const float*>(c->d_table + 2 * c->tab_n);
+  a.tab_nuf = a.tab_srf + c->tab_n;
+  return step_single(c, nsteps, a);
 }
 
 int lb_step_trt(lb_ctx* c, int32_t nsteps, double tau_sym, double tau_anti, const double* force) {

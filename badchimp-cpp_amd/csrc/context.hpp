@@ -52,12 +52,22 @@ struct BgkArgs {
   double tau_inv;     // 1.0 / tau                (LBcollision.h:40); 1 / tauSym for TRT (:62)
   double tau_factor;  // 1 - 0.5 / tau            (LBcollision.h:206); tauSym_factor for TRT (:223)
   double F[3];        // body force
-  // two-relaxation-time operator (calcOmegaBGKTRT / calcDeltaOmegaFTRT with
-  // phi = 1, LBcollision.h:50-75, 214-233); used when trt != 0
-  int32_t trt;
+  // collision operator: kBgk; kTrt, the two-relaxation-time operator
+  // (calcOmegaBGKTRT / calcDeltaOmegaFTRT with phi = 1, LBcollision.h:50-75,
+  // 214-233); kRheology, GeneralizedNewtonian::omegaBGK with the node's
+  // relaxation time from a viscosity table (LBrheology.h:321-476)
+  int32_t mode;
   double tauA_inv;     // 1.0 / tauAnti                  (LBcollision.h:63)
   double tauA_factor;  // 1 - 0.5 / tauAnti              (LBcollision.h:224)
+  // viscosity table with its constant extension, tab_n = rows + 2
+  // (LBrheology.h:370-384); FP64 for the FP64 kernels, float otherwise
+  const double* tab_sr;
+  const double* tab_nu;
+  const float* tab_srf;
+  const float* tab_nuf;
+  int32_t tab_n;
 };
+constexpr int kBgk = 0, kTrt = 1, kRheology = 2;
 
 // One solve of the batched form: its two population buffers (each Q * NP
 // elements, AoSoA as above), its row-window address table, its block

@@ -45,6 +45,9 @@ struct lb_ctx {
   double* d_scalar = nullptr;    // [8]: 0 force x, 1 flux sum, 4..6 reductions
   int64_t n_blocks = 0;
   bool have_macro = false;
+  bool have_visc = false;        // the last stored step was a rheology step
+  double* d_table = nullptr;     // viscosity table: sr[tab_n], nu[tab_n] (FP64), then the same in float
+  int32_t tab_n = 0;             // rows + 2 (constant extension at both ends)
   double last_force[3] = {0, 0, 0};
   int64_t bytes = 0;
 };

@@ -16,6 +16,7 @@
 
 #include "kernels_f32.hpp"
 #include "lattice.hpp"
+#include "table.hpp"
 
 namespace lbamd {
 namespace {
@@ -71,16 +72,63 @@ __device__ __forceinline__ void load_g(const S* __restrict__ gin, const A& ad, i
   g[L::nQ - 1] = Store<S>::get(ld_elem<O>(gin, pidx<L::nQ>(L::nQ - 1, i)));
 }
 
+// Viscosity of a node from its strain rate (GeneralizedNewtonian,
+// LBrheology.h:436-462, source 0) on shifted populations:
+// f - feq = g - w (a0 + 4.5 rho cu^2 + 3 rho cu).
+template <class L>
+__device__ __forceinline__ float rheology_visc_lp(const float (&g)[L::nQ], float a0, float r45, float r3,
+                                                  const float* u, const float (&F)[3], const float* __restrict__ sr,
+                                                  const float* __restrict__ nu, int n) {
+  constexpr int Q = L::nQ, D = L::nD;
+  float E[D][D];
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = 0; j < D; ++j) E[i][j] = 0.0f;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const float cu = cdot<L>(q, u);
+    const float neq = g[q] - float(L::w(q)) * (a0 + r45 * cu * cu + r3 * cu);
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      const int ci = L::c(q, i);
+      E[i][i] += neq * (float(ci * ci) - float(L::c2));
+#pragma unroll
+      for (int j = i + 1; j < D; ++j) {
+        const int cc = ci * L::c(q, j);
+        if (cc > 0) E[i][j] += neq;
+        else if (cc < 0) E[i][j] -= neq;
+      }
+    }
+  }
+  float sq = 0.0f;
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    E[i][i] += u[i] * F[i];
+    sq += E[i][i] * E[i][i];
+#pragma unroll
+    for (int j = i + 1; j < D; ++j) {
+      E[i][j] += 0.5f * (u[i] * F[j] + u[j] * F[i]);
+      sq += 2.0f * E[i][j] * E[i][j];
+    }
+  }
+  const int k = table_index(sr, n, sq);
+  return (nu[k + 1] - nu[k]) * (sq - sr[k]) / (sr[k + 1] - sr[k]) + nu[k];
+}
+
 // One node of the BGK + Guo step on shifted populations; returns j_x (FP64)
-// for the permeability sum when REDUCE.  TRT: the two-relaxation-time
+// for the permeability sum when REDUCE.  MODE kTrt: the two-relaxation-time
 // operator (LBcollision.h:50-75, 214-233) with 1/tauAnti = tauA_inv and
-// tauAnti_factor = tauA_factor; tau_inv, tau_factor are then the symmetric ones.
-template <class L, typename S, class A, bool FIRST, bool STORE, bool REDUCE, bool TRT = false>
+// tauAnti_factor = tauA_factor; tau_inv, tau_factor are then the symmetric
+// ones.  MODE kRheology: tau_inv, tau_factor from the node's viscosity
+// (table sr/nu of n rows), stored beside rho (rho_o[NP + i]) when STORE.
+template <class L, typename S, class A, bool FIRST, bool STORE, bool REDUCE, int MODE = kBgk>
 __device__ __forceinline__ double bgk_node_lp(const S* __restrict__ gin, S* __restrict__ gout, const A& ad,
                                               int32_t i, int64_t NP, float tau_inv, float tau_factor,
                                               const float (&F)[3], double* __restrict__ rho_o,
                                               double* __restrict__ vel_o, float tauA_inv = 0.0f,
-                                              float tauA_factor = 0.0f) {
+                                              float tauA_factor = 0.0f, const float* __restrict__ sr = nullptr,
+                                              const float* __restrict__ nu = nullptr, int tab_n = 0) {
   constexpr int Q = L::nQ, D = L::nD;
   float g[Q];
   load_g<L, S, FIRST>(gin, ad, i, NP, g);
@@ -107,11 +155,18 @@ __device__ __forceinline__ double bgk_node_lp(const S* __restrict__ gin, S* __re
   //   g'   = (1 - 1/tau) g + geq/tau + dF
   const float a0 = drho - 1.5f * rho * dot<L>(u, u);
   const float m3uF = -3.0f * dot<L>(u, F);
-  const float omt = 1.0f - tau_inv;
   const float r45 = 4.5f * rho, r3 = 3.0f * rho;
+  if constexpr (MODE == kRheology) {
+    const float visc = rheology_visc_lp<L>(g, a0, r45, r3, u, F, sr, nu, tab_n);
+    const float tau = visc * 3.0f / rho + 0.5f;
+    tau_inv = 1.0f / tau;
+    tau_factor = 1.0f - 0.5f / tau;
+    if (STORE) rho_o[NP + i] = static_cast<double>(visc);
+  }
+  const float omt = 1.0f - tau_inv;
   // one 64-bit address per node; the direction offsets fold into the stores
   S* __restrict__ out = elem_ptr<A::kOff32>(gout, pidx<Q>(0, i));
-  if constexpr (TRT) {
+  if constexpr (MODE == kTrt) {
     // Each pair relaxes its even part (gs = (g_q + g_r)/2 = f_sym - w, towards
     // w(a0 + 4.5 rho cu^2)) with 1/tauSym and its odd part (ga = (g_q - g_r)/2,
     // towards 3 w rho cu) with 1/tauAnti; the Guo term splits the same way.
@@ -164,21 +219,22 @@ __device__ __forceinline__ double bgk_node_lp(const S* __restrict__ gin, S* __re
   return REDUCE ? static_cast<double>(jd[0]) : 0.0;
 }
 
-template <class L, typename S, class A, bool FIRST, bool STORE, bool REDUCE, bool TRT>
+template <class L, typename S, class A, bool FIRST, bool STORE, bool REDUCE, int MODE>
 __global__ __launch_bounds__(kBlock) void k_bgk_lp(const S* __restrict__ gin, S* __restrict__ gout, A ad,
                                                    int32_t n, int64_t NP, float tau_inv, float tau_factor,
                                                    float F0, float F1, float F2,
                                                    double* __restrict__ rho_o,
                                                    double* __restrict__ vel_o,
                                                    double* __restrict__ partial, float tauA_inv,
-                                                   float tauA_factor) {
+                                                   float tauA_factor, const float* __restrict__ sr,
+                                                   const float* __restrict__ nu, int tab_n) {
   const int32_t b = static_cast<int32_t>(blockIdx.x);
   const int32_t i = b * kBlock + threadIdx.x;
   double jx = 0.0;
   if (i < n) {
     const float F[3] = {F0, F1, F2};
-    jx = bgk_node_lp<L, S, A, FIRST, STORE, REDUCE, TRT>(gin, gout, ad, i, NP, tau_inv, tau_factor, F, rho_o,
-                                                         vel_o, tauA_inv, tauA_factor);
+    jx = bgk_node_lp<L, S, A, FIRST, STORE, REDUCE, MODE>(gin, gout, ad, i, NP, tau_inv, tau_factor, F, rho_o,
+                                                          vel_o, tauA_inv, tauA_factor, sr, nu, tab_n);
   }
   if (REDUCE) {
     const double s = block_sum(jx);
@@ -657,7 +713,7 @@ hipError_t bgk_lowp(int precision, int lattice, bool first, bool store, bool red
     const dim3 g = grid_for(n), b(kBlock);
 #define LB_K(F_, S_, R_, T_)                                                                             \
   k_bgk_lp<L, S, A, F_, S_, R_, T_><<<g, b, 0, s>>>(gi, go, addr, n, NP, ti, tf, F0, F1, F2, rho, vel, partial, \
-                                                    tai, taf)
+                                                    tai, taf, a.tab_srf, a.tab_nuf, a.tab_n)
 #define LB_KEYS(T_)                                 \
   switch (key) {                                    \
     case 0: LB_K(false, false, false, T_); break;   \
@@ -667,10 +723,12 @@ hipError_t bgk_lowp(int precision, int lattice, bool first, bool store, bool red
     case 6: LB_K(true, true, false, T_); break;     \
     default: LB_K(true, true, true, T_); break;     \
   }
-    if (a.trt) {
-      LB_KEYS(true)
+    if (a.mode == kTrt) {
+      LB_KEYS(kTrt)
+    } else if (a.mode == kRheology) {
+      LB_KEYS(kRheology)
     } else {
-      LB_KEYS(false)
+      LB_KEYS(kBgk)
     }
 #undef LB_KEYS
 #undef LB_K

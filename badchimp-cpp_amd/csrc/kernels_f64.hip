@@ -14,6 +14,7 @@
 
 #include "kernels.hpp"
 #include "lattice.hpp"
+#include "table.hpp"
 
 namespace lbamd {
 namespace {
@@ -60,11 +61,62 @@ __device__ __forceinline__ double block_sum(double v) {
 }
 
 // ------------------------------------------------------------------ BGK
+// GeneralizedNewtonian::omegaBGK's viscosity (LBrheology.h:436-462, source
+// 0): strain-rate tensor from the non-equilibrium moments and the Guo
+// terms, its square sum, table lookup and linear interpolation.  Terms
+// that are +-0 (c_i c_j = 0 off the diagonal, the source terms) are left
+// out: a sum that starts at +0 never holds -0, so adding +-0 never changes it.
+template <class L>
+__device__ __forceinline__ double rheology_visc(const double (&f)[L::nQ], double rho, const double* u,
+                                                double c2u2, const BgkArgs& a) {
+  constexpr int Q = L::nQ, D = L::nD;
+  double S[D][D];
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = 0; j < D; ++j) S[i][j] = 0.0;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const double cu = cdot<L>(q, u);
+    const double feq = (L::w(q) * rho) * ((1 + L::c2Inv * cu) + L::c4Inv0_5 * (cu * cu - c2u2));
+    const double neq = f[q] - feq;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      const int ci = L::c(q, i);
+      S[i][i] += neq * (static_cast<double>(ci * ci) - L::c2);
+#pragma unroll
+      for (int j = i + 1; j < D; ++j) {
+        const int cc = ci * L::c(q, j);
+        if (cc > 0) S[i][j] += neq;
+        else if (cc < 0) S[i][j] += -neq;
+      }
+    }
+  }
+  double sq = 0;
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    S[i][i] += (u[i] * a.F[i]);
+    sq += S[i][i] * S[i][i];
+#pragma unroll
+    for (int j = i + 1; j < D; ++j) {
+      S[i][j] += 0.5 * (u[i] * a.F[j] + u[j] * a.F[i]);
+      sq += (2 * S[i][j]) * S[i][j];
+    }
+  }
+  const double* __restrict__ sr = a.tab_sr;
+  const double* __restrict__ nu = a.tab_nu;
+  const int k = table_index(sr, a.tab_n, sq);
+  return ((nu[k + 1] - nu[k]) * (sq - sr[k])) / (sr[k + 1] - sr[k]) + nu[k];
+}
+
 // One node of the std_case loop body (std_case/main.cpp:110-145); returns the
 // node's j_x = sum_q f c_x + F_x/2 for the permeability sum when REDUCE.
-// TRT: the loop body with the two-relaxation-time operator in place of
-// calcOmegaBGK / calcDeltaOmegaF (LBcollision.h:50-75, 214-233).
-template <class L, class A, bool FIRST, bool STORE, bool REDUCE, bool TRT = false>
+// MODE kTrt: the loop body with the two-relaxation-time operator in place of
+// calcOmegaBGK / calcDeltaOmegaF (LBcollision.h:50-75, 214-233).  MODE
+// kRheology: polymer_viscosity_tabulation/main.cpp:105-155, the relaxation
+// time from the node's strain rate and the viscosity table; the viscosity
+// is stored beside rho (rho_o[NP + i]) when STORE.
+template <class L, class A, bool FIRST, bool STORE, bool REDUCE, int MODE = kBgk>
 __device__ __forceinline__ double bgk_node(const double* __restrict__ fin, double* __restrict__ fout, const A& ad,
                                            int32_t i, int64_t NP, const BgkArgs& a, double* __restrict__ rho_o,
                                            double* __restrict__ vel_o) {
@@ -88,7 +140,26 @@ __device__ __forceinline__ double bgk_node(const double* __restrict__ fin, doubl
   const double c2u2 = L::c2 * u2;
   const double c2uF = L::c2 * uF;
   double* __restrict__ out = elem_ptr<A::kOff32>(fout, pidx<Q>(0, i));
-  if constexpr (TRT) {
+  if constexpr (MODE == kRheology) {
+    const double visc = rheology_visc<L>(f[0], rho, u, c2u2, a);
+    // tau_ = visc_ * c2Inv / rho + 0.5 (LBrheology.h:463); calcDeltaOmegaF's
+    // tau_factor at that tau (LBcollision.h:206)
+    const double tau = (visc * L::c2Inv) / rho + 0.5;
+    const double tau_inv = 1.0 / tau;
+    const double tau_factor = 1 - 0.5 / tau;
+    if (STORE) rho_o[NP + i] = visc;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const double cu = cdot<L>(q, u);
+      const double cF = cdot<L>(q, a.F);
+      const double feq = (L::w(q) * rho) * ((1 + L::c2Inv * cu) + L::c4Inv0_5 * (cu * cu - c2u2));
+      const double om = (-tau_inv) * (f[0][q] - feq);
+      const double dF = (L::w(q) * tau_factor) * (L::c2Inv * cF + L::c4Inv * (cF * cu - c2uF));
+      __builtin_nontemporal_store((f[0][q] + om) + dF, out + (q << kPLog));
+    }
+    return jx;
+  }
+  if constexpr (MODE == kTrt) {
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int r = L::rev(q);
@@ -120,7 +191,7 @@ __device__ __forceinline__ double bgk_node(const double* __restrict__ fin, doubl
   return jx;
 }
 
-template <class L, class A, bool FIRST, bool STORE, bool REDUCE, bool TRT>
+template <class L, class A, bool FIRST, bool STORE, bool REDUCE, int MODE>
 __global__ __launch_bounds__(kBlock) void k_bgk(const double* __restrict__ fin,
                                                 double* __restrict__ fout, A ad, int32_t n,
                                                 int64_t NP, BgkArgs a, double* __restrict__ rho_o,
@@ -129,7 +200,7 @@ __global__ __launch_bounds__(kBlock) void k_bgk(const double* __restrict__ fin,
   const int32_t b = static_cast<int32_t>(blockIdx.x);
   const int32_t i = b * kBlock + threadIdx.x;
   double jx = 0.0;
-  if (i < n) jx = bgk_node<L, A, FIRST, STORE, REDUCE, TRT>(fin, fout, ad, i, NP, a, rho_o, vel_o);
+  if (i < n) jx = bgk_node<L, A, FIRST, STORE, REDUCE, MODE>(fin, fout, ad, i, NP, a, rho_o, vel_o);
   if (REDUCE) {
     const double s = block_sum(jx);
     if (threadIdx.x == 0) partial[b] = s;
@@ -442,10 +513,12 @@ static void launch_bgk(bool first, bool store, bool reduce, dim3 g, hipStream_t 
     case 6: LB_K(true, true, false, T_); break;   \
     default: LB_K(true, true, true, T_); break;   \
   }
-  if (a.trt) {
-    LB_KEYS(true)
+  if (a.mode == kTrt) {
+    LB_KEYS(kTrt)
+  } else if (a.mode == kRheology) {
+    LB_KEYS(kRheology)
   } else {
-    LB_KEYS(false)
+    LB_KEYS(kBgk)
   }
 #undef LB_KEYS
 #undef LB_K

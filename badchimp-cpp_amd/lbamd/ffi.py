@@ -15,7 +15,7 @@ LIB_PATH = os.environ.get("LBAMD_LIB") or os.path.join(os.path.dirname(HERE), "l
 
 D2Q9, D3Q19 = 0, 1
 FP64, FP32S, FP16S = 0, 1, 2
-RHO, PHI = 0, 1
+RHO, PHI, VISC = 0, 1, 2
 VEL = 0
 FORCE_FIXED, FORCE_MOMX = 0, 1
 TOPO_ALLOW_GHOST0 = 1
@@ -59,6 +59,8 @@ SIGNATURES = {
     "lb_set_wall_densities": (C.c_int, [_vp, _f64p]),
     "lb_step_bgk": (C.c_int, [_vp, C.c_int32, C.c_double, _f64p]),
     "lb_step_trt": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_double, _f64p]),
+    "lb_set_viscosity_table": (C.c_int, [_vp, C.c_int32, _f64p, _f64p]),
+    "lb_step_rheology": (C.c_int, [_vp, C.c_int32, _f64p]),
     "lb_step_cg": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double,
                              C.c_int32, _f64p]),
     "lb_last_force": (C.c_int, [_vp, _f64p]),
@@ -421,6 +423,21 @@ class Context:
         F = np.zeros(3)
         F[: self.nD] = force[: self.nD]
         self._check(lib().lb_step_trt(self._h, int(nsteps), float(tau_sym), float(tau_anti), F))
+
+    def set_viscosity_table(self, strain_rate_sq, viscosity):
+        """GeneralizedNewtonian's table (LBrheology.h:358-388)."""
+        sr = np.ascontiguousarray(strain_rate_sq, dtype=np.float64)
+        nu = np.ascontiguousarray(viscosity, dtype=np.float64)
+        if sr.shape != nu.shape or sr.ndim != 1:
+            raise ValueError("strain-rate squares and viscosities must be 1-D arrays of one length")
+        self._check(lib().lb_set_viscosity_table(self._h, sr.size, sr, nu))
+
+    def step_rheology(self, nsteps, force):
+        """std_case steps with the generalized-Newtonian relaxation time
+        (polymer_viscosity_tabulation/main.cpp)."""
+        F = np.zeros(3)
+        F[: self.nD] = force[: self.nD]
+        self._check(lib().lb_step_rheology(self._h, int(nsteps), F))
 
     def step_cg(self, nsteps, tau0, tau1, sigma, beta, force_mode, force):
         F = np.zeros(3)

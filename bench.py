@@ -149,7 +149,7 @@ def make_ctx(precision, nb, types, bulk, pos, layout):
     return ctx
 
 
-def gpu_leg(topo, nb, types, bulk, pos, layout, precision, steps, warmup, tau, force, tau_anti=None):
+def gpu_leg(topo, nb, types, bulk, pos, layout, precision, steps, warmup, tau, force, tau_anti=None, table=None):
     ctx = make_ctx(precision, nb, types, bulk, pos, layout)
     got_layout = {2: "row windows", 1: "rows"}.get(ctx.layout, "table")
     if ctx.layout_note and ctx.layout_note != got_layout:
@@ -159,8 +159,13 @@ def gpu_leg(topo, nb, types, bulk, pos, layout, precision, steps, warmup, tau, f
     f0[bulk] = w[None, :]
     ctx.upload_f(f0)
 
+    if table is not None:
+        ctx.set_viscosity_table(table[:, 0], table[:, 1])
+
     def step(n):
-        if tau_anti is None:
+        if table is not None:
+            ctx.step_rheology(n, force)
+        elif tau_anti is None:
             ctx.step_bgk(n, tau, force)
         else:
             ctx.step_trt(n, tau, tau_anti, force)
@@ -197,7 +202,8 @@ def permeability(nb, types, bulk, pos, layout, n_nodes, precision, tau, force, n
 def trt_leg(args, geo, topo, nb, types, bulk, pos, force, n_bulk, k_bgk):
     """Two-relaxation-time step on the same pack: tau_sym = 1, tau_anti =
     7/8 ((tau_sym - 1/2)(tau_anti - 1/2) = 3/16), timed in FP32S and FP64, and
-    its permeability in FP64 and FP32S beside BGK's at tau = 1."""
+    its permeability in FP64 and FP32S beside BGK's at tau = 1; and the
+    generalized-Newtonian step timed the same way."""
     ts, ta = 1.0, 0.875
     out = {"tau_sym": ts, "tau_anti": ta}
     for name, prec, bpl in (("fp32s", lbamd.FP32S, 2 * 19 * 4), ("fp64", lbamd.FP64, 2 * 19 * 8)):
@@ -207,6 +213,19 @@ def trt_leg(args, geo, topo, nb, types, bulk, pos, force, n_bulk, k_bgk):
         out[name] = {"value": round(n_bulk * args.steps / t / 1e6, 2), "unit": "MLUPS", "avg_kernel_ms": kms,
                      "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
         log("TRT %s: %.1f MLUPS, kernel %.3f ms" % (name, out[name]["value"], kms))
+    # generalized-Newtonian step: per-node relaxation time from a 40-row
+    # shear-thinning (Carreau) viscosity table
+    sr = np.logspace(-16, -4, 40)
+    table = np.stack([sr, 0.02 + (1.0 / 6.0 - 0.02) * (1.0 + 1e10 * sr) ** -0.25], 1)
+    out["rheology"] = {"table_rows": 40}
+    for name, prec, bpl in (("fp32s", lbamd.FP32S, 2 * 19 * 4), ("fp64", lbamd.FP64, 2 * 19 * 8)):
+        t, kms, _ = gpu_leg(topo, nb, types, bulk, pos, args.layout, prec, args.steps, args.warmup, 1.0, force,
+                            table=table)
+        gbs = n_bulk * bpl / (kms * 1e-3) / 1e9
+        out["rheology"][name] = {"value": round(n_bulk * args.steps / t / 1e6, 2), "unit": "MLUPS",
+                                 "avg_kernel_ms": kms, "achieved_GBs": round(gbs, 1),
+                                 "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        log("rheology %s: %.1f MLUPS, kernel %.3f ms" % (name, out["rheology"][name]["value"], kms))
     if not args.no_perm:
         n_box = float(geo.size)
         k64, s64, _ = permeability(nb, types, bulk, pos, args.layout, topo.n_nodes, lbamd.FP64, ts, force, n_box,

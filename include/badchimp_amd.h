@@ -60,7 +60,7 @@ enum lb_status {
                                   row layout would apply */
 
 /* field selectors */
-enum lb_scalar { LB_RHO = 0, LB_PHI = 1 };
+enum lb_scalar { LB_RHO = 0, LB_PHI = 1, LB_VISC = 2 };
 enum lb_vector { LB_VEL = 0 };
 
 /* two-phase body-force modes */
@@ -155,6 +155,19 @@ int lb_write_vector(lb_ctx* ctx, int32_t which, const char* prefix);
 /* nsteps iterations of the std_case loop body (std_case/main.cpp:110-145):
  * BGK + Guo forcing, streaming and half-way bounce-back, force uniform. */
 int lb_step_bgk(lb_ctx* ctx, int32_t nsteps, double tau, const double* force);
+
+/* Generalized-Newtonian fluid (GeneralizedNewtonian, LBrheology.h:321-476,
+ * the case polymer_viscosity_tabulation/main.cpp:105-155): the viscosity
+ * table is n rows of (strain-rate square sum_ij E_ij^2, viscosity) with
+ * increasing strain-rate squares, extended by one constant row below and
+ * above as the class's constructor does.  lb_step_rheology runs the
+ * std_case loop with each node's relaxation time tau = nu c2Inv / rho + 1/2
+ * looked up from its strain rate (upper_bound, linear interpolation) and
+ * the Guo term at that tau.  The node viscosities of the last step are
+ * downloaded with lb_download_scalar(ctx, LB_VISC, 1, ...). */
+int lb_set_viscosity_table(lb_ctx* ctx, int32_t n, const double* strain_rate_sq,
+                           const double* viscosity);
+int lb_step_rheology(lb_ctx* ctx, int32_t nsteps, const double* force);
 
 /* The same loop body with the two-relaxation-time operator in place of
  * calcOmegaBGK / calcDeltaOmegaF: calcOmegaBGKTRT(f, tau_sym, tau_anti, ...)

@@ -41,6 +41,9 @@ def lib():
         L.orc_std_case_trt.restype = C.c_int
         L.orc_std_case_trt.argtypes = [C.c_int, C.c_int32, _i32p, _i16p, _i32p, C.c_int32, _f64p, _f64p,
                                        C.c_double, C.c_double, _f64p, C.c_int32, _f64p, _f64p, C.c_int]
+        L.orc_rheology.restype = C.c_int
+        L.orc_rheology.argtypes = [C.c_int, C.c_int32, _i32p, _i16p, _i32p, C.c_int32, _f64p, _f64p, C.c_int32,
+                                   _f64p, _f64p, _f64p, C.c_int32, _f64p, _f64p, _f64p, C.c_int]
         L.orc_twophase.restype = C.c_int
         L.orc_twophase.argtypes = [C.c_int, C.c_int32, _i32p, _i16p, _i32p, C.c_int32, _f64p, _f64p,
                                    _f64p, _f64p, _f64p, C.c_double, C.c_double, C.c_double,
@@ -79,6 +82,28 @@ def run_std_case(lattice, neighbor, node_type, bulk, f, tau, force, nsteps, nthr
     else:
         lib().orc_std_case(*args, float(tau), *tail)
     return f.reshape(n_nodes, Q), rho, vel.reshape(n_nodes, D)
+
+
+def run_rheology(lattice, neighbor, node_type, bulk, f, table, force, nsteps, nthreads=1):
+    """nsteps iterations of the generalized-Newtonian loop
+    (polymer_viscosity_tabulation/main.cpp).  table: (m, 2) rows of
+    (strain-rate square, viscosity).  Returns f, rho, vel and the node
+    viscosities of the last iteration."""
+    n_nodes = node_type.shape[0]
+    Q, D = Q_OF[lattice], ND_OF[lattice]
+    f = _c(f, np.float64).reshape(-1).copy()
+    ftmp = f.copy()
+    rho, visc = np.zeros(n_nodes), np.zeros(n_nodes)
+    vel = np.zeros(n_nodes * D)
+    F = np.zeros(3)
+    F[:D] = force[:D]
+    bulk = _c(bulk, np.int32)
+    table = _c(table, np.float64)
+    lib().orc_rheology(lattice, n_nodes, _c(neighbor, np.int32).ravel(), _c(node_type, np.int16),
+                       bulk if bulk.size else np.zeros(1, np.int32), bulk.size, f, ftmp, table.shape[0],
+                       _c(table[:, 0], np.float64), _c(table[:, 1], np.float64), F, int(nsteps), rho, vel, visc,
+                       int(nthreads))
+    return f.reshape(n_nodes, Q), rho, vel.reshape(n_nodes, D), visc
 
 
 def run_twophase(lattice, neighbor, node_type, bulk, f, rho, tau0, tau1, sigma, beta,

@@ -13,6 +13,15 @@
 #                   calcOmegaBGKTRT (LBcollision.h:50-75) and calcDeltaOmegaF
 #                   -> calcDeltaOmegaFTRT with phi = 1 (LBcollision.h:214-233);
 #                   tau = fluid.tau[0] (symmetric), tauAnti = fluid.tau[1]
+#   rheology_d2q9, rheology_d3q19
+#                   src/polymer_viscosity_tabulation/main.cpp (the std_case
+#                   loop with GeneralizedNewtonian, LBrheology.h:321-476)
+#                   with its hard-coded input directory switched to ./../
+#                   (the std_case layout), its body force read the way
+#                   std_case/main.cpp:59 reads it (the shipped
+#                   `bodyForce.set(0, 0) = input[...]` line does not compile
+#                   against the current Input class: ambiguous operator=),
+#                   and, for D3Q19, its lattice line
 #   std_case_gpu_d2q9, std_case_gpu_d3q19, twophase_gpu
 #                   tests/integration/{std_case,twophase}_gpu.cpp: the
 #                   reference's set-up classes driving this repository's
@@ -70,6 +79,22 @@ for lat in D2Q9 D3Q19; do
   if cmp -s "$f.new" "$f"; then rm "$f.new"; else mv "$f.new" "$f"; fi
 done
 
+# the generalized-Newtonian case with a relative input directory
+for lat in D2Q9 D3Q19; do
+  f="$OUT/rheology_$(echo $lat | tr 'A-Z' 'a-z').cpp"
+  sed -e "s/^#define LT D2Q9/#define LT $lat/" \
+      -e 's|std::string chimpDir = "/home/[^"]*";|std::string chimpDir = "./../";|' \
+      -e '/^    VectorField<LT> bodyForce(1, 1);$/d' \
+      -e 's|^    bodyForce.set(0, 0) = input\["fluid"\]\["bodyforce"\];|    VectorField<LT> bodyForce(1, 1, input["fluid"]["bodyforce"]);|' \
+      "$REF/polymer_viscosity_tabulation/main.cpp" > "$f.new"
+  if ! grep -q 'chimpDir = "./../"' "$f.new" || ! grep -q "GeneralizedNewtonian" "$f.new" ||
+     ! grep -q 'VectorField<LT> bodyForce(1, 1, input' "$f.new"; then
+    echo "build_ref: the input-directory switch did not apply to polymer_viscosity_tabulation/main.cpp" >&2
+    exit 1
+  fi
+  if cmp -s "$f.new" "$f"; then rm "$f.new"; else mv "$f.new" "$f"; fi
+done
+
 build() {  # name, main source, extra include dir
   local name=$1 src=$2 inc=$3
   if [ ! -x "$OUT/$name" ] || [ "$src" -nt "$OUT/$name" ]; then
@@ -82,6 +107,8 @@ build std_case_d3q19 "$OUT/std_case_d3q19.cpp" "$REF/std_case"
 build twophase "$REF/twophase/main_TWOPHASE.cpp" "$REF/twophase"
 build std_case_trt_d2q9 "$OUT/std_case_trt_d2q9.cpp" "$REF/std_case"
 build std_case_trt_d3q19 "$OUT/std_case_trt_d3q19.cpp" "$REF/std_case"
+build rheology_d2q9 "$OUT/rheology_d2q9.cpp" "$REF/polymer_viscosity_tabulation"
+build rheology_d3q19 "$OUT/rheology_d3q19.cpp" "$REF/polymer_viscosity_tabulation"
 
 # the drop-in demonstration (test infrastructure): reference set-up + shim
 REPO="$(cd "$HERE/.." && pwd)"

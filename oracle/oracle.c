@@ -18,6 +18,7 @@
  *   calcDeltaOmegaQ            LBcollision.h:79-98
  *   calcDeltaOmegaF            LBcollision.h:194-213
  *   calcDeltaOmegaFTRT         LBcollision.h:214-233
+ *   GeneralizedNewtonian       LBrheology.h:321-476 (polymer_viscosity_tabulation/main.cpp:105-155)
  *   calcDeltaOmegaST / RC      LBcollision2phase.h:7-20, 70-86
  *   grad / vecNorm             LButilities.h:12-22, 62-66
  *   std_case loop              std_case/main.cpp:109-145
@@ -292,6 +293,127 @@ int orc_std_case_trt(int lattice, int32_t n_nodes, const int32_t *nb, const int1
                      int nthreads) {
   return std_case_run(lattice, n_nodes, nb, type, bulk, n_bulk, f, ftmp, tau_sym, 1, tau_anti, F, nsteps,
                       rho, vel, nthreads);
+}
+
+/* ------------------------------------------------------------ rheology */
+/* GeneralizedNewtonian (LBrheology.h:321-476): table with a constant
+ * extension below and above (:370-384), viscosity by linear interpolation
+ * of the strain-rate square, located with upper_bound over the inner
+ * entries (:459-462). */
+typedef struct {
+  int n;          /* table_length + 2 */
+  double *sr, *nu;
+} visc_table_t;
+
+static visc_table_t table_setup(int32_t m, const double *sr, const double *nu) {
+  visc_table_t t;
+  t.n = m + 2;
+  t.sr = malloc(sizeof(double) * t.n);
+  t.nu = malloc(sizeof(double) * t.n);
+  for (int i = 0; i < m; ++i) {
+    t.sr[i + 1] = sr[i];
+    t.nu[i + 1] = nu[i];
+  }
+  t.sr[0] = t.sr[1] - 1;
+  t.nu[0] = t.nu[1];
+  t.sr[m + 1] = t.sr[m] + 1;
+  t.nu[m + 1] = t.nu[m];
+  return t;
+}
+
+/* std::upper_bound(sr + 1, sr + n - 1, v) - (sr + 1), libstdc++'s loop */
+static int upper_index(const visc_table_t *t, double v) {
+  int first = 1, len = t->n - 2;
+  while (len > 0) {
+    const int half = len >> 1, mid = first + half;
+    if (v < t->sr[mid]) {
+      len = half;
+    } else {
+      first = mid + 1;
+      len = len - half - 1;
+    }
+  }
+  return first - 1;
+}
+
+/* nsteps iterations of polymer_viscosity_tabulation/main.cpp:105-155: the
+ * std_case loop with GeneralizedNewtonian::omegaBGK (source 0) and the Guo
+ * term at the node's relaxation time; visc: the node viscosities of the
+ * last iteration. */
+int orc_rheology(int lattice, int32_t n_nodes, const int32_t *nb, const int16_t *type, const int32_t *bulk,
+                 int32_t n_bulk, double *f, double *ftmp, int32_t n_table, const double *sr, const double *nu,
+                 const double *F, int32_t nsteps, double *rho, double *vel, double *visc, int nthreads) {
+  const lattice_t L = get_lattice(lattice);
+  const int Q = L.nQ, D = L.nD;
+  int32_t *fb = malloc(sizeof(int32_t) * (n_nodes ? n_nodes : 1));
+  int nfb = 0;
+  for (int32_t n = 1; n < n_nodes; ++n)
+    if (type[n] == 2) fb[nfb++] = n;
+  bb_t bb = bb_setup(&L, fb, nfb, nb, type);
+  free(fb);
+  visc_table_t tab = table_setup(n_table, sr, nu);
+  double cF[19];
+  cdotall(&L, F, cF);
+  double *a = f, *b = ftmp;
+  (void)nthreads;
+  for (int32_t it = 0; it < nsteps; ++it) {
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int32_t k = 0; k < n_bulk; ++k) {
+      const int32_t node = bulk[k];
+      const double *fn = a + (int64_t)node * Q;
+      const double rhoN = qsum(&L, fn);
+      double j[3], u[3], cu[19], feq[19], S[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      qsumc(&L, fn, j);
+      for (int d = 0; d < D; ++d) u[d] = (j[d] + 0.5 * F[d]) / rhoN;
+      rho[node] = rhoN;
+      for (int d = 0; d < D; ++d) vel[(int64_t)node * D + d] = u[d];
+      const double u2 = dot(&L, u, u);
+      cdotall(&L, u, cu);
+      /* LBrheology.h:437-447 */
+      for (int q = 0; q < Q; ++q) {
+        feq[q] = L.w[q] * rhoN * (1 + c2Inv * cu[q] + c4Inv0_5 * (cu[q] * cu[q] - c2 * u2));
+        for (int i = 0; i < D; ++i) {
+          const int ci = L.c[q * D + i];
+          S[i + D * i] += (fn[q] - feq[q]) * (ci * ci - c2);
+          for (int jj = i + 1; jj < D; ++jj) S[i + D * jj] += (fn[q] - feq[q]) * ci * L.c[q * D + jj];
+        }
+      }
+      /* LBrheology.h:449-457, source = 0 */
+      const double source = 0.0;
+      double sq = 0;
+      for (int i = 0; i < D; ++i) {
+        S[i + D * i] += (u[i] * F[i]);
+        S[i + D * i] += 0.5 * u2 * source;
+        sq += S[i + D * i] * S[i + D * i];
+        for (int jj = i + 1; jj < D; ++jj) {
+          S[i + D * jj] += 0.5 * (u[i] * F[jj] + u[jj] * F[i]);
+          S[i + D * jj] += 0.5 * u[i] * u[jj] * source;
+          sq += 2 * S[i + D * jj] * S[i + D * jj];
+        }
+      }
+      /* LBrheology.h:459-465 */
+      const int i0 = upper_index(&tab, sq);
+      const double nuN = (tab.nu[i0 + 1] - tab.nu[i0]) * (sq - tab.sr[i0]) / (tab.sr[i0 + 1] - tab.sr[i0]) + tab.nu[i0];
+      const double tau = nuN * c2Inv / rhoN + 0.5;
+      const double tau_inv = 1.0 / tau;
+      visc[node] = nuN;
+      /* calcDeltaOmegaF at the node's tau (main.cpp:130-134) */
+      const double uF = dot(&L, u, F);
+      const double tau_factor = (1 - 0.5 / tau);
+      for (int q = 0; q < Q; ++q) {
+        const double om = -tau_inv * (fn[q] - feq[q]);
+        const double dF = L.w[q] * tau_factor * (c2Inv * cF[q] + c4Inv * (cF[q] * cu[q] - c2 * uF));
+        b[(int64_t)nb[(int64_t)node * Q + q] * Q + q] = fn[q] + om + dF;
+      }
+    }
+    double *t = a; a = b; b = t;
+    bb_apply(&L, &bb, a, 1, 0, nb);
+  }
+  if (a != f) memcpy(f, a, sizeof(double) * (size_t)n_nodes * Q);
+  bb_free(&bb);
+  free(tab.sr);
+  free(tab.nu);
+  return 0;
 }
 
 /* ------------------------------------------------------------ twophase */

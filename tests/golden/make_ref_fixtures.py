@@ -8,7 +8,8 @@ container only; /root/reference is not available elsewhere).  For every case:
      PythonScripts/vtklb.py, see make_vtklb_fixtures.py),
   2. an input.dat with the case parameters is written,
   3. the reference case (std_case D2Q9 / D3Q19, its two-relaxation-time
-     variant, or twophase) runs with
+     variant, the generalized-Newtonian polymer_viscosity_tabulation, or
+     twophase) runs with
      iterations.write = iterations.max, so it writes VTK at i = 0 and i = max,
   4. the VTU of i = max is parsed: rho (rho0, rho1) and vel on the bulk nodes,
      in bulk order, i.e. the fields of the pre-collision state after max
@@ -84,7 +85,8 @@ def keep_output(outdir, base, name):
     shutil.copyfile(os.path.join(outdir, "%s_%s.pvtu" % (base, n[:-4])), os.path.join(dst, name + ".pvtu"))
 
 
-def run_case(binary, geo, lattice, periodic, attrs, dat, twophase=False, outsub="tp/", keep=None):
+def run_case(binary, geo, lattice, periodic, attrs, dat, twophase=False, outsub="tp/", keep=None,
+             extra_inputs=None):
     tmp = tempfile.mkdtemp(prefix="lbref_")
     try:
         g = Geometry(geo, lattice, periodic)
@@ -98,6 +100,9 @@ def run_case(binary, geo, lattice, periodic, attrs, dat, twophase=False, outsub=
             fh.write(g.vtklb_text(1, attrs))
         with open(os.path.join(root, "input", "input.dat"), "w") as fh:
             fh.write(dat)
+        for fname, text in (extra_inputs or {}).items():
+            with open(os.path.join(root, "input", fname), "w") as fh:
+                fh.write(text)
         cwd = root if twophase else os.path.join(tmp, "run")
         os.makedirs(cwd, exist_ok=True)
         env = dict(os.environ, LD_LIBRARY_PATH=os.path.join(REF, "mpilib"), OMP_NUM_THREADS="1")
@@ -178,6 +183,35 @@ def trt_cases():
         [1e-5, 0.0, 0.0], 200
 
 
+def carreau_table(n=40, lo=1e-16, hi=1e-4, nu0=1.0 / 6.0, nu_inf=0.02, lam2=1e10, power=0.5):
+    """Viscosity against the strain-rate square sum_ij E_ij^2 (the table
+    GeneralizedNewtonian reads, LBrheology.h:358-388): a shear-thinning
+    Carreau curve sampled log-uniformly, printed with 17 digits."""
+    sr = np.logspace(np.log10(lo), np.log10(hi), n)
+    nu = nu_inf + (nu0 - nu_inf) * (1.0 + lam2 * sr) ** ((power - 1.0) / 2.0)
+    text = "%d\n" % n + "".join("%.17g %.17g\n" % (a, b) for a, b in zip(sr, nu))
+    table = np.array([[float(t) for t in line.split()] for line in text.splitlines()[1:]])
+    return text, table
+
+
+def rheo_cases():
+    """polymer_viscosity_tabulation (oracle/_ref/rheology_*):
+    (name, lattice, geo, periodic, attrs, force, max_it, table kwargs)"""
+    g = np.ones((48, 24), dtype=np.int64)
+    g[:, 0] = 0
+    g[:, -1] = 0
+    yield "rheo_d2q9_channel_48x24", "D2Q9", g, "x", [("init_rho", np.ones(g.shape))], [2e-5, 0.0], 400, {}
+    g = load_geo("d2q9_random_7x5_pxy")
+    r = np.load(os.path.join(VT, "d2q9_random_7x5_pxy_attr_init_rho.npy"))
+    yield "rheo_d2q9_random_7x5", "D2Q9", g, "xy", [("init_rho", r)], [1e-4, -3e-5], 60, {"lam2": 1e8}
+    g = load_geo("d3q19_pack_6x5x4_px_walls")
+    yield "rheo_d3q19_pack_6x5x4_px_walls", "D3Q19", g, "x", [("init_rho", np.ones(g.shape))], [1e-4, 0.0, 0.0], \
+        40, {"lam2": 1e9}
+    g, _ = sphere_geo((20, 20, 20), 4.0, 0.5, 7, "x", True)
+    yield "rheo_d3q19_pack_20", "D3Q19", g, "x", [("init_rho", np.ones(g.shape))], [5e-5, 1e-5, 0.0], 150, \
+        {"lam2": 1e9}
+
+
 def tp_cases():
     """(name, geo, periodic, rho0, wett, tau0, tau1, sigma, beta, momx, force, max_it)"""
     rng = np.random.default_rng(99)
@@ -230,6 +264,18 @@ def main(selected):
                             params=json.dumps(params), init_rho=attrs[0][1].astype(np.float64),
                             rho=data["rho"], vel=data["vel"])
         print("wrote ref_%s.npz" % name, data["rho"].shape)
+    for name, lat, g, per, attrs, force, max_it, tk in rheo_cases():
+        if selected and name not in selected:
+            continue
+        text, table = carreau_table(**tk)
+        binary = "rheology_d2q9" if lat == "D2Q9" else "rheology_d3q19"
+        dat = input_dat(max_it, ["tau 1.0", "bodyforce %s" % fmt(force)])
+        data = run_case(binary, g, lat, per, attrs, dat, extra_inputs={"test.dat": text})
+        params = dict(case="rheology", lattice=lat, periodic=per, force=force, steps=max_it)
+        np.savez_compressed(os.path.join(HERE, "ref_%s.npz" % name), geo=g.astype(np.int8),
+                            params=json.dumps(params), init_rho=attrs[0][1].astype(np.float64), table=table,
+                            rho=data["rho"], vel=data["vel"], viscosity=data["viscosity"])
+        print("wrote ref_%s.npz" % name, data["rho"].shape, data["viscosity"].min(), data["viscosity"].max())
     for name, g, per, r0, wett, tau0, tau1, sigma, beta, momx, force, max_it in tp_cases():
         if selected and name not in selected:
             continue

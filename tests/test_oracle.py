@@ -96,6 +96,77 @@ def test_trt_magic_parameter_gives_exact_poiseuille(tau_sym, tau_anti):
     assert np.abs(vel[T.bulk, 1]).max() <= 1e-12 * exact.max()
 
 
+@pytest.mark.parametrize("name", ref_names("rheo_"))
+def test_rheology_bitexact_vs_compiled_reference(name):
+    """polymer_viscosity_tabulation (GeneralizedNewtonian) as compiled from the
+    reference by oracle/build_ref.sh: rho, vel and the node viscosities."""
+    d = load_ref(name)
+    p = d["params"]
+    T = OracleTopo(d["geo"].astype(np.int64), p["lattice"], p["periodic"])
+    f0 = oracle.rest_state(T.lattice, T.n_nodes, T.bulk, T.attribute(d["init_rho"]))
+    f, rho, vel, visc = oracle.run_rheology(T.lattice, T.neighbors, T.node_types, T.bulk, f0, d["table"],
+                                            np.array(p["force"]), p["steps"] + 1)
+    b = T.bulk
+    assert np.array_equal(rho[b], d["rho"])
+    assert np.array_equal(vel[b], d["vel"][:, :vel.shape[1]])
+    assert np.array_equal(visc[b], d["viscosity"])
+
+
+def test_rheology_constant_table_is_bgk():
+    """A table of one viscosity nu: tau = 3 nu / rho + 1/2 at every node, BGK
+    with tau = 3 nu + 1/2 up to the density's departure from 1."""
+    rng = np.random.default_rng(4)
+    geo = (rng.random((14, 11, 9)) > 0.3).astype(np.int64)
+    T = OracleTopo(geo, "D3Q19", "xyz")
+    f0 = oracle.rest_state(1, T.n_nodes, T.bulk, np.ones(T.n_nodes))
+    F = np.array([2e-5, 0.0, 1e-5])
+    nu = 0.1
+    table = np.array([[1e-12, nu], [1e-6, nu]])
+    fr, rr, vr, visc = oracle.run_rheology(1, T.neighbors, T.node_types, T.bulk, f0, table, F, 60)
+    fb, rb, vb = oracle.run_std_case(1, T.neighbors, T.node_types, T.bulk, f0, 3 * nu + 0.5, F, 60)
+    b = T.bulk
+    assert np.all(visc[b] == nu)
+    drho = np.abs(rb[b] - 1).max()
+    assert 0 < drho < 1e-3
+    assert np.abs(vr[b] - vb[b]).max() <= 10 * drho * np.abs(vb[b]).max()
+
+
+def test_rheology_channel_stress_balance():
+    """Steady generalized-Newtonian channel flow: the node viscosities
+    satisfy mu(y) du/dy = F (y_c - y), the momentum balance, to the
+    discretisation error; a shear-thinning fluid is least viscous at the
+    walls.  The table is the Carreau curve of the fixtures."""
+    import sys
+    from helpers import GOLDEN
+    sys.path.insert(0, GOLDEN)
+    from make_ref_fixtures import carreau_table
+    _, table = carreau_table()
+    H = 22
+    geo = np.ones((4, H + 2), dtype=np.int64)
+    geo[:, 0] = geo[:, -1] = 0
+    T = OracleTopo(geo, "D2Q9", "x")
+    f0 = oracle.rest_state(0, T.n_nodes, T.bulk, np.ones(T.n_nodes))
+    F = 2e-5
+    f, rho, vel, visc = oracle.run_rheology(0, T.neighbors, T.node_types, T.bulk, f0, table, np.array([F, 0.0]),
+                                            40000)
+    y = T.positions[T.bulk, 1]
+    col = T.positions[T.bulk, 0] == 0
+    yy, u, mu = y[col], vel[T.bulk, 0][col], visc[T.bulk][col]
+    s = np.argsort(yy)
+    yy, u, mu = yy[s], u[s], mu[s]
+    # fluid rows y = 1..H, half-way walls at 1/2 and H + 1/2; interior faces
+    # between rows k and k+1: stress mu du/dy against F (y_c - y_face)
+    yc = (H + 1) / 2.0
+    yface = 0.5 * (yy[1:] + yy[:-1])
+    stress = 0.5 * (mu[1:] + mu[:-1]) * np.diff(u)
+    exact = F * (yc - yface)
+    # the face viscosity is the mean of its two nodes: a few per cent where mu
+    # changes fastest (the Carreau knee near the centre)
+    assert np.abs(stress - exact).max() <= 0.05 * np.abs(exact).max()
+    assert mu[H // 2] > mu[0] and mu[H // 2] > mu[-1]
+    assert np.allclose(mu, mu[::-1], rtol=0, atol=1e-12)
+
+
 def tp_start(d, T):
     r0 = float_round(T.attribute(d["rho0_attr"]))
     r1 = float_round(T.attribute(1.0 - d["rho0_attr"]))
