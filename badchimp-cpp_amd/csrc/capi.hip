@@ -57,6 +57,17 @@ int dev_alloc(lb_ctx* c, T** p, size_t count) {
   return LB_OK;
 }
 
+void free_tags(lb_ctx* c) {
+  for (void* p : {static_cast<void*>(c->d_tlink), static_cast<void*>(c->d_tabb), static_cast<void*>(c->d_tinfo),
+                  static_cast<void*>(c->d_tfon), static_cast<void*>(c->d_tscale), static_cast<void*>(c->d_tpartial)})
+    if (p) hipFree(p);
+  c->d_tlink = c->d_tabb = c->d_tinfo = nullptr;
+  c->d_tfon = c->d_tscale = c->d_tpartial = nullptr;
+  c->tag_ready = false;
+  c->tag_labels = 0;
+  c->tag_pressure.clear();
+}
+
 void free_fields(lb_ctx* c) {
   for (auto& p : c->d_f) {
     if (p) hipFree(p);
@@ -335,6 +346,7 @@ int setup_topology(lb_ctx* c, int32_t n_nodes, const int32_t* neighbor, const in
   // device allocation
   hipStreamSynchronize(c->stream);
   free_fields(c);
+  free_tags(c);
   for (void* p : {static_cast<void*>(c->d_coord), static_cast<void*>(c->d_rows), static_cast<void*>(c->d_win)})
     if (p) hipFree(p);
   c->d_coord = nullptr;
@@ -434,6 +446,7 @@ int lb_ctx_destroy(lb_ctx* c) {
   hipSetDevice(c->device);
   hipStreamSynchronize(c->stream);
   free_fields(c);
+  free_tags(c);
   for (void* p : {static_cast<void*>(c->d_link), static_cast<void*>(c->d_rho), static_cast<void*>(c->d_vel),
                   static_cast<void*>(c->d_phi), static_cast<void*>(c->d_phi_wall), static_cast<void*>(c->d_partial),
                   static_cast<void*>(c->d_scalar), static_cast<void*>(c->d_coord),
@@ -769,6 +782,168 @@ int lb_step_rheology(lb_ctx* c, int32_t nsteps, const double* force) {
   a.tab_srf = reinterpret_cast<const float*>(c->d_table + 2 * c->tab_n);
   a.tab_nuf = a.tab_srf + c->tab_n;
   return step_single(c, nsteps, a);
+}
+
+// ------------------------------------------------------------ tagged boundaries
+int lb_set_node_tags(lb_ctx* c, int32_t n_nodes, const int32_t* neighbor, const int32_t* tags,
+                     const int32_t* interior_domains, const double* force_on) {
+  if (!c) return LB_ERR_ARG;
+  if (!c->has_topo) return fail(c, LB_ERR_STATE, "no topology: call lb_set_topology first");
+  if (c->precision != LB_FP64)
+    return fail(c, LB_ERR_UNSUPPORTED, "tagged boundaries are implemented for LB_FP64 contexts");
+  if (!neighbor || !tags || !interior_domains || !force_on || n_nodes != c->n_nodes)
+    return fail(c, LB_ERR_ARG, "lb_set_node_tags: bad argument (arrays over the %d nodes of the topology)",
+                c->n_nodes);
+  const int Q = c->Q;
+  const int64_t NP = c->NP;
+  std::vector<int32_t> compact(n_nodes, -1);
+  for (int32_t i = 0; i < c->n_bulk; ++i) compact[c->node_of[i]] = i;
+  auto tag = [&](int32_t n) { return static_cast<int>(static_cast<int16_t>(tags[n])); };  // Nodes keeps a short
+  std::vector<uint32_t> link(static_cast<size_t>(Q - 1) * NP, 0u), abb(NP, 0u), info(NP, 0u);
+  std::vector<double> fon(NP, 0.0);
+  for (int32_t i = 0; i < c->n_bulk; ++i) {
+    const int32_t n = c->node_of[i];
+    const int32_t* nb = neighbor + static_cast<size_t>(n) * Q;
+    const int t = tag(n);
+    for (int q = 0; q < Q - 1; ++q) {
+      const int qo = rev_of(c->lattice, q);  // outgoing direction whose neighbour sends into q
+      const int32_t s = nb[qo];
+      if (s < 0 || s >= n_nodes) return fail(c, LB_ERR_ARG, "neighbour index out of range");
+      const int ts = tag(s);
+      // the lists of std_one_phase/main.cpp:27-126, seen from the node that receives
+      const bool solid_fluid = ((t >> 3) & 1) && (ts & 3) == 0;
+      const bool pressure = ((t >> 4) & 1) && (ts & 3) == 3;
+      const bool fluid_fluid = (((t >> 2) & 1) && (t & 3) == 1 && (ts & 3) == 2) ||
+                               ((t & 3) == 2 && ((ts >> 2) & 1) && (ts & 3) == 1);
+      uint32_t l;
+      if (s == 0 && (solid_fluid || pressure || fluid_fluid || compact[s] < 0))
+        return fail(c, LB_ERR_UNSUPPORTED, "node %d links to the ghost node 0 (direction %d)", n, qo);
+      if (pressure) {
+        if (compact[s] >= 0)
+          return fail(c, LB_ERR_UNSUPPORTED, "pressure link of node %d to node %d, which is a bulk node", n, s);
+        abb[i] |= 1u << qo;
+        l = kLinkBounce << 30;
+      } else if (solid_fluid || fluid_fluid) {
+        l = kLinkBounce << 30;
+      } else if (compact[s] >= 0) {
+        l = (kLinkStream << 30) | static_cast<uint32_t>(compact[s]);
+      } else {
+        return fail(c, LB_ERR_UNSUPPORTED,
+                    "node %d (tag %d) receives direction %d from non-bulk node %d (tag %d) with no boundary tag: "
+                    "the reference leaves that population stale",
+                    n, t, q, s, ts);
+      }
+      link[static_cast<size_t>(q) * NP + i] = l;
+    }
+  }
+  // interior domains and their mass-source scale factors (main.cpp:322-354)
+  int lmax = 0;
+  for (int32_t n = 1; n < n_nodes; ++n) {
+    if (interior_domains[n] < 0) return fail(c, LB_ERR_ARG, "negative interior domain label at node %d", n);
+    lmax = std::max(lmax, interior_domains[n]);
+  }
+  std::vector<double> scale(2 * static_cast<size_t>(lmax + 1), 0.0);
+  for (int32_t n = 1; n < n_nodes; ++n)
+    if (interior_domains[n] > 0 && tag(n) < 3) scale[interior_domains[n]] += 1;
+  for (int l = 1; l <= lmax; ++l) {
+    if (scale[l] == 0.0) return fail(c, LB_ERR_ARG, "Interior domain %d has no interior nodes", l);
+    scale[l] = 1.0 / scale[l];
+  }
+  for (int32_t i = 0; i < c->n_bulk; ++i) {
+    const int32_t n = c->node_of[i];
+    const bool add = interior_domains[n] > 0 && tag(n) < 3;
+    info[i] = static_cast<uint32_t>(interior_domains[n]) | (add ? 0x80000000u : 0u);
+    fon[i] = force_on[n];
+  }
+  std::vector<std::pair<int32_t, int>> pres;
+  for (int32_t n = 1; n < n_nodes; ++n)
+    if ((tag(n) >> 4) & 1) pres.emplace_back(n, (tag(n) & 3) - 1);
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  free_tags(c);
+  int rc;
+  if ((rc = dev_alloc(c, &c->d_tlink, link.size())) != LB_OK) return rc;
+  if ((rc = dev_alloc(c, &c->d_tabb, abb.size())) != LB_OK) return rc;
+  if ((rc = dev_alloc(c, &c->d_tinfo, info.size())) != LB_OK) return rc;
+  if ((rc = dev_alloc(c, &c->d_tfon, fon.size())) != LB_OK) return rc;
+  if ((rc = dev_alloc(c, &c->d_tscale, scale.size())) != LB_OK) return rc;
+  if ((rc = dev_alloc(c, &c->d_tpartial, static_cast<size_t>(std::max(lmax, 1)) * c->n_blocks)) != LB_OK) return rc;
+  LB_HIP(c, hipMemcpy(c->d_tlink, link.data(), link.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  LB_HIP(c, hipMemcpy(c->d_tabb, abb.data(), abb.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  LB_HIP(c, hipMemcpy(c->d_tinfo, info.data(), info.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  LB_HIP(c, hipMemcpy(c->d_tfon, fon.data(), fon.size() * sizeof(double), hipMemcpyHostToDevice));
+  LB_HIP(c, hipMemcpy(c->d_tscale, scale.data(), scale.size() * sizeof(double), hipMemcpyHostToDevice));
+  c->tag_labels = lmax;
+  c->tag_pressure = std::move(pres);
+  c->tag_ready = true;
+  return LB_OK;
+}
+
+int lb_step_tagged(lb_ctx* c, int32_t nsteps, double tau, const double* force, double rho_w) {
+  if (!c) return LB_ERR_ARG;
+  int rc = check_ready(c, 1);
+  if (rc != LB_OK) return rc;
+  if (nsteps < 0 || !force || !(tau > 0.0)) return fail(c, LB_ERR_ARG, "lb_step_tagged: bad argument");
+  if (!c->tag_ready) return fail(c, LB_ERR_STATE, "lb_step_tagged: no node tags (lb_set_node_tags)");
+  if (nsteps == 0) return LB_OK;
+  hipSetDevice(c->device);
+  const BgkArgs a = single_phase_args(c, tau, 0.0, force);
+  const TableAddr ad{c->d_tlink, static_cast<uint32_t>(c->NP)};
+  const int L = c->tag_labels;
+  double* mass = c->d_tscale + (L + 1);  // mass[0] stays 0 (its scale factor is 0)
+  Timer t(c);
+  int64_t launches = 0;
+  for (int32_t s = 0; s < nsteps; ++s) {
+    const bool last = s == nsteps - 1;
+    const double* fin = static_cast<const double*>(c->d_f[c->cur]);
+    hipError_t e = hipSuccess;
+    if (L > 0) {
+      // per-domain mass changes of the state this step starts from (main.cpp:527-535)
+      e = tag_mass_f64(c->lattice, !c->post, fin, ad, c->n_bulk, c->NP, c->d_tinfo, L, c->d_tpartial, c->stream);
+      if (e == hipSuccess) e = sum_partials(c->d_tpartial, c->n_blocks, L, mass + 1, c->stream);
+      launches += 2;
+    }
+    if (e == hipSuccess)
+      e = tagged_f64(c->lattice, !c->post, last, fin, static_cast<double*>(c->d_f[1 - c->cur]), ad, c->n_bulk,
+                     c->NP, a, rho_w, c->d_tinfo, c->d_tfon, c->d_tabb, c->d_tscale, mass, c->d_rho, c->d_vel,
+                     c->stream);
+    if (e != hipSuccess) return fail(c, LB_ERR_HIP, "tagged step launch: %s", hipGetErrorString(e));
+    c->cur = 1 - c->cur;
+    c->post = true;
+    ++launches;
+  }
+  rc = t.stop(launches);
+  if (rc != LB_OK) return rc;
+  c->have_macro = true;
+  c->have_visc = false;
+  return LB_OK;
+}
+
+int lb_pressure_flux(lb_ctx* c, double* q) {
+  if (!c) return LB_ERR_ARG;
+  if (!q) return fail(c, LB_ERR_ARG, "null output");
+  if (!c->tag_ready) return fail(c, LB_ERR_STATE, "no node tags (lb_set_node_tags)");
+  if (!c->have_macro) return fail(c, LB_ERR_STATE, "no step has stored macroscopic fields yet");
+  if (c->nD != 3) return fail(c, LB_ERR_UNSUPPORTED, "the mass flux is the z component (D3Q19)");
+  hipSetDevice(c->device);
+  const int64_t NP = c->NP;
+  std::vector<double> rho(NP), vz(NP);
+  LB_HIP(c, hipMemcpy(rho.data(), c->d_rho, NP * sizeof(double), hipMemcpyDeviceToHost));
+  LB_HIP(c, hipMemcpy(vz.data(), c->d_vel + 2 * NP, NP * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<int32_t> compact(c->n_nodes, -1);
+  for (int32_t i = 0; i < c->n_bulk; ++i) compact[c->node_of[i]] = i;
+  // main.cpp:600-612: ascending node order, per phase, halved
+  double m[2] = {0.0, 0.0};
+  for (const auto& pr : c->tag_pressure) {
+    if (pr.second != 0 && pr.second != 1)
+      return fail(c, LB_ERR_STATE, "Fluid phase = %d in write mass flux (node %d)", pr.second, pr.first);
+    const int32_t i = compact[pr.first];
+    if (i < 0) return fail(c, LB_ERR_STATE, "pressure-boundary node %d is not a bulk node", pr.first);
+    m[pr.second] += vz[i] * rho[i];
+  }
+  q[0] = 0.5 * m[0];
+  q[1] = 0.5 * m[1];
+  return LB_OK;
 }
 
 int lb_step_trt(lb_ctx* c, int32_t nsteps, double tau_sym, double tau_anti, const double* force) {

@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "kernels.hpp"
@@ -46,6 +47,18 @@ struct lb_ctx {
   int64_t n_blocks = 0;
   bool have_macro = false;
   bool have_visc = false;        // the last stored step was a rheology step
+  // tagged boundaries (std_one_phase): pull table, anti-bounce-back masks,
+  // domain labels with mass-source flags, force attribute, per-domain
+  // scale factors and mass changes, partials of the mass pass
+  bool tag_ready = false;
+  int tag_labels = 0;            // highest interior-domain label
+  uint32_t* d_tlink = nullptr;   // [Q-1][NP]
+  uint32_t* d_tabb = nullptr;    // [NP]
+  uint32_t* d_tinfo = nullptr;   // [NP]
+  double* d_tfon = nullptr;      // [NP]
+  double* d_tscale = nullptr;    // [labels + 1], then mass changes [labels + 1]
+  double* d_tpartial = nullptr;  // [labels][n_blocks]
+  std::vector<std::pair<int32_t, int>> tag_pressure;  // pressure-boundary nodes (ascending) and phase - 1
   double* d_table = nullptr;     // viscosity table: sr[tab_n], nu[tab_n] (FP64), then the same in float
   int32_t tab_n = 0;             // rows + 2 (constant extension at both ends)
   double last_force[3] = {0, 0, 0};

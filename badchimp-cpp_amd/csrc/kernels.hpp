@@ -46,6 +46,18 @@ hipError_t flux_f64(int lattice, int nfields, bool first, const double* fin, con
 hipError_t init_eq_f64(int lattice, double* f, int field, int32_t n, int64_t NP,
                        const double* rho, const double* vel, hipStream_t s);
 
+// ---- tagged boundaries (std_one_phase/main.cpp:27-204, 513-575) -------------
+// One step with the tag-derived pull table ad, the anti-bounce-back
+// direction masks abb, the domain labels/mass-source flags info, the force
+// attribute fon and the per-domain scale factors and mass changes.
+hipError_t tagged_f64(int lattice, bool first, bool store, const double* fin, double* fout, const TableAddr& ad,
+                      int32_t n, int64_t NP, const BgkArgs& a, double rho_w, const uint32_t* info,
+                      const double* fon, const uint32_t* abb, const double* scale, const double* mass, double* rho,
+                      double* vel, hipStream_t s);
+// Block partials of sum (1 - rho) per domain label 1..n_label (partial[(l-1)*nblocks + b]).
+hipError_t tag_mass_f64(int lattice, bool first, const double* fin, const TableAddr& ad, int32_t n, int64_t NP,
+                        const uint32_t* info, int n_label, double* partial, hipStream_t s);
+
 // ---- batched form -----------------------------------------------------------
 // One FP64 step of every solve listed in items (see k_batch_bgk): grid
 // workgroups of 256 threads, each over a run of per_wg items; reduce writes

@@ -265,6 +265,96 @@ __global__ __launch_bounds__(kBlock) void k_batch_rest(double* __restrict__ f, i
   f[e] = L::w(q);
 }
 
+// ------------------------------------------------------------------ tagged boundaries
+// std_one_phase/main.cpp:513-575 with its boundary lists (main.cpp:27-204):
+// the pull table t.link carries the links the tags make bounce-back (own
+// reversed slot) beside the streaming ones; abb[i] marks the outgoing
+// directions whose post-collision value becomes the anti-bounce-back value
+// -f + 2 w rho_w (1 + 0.5 (9 cu^2 - 3 u^2)) (main.cpp:170-173), which the
+// node then reads back through its bounce-back link.  info[i]: interior
+// domain label (bits 0-30) and the mass-source flag (bit 31); fon[i]: the
+// force attribute.
+
+// Per-domain sums of 1 - rho over the pre-collision state, one block partial
+// per domain label 1..n_label (main.cpp:527-535).
+template <class L, bool FIRST>
+__global__ __launch_bounds__(kBlock) void k_tag_mass(const double* __restrict__ fin, TableAddr ad, int32_t n,
+                                                     int64_t NP, const uint32_t* __restrict__ info, int n_label,
+                                                     double* __restrict__ partial) {
+  const int32_t b = static_cast<int32_t>(blockIdx.x);
+  const int32_t i = b * kBlock + threadIdx.x;
+  double v = 0.0;
+  int lab = 0;
+  if (i < n) {
+    double f[1][L::nQ];
+    load_node<L, 1, FIRST>(fin, ad, i, NP, f);
+    v = 1.0 - qsum<L>(f[0]);
+    lab = static_cast<int>(info[i] & 0x7fffffffu);
+  }
+  for (int l = 1; l <= n_label; ++l) {
+    const double s = block_sum(lab == l ? v : 0.0);
+    if (threadIdx.x == 0) partial[static_cast<int64_t>(l - 1) * gridDim.x + b] = s;
+  }
+}
+
+template <class L, bool FIRST, bool STORE>
+__global__ __launch_bounds__(kBlock) void k_tagged(const double* __restrict__ fin, double* __restrict__ fout,
+                                                   TableAddr ad, int32_t n, int64_t NP, BgkArgs a, double rho_w,
+                                                   const uint32_t* __restrict__ info,
+                                                   const double* __restrict__ fon,
+                                                   const uint32_t* __restrict__ abb,
+                                                   const double* __restrict__ scale,
+                                                   const double* __restrict__ mass, double* __restrict__ rho_o,
+                                                   double* __restrict__ vel_o) {
+  constexpr int Q = L::nQ, D = L::nD;
+  const int32_t i = static_cast<int32_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double f[1][Q];
+  load_node<L, 1, FIRST>(fin, ad, i, NP, f);
+  // calcRho plus half the mass source (main.cpp:546-552)
+  const uint32_t inf = info[i];
+  const int lab = static_cast<int>(inf & 0x7fffffffu);
+  const double add = (inf >> 31) ? 1.0 : 0.0;
+  const double qm = (((0.9 * 2) * scale[lab]) * mass[lab]) * add;
+  const double rho = qsum<L>(f[0]) + 0.5 * qm;
+  // force = bodyforce * force attribute (main.cpp:554)
+  const double fo = fon[i];
+  double F[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) F[d] = a.F[d] * fo;
+  double u[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) u[d] = (qsumc<L>(f[0], d) + 0.5 * F[d]) / rho;
+  if (STORE) {
+    rho_o[i] = rho;
+#pragma unroll
+    for (int d = 0; d < D; ++d) vel_o[d * NP + i] = u[d];
+  }
+  const double u2 = dot<L>(u, u);
+  const double uF = dot<L>(u, F);
+  const double c2u2 = L::c2 * u2;
+  const double c2uF = L::c2 * uF;
+  const uint32_t anti = abb[i];
+  double* __restrict__ out = elem_ptr<false>(fout, pidx<Q>(0, i));
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const double cu = cdot<L>(q, u);
+    const double cF = cdot<L>(q, F);
+    const double feq = (rho * L::w(q)) * ((1.0 + L::c2Inv * cu) + L::c4Inv0_5 * (cu * cu - c2u2));
+    const double om = (-a.tau_inv) * (f[0][q] - feq);
+    const double dF = (L::w(q) * a.tau_factor) * (L::c2Inv * cF + L::c4Inv * (cF * cu - c2uF));
+    // calcDeltaOmegaQ (LBcollision.h:91-95)
+    const double dQ = ((a.tau_factor * qm) * L::w(q)) * ((1.0 + L::c2Inv * cu) + L::c4Inv0_5 * (cu * cu - c2u2));
+    double v = ((f[0][q] + om) + dF) + dQ;
+    if ((anti >> q) & 1u) {
+      // -f(q, wall) + 2*w*rho_w*(1 + 0.5*(c4Inv*cu*cu - c2Inv*u2)) (main.cpp:173)
+      const double bnd = ((2 * L::w(q)) * rho_w) * (1 + 0.5 * ((L::c4Inv * cu) * cu - L::c2Inv * u2));
+      v = (-v) + bnd;
+    }
+    __builtin_nontemporal_store(v, out + (q << kPLog));
+  }
+}
+
 // ------------------------------------------------------------------ two-phase pass 1
 template <class L, bool FIRST, bool MOMX, class A>
 __global__ __launch_bounds__(kBlock) void k_cg_phi(const double* __restrict__ fin, A ad,
@@ -640,6 +730,32 @@ hipError_t batch_rest_f64(int lattice, double* f, int64_t NP, hipStream_t s) {
   LB_DISPATCH_LATTICE(lattice, L, {
     const int64_t n = static_cast<int64_t>(L::nQ) * NP;
     k_batch_rest<L><<<dim3(static_cast<unsigned>((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s>>>(f, NP);
+  });
+  return hipGetLastError();
+}
+
+hipError_t tagged_f64(int lattice, bool first, bool store, const double* fin, double* fout, const TableAddr& ad,
+                      int32_t n, int64_t NP, const BgkArgs& a, double rho_w, const uint32_t* info,
+                      const double* fon, const uint32_t* abb, const double* scale, const double* mass, double* rho,
+                      double* vel, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const dim3 g = grid_for(n), b(kBlock);
+  LB_DISPATCH_LATTICE(lattice, L, {
+    if (first && store) k_tagged<L, true, true><<<g, b, 0, s>>>(fin, fout, ad, n, NP, a, rho_w, info, fon, abb, scale, mass, rho, vel);
+    else if (first) k_tagged<L, true, false><<<g, b, 0, s>>>(fin, fout, ad, n, NP, a, rho_w, info, fon, abb, scale, mass, rho, vel);
+    else if (store) k_tagged<L, false, true><<<g, b, 0, s>>>(fin, fout, ad, n, NP, a, rho_w, info, fon, abb, scale, mass, rho, vel);
+    else k_tagged<L, false, false><<<g, b, 0, s>>>(fin, fout, ad, n, NP, a, rho_w, info, fon, abb, scale, mass, rho, vel);
+  });
+  return hipGetLastError();
+}
+
+hipError_t tag_mass_f64(int lattice, bool first, const double* fin, const TableAddr& ad, int32_t n, int64_t NP,
+                        const uint32_t* info, int n_label, double* partial, hipStream_t s) {
+  if (n <= 0 || n_label <= 0) return hipSuccess;
+  const dim3 g = grid_for(n), b(kBlock);
+  LB_DISPATCH_LATTICE(lattice, L, {
+    if (first) k_tag_mass<L, true><<<g, b, 0, s>>>(fin, ad, n, NP, info, n_label, partial);
+    else k_tag_mass<L, false><<<g, b, 0, s>>>(fin, ad, n, NP, info, n_label, partial);
   });
   return hipGetLastError();
 }

@@ -60,6 +60,9 @@ SIGNATURES = {
     "lb_step_bgk": (C.c_int, [_vp, C.c_int32, C.c_double, _f64p]),
     "lb_step_trt": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_double, _f64p]),
     "lb_set_viscosity_table": (C.c_int, [_vp, C.c_int32, _f64p, _f64p]),
+    "lb_set_node_tags": (C.c_int, [_vp, C.c_int32, _i32p, _i32p, _i32p, _f64p]),
+    "lb_step_tagged": (C.c_int, [_vp, C.c_int32, C.c_double, _f64p, C.c_double]),
+    "lb_pressure_flux": (C.c_int, [_vp, _f64p]),
     "lb_step_rheology": (C.c_int, [_vp, C.c_int32, _f64p]),
     "lb_step_cg": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double,
                              C.c_int32, _f64p]),
@@ -438,6 +441,24 @@ class Context:
         F = np.zeros(3)
         F[: self.nD] = force[: self.nD]
         self._check(lib().lb_step_rheology(self._h, int(nsteps), F))
+
+    def set_node_tags(self, neighbors, tags, interior_domains, force_on):
+        """std_one_phase's node tags, interior domains and force attribute
+        over all nodes (tags[0] = -1)."""
+        nb = np.ascontiguousarray(neighbors, dtype=np.int32)
+        self._check(lib().lb_set_node_tags(self._h, nb.shape[0], nb.ravel(), np.ascontiguousarray(tags, np.int32),
+                                           np.ascontiguousarray(interior_domains, np.int32),
+                                           np.ascontiguousarray(force_on, np.float64)))
+
+    def step_tagged(self, nsteps, tau, force, rho_w=1.0):
+        F = np.zeros(3)
+        F[: self.nD] = force[: self.nD]
+        self._check(lib().lb_step_tagged(self._h, int(nsteps), float(tau), F, float(rho_w)))
+
+    def pressure_flux(self):
+        out = np.zeros(2)
+        self._check(lib().lb_pressure_flux(self._h, out))
+        return out
 
     def step_cg(self, nsteps, tau0, tau1, sigma, beta, force_mode, force):
         F = np.zeros(3)

@@ -169,6 +169,28 @@ int lb_set_viscosity_table(lb_ctx* ctx, int32_t n, const double* strain_rate_sq,
                            const double* viscosity);
 int lb_step_rheology(lb_ctx* ctx, int32_t nsteps, const double* force);
 
+/* Tagged-link boundaries and per-domain mass sources of std_one_phase
+ * (main.cpp:27-204, 322-354, 513-612), one rank, FP64 contexts.
+ * tags[n]: the node tag (nodetags attribute, Nodes::getTag; tags[0] = -1
+ * as Nodes sets it): bits 0-1 phase (0 solid, 1, 2 fluid, 3 pressure
+ * ghost), bit 2 fluid-fluid interface, bit 3 fluid-solid interface, bit 4
+ * pressure boundary.  The links the reference's boundary lists rewrite
+ * become bounce-back (solid-fluid and fluid-fluid) or anti-bounce-back
+ * -f + 2 w rho_w (1 + 0.5 (9 cu^2 - 3 u^2)) (pressure).  A link from a bulk
+ * node to a non-bulk node that no tag covers, a pressure link to a bulk
+ * node and any link to node 0 give LB_ERR_UNSUPPORTED.
+ * interior_domains[n]: domain label (0 none); force_on[n]: the force
+ * attribute.  lb_step_tagged runs the loop: per-domain mass change
+ * sum(1 - rho), mass source q = 1.8 (mass change)/(domain size) on the
+ * domain's untagged nodes, body force times force_on, BGK with the Guo and
+ * source terms, then the boundary links.  lb_pressure_flux gives the mass
+ * fluxes q1, q2 (half the sums of rho u_z over the pressure-boundary nodes
+ * of each phase, in node order) of the last step. */
+int lb_set_node_tags(lb_ctx* ctx, int32_t n_nodes, const int32_t* neighbor, const int32_t* tags,
+                     const int32_t* interior_domains, const double* force_on);
+int lb_step_tagged(lb_ctx* ctx, int32_t nsteps, double tau, const double* force, double rho_w);
+int lb_pressure_flux(lb_ctx* ctx, double* q);
+
 /* The same loop body with the two-relaxation-time operator in place of
  * calcOmegaBGK / calcDeltaOmegaF: calcOmegaBGKTRT(f, tau_sym, tau_anti, ...)
  * and calcDeltaOmegaFTRT(tau_sym, tau_anti, 1.0, ...) (LBcollision.h:50-75,

@@ -41,6 +41,9 @@ def lib():
         L.orc_std_case_trt.restype = C.c_int
         L.orc_std_case_trt.argtypes = [C.c_int, C.c_int32, _i32p, _i16p, _i32p, C.c_int32, _f64p, _f64p,
                                        C.c_double, C.c_double, _f64p, C.c_int32, _f64p, _f64p, C.c_int]
+        L.orc_one_phase.restype = C.c_int
+        L.orc_one_phase.argtypes = [C.c_int, C.c_int32, _i32p, _i32p, C.c_int32, _i32p, _i32p, _f64p, _f64p,
+                                    _f64p, _f64p, C.c_double, _f64p, C.c_double, C.c_int32, _f64p, _f64p, _f64p]
         L.orc_rheology.restype = C.c_int
         L.orc_rheology.argtypes = [C.c_int, C.c_int32, _i32p, _i16p, _i32p, C.c_int32, _f64p, _f64p, C.c_int32,
                                    _f64p, _f64p, _f64p, C.c_int32, _f64p, _f64p, _f64p, C.c_int]
@@ -82,6 +85,32 @@ def run_std_case(lattice, neighbor, node_type, bulk, f, tau, force, nsteps, nthr
     else:
         lib().orc_std_case(*args, float(tau), *tail)
     return f.reshape(n_nodes, Q), rho, vel.reshape(n_nodes, D)
+
+
+def run_one_phase(lattice, neighbor, bulk, tags, label, force_on, normals, f, tau, force, rho_w, nsteps):
+    """nsteps iterations of std_one_phase/main.cpp (one rank): tagged-link
+    bounce-back, anti-bounce-back and fluid-fluid swaps, per-domain mass
+    sources.  tags, label: int per node (tags[0] = -1 as Nodes sets it);
+    force_on: per node; normals: (n_nodes, 3).  Returns f, rho, vel and the
+    mass fluxes (q1, q2) of the last iteration."""
+    n_nodes = len(tags)
+    Q, D = Q_OF[lattice], ND_OF[lattice]
+    f = _c(f, np.float64).reshape(-1).copy()
+    ftmp = np.zeros_like(f)
+    rho = np.zeros(n_nodes)
+    vel = np.zeros(n_nodes * D)
+    flux = np.zeros(2)
+    F = np.zeros(3)
+    F[:D] = force[:D]
+    bulk = _c(bulk, np.int32)
+    rc = lib().orc_one_phase(lattice, n_nodes, _c(neighbor, np.int32).ravel(), bulk if bulk.size else
+                             np.zeros(1, np.int32), bulk.size, _c(tags, np.int32), _c(label, np.int32),
+                             _c(force_on, np.float64), _c(normals, np.float64).ravel(), f, ftmp, float(tau), F,
+                             float(rho_w), int(nsteps), rho, vel, flux)
+    if rc != 0:
+        raise ValueError("orc_one_phase: %s" % ("an interior domain has no interior nodes" if rc == -1
+                                                else "a pressure-boundary node is not fluid"))
+    return f.reshape(n_nodes, Q), rho, vel.reshape(n_nodes, D), flux
 
 
 def run_rheology(lattice, neighbor, node_type, bulk, f, table, force, nsteps, nthreads=1):

@@ -22,6 +22,9 @@
 #                   `bodyForce.set(0, 0) = input[...]` line does not compile
 #                   against the current Input class: ambiguous operator=),
 #                   and, for D3Q19, its lattice line
+#   one_phase       src/std_one_phase/main.cpp (tagged-link boundaries and
+#                   per-domain mass sources, D3Q19) with its hard-coded
+#                   input directory switched to ./../
 #   std_case_gpu_d2q9, std_case_gpu_d3q19, twophase_gpu
 #                   tests/integration/{std_case,twophase}_gpu.cpp: the
 #                   reference's set-up classes driving this repository's
@@ -95,6 +98,15 @@ for lat in D2Q9 D3Q19; do
   if cmp -s "$f.new" "$f"; then rm "$f.new"; else mv "$f.new" "$f"; fi
 done
 
+f="$OUT/one_phase.cpp"
+sed -e 's|std::string chimpDir = "/home/[^"]*";|std::string chimpDir = "./../";|' \
+    "$REF/std_one_phase/main.cpp" > "$f.new"
+if ! grep -q 'chimpDir = "./../"' "$f.new"; then
+  echo "build_ref: the input-directory switch did not apply to std_one_phase/main.cpp" >&2
+  exit 1
+fi
+if cmp -s "$f.new" "$f"; then rm "$f.new"; else mv "$f.new" "$f"; fi
+
 build() {  # name, main source, extra include dir
   local name=$1 src=$2 inc=$3
   if [ ! -x "$OUT/$name" ] || [ "$src" -nt "$OUT/$name" ]; then
@@ -109,6 +121,7 @@ build std_case_trt_d2q9 "$OUT/std_case_trt_d2q9.cpp" "$REF/std_case"
 build std_case_trt_d3q19 "$OUT/std_case_trt_d3q19.cpp" "$REF/std_case"
 build rheology_d2q9 "$OUT/rheology_d2q9.cpp" "$REF/polymer_viscosity_tabulation"
 build rheology_d3q19 "$OUT/rheology_d3q19.cpp" "$REF/polymer_viscosity_tabulation"
+build one_phase "$OUT/one_phase.cpp" "$REF/std_one_phase"
 
 # the drop-in demonstration (test infrastructure): reference set-up + shim
 REPO="$(cd "$HERE/.." && pwd)"

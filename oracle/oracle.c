@@ -19,6 +19,7 @@
  *   calcDeltaOmegaF            LBcollision.h:194-213
  *   calcDeltaOmegaFTRT         LBcollision.h:214-233
  *   GeneralizedNewtonian       LBrheology.h:321-476 (polymer_viscosity_tabulation/main.cpp:105-155)
+ *   tagged links, mass sources std_one_phase/main.cpp:27-204, 322-354, 513-612
  *   calcDeltaOmegaST / RC      LBcollision2phase.h:7-20, 70-86
  *   grad / vecNorm             LButilities.h:12-22, 62-66
  *   std_case loop              std_case/main.cpp:109-145
@@ -293,6 +294,164 @@ int orc_std_case_trt(int lattice, int32_t n_nodes, const int32_t *nb, const int1
                      int nthreads) {
   return std_case_run(lattice, n_nodes, nb, type, bulk, n_bulk, f, ftmp, tau_sym, 1, tau_anti, F, nsteps,
                       rho, vel, nthreads);
+}
+
+/* ------------------------------------------------------------ std_one_phase */
+/* Tagged-link boundaries and per-domain mass sources of
+ * std_one_phase/main.cpp, one rank.  Node tags (Nodes::getTag, short, -1 by
+ * default; main.cpp:274-281): bits 0-1 phase (0 solid, 1, 2 fluid phases,
+ * 3 pressure ghost), bit 2 fluid-fluid interface, bit 3 fluid-solid
+ * interface, bit 4 pressure boundary.  Link lists (main.cpp:27-126), each
+ * entry {node, q unknown = rev q, neighbour, q known = q}.  f, ftmp: LbField
+ * (1 field); label, force_on: interior_domains and force attributes;
+ * flux[2]: the mass fluxes q1, q2 of the last iteration (main.cpp:610-624). */
+typedef struct {
+  int32_t n, *e; /* [n][4] */
+} links_t;
+
+static void links_add(links_t *l, int32_t a, int32_t b, int32_t c, int32_t d) {
+  l->e = realloc(l->e, sizeof(int32_t) * 4 * (size_t)(l->n + 1));
+  l->e[4 * l->n] = a;
+  l->e[4 * l->n + 1] = b;
+  l->e[4 * l->n + 2] = c;
+  l->e[4 * l->n + 3] = d;
+  l->n++;
+}
+
+static double cdotref(const lattice_t *L, int q, const double *v) {
+  /* c(q,0)*v[0] + c(q,1)*v[1] (+ c(q,2)*v[2]), LBd3q19.h:124-127 */
+  double r = L->c[q * L->nD] * v[0] + L->c[q * L->nD + 1] * v[1];
+  if (L->nD == 3) r = r + L->c[q * L->nD + 2] * v[2];
+  return r;
+}
+
+int orc_one_phase(int lattice, int32_t n_nodes, const int32_t *nb, const int32_t *bulk, int32_t n_bulk,
+                  const int32_t *tags, const int32_t *label, const double *force_on, const double *normals,
+                  double *f, double *ftmp, double tau, const double *F, double rho_w, int32_t nsteps,
+                  double *rho, double *vel, double *flux) {
+  const lattice_t L = get_lattice(lattice);
+  const int Q = L.nQ, D = L.nD;
+  links_t sf = {0, NULL}, pf = {0, NULL}, ff = {0, NULL};
+  for (int32_t n = 1; n < n_nodes; ++n) {
+    const int t = (short)tags[n];
+    for (int q = 0; q < Q - 1; ++q) {
+      const int32_t nn = nb[(int64_t)n * Q + q];
+      const int tn = (short)tags[nn];
+      if (((t >> 3) & 1) && (tn & 3) == 0) links_add(&sf, n, L.rev[q], nn, q);
+      if (((t >> 4) & 1) && (tn & 3) == 3) links_add(&pf, n, L.rev[q], nn, q);
+      if (((t >> 2) & 1) && (t & 3) == 1 && (tn & 3) == 2) links_add(&ff, n, L.rev[q], nn, q);
+    }
+  }
+  /* mass-source set-up (main.cpp:322-354) */
+  int lmax = 0;
+  for (int32_t n = 1; n < n_nodes; ++n)
+    if (label[n] > lmax) lmax = label[n];
+  double *scale = calloc((size_t)lmax + 1, sizeof(double));
+  double *mc = calloc((size_t)lmax + 1, sizeof(double));
+  double *add = calloc((size_t)n_nodes, sizeof(double));
+  for (int32_t n = 1; n < n_nodes; ++n)
+    if (label[n] > 0 && (short)tags[n] < 3) {
+      scale[label[n]] += 1;
+      add[n] = 1.0;
+    }
+  int rc = 0;
+  for (int i = 1; i <= lmax; ++i) {
+    if (scale[i] == 0.0) rc = -1;
+    else scale[i] = 1.0 / scale[i];
+  }
+  double *a = f, *b = ftmp;
+  for (int32_t it = 0; it < nsteps && rc == 0; ++it) {
+    for (int i = 0; i <= lmax; ++i) mc[i] = 0.0;
+    for (int32_t k = 0; k < n_bulk; ++k) {
+      const int32_t node = bulk[k];
+      mc[label[node]] += 1.0 - qsum(&L, a + (int64_t)node * Q);
+    }
+    for (int32_t k = 0; k < n_bulk; ++k) {
+      const int32_t node = bulk[k];
+      const double *fn = a + (int64_t)node * Q;
+      double rhoN = qsum(&L, fn);
+      const int lb = label[node];
+      const double qm = 0.9 * 2 * scale[lb] * mc[lb] * add[node];
+      rhoN += 0.5 * qm;
+      double Fn[3] = {0, 0, 0}, j[3], u[3], cu[19], cF[19];
+      for (int d = 0; d < D; ++d) Fn[d] = F[d] * force_on[node];
+      qsumc(&L, fn, j);
+      for (int d = 0; d < D; ++d) u[d] = (j[d] + 0.5 * Fn[d]) / rhoN;
+      rho[node] = rhoN;
+      for (int d = 0; d < D; ++d) vel[(int64_t)node * D + d] = u[d];
+      const double u2 = dot(&L, u, u);
+      cdotall(&L, u, cu);
+      const double uF = dot(&L, u, Fn);
+      cdotall(&L, Fn, cF);
+      const double tau_inv = 1.0 / tau, tau_factor = (1 - 0.5 / tau);
+      for (int q = 0; q < Q; ++q) {
+        const double om = -tau_inv * (fn[q] - rhoN * L.w[q] * (1.0 + c2Inv * cu[q] + c4Inv0_5 * (cu[q] * cu[q] - c2 * u2)));
+        const double dF = L.w[q] * tau_factor * (c2Inv * cF[q] + c4Inv * (cF[q] * cu[q] - c2 * uF));
+        const double dQ = tau_factor * qm * L.w[q] * (1.0 + c2Inv * cu[q] + c4Inv0_5 * (cu[q] * cu[q] - c2 * u2));
+        b[(int64_t)nb[(int64_t)node * Q + q] * Q + q] = fn[q] + om + dF + dQ;
+      }
+    }
+    double *t = a; a = b; b = t;
+    /* applySolidFluidBoundary (main.cpp:138-153) */
+    for (int32_t k = 0; k < sf.n; ++k) {
+      const int32_t *e = sf.e + 4 * k;
+      a[(int64_t)e[0] * Q + e[1]] = a[(int64_t)e[2] * Q + e[3]];
+    }
+    /* applyPressureFluidBoundary (main.cpp:156-175) */
+    for (int32_t k = 0; k < pf.n; ++k) {
+      const int32_t *e = pf.e + 4 * k;
+      const double *v = vel + (int64_t)e[0] * D;
+      const double u2 = dot(&L, v, v);
+      const double cu = cdotref(&L, e[3], v);
+      const double w = L.w[e[3]];
+      a[(int64_t)e[0] * Q + e[1]] = -a[(int64_t)e[2] * Q + e[3]] + 2 * w * rho_w * (1 + 0.5 * (c4Inv * cu * cu - c2Inv * u2));
+    }
+    /* applyFluidFluidBoundary (main.cpp:178-204) */
+    for (int32_t k = 0; k < ff.n; ++k) {
+      const int32_t *e = ff.e + 4 * k;
+      const int32_t n1 = e[0], qu = e[1], n2 = e[2], qk = e[3];
+      const double f1 = a[(int64_t)n1 * Q + qu];
+      double uw[3] = {0, 0, 0}, nv[3] = {0, 0, 0};
+      for (int d = 0; d < D; ++d) {
+        uw[d] = 0.5 * (vel[(int64_t)n1 * D + d] + vel[(int64_t)n2 * D + d]);
+        nv[d] = 0.5 * (normals[(int64_t)n1 * 3 + d] + normals[(int64_t)n2 * 3 + d]);
+      }
+      const double length = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+      for (int d = 0; d < D; ++d) nv[d] /= (length + 1e-14);
+      const double cn = cdotref(&L, qu, nv);
+      const double cu = cdotref(&L, qu, uw);
+      const double un = dot(&L, uw, nv);
+      const double w = L.w[qu];
+      const int isbnd1 = 1 - (((short)tags[n1] >> 3) & 1);
+      const int isbnd2 = 1 - (((short)tags[n2] >> 3) & 1);
+      const double dfu = 0 * 6 * w * (cu - un * cn) * isbnd1 * isbnd2;
+      a[(int64_t)n1 * Q + qu] = a[(int64_t)n2 * Q + qk] + dfu;
+      a[(int64_t)n2 * Q + qk] = f1 - dfu;
+    }
+    /* mass fluxes over the pressure-boundary nodes (main.cpp:600-612) */
+    if (flux && it == nsteps - 1 && D == 3) {
+      double m[2] = {0, 0};
+      for (int32_t n = 1; n < n_nodes; ++n) {
+        if ((((short)tags[n] >> 4) & 1) == 0) continue;
+        const int ph = ((short)tags[n] & 3) - 1;
+        if (ph != 0 && ph != 1) {
+          rc = -2;
+          break;
+        }
+        m[ph] += vel[(int64_t)n * 3 + 2] * rho[n];
+      }
+      flux[0] = 0.5 * m[0];
+      flux[1] = 0.5 * m[1];
+    }
+  }
+  if (a != f) memcpy(f, a, sizeof(double) * (size_t)n_nodes * Q);
+  free(sf.e);
+  free(pf.e);
+  free(ff.e);
+  free(scale);
+  free(mc);
+  free(add);
+  return rc;
 }
 
 /* ------------------------------------------------------------ rheology */

@@ -8,8 +8,8 @@ container only; /root/reference is not available elsewhere).  For every case:
      PythonScripts/vtklb.py, see make_vtklb_fixtures.py),
   2. an input.dat with the case parameters is written,
   3. the reference case (std_case D2Q9 / D3Q19, its two-relaxation-time
-     variant, the generalized-Newtonian polymer_viscosity_tabulation, or
-     twophase) runs with
+     variant, the generalized-Newtonian polymer_viscosity_tabulation,
+     std_one_phase with tagged boundaries, or twophase) runs with
      iterations.write = iterations.max, so it writes VTK at i = 0 and i = max,
   4. the VTU of i = max is parsed: rho (rho0, rho1) and vel on the bulk nodes,
      in bulk order, i.e. the fields of the pre-collision state after max
@@ -72,6 +72,33 @@ def input_dat(max_it, fluid_lines, extra=""):
 
 def fmt(v):
     return " ".join(repr(float(x)) for x in np.atleast_1d(v))
+
+
+def run_one_phase(geo, attrs, dat):
+    """std_one_phase (oracle/_ref/one_phase): output under SPE2024/<directory>/,
+    the mass fluxes in <directory>.flux in the working directory."""
+    tmp = tempfile.mkdtemp(prefix="lbref_")
+    try:
+        g = Geometry(geo, "D3Q19", "xy")
+        os.makedirs(os.path.join(tmp, "input", "mpi"))
+        os.makedirs(os.path.join(tmp, "SPE2024"))
+        os.makedirs(os.path.join(tmp, "run"))
+        with open(os.path.join(tmp, "input", "mpi", "tmp0.vtklb"), "w") as fh:
+            fh.write(g.vtklb_text(1, attrs))
+        with open(os.path.join(tmp, "input", "input.dat"), "w") as fh:
+            fh.write(dat)
+        env = dict(os.environ, LD_LIBRARY_PATH=os.path.join(REF, "mpilib"), OMP_NUM_THREADS="1")
+        res = subprocess.run([os.path.join(REF, "one_phase")], cwd=os.path.join(tmp, "run"), env=env,
+                             capture_output=True, text=True, timeout=3600)
+        if res.returncode != 0:
+            raise RuntimeError("one_phase failed: %s %s" % (res.stdout[-2000:], res.stderr[-2000:]))
+        data = parse_vtu(last_vtu(os.path.join(tmp, "SPE2024", "run"), "lb_run"))
+        text = open(os.path.join(tmp, "run", "run.flux")).read()
+        q = re.findall(r"q(\d) = (\S+) \(", text)
+        data["flux"] = np.array([float(q[-2][1]), float(q[-1][1])])
+        return data
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
 
 
 def keep_output(outdir, base, name):
@@ -212,6 +239,55 @@ def rheo_cases():
         {"lam2": 1e9}
 
 
+D3Q19_C = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 0], [1, -1, 0], [1, 0, 1], [1, 0, -1], [0, 1, 1],
+                    [0, 1, -1], [-1, 0, 0], [0, -1, 0], [0, 0, -1], [-1, -1, 0], [-1, 1, 0], [-1, 0, -1],
+                    [-1, 0, 1], [0, -1, -1], [0, -1, 1]])
+
+
+def one_phase_geometry(shape, seed, with_labels=True):
+    """A std_one_phase input: pressure-ghost layers (tag 3, solid in the
+    geometry) at z = 0 and z = nz-1, spherical obstacles (tag 0), fluid of
+    phase 1 for x < nx/2 and phase 2 beyond, periodic in x and y.  Interface
+    bits as std_one_phase reads them (main.cpp:266-273): 4 fluid-fluid,
+    8 fluid-solid, 16 pressure boundary.  Interior domains 1 and 2 split the
+    phase-2 fluid at z = nz/2; the force acts on phase 1 only."""
+    nx, ny, nz = shape
+    rng = np.random.default_rng(seed)
+    x, y, z = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    geo = np.ones(shape, dtype=np.int64)
+    solid = np.zeros(shape, dtype=bool)
+    for _ in range(3):
+        c = rng.uniform([0, 0, 2], [nx, ny, nz - 2])
+        r = rng.uniform(1.2, 2.2)
+        solid |= (x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2 < r * r
+    ghost = (z == 0) | (z == nz - 1)
+    geo[solid | ghost] = 0
+    phase = np.where(x < nx // 2, 1, 2)
+    base = np.where(geo == 1, phase, np.where(ghost, 3, 0))
+    tags = base.copy()
+    fluid = geo == 1
+    for c in D3Q19_C:
+        nbp = np.roll(base, shift=(-c[0], -c[1]), axis=(0, 1))
+        nbp = np.roll(nbp, shift=-c[2], axis=2)  # z wrap only pairs ghost layers, never read for fluid
+        other = (nbp == 3 - base) & (nbp >= 1) & (nbp <= 2)
+        tags |= np.where(fluid & other, 4, 0)
+        tags |= np.where(fluid & (nbp == 0), 8, 0)
+        tags |= np.where(fluid & (nbp == 3), 16, 0)
+    tags = np.where(fluid, tags, base)
+    label = np.zeros(shape, dtype=np.int64)
+    if with_labels:
+        label[fluid & (phase == 2) & (z < nz // 2)] = 1
+        label[fluid & (phase == 2) & (z >= nz // 2)] = 2
+    force = np.where(fluid & (phase == 1), 1, 0).astype(np.int64)
+    return geo, tags, label, force
+
+
+def one_phase_cases():
+    """(name, shape, seed, with_labels, tau, force, max_it, write)"""
+    yield "op_tagged_10x8x12", (10, 8, 12), 5, True, 0.8, [0.0, 0.0, 1e-5], 100, 2
+    yield "op_tagged_nolabel_9x7x10", (9, 7, 10), 8, False, 1.1, [2e-6, 0.0, 2e-5], 50, 1
+
+
 def tp_cases():
     """(name, geo, periodic, rho0, wett, tau0, tau1, sigma, beta, momx, force, max_it)"""
     rng = np.random.default_rng(99)
@@ -276,6 +352,23 @@ def main(selected):
                             params=json.dumps(params), init_rho=attrs[0][1].astype(np.float64), table=table,
                             rho=data["rho"], vel=data["vel"], viscosity=data["viscosity"])
         print("wrote ref_%s.npz" % name, data["rho"].shape, data["viscosity"].min(), data["viscosity"].max())
+    for name, shape, seed, with_labels, tau, force, max_it, write in one_phase_cases():
+        if selected and name not in selected:
+            continue
+        geo, tags, label, fon = one_phase_geometry(shape, seed, with_labels)
+        attrs = [("nodetags", tags), ("domains", np.zeros(shape, dtype=np.int64)), ("force", fon),
+                 ("interior_domains", label), ("normal_x", np.zeros(shape)), ("normal_y", np.zeros(shape)),
+                 ("normal_z", np.zeros(shape))]
+        dat = ("<names>\n    file tmp\n    directory run\n<end>\n"
+               "<iterations>\n    max %d\n    write %d\n<end>\n<fluid>\n    tau %s\n    bodyforce %s\n<end>\n"
+               % (max_it, write, fmt(tau), fmt(force)))
+        data = run_one_phase(geo, attrs, dat)
+        params = dict(case="std_one_phase", lattice="D3Q19", periodic="xy", tau=tau, force=force, steps=max_it,
+                      rho_w=1.0)
+        np.savez_compressed(os.path.join(HERE, "ref_%s.npz" % name), geo=geo.astype(np.int8), tags=tags,
+                            label=label, force_on=fon, params=json.dumps(params), rho=data["rho"],
+                            vel=data["vel"], flux=data["flux"])
+        print("wrote ref_%s.npz" % name, data["rho"].shape, data["flux"])
     for name, g, per, r0, wett, tau0, tau1, sigma, beta, momx, force, max_it in tp_cases():
         if selected and name not in selected:
             continue

@@ -62,8 +62,9 @@ def ref_names(prefix=""):
 
 
 def std_ref_names():
-    """The BGK std_case fixtures (not two-phase, two-relaxation-time or rheology)."""
-    return [n for n in ref_names() if not n.startswith(("tp_", "trt_", "rheo_"))]
+    """The BGK std_case fixtures (not two-phase, two-relaxation-time, rheology
+    or std_one_phase)."""
+    return [n for n in ref_names() if not n.startswith(("tp_", "trt_", "rheo_", "op_"))]
 
 
 def float_round(a):

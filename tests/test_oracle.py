@@ -167,6 +167,82 @@ def test_rheology_channel_stress_balance():
     assert np.allclose(mu, mu[::-1], rtol=0, atol=1e-12)
 
 
+def one_phase_inputs(d, T):
+    """Per-node tags (node 0: -1, as Nodes sets it), domain labels and force
+    attribute of a std_one_phase fixture."""
+    tags = np.round(T.attribute(d["tags"])).astype(np.int32)
+    tags[0] = -1
+    label = np.round(T.attribute(d["label"])).astype(np.int32)
+    fon = T.attribute(d["force_on"])
+    return tags, label, fon
+
+
+@pytest.mark.parametrize("name", ref_names("op_"))
+def test_one_phase_bitexact_vs_compiled_reference(name):
+    """std_one_phase (tagged bounce-back, anti-bounce-back, fluid-fluid swaps,
+    per-domain mass sources) as compiled from the reference: rho and vel bit
+    for bit; the mass fluxes to the 6 digits the case prints."""
+    d = load_ref(name)
+    p = d["params"]
+    T = OracleTopo(d["geo"].astype(np.int64), "D3Q19", p["periodic"])
+    tags, label, fon = one_phase_inputs(d, T)
+    f0 = oracle.rest_state(1, T.n_nodes, T.bulk, np.ones(T.n_nodes))
+    f, rho, vel, flux = oracle.run_one_phase(1, T.neighbors, T.bulk, tags, label, fon, np.zeros((T.n_nodes, 3)), f0,
+                                             p["tau"], np.array(p["force"]), p["rho_w"], p["steps"] + 1)
+    b = T.bulk
+    assert np.array_equal(rho[b], d["rho"])
+    assert np.array_equal(vel[b], d["vel"])
+    assert np.allclose(flux, d["flux"], rtol=1e-5, atol=1e-21)
+
+
+def test_one_phase_without_tags_conserves_and_bounces():
+    """Closed box, every fluid node next to solid tagged fluid-solid and no
+    sources: the tagged bounce-back is the half-way bounce-back, so the loop
+    equals std_case's bit for bit; and a pressure-free, source-free run
+    conserves mass."""
+    rng = np.random.default_rng(12)
+    geo = (rng.random((9, 8, 7)) > 0.25).astype(np.int64)
+    geo[:, :, 0] = geo[:, :, -1] = 0
+    T = OracleTopo(geo, "D3Q19", "xy")
+    tags = np.zeros(T.n_nodes, np.int32)
+    tags[0] = -1
+    fluid = T.node_types > 1
+    tags[fluid] = 1
+    nbt = T.node_types[T.neighbors]
+    tags[fluid & (nbt < 2).any(axis=1)] |= 8
+    f0 = oracle.rest_state(1, T.n_nodes, T.bulk, np.ones(T.n_nodes))
+    F = np.array([1e-5, 2e-6, 0.0])
+    f1, r1, v1, _ = oracle.run_one_phase(1, T.neighbors, T.bulk, tags, np.zeros(T.n_nodes, np.int32),
+                                         np.ones(T.n_nodes), np.zeros((T.n_nodes, 3)), f0, 0.7, F, 1.0, 40)
+    f2, r2, v2 = oracle.run_std_case(1, T.neighbors, T.node_types, T.bulk, f0, 0.7, F, 40)
+    b = T.bulk
+    assert np.array_equal(f1[b], f2[b]) and np.array_equal(v1[b], v2[b])
+    assert abs(f1[b].sum() - f0[b].sum()) <= 1e-12 * f0[b].sum()
+
+
+def test_one_phase_mass_source_restores_domain_mass():
+    """A closed domain that starts with a mass deficit: the source
+    0.9*2*(mass change)/(domain size) returns its mass towards rho = 1."""
+    geo = np.ones((6, 6, 8), dtype=np.int64)
+    geo[:, :, 0] = geo[:, :, -1] = 0
+    T = OracleTopo(geo, "D3Q19", "xy")
+    tags = np.zeros(T.n_nodes, np.int32)
+    tags[0] = -1
+    fluid = T.node_types > 1
+    tags[fluid] = 2
+    nbt = T.node_types[T.neighbors]
+    tags[fluid & (nbt < 2).any(axis=1)] |= 8
+    label = np.where(fluid, 1, 0).astype(np.int32)
+    f0 = oracle.rest_state(1, T.n_nodes, T.bulk, np.full(T.n_nodes, 0.98))
+    b = T.bulk
+    m0 = f0[b].sum()
+    f, rho, vel, _ = oracle.run_one_phase(1, T.neighbors, T.bulk, tags, label, np.zeros(T.n_nodes),
+                                          np.zeros((T.n_nodes, 3)), f0, 1.0, np.zeros(3), 1.0, 60)
+    deficit0 = b.size - m0
+    deficit = b.size - f[b].sum()
+    assert 0 < deficit < 1e-3 * deficit0
+
+
 def tp_start(d, T):
     r0 = float_round(T.attribute(d["rho0_attr"]))
     r1 = float_round(T.attribute(1.0 - d["rho0_attr"]))
