@@ -27,11 +27,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/badchimp_amd.h"
@@ -161,6 +163,7 @@ struct lb_batch {
   // solves
   std::vector<Slot> slots;
   BatchSolve* d_solves = nullptr;
+  std::vector<BatchSolve> h_solves;  // host copies, the source of the descriptor uploads
   uint32_t* d_items = nullptr;
   int64_t items_cap = 0;
   int32_t* d_slots = nullptr;
@@ -470,26 +473,54 @@ struct AddArgs {
   int64_t tag = 0;
 };
 
-int add_solve(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* geo, int32_t periodic_mask,
-              const AddArgs& p) {
-  hipSetDevice(b->device);
+// The host side of an admission: row layout and row windows of a geometry,
+// checked node by node.  Pure host work, so many run in parallel.
+struct Prepared {
   RowBuild rb;
-  std::string why;
-  if (!rows_from_geometry(b->lattice, dims, geo, periodic_mask, rb, why))
-    return bfail(b, LB_ERR_UNSUPPORTED, "lb_batch_add: %s", why.c_str());
   std::vector<uint4> win;
   int32_t PY = 0;
-  if (!build_windows(rb, win, PY, why)) return bfail(b, LB_ERR_UNSUPPORTED, "lb_batch_add: %s", why.c_str());
+  int code = LB_OK;
+  std::string why;
+};
+
+void prepare(int lattice, const int32_t* dims, const int32_t* geo, int32_t periodic_mask, Prepared& pr) {
+  if (!rows_from_geometry(lattice, dims, geo, periodic_mask, pr.rb, pr.why) ||
+      !build_windows(pr.rb, pr.win, pr.PY, pr.why)) {
+    pr.code = LB_ERR_UNSUPPORTED;
+    return;
+  }
+  const RowBuild& rb = pr.rb;
   const int64_t n = static_cast<int64_t>(rb.coord.size());
   // every node's own window must name it (a cheap check of the construction)
   for (int64_t i = 0; i < n; ++i) {
     const uint32_t c = rb.coord[i];
     const uint32_t X = c >> 22, Y = (c >> 12) & 1023u, Z = c & 4095u, w = Z >> 5, t = Z - 32 * w + 16;
-    const uint4 e = win[(static_cast<size_t>(X + 1) * PY + (Y + 1)) * rb.g.W + w];
+    const uint4 e = pr.win[(static_cast<size_t>(X + 1) * pr.PY + (Y + 1)) * rb.g.W + w];
     const uint64_t m = (static_cast<uint64_t>(e.z) << 32) | e.y;
-    if (!((m >> t) & 1u) || e.x + static_cast<uint32_t>(__builtin_popcountll(m & ((1ull << t) - 1ull))) != i)
-      return bfail(b, LB_ERR_STATE, "lb_batch_add: row windows inconsistent at node %lld", (long long)i);
+    if (!((m >> t) & 1u) || e.x + static_cast<uint32_t>(__builtin_popcountll(m & ((1ull << t) - 1ull))) != i) {
+      pr.code = LB_ERR_STATE;
+      pr.why = "row windows inconsistent at node " + std::to_string(i);
+      return;
+    }
   }
+}
+
+// Admit a solve.  pre: its prepared rows (built here when null); sync:
+// wait for the uploads (else the caller synchronises b->stream before the
+// prepared host buffers go away).
+int add_solve(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* geo, int32_t periodic_mask,
+              const AddArgs& p, const Prepared* pre = nullptr, bool sync = true) {
+  hipSetDevice(b->device);
+  Prepared local;
+  if (!pre) {
+    prepare(b->lattice, dims, geo, periodic_mask, local);
+    pre = &local;
+  }
+  if (pre->code != LB_OK) return bfail(b, pre->code, "lb_batch_add: %s", pre->why.c_str());
+  const RowBuild& rb = pre->rb;
+  const std::vector<uint4>& win = pre->win;
+  const int32_t PY = pre->PY;
+  const int64_t n = static_cast<int64_t>(rb.coord.size());
   const int nf = p.kind == 1 ? 2 : 1;
   const Footprint fp = footprint_of(b->Q, b->esz, n, static_cast<int64_t>(win.size()), nf);
   if (static_cast<uint64_t>(nf) * b->Q * static_cast<uint64_t>(fp.NP) * b->esz >= (1ull << 32) ||
@@ -593,9 +624,13 @@ int add_solve(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* geo, 
         e = store_lowp(b->precision, b->lattice, 2, tmp, base + fp.off_f0, static_cast<int32_t>(n), fp.NP, b->stream);
     }
   }
+  // one fixed array (never reallocated: earlier uploads may still read it)
+  if (b->h_solves.empty()) b->h_solves.resize(kMaxSlots);
+  b->h_solves[slot] = d;
   if (e == hipSuccess)
-    e = hipMemcpyAsync(b->d_solves + slot, &d, sizeof(BatchSolve), hipMemcpyHostToDevice, b->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    e = hipMemcpyAsync(b->d_solves + slot, &b->h_solves[slot], sizeof(BatchSolve), hipMemcpyHostToDevice,
+                       b->stream);
+  if (e == hipSuccess && (sync || tmp)) e = hipStreamSynchronize(b->stream);
   if (tmp) hipFree(tmp);
   if (e != hipSuccess) {
     b->alloc.release(off, fp.bytes);
@@ -640,6 +675,50 @@ int lb_batch_add(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* ge
   p.max_steps = max_steps;
   p.tag = tag;
   return add_solve(b, nd, dims, geo, periodic_mask, p);
+}
+
+int lb_batch_add_many(lb_batch* b, int32_t count, int32_t nd, const int32_t* dims, const int32_t* const* geos,
+                      int32_t periodic_mask, double tau, const double* force, double tol, int64_t max_steps,
+                      const int64_t* tags, int32_t* n_added) {
+  if (!b) return LB_ERR_ARG;
+  if (n_added) *n_added = 0;
+  if (count < 0 || nd != b->nD || (count > 0 && (!dims || !geos || !tags)) || !force || !(tau > 0.5) ||
+      !(tol > 0.0) || max_steps < 1 || force[0] == 0.0)
+    return bfail(b, LB_ERR_ARG, "lb_batch_add_many: bad argument");
+  for (int32_t k = 0; k < count; ++k)
+    if (!geos[k]) return bfail(b, LB_ERR_ARG, "lb_batch_add_many: geometry %d is null", k);
+  // row layouts of all geometries on the host cores, then admission in order
+  std::vector<Prepared> pre(count);
+  std::atomic<int32_t> next(0);
+  const int nthr = std::max(1, std::min<int>(16, std::min<int>(count, static_cast<int>(std::thread::hardware_concurrency()))));
+  auto work = [&]() {
+    for (int32_t k; (k = next.fetch_add(1)) < count;)
+      prepare(b->lattice, dims + static_cast<size_t>(k) * nd, geos[k], periodic_mask, pre[k]);
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nthr; ++t) pool.emplace_back(work);
+  work();
+  for (auto& t : pool) t.join();
+  AddArgs p;
+  p.kind = 0;
+  p.tau = tau;
+  p.force = force;
+  p.tol = tol;
+  p.max_steps = max_steps;
+  int rc = LB_OK;
+  int32_t added = 0;
+  for (int32_t k = 0; k < count; ++k) {
+    p.tag = tags[k];
+    rc = add_solve(b, nd, dims + static_cast<size_t>(k) * nd, geos[k], periodic_mask, p, &pre[k], false);
+    if (rc != LB_OK) break;
+    ++added;
+  }
+  // the uploads read the prepared host buffers
+  const hipError_t e = hipStreamSynchronize(b->stream);
+  if (n_added) *n_added = added;
+  if (rc == LB_ERR_NOMEM) rc = LB_OK;  // the rest waits for retirements
+  if (rc == LB_OK && e != hipSuccess) return bfail(b, LB_ERR_HIP, "lb_batch_add_many: %s", hipGetErrorString(e));
+  return rc;
 }
 
 int lb_batch_add_twophase(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* geo, int32_t periodic_mask,

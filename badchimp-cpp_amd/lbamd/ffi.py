@@ -30,6 +30,7 @@ _lib = None
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _i16p = np.ctypeslib.ndpointer(dtype=np.int16, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 _vp = C.c_void_p
 
 
@@ -120,6 +121,8 @@ SIGNATURES = {
     "lb_batch_last_error": (C.c_char_p, [_vp]),
     "lb_batch_footprint": (C.c_int, [C.c_int, C.c_int, C.c_int32, C.c_int32, _i32p, _i32p, C.c_int32,
                                      C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "lb_batch_add_many": (C.c_int, [_vp, C.c_int32, C.c_int32, _i32p, C.c_void_p, C.c_int32, C.c_double, _f64p,
+                                    C.c_double, C.c_int64, _i64p, C.POINTER(C.c_int32)]),
     "lb_batch_add_twophase": (C.c_int, [_vp, C.c_int32, _i32p, _i32p, C.c_int32, _f64p, C.c_double, C.c_double,
                                         C.c_double, C.c_double, C.c_double, _f64p, C.c_double, C.c_int64,
                                         C.c_int64]),
@@ -621,6 +624,23 @@ class Batch:
             return False
         self._check(rc)
         return True
+
+    def add_many(self, geos, periodic, tau, force, tol, max_steps, tags):
+        """Admit many solves (row layouts built on host threads); returns how
+        many, in order, fitted into the pool now."""
+        gs = [np.ascontiguousarray(g, dtype=np.int32) for g in geos]
+        if not gs:
+            return 0
+        dims = np.ascontiguousarray([g.shape for g in gs], dtype=np.int32)
+        ptrs = (C.c_void_p * len(gs))(*[g.ctypes.data for g in gs])
+        F = np.zeros(3)
+        F[: self.nD] = np.asarray(force, dtype=float)[: self.nD]
+        tg = np.ascontiguousarray(tags, dtype=np.int64)
+        got = C.c_int32()
+        self._check(lib().lb_batch_add_many(self._h, len(gs), dims.shape[1], dims.ravel(), ptrs,
+                                            periodic_mask(periodic), float(tau), F, float(tol), int(max_steps),
+                                            tg, C.byref(got)))
+        return got.value
 
     def add_twophase(self, geo, periodic, rho0, wettability, tau0, tau1, sigma, beta, force, tol, max_steps, tag):
         """Admit a colour-gradient solve (rho0 per geometry cell); False when

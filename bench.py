@@ -285,8 +285,7 @@ def batched_leg(n, every, tau, force):
     def run(precision):
         b = lbamd.Batch(lbamd.D3Q19, precision, pool, 100)
         t0 = time.perf_counter()
-        for i, g in enumerate(geos):
-            assert b.add(g, "x", tau, force, 1e-6, 200000, i)
+        assert b.add_many(geos, "x", tau, force, 1e-6, 200000, list(range(n))) == n
         t_adm = time.perf_counter() - t0
         res = {}
         while len(res) < n:

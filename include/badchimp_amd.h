@@ -377,6 +377,15 @@ int lb_batch_add(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* ge
  * permeabilities k_i = nu_i <phi_i sum_q f c_x> / F_x (nu_i = (tau_i - 1/2)/3,
  * LBglobalforcing.h:61-78) change by at most tol (|k_0| + |k_1|) between
  * checks. */
+/* Admit count single-phase solves (geometry k: dims[k*nd..], geos[k]) with
+ * the parameters of lb_batch_add, tagged tags[k].  The row layouts are built
+ * on up to 16 host threads, then the solves are admitted in order until one
+ * does not fit: *n_added says how many were (LB_OK; the rest waits for
+ * retirements).  Any other error stops it with that error. */
+int lb_batch_add_many(lb_batch* b, int32_t count, int32_t nd, const int32_t* dims,
+                      const int32_t* const* geos, int32_t periodic_mask, double tau,
+                      const double* force, double tol, int64_t max_steps, const int64_t* tags,
+                      int32_t* n_added);
 int lb_batch_add_twophase(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* geo,
                           int32_t periodic_mask, const double* rho0, double wettability,
                           double tau0, double tau1, double sigma, double beta,

@@ -105,6 +105,29 @@ def test_admission_by_footprint(gpu_available, geos):
     b.close()
 
 
+def test_add_many_equals_one_by_one(gpu_available, geos):
+    """lb_batch_add_many (row layouts built on host threads) admits the same
+    solves as lb_batch_add one by one: same results, same footprint; it
+    stops at the first solve that does not fit and reports how many went in."""
+    extra = [pack((28, 28, 28), 3.5, 0.42, 9), pack((16, 16, 16), 2.5, 0.5, 10)]
+    gs = geos + extra
+    ref = run_batch(gs, lbamd.FP32S, 50, 1e-6, 20000)
+    b = lbamd.Batch(lbamd.D3Q19, lbamd.FP32S, 1 << 28, 50)
+    assert b.add_many(gs, "x", TAU, FORCE, 1e-6, 20000, list(range(len(gs)))) == len(gs)
+    out = {}
+    while len(out) < len(gs):
+        b.advance(1)
+        for r in b.collect():
+            out[r["tag"]] = (r["k"], r["steps"], r["status"])
+    b.close()
+    assert out == ref
+    sizes = [lbamd.batch_footprint(lbamd.D3Q19, lbamd.FP32S, g, "x")[0] for g in gs]
+    b = lbamd.Batch(lbamd.D3Q19, lbamd.FP32S, sizes[0] + sizes[1] + 1024, 50)
+    assert b.add_many(gs, "x", TAU, FORCE, 1e-6, 20000, list(range(len(gs)))) == 2
+    assert b.status()["bytes_used"] == sizes[0] + sizes[1]
+    b.close()
+
+
 def test_fp16s_batch_error(gpu_available, geos):
     ref = run_batch(geos, lbamd.FP64, 50, 1e-6, 20000)
     got = run_batch(geos, lbamd.FP16S, 50, 1e-6, 20000)
