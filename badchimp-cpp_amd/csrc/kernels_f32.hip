@@ -228,7 +228,7 @@ __global__ __launch_bounds__(kBlock) void k_bgk_lp(const S* __restrict__ gin, S*
                                                    double* __restrict__ partial, float tauA_inv,
                                                    float tauA_factor, const float* __restrict__ sr,
                                                    const float* __restrict__ nu, int tab_n) {
-  const int32_t b = static_cast<int32_t>(blockIdx.x);
+  const int32_t b = node_block(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
   const int32_t i = b * kBlock + threadIdx.x;
   double jx = 0.0;
   if (i < n) {
@@ -470,7 +470,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_phi_lp(const S* __restrict__ gin,
                                                       int64_t NP, float* __restrict__ phi,
                                                       double* __restrict__ partial) {
   constexpr int Q = L::nQ;
-  const int32_t b = static_cast<int32_t>(blockIdx.x);
+  const int32_t b = node_block(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
   const int32_t i = b * kBlock + threadIdx.x;
   double mx = 0.0;
   if (i < n) {
@@ -494,7 +494,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_lp(const S* __restrict__ gin, S* 
                                                   const double* __restrict__ dscalar,
                                                   double* __restrict__ rho_o, double* __restrict__ vel_o) {
   constexpr int Q = L::nQ;
-  const int32_t b = static_cast<int32_t>(blockIdx.x);
+  const int32_t b = node_block(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
   const int32_t i = b * kBlock + threadIdx.x;
   if (i >= n) return;
   const uint32_t fs = static_cast<uint32_t>(Q * NP);

@@ -197,7 +197,7 @@ __global__ __launch_bounds__(kBlock) void k_bgk(const double* __restrict__ fin,
                                                 int64_t NP, BgkArgs a, double* __restrict__ rho_o,
                                                 double* __restrict__ vel_o,
                                                 double* __restrict__ partial) {
-  const int32_t b = static_cast<int32_t>(blockIdx.x);
+  const int32_t b = node_block(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
   const int32_t i = b * kBlock + threadIdx.x;
   double jx = 0.0;
   if (i < n) jx = bgk_node<L, A, FIRST, STORE, REDUCE, MODE>(fin, fout, ad, i, NP, a, rho_o, vel_o);
@@ -281,7 +281,7 @@ template <class L, bool FIRST>
 __global__ __launch_bounds__(kBlock) void k_tag_mass(const double* __restrict__ fin, TableAddr ad, int32_t n,
                                                      int64_t NP, const uint32_t* __restrict__ info, int n_label,
                                                      double* __restrict__ partial) {
-  const int32_t b = static_cast<int32_t>(blockIdx.x);
+  const int32_t b = node_block(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
   const int32_t i = b * kBlock + threadIdx.x;
   double v = 0.0;
   int lab = 0;
@@ -361,7 +361,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_phi(const double* __restrict__ fi
                                                    int32_t n, int64_t NP, double* __restrict__ phi,
                                                    double* __restrict__ partial) {
   constexpr int Q = L::nQ;
-  const int32_t b = static_cast<int32_t>(blockIdx.x);
+  const int32_t b = node_block(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
   const int32_t i = b * kBlock + threadIdx.x;
   double mx = 0.0;
   if (i < n) {
@@ -393,7 +393,7 @@ __global__ __launch_bounds__(kBlock) void k_cg(const double* __restrict__ fin,
                                                double* __restrict__ rho_o,
                                                double* __restrict__ vel_o) {
   constexpr int Q = L::nQ, D = L::nD;
-  const int32_t b = static_cast<int32_t>(blockIdx.x);
+  const int32_t b = node_block(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
   const int32_t i = b * kBlock + threadIdx.x;
   if (i >= n) return;
   double f[2][Q];

@@ -701,7 +701,8 @@ def main():
                      "aosoa_stream_GBs": round(stream_gbs, 1)},
         "fp16s": {"value": round(mlups16, 2), "unit": "MLUPS", "avg_kernel_ms": k16_ms,
                   "achieved_GBs": round(n_bulk * 76 / (k16_ms * 1e-3) / 1e9, 1),
-                  "frac": round(n_bulk * 76 / (k16_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "bytes_per_lup": 76},
+                  "frac": round(n_bulk * 76 / (k16_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "bytes_per_lup": 76,
+                  "traffic": profiled_traffic("FP16S") if args.size == 256 else None},
         "fp64": {"value": round(mlups64, 2), "unit": "MLUPS", "avg_kernel_ms": k64_ms,
                  "achieved_GBs": round(ach64, 1), "frac": round(ach64 / HBM_PEAK_GBS, 4),
                  "bytes_per_lup": bytes64, "traffic": traffic64},
