@@ -575,6 +575,7 @@ int add_solve(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* geo, 
   if (e == hipSuccess)
     e = hipMemcpyAsync(base + fp.off_win, win.data(), win.size() * sizeof(uint4), hipMemcpyHostToDevice, b->stream);
   double* tmp = nullptr;
+  std::vector<double> r0c;  // read by an async upload: lives until the synchronisation below
   if (e == hipSuccess) {
     if (p.kind == 0) {
       // rest state f = w: zero in the shifted forms; a pull from it is the
@@ -584,12 +585,10 @@ int add_solve(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* geo, 
       else
         e = hipMemsetAsync(base + fp.off_f0, 0, static_cast<size_t>(b->Q) * fp.NP * b->esz, b->stream);
     } else {
-      // f_k = w rho_k at rest, placed where the first pull reads it: element
-      // (q, i) goes to the source of direction q of node i (the pull is a
-      // permutation of the elements), so every step is the same kernel
-      const size_t fsz = static_cast<size_t>(b->Q) * fp.NP;
-      std::vector<double> h(2 * fsz, 0.0);
-      const int Q = b->Q;
+      // f_k = w rho_k at rest, placed on the device where the first pull
+      // reads it (k_prestream): the host only maps the phase-0 density of
+      // each geometry cell to its compact node
+      r0c.resize(static_cast<size_t>(n));
       for (int64_t i = 0; i < n; ++i) {
         const uint32_t c = rb.coord[i];
         const int32_t X = static_cast<int32_t>(c >> 22), Y = static_cast<int32_t>((c >> 12) & 1023u),
@@ -602,26 +601,17 @@ int add_solve(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* geo, 
         }
         const int64_t cell = b->nD == 3 ? (static_cast<int64_t>(P[0]) * dims[1] + P[1]) * dims[2] + P[2]
                                         : static_cast<int64_t>(P[0]) * dims[1] + P[2];
-        const double r0 = p.rho0[cell], r1 = 1.0 - p.rho0[cell];
-        for (int q = 0; q < Q; ++q) {
-          uint32_t el;
-          const double w = b->lattice == LB_D2Q9 ? D2Q9::w(q) : D3Q19::w(q);
-          if (q == Q - 1) {
-            el = pidx_host(Q, q, static_cast<uint32_t>(i));
-          } else {
-            const auto src = win_source(b->lattice, rb, win, PY, q, static_cast<int32_t>(i));
-            const int rq = b->lattice == LB_D2Q9 ? D2Q9::rev(q) : D3Q19::rev(q);
-            el = src.first ? pidx_host(Q, q, static_cast<uint32_t>(src.second))
-                           : pidx_host(Q, rq, static_cast<uint32_t>(i));
-          }
-          h[el] = w * r0;
-          h[fsz + el] = w * r1;
-        }
+        r0c[i] = p.rho0[cell];
       }
-      e = hipMalloc(&tmp, 2 * fsz * sizeof(double));
-      if (e == hipSuccess) e = hipMemcpyAsync(tmp, h.data(), 2 * fsz * sizeof(double), hipMemcpyHostToDevice, b->stream);
+      e = hipMalloc(&tmp, std::max<size_t>(1, r0c.size()) * sizeof(double));
       if (e == hipSuccess)
-        e = store_lowp(b->precision, b->lattice, 2, tmp, base + fp.off_f0, static_cast<int32_t>(n), fp.NP, b->stream);
+        e = hipMemcpyAsync(tmp, r0c.data(), r0c.size() * sizeof(double), hipMemcpyHostToDevice, b->stream);
+      if (e == hipSuccess) {
+        const RowWin ad{reinterpret_cast<const uint32_t*>(base + fp.off_coord),
+                        reinterpret_cast<const uint4*>(base + fp.off_win), static_cast<uint32_t>(fp.NP), PY, rb.g.W};
+        e = prestream_lowp(b->precision, b->lattice, ad, tmp, static_cast<int32_t>(n), fp.NP, base + fp.off_f0,
+                           b->stream);
+      }
     }
   }
   // one fixed array (never reallocated: earlier uploads may still read it)

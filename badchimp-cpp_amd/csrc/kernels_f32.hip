@@ -278,6 +278,30 @@ __global__ __launch_bounds__(kBlock) void k_batch_bgk_lp(const BatchSolve* __res
 //   t = f0 + f1 - w,   d = f0 - f1,
 // so that the flow, which lives in the total, keeps the shifted precision and
 // the colour keeps the full relative precision of d.
+// Rest state of a two-phase solve placed where its first pull reads it:
+// f_k = w rho_k of node i goes to the source element of each direction of
+// node i (the pull is a permutation of the elements), stored as
+// (f0 + f1 - w, f0 - f1) with k_store's arithmetic.  r0: phase-0 density per
+// compact node (phase 1: 1 - r0).
+template <class L, typename S>
+__global__ __launch_bounds__(kBlock) void k_prestream(RowWin ad, const double* __restrict__ r0, int32_t n, int64_t NP,
+                                                      S* __restrict__ g) {
+  constexpr int Q = L::nQ;
+  const int32_t i = static_cast<int32_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  uint32_t src[Q - 1];
+  ad.template sources<L>(i, src);
+  const double a = r0[i], b = 1.0 - a;
+  const int64_t fs = static_cast<int64_t>(Q) * NP;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int64_t o = q < Q - 1 ? static_cast<int64_t>(src[q]) : static_cast<int64_t>(pidx<Q>(Q - 1, i));
+    const double f0 = L::w(q) * a, f1 = L::w(q) * b;
+    g[o] = Store<S>::put(static_cast<float>((f0 + f1) - L::w(q)));
+    g[fs + o] = Store<S>::put(static_cast<float>(f0 - f1));
+  }
+}
+
 template <class L, typename S>
 __global__ __launch_bounds__(kBlock) void k_store(const double* __restrict__ f, S* __restrict__ g,
                                                   int32_t n, int64_t NP, int nf) {
@@ -661,6 +685,17 @@ hipError_t dispatch(int precision, int lattice, const Addr& ad, F&& f) {
 }
 
 }  // namespace
+
+hipError_t prestream_lowp(int precision, int lattice, const RowWin& ad, const double* r0, int32_t n, int64_t NP,
+                          void* g, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  return dispatch(precision, lattice, Addr{}, [&](auto l, auto st, auto) {
+    using L = decltype(l);
+    using S = decltype(st);
+    k_prestream<L, S><<<dim3(static_cast<unsigned>((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s>>>(
+        ad, r0, n, NP, static_cast<S*>(g));
+  });
+}
 
 hipError_t store_lowp(int precision, int lattice, int nf, const double* f, void* g, int32_t n,
                       int64_t NP, hipStream_t s) {

@@ -17,6 +17,10 @@ namespace lbamd {
 // the normal half range (6.1e-5 .. 65504) after multiplying by 2^12.
 constexpr float kHalfScale = 4096.0f;
 
+// Two-phase rest state f_k = w rho_k (rho_0 = r0 per compact node, rho_1 = 1 -
+// r0) written where the first pull through ad reads it, in the (t, d) form.
+hipError_t prestream_lowp(int precision, int lattice, const RowWin& ad, const double* r0, int32_t n, int64_t NP,
+                          void* g, hipStream_t s);
 hipError_t store_lowp(int precision, int lattice, int nf, const double* f, void* g, int32_t n,
                       int64_t NP, hipStream_t s);
 hipError_t load_lowp(int precision, int lattice, int nf, const void* g, double* f, int32_t n,
