@@ -51,9 +51,10 @@ namespace lbamd {
 // all CUs still sweep one compact front whose halo stays in the shared MALL.
 // On the 256^3 pack runs of 4 blocks are 2.4 % faster than runs of 1 (the
 // plain order) in FP32S and 2 % in FP64; runs of 8-32 give about the same,
-// one contiguous range per XCD (eight fronts) is 3 % slower.  The mapping
-// only changes which workgroup computes a block: per-block partial sums keep
-// their block index, so results do not change.
+// one contiguous range per XCD (eight fronts) is 3 % slower.  The batched
+// kernels map their workgroups the same way (+0.7 % on the 512-solve
+// ensemble).  The mapping only changes which workgroup computes a block:
+// per-block partial sums keep their block index, so results do not change.
 constexpr int32_t kXcdChunk = 4;
 __device__ __forceinline__ int32_t node_block(int32_t b, int32_t nb) {
   const int32_t full = (nb / (8 * kXcdChunk)) * (8 * kXcdChunk);

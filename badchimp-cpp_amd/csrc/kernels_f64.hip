@@ -217,7 +217,7 @@ template <class L, bool REDUCE>
 __global__ __launch_bounds__(kBlock) void k_batch_bgk(const BatchSolve* __restrict__ solves,
                                                       const uint32_t* __restrict__ items, int32_t n_items,
                                                       int32_t per_wg, int32_t parity) {
-  const int32_t g = static_cast<int32_t>(blockIdx.x);
+  const int32_t g = node_block(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
   const int32_t beg = g * per_wg;
   const int32_t end = beg + per_wg < n_items ? beg + per_wg : n_items;
   for (int32_t it = beg; it < end; ++it) {
