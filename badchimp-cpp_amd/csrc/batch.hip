@@ -169,6 +169,8 @@ struct lb_batch {
   int32_t* d_slots = nullptr;
   double* d_out = nullptr;
   double* h_out = nullptr;  // pinned
+  uint32_t* h_step = nullptr;  // pinned: global step at the start of an interval
+  uint32_t* d_step = nullptr;
   std::vector<uint32_t> items;  // single-phase items, then two-phase items
   int32_t n_items_bgk = 0, n_items_cg = 0;
   std::vector<int32_t> active;
@@ -222,7 +224,7 @@ void shape_dims(const lb_batch* b, int32_t shape, int32_t n_items, int32_t* grid
   *grid = (n_items + *per - 1) / *per;
 }
 
-hipError_t launch_step(lb_batch* b, bool reduce, int32_t shape) {
+hipError_t launch_step(lb_batch* b, bool reduce, int32_t shape, int32_t koff) {
   const int32_t parity = static_cast<int32_t>(b->global_step & 1);
   int32_t grid, per;
   hipError_t e = hipSuccess;
@@ -233,13 +235,13 @@ hipError_t launch_step(lb_batch* b, bool reduce, int32_t shape) {
       e = batch_bgk_f64(b->lattice, reduce, b->d_solves, b->d_items, n, grid, per, parity, b->stream);
     else
       e = batch_bgk_lowp(b->precision, b->lattice, reduce, b->d_solves, b->d_items, n, grid, per, parity,
-                         b->stream);
+                         b->d_step, koff, b->stream);
   }
   if (e == hipSuccess && b->n_items_cg > 0) {
     const int32_t n = b->n_items_cg;
     shape_dims(b, shape, n, &grid, &per);
     e = batch_cg_lowp(b->precision, b->lattice, reduce, b->d_solves, b->d_items + b->n_items_bgk, n, grid, per,
-                      parity, b->stream);
+                      parity, b->d_step, koff, b->stream);
   }
   return e;
 }
@@ -285,13 +287,17 @@ int run_interval(lb_batch* b, bool tune) {
   const int32_t cand[] = {0, 1, 2, 4, 8};
   constexpr int kCand = 5;
   const int start_parity = static_cast<int>(b->global_step & 1);
+  // the global step at the start of the interval, read by the kernels (the
+  // copy runs outside any captured graph)
+  *b->h_step = static_cast<uint32_t>(b->global_step);
+  LB_BHIP(b, hipMemcpyAsync(b->d_step, b->h_step, sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
   if (tune) {
     // time each candidate shape on live steps; results do not depend on the shape
     std::vector<hipEvent_t> ev(2 * M);
     for (auto& e : ev) LB_BHIP(b, hipEventCreate(&e));
     for (int s = 0; s < M; ++s) {
       hipEventRecord(ev[2 * s], b->stream);
-      hipError_t e = launch_step(b, s == M - 1, cand[s % kCand]);
+      hipError_t e = launch_step(b, s == M - 1, cand[s % kCand], s);
       hipEventRecord(ev[2 * s + 1], b->stream);
       if (e != hipSuccess) return bfail(b, LB_ERR_HIP, "batch step: %s", hipGetErrorString(e));
       ++b->global_step;
@@ -329,7 +335,7 @@ int run_interval(lb_batch* b, bool tune) {
       hipError_t e = hipSuccess;
       const int64_t gs = b->global_step;
       for (int s = 0; s < M && e == hipSuccess; ++s) {
-        e = launch_step(b, s == M - 1, b->shape);
+        e = launch_step(b, s == M - 1, b->shape, s);
         ++b->global_step;
       }
       b->global_step = gs;
@@ -406,6 +412,8 @@ int lb_batch_create(int lattice, int precision, int device, int64_t pool_bytes, 
   if (e == hipSuccess) e = hipMalloc(&b->d_slots, kMaxSlots * sizeof(int32_t));
   if (e == hipSuccess) e = hipMalloc(&b->d_out, 3 * kMaxSlots * sizeof(double));
   if (e == hipSuccess) e = hipHostMalloc(&b->h_out, 3 * kMaxSlots * sizeof(double), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc(&b->h_step, sizeof(uint32_t), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc(&b->d_step, sizeof(uint32_t));
   if (e != hipSuccess) {
     const int rc = fail_global(e == hipErrorOutOfMemory ? LB_ERR_NOMEM : LB_ERR_HIP, "lb_batch_create: %s",
                                hipGetErrorString(e));
@@ -428,6 +436,8 @@ int lb_batch_destroy(lb_batch* b) {
   if (b->d_slots) hipFree(b->d_slots);
   if (b->d_out) hipFree(b->d_out);
   if (b->h_out) hipHostFree(b->h_out);
+  if (b->h_step) hipHostFree(b->h_step);
+  if (b->d_step) hipFree(b->d_step);
   if (b->ev0) hipEventDestroy(b->ev0);
   if (b->ev1) hipEventDestroy(b->ev1);
   if (b->stream) hipStreamDestroy(b->stream);
@@ -555,6 +565,7 @@ int add_solve(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* geo, 
   d.W = rb.g.W;
   d.n = static_cast<int32_t>(n);
   d.parity = static_cast<int32_t>(b->global_step & 1);  // f[0] holds the state at the next step
+  d.step_base = static_cast<uint32_t>(b->global_step);   // the solve's own steps start here
   d.kind = p.kind;
   const double tau = p.kind == 1 ? p.tau0 : p.tau;
   d.a.tau_inv = 1.0 / tau;

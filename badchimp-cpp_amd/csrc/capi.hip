@@ -88,6 +88,7 @@ int alloc_fields(lb_ctx* c, int nf) {
   c->n_fields = nf;
   c->cur = 0;
   c->post = false;
+  c->step_count = 0;
   return LB_OK;
 }
 
@@ -519,6 +520,7 @@ int lb_upload_f(lb_ctx* c, int32_t nf, const double* f) {
   }
   c->cur = 0;
   c->post = false;
+  c->step_count = 0;
   c->have_macro = false;
   return LB_OK;
 }
@@ -680,6 +682,11 @@ int lb_set_wall_densities(lb_ctx* c, const double* rho2) {
 }  // extern "C"
 
 namespace {
+BgkArgs with_step(BgkArgs a, uint32_t step) {
+  a.step = step;
+  return a;
+}
+
 // Step parameters of the single-phase loop body: BGK (tau_anti <= 0) or the
 // two-relaxation-time operator, with the reference's constants
 // (LBcollision.h:40, 62-63, 206, 223-224).
@@ -708,10 +715,11 @@ int step_single(lb_ctx* c, int32_t nsteps, const BgkArgs& a) {
                   c->d_vel, c->d_partial, c->stream);
     else
       e = bgk_lowp(c->precision, c->lattice, !c->post, last, false, c->d_f[c->cur], c->d_f[1 - c->cur],
-                   c->addr, c->n_bulk, c->NP, a, c->d_rho, c->d_vel, c->d_partial, c->stream);
+                   c->addr, c->n_bulk, c->NP, with_step(a, c->step_count), c->d_rho, c->d_vel, c->d_partial, c->stream);
     if (e != hipSuccess) return fail(c, LB_ERR_HIP, "bgk launch: %s", hipGetErrorString(e));
     c->cur = 1 - c->cur;
     c->post = true;
+    ++c->step_count;
   }
   const int rc = t.stop(nsteps);
   if (rc != LB_OK) return rc;
@@ -910,6 +918,7 @@ int lb_step_tagged(lb_ctx* c, int32_t nsteps, double tau, const double* force, d
     if (e != hipSuccess) return fail(c, LB_ERR_HIP, "tagged step launch: %s", hipGetErrorString(e));
     c->cur = 1 - c->cur;
     c->post = true;
+    ++c->step_count;
     ++launches;
   }
   rc = t.stop(launches);
@@ -1006,6 +1015,8 @@ int lb_step_cg(lb_ctx* c, int32_t nsteps, double tau0, double tau1, double sigma
       if (e == hipSuccess && momx)
         e = finish_momx(c->d_partial, c->n_blocks, a.momx, c->n_global, c->d_scalar, c->stream);
       if (e == hipSuccess)
+      a.step = c->step_count;
+      if (e == hipSuccess)
         e = cg_lowp(c->precision, c->lattice, !c->post, last, c->d_f[c->cur], c->d_f[1 - c->cur], c->addr, rowwin,
                     c->n_bulk, c->NP, a, phi, wp, c->d_scalar, c->d_rho, c->d_vel, c->stream);
     }
@@ -1013,6 +1024,7 @@ int lb_step_cg(lb_ctx* c, int32_t nsteps, double tau0, double tau1, double sigma
     launches += momx ? 3 : 2;
     c->cur = 1 - c->cur;
     c->post = true;
+    ++c->step_count;
   }
   rc = t.stop(launches);
   if (rc != LB_OK) return rc;
@@ -1089,10 +1101,11 @@ int solve_permeability(lb_ctx* c, const char* name, double tau, double tau_anti,
                     c->d_vel, c->d_partial, c->stream);
       else
         e = bgk_lowp(c->precision, c->lattice, !c->post, last, last, c->d_f[c->cur], c->d_f[1 - c->cur],
-                     c->addr, c->n_bulk, c->NP, a, c->d_rho, c->d_vel, c->d_partial, c->stream);
+                     c->addr, c->n_bulk, c->NP, with_step(a, c->step_count), c->d_rho, c->d_vel, c->d_partial, c->stream);
       if (e != hipSuccess) return fail(c, LB_ERR_HIP, "bgk launch: %s", hipGetErrorString(e));
       c->cur = 1 - c->cur;
       c->post = true;
+      ++c->step_count;
       ++launches;
     }
     steps += chunk;

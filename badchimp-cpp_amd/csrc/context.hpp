@@ -66,6 +66,7 @@ struct BgkArgs {
   const float* tab_srf;
   const float* tab_nuf;
   int32_t tab_n;
+  uint32_t step;       // steps since the state was set: seeds the FP16 stochastic rounding
 };
 constexpr int kBgk = 0, kTrt = 1, kRheology = 2;
 
@@ -85,6 +86,7 @@ struct BatchSolve {
   int32_t kind;     // 0 single-phase BGK, 1 colour-gradient two-phase
   BgkArgs a;        // tau_inv, tau_factor (single-phase) and the body force
   double nu0Inv, nu1Inv, sigma, beta, phi_w;  // two-phase parameters, wall colour
+  uint32_t step_base;  // global batch step at admission: the solve's own step count is global - step_base
 };
 // Work item of a batched launch: slot << kItemShift | 256-node block of the solve.
 constexpr int kItemShift = 18;
@@ -97,6 +99,7 @@ struct CgArgs {
   double F[3];            // fixed force; F[0] replaced by the device value in MOMX mode
   double momx;
   int32_t momx_mode;
+  uint32_t step;          // steps since the state was set (FP16 stochastic rounding)
 };
 
 }  // namespace lbamd

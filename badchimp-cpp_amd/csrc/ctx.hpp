@@ -46,6 +46,7 @@ struct lb_ctx {
   double* d_scalar = nullptr;    // [8]: 0 force x, 1 flux sum, 4..6 reductions
   int64_t n_blocks = 0;
   bool have_macro = false;
+  uint32_t step_count = 0;       // steps since the populations were set (seeds FP16 stochastic rounding)
   bool have_visc = false;        // the last stored step was a rheology step
   // tagged boundaries (std_one_phase): pull table, anti-bounce-back masks,
   // domain labels with mass-source flags, force attribute, per-domain

@@ -27,6 +27,7 @@ template <>
 struct Store<float> {
   static __device__ __forceinline__ float get(float v) { return v; }
   static __device__ __forceinline__ float put(float v) { return v; }
+  static __device__ __forceinline__ float put_sr(float v, uint32_t) { return v; }
 };
 // half values travel as their 16-bit patterns
 template <>
@@ -37,7 +38,39 @@ struct Store<uint16_t> {
   static __device__ __forceinline__ uint16_t put(float v) {
     return __half_as_ushort(__float2half_rn(v * kHalfScale));
   }
+  // Stochastic rounding: the half below |v| (towards zero) or the next one
+  // away from zero, the latter with probability equal to the distance from
+  // the lower one in units of their gap (r: 13 uniform bits).  Round to
+  // nearest freezes a relaxation once its increments drop below half a unit
+  // of the last place, which stops a steady-state solve early with a biased
+  // result; stochastic rounding applies such increments on average.
+  // In the range of normal halves a half keeps the top 10 of a float's 23
+  // fraction bits, so adding r to the float's magnitude bits and clearing
+  // the low 13 carries into the kept part with probability low13 / 8192;
+  // the conversion of the truncated float is then exact.  Inf and NaN keep
+  // their bits; below the normal halves the conversion rounds to nearest.
+  static __device__ __forceinline__ uint16_t put_sr(float v, uint32_t r) {
+    uint32_t u = __float_as_uint(v * kHalfScale);
+    if ((u & 0x7f800000u) != 0x7f800000u) u = (u + r) & ~0x1fffu;
+    return __half_as_ushort(__float2half_rn(__uint_as_float(u)));
+  }
 };
+
+// Random bits of the stochastic rounding: a hash of the node (its index in
+// the solve) and the solve's own step count, then 13 bits per stored value
+// k.  The same solve therefore rounds alike alone, in a batch or in chunks.
+__device__ __forceinline__ uint32_t sr_node(int32_t i, uint32_t step) {
+  uint32_t x = (static_cast<uint32_t>(i) * 0x9E3779B9u) ^ (step * 0x85EBCA6Bu);
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+__device__ __forceinline__ uint32_t sr_bits(uint32_t hn, uint32_t k) {
+  return ((hn ^ (k * 0x9E3779B9u)) * 0x2C1B3C6Du) >> 19;
+}
 
 __device__ __forceinline__ double block_sum(double v) {
   __shared__ double red[kBlock / 64];
@@ -128,8 +161,10 @@ __device__ __forceinline__ double bgk_node_lp(const S* __restrict__ gin, S* __re
                                               const float (&F)[3], double* __restrict__ rho_o,
                                               double* __restrict__ vel_o, float tauA_inv = 0.0f,
                                               float tauA_factor = 0.0f, const float* __restrict__ sr = nullptr,
-                                              const float* __restrict__ nu = nullptr, int tab_n = 0) {
+                                              const float* __restrict__ nu = nullptr, int tab_n = 0,
+                                              uint32_t step = 0) {
   constexpr int Q = L::nQ, D = L::nD;
+  const uint32_t hn = sr_node(i, step);  // unused (removed) for float storage
   float g[Q];
   load_g<L, S, FIRST>(gin, ad, i, NP, g);
   float drho = 0.0f;
@@ -182,14 +217,14 @@ __device__ __forceinline__ double bgk_node_lp(const S* __restrict__ gin, S* __re
       const float ga = 0.5f * (g[q] - g[rq]);
       const float sym = omt * gs + (w * tau_inv) * (a0 + r45 * cu * cu) + (w * tau_factor) * (9.0f * cF * cu + m3uF);
       const float anti = oma * ga + (w * tauA_inv) * (r3 * cu) + (w * tauA_factor) * (3.0f * cF);
-      __builtin_nontemporal_store(Store<S>::put(sym + anti), out + (q << kPLog));
-      __builtin_nontemporal_store(Store<S>::put(sym - anti), out + (rq << kPLog));
+      __builtin_nontemporal_store(Store<S>::put_sr(sym + anti, sr_bits(hn, q)), out + (q << kPLog));
+      __builtin_nontemporal_store(Store<S>::put_sr(sym - anti, sr_bits(hn, rq)), out + (rq << kPLog));
     }
     {
       constexpr int q = Q - 1;  // rest: even only
       const float w = float(L::w(q));
       const float gq = omt * g[q] + (w * tau_inv) * a0 + (w * tau_factor) * m3uF;
-      __builtin_nontemporal_store(Store<S>::put(gq), out + (q << kPLog));
+      __builtin_nontemporal_store(Store<S>::put_sr(gq, sr_bits(hn, q)), out + (q << kPLog));
     }
     return REDUCE ? static_cast<double>(jd[0]) : 0.0;
   }
@@ -207,14 +242,14 @@ __device__ __forceinline__ double bgk_node_lp(const S* __restrict__ gin, S* __re
     const float fod = 3.0f * cF;
     const float gq = omt * g[q] + wt * (ev + od) + wf * (fev + fod);
     const float gr = omt * g[rq] + wt * (ev - od) + wf * (fev - fod);
-    __builtin_nontemporal_store(Store<S>::put(gq), out + (q << kPLog));
-    __builtin_nontemporal_store(Store<S>::put(gr), out + (rq << kPLog));
+    __builtin_nontemporal_store(Store<S>::put_sr(gq, sr_bits(hn, q)), out + (q << kPLog));
+    __builtin_nontemporal_store(Store<S>::put_sr(gr, sr_bits(hn, rq)), out + (rq << kPLog));
   }
   {
     constexpr int q = Q - 1;  // rest: c = 0
     const float w = float(L::w(q));
     const float gq = omt * g[q] + (w * tau_inv) * a0 + (w * tau_factor) * m3uF;
-    __builtin_nontemporal_store(Store<S>::put(gq), out + (q << kPLog));
+    __builtin_nontemporal_store(Store<S>::put_sr(gq, sr_bits(hn, q)), out + (q << kPLog));
   }
   return REDUCE ? static_cast<double>(jd[0]) : 0.0;
 }
@@ -227,14 +262,14 @@ __global__ __launch_bounds__(kBlock) void k_bgk_lp(const S* __restrict__ gin, S*
                                                    double* __restrict__ vel_o,
                                                    double* __restrict__ partial, float tauA_inv,
                                                    float tauA_factor, const float* __restrict__ sr,
-                                                   const float* __restrict__ nu, int tab_n) {
+                                                   const float* __restrict__ nu, int tab_n, uint32_t step) {
   const int32_t b = node_block(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
   const int32_t i = b * kBlock + threadIdx.x;
   double jx = 0.0;
   if (i < n) {
     const float F[3] = {F0, F1, F2};
     jx = bgk_node_lp<L, S, A, FIRST, STORE, REDUCE, MODE>(gin, gout, ad, i, NP, tau_inv, tau_factor, F, rho_o,
-                                                          vel_o, tauA_inv, tauA_factor, sr, nu, tab_n);
+                                                          vel_o, tauA_inv, tauA_factor, sr, nu, tab_n, step);
   }
   if (REDUCE) {
     const double s = block_sum(jx);
@@ -247,7 +282,9 @@ __global__ __launch_bounds__(kBlock) void k_bgk_lp(const S* __restrict__ gin, S*
 template <class L, typename S, bool REDUCE>
 __global__ __launch_bounds__(kBlock) void k_batch_bgk_lp(const BatchSolve* __restrict__ solves,
                                                          const uint32_t* __restrict__ items, int32_t n_items,
-                                                         int32_t per_wg, int32_t parity) {
+                                                         int32_t per_wg, int32_t parity,
+                                                         const uint32_t* __restrict__ step0, int32_t koff) {
+  const uint32_t gstep = *step0 + static_cast<uint32_t>(koff);  // global step of this launch
   const int32_t g = node_block(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
   const int32_t beg = g * per_wg;
   const int32_t end = beg + per_wg < n_items ? beg + per_wg : n_items;
@@ -263,7 +300,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_bgk_lp(const BatchSolve* __res
       const float F[3] = {static_cast<float>(s.a.F[0]), static_cast<float>(s.a.F[1]), static_cast<float>(s.a.F[2])};
       jx = bgk_node_lp<L, S, RowWin, false, false, REDUCE>(
           static_cast<const S*>(s.f[p]), static_cast<S*>(s.f[p ^ 1]), ad, i, s.NP, static_cast<float>(s.a.tau_inv),
-          static_cast<float>(s.a.tau_factor), F, nullptr, nullptr);
+          static_cast<float>(s.a.tau_factor), F, nullptr, nullptr, 0.0f, 0.0f, nullptr, nullptr, 0,
+          gstep - s.step_base);
     }
     if (REDUCE) {
       const double t = block_sum(jx);
@@ -425,8 +463,9 @@ template <class L, typename S, bool O, bool STORE>
 __device__ __forceinline__ void cg_collide(const float (&t)[L::nQ], const float (&dd)[L::nQ], float (&pn)[L::nQ],
                                            const float (&F)[3], float nu0Inv, float nu1Inv, float sigma, float beta,
                                            S* __restrict__ gout, uint32_t fs, int32_t i, int64_t NP,
-                                           double* __restrict__ rho_o, double* __restrict__ vel_o) {
+                                           double* __restrict__ rho_o, double* __restrict__ vel_o, uint32_t step) {
   constexpr int Q = L::nQ, D = L::nD;
+  const uint32_t hn = sr_node(i, step);  // unused (removed) for float storage
   float drho = 0.0f, sd = 0.0f;
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
@@ -482,8 +521,8 @@ __device__ __forceinline__ void cg_collide(const float (&t)[L::nQ], const float 
       rc = 0.0f;
     }
     const float bs = omt * t[q] + ti * geq + dF + st;
-    __builtin_nontemporal_store(Store<S>::put(bs), out + (q << kPLog));
-    __builtin_nontemporal_store(Store<S>::put(phi_n * (w + bs) + 2.0f * rc), outd + (q << kPLog));
+    __builtin_nontemporal_store(Store<S>::put_sr(bs, sr_bits(hn, q)), out + (q << kPLog));
+    __builtin_nontemporal_store(Store<S>::put_sr(phi_n * (w + bs) + 2.0f * rc, sr_bits(hn, q + 32)), outd + (q << kPLog));
   }
 }
 
@@ -516,7 +555,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_lp(const S* __restrict__ gin, S* 
                                                   float sigma, float beta, float F0, float F1, float F2,
                                                   int32_t momx_mode, const float* __restrict__ phi, WallPhi wp,
                                                   const double* __restrict__ dscalar,
-                                                  double* __restrict__ rho_o, double* __restrict__ vel_o) {
+                                                  double* __restrict__ rho_o, double* __restrict__ vel_o,
+                                                  uint32_t step) {
   constexpr int Q = L::nQ;
   const int32_t b = node_block(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
   const int32_t i = b * kBlock + threadIdx.x;
@@ -528,7 +568,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_lp(const S* __restrict__ gin, S* 
   if (momx_mode) F[0] = static_cast<float>(dscalar[0]);
   float pn[Q];
   neighbour_phi<L>(ad, i, phi, wp, pn);
-  cg_collide<L, S, A::kOff32, STORE>(t, dd, pn, F, nu0Inv, nu1Inv, sigma, beta, gout, fs, i, NP, rho_o, vel_o);
+  cg_collide<L, S, A::kOff32, STORE>(t, dd, pn, F, nu0Inv, nu1Inv, sigma, beta, gout, fs, i, NP, rho_o, vel_o,
+                                     step);
 }
 
 // Batched two-phase passes (items as k_batch_bgk_lp; fixed force; every solid
@@ -573,8 +614,10 @@ __global__ __launch_bounds__(kBlock) void k_batch_cg_phi_lp(const BatchSolve* __
 template <class L, typename S>
 __global__ __launch_bounds__(kBlock) void k_batch_cg_lp(const BatchSolve* __restrict__ solves,
                                                         const uint32_t* __restrict__ items, int32_t n_items,
-                                                        int32_t per_wg, int32_t parity) {
+                                                        int32_t per_wg, int32_t parity,
+                                                        const uint32_t* __restrict__ step0, int32_t koff) {
   constexpr int Q = L::nQ;
+  const uint32_t gstep = *step0 + static_cast<uint32_t>(koff);  // global step of this launch
   const int32_t g = node_block(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
   const int32_t beg = g * per_wg;
   const int32_t end = beg + per_wg < n_items ? beg + per_wg : n_items;
@@ -602,7 +645,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_cg_lp(const BatchSolve* __rest
     const float F[3] = {static_cast<float>(s.a.F[0]), static_cast<float>(s.a.F[1]), static_cast<float>(s.a.F[2])};
     cg_collide<L, S, true, false>(t, dd, pn, F, static_cast<float>(s.nu0Inv), static_cast<float>(s.nu1Inv),
                                   static_cast<float>(s.sigma), static_cast<float>(s.beta),
-                                  static_cast<S*>(s.f[p ^ 1]), fs, i, s.NP, nullptr, nullptr);
+                                  static_cast<S*>(s.f[p ^ 1]), fs, i, s.NP, nullptr, nullptr, gstep - s.step_base);
   }
 }
 
@@ -748,7 +791,7 @@ hipError_t bgk_lowp(int precision, int lattice, bool first, bool store, bool red
     const dim3 g = grid_for(n), b(kBlock);
 #define LB_K(F_, S_, R_, T_)                                                                             \
   k_bgk_lp<L, S, A, F_, S_, R_, T_><<<g, b, 0, s>>>(gi, go, addr, n, NP, ti, tf, F0, F1, F2, rho, vel, partial, \
-                                                    tai, taf, a.tab_srf, a.tab_nuf, a.tab_n)
+                                                    tai, taf, a.tab_srf, a.tab_nuf, a.tab_n, a.step)
 #define LB_KEYS(T_)                                 \
   switch (key) {                                    \
     case 0: LB_K(false, false, false, T_); break;   \
@@ -795,26 +838,27 @@ hipError_t flux_lowp(int precision, int lattice, int nf, bool first, const void*
 }
 
 hipError_t batch_bgk_lowp(int precision, int lattice, bool reduce, const BatchSolve* solves, const uint32_t* items,
-                          int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, hipStream_t s) {
+                          int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, const uint32_t* step0,
+                          int32_t koff, hipStream_t s) {
   return dispatch(precision, lattice, Addr{}, [&](auto l, auto st, auto) {
     using L = decltype(l);
     using S = decltype(st);
-    if (reduce)
-      k_batch_bgk_lp<L, S, true><<<dim3(grid), dim3(kBlock), 0, s>>>(solves, items, n_items, per_wg, parity);
-    else
-      k_batch_bgk_lp<L, S, false><<<dim3(grid), dim3(kBlock), 0, s>>>(solves, items, n_items, per_wg, parity);
+    const dim3 g(grid), b(kBlock);
+    if (reduce) k_batch_bgk_lp<L, S, true><<<g, b, 0, s>>>(solves, items, n_items, per_wg, parity, step0, koff);
+    else k_batch_bgk_lp<L, S, false><<<g, b, 0, s>>>(solves, items, n_items, per_wg, parity, step0, koff);
   });
 }
 
 hipError_t batch_cg_lowp(int precision, int lattice, bool reduce, const BatchSolve* solves, const uint32_t* items,
-                         int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, hipStream_t s) {
+                         int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, const uint32_t* step0,
+                         int32_t koff, hipStream_t s) {
   return dispatch(precision, lattice, Addr{}, [&](auto l, auto st, auto) {
     using L = decltype(l);
     using S = decltype(st);
     const dim3 g(grid), b(kBlock);
     if (reduce) k_batch_cg_phi_lp<L, S, true><<<g, b, 0, s>>>(solves, items, n_items, per_wg, parity);
     else k_batch_cg_phi_lp<L, S, false><<<g, b, 0, s>>>(solves, items, n_items, per_wg, parity);
-    k_batch_cg_lp<L, S><<<g, b, 0, s>>>(solves, items, n_items, per_wg, parity);
+    k_batch_cg_lp<L, S><<<g, b, 0, s>>>(solves, items, n_items, per_wg, parity, step0, koff);
   });
 }
 
@@ -865,7 +909,7 @@ hipError_t cg_lowp(int precision, int lattice, bool first, bool store, const voi
       const dim3 g = grid_for(n), b(kBlock);
 #define LB_K(F_, S_)                                                                                        \
   k_cg_lp<L, S, F_, S_, A><<<g, b, 0, s>>>(gi, go, addr, n, NP, n0, n1, sg, be, F0, F1, F2, a.momx_mode, phi, \
-                                           wp, dscalar, rho, vel)
+                                           wp, dscalar, rho, vel, a.step)
       if (first) {
         if (store) LB_K(true, true);
         else LB_K(true, false);

@@ -34,11 +34,16 @@ hipError_t flux_lowp(int precision, int lattice, int nf, bool first, const void*
                      const Addr& ad, int32_t n, int64_t NP, int dir, double* partial,
                      hipStream_t s);
 // Batched form in reduced precision (see batch_bgk_f64).
+// step0: the global step at the start of the launch sequence, koff: this
+// launch's offset from it (the solves' own step counts seed the FP16
+// stochastic rounding).
 hipError_t batch_bgk_lowp(int precision, int lattice, bool reduce, const BatchSolve* solves, const uint32_t* items,
-                          int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, hipStream_t s);
+                          int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, const uint32_t* step0,
+                          int32_t koff, hipStream_t s);
 // Both passes of one batched two-phase step (see k_batch_cg_phi_lp).
 hipError_t batch_cg_lowp(int precision, int lattice, bool reduce, const BatchSolve* solves, const uint32_t* items,
-                         int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, hipStream_t s);
+                         int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, const uint32_t* step0,
+                         int32_t koff, hipStream_t s);
 // Two-phase populations are stored as (t, d) = (f0 + f1 - w, f0 - f1).
 // rowwin: the row windows of ad with one wall colour, else the link table.
 hipError_t cg_phi_lowp(int precision, int lattice, bool first, bool momx, const void* gin, const Addr& ad,

@@ -65,7 +65,7 @@ def geos():
     return [pack((24, 20, 20), 3.0, 0.45, 1), pack((32, 28, 24), 4.0, 0.40, 2), pack((20, 24, 36), 3.0, 0.5, 3)]
 
 
-@pytest.mark.parametrize("precision", [lbamd.FP64, lbamd.FP32S])
+@pytest.mark.parametrize("precision", [lbamd.FP64, lbamd.FP32S, lbamd.FP16S])
 def test_batch_equals_single_context(gpu_available, geos, precision):
     got = run_batch(geos, precision, 50, 1e-6, 20000)
     for i, g in enumerate(geos):
@@ -74,12 +74,15 @@ def test_batch_equals_single_context(gpu_available, geos, precision):
         assert got[i][0] == k and got[i][1] == steps, (i, got[i], k, steps)
 
 
-def test_batch_alone_equals_mix_and_shapes(gpu_available, geos):
+@pytest.mark.parametrize("precision", [lbamd.FP32S, lbamd.FP16S])
+def test_batch_alone_equals_mix_and_shapes(gpu_available, geos, precision):
+    """FP16S also checks the stochastic rounding's step count: it is the
+    solve's own, whenever the solve was admitted."""
     extra = [pack((28, 28, 28), 3.5, 0.42, 9), pack((16, 16, 16), 2.5, 0.5, 10)]
-    alone = run_batch([geos[0]], lbamd.FP32S, 50, 1e-6, 20000)[0]
+    alone = run_batch([geos[0]], precision, 50, 1e-6, 20000)[0]
     # in a mix, admitted later than the others, under each launch shape
     for shape in (-1, 0, 1, 3):
-        mix = run_batch(extra + [geos[0], geos[1]], lbamd.FP32S, 50, 1e-6, 20000, shape=shape, late={3: [2]})
+        mix = run_batch(extra + [geos[0], geos[1]], precision, 50, 1e-6, 20000, shape=shape, late={3: [2]})
         assert mix[2] == alone, (shape, mix[2], alone)
 
 
@@ -132,7 +135,8 @@ def test_fp16s_batch_error(gpu_available, geos):
     ref = run_batch(geos, lbamd.FP64, 50, 1e-6, 20000)
     got = run_batch(geos, lbamd.FP16S, 50, 1e-6, 20000)
     for i in ref:
-        assert abs(got[i][0] - ref[i][0]) / ref[i][0] < 2e-2
+        print("FP16S batch k err %.3e" % (abs(got[i][0] - ref[i][0]) / ref[i][0]))
+        assert abs(got[i][0] - ref[i][0]) / ref[i][0] < 1e-4
 
 
 def tp_single(geo, rho0, wett, precision, tau0, tau1, sigma, beta, steps):

@@ -49,7 +49,7 @@ def test_single_fluid_cell(gpu_available, lattice, shape, periodic):
     assert np.array_equal(f.reshape(T.n_nodes, Q)[T.bulk], fo.reshape(T.n_nodes, Q)[T.bulk])
 
 
-@pytest.mark.parametrize("precision,vtol", [(lbamd.FP64, 0.0), (lbamd.FP32S, 1e-4), (lbamd.FP16S, 2e-2)])
+@pytest.mark.parametrize("precision,vtol", [(lbamd.FP64, 0.0), (lbamd.FP32S, 1e-4), (lbamd.FP16S, 5e-3)])
 def test_d2q9_thin_channel(gpu_available, precision, vtol):
     """A channel three cells wide (D2Q9), every form against the oracle."""
     geo = np.ones((16, 5), dtype=np.int64)
@@ -67,4 +67,5 @@ def test_d2q9_thin_channel(gpu_available, precision, vtol):
     ctx.close()
     b = T.bulk
     err = np.abs(v[b] - vo[b]).max() / np.abs(vo[b]).max()
+    print("vel err %.3e" % err)
     assert err <= vtol

@@ -3,10 +3,11 @@ float arithmetic, FP64 reductions) against the FP64 oracle.
 
 Tolerances (stated per quantity, norm-wise: max |a - b| / max |b|):
   FP32S  velocity 1e-4, density deviation 1e-3 (absolute 1e-9), permeability 1e-4
-  FP16S  velocity 2e-2, permeability 2e-2
+  FP16S  velocity 5e-3, permeability 1e-4 (stores round stochastically, so
+         slow relaxations are not cut short by round-to-nearest)
 two-phase (phase field absolute, velocity norm-wise, fluxes relative):
   FP32S  phi 1e-4, velocity 1e-4, fluxes 1e-5
-  FP16S  phi 1e-2, velocity 2e-2, fluxes 2e-3
+  FP16S  phi 2e-3, velocity 2e-2, fluxes 2e-3
 and, for every layout, the same precision gives bit-identical results
 (the per-node arithmetic does not depend on how sources are addressed).
 """
@@ -43,7 +44,7 @@ def normerr(a, b):
     return np.abs(a - b).max() / np.abs(b).max()
 
 
-@pytest.mark.parametrize("precision,vtol", [(lbamd.FP32S, 1e-4), (lbamd.FP16S, 2e-2)])
+@pytest.mark.parametrize("precision,vtol", [(lbamd.FP32S, 1e-4), (lbamd.FP16S, 5e-3)])
 def test_reduced_precision_vs_oracle(gpu_available, precision, vtol):
     geo, T = pack((28, 24, 20), 4.0, 0.5, 21)
     f0 = oracle.rest_state(1, T.n_nodes, T.bulk, np.ones(T.n_nodes))
@@ -52,6 +53,7 @@ def test_reduced_precision_vs_oracle(gpu_available, precision, vtol):
     fo, ro, vo = oracle.run_std_case(1, T.neighbors, T.node_types, T.bulk, f0, 0.9, force, steps)
     r, v, f = run(T, precision, f0, 0.9, force, steps, "rows")
     b = T.bulk
+    print("vel err %.3e" % normerr(v[b], vo[b]))
     assert normerr(v[b], vo[b]) < vtol
     if precision == lbamd.FP32S:
         assert np.abs(r[b] - ro[b]).max() < max(1e-3 * np.abs(ro[b] - 1).max(), 1e-9)
@@ -69,7 +71,7 @@ def test_layouts_give_identical_results(gpu_available, precision):
         assert np.array_equal(x[T.bulk], y[T.bulk])
 
 
-@pytest.mark.parametrize("precision,ktol", [(lbamd.FP32S, 1e-4), (lbamd.FP16S, 2e-2)])
+@pytest.mark.parametrize("precision,ktol", [(lbamd.FP32S, 1e-4), (lbamd.FP16S, 1e-4)])
 def test_permeability_error(gpu_available, precision, ktol):
     geo, T = pack((48, 40, 40), 5.0, 0.45, 9)
     f0 = oracle.rest_state(1, T.n_nodes, T.bulk, np.ones(T.n_nodes))
@@ -83,6 +85,7 @@ def test_permeability_error(gpu_available, precision, ktol):
     k64, s64 = ks[lbamd.FP64]
     k, s = ks[precision]
     assert k64 > 0 and s64 < 40000
+    print("k err %.3e steps %d / %d" % (abs(k - k64) / k64, s, s64))
     assert abs(k - k64) / k64 < ktol
 
 
@@ -125,7 +128,7 @@ def tp_case(shape, seed, sat=0.5, wett=1.0):
 
 
 @pytest.mark.parametrize("precision,ptol,vtol,qtol", [(lbamd.FP32S, 1e-4, 1e-4, 1e-5),
-                                                      (lbamd.FP16S, 1e-2, 2e-2, 2e-3)])
+                                                      (lbamd.FP16S, 2e-3, 2e-2, 2e-3)])
 def test_twophase_reduced_precision_vs_oracle(gpu_available, precision, ptol, vtol, qtol):
     """Colour-gradient steps on (f0 + f1 - w, f0 - f1) in reduced precision
     against the FP64 oracle: phase field (absolute), velocity (norm-wise) and

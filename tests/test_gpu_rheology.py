@@ -4,7 +4,7 @@ polymer_viscosity_tabulation case and the oracle.
 
 Bar: FP64 bit-exact (array_equal) for populations, rho, vel and the node
 viscosities.  Reduced precision norm-wise against the FP64 oracle: FP32S
-velocity 1e-4 and viscosity 1e-5, FP16S velocity 3e-2.
+velocity 1e-4 and viscosity 1e-5, FP16S velocity 5e-3.
 """
 import os
 import sys
@@ -81,7 +81,7 @@ def test_rheology_vs_oracle_random(gpu_available, lattice, shape, periodic, seed
     assert np.unique(wo[b]).size > 10  # the table is exercised, not one row
 
 
-@pytest.mark.parametrize("precision,vtol,wtol", [(lbamd.FP32S, 1e-4, 1e-5), (lbamd.FP16S, 3e-2, None)])
+@pytest.mark.parametrize("precision,vtol,wtol", [(lbamd.FP32S, 1e-4, 1e-5), (lbamd.FP16S, 5e-3, None)])
 def test_rheology_reduced_precision_vs_oracle(gpu_available, precision, vtol, wtol):
     geo, _ = lbamd.sphere_pack((28, 24, 20), 4.0, 0.5, 21, periodic="x", walls=True)
     T = OracleTopo(geo.astype(np.int64), "D3Q19", "x")
@@ -91,6 +91,8 @@ def test_rheology_reduced_precision_vs_oracle(gpu_available, precision, vtol, wt
     fo, ro, vo, wo = oracle.run_rheology(1, T.neighbors, T.node_types, T.bulk, f0, table, force, 300)
     f, r, v, w = gpu_rheo(T, f0, table, force, [300], precision=precision, layout="rows")
     b = T.bulk
+    print("vel err %.3e visc err %.3e" % (np.abs(v[b] - vo[b]).max() / np.abs(vo[b]).max(),
+                                          np.abs(w[b] - wo[b]).max() / np.abs(wo[b]).max()))
     assert np.abs(v[b] - vo[b]).max() / np.abs(vo[b]).max() < vtol
     if wtol is not None:
         assert np.abs(w[b] - wo[b]).max() / np.abs(wo[b]).max() < wtol

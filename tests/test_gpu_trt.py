@@ -4,7 +4,7 @@ the oracle.
 
 Bar: FP64 bit-exact (array_equal) for populations, rho and vel; the
 permeability sum 1e-12 relative.  Reduced precision as for BGK, norm-wise:
-FP32S velocity 1e-4, FP16S 2e-2.
+FP32S velocity 1e-4, FP16S 5e-3.
 """
 import numpy as np
 import pytest
@@ -83,7 +83,7 @@ def pack(shape, radius, porosity, seed, periodic="x"):
     return geo.astype(np.int64), OracleTopo(geo.astype(np.int64), "D3Q19", periodic)
 
 
-@pytest.mark.parametrize("precision,vtol", [(lbamd.FP32S, 1e-4), (lbamd.FP16S, 2e-2)])
+@pytest.mark.parametrize("precision,vtol", [(lbamd.FP32S, 1e-4), (lbamd.FP16S, 5e-3)])
 def test_trt_reduced_precision_vs_oracle(gpu_available, precision, vtol):
     geo, T = pack((28, 24, 20), 4.0, 0.5, 21)
     f0 = oracle.rest_state(1, T.n_nodes, T.bulk, np.ones(T.n_nodes))
@@ -92,6 +92,7 @@ def test_trt_reduced_precision_vs_oracle(gpu_available, precision, vtol):
     fo, ro, vo = oracle.run_std_case(1, T.neighbors, T.node_types, T.bulk, f0, taus, force, 300)
     f, r, v = gpu_trt(T, f0, taus, force, [300], precision=precision, layout="rows")
     b = T.bulk
+    print("vel err %.3e" % (np.abs(v[b] - vo[b]).max() / np.abs(vo[b]).max()))
     assert np.abs(v[b] - vo[b]).max() / np.abs(vo[b]).max() < vtol
     if precision == lbamd.FP32S:
         assert np.abs(r[b] - ro[b]).max() < max(1e-3 * np.abs(ro[b] - 1).max(), 1e-9)
