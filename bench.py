@@ -13,8 +13,11 @@ roofline = the dominant kernel (the collide-stream step) against 8 TB/s,
            (populations read once and written once; s = 4 B FP32S, 8 B FP64).
 permeability = k of both forms run to |dk|/k < 1e-7 (checked every 100
            steps), and the relative error of FP32S against FP64.
-cpu_baseline = the oracle's CPU restatement of the reference loop on the same
-           node list, on the host's cores, for a bounded number of steps.
+cpu_baseline = the compiled reference std_case (oracle/_ref, built from the
+           reference's sources with MPICH) on the host's cores, one MPI rank
+           per core (at most 16), on a 128^3 pack of the same generator, from
+           two differenced runs; "port" beside it is the oracle's OpenMP
+           restatement of the same loop on the full 256^3 node list.
 batched  = the ensemble of independent sphere-pack permeability solves (cubic
            sizes log-uniform in 32..128, porosity U(0.30, 0.50), radius
            U(3, 8), one seed; |dk|/k < 1e-6 checked every 100 steps), all
@@ -49,6 +52,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256]
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -254,6 +258,24 @@ def cpu_baseline(nb, types, bulk, n_nodes, tau, force, budget_s=15.0):
     return {"value": bulk.size * steps / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
             "sample": "oracle std_case loop (push + bounce-back, FP64) on the same 256^3 node list, "
                       "%d steps, %d OpenMP threads" % (steps, threads)}
+
+
+def cpu_reference(tau, force, size=128, n_a=20, n_b=520):
+    """The compiled reference std_case (oracle/_ref, MPICH) on the host's
+    cores, one MPI rank per core, on a bounded sample of the workload: the
+    same sphere-pack generator (radius 8, porosity 0.40, seed 1, periodic x,
+    walls) at size^3, steps n_b - n_a from two differenced runs."""
+    from oracle import ref_run
+    if not ref_run.available():
+        raise RuntimeError("reference build or MPICH runtime absent")
+    ranks = min(16, os.cpu_count() or 1)
+    geo, info = lbamd.sphere_pack((size,) * 3, 8.0, 0.40, 1, periodic="x", walls=True)
+    dt, n_bulk, wall = ref_run.time_std_case(geo, "x", tau, force, ranks, n_a, n_b)
+    return {"value": n_bulk * (n_b - n_a) / dt / 1e6, "unit": "MLUPS", "cores": ranks, "kind": "reference",
+            "sample": "reference std_case (src/std_case/main.cpp, D3Q19, -O3, MPICH) on a %d^3 pack of the same "
+                      "generator (%d bulk nodes, porosity %.3f), %d MPI ranks (x slabs), %d steps: runs of %d and "
+                      "%d steps differenced (%.1f s, %.1f s)" % (size, n_bulk, info["porosity"], ranks, n_b - n_a,
+                                                               n_a, n_b, wall[n_a], wall[n_b])}
 
 
 def ensemble(n, seed=2024):
@@ -674,8 +696,15 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(nb, types, bulk, topo.n_nodes, tau, force)
-        log("CPU baseline: %.2f MLUPS on %d threads" % (cpu["value"], cpu["cores"]))
+        port = cpu_baseline(nb, types, bulk, topo.n_nodes, tau, force)
+        log("CPU port: %.2f MLUPS on %d threads" % (port["value"], port["cores"]))
+        try:
+            cpu = cpu_reference(tau, force)
+            cpu["port"] = port
+            log("CPU reference: %.2f MLUPS on %d MPI ranks" % (cpu["value"], cpu["cores"]))
+        except (RuntimeError, OSError, subprocess.SubprocessError) as e:
+            cpu = dict(port, reference_error=str(e)[:300])
+            log("CPU reference not timed (%s); the port stands in" % e)
 
     line = {
         "metric": METRIC,

@@ -6,6 +6,7 @@ CPU-baseline leg of ``bench.py`` import this package, and only as the checker.
   oracle.vtklb_oracle   numpy restatement of PythonScripts/vtklb.py (node list)
   oracle.oracle.c       C restatement of the std_case / twophase loops
   oracle.run_*          ctypes wrappers of liboracle.so (built by oracle/Makefile)
+  oracle.ref_run        timing of the compiled reference (oracle/_ref) under MPI
 """
 import ctypes as C
 import os
