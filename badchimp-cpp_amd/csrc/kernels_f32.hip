@@ -29,6 +29,9 @@ struct Store<float> {
   static __device__ __forceinline__ float put(float v) { return v; }
   static __device__ __forceinline__ float put_sr(float v, uint32_t) { return v; }
 };
+// two halves in one register (the operand type of the half conversions)
+typedef __fp16 HalfPair __attribute__((ext_vector_type(2)));
+
 // half values travel as their 16-bit patterns
 template <>
 struct Store<uint16_t> {
@@ -40,36 +43,46 @@ struct Store<uint16_t> {
   }
   // Stochastic rounding: the half below |v| (towards zero) or the next one
   // away from zero, the latter with probability equal to the distance from
-  // the lower one in units of their gap (r: 13 uniform bits).  Round to
-  // nearest freezes a relaxation once its increments drop below half a unit
-  // of the last place, which stops a steady-state solve early with a biased
-  // result; stochastic rounding applies such increments on average.
-  // In the range of normal halves a half keeps the top 10 of a float's 23
-  // fraction bits, so adding r to the float's magnitude bits and clearing
-  // the low 13 carries into the kept part with probability low13 / 8192;
-  // the conversion of the truncated float is then exact.  Inf and NaN keep
-  // their bits; below the normal halves the conversion rounds to nearest.
+  // the lower one in units of their gap, drawn from the uniform word r.
+  // Round to nearest freezes a relaxation once its increments drop below
+  // half a unit of the last place, which stops a steady-state solve early
+  // with a biased result; stochastic rounding applies such increments on
+  // average.  gfx950 converts with stochastic rounding in one instruction
+  // (v_cvt_sr_f16_f32).  Its behaviour on this hardware: for every
+  // normal-range half the result is one of those two neighbours and the mean
+  // over all 13-bit random patterns equals the float exactly; zero, inf,
+  // NaN and overflow convert as with round to nearest.
   static __device__ __forceinline__ uint16_t put_sr(float v, uint32_t r) {
-    uint32_t u = __float_as_uint(v * kHalfScale);
-    if ((u & 0x7f800000u) != 0x7f800000u) u = (u + r) & ~0x1fffu;
-    return __half_as_ushort(__float2half_rn(__uint_as_float(u)));
+    HalfPair o = __builtin_bit_cast(HalfPair, r);  // the high half is not used
+    o = __builtin_amdgcn_cvt_sr_f16_f32(o, v * kHalfScale, r, false);
+    return static_cast<uint16_t>(__builtin_bit_cast(uint32_t, o));
   }
 };
 
-// Random bits of the stochastic rounding: a hash of the node (its index in
-// the solve) and the solve's own step count, then 13 bits per stored value
-// k.  The same solve therefore rounds alike alone, in a batch or in chunks.
-__device__ __forceinline__ uint32_t sr_node(int32_t i, uint32_t step) {
+// Random words of the stochastic rounding: two hashes of the node (its index
+// in the solve) and the solve's own step count, then one word per stored
+// value k: a rotation of the first hash, xor the second, plus a per-k
+// constant.  Each word is uniform; the rotations differ between the values
+// of one node.  The same solve therefore rounds alike alone, in a batch or
+// in chunks.
+struct SrSeed {
+  uint32_t a, b;
+};
+__device__ __forceinline__ SrSeed sr_node(int32_t i, uint32_t step) {
   uint32_t x = (static_cast<uint32_t>(i) * 0x9E3779B9u) ^ (step * 0x85EBCA6Bu);
   x ^= x >> 16;
   x *= 0x7feb352du;
   x ^= x >> 15;
   x *= 0x846ca68bu;
   x ^= x >> 16;
-  return x;
+  uint32_t y = (x ^ 0x5bd1e995u) * 0x2C1B3C6Du;
+  y ^= y >> 15;
+  return SrSeed{x, y};
 }
-__device__ __forceinline__ uint32_t sr_bits(uint32_t hn, uint32_t k) {
-  return ((hn ^ (k * 0x9E3779B9u)) * 0x2C1B3C6Du) >> 19;
+// k is a constant once the loops over directions are unrolled
+__device__ __forceinline__ uint32_t sr_bits(SrSeed h, uint32_t k) {
+  const uint32_t s = (11u * k + (k >> 5) * 5u) & 31u;  // distinct for the values of one node
+  return (__builtin_amdgcn_alignbit(h.a, h.a, s) ^ h.b) + k * 0x9E3779B9u;
 }
 
 __device__ __forceinline__ double block_sum(double v) {
@@ -164,7 +177,7 @@ __device__ __forceinline__ double bgk_node_lp(const S* __restrict__ gin, S* __re
                                               const float* __restrict__ nu = nullptr, int tab_n = 0,
                                               uint32_t step = 0) {
   constexpr int Q = L::nQ, D = L::nD;
-  const uint32_t hn = sr_node(i, step);  // unused (removed) for float storage
+  const SrSeed hn = sr_node(i, step);  // unused (removed) for float storage
   float g[Q];
   load_g<L, S, FIRST>(gin, ad, i, NP, g);
   float drho = 0.0f;
@@ -465,7 +478,7 @@ __device__ __forceinline__ void cg_collide(const float (&t)[L::nQ], const float 
                                            S* __restrict__ gout, uint32_t fs, int32_t i, int64_t NP,
                                            double* __restrict__ rho_o, double* __restrict__ vel_o, uint32_t step) {
   constexpr int Q = L::nQ, D = L::nD;
-  const uint32_t hn = sr_node(i, step);  // unused (removed) for float storage
+  const SrSeed hn = sr_node(i, step);  // unused (removed) for float storage
   float drho = 0.0f, sd = 0.0f;
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
