@@ -33,19 +33,20 @@ channel  = the D2Q9 256x128 channel (solid rows y = 0, 127, periodic x, tau 1,
 twophase = the colour-gradient case: 192^3 sphere pack (radius 8, porosity
            0.38), phases drawn per pore node with saturation 0.5, solid
            wettability 1, tau0 = tau1 = 1, sigma = 0.01, beta = 0.7, body force
-           1e-6 on the total, 50000 steps in FP64 and in FP32S: MLUPS of both,
+           1e-6 on the total, 50000 steps in FP64, FP32S and FP16S: MLUPS,
            the phase field difference and the phase permeabilities
            k_i = nu_i <phi_i sum_q f c_x> / F_x of both.
 serving  = a stream of requests, 70 % permeability (single-phase) and 30 %
            relative permeability (colour-gradient, saturation U(0.2, 0.8),
            phases drawn per pore cell), cubic sizes U(48, 160), seeded sphere
            packs; admitted by HBM footprint into one pool, advanced together
-           (continuous batching, FP32S), each resident until it converges.
-           First a burst (all requests at t = 0) gives the capacity in
-           solves/s; then Poisson arrivals at 0.7 x that capacity give p50/p99
-           latency.  The same stop rules on the plain FP64 path one request at
-           a time (a size-stratified subset, scaled) give the baseline
-           solves/s and the k error.
+           (continuous batching), each resident until it converges.  In FP32S
+           (the top-level figures) and in FP16S ("fp16s"): first a burst (all
+           requests at t = 0) gives the capacity in solves/s; then Poisson
+           arrivals at 0.7 x that capacity give p50/p99 latency.  The same
+           stop rules on the plain FP64 path one request at a time (a
+           size-stratified subset, scaled) give the baseline solves/s and the
+           k error of both forms.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256]
 """
@@ -540,78 +541,83 @@ def twophase_leg(size, steps, seed=3):
     return out
 
 
-def serving_leg(n_req, pool_gb, every):
+def serve_run(pool, precision, n_req, seed, t_rate=None):
+    """One serving stream: (results, wall seconds, kernel ms, LUPs)."""
     from lbamd.serve import Server, run_stream
-    pool = int(pool_gb * 1e9)
-    # 1) burst: capacity
-    reqs = request_mix(n_req, 7)
-    import copy
-    base_reqs = {r.tag: copy.copy(r) for r in reqs}  # the server drops its copies' geometries
-    srv = Server(pool, precision=lbamd.FP32S, check_every=100)
-    done, wall = run_stream(srv, reqs)
+    srv = Server(pool, precision=precision, check_every=100)
+    done, wall = run_stream(srv, request_mix(n_req, seed, t_rate=t_rate))
     kms, _, lups = srv.batch.kernel_time()
     srv.close()
-    cap = len(done) / wall
-    lat_b = np.array([d["latency"] for d in done])
-    log("serving burst: %d requests in %.1f s = %.2f solves/s (p50 %.2f s, p99 %.2f s)" %
-        (len(done), wall, cap, np.percentile(lat_b, 50), np.percentile(lat_b, 99)))
-    # 2) Poisson arrivals at 0.7 x capacity
-    rate = 0.7 * cap
-    reqs2 = request_mix(n_req, 8, t_rate=rate)
-    srv = Server(pool, precision=lbamd.FP32S, check_every=100)
-    done2, wall2 = run_stream(srv, reqs2)
-    srv.close()
-    lat = np.array([d["latency"] for d in done2])
-    log("serving Poisson %.2f/s: %d requests in %.1f s, p50 %.2f s, p99 %.2f s" %
-        (rate, len(done2), wall2, np.percentile(lat, 50), np.percentile(lat, 99)))
-    # 3) plain FP64 one request at a time on a subset of the burst stratified by
-    #    kind and size (every m-th request of each kind by cell count), each
-    #    kind's time scaled by its own count
-    byres = {d["tag"]: d for d in done}
-    est, errs, errs_tp, n_sub = 0.0, [], [], 0
+    return done, wall, kms, lups
+
+
+def serving_leg(n_req, pool_gb, every):
+    pool = int(pool_gb * 1e9)
+    base_reqs = {r.tag: r for r in request_mix(n_req, 7)}
+    # 3) the plain FP64 path one request at a time on a subset of the burst
+    #    stratified by kind and size (every m-th request of each kind by cell
+    #    count), each kind's time scaled by its own count
+    ref, est, n_sub = {}, 0.0, 0
     for kind in ("perm", "relperm"):
-        tags = [t for t in range(n_req) if base_reqs[t].kind == kind]
+        tags = sorted((t for t in range(n_req) if base_reqs[t].kind == kind), key=lambda t: base_reqs[t].geo.size)
         if not tags:
             continue
-        tags.sort(key=lambda t: base_reqs[t].geo.size)
         m = max(1, min(every, len(tags)))
         sub = tags[m // 2::m] or tags[:1]
         t_kind = 0.0
         for t in sub:
             k, steps, dt = fp64_one(base_reqs[t])
+            ref[t] = (kind, k)
             t_kind += dt
-            got = byres[t]
-            if kind == "perm":
-                errs.append(abs(got["k"] - k) / abs(k))
-            else:
-                scale = abs(k[0]) + abs(k[1])
-                errs_tp.append(max(abs(got["k_phase"][0] - k[0]), abs(got["k_phase"][1] - k[1])) / scale)
         est += t_kind * len(tags) / len(sub)
         n_sub += len(sub)
-    per_kind = {}
-    for kind in ("perm", "relperm"):
-        ds = [d for d in done if d["kind"] == kind]
-        if ds:
-            per_kind[kind] = {"n": len(ds), "steps_median": int(np.median([d["steps"] for d in ds])),
-                              "max_steps_reached": int(sum(1 for d in ds if d["status"] == 1)),
-                              "latency_p50_s": float(np.percentile([d["latency"] for d in ds], 50))}
-    subset = range(n_sub)
+    fp64_rate = n_req / est
+
+    def errors(done):
+        byres = {d["tag"]: d for d in done}
+        e_perm, e_rel = [], []
+        for t, (kind, k) in ref.items():
+            got = byres[t]
+            if kind == "perm":
+                e_perm.append(abs(got["k"] - k) / abs(k))
+            else:
+                scale = abs(k[0]) + abs(k[1])
+                e_rel.append(max(abs(got["k_phase"][0] - k[0]), abs(got["k_phase"][1] - k[1])) / scale)
+        return (float(max(e_perm)) if e_perm else None), (float(max(e_rel)) if e_rel else None)
+
     out = {"n_requests": n_req, "mix": "70% permeability / 30% relative permeability, L U(48,160)",
-           "pool_GB": pool / 1e9,
-           "burst_solves_per_s": cap, "burst_wall_s": wall,
-           "burst_device_GLUPS": lups / (kms * 1e-3) / 1e9 if kms > 0 else None,
-           "burst_device_busy_frac": kms * 1e-3 / wall,
-           "poisson_rate_per_s": rate, "poisson_wall_s": wall2, "poisson_solves_per_s": len(done2) / wall2,
-           "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
-           "fp64_one_at_a_time_solves_per_s": n_req / est, "fp64_subset": int(len(subset)),
-           "speedup_vs_fp64_one_at_a_time": est / wall,
-           "k_rel_err_max_perm": float(max(errs)) if errs else None,
-           "k_rel_err_max_relperm": float(max(errs_tp)) if errs_tp else None,
-           "two_phase_stop": "|dk_i| <= 1e-4 (|k_0|+|k_1|) per 100 steps, at most 20000 steps",
-           "steps_median": int(np.median([d["steps"] for d in done])), "per_kind_burst": per_kind}
-    log("FP64 one at a time (subset %d, scaled): %.3f solves/s -> x%.2f; k err perm %s relperm %s" %
-        (len(subset), out["fp64_one_at_a_time_solves_per_s"], out["speedup_vs_fp64_one_at_a_time"],
-         out["k_rel_err_max_perm"], out["k_rel_err_max_relperm"]))
+           "pool_GB": pool / 1e9, "fp64_one_at_a_time_solves_per_s": fp64_rate, "fp64_subset": n_sub,
+           "two_phase_stop": "|dk_i| <= 1e-4 (|k_0|+|k_1|) per 100 steps, at most 20000 steps"}
+    for precision, name in ((lbamd.FP32S, "fp32s"), (lbamd.FP16S, "fp16s")):
+        # 1) burst: capacity
+        done, wall, kms, lups = serve_run(pool, precision, n_req, 7)
+        cap = len(done) / wall
+        # 2) Poisson arrivals at 0.7 x capacity
+        done2, wall2, _, _ = serve_run(pool, precision, n_req, 8, t_rate=0.7 * cap)
+        lat = np.array([d["latency"] for d in done2])
+        e_perm, e_rel = errors(done)
+        per_kind = {}
+        for kind in ("perm", "relperm"):
+            ds = [d for d in done if d["kind"] == kind]
+            if ds:
+                per_kind[kind] = {"n": len(ds), "steps_median": int(np.median([d["steps"] for d in ds])),
+                                  "max_steps_reached": int(sum(1 for d in ds if d["status"] == 1)),
+                                  "latency_p50_s": float(np.percentile([d["latency"] for d in ds], 50))}
+        leg = {"burst_solves_per_s": cap, "burst_wall_s": wall,
+               "burst_device_GLUPS": lups / (kms * 1e-3) / 1e9 if kms > 0 else None,
+               "burst_device_busy_frac": kms * 1e-3 / wall,
+               "poisson_rate_per_s": 0.7 * cap, "poisson_wall_s": wall2, "poisson_solves_per_s": len(done2) / wall2,
+               "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+               "speedup_vs_fp64_one_at_a_time": cap / fp64_rate,
+               "k_rel_err_max_perm": e_perm, "k_rel_err_max_relperm": e_rel,
+               "steps_median": int(np.median([d["steps"] for d in done])), "per_kind_burst": per_kind}
+        log("serving %s: burst %.2f solves/s (x%.2f vs FP64 one at a time %.3f/s), Poisson %.2f/s p50 %.2f s "
+            "p99 %.2f s; k err perm %s relperm %s" % (name, cap, cap / fp64_rate, fp64_rate, 0.7 * cap,
+                                                      leg["p50_latency_s"], leg["p99_latency_s"], e_perm, e_rel))
+        if precision == lbamd.FP32S:
+            out.update(leg)
+        else:
+            out["fp16s"] = leg
     return out
 
 
