@@ -1,5 +1,6 @@
 """Edge cases of the node list: no bulk node, a single fluid cell, one-cell
-wide channels, D2Q9 in the reduced-precision forms."""
+wide channels, D2Q9 in the reduced-precision forms, a box beyond the row
+layouts' coordinate range."""
 import numpy as np
 import pytest
 
@@ -69,3 +70,31 @@ def test_d2q9_thin_channel(gpu_available, precision, vtol):
     err = np.abs(v[b] - vo[b]).max() / np.abs(vo[b]).max()
     print("vel err %.3e" % err)
     assert err <= vtol
+
+
+@pytest.mark.parametrize("precision", [lbamd.FP64, lbamd.FP32S])
+def test_box_beyond_packed_coordinates_uses_link_table(gpu_available, precision):
+    """A box longer than the row layouts' packed coordinates allow (x > 1024
+    cells) is refused by them with the reason in the layout note; the link
+    table carries it, FP64 bit-exact against the oracle, FP32S within 1e-4."""
+    geo = np.ones((1100, 5, 3), dtype=np.int64)
+    geo[:, 0] = 0
+    geo[:, -1] = 0
+    geo[::7, 2, 1] = 0  # obstacles along the channel
+    T = OracleTopo(geo, "D3Q19", "xz")
+    rng = np.random.default_rng(9)
+    f0 = oracle.rest_state(1, T.n_nodes, T.bulk, 1.0 + 0.01 * rng.random(T.n_nodes))
+    F = np.array([1e-5, 0.0, 2e-6])
+    fo, ro, vo = oracle.run_std_case(1, T.neighbors, T.node_types, T.bulk, f0, 0.8, F, 30)
+    ctx = lbamd.Context(lbamd.D3Q19, precision)
+    ctx.set_topology_grid(T.neighbors, T.node_types, T.bulk, T.positions)
+    assert ctx.layout == 0 and "too large" in ctx.layout_note, (ctx.layout, ctx.layout_note)
+    ctx.upload_f(f0)
+    ctx.step_bgk(30, 0.8, F)
+    r, v = ctx.download_scalar(lbamd.RHO), ctx.download_vel()
+    ctx.close()
+    b = T.bulk
+    if precision == lbamd.FP64:
+        assert np.array_equal(r[b], ro[b]) and np.array_equal(v[b], vo[b])
+    else:
+        assert np.abs(v[b] - vo[b]).max() / np.abs(vo[b]).max() < 1e-4
