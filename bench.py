@@ -44,9 +44,11 @@ serving  = a stream of requests, 70 % permeability (single-phase) and 30 %
            (the top-level figures) and in FP16S ("fp16s"): first a burst (all
            requests at t = 0) gives the capacity in solves/s; then Poisson
            arrivals at 0.7 x that capacity give p50/p99 latency.  The same
-           stop rules on the plain FP64 path one request at a time (a
-           size-stratified subset, scaled) give the baseline solves/s and the
-           k error of both forms.
+           stop rules on the plain FP64 path one request at a time (every
+           relative-permeability request and a size-stratified subset of the
+           permeability requests, scaled) give the baseline solves/s and the
+           k error of both forms, the relative-permeability error split by
+           whether the FP64 run converged or stopped at its step limit.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256]
 """
@@ -554,36 +556,45 @@ def serve_run(pool, precision, n_req, seed, t_rate=None):
 def serving_leg(n_req, pool_gb, every):
     pool = int(pool_gb * 1e9)
     base_reqs = {r.tag: r for r in request_mix(n_req, 7)}
-    # 3) the plain FP64 path one request at a time on a subset of the burst
-    #    stratified by kind and size (every m-th request of each kind by cell
-    #    count), each kind's time scaled by its own count
+    # 3) the plain FP64 path one request at a time: every relative-permeability
+    #    request, and a subset of the permeability requests stratified by size
+    #    (every m-th by cell count); each kind's time scaled by its own count
     ref, est, n_sub = {}, 0.0, 0
     for kind in ("perm", "relperm"):
         tags = sorted((t for t in range(n_req) if base_reqs[t].kind == kind), key=lambda t: base_reqs[t].geo.size)
         if not tags:
             continue
-        m = max(1, min(every, len(tags)))
+        m = max(1, min(every, len(tags))) if kind == "perm" else 1
         sub = tags[m // 2::m] or tags[:1]
         t_kind = 0.0
         for t in sub:
             k, steps, dt = fp64_one(base_reqs[t])
-            ref[t] = (kind, k)
+            ref[t] = (kind, k, steps)
             t_kind += dt
         est += t_kind * len(tags) / len(sub)
         n_sub += len(sub)
     fp64_rate = n_req / est
 
     def errors(done):
+        """Max k error of the permeability subset; the relative-permeability
+        error (max over both phases, of k0 + k1) of every request, split by
+        whether the FP64 run converged or stopped at max_steps."""
         byres = {d["tag"]: d for d in done}
-        e_perm, e_rel = [], []
-        for t, (kind, k) in ref.items():
+        e_perm, e_conv, e_cap = [], [], []
+        for t, (kind, k, steps) in ref.items():
             got = byres[t]
             if kind == "perm":
                 e_perm.append(abs(got["k"] - k) / abs(k))
             else:
                 scale = abs(k[0]) + abs(k[1])
-                e_rel.append(max(abs(got["k_phase"][0] - k[0]), abs(got["k_phase"][1] - k[1])) / scale)
-        return (float(max(e_perm)) if e_perm else None), (float(max(e_rel)) if e_rel else None)
+                e = max(abs(got["k_phase"][0] - k[0]), abs(got["k_phase"][1] - k[1])) / scale
+                (e_cap if steps >= base_reqs[t].max_steps else e_conv).append(e)
+        mx = lambda v: float(max(v)) if v else None  # noqa: E731
+        e_all = e_conv + e_cap
+        return {"k_rel_err_max_perm": mx(e_perm), "k_rel_err_max_relperm": mx(e_all),
+                "k_rel_err_median_relperm": float(np.median(e_all)) if e_all else None,
+                "k_rel_err_max_relperm_converged": mx(e_conv), "n_relperm_converged": len(e_conv),
+                "k_rel_err_max_relperm_at_max_steps": mx(e_cap), "n_relperm_at_max_steps": len(e_cap)}
 
     out = {"n_requests": n_req, "mix": "70% permeability / 30% relative permeability, L U(48,160)",
            "pool_GB": pool / 1e9, "fp64_one_at_a_time_solves_per_s": fp64_rate, "fp64_subset": n_sub,
@@ -595,7 +606,7 @@ def serving_leg(n_req, pool_gb, every):
         # 2) Poisson arrivals at 0.7 x capacity
         done2, wall2, _, _ = serve_run(pool, precision, n_req, 8, t_rate=0.7 * cap)
         lat = np.array([d["latency"] for d in done2])
-        e_perm, e_rel = errors(done)
+        err = errors(done)
         per_kind = {}
         for kind in ("perm", "relperm"):
             ds = [d for d in done if d["kind"] == kind]
@@ -609,11 +620,13 @@ def serving_leg(n_req, pool_gb, every):
                "poisson_rate_per_s": 0.7 * cap, "poisson_wall_s": wall2, "poisson_solves_per_s": len(done2) / wall2,
                "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
                "speedup_vs_fp64_one_at_a_time": cap / fp64_rate,
-               "k_rel_err_max_perm": e_perm, "k_rel_err_max_relperm": e_rel,
                "steps_median": int(np.median([d["steps"] for d in done])), "per_kind_burst": per_kind}
+        leg.update(err)
         log("serving %s: burst %.2f solves/s (x%.2f vs FP64 one at a time %.3f/s), Poisson %.2f/s p50 %.2f s "
-            "p99 %.2f s; k err perm %s relperm %s" % (name, cap, cap / fp64_rate, fp64_rate, 0.7 * cap,
-                                                      leg["p50_latency_s"], leg["p99_latency_s"], e_perm, e_rel))
+            "p99 %.2f s; k err perm %s, relperm converged %s (%d), at max steps %s (%d)" % (
+                name, cap, cap / fp64_rate, fp64_rate, 0.7 * cap, leg["p50_latency_s"], leg["p99_latency_s"],
+                err["k_rel_err_max_perm"], err["k_rel_err_max_relperm_converged"], err["n_relperm_converged"],
+                err["k_rel_err_max_relperm_at_max_steps"], err["n_relperm_at_max_steps"]))
         if precision == lbamd.FP32S:
             out.update(leg)
         else:
