@@ -10,6 +10,8 @@ two-phase (phase field absolute, velocity norm-wise, fluxes relative):
   FP16S  phi 2e-3, velocity 2e-2, fluxes 2e-3
 and, for every layout, the same precision gives bit-identical results
 (the per-node arithmetic does not depend on how sources are addressed).
+FP16S stores round stochastically: increments below half a unit of a
+stored half are kept on average (test_fp16s_small_increments_are_not_lost).
 """
 import numpy as np
 import pytest
@@ -194,3 +196,28 @@ def test_bgk_chunking_is_exact(gpu_available, precision):
         ctx.close()
     assert np.array_equal(outs[0][0][T.bulk], outs[1][0][T.bulk])
     assert np.array_equal(outs[0][1][T.bulk], outs[1][1][T.bulk])
+
+
+@pytest.mark.parametrize("force,steps", [(5e-9, 3000), (2e-8, 3000), (1e-7, 2000)])
+def test_fp16s_small_increments_are_not_lost(gpu_available, force, steps):
+    """A channel driven by a force so small that the per-step change of a
+    stored half is below half a unit of its last place: FP16S stores round
+    stochastically, so the mean velocity follows FP64 to 1e-4 (measured
+    1e-6 to 6e-6).  Round to nearest loses those increments: a build that
+    rounds to nearest was off by 1.3e-2 to 2.8e-2 on these cases."""
+    geo = np.ones((16, 14, 8), dtype=np.int64)
+    geo[:, 0] = 0
+    geo[:, -1] = 0
+    T = OracleTopo(geo, "D3Q19", "xz")
+    f0 = oracle.rest_state(1, T.n_nodes, T.bulk, np.ones(T.n_nodes))
+    mean = {}
+    for p in (lbamd.FP64, lbamd.FP16S):
+        ctx = lbamd.Context(lbamd.D3Q19, p)
+        ctx.set_topology_grid(T.neighbors, T.node_types, T.bulk, T.positions)
+        ctx.upload_f(f0)
+        ctx.step_bgk(steps, 1.0, np.array([force, 0.0, 0.0]))
+        mean[p] = ctx.download_vel()[T.bulk, 0].mean()
+        ctx.close()
+    err = abs(mean[lbamd.FP16S] - mean[lbamd.FP64]) / abs(mean[lbamd.FP64])
+    print("mean u_x error %.3e" % err)
+    assert err < 1e-4
