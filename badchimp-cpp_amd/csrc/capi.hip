@@ -16,6 +16,7 @@
 #include "kernels.hpp"
 #include "kernels_f32.hpp"
 #include "lattice.hpp"
+#include "layout_select.hpp"
 #include "rowbuild.hpp"
 
 using namespace lbamd;
@@ -344,8 +345,15 @@ int setup_topology(lb_ctx* c, int32_t n_nodes, const int32_t* neighbor, const in
         break;
       }
     }
-  // device allocation
+  // device allocation: the context holds no topology until every buffer
+  // below is in place, so a failure leaves no pointer to freed memory
   hipStreamSynchronize(c->stream);
+  c->has_topo = false;
+  c->have_macro = false;
+  c->have_visc = false;
+  c->addr = Addr{};
+  c->n_bulk = 0;
+  c->NP = 0;
   free_fields(c);
   free_tags(c);
   for (void* p : {static_cast<void*>(c->d_coord), static_cast<void*>(c->d_rows), static_cast<void*>(c->d_win)})
@@ -522,6 +530,7 @@ int lb_upload_f(lb_ctx* c, int32_t nf, const double* f) {
   c->post = false;
   c->step_count = 0;
   c->have_macro = false;
+  c->have_visc = false;  // the viscosity slot holds no rheology result now
   return LB_OK;
 }
 
@@ -600,6 +609,7 @@ int lb_init_equilibrium(lb_ctx* c, int32_t n_fields, int32_t field, int32_t rho_
   hipFree(dv);
   if (e != hipSuccess) return fail(c, LB_ERR_HIP, "init_equilibrium: %s", hipGetErrorString(e));
   c->have_macro = false;
+  c->have_visc = false;  // the viscosity slot holds no rheology result now
   return LB_OK;
 }
 
@@ -990,7 +1000,7 @@ int lb_step_cg(lb_ctx* c, int32_t nsteps, double tau0, double tau1, double sigma
   // row windows when every solid neighbour has one colour and both fields fit
   // the 32-bit byte offsets of that layout; else the link table
   const bool rowwin = c->addr.layout == 2 && c->wall_uniform &&
-                      2ull * c->Q * static_cast<uint64_t>(c->NP) * elem_size(c->precision) < (1ull << 32);
+                      rowwin_fits(c->Q, c->NP, 2, static_cast<int64_t>(elem_size(c->precision)));
   const WallPhi wp{c->d_phi_wall, c->wall_phi};
   Timer t(c);
   int64_t launches = 0;
@@ -1038,6 +1048,7 @@ int lb_step_cg(lb_ctx* c, int32_t nsteps, double tau0, double tau1, double sigma
   c->last_force[1] = a.F[1];
   c->last_force[2] = a.F[2];
   c->have_macro = true;
+  c->have_visc = false;  // the viscosity slot holds no rheology result now
   return LB_OK;
 }
 
@@ -1124,6 +1135,7 @@ int solve_permeability(lb_ctx* c, const char* name, double tau, double tau_anti,
   rc = t.stop(launches);
   if (rc != LB_OK) return rc;
   c->have_macro = true;
+  c->have_visc = false;  // the viscosity slot holds no rheology result now
   if (k_out) *k_out = k;
   if (steps_out) *steps_out = steps;
   return LB_OK;

@@ -14,6 +14,7 @@
 
 #include "kernels.hpp"
 #include "lattice.hpp"
+#include "layout_select.hpp"
 #include "table.hpp"
 
 namespace lbamd {
@@ -694,8 +695,10 @@ hipError_t flux_f64(int lattice, int nfields, bool first, const double* fin, con
   const dim3 g = grid_for(n > 0 ? n : 1);
   const int64_t nb = g.x;
   LB_DISPATCH_LATTICE(lattice, L, {
-    if (ad.layout == 2) launch_flux<L>(nfields, first, g, s, fin, ad.win, n, NP, dir, partial, nb);
-    else if (ad.layout == 1) launch_flux<L>(nfields, first, g, s, fin, ad.rows, n, NP, dir, partial, nb);
+    // two fields may not fit the row windows' 32-bit offsets
+    const int lay = launch_layout(ad.layout, L::nQ, NP, nfields, sizeof(double));
+    if (lay == 2) launch_flux<L>(nfields, first, g, s, fin, ad.win, n, NP, dir, partial, nb);
+    else if (lay == 1) launch_flux<L>(nfields, first, g, s, fin, ad.rows, n, NP, dir, partial, nb);
     else launch_flux<L>(nfields, first, g, s, fin, ad.table, n, NP, dir, partial, nb);
   });
   return hipGetLastError();

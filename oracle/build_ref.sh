@@ -25,6 +25,9 @@
 #   one_phase       src/std_one_phase/main.cpp (tagged-link boundaries and
 #                   per-domain mass sources, D3Q19) with its hard-coded
 #                   input directory switched to ./../
+#   flux_ref        oracle/flux_harness.cpp: the reference's
+#                   calcCapNumbForceCartDir and calcFluxForceCartDir
+#                   (LBglobalforcing.h:8-98) on fields read from a file
 #   std_case_gpu_d2q9, std_case_gpu_d3q19, twophase_gpu
 #                   tests/integration/{std_case,twophase}_gpu.cpp: the
 #                   reference's set-up classes driving this repository's
@@ -122,6 +125,12 @@ build std_case_trt_d3q19 "$OUT/std_case_trt_d3q19.cpp" "$REF/std_case"
 build rheology_d2q9 "$OUT/rheology_d2q9.cpp" "$REF/polymer_viscosity_tabulation"
 build rheology_d3q19 "$OUT/rheology_d3q19.cpp" "$REF/polymer_viscosity_tabulation"
 build one_phase "$OUT/one_phase.cpp" "$REF/std_one_phase"
+# the reference's global-forcing functions on given fields (our own harness
+# source, compiled against the reference's headers)
+if [ ! -x "$OUT/flux_ref" ] || [ "$HERE/flux_harness.cpp" -nt "$OUT/flux_ref" ]; then
+  g++ $CXXFLAGS -I"$REF/lbsolver" "$HERE/flux_harness.cpp" $LIBS -o "$OUT/flux_ref"
+  echo "build_ref: built flux_ref"
+fi
 
 # the drop-in demonstration (test infrastructure): reference set-up + shim
 REPO="$(cd "$HERE/.." && pwd)"

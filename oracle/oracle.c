@@ -32,7 +32,10 @@
  * Pinned by: the compiled reference's own outputs on the committed
  * geometries (tests/golden/ref_*.npz) -- bit-exact -- and by the closed
  * forms in tests/test_oracle.py (rest state, uniform-force growth, mass
- * conservation, Poiseuille start-up series).
+ * conservation, Poiseuille start-up series).  The flux sums are pinned
+ * bit-exact against the reference's own calcCapNumbForceCartDir and
+ * calcFluxForceCartDir run on two-phase fields by a harness compiled from
+ * its headers (tests/golden/forcing_ref.npz, oracle/flux_harness.cpp).
  */
 #include <math.h>
 #include <stdint.h>
@@ -687,7 +690,9 @@ int orc_twophase(int lattice, int32_t n_nodes, const int32_t *nb, const int16_t 
 
 /* ------------------------------------------------------------ flux sums */
 /* LBglobalforcing.h:23-26 (sum of f c over bulk, field sum for nF = 2) and
- * :61-78 (phi-weighted sums); sequential in bulk order.  out[3]. */
+ * :61-78 (phi-weighted sums of calcCapNumbForceCartDir, with f the total
+ * f0 + f1 and phi_k = rho_k / rho from the two fields); sequential in bulk
+ * order.  out[3]. */
 int orc_flux(int lattice, int32_t n_nodes, const int32_t *bulk, int32_t n_bulk, const double *f,
              int nF, int dir, double *out) {
   const lattice_t L = get_lattice(lattice);

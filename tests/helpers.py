@@ -70,3 +70,20 @@ def std_ref_names():
 def float_round(a):
     """Values as the reference reads them with `ifs >> float` (via the 18-digit text)."""
     return np.array(["%.18e" % v for v in np.ravel(a)], dtype=np.float32).astype(np.float64).reshape(np.shape(a))
+
+
+def tp_start(d, T):
+    """Rest state of a two-phase fixture: f_k = w rho_k on the bulk nodes, and
+    the phase densities with the wall wettability on solid-boundary nodes,
+    read as the reference reads attributes (float)."""
+    r0 = float_round(T.attribute(d["rho0_attr"]))
+    r1 = float_round(T.attribute(1.0 - d["rho0_attr"]))
+    w = float_round(T.attribute(d["wett_attr"]))
+    sb = T.node_types == 1
+    r0[sb] = w[sb]
+    r1[sb] = 1 - w[sb]
+    f0 = np.zeros((T.n_nodes, 2, 19))
+    W = oracle.WEIGHTS[1]
+    f0[T.bulk, 0] = W[None] * r0[T.bulk, None]
+    f0[T.bulk, 1] = W[None] * r1[T.bulk, None]
+    return f0, np.stack([r0, r1], 1)

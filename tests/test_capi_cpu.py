@@ -113,3 +113,36 @@ def test_batch_create_without_gpu_fails_loudly():
         lbamd.lib().lb_batch_destroy(h)
     else:
         assert rc != 0 and b"device" in lbamd.lib().lb_global_error()
+
+
+def test_two_field_launches_leave_the_row_windows_above_4gib(tmp_path):
+    """The row windows address one population buffer with 32-bit byte
+    offsets: a launch over both fields of a two-phase FP64 context whose
+    buffer exceeds 4 GiB (D3Q19, 14.1 M to 28.3 M nodes) must take the plain
+    rows, while a one-field launch of the same context keeps the windows."""
+    import subprocess
+    src = tmp_path / "sel.cpp"
+    src.write_text(r'''
+#include <cstdio>
+#include "layout_select.hpp"
+int main() {
+  using namespace lbamd;
+  const long long nps[] = {256, 14000128, 14128384, 20000000, 28246784, 28246528 + 256 * 2};
+  for (long long NP : nps)
+    std::printf("%lld %d %d %d %d %d\n", NP, launch_layout(2, 19, NP, 1, 8), launch_layout(2, 19, NP, 2, 8),
+                launch_layout(2, 19, NP, 2, 4), launch_layout(1, 19, NP, 2, 8), launch_layout(0, 19, NP, 2, 8));
+  return 0;
+}
+''')
+    exe = tmp_path / "sel"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "badchimp-cpp_amd", "csrc"), str(src),
+                    "-o", str(exe)], check=True)
+    rows = [list(map(int, l.split())) for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                             check=True).stdout.split("\n") if l]
+    for NP, one64, two64, two32, rows_layout, table in rows:
+        assert one64 == (2 if 19 * NP * 8 < 2**32 else 1)
+        assert two64 == (2 if 2 * 19 * NP * 8 < 2**32 else 1)
+        assert two32 == (2 if 2 * 19 * NP * 4 < 2**32 else 1)
+        assert rows_layout == 1 and table == 0
+    # the case the selection exists for: one FP64 field fits, two do not
+    assert any(r[1] == 2 and r[2] == 1 for r in rows)

@@ -7,12 +7,13 @@
    symmetry and the one-phase limit of the colour-gradient model.
 """
 import math
+import os
 
 import numpy as np
 import pytest
 
 import oracle
-from helpers import OracleTopo, float_round, load_ref, ref_names, std_ref_names
+from helpers import OracleTopo, float_round, load_ref, ref_names, std_ref_names, tp_start
 
 
 def ref_std_run(name):
@@ -243,20 +244,6 @@ def test_one_phase_mass_source_restores_domain_mass():
     assert 0 < deficit < 1e-3 * deficit0
 
 
-def tp_start(d, T):
-    r0 = float_round(T.attribute(d["rho0_attr"]))
-    r1 = float_round(T.attribute(1.0 - d["rho0_attr"]))
-    w = float_round(T.attribute(d["wett_attr"]))
-    sb = T.node_types == 1
-    r0[sb] = w[sb]
-    r1[sb] = 1 - w[sb]
-    f0 = np.zeros((T.n_nodes, 2, 19))
-    W = oracle.WEIGHTS[1]
-    f0[T.bulk, 0] = W[None] * r0[T.bulk, None]
-    f0[T.bulk, 1] = W[None] * r1[T.bulk, None]
-    return f0, np.stack([r0, r1], 1)
-
-
 @pytest.mark.parametrize("name", ref_names("tp_"))
 def test_twophase_bitexact_vs_compiled_reference(name):
     d = load_ref(name)
@@ -390,3 +377,74 @@ def test_flux_sums_match_numpy():
     assert np.isclose(s[0], fc.sum(), rtol=1e-13, atol=1e-18)
     assert np.isclose(s[1], (r0 / (r0 + r1) * fc).sum(), rtol=1e-13, atol=1e-18)
     assert np.isclose(s[2], (r1 / (r0 + r1) * fc).sum(), rtol=1e-13, atol=1e-18)
+
+
+def _forcing_fields():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        "make_flux_fixture", os.path.join(os.path.dirname(__file__), "golden", "make_flux_fixture.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def _seq(v):
+    """Left-fold sum in order (the reference's accumulation loops)."""
+    s = 0.0
+    for x in v:
+        s += float(x)
+    return s
+
+
+def test_flux_sums_pinned_to_reference_global_forcing():
+    """oracle.flux against the reference's own calcCapNumbForceCartDir and
+    calcFluxForceCartDir (LBglobalforcing.h:8-98), run by the compiled harness
+    on the populations of every two-phase fixture and on random fields
+    (forcing_ref.npz, tests/golden/make_flux_fixture.py).  With nu1 = 0 the
+    capillary-number force is -2 <phi0 fc> / <phi0>, with nu0 = 0 it is
+    -2 <phi1 fc> / <phi1>: each phase-weighted sum is pinned on its own, in
+    each direction.  The flux forces pin sum fc of the total and of each
+    field."""
+    mod = _forcing_fields()
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "forcing_ref.npz"))
+    seen = 0
+    for name, T, f in mod.fields():
+        assert str(ref[name + "/sha256"]) == mod.digest(f, T.bulk), "regenerated input differs for " + name
+        b = T.bulk
+        r0 = np.array([_seq(f[n, 0]) for n in b])
+        r1 = np.array([_seq(f[n, 1]) for n in b])
+        m0 = _seq(r0 / (r0 + r1))
+        m1 = _seq(r1 / (r0 + r1))
+        for (d, sca, nu0, nu1, ng), res in zip(ref[name + "/cases"], ref[name + "/result"]):
+            d, ng = int(d), int(ng)
+            s = oracle.flux(1, b, f, 2, d)
+            s0a = oracle.flux(1, b, f[:, 0].copy(), 1, d)[0]
+            s1a = oracle.flux(1, b, f[:, 1].copy(), 1, d)[0]
+            cap = 2 * (sca - ((s[1] / ng) * nu0 + (s[2] / ng) * nu1)) / ((m0 / ng) * nu0 + (m1 / ng) * nu1)
+            assert cap == res[0], (name, d, cap, res[0])
+            assert 2 * (sca - s[0] / ng) == res[1], (name, d)
+            assert 2 * (sca - s0a / ng) == res[2], (name, d)
+            assert 2 * (sca - s1a / ng) == res[3], (name, d)
+            seen += 1
+    assert seen == 60
+
+
+def test_flux_phase_sums_invariants():
+    """s1 + s2 = s0 to rounding; one phase alone carries all of it; swapping
+    the fields swaps the two phase sums exactly."""
+    d = load_ref("tp_walls_12x14x10")
+    p = d["params"]
+    T = OracleTopo(d["geo"].astype(np.int64), p["lattice"], p["periodic"])
+    f0, rho = tp_start(d, T)
+    f, *_ = oracle.run_twophase(1, T.neighbors, T.node_types, T.bulk, f0, rho, 1.0, 1.0, 0.01, 0.7, 0,
+                                [1e-5, 2e-6, 0], 40)
+    for direction in range(3):
+        s = oracle.flux(1, T.bulk, f, 2, direction)
+        assert abs(s[1] + s[2] - s[0]) <= 1e-13 * np.abs(f[T.bulk]).sum()
+        sw = oracle.flux(1, T.bulk, f[:, ::-1].copy(), 2, direction)
+        assert sw[0] == s[0] and sw[1] == s[2] and sw[2] == s[1]
+        one = f.copy()
+        one[:, 0] += one[:, 1]
+        one[:, 1] = 0.0
+        so = oracle.flux(1, T.bulk, one, 2, direction)
+        assert so[1] == so[0] and so[2] == 0.0
