@@ -575,10 +575,14 @@ int lb_init_equilibrium(lb_ctx* c, int32_t n_fields, int32_t field, int32_t rho_
     return fail(c, LB_ERR_ARG, "lb_init_equilibrium: bad argument");
   hipSetDevice(c->device);
   if (c->n_fields != n_fields || c->post) {
-    // (re)start from an unset state: all fields are to be initialised
+    // (re)start from an unset state: all fields are to be initialised; the
+    // fresh buffer starts at zero, so that a field not yet initialised holds
+    // finite values (the reduced-precision forms store two fields jointly)
     free_fields(c);
     int rc = alloc_fields(c, n_fields);
     if (rc != LB_OK) return rc;
+    LB_HIP(c, hipMemset(c->d_f[c->cur], 0,
+                        static_cast<size_t>(n_fields) * c->Q * c->NP * elem_size(c->precision)));
   }
   const int64_t NP = c->NP;
   std::vector<double> hr(NP, 0.0), hv(static_cast<size_t>(c->nD) * NP, 0.0);
@@ -588,9 +592,9 @@ int lb_init_equilibrium(lb_ctx* c, int32_t n_fields, int32_t field, int32_t rho_
     for (int d = 0; d < c->nD; ++d) hv[d * NP + i] = vel[static_cast<size_t>(n) * c->nD + d];
   }
   double *dr = nullptr, *dv = nullptr;
-  LB_HIP(c, hipMalloc(&dr, NP * sizeof(double)));
-  LB_HIP(c, hipMalloc(&dv, c->nD * NP * sizeof(double)));
-  hipError_t e = hipMemcpy(dr, hr.data(), NP * sizeof(double), hipMemcpyHostToDevice);
+  hipError_t e = hipMalloc(&dr, NP * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&dv, c->nD * NP * sizeof(double));
+  if (e == hipSuccess) e = hipMemcpy(dr, hr.data(), NP * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(dv, hv.data(), c->nD * NP * sizeof(double), hipMemcpyHostToDevice);
   if (c->precision == LB_FP64) {
     if (e == hipSuccess) e = init_eq_f64(c->lattice, static_cast<double*>(c->d_f[c->cur]), field, c->n_bulk, NP, dr, dv, c->stream);
@@ -605,8 +609,8 @@ int lb_init_equilibrium(lb_ctx* c, int32_t n_fields, int32_t field, int32_t rho_
     if (tmp) hipFree(tmp);
   }
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  hipFree(dr);
-  hipFree(dv);
+  if (dr) hipFree(dr);
+  if (dv) hipFree(dv);
   if (e != hipSuccess) return fail(c, LB_ERR_HIP, "init_equilibrium: %s", hipGetErrorString(e));
   c->have_macro = false;
   c->have_visc = false;  // the viscosity slot holds no rheology result now

@@ -54,6 +54,9 @@ def lib():
                                    C.c_double, C.c_int, _f64p, C.c_int64, C.c_int32, _f64p]
         L.orc_flux.restype = C.c_int
         L.orc_flux.argtypes = [C.c_int, C.c_int32, _i32p, C.c_int32, _f64p, C.c_int, C.c_int, _f64p]
+        L.orc_init_eq.restype = C.c_int
+        L.orc_init_eq.argtypes = [C.c_int, _i32p, C.c_int32, _f64p, C.c_int, C.c_int, _f64p, C.c_int, C.c_int,
+                                  _f64p, C.c_int, C.c_int]
         L.orc_node_types.restype = C.c_int
         L.orc_node_types.argtypes = [C.c_int, C.c_int32, _i32p, _i32p, _i32p, C.c_int32, _i16p]
         _lib = L
@@ -167,6 +170,20 @@ def flux(lattice, bulk, f, n_fields, direction):
     n_nodes = f.size // (n_fields * Q_OF[lattice])
     lib().orc_flux(lattice, n_nodes, bulk, bulk.size, f, n_fields, direction, out)
     return out
+
+
+def init_equilibrium(lattice, bulk, f, lb_field, rho, rho_field, vel, vel_field):
+    """initiateLbField (LBinitiatefield.h:33-57) in place on f (n_nodes,
+    n_lb, Q); rho (n_nodes, n_rho), vel (n_nodes, n_vel, nD)."""
+    assert f.dtype == np.float64 and f.flags.c_contiguous
+    rho = _c(rho, np.float64)
+    vel = _c(vel, np.float64)
+    n_nodes = f.shape[0]
+    bulk = _c(bulk, np.int32)
+    lib().orc_init_eq(lattice, bulk, bulk.size, f.reshape(-1), f.shape[1], int(lb_field), rho.reshape(-1),
+                      rho.size // n_nodes, int(rho_field), vel.reshape(-1), vel.size // (n_nodes * ND_OF[lattice]),
+                      int(vel_field))
+    return f
 
 
 def node_types(lattice, neighbor, nodetype_attr, ranks, my_rank):

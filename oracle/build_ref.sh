@@ -28,6 +28,8 @@
 #   flux_ref        oracle/flux_harness.cpp: the reference's
 #                   calcCapNumbForceCartDir and calcFluxForceCartDir
 #                   (LBglobalforcing.h:8-98) on fields read from a file
+#   init_ref        oracle/init_harness.cpp: the reference's initiateLbField
+#                   (LBinitiatefield.h:33-57) on fields read from a file
 #   std_case_gpu_d2q9, std_case_gpu_d3q19, twophase_gpu
 #                   tests/integration/{std_case,twophase}_gpu.cpp: the
 #                   reference's set-up classes driving this repository's
@@ -130,6 +132,10 @@ build one_phase "$OUT/one_phase.cpp" "$REF/std_one_phase"
 if [ ! -x "$OUT/flux_ref" ] || [ "$HERE/flux_harness.cpp" -nt "$OUT/flux_ref" ]; then
   g++ $CXXFLAGS -I"$REF/lbsolver" "$HERE/flux_harness.cpp" $LIBS -o "$OUT/flux_ref"
   echo "build_ref: built flux_ref"
+fi
+if [ ! -x "$OUT/init_ref" ] || [ "$HERE/init_harness.cpp" -nt "$OUT/init_ref" ]; then
+  g++ $CXXFLAGS -I"$REF/lbsolver" "$HERE/init_harness.cpp" $LIBS -o "$OUT/init_ref"
+  echo "build_ref: built init_ref"
 fi
 
 # the drop-in demonstration (test infrastructure): reference set-up + shim

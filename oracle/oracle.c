@@ -13,6 +13,7 @@
  *   D2Q9  tables and helpers   LBd2q9.h:14-40, 128-177
  *   D3Q19 tables and helpers   LBd3q19.h:14-40, 111-190
  *   calcRho / calcVel          LBmacroscopic.h:11-45
+ *   initiateLbField            LBinitiatefield.h:33-57
  *   calcOmegaBGK               LBcollision.h:27-46
  *   calcOmegaBGKTRT            LBcollision.h:50-75
  *   calcDeltaOmegaQ            LBcollision.h:79-98
@@ -685,6 +686,29 @@ int orc_twophase(int lattice, int32_t n_nodes, const int32_t *nb, const int16_t 
   if (a != f) memcpy(f, a, sizeof(double) * (size_t)n_nodes * nF * Q);
   if (force_out) for (int d = 0; d < D; ++d) force_out[d] = F[d];
   bb_free(&bb);
+  return 0;
+}
+
+/* ------------------------------------------------------------ equilibrium init */
+/* initiateLbField (LBinitiatefield.h:33-57, the overload with separate field
+ * numbers): f(lb_field, q, n) = w_q rho(rho_field, n) (1 + c2Inv cu_q +
+ * c4Inv0_5 (cu_q^2 - c2 uu)) on the bulk nodes, with cu = cDotAll(u) and
+ * uu = dot(u, u) of velocity field vel_field.  f: LbField with n_lb fields;
+ * rho: ScalarField with n_rho fields; vel: VectorField with n_vel fields. */
+int orc_init_eq(int lattice, const int32_t *bulk, int32_t n_bulk, double *f, int n_lb, int lb_field,
+                const double *rho, int n_rho, int rho_field, const double *vel, int n_vel, int vel_field) {
+  const lattice_t L = get_lattice(lattice);
+  const int Q = L.nQ, D = L.nD;
+  for (int32_t k = 0; k < n_bulk; ++k) {
+    const int64_t n = bulk[k];
+    const double *u = vel + (n * n_vel + vel_field) * D;
+    double cu[19];
+    cdotall(&L, u, cu);
+    const double uu = dot(&L, u, u);
+    const double r = rho[n * n_rho + rho_field];
+    double *fn = f + (n * n_lb + lb_field) * Q;
+    for (int q = 0; q < Q; ++q) fn[q] = L.w[q] * r * (1.0 + c2Inv * cu[q] + c4Inv0_5 * (cu[q] * cu[q] - c2 * uu));
+  }
   return 0;
 }
 

@@ -448,3 +448,54 @@ def test_flux_phase_sums_invariants():
         one[:, 1] = 0.0
         so = oracle.flux(1, T.bulk, one, 2, direction)
         assert so[1] == so[0] and so[2] == 0.0
+
+
+def _init_cases():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        "make_init_fixture", os.path.join(os.path.dirname(__file__), "golden", "make_init_fixture.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_init_equilibrium_matches_reference_initiate():
+    """oracle.init_equilibrium against the reference's own initiateLbField
+    (LBinitiatefield.h:33-57) run by the compiled harness (init_ref.npz),
+    bit for bit, on every case and field."""
+    mod = _init_cases()
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "init_ref.npz"))
+    for name, lattice, geo, periodic, n_lb, n_rho, calls, seed in mod.cases():
+        T, rho, vel = mod.inputs(name, lattice, geo, periodic, n_lb, n_rho, seed)
+        assert np.array_equal(rho, ref[name + "/rho"]) and np.array_equal(vel, ref[name + "/vel"])
+        Q = 9 if lattice == "D2Q9" else 19
+        f = np.zeros((T.n_nodes, n_lb, Q))
+        for lb, rf, vf in calls:
+            oracle.init_equilibrium(T.lattice, T.bulk, f, lb, rho, rf, vel, vf)
+        assert np.array_equal(f, ref[name + "/f"]), name
+
+
+@pytest.mark.parametrize("lattice", [0, 1])
+def test_init_equilibrium_moments(lattice):
+    """The equilibrium carries rho, rho u and the momentum flux
+    rho (c2 I + u u) (closed forms of the second-order equilibrium)."""
+    from oracle.vtklb_oracle import BASIS
+    c = BASIS["D2Q9" if lattice == 0 else "D3Q19"].astype(float)
+    D = c.shape[1]
+    n = 40
+    rng = np.random.default_rng(5)
+    rho = rng.uniform(0.5, 1.5, (n, 1))
+    vel = rng.normal(0.0, 0.05, (n, 1, D))
+    f = np.zeros((n, 1, c.shape[0]))
+    oracle.init_equilibrium(lattice, np.arange(n), f, 0, rho, 0, vel, 0)
+    F = f[:, 0]
+    u = vel[:, 0]
+    assert np.allclose(F.sum(1), rho[:, 0], rtol=0, atol=1e-15)
+    assert np.allclose(F @ c, rho * u, rtol=0, atol=1e-15)
+    Pi = np.einsum("nq,qi,qj->nij", F, c, c)
+    want = rho[:, :, None] * (np.eye(D)[None] / 3.0 + u[:, :, None] * u[:, None, :])
+    assert np.allclose(Pi, want, rtol=0, atol=1e-15)
+    # rest: f = w rho
+    f0 = np.zeros_like(f)
+    oracle.init_equilibrium(lattice, np.arange(n), f0, 0, rho, 0, np.zeros_like(vel), 0)
+    assert np.array_equal(f0[:, 0], rho * oracle.WEIGHTS[lattice][None, :])
