@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -22,6 +23,53 @@
 using namespace lbamd;
 
 namespace {
+
+// Staging memory of lb_download_f_noalloc: a fixed array in each device's
+// memory, so that the entry point allocates nothing.  Every device the
+// library runs on holds its copy until the process ends.  Host code never
+// uses the name `scratch` as an address (it stands for the host-side shadow
+// of the symbol): the device address comes from hipGetSymbolAddress, taken
+// once per device with that device current, at the first context creation
+// on the device.
+constexpr size_t kScratchBytes = size_t(128) << 20;
+__device__ __align__(256) static double scratch[kScratchBytes / sizeof(double)];
+
+constexpr int kMaxDevices = 64;
+struct ScratchSlot {
+  bool tried = false;
+  double* ptr = nullptr;
+  std::string why;
+};
+std::mutex g_scratch_mu;
+ScratchSlot g_scratch[kMaxDevices];
+
+// Resolve the device address of `scratch` on `device` (current).  A failure
+// is remembered and reported by the entry point; contexts work without it.
+void scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  ScratchSlot& sl = g_scratch[device];
+  if (sl.tried) return;
+  sl.tried = true;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    sl.why = std::string("hipGetSymbolAddress: ") + hipGetErrorString(e);
+    return;
+  }
+  sl.ptr = static_cast<double*>(p);
+}
+
+double* scratch_on(int device, std::string& why) {
+  if (device < 0 || device >= kMaxDevices) {
+    why = "device number beyond the scratch table";
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  const ScratchSlot& sl = g_scratch[device];
+  if (!sl.ptr) why = sl.tried ? sl.why : "scratch not set up on this device";
+  return sl.ptr;
+}
 
 int fail(lb_ctx* c, int code, const char* fmt, ...) {
   va_list ap;
@@ -408,6 +456,24 @@ int setup_topology(lb_ctx* c, int32_t n_nodes, const int32_t* neighbor, const in
   c->n_global = n_bulk;
   c->node_of.resize(n_bulk);
   for (int32_t i = 0; i < n_bulk; ++i) c->node_of[i] = bulk[order[i]];
+  {
+    // positions of the bulk nodes in ascending node order, and their runs
+    std::vector<int32_t> by_node(n_bulk), rank(NP, 0);
+    for (int32_t i = 0; i < n_bulk; ++i) by_node[i] = i;
+    std::sort(by_node.begin(), by_node.end(),
+              [&](int32_t a, int32_t b) { return c->node_of[a] < c->node_of[b]; });
+    c->bulk_runs.clear();
+    for (int32_t r = 0; r < n_bulk; ++r) {
+      const int32_t i = by_node[r], n = c->node_of[i];
+      rank[i] = r;
+      if (!c->bulk_runs.empty() && c->bulk_runs.back()[0] + c->bulk_runs.back()[2] == n)
+        ++c->bulk_runs.back()[2];
+      else
+        c->bulk_runs.push_back({n, r, 1});
+    }
+    if ((rc = dev_alloc(c, &c->d_rank, rank.size())) != LB_OK) return rc;
+    LB_HIP(c, hipMemcpy(c->d_rank, rank.data(), rank.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  }
   c->wall_nodes = walls;
   c->node_type.assign(node_type, node_type + n_nodes);
   c->has_topo = true;
@@ -446,6 +512,7 @@ int lb_ctx_create(int lattice, int precision, int device, lb_ctx** out) {
     delete c;
     return fail_global(LB_ERR_HIP, "stream/event creation failed");
   }
+  scratch_setup(device);
   *out = c;
   return LB_OK;
 }
@@ -459,7 +526,8 @@ int lb_ctx_destroy(lb_ctx* c) {
   for (void* p : {static_cast<void*>(c->d_link), static_cast<void*>(c->d_rho), static_cast<void*>(c->d_vel),
                   static_cast<void*>(c->d_phi), static_cast<void*>(c->d_phi_wall), static_cast<void*>(c->d_partial),
                   static_cast<void*>(c->d_scalar), static_cast<void*>(c->d_coord),
-                  static_cast<void*>(c->d_rows), static_cast<void*>(c->d_win), static_cast<void*>(c->d_table)})
+                  static_cast<void*>(c->d_rows), static_cast<void*>(c->d_win), static_cast<void*>(c->d_table),
+                  static_cast<void*>(c->d_rank)})
     if (p) hipFree(p);
   hipEventDestroy(c->ev0);
   hipEventDestroy(c->ev1);
@@ -563,6 +631,41 @@ int lb_download_f(lb_ctx* c, int32_t nf, double* f) {
     for (int k = 0; k < nf; ++k)
       for (int q = 0; q < Q; ++q) dst[k * Q + q] = h[static_cast<size_t>(k) * Q * NP + pidx_host(Q, q, i)];
   }
+  return LB_OK;
+}
+
+int lb_download_f_noalloc(lb_ctx* c, int32_t nf, double* f) {
+  if (!c) return LB_ERR_ARG;
+  if (!f) return fail(c, LB_ERR_ARG, "null output");
+  int rc = check_ready(c, nf);
+  if (rc != LB_OK) return rc;
+  const int Q = c->Q;
+  const size_t bytes = static_cast<size_t>(c->n_bulk) * nf * Q * sizeof(double);
+  if (bytes > kScratchBytes)
+    return fail(c, LB_ERR_UNSUPPORTED,
+                "lb_download_f_noalloc: %d bulk nodes x %d field(s) x %d directions need %zu bytes of staging, "
+                "above its fixed %zu MiB; use lb_download_f",
+                c->n_bulk, nf, Q, bytes, kScratchBytes >> 20);
+  std::string why;
+  double* stage = scratch_on(c->device, why);
+  if (!stage) return fail(c, LB_ERR_HIP, "lb_download_f_noalloc: no staging memory: %s", why.c_str());
+  hipSetDevice(c->device);
+  const void* src = nullptr;
+  if ((rc = pre_state(c, &src)) != LB_OK) return rc;
+  hipError_t e;
+  if (c->precision == LB_FP64)
+    e = unstage_f64(c->lattice, nf, static_cast<const double*>(src), stage, c->n_bulk, c->NP, c->d_rank,
+                    c->stream);
+  else
+    e = load_lowp(c->precision, c->lattice, nf, src, stage, c->n_bulk, c->NP, c->stream, c->d_rank);
+  const size_t rec = static_cast<size_t>(nf) * Q;
+  for (size_t k = 0; k < c->bulk_runs.size() && e == hipSuccess; ++k) {
+    const auto& r = c->bulk_runs[k];
+    e = hipMemcpyAsync(f + static_cast<size_t>(r[0]) * rec, stage + static_cast<size_t>(r[1]) * rec,
+                       static_cast<size_t>(r[2]) * rec * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return fail(c, LB_ERR_HIP, "lb_download_f_noalloc: %s", hipGetErrorString(e));
   return LB_OK;
 }
 

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <string>
 #include <utility>
@@ -23,6 +24,11 @@ struct lb_ctx {
   int64_t NP = 0;
   int64_t n_global = 0;
   std::vector<int32_t> node_of;     // compact index -> reference node number
+  // bulk nodes in ascending node order: rank[i] is the position of compact
+  // node i among them; runs of consecutive node numbers {first node, first
+  // position, length} (lb_download_f_noalloc copies each run in one piece)
+  int32_t* d_rank = nullptr;
+  std::vector<std::array<int32_t, 3>> bulk_runs;
   std::vector<int32_t> wall_nodes;  // wall slot -> node number
   std::vector<int16_t> node_type;
   uint32_t* d_link = nullptr;       // pull table in compact order (both layouts)

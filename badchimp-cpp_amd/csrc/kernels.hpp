@@ -43,6 +43,10 @@ hipError_t gather_f64(int lattice, int nfields, const double* fin, double* fout,
 hipError_t flux_f64(int lattice, int nfields, bool first, const double* fin, const Addr& ad,
                     int32_t n, int64_t NP, int dir, double* partial, hipStream_t s);
 // Equilibrium initialisation on bulk nodes (LBinitiatefield.h:50-56).
+// FP64 populations of nf fields (AoSoA) to the LbField layout over the bulk
+// nodes in ascending node order (rank[i]: position of compact node i).
+hipError_t unstage_f64(int lattice, int nf, const double* g, double* f, int32_t n, int64_t NP,
+                       const int32_t* rank, hipStream_t s);
 hipError_t init_eq_f64(int lattice, double* f, int field, int32_t n, int64_t NP,
                        const double* rho, const double* vel, hipStream_t s);
 

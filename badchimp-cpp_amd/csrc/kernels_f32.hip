@@ -371,22 +371,27 @@ __global__ __launch_bounds__(kBlock) void k_store(const double* __restrict__ f, 
   }
 }
 
+// rank == null: f in the same AoSoA layout as g.  Otherwise node i goes to
+// f[(rank[i] nf + k) Q + q], the LbField layout over the bulk nodes in
+// ascending node order (rank[i]: the position of node i among them).
 template <class L, typename S>
 __global__ __launch_bounds__(kBlock) void k_load(const S* __restrict__ g, double* __restrict__ f,
-                                                 int32_t n, int64_t NP, int nf) {
+                                                 int32_t n, int64_t NP, int nf, const int32_t* __restrict__ rank) {
   const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const int64_t fs = static_cast<int64_t>(L::nQ) * NP;
+  const int64_t r = rank ? static_cast<int64_t>(rank[i]) * nf * L::nQ : 0;
 #pragma unroll
   for (int q = 0; q < L::nQ; ++q) {
     const int64_t o = pidx<L::nQ>(q, i);
+    const int64_t o0 = rank ? r + q : o, o1 = rank ? r + L::nQ + q : fs + o;
     const double t = static_cast<double>(Store<S>::get(g[o])) + L::w(q);
     if (nf == 2) {
       const double d = static_cast<double>(Store<S>::get(g[fs + o]));
-      f[o] = 0.5 * (t + d);
-      f[fs + o] = 0.5 * (t - d);
+      f[o0] = 0.5 * (t + d);
+      f[o1] = 0.5 * (t - d);
     } else {
-      f[o] = t;
+      f[o0] = t;
     }
   }
 }
@@ -764,12 +769,12 @@ hipError_t store_lowp(int precision, int lattice, int nf, const double* f, void*
 }
 
 hipError_t load_lowp(int precision, int lattice, int nf, const void* g, double* f, int32_t n,
-                     int64_t NP, hipStream_t s) {
+                     int64_t NP, hipStream_t s, const int32_t* rank) {
   if (n <= 0) return hipSuccess;
   return dispatch(precision, lattice, Addr{}, [&](auto l, auto st, auto) {
     using L = decltype(l);
     using S = decltype(st);
-    k_load<L, S><<<grid_for(n), dim3(kBlock), 0, s>>>(static_cast<const S*>(g), f, n, NP, nf);
+    k_load<L, S><<<grid_for(n), dim3(kBlock), 0, s>>>(static_cast<const S*>(g), f, n, NP, nf, rank);
   });
 }
 

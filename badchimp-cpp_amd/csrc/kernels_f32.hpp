@@ -23,8 +23,10 @@ hipError_t prestream_lowp(int precision, int lattice, const RowWin& ad, const do
                           void* g, hipStream_t s);
 hipError_t store_lowp(int precision, int lattice, int nf, const double* f, void* g, int32_t n,
                       int64_t NP, hipStream_t s);
+// FP64 populations from the stored form: AoSoA like g, or with rank given,
+// the LbField layout over the bulk nodes in ascending node order.
 hipError_t load_lowp(int precision, int lattice, int nf, const void* g, double* f, int32_t n,
-                     int64_t NP, hipStream_t s);
+                     int64_t NP, hipStream_t s, const int32_t* rank = nullptr);
 hipError_t gather_lowp(int precision, int lattice, int nf, const void* gin, void* gout,
                        const Addr& ad, int32_t n, int64_t NP, hipStream_t s);
 hipError_t bgk_lowp(int precision, int lattice, bool first, bool store, bool reduce,

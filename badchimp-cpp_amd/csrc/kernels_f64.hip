@@ -544,6 +544,21 @@ __global__ __launch_bounds__(kBlock) void k_init_eq(double* __restrict__ f, int 
   }
 }
 
+// Populations of nf fields in the LbField layout over the bulk nodes in
+// ascending node order: node i goes to f[(rank[i] nf + k) Q + q].
+template <class L>
+__global__ __launch_bounds__(kBlock) void k_unstage(const double* __restrict__ g, double* __restrict__ f,
+                                                    int32_t n, int64_t NP, int nf,
+                                                    const int32_t* __restrict__ rank) {
+  constexpr int Q = L::nQ;
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int64_t r = static_cast<int64_t>(rank[i]) * nf * Q;
+  for (int k = 0; k < nf; ++k)
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[r + k * Q + q] = g[static_cast<int64_t>(k) * Q * NP + pidx<Q>(q, i)];
+}
+
 __global__ __launch_bounds__(kBlock) void k_sum_partials(const double* __restrict__ partial,
                                                          int64_t nblocks, int ncomp,
                                                          double* __restrict__ out) {
@@ -709,6 +724,15 @@ hipError_t init_eq_f64(int lattice, double* f, int field, int32_t n, int64_t NP,
   if (n <= 0) return hipSuccess;
   LB_DISPATCH_LATTICE(lattice, L, {
     k_init_eq<L><<<grid_for(n), dim3(kBlock), 0, s>>>(f, field, n, NP, rho, vel);
+  });
+  return hipGetLastError();
+}
+
+hipError_t unstage_f64(int lattice, int nf, const double* g, double* f, int32_t n, int64_t NP,
+                       const int32_t* rank, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  LB_DISPATCH_LATTICE(lattice, L, {
+    k_unstage<L><<<grid_for(n), dim3(kBlock), 0, s>>>(g, f, n, NP, nf, rank);
   });
   return hipGetLastError();
 }

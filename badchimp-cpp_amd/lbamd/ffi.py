@@ -54,6 +54,7 @@ SIGNATURES = {
     "lb_set_global_bulk_count": (C.c_int, [_vp, C.c_int64]),
     "lb_upload_f": (C.c_int, [_vp, C.c_int32, _f64p]),
     "lb_download_f": (C.c_int, [_vp, C.c_int32, _f64p]),
+    "lb_download_f_noalloc": (C.c_int, [_vp, C.c_int32, _f64p]),
     "lb_init_equilibrium": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _f64p, _f64p]),
     "lb_download_scalar": (C.c_int, [_vp, C.c_int32, C.c_int32, _f64p]),
     "lb_download_vector": (C.c_int, [_vp, C.c_int32, C.c_int32, _f64p]),
@@ -395,6 +396,14 @@ class Context:
         if out is None:
             out = np.zeros(self.n_nodes * nf * self.Q)
         self._check(lib().lb_download_f(self._h, nf, out))
+        return out.reshape(self.n_nodes, nf, self.Q)
+
+    def download_f_noalloc(self, nf=1, out=None):
+        """lb_download_f through the library's fixed staging memory (no device
+        allocation; at most 128 MiB of FP64 populations)."""
+        if out is None:
+            out = np.zeros(self.n_nodes * nf * self.Q)
+        self._check(lib().lb_download_f_noalloc(self._h, nf, out))
         return out.reshape(self.n_nodes, nf, self.Q)
 
     def init_equilibrium(self, n_fields, field, rho, vel, rho_field=0):

@@ -118,6 +118,18 @@ int lb_set_global_bulk_count(lb_ctx* ctx, int64_t n);
 int lb_upload_f(lb_ctx* ctx, int32_t n_fields, const double* f);
 int lb_download_f(lb_ctx* ctx, int32_t n_fields, double* f);
 
+/* lb_download_f that allocates no device memory: no counterpart in the
+ * reference.  It is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * populations are converted into a fixed 128 MiB staging array that the
+ * library keeps in each device's memory until the process ends, then copied
+ * out run by run of consecutive bulk node numbers.  Limit: n_bulk * n_fields
+ * * Q * 8 bytes <= 128 MiB (a larger context gets LB_ERR_UNSUPPORTED).
+ * Results equal lb_download_f's; entries of non-bulk nodes are left as they
+ * are, as there.  Every call on one device shares that device's array:
+ * calls that overlap on one GPU (from several host threads) race. */
+int lb_download_f_noalloc(lb_ctx* ctx, int32_t n_fields, double* f);
+
 /* Equilibrium initialisation f = w rho (1 + 3cu + 4.5(cu^2 - u^2/3)) of field
  * `field` (of n_fields population fields) from ScalarField rho (field
  * rho_field of rho_nfields) and VectorField vel (field 0), over bulk nodes
