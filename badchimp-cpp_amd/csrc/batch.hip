@@ -32,6 +32,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <string>
 #include <thread>
 #include <vector>
@@ -123,21 +124,18 @@ Footprint footprint_of(int Q, size_t esz, int64_t n, int64_t nwin, int nf) {
   fp.off_win = fp.off_coord + align_up(fp.NP * 4);
   fp.off_part = fp.off_win + align_up(nwin * 16);
   fp.off_phi = fp.off_part + align_up((nf == 2 ? 3 : 1) * fp.nblk * 8);
-  fp.bytes = fp.off_phi + (nf == 2 ? align_up(fp.NP * 4) : 0);
+  // colour field: double in FP64, float otherwise
+  fp.bytes = fp.off_phi + (nf == 2 ? align_up(fp.NP * (esz == 8 ? 8 : 4)) : 0);
   return fp;
 }
 
 struct Slot {
   bool used = false;
   int kind = 0;  // 0 single-phase, 1 two-phase
-  double nu0 = 0.0, nu1 = 0.0, k0 = 0.0, k1 = 0.0, k0_prev = 0.0, k1_prev = 0.0;
   int64_t tag = 0;
   int64_t off = 0;
   Footprint fp;
-  double tau = 1.0, F0 = 0.0, nu = 0.0, n_box = 1.0, tol = 0.0;
-  int64_t max_steps = 0, steps = 0;
-  double k = 0.0, k_prev = 0.0;
-  bool have_prev = false;
+  int64_t steps_seen = 0;  // steps of the solve at the last read-back of its stop state
   double t_admit = 0.0;
 };
 
@@ -151,7 +149,8 @@ struct Result {
 
 struct lb_batch {
   int lattice = 0, precision = 0, device = 0, Q = 0, nD = 0;
-  size_t esz = 0;
+  // precision of the solves of each kind (0 single-phase, 1 two-phase)
+  int prec[2] = {0, 0};
   int32_t check_every = 100;
   std::string err;
   hipStream_t stream = nullptr;
@@ -168,9 +167,19 @@ struct lb_batch {
   int64_t items_cap = 0;
   int32_t* d_slots = nullptr;
   double* d_out = nullptr;
-  double* h_out = nullptr;  // pinned
+  SolveCheck* d_chk = nullptr;  // stop state of each slot, advanced on the device
+  SolveCheck* h_chk = nullptr;  // pinned: read back at the end of each launch sequence
   uint32_t* h_step = nullptr;  // pinned: global step at the start of an interval
   uint32_t* d_step = nullptr;
+  // admissions: their uploads and rest-state kernels run on a stream of their
+  // own, so that they overlap a launch sequence in flight; the host buffers
+  // they read stay here until the next wait
+  hipStream_t up = nullptr;
+  hipEvent_t up_ev = nullptr;
+  bool up_pending = false;
+  std::vector<std::shared_ptr<void>> staged;
+  bool in_flight = false;
+  int32_t seq_intervals = 0;
   std::vector<uint32_t> items;  // single-phase items, then two-phase items
   int32_t n_items_bgk = 0, n_items_cg = 0;
   std::vector<int32_t> active;
@@ -231,22 +240,27 @@ hipError_t launch_step(lb_batch* b, bool reduce, int32_t shape, int32_t koff) {
   if (b->n_items_bgk > 0) {
     const int32_t n = b->n_items_bgk;
     shape_dims(b, shape, n, &grid, &per);
-    if (b->precision == LB_FP64)
+    if (b->prec[0] == LB_FP64)
       e = batch_bgk_f64(b->lattice, reduce, b->d_solves, b->d_items, n, grid, per, parity, b->stream);
     else
-      e = batch_bgk_lowp(b->precision, b->lattice, reduce, b->d_solves, b->d_items, n, grid, per, parity,
+      e = batch_bgk_lowp(b->prec[0], b->lattice, reduce, b->d_solves, b->d_items, n, grid, per, parity,
                          b->d_step, koff, b->stream);
   }
   if (e == hipSuccess && b->n_items_cg > 0) {
     const int32_t n = b->n_items_cg;
+    const uint32_t* it = b->d_items + b->n_items_bgk;
     shape_dims(b, shape, n, &grid, &per);
-    e = batch_cg_lowp(b->precision, b->lattice, reduce, b->d_solves, b->d_items + b->n_items_bgk, n, grid, per,
-                      parity, b->d_step, koff, b->stream);
+    if (b->prec[1] == LB_FP64)
+      e = batch_cg_f64(b->lattice, reduce, b->d_solves, it, n, grid, per, parity, b->stream);
+    else
+      e = batch_cg_lowp(b->prec[1], b->lattice, reduce, b->d_solves, it, n, grid, per, parity, b->d_step, koff,
+                        b->stream);
   }
   return e;
 }
 
-// rebuild the item and slot lists after admissions / retirements
+// rebuild the item and slot lists after admissions / retirements (at a
+// point where no launch sequence is in flight)
 int refresh(lb_batch* b) {
   b->items.clear();
   b->active.clear();
@@ -269,111 +283,191 @@ int refresh(lb_batch* b) {
     b->items_cap = std::max<int64_t>(static_cast<int64_t>(b->items.size()) * 2, 4096);
     LB_BHIP(b, hipMalloc(&b->d_items, b->items_cap * sizeof(uint32_t)));
   }
+  // the steps read what the admissions uploaded on their own stream
+  if (b->up_pending) LB_BHIP(b, hipStreamWaitEvent(b->stream, b->up_ev, 0));
   if (!b->items.empty())
     LB_BHIP(b, hipMemcpyAsync(b->d_items, b->items.data(), b->items.size() * sizeof(uint32_t),
                               hipMemcpyHostToDevice, b->stream));
   if (!b->active.empty())
     LB_BHIP(b, hipMemcpyAsync(b->d_slots, b->active.data(), b->active.size() * sizeof(int32_t),
                               hipMemcpyHostToDevice, b->stream));
+  // the global step at the start of the next interval, read by the kernels;
+  // the intervals advance it on the device from here
+  *b->h_step = static_cast<uint32_t>(b->global_step);
+  LB_BHIP(b, hipMemcpyAsync(b->d_step, b->h_step, sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
   LB_BHIP(b, hipStreamSynchronize(b->stream));
   drop_graphs(b);
   b->dirty = false;
   return LB_OK;
 }
 
-// one check interval of steps (graph when available), then the per-solve sums
-int run_interval(lb_batch* b, bool tune) {
+// The per-solve sums and stop tests that close an interval.
+hipError_t close_interval(lb_batch* b) {
+  const int32_t n = static_cast<int32_t>(b->active.size());
+  hipError_t e = batch_sum(b->d_solves, b->d_slots, n, b->d_out, b->stream);
+  if (e == hipSuccess) e = batch_check(b->d_solves, b->d_chk, b->d_slots, n, b->d_out, b->check_every, b->d_step,
+                                       b->stream);
+  return e;
+}
+
+// One check interval as individual launches, timing each candidate launch
+// shape on live steps (the shape never changes a result); keeps the fastest.
+int tune_interval(lb_batch* b) {
   const int M = b->check_every;
   const int32_t cand[] = {0, 1, 2, 4, 8};
   constexpr int kCand = 5;
-  const int start_parity = static_cast<int>(b->global_step & 1);
-  // the global step at the start of the interval, read by the kernels (the
-  // copy runs outside any captured graph)
-  *b->h_step = static_cast<uint32_t>(b->global_step);
-  LB_BHIP(b, hipMemcpyAsync(b->d_step, b->h_step, sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
-  if (tune) {
-    // time each candidate shape on live steps; results do not depend on the shape
-    std::vector<hipEvent_t> ev(2 * M);
-    for (auto& e : ev) LB_BHIP(b, hipEventCreate(&e));
-    for (int s = 0; s < M; ++s) {
-      hipEventRecord(ev[2 * s], b->stream);
-      hipError_t e = launch_step(b, s == M - 1, cand[s % kCand], s);
-      hipEventRecord(ev[2 * s + 1], b->stream);
-      if (e != hipSuccess) return bfail(b, LB_ERR_HIP, "batch step: %s", hipGetErrorString(e));
-      ++b->global_step;
-    }
-    LB_BHIP(b, hipStreamSynchronize(b->stream));
-    double t[kCand] = {0, 0, 0, 0, 0};
-    int cnt[kCand] = {0, 0, 0, 0, 0};
-    for (int s = 0; s < M; ++s) {
-      float ms = 0.f;
-      hipEventElapsedTime(&ms, ev[2 * s], ev[2 * s + 1]);
-      // the first launch of each shape warms up; skip it when there are others
-      if (s >= kCand || M <= kCand) {
-        t[s % kCand] += ms;
-        ++cnt[s % kCand];
-      }
-      b->kernel_ms += ms;
-    }
-    for (auto& e : ev) hipEventDestroy(e);
-    int best = 3;
-    double bt = 1e300;
-    for (int c = 0; c < kCand; ++c)
-      if (cnt[c] > 0 && t[c] / cnt[c] < bt) {
-        bt = t[c] / cnt[c];
-        best = c;
-      }
-    b->shape = cand[best];
-    b->tuned_nodes = b->resident_nodes;
-    hipEventRecord(b->ev0, b->stream);
-  } else {
-    hipEventRecord(b->ev0, b->stream);
-    hipGraphExec_t& g = b->graph[start_parity];
-    if (!g) {
-      hipGraph_t graph = nullptr;
-      LB_BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-      hipError_t e = hipSuccess;
-      const int64_t gs = b->global_step;
-      for (int s = 0; s < M && e == hipSuccess; ++s) {
-        e = launch_step(b, s == M - 1, b->shape, s);
-        ++b->global_step;
-      }
-      b->global_step = gs;
-      hipError_t e2 = hipStreamEndCapture(b->stream, &graph);
-      if (e != hipSuccess || e2 != hipSuccess) {
-        if (graph) hipGraphDestroy(graph);
-        return bfail(b, LB_ERR_HIP, "graph capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
-      }
-      e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
-      hipGraphDestroy(graph);
-      if (e != hipSuccess) {
-        g = nullptr;
-        return bfail(b, LB_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
-      }
-    }
-    LB_BHIP(b, hipGraphLaunch(g, b->stream));
-    b->global_step += M;
+  std::vector<hipEvent_t> ev(2 * M);
+  for (auto& e : ev) LB_BHIP(b, hipEventCreate(&e));
+  const int64_t gs = b->global_step;
+  for (int s = 0; s < M; ++s) {
+    hipEventRecord(ev[2 * s], b->stream);
+    hipError_t e = launch_step(b, s == M - 1, cand[s % kCand], s);
+    hipEventRecord(ev[2 * s + 1], b->stream);
+    if (e != hipSuccess) return bfail(b, LB_ERR_HIP, "batch step: %s", hipGetErrorString(e));
+    ++b->global_step;
   }
-  hipError_t e = batch_sum(b->d_solves, b->d_slots, static_cast<int32_t>(b->active.size()), b->d_out, b->stream);
-  if (e != hipSuccess) return bfail(b, LB_ERR_HIP, "batch sum: %s", hipGetErrorString(e));
-  hipEventRecord(b->ev1, b->stream);
-  LB_BHIP(b, hipMemcpyAsync(b->h_out, b->d_out, 3 * b->slots.size() * sizeof(double), hipMemcpyDeviceToHost,
-                            b->stream));
+  b->global_step = gs;
+  hipError_t e = close_interval(b);
+  if (e != hipSuccess) return bfail(b, LB_ERR_HIP, "batch check: %s", hipGetErrorString(e));
   LB_BHIP(b, hipStreamSynchronize(b->stream));
-  float ms = 0.f;
-  hipEventElapsedTime(&ms, b->ev0, b->ev1);
-  if (!tune) b->kernel_ms += ms;
-  b->launches += M + 1;
-  b->lups += b->resident_nodes * M;
+  double t[kCand] = {0, 0, 0, 0, 0};
+  int cnt[kCand] = {0, 0, 0, 0, 0};
+  for (int s = 0; s < M; ++s) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, ev[2 * s], ev[2 * s + 1]);
+    // the first launch of each shape warms up; skip it when there are others
+    if (s >= kCand || M <= kCand) {
+      t[s % kCand] += ms;
+      ++cnt[s % kCand];
+    }
+    b->kernel_ms += ms;
+  }
+  for (auto& e2 : ev) hipEventDestroy(e2);
+  int best = 3;
+  double bt = 1e300;
+  for (int c = 0; c < kCand; ++c)
+    if (cnt[c] > 0 && t[c] / cnt[c] < bt) {
+      bt = t[c] / cnt[c];
+      best = c;
+    }
+  b->shape = cand[best];
+  b->tuned_nodes = b->resident_nodes;
+  b->launches += M + 3;
   return LB_OK;
 }
 
-void retire(lb_batch* b, int32_t s, int32_t status) {
+// The graph of one interval starting at this parity: M steps, the sums, the
+// stop tests and the step counter, so that a sequence of intervals runs
+// without the host.
+int interval_graph(lb_batch* b, int parity, hipGraphExec_t* out) {
+  hipGraphExec_t& g = b->graph[parity];
+  if (!g) {
+    const int M = b->check_every;
+    hipGraph_t graph = nullptr;
+    LB_BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    const int64_t gs = b->global_step;
+    b->global_step = parity;  // launch_step reads only the parity of the step
+    for (int s = 0; s < M && e == hipSuccess; ++s) {
+      e = launch_step(b, s == M - 1, b->shape, s);
+      ++b->global_step;
+    }
+    b->global_step = gs;
+    if (e == hipSuccess) e = close_interval(b);
+    hipError_t e2 = hipStreamEndCapture(b->stream, &graph);
+    if (e != hipSuccess || e2 != hipSuccess) {
+      if (graph) hipGraphDestroy(graph);
+      return bfail(b, LB_ERR_HIP, "graph capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      g = nullptr;
+      return bfail(b, LB_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+    }
+  }
+  *out = g;
+  return LB_OK;
+}
+
+// Enqueue `intervals` check intervals of every resident solve and the read-back
+// of their stop states; the host does not wait.
+int launch_sequence(lb_batch* b, int32_t intervals) {
+  if (b->dirty) {
+    const int rc = refresh(b);
+    if (rc != LB_OK) return rc;
+  }
+  if (b->active.empty() || intervals <= 0) return LB_OK;
+  const int M = b->check_every;
+  int32_t k = 0;
+  const bool tune = b->shape_mode < 0 &&
+                    (b->tuned_nodes < 0 || std::llabs(b->resident_nodes - b->tuned_nodes) * 5 > b->tuned_nodes);
+  if (b->shape_mode >= 0) b->shape = b->shape_mode;
+  if (tune) {
+    const int rc = tune_interval(b);
+    if (rc != LB_OK) return rc;
+    b->global_step += M;
+    ++k;
+  }
+  hipEventRecord(b->ev0, b->stream);
+  for (; k < intervals; ++k) {
+    hipGraphExec_t g = nullptr;
+    const int rc = interval_graph(b, static_cast<int>(b->global_step & 1), &g);
+    if (rc != LB_OK) return rc;
+    LB_BHIP(b, hipGraphLaunch(g, b->stream));
+    b->global_step += M;
+    b->launches += M + 3;
+  }
+  hipEventRecord(b->ev1, b->stream);
+  LB_BHIP(b, hipMemcpyAsync(b->h_chk, b->d_chk, b->slots.size() * sizeof(SolveCheck), hipMemcpyDeviceToHost,
+                            b->stream));
+  b->in_flight = true;
+  b->seq_intervals = intervals - (tune ? 1 : 0);
+  return LB_OK;
+}
+
+void retire(lb_batch* b, int32_t s) {
   Slot& sl = b->slots[s];
-  b->done.push_back(Result{sl.tag, sl.steps, sl.k, sl.k0, sl.k1, now_s() - sl.t_admit, status});
+  const SolveCheck& c = b->h_chk[s];
+  b->done.push_back(Result{sl.tag, c.steps, c.k[0], c.k[1], c.k[2], now_s() - sl.t_admit, c.status});
   b->alloc.release(sl.off, sl.fp.bytes);
   sl.used = false;
   b->dirty = true;
+}
+
+// Wait for the sequence in flight and for the admissions' uploads, then
+// retire the solves whose stop test decided.
+int wait_sequence(lb_batch* b, int32_t* n_finished) {
+  int32_t fin = 0;
+  if (n_finished) *n_finished = 0;
+  hipError_t e = hipStreamSynchronize(b->stream);
+  if (e == hipSuccess && b->up_pending) e = hipEventSynchronize(b->up_ev);
+  if (b->up_pending) {
+    b->staged.clear();
+    b->up_pending = false;
+  }
+  const bool was = b->in_flight;
+  b->in_flight = false;
+  if (e != hipSuccess) return bfail(b, LB_ERR_HIP, "batch step: %s", hipGetErrorString(e));
+  if (!was) return LB_OK;
+  if (b->seq_intervals > 0) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, b->ev0, b->ev1);
+    b->kernel_ms += ms;
+  }
+  for (int32_t s : b->active) {
+    Slot& sl = b->slots[s];
+    if (!sl.used) continue;
+    const SolveCheck& c = b->h_chk[s];
+    b->lups += (c.steps - sl.steps_seen) * sl.fp.n;
+    sl.steps_seen = c.steps;
+    if (c.status >= 0) {
+      retire(b, s);
+      ++fin;
+    }
+  }
+  if (n_finished) *n_finished = fin;
+  return LB_OK;
 }
 
 }  // namespace
@@ -396,22 +490,25 @@ int lb_batch_create(int lattice, int precision, int device, int64_t pool_bytes, 
   if (!b) return fail_global(LB_ERR_NOMEM, "lb_batch_create: out of host memory");
   b->lattice = lattice;
   b->precision = precision;
+  b->prec[0] = b->prec[1] = precision;
   b->device = device;
   b->Q = lattice == LB_D2Q9 ? 9 : 19;
   b->nD = lattice == LB_D2Q9 ? 2 : 3;
-  b->esz = elem_bytes(precision);
   b->check_every = check_every;
   hipSetDevice(device);
   hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, device);
   if (b->n_cu <= 0) b->n_cu = 256;
   hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->up, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->up_ev, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreate(&b->ev0);
   if (e == hipSuccess) e = hipEventCreate(&b->ev1);
   if (e == hipSuccess) e = hipMalloc(&b->pool, static_cast<size_t>(pool_bytes));
   if (e == hipSuccess) e = hipMalloc(&b->d_solves, kMaxSlots * sizeof(BatchSolve));
   if (e == hipSuccess) e = hipMalloc(&b->d_slots, kMaxSlots * sizeof(int32_t));
   if (e == hipSuccess) e = hipMalloc(&b->d_out, 3 * kMaxSlots * sizeof(double));
-  if (e == hipSuccess) e = hipHostMalloc(&b->h_out, 3 * kMaxSlots * sizeof(double), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc(&b->d_chk, kMaxSlots * sizeof(SolveCheck));
+  if (e == hipSuccess) e = hipHostMalloc(&b->h_chk, kMaxSlots * sizeof(SolveCheck), hipHostMallocDefault);
   if (e == hipSuccess) e = hipHostMalloc(&b->h_step, sizeof(uint32_t), hipHostMallocDefault);
   if (e == hipSuccess) e = hipMalloc(&b->d_step, sizeof(uint32_t));
   if (e != hipSuccess) {
@@ -429,17 +526,22 @@ int lb_batch_destroy(lb_batch* b) {
   if (!b) return LB_OK;
   hipSetDevice(b->device);
   if (b->stream) hipStreamSynchronize(b->stream);
+  if (b->up) hipStreamSynchronize(b->up);
+  b->staged.clear();
   drop_graphs(b);
   if (b->pool) hipFree(b->pool);
   if (b->d_solves) hipFree(b->d_solves);
   if (b->d_items) hipFree(b->d_items);
   if (b->d_slots) hipFree(b->d_slots);
   if (b->d_out) hipFree(b->d_out);
-  if (b->h_out) hipHostFree(b->h_out);
+  if (b->d_chk) hipFree(b->d_chk);
+  if (b->h_chk) hipHostFree(b->h_chk);
   if (b->h_step) hipHostFree(b->h_step);
   if (b->d_step) hipFree(b->d_step);
   if (b->ev0) hipEventDestroy(b->ev0);
   if (b->ev1) hipEventDestroy(b->ev1);
+  if (b->up_ev) hipEventDestroy(b->up_ev);
+  if (b->up) hipStreamDestroy(b->up);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;
   return LB_OK;
@@ -515,16 +617,23 @@ void prepare(int lattice, const int32_t* dims, const int32_t* geo, int32_t perio
   }
 }
 
-// Admit a solve.  pre: its prepared rows (built here when null); sync:
-// wait for the uploads (else the caller synchronises b->stream before the
-// prepared host buffers go away).
+// Host data that an admission's asynchronous uploads read: kept until the
+// next wait.
+struct Staged {
+  std::shared_ptr<Prepared> pre;
+  std::vector<double> r0c;
+  SolveCheck chk;
+};
+
+// Admit a solve.  pre: its prepared rows (built here when null).  The
+// uploads and the rest-state kernel run on the admission stream; with sync
+// (and no launch sequence in flight) the call waits for them.
 int add_solve(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* geo, int32_t periodic_mask,
-              const AddArgs& p, const Prepared* pre = nullptr, bool sync = true) {
+              const AddArgs& p, std::shared_ptr<Prepared> pre = nullptr, bool sync = true) {
   hipSetDevice(b->device);
-  Prepared local;
   if (!pre) {
-    prepare(b->lattice, dims, geo, periodic_mask, local);
-    pre = &local;
+    pre = std::make_shared<Prepared>();
+    prepare(b->lattice, dims, geo, periodic_mask, *pre);
   }
   if (pre->code != LB_OK) return bfail(b, pre->code, "lb_batch_add: %s", pre->why.c_str());
   const RowBuild& rb = pre->rb;
@@ -532,8 +641,10 @@ int add_solve(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* geo, 
   const int32_t PY = pre->PY;
   const int64_t n = static_cast<int64_t>(rb.coord.size());
   const int nf = p.kind == 1 ? 2 : 1;
-  const Footprint fp = footprint_of(b->Q, b->esz, n, static_cast<int64_t>(win.size()), nf);
-  if (static_cast<uint64_t>(nf) * b->Q * static_cast<uint64_t>(fp.NP) * b->esz >= (1ull << 32) ||
+  const int prec = b->prec[p.kind];
+  const size_t esz = elem_bytes(prec);
+  const Footprint fp = footprint_of(b->Q, esz, n, static_cast<int64_t>(win.size()), nf);
+  if (static_cast<uint64_t>(nf) * b->Q * static_cast<uint64_t>(fp.NP) * esz >= (1ull << 32) ||
       fp.nblk > static_cast<int64_t>(kItemMask) + 1)
     return bfail(b, LB_ERR_UNSUPPORTED, "lb_batch_add: %lld fluid nodes exceed one solve", (long long)n);
   int32_t slot = -1;
@@ -542,16 +653,16 @@ int add_solve(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* geo, 
       slot = s;
       break;
     }
-  if (slot < 0) {
-    if (b->slots.size() >= static_cast<size_t>(kMaxSlots))
-      return bfail(b, LB_ERR_NOMEM, "lb_batch_add: all %d slots in use", kMaxSlots);
-    b->slots.emplace_back();
-    slot = static_cast<int32_t>(b->slots.size()) - 1;
-  }
+  if (slot < 0 && b->slots.size() >= static_cast<size_t>(kMaxSlots))
+    return bfail(b, LB_ERR_NOMEM, "lb_batch_add: all %d slots in use", kMaxSlots);
   const int64_t off = b->alloc.alloc(fp.bytes);
   if (off < 0)
     return bfail(b, LB_ERR_NOMEM, "lb_batch_add: %lld bytes do not fit (largest free range %lld of %lld free)",
                  (long long)fp.bytes, (long long)b->alloc.largest(), (long long)(b->alloc.total - b->alloc.used));
+  if (slot < 0) {
+    b->slots.emplace_back();
+    slot = static_cast<int32_t>(b->slots.size()) - 1;
+  }
   char* base = b->pool + off;
   BatchSolve d{};
   d.coord = reinterpret_cast<const uint32_t*>(base + fp.off_coord);
@@ -559,14 +670,17 @@ int add_solve(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* geo, 
   d.f[0] = base + fp.off_f0;
   d.f[1] = base + fp.off_f1;
   d.partial = reinterpret_cast<double*>(base + fp.off_part);
-  d.phi = nf == 2 ? reinterpret_cast<float*>(base + fp.off_phi) : nullptr;
+  d.phi = nf == 2 ? static_cast<void*>(base + fp.off_phi) : nullptr;
   d.NP = static_cast<uint32_t>(fp.NP);
   d.PY = PY;
   d.W = rb.g.W;
   d.n = static_cast<int32_t>(n);
+  // the solve starts at the global step of the next launch sequence (the
+  // host counter already covers a sequence in flight)
   d.parity = static_cast<int32_t>(b->global_step & 1);  // f[0] holds the state at the next step
   d.step_base = static_cast<uint32_t>(b->global_step);   // the solve's own steps start here
   d.kind = p.kind;
+  d.live = 1;
   const double tau = p.kind == 1 ? p.tau0 : p.tau;
   d.a.tau_inv = 1.0 / tau;
   d.a.tau_factor = 1 - 0.5 / tau;
@@ -581,29 +695,45 @@ int add_solve(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* geo, 
     const double r0 = p.wett, r1 = 1.0 - p.wett;
     d.phi_w = (r0 - r1) / (r0 + r1);
   }
+  auto st = std::make_shared<Staged>();
+  st->pre = pre;
+  // the stop state, as lb_solve_permeability / lb_solve_relperm set it up
+  SolveCheck& c = st->chk;
+  c = SolveCheck{};
+  c.nu[0] = (tau - 0.5) / 3.0;
+  c.nu[1] = (p.tau0 - 0.5) / 3.0;
+  c.nu[2] = (p.tau1 - 0.5) / 3.0;
+  int64_t nbox = 1;
+  for (int k = 0; k < nd; ++k) nbox *= dims[k];
+  c.n_box = static_cast<double>(nbox);
+  c.F0 = p.force[0];
+  c.tol = p.tol;
+  c.max_steps = p.max_steps;
+  c.status = -1;
+  c.kind = p.kind;
+  hipStream_t up = b->up;
   hipError_t e = hipMemcpyAsync(base + fp.off_coord, rb.coord.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice,
-                                b->stream);
+                                up);
   if (e == hipSuccess)
-    e = hipMemcpyAsync(base + fp.off_win, win.data(), win.size() * sizeof(uint4), hipMemcpyHostToDevice, b->stream);
-  double* tmp = nullptr;
-  std::vector<double> r0c;  // read by an async upload: lives until the synchronisation below
+    e = hipMemcpyAsync(base + fp.off_win, win.data(), win.size() * sizeof(uint4), hipMemcpyHostToDevice, up);
   if (e == hipSuccess) {
     if (p.kind == 0) {
       // rest state f = w: zero in the shifted forms; a pull from it is the
       // identity, so the first step needs no special read
-      if (b->precision == LB_FP64)
-        e = batch_rest_f64(b->lattice, reinterpret_cast<double*>(base + fp.off_f0), fp.NP, b->stream);
+      if (prec == LB_FP64)
+        e = batch_rest_f64(b->lattice, reinterpret_cast<double*>(base + fp.off_f0), fp.NP, up);
       else
-        e = hipMemsetAsync(base + fp.off_f0, 0, static_cast<size_t>(b->Q) * fp.NP * b->esz, b->stream);
+        e = hipMemsetAsync(base + fp.off_f0, 0, static_cast<size_t>(b->Q) * fp.NP * esz, up);
     } else {
       // f_k = w rho_k at rest, placed on the device where the first pull
       // reads it (k_prestream): the host only maps the phase-0 density of
       // each geometry cell to its compact node
+      std::vector<double>& r0c = st->r0c;
       r0c.resize(static_cast<size_t>(n));
       for (int64_t i = 0; i < n; ++i) {
-        const uint32_t c = rb.coord[i];
-        const int32_t X = static_cast<int32_t>(c >> 22), Y = static_cast<int32_t>((c >> 12) & 1023u),
-                      Z = static_cast<int32_t>(c & 4095u);
+        const uint32_t cw = rb.coord[i];
+        const int32_t X = static_cast<int32_t>(cw >> 22), Y = static_cast<int32_t>((cw >> 12) & 1023u),
+                      Z = static_cast<int32_t>(cw & 4095u);
         // geometry cell of the compact node (inverse of rows_from_geometry)
         int32_t P[3] = {X, Y, Z};
         for (int a = 0; a < 3; ++a) {
@@ -614,14 +744,17 @@ int add_solve(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* geo, 
                                         : static_cast<int64_t>(P[0]) * dims[1] + P[2];
         r0c[i] = p.rho0[cell];
       }
-      e = hipMalloc(&tmp, std::max<size_t>(1, r0c.size()) * sizeof(double));
-      if (e == hipSuccess)
-        e = hipMemcpyAsync(tmp, r0c.data(), r0c.size() * sizeof(double), hipMemcpyHostToDevice, b->stream);
+      // staged in the second population buffer (2 Q NP elements, at least
+      // 8 n bytes), which the first step overwrites
+      double* r0d = reinterpret_cast<double*>(base + fp.off_f1);
+      e = hipMemcpyAsync(r0d, r0c.data(), r0c.size() * sizeof(double), hipMemcpyHostToDevice, up);
       if (e == hipSuccess) {
         const RowWin ad{reinterpret_cast<const uint32_t*>(base + fp.off_coord),
                         reinterpret_cast<const uint4*>(base + fp.off_win), static_cast<uint32_t>(fp.NP), PY, rb.g.W};
-        e = prestream_lowp(b->precision, b->lattice, ad, tmp, static_cast<int32_t>(n), fp.NP, base + fp.off_f0,
-                           b->stream);
+        e = prec == LB_FP64 ? prestream_f64(b->lattice, ad, r0d, static_cast<int32_t>(n), fp.NP,
+                                            reinterpret_cast<double*>(base + fp.off_f0), up)
+                            : prestream_lowp(prec, b->lattice, ad, r0d, static_cast<int32_t>(n), fp.NP,
+                                             base + fp.off_f0, up);
       }
     }
   }
@@ -629,11 +762,14 @@ int add_solve(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* geo, 
   if (b->h_solves.empty()) b->h_solves.resize(kMaxSlots);
   b->h_solves[slot] = d;
   if (e == hipSuccess)
-    e = hipMemcpyAsync(b->d_solves + slot, &b->h_solves[slot], sizeof(BatchSolve), hipMemcpyHostToDevice,
-                       b->stream);
-  if (e == hipSuccess && (sync || tmp)) e = hipStreamSynchronize(b->stream);
-  if (tmp) hipFree(tmp);
+    e = hipMemcpyAsync(b->d_solves + slot, &b->h_solves[slot], sizeof(BatchSolve), hipMemcpyHostToDevice, up);
+  if (e == hipSuccess) e = hipMemcpyAsync(b->d_chk + slot, &st->chk, sizeof(SolveCheck), hipMemcpyHostToDevice, up);
+  if (e == hipSuccess) e = hipEventRecord(b->up_ev, up);
+  b->staged.push_back(st);
+  b->up_pending = true;
+  if (e == hipSuccess && sync && !b->in_flight) e = hipStreamSynchronize(up);
   if (e != hipSuccess) {
+    hipStreamSynchronize(up);
     b->alloc.release(off, fp.bytes);
     return bfail(b, LB_ERR_HIP, "lb_batch_add: %s", hipGetErrorString(e));
   }
@@ -644,16 +780,6 @@ int add_solve(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* geo, 
   sl.tag = p.tag;
   sl.off = off;
   sl.fp = fp;
-  sl.tau = tau;
-  sl.F0 = p.force[0];
-  sl.nu = (tau - 0.5) / 3.0;
-  sl.nu0 = (p.tau0 - 0.5) / 3.0;
-  sl.nu1 = (p.tau1 - 0.5) / 3.0;
-  int64_t nbox = 1;
-  for (int k = 0; k < nd; ++k) nbox *= dims[k];
-  sl.n_box = static_cast<double>(nbox);
-  sl.tol = p.tol;
-  sl.max_steps = p.max_steps;
   sl.t_admit = now_s();
   b->dirty = true;
   return LB_OK;
@@ -689,12 +815,13 @@ int lb_batch_add_many(lb_batch* b, int32_t count, int32_t nd, const int32_t* dim
   for (int32_t k = 0; k < count; ++k)
     if (!geos[k]) return bfail(b, LB_ERR_ARG, "lb_batch_add_many: geometry %d is null", k);
   // row layouts of all geometries on the host cores, then admission in order
-  std::vector<Prepared> pre(count);
+  std::vector<std::shared_ptr<Prepared>> pre(count);
+  for (auto& x : pre) x = std::make_shared<Prepared>();
   std::atomic<int32_t> next(0);
   const int nthr = std::max(1, std::min<int>(16, std::min<int>(count, static_cast<int>(std::thread::hardware_concurrency()))));
   auto work = [&]() {
     for (int32_t k; (k = next.fetch_add(1)) < count;)
-      prepare(b->lattice, dims + static_cast<size_t>(k) * nd, geos[k], periodic_mask, pre[k]);
+      prepare(b->lattice, dims + static_cast<size_t>(k) * nd, geos[k], periodic_mask, *pre[k]);
   };
   std::vector<std::thread> pool;
   for (int t = 1; t < nthr; ++t) pool.emplace_back(work);
@@ -710,12 +837,13 @@ int lb_batch_add_many(lb_batch* b, int32_t count, int32_t nd, const int32_t* dim
   int32_t added = 0;
   for (int32_t k = 0; k < count; ++k) {
     p.tag = tags[k];
-    rc = add_solve(b, nd, dims + static_cast<size_t>(k) * nd, geos[k], periodic_mask, p, &pre[k], false);
+    rc = add_solve(b, nd, dims + static_cast<size_t>(k) * nd, geos[k], periodic_mask, p, pre[k], false);
     if (rc != LB_OK) break;
     ++added;
   }
-  // the uploads read the prepared host buffers
-  const hipError_t e = hipStreamSynchronize(b->stream);
+  // report upload errors here unless a launch sequence is in flight (the
+  // staged host buffers live until the next wait either way)
+  const hipError_t e = b->in_flight ? hipSuccess : hipStreamSynchronize(b->up);
   if (n_added) *n_added = added;
   if (rc == LB_ERR_NOMEM) rc = LB_OK;  // the rest waits for retirements
   if (rc == LB_OK && e != hipSuccess) return bfail(b, LB_ERR_HIP, "lb_batch_add_many: %s", hipGetErrorString(e));
@@ -729,8 +857,6 @@ int lb_batch_add_twophase(lb_batch* b, int32_t nd, const int32_t* dims, const in
   if (nd != b->nD || !dims || !geo || !rho0 || !force || !(tau0 > 0.5) || !(tau1 > 0.5) || !(tol > 0.0) ||
       max_steps < 1 || force[0] == 0.0 || !(wettability >= 0.0 && wettability <= 1.0))
     return bfail(b, LB_ERR_ARG, "lb_batch_add_twophase: bad argument");
-  if (b->precision == LB_FP64)
-    return bfail(b, LB_ERR_UNSUPPORTED, "lb_batch_add_twophase: two-phase batches run in FP32S or FP16S");
   AddArgs p;
   p.kind = 1;
   p.tau0 = tau0;
@@ -748,60 +874,27 @@ int lb_batch_add_twophase(lb_batch* b, int32_t nd, const int32_t* dims, const in
 
 int lb_batch_advance(lb_batch* b, int32_t intervals, int32_t* n_finished) {
   if (!b) return LB_ERR_ARG;
+  if (n_finished) *n_finished = 0;
   if (intervals < 0) return bfail(b, LB_ERR_ARG, "lb_batch_advance: bad argument");
+  if (b->in_flight) return bfail(b, LB_ERR_STATE, "lb_batch_advance: a launch sequence is in flight (lb_batch_wait)");
   hipSetDevice(b->device);
-  int32_t fin = 0;
-  for (int32_t it = 0; it < intervals; ++it) {
-    if (b->dirty) {
-      const int rc = refresh(b);
-      if (rc != LB_OK) return rc;
-    }
-    if (b->active.empty()) break;
-    const bool tune = b->shape_mode < 0 &&
-                      (b->tuned_nodes < 0 || std::llabs(b->resident_nodes - b->tuned_nodes) * 5 > b->tuned_nodes);
-    if (b->shape_mode >= 0) b->shape = b->shape_mode;
-    int rc = run_interval(b, tune);
-    if (rc != LB_OK) return rc;
-    for (int32_t s : b->active) {
-      Slot& sl = b->slots[s];
-      sl.steps += b->check_every;
-      const double* o = b->h_out + 3 * s;
-      bool conv;
-      if (sl.kind == 0) {
-        // k = nu <j_x> / F_x with j_x summed over the state the last step started from
-        sl.k = sl.nu * (o[0] / sl.n_box) / sl.F0;
-        conv = sl.have_prev && std::fabs(sl.k - sl.k_prev) <= sl.tol * std::fabs(sl.k);
-      } else {
-        // phase permeabilities k_i = nu_i <phi_i sum_q f c_x> / F_x (LBglobalforcing.h:61-78)
-        sl.k0 = sl.nu0 * (o[1] / sl.n_box) / sl.F0;
-        sl.k1 = sl.nu1 * (o[2] / sl.n_box) / sl.F0;
-        sl.k = sl.k0 + sl.k1;
-        const double scale = std::fabs(sl.k0) + std::fabs(sl.k1);
-        conv = sl.have_prev && std::fabs(sl.k0 - sl.k0_prev) <= sl.tol * scale &&
-               std::fabs(sl.k1 - sl.k1_prev) <= sl.tol * scale;
-      }
-      if (!std::isfinite(sl.k)) {
-        retire(b, s, 2);
-        ++fin;
-        continue;
-      }
-      if (conv) {
-        retire(b, s, 0);
-        ++fin;
-        continue;
-      }
-      sl.k_prev = sl.k;
-      sl.k0_prev = sl.k0;
-      sl.k1_prev = sl.k1;
-      sl.have_prev = true;
-      if (sl.steps >= sl.max_steps) {
-        retire(b, s, 1);
-        ++fin;
-      }
-    }
-  }
-  if (n_finished) *n_finished = fin;
-  return LB_OK;
+  int rc = launch_sequence(b, intervals);
+  if (rc != LB_OK) return rc;
+  return wait_sequence(b, n_finished);
+}
+
+int lb_batch_launch(lb_batch* b, int32_t intervals) {
+  if (!b) return LB_ERR_ARG;
+  if (intervals < 0) return bfail(b, LB_ERR_ARG, "lb_batch_launch: bad argument");
+  if (b->in_flight) return bfail(b, LB_ERR_STATE, "lb_batch_launch: a launch sequence is in flight (lb_batch_wait)");
+  hipSetDevice(b->device);
+  return launch_sequence(b, intervals);
+}
+
+int lb_batch_wait(lb_batch* b, int32_t* n_finished) {
+  if (!b) return LB_ERR_ARG;
+  hipSetDevice(b->device);
+  return wait_sequence(b, n_finished);
 }
 
 int lb_batch_collect(lb_batch* b, int32_t max_n, int64_t* tags, double* k, double* k_phase, int64_t* steps,
@@ -844,8 +937,22 @@ int lb_batch_status(const lb_batch* b, int32_t* n_resident, int64_t* resident_no
   return LB_OK;
 }
 
+int lb_batch_set_precision(lb_batch* b, int32_t kind, int32_t precision) {
+  if (!b) return LB_ERR_ARG;
+  if (b->in_flight) return bfail(b, LB_ERR_STATE, "lb_batch_set_precision: a launch sequence is in flight");
+  if ((kind != 0 && kind != 1) || (precision != LB_FP64 && precision != LB_FP32S && precision != LB_FP16S))
+    return bfail(b, LB_ERR_ARG, "lb_batch_set_precision: bad argument");
+  for (const Slot& sl : b->slots)
+    if (sl.used && sl.kind == kind)
+      return bfail(b, LB_ERR_STATE, "lb_batch_set_precision: solves of this kind are resident");
+  b->prec[kind] = precision;
+  drop_graphs(b);
+  return LB_OK;
+}
+
 int lb_batch_set_shape(lb_batch* b, int32_t wg_per_cu) {
   if (!b) return LB_ERR_ARG;
+  if (b->in_flight) return bfail(b, LB_ERR_STATE, "lb_batch_set_shape: a launch sequence is in flight");
   if (wg_per_cu < -1 || wg_per_cu > 64) return bfail(b, LB_ERR_ARG, "lb_batch_set_shape: bad argument");
   b->shape_mode = wg_per_cu;
   if (wg_per_cu >= 0) b->shape = wg_per_cu;

@@ -1159,6 +1159,51 @@ int lb_step_cg(lb_ctx* c, int32_t nsteps, double tau0, double tau1, double sigma
   return LB_OK;
 }
 
+int lb_solve_relperm(lb_ctx* c, double tau0, double tau1, double sigma, double beta, const double* force,
+                     int32_t check_every, double tol, int64_t max_steps, double n_box, double* k_phase,
+                     int64_t* steps_out) {
+  if (!c) return LB_ERR_ARG;
+  int rc = check_ready(c, 2);
+  if (rc != LB_OK) return rc;
+  if (!force || !(tau0 > 0.5) || !(tau1 > 0.5) || check_every < 1 || !(tol > 0.0) || max_steps < 1 ||
+      !(n_box > 0.0) || force[0] == 0.0)
+    return fail(c, LB_ERR_ARG, "lb_solve_relperm: bad argument");
+  const double nu0 = (tau0 - 0.5) / 3.0, nu1 = (tau1 - 0.5) / 3.0;
+  double k[2] = {0.0, 0.0}, kp[2] = {0.0, 0.0};
+  bool have_prev = false;
+  int held = 0;
+  int64_t steps = 0;
+  while (steps < max_steps) {
+    const int64_t chunk = std::min<int64_t>(check_every, max_steps - steps);
+    // the phase fluxes of the state the chunk's last step starts from
+    if ((rc = lb_step_cg(c, static_cast<int32_t>(chunk - 1), tau0, tau1, sigma, beta, LB_FORCE_FIXED, force)) !=
+        LB_OK)
+      return rc;
+    double s1 = 0.0, s2 = 0.0;
+    if ((rc = lb_reduce_flux(c, 0, nullptr, &s1, &s2)) != LB_OK) return rc;
+    if ((rc = lb_step_cg(c, 1, tau0, tau1, sigma, beta, LB_FORCE_FIXED, force)) != LB_OK) return rc;
+    steps += chunk;
+    // phase permeabilities k_i = nu_i <phi_i sum_q f c_x> / F_x (LBglobalforcing.h:61-78)
+    k[0] = nu0 * (s1 / n_box) / force[0];
+    k[1] = nu1 * (s2 / n_box) / force[0];
+    if (!std::isfinite(k[0]) || !std::isfinite(k[1]))
+      return fail(c, LB_ERR_STATE, "phase permeability is not finite after %lld steps", (long long)steps);
+    const double scale = std::fabs(k[0]) + std::fabs(k[1]);
+    const bool ok = have_prev && std::fabs(k[0] - kp[0]) <= tol * scale && std::fabs(k[1] - kp[1]) <= tol * scale;
+    held = ok ? held + 1 : 0;
+    if (held >= kStopHold) break;
+    kp[0] = k[0];
+    kp[1] = k[1];
+    have_prev = true;
+  }
+  if (k_phase) {
+    k_phase[0] = k[0];
+    k_phase[1] = k[1];
+  }
+  if (steps_out) *steps_out = steps;
+  return LB_OK;
+}
+
 int lb_last_force(lb_ctx* c, double* force) {
   if (!c) return LB_ERR_ARG;
   if (!force) return fail(c, LB_ERR_ARG, "null output");
@@ -1206,6 +1251,7 @@ int solve_permeability(lb_ctx* c, const char* name, double tau, double tau_anti,
   double k_prev = 0.0, k = 0.0;
   int64_t steps = 0;
   bool have_prev = false;
+  int held = 0;
   Timer t(c);
   int64_t launches = 0;
   while (steps < max_steps) {
@@ -1235,7 +1281,8 @@ int solve_permeability(lb_ctx* c, const char* name, double tau, double tau_anti,
     // j_x summed over the state the last step started from; k = nu <j_x> / F_x
     k = nu * (jx / n_box) / a.F[0];
     if (!std::isfinite(k)) return fail(c, LB_ERR_STATE, "permeability is not finite after %lld steps", (long long)steps);
-    if (have_prev && std::fabs(k - k_prev) <= tol * std::fabs(k)) break;
+    held = have_prev && std::fabs(k - k_prev) <= tol * std::fabs(k) ? held + 1 : 0;
+    if (held >= kStopHold) break;
     k_prev = k;
     have_prev = true;
   }

@@ -28,6 +28,11 @@ constexpr uint32_t kLinkBounce = 1u;
 constexpr uint32_t kLinkGhost0 = 2u;
 constexpr uint32_t kLinkIndexMask = (1u << 30) - 1u;
 constexpr int kBlock = 256;
+// Steady-state stop of the permeability solves: the relative change of k
+// between two checks is at most the tolerance at this many consecutive
+// checks.  One check alone can pass by chance while k still drifts (the
+// stochastically rounded FP16 storage adds noise of about the tolerance).
+constexpr int kStopHold = 2;
 
 // Population element (direction q, node i) of one field: blocks of kPB
 // consecutive nodes keep their Q directions one after another (AoSoA), so a
@@ -79,7 +84,7 @@ struct BatchSolve {
   const uint4* win;
   void* f[2];
   double* partial;  // [ncomp][blocks]: 1 component single-phase, 3 two-phase
-  float* phi;       // two-phase colour field (NP), null for single-phase
+  void* phi;        // two-phase colour field (NP; float, double in FP64), null for single-phase
   uint32_t NP;
   int32_t PY, W, n;
   int32_t parity;
@@ -87,6 +92,21 @@ struct BatchSolve {
   BgkArgs a;        // tau_inv, tau_factor (single-phase) and the body force
   double nu0Inv, nu1Inv, sigma, beta, phi_w;  // two-phase parameters, wall colour
   uint32_t step_base;  // global batch step at admission: the solve's own step count is global - step_base
+  int32_t live;        // 1 while the solve advances; 0 once its stop test decided (set on the device)
+};
+
+// Stop test of one batched solve, kept and evaluated on the device after each
+// check interval: the relative change of k (single-phase, k[0]) or of both
+// phase permeabilities k[1], k[2] (two-phase, against |k[1]| + |k[2]|) is at
+// most tol at kStopHold consecutive checks (status 0), or the step limit is
+// reached (status 1), or k is not finite (status 2); -1 while running.
+struct SolveCheck {
+  double k[3];   // at the last check: k (k0 + k1 for two-phase), k0, k1
+  double kp[3];  // at the check before
+  double nu[3];  // nu, nu0, nu1
+  double n_box, F0, tol;
+  int64_t steps, max_steps;
+  int32_t held, have_prev, status, kind;
 };
 // Work item of a batched launch: slot << kItemShift | 256-node block of the solve.
 constexpr int kItemShift = 18;

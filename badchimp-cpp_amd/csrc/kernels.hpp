@@ -68,9 +68,22 @@ hipError_t tag_mass_f64(int lattice, bool first, const double* fin, const TableA
 // each solve's block partials of j_x.
 hipError_t batch_bgk_f64(int lattice, bool reduce, const BatchSolve* solves, const uint32_t* items,
                          int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, hipStream_t s);
+// Two-phase rest state (phase-0 density r0 per compact node) placed where the
+// first pull through ad reads it (FP64 form).
+hipError_t prestream_f64(int lattice, const RowWin& ad, const double* r0, int32_t n, int64_t NP, double* f,
+                         hipStream_t s);
+// Both passes of one FP64 colour-gradient step of every two-phase solve listed
+// in items (as batch_bgk_f64); reduce writes each solve's three flux partials.
+hipError_t batch_cg_f64(int lattice, bool reduce, const BatchSolve* solves, const uint32_t* items,
+                        int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, hipStream_t s);
 // out[slot] = sum of the solve's block partials, for each slot listed.
 hipError_t batch_sum(const BatchSolve* solves, const int32_t* slots, int32_t n_slots, double* out,
                      hipStream_t s);
+// Stop tests of the listed solves after an interval of M steps (from the sums
+// in out), marking decided solves not live; then advances the step counter
+// *step by M.
+hipError_t batch_check(BatchSolve* solves, SolveCheck* chk, const int32_t* slots, int32_t n_slots, const double* out,
+                       int32_t M, uint32_t* step, hipStream_t s);
 // Rest state (f = w) of one FP64 buffer of Q * NP elements.
 hipError_t batch_rest_f64(int lattice, double* f, int64_t NP, hipStream_t s);
 

@@ -304,6 +304,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_bgk_lp(const BatchSolve* __res
   for (int32_t it = beg; it < end; ++it) {
     const uint32_t item = items[it];
     const BatchSolve& s = solves[item >> kItemShift];
+    if (!s.live) continue;  // stopped at an earlier check of this launch sequence
     const int32_t lb = static_cast<int32_t>(item & kItemMask);
     const int p = (parity ^ s.parity) & 1;
     const RowWin ad{s.coord, s.win, s.NP, s.PY, s.W};
@@ -604,6 +605,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_cg_phi_lp(const BatchSolve* __
   for (int32_t it = beg; it < end; ++it) {
     const uint32_t item = items[it];
     const BatchSolve& s = solves[item >> kItemShift];
+    if (!s.live) continue;  // stopped at an earlier check of this launch sequence
     const int32_t lb = static_cast<int32_t>(item & kItemMask);
     const int p = (parity ^ s.parity) & 1;
     const RowWin ad{s.coord, s.win, s.NP, s.PY, s.W};
@@ -612,7 +614,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_cg_phi_lp(const BatchSolve* __
     if (i < s.n) {
       float t[Q], d[Q];
       load_td<L, S, false>(static_cast<const S*>(s.f[p]), Q * s.NP, ad, i, t, d);
-      s.phi[i] = phi_td<L>(t, d);
+      static_cast<float*>(s.phi)[i] = phi_td<L>(t, d);
       if (REDUCE) flux_td<L>(t, d, 0, s0, s1, s2);
     }
     if (REDUCE) {
@@ -642,6 +644,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_cg_lp(const BatchSolve* __rest
   for (int32_t it = beg; it < end; ++it) {
     const uint32_t item = items[it];
     const BatchSolve& s = solves[item >> kItemShift];
+    if (!s.live) continue;  // stopped at an earlier check of this launch sequence
     const int32_t lb = static_cast<int32_t>(item & kItemMask);
     const int p = (parity ^ s.parity) & 1;
     const RowWin ad{s.coord, s.win, s.NP, s.PY, s.W};
@@ -657,7 +660,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_cg_lp(const BatchSolve* __rest
     for (int qq = 0; qq < Q - 1; ++qq) {
       const int q = L::rev(qq);
       const bool f = (fluid >> q) & 1u;
-      const float v = s.phi[f ? j[q] : static_cast<uint32_t>(i)];
+      const float v = static_cast<const float*>(s.phi)[f ? j[q] : static_cast<uint32_t>(i)];
       pn[qq] = f ? v : pw;
     }
     const float F[3] = {static_cast<float>(s.a.F[0]), static_cast<float>(s.a.F[1]), static_cast<float>(s.a.F[2])};

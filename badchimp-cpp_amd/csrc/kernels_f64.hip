@@ -224,6 +224,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_bgk(const BatchSolve* __restri
   for (int32_t it = beg; it < end; ++it) {
     const uint32_t item = items[it];
     const BatchSolve& s = solves[item >> kItemShift];
+    if (!s.live) continue;  // stopped at an earlier check of this launch sequence
     const int32_t lb = static_cast<int32_t>(item & kItemMask);
     const int p = (parity ^ s.parity) & 1;
     const RowWin ad{s.coord, s.win, s.NP, s.PY, s.W};
@@ -247,6 +248,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_sum(const BatchSolve* __restri
                                                       double* __restrict__ out) {
   const int32_t slot = slots[blockIdx.x];
   const BatchSolve& s = solves[slot];
+  if (!s.live) return;
   const int64_t nb = s.n > 0 ? (s.n + kBlock - 1) / kBlock : 1;
   const int ncomp = s.kind == 1 ? 3 : 1;
   for (int c = 0; c < ncomp; ++c) {
@@ -254,6 +256,75 @@ __global__ __launch_bounds__(kBlock) void k_batch_sum(const BatchSolve* __restri
     for (int64_t b = threadIdx.x; b < nb; b += kBlock) v += s.partial[c * nb + b];
     v = block_sum(v);
     if (threadIdx.x == 0) out[3 * slot + c] = v;
+  }
+}
+
+// The stop test of every listed solve after a check interval of M steps
+// (one thread per solve): the arithmetic of lb_solve_permeability and
+// lb_solve_relperm, so a solve stops at the same step in a batch and alone.
+// A decided solve is marked not live; the step kernels then skip it.
+__global__ __launch_bounds__(kBlock) void k_batch_check(BatchSolve* __restrict__ solves,
+                                                        SolveCheck* __restrict__ chk,
+                                                        const int32_t* __restrict__ slots, int32_t n_slots,
+                                                        const double* __restrict__ out, int32_t M) {
+  const int32_t t = static_cast<int32_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (t >= n_slots) return;
+  const int32_t slot = slots[t];
+  SolveCheck& c = chk[slot];
+  if (c.status >= 0) return;
+  c.steps += M;
+  const double* o = out + 3 * slot;
+  bool conv;
+  if (c.kind == 0) {
+    // k = nu <j_x> / F_x with j_x summed over the state the last step started from
+    c.k[0] = c.nu[0] * (o[0] / c.n_box) / c.F0;
+    conv = c.have_prev && fabs(c.k[0] - c.kp[0]) <= c.tol * fabs(c.k[0]);
+  } else {
+    // phase permeabilities k_i = nu_i <phi_i sum_q f c_x> / F_x (LBglobalforcing.h:61-78)
+    c.k[1] = c.nu[1] * (o[1] / c.n_box) / c.F0;
+    c.k[2] = c.nu[2] * (o[2] / c.n_box) / c.F0;
+    c.k[0] = c.k[1] + c.k[2];
+    const double scale = fabs(c.k[1]) + fabs(c.k[2]);
+    conv = c.have_prev && fabs(c.k[1] - c.kp[1]) <= c.tol * scale && fabs(c.k[2] - c.kp[2]) <= c.tol * scale;
+  }
+  if (!isfinite(c.k[0])) {
+    c.status = 2;
+  } else {
+    c.held = conv ? c.held + 1 : 0;
+    if (c.held >= kStopHold) {
+      c.status = 0;
+    } else {
+      for (int j = 0; j < 3; ++j) c.kp[j] = c.k[j];
+      c.have_prev = 1;
+      if (c.steps >= c.max_steps) c.status = 1;
+    }
+  }
+  if (c.status >= 0) solves[slot].live = 0;
+}
+
+// The global step at the start of the next interval (read by the FP16
+// stochastic rounding): one thread, inside the captured interval.
+__global__ void k_batch_next_step(uint32_t* __restrict__ step, int32_t M) { *step += static_cast<uint32_t>(M); }
+
+// Rest state f_k = w rho_k of a two-phase solve (rho_0 = r0 per compact
+// node, rho_1 = 1 - r0) written where the first pull through ad reads it: the
+// value of direction q of node i goes to the element its gather of q reads
+// (the pull is a permutation of the elements).
+template <class L>
+__global__ __launch_bounds__(kBlock) void k_prestream(RowWin ad, const double* __restrict__ r0, int32_t n,
+                                                      int64_t NP, double* __restrict__ f) {
+  constexpr int Q = L::nQ;
+  const int32_t i = static_cast<int32_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  uint32_t src[Q - 1];
+  ad.template sources<L>(i, src);
+  const double a = r0[i], b = 1.0 - a;
+  const int64_t fs = static_cast<int64_t>(Q) * NP;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int64_t o = q < Q - 1 ? static_cast<int64_t>(src[q]) : static_cast<int64_t>(pidx<Q>(Q - 1, i));
+    f[o] = L::w(q) * a;
+    f[fs + o] = L::w(q) * b;
   }
 }
 
@@ -385,24 +456,16 @@ __global__ __launch_bounds__(kBlock) void k_cg_phi(const double* __restrict__ fi
 }
 
 // ------------------------------------------------------------------ two-phase pass 3
+// One node of the colour-gradient collision (main_TWOPHASE.cpp:318-392) with
+// the body force F; phi: the colour field of this step (pass 1).
 template <class L, bool FIRST, bool STORE, class A>
-__global__ __launch_bounds__(kBlock) void k_cg(const double* __restrict__ fin,
-                                               double* __restrict__ fout, A ad, int32_t n,
-                                               int64_t NP, CgArgs a,
-                                               const double* __restrict__ phi, WallPhi wp,
-                                               const double* __restrict__ dscalar,
-                                               double* __restrict__ rho_o,
-                                               double* __restrict__ vel_o) {
+__device__ __forceinline__ void cg_node(const double* __restrict__ fin, double* __restrict__ fout, const A& ad,
+                                        int32_t i, int64_t NP, const CgArgs& a, const double* __restrict__ phi,
+                                        const WallPhi& wp, const double (&F)[L::nD], double* __restrict__ rho_o,
+                                        double* __restrict__ vel_o) {
   constexpr int Q = L::nQ, D = L::nD;
-  const int32_t b = node_block(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
-  const int32_t i = b * kBlock + threadIdx.x;
-  if (i >= n) return;
   double f[2][Q];
   load_node<L, 2, FIRST>(fin, ad, i, NP, f);
-  double F[D];
-#pragma unroll
-  for (int d = 0; d < D; ++d) F[d] = a.F[d];
-  if (a.momx_mode) F[0] = dscalar[0];
   double r0 = qsum<L>(f[0]);
   double r1 = qsum<L>(f[1]);
   double ft[Q];
@@ -469,6 +532,108 @@ __global__ __launch_bounds__(kBlock) void k_cg(const double* __restrict__ fin,
     const double base = ((ft[q] + om) + dF) + st;
     __builtin_nontemporal_store(((c0 * base) + rc) + dQ0, out0 + (q << kPLog));
     __builtin_nontemporal_store(((c1 * base) - rc) + dQ1, out1 + (q << kPLog));
+  }
+}
+
+template <class L, bool FIRST, bool STORE, class A>
+__global__ __launch_bounds__(kBlock) void k_cg(const double* __restrict__ fin,
+                                               double* __restrict__ fout, A ad, int32_t n,
+                                               int64_t NP, CgArgs a,
+                                               const double* __restrict__ phi, WallPhi wp,
+                                               const double* __restrict__ dscalar,
+                                               double* __restrict__ rho_o,
+                                               double* __restrict__ vel_o) {
+  constexpr int D = L::nD;
+  const int32_t b = node_block(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
+  const int32_t i = b * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double F[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) F[d] = a.F[d];
+  if (a.momx_mode) F[0] = dscalar[0];
+  cg_node<L, FIRST, STORE>(fin, fout, ad, i, NP, a, phi, wp, F, rho_o, vel_o);
+}
+
+// Batched colour-gradient step in FP64 (items as k_batch_bgk; fixed force;
+// every solid neighbour carries the solve's wall colour phi_w).  Pass 1
+// writes the colour field and, on the interval's last step, the three flux
+// partials per block with k_flux's arithmetic; pass 2 is cg_node.
+template <class L, bool REDUCE>
+__global__ __launch_bounds__(kBlock) void k_batch_cg_phi(const BatchSolve* __restrict__ solves,
+                                                         const uint32_t* __restrict__ items, int32_t n_items,
+                                                         int32_t per_wg, int32_t parity) {
+  constexpr int Q = L::nQ;
+  const int32_t g = node_block(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
+  const int32_t beg = g * per_wg;
+  const int32_t end = beg + per_wg < n_items ? beg + per_wg : n_items;
+  for (int32_t it = beg; it < end; ++it) {
+    const uint32_t item = items[it];
+    const BatchSolve& s = solves[item >> kItemShift];
+    if (!s.live) continue;  // stopped at an earlier check of this launch sequence
+    const int32_t lb = static_cast<int32_t>(item & kItemMask);
+    const int p = (parity ^ s.parity) & 1;
+    const RowWin ad{s.coord, s.win, s.NP, s.PY, s.W};
+    const int32_t i = lb * kBlock + static_cast<int32_t>(threadIdx.x);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    if (i < s.n) {
+      double f[2][Q];
+      load_node<L, 2, false>(static_cast<const double*>(s.f[p]), ad, i, s.NP, f);
+      const double r0 = qsum<L>(f[0]);
+      const double r1 = qsum<L>(f[1]);
+      static_cast<double*>(s.phi)[i] = (r0 - r1) / (r0 + r1);
+      if (REDUCE) {
+        double ft[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) ft[q] = f[0][q] + f[1][q];
+        const double fc = qsumc<L>(ft, 0);
+        const double rt = r0 + r1;
+        s0 = fc;
+        s1 = (r0 / rt) * fc;
+        s2 = (r1 / rt) * fc;
+      }
+    }
+    if (REDUCE) {
+      const int64_t nb = (s.n + kBlock - 1) / kBlock;
+      s0 = block_sum(s0);
+      s1 = block_sum(s1);
+      s2 = block_sum(s2);
+      if (threadIdx.x == 0) {
+        s.partial[lb] = s0;
+        s.partial[nb + lb] = s1;
+        s.partial[2 * nb + lb] = s2;
+      }
+    }
+  }
+}
+
+template <class L>
+__global__ __launch_bounds__(kBlock) void k_batch_cg(const BatchSolve* __restrict__ solves,
+                                                     const uint32_t* __restrict__ items, int32_t n_items,
+                                                     int32_t per_wg, int32_t parity) {
+  constexpr int D = L::nD;
+  const int32_t g = node_block(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
+  const int32_t beg = g * per_wg;
+  const int32_t end = beg + per_wg < n_items ? beg + per_wg : n_items;
+  for (int32_t it = beg; it < end; ++it) {
+    const uint32_t item = items[it];
+    const BatchSolve& s = solves[item >> kItemShift];
+    if (!s.live) continue;  // stopped at an earlier check of this launch sequence
+    const int32_t lb = static_cast<int32_t>(item & kItemMask);
+    const int p = (parity ^ s.parity) & 1;
+    const RowWin ad{s.coord, s.win, s.NP, s.PY, s.W};
+    const int32_t i = lb * kBlock + static_cast<int32_t>(threadIdx.x);
+    if (i >= s.n) continue;
+    CgArgs a{};
+    a.nu0Inv = s.nu0Inv;
+    a.nu1Inv = s.nu1Inv;
+    a.sigma = s.sigma;
+    a.beta = s.beta;
+    double F[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) F[d] = s.a.F[d];
+    const WallPhi wp{nullptr, s.phi_w};
+    cg_node<L, false, false>(static_cast<const double*>(s.f[p]), static_cast<double*>(s.f[p ^ 1]), ad, i, s.NP, a,
+                             static_cast<const double*>(s.phi), wp, F, nullptr, nullptr);
   }
 }
 
@@ -733,6 +898,35 @@ hipError_t unstage_f64(int lattice, int nf, const double* g, double* f, int32_t 
   if (n <= 0) return hipSuccess;
   LB_DISPATCH_LATTICE(lattice, L, {
     k_unstage<L><<<grid_for(n), dim3(kBlock), 0, s>>>(g, f, n, NP, nf, rank);
+  });
+  return hipGetLastError();
+}
+
+hipError_t batch_check(BatchSolve* solves, SolveCheck* chk, const int32_t* slots, int32_t n_slots, const double* out,
+                       int32_t M, uint32_t* step, hipStream_t s) {
+  if (n_slots > 0)
+    k_batch_check<<<dim3(static_cast<unsigned>((n_slots + kBlock - 1) / kBlock)), dim3(kBlock), 0, s>>>(
+        solves, chk, slots, n_slots, out, M);
+  k_batch_next_step<<<dim3(1), dim3(1), 0, s>>>(step, M);
+  return hipGetLastError();
+}
+
+hipError_t prestream_f64(int lattice, const RowWin& ad, const double* r0, int32_t n, int64_t NP, double* f,
+                         hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  LB_DISPATCH_LATTICE(lattice, L, {
+    k_prestream<L><<<grid_for(n), dim3(kBlock), 0, s>>>(ad, r0, n, NP, f);
+  });
+  return hipGetLastError();
+}
+
+hipError_t batch_cg_f64(int lattice, bool reduce, const BatchSolve* solves, const uint32_t* items,
+                        int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, hipStream_t s) {
+  LB_DISPATCH_LATTICE(lattice, L, {
+    const dim3 g(grid), b(kBlock);
+    if (reduce) k_batch_cg_phi<L, true><<<g, b, 0, s>>>(solves, items, n_items, per_wg, parity);
+    else k_batch_cg_phi<L, false><<<g, b, 0, s>>>(solves, items, n_items, per_wg, parity);
+    k_batch_cg<L><<<g, b, 0, s>>>(solves, items, n_items, per_wg, parity);
   });
   return hipGetLastError();
 }

@@ -73,6 +73,8 @@ SIGNATURES = {
                                  C.POINTER(C.c_double)]),
     "lb_solve_permeability": (C.c_int, [_vp, C.c_double, _f64p, C.c_int32, C.c_double, C.c_int64,
                                         C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "lb_solve_relperm": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_double, _f64p, C.c_int32,
+                                   C.c_double, C.c_int64, C.c_double, _f64p, C.POINTER(C.c_int64)]),
     "lb_solve_permeability_trt": (C.c_int, [_vp, C.c_double, C.c_double, _f64p, C.c_int32, C.c_double,
                                             C.c_int64, C.c_double, C.POINTER(C.c_double),
                                             C.POINTER(C.c_int64)]),
@@ -130,11 +132,14 @@ SIGNATURES = {
     "lb_batch_add": (C.c_int, [_vp, C.c_int32, _i32p, _i32p, C.c_int32, C.c_double, _f64p, C.c_double,
                                C.c_int64, C.c_int64]),
     "lb_batch_advance": (C.c_int, [_vp, C.c_int32, C.POINTER(C.c_int32)]),
+    "lb_batch_launch": (C.c_int, [_vp, C.c_int32]),
+    "lb_batch_wait": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
     "lb_batch_collect": (C.c_int, [_vp, C.c_int32, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
                                    _f64p, _f64p, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
                                    _f64p, _i32p, C.POINTER(C.c_int32)]),
     "lb_batch_status": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                   C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "lb_batch_set_precision": (C.c_int, [_vp, C.c_int32, C.c_int32]),
     "lb_batch_set_shape": (C.c_int, [_vp, C.c_int32]),
     "lb_batch_shape": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
     "lb_batch_kernel_time": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
@@ -496,6 +501,16 @@ class Context:
                                                 int(max_steps), float(n_box), C.byref(k), C.byref(s)))
         return k.value, s.value
 
+    def solve_relperm(self, tau0, tau1, sigma, beta, force, check_every, tol, max_steps, n_box):
+        """lb_solve_relperm: ((k0, k1), steps)."""
+        F = np.zeros(3)
+        F[: self.nD] = force[: self.nD]
+        k, s = np.zeros(2), C.c_int64()
+        self._check(lib().lb_solve_relperm(self._h, float(tau0), float(tau1), float(sigma), float(beta), F,
+                                           int(check_every), float(tol), int(max_steps), float(n_box), k,
+                                           C.byref(s)))
+        return (float(k[0]), float(k[1])), s.value
+
     def solve_permeability_trt(self, tau_sym, tau_anti, force, check_every, tol, max_steps, n_box):
         F = np.zeros(3)
         F[: self.nD] = force[: self.nD]
@@ -603,6 +618,7 @@ class Batch:
         self._h = h
         self.lattice = lattice
         self.precision = precision
+        self.prec = [precision, precision]  # per kind: single-phase, two-phase
         self.nD = ND_OF[lattice]
         self.check_every = int(check_every)
 
@@ -670,8 +686,20 @@ class Batch:
         return True
 
     def advance(self, intervals=1):
+        """Run up to `intervals` check intervals and wait; returns how many
+        solves finished."""
         n = C.c_int32()
         self._check(lib().lb_batch_advance(self._h, int(intervals), C.byref(n)))
+        return n.value
+
+    def launch(self, intervals=1):
+        """Enqueue `intervals` check intervals and return at once."""
+        self._check(lib().lb_batch_launch(self._h, int(intervals)))
+
+    def wait(self):
+        """Wait for the launched intervals; returns how many solves finished."""
+        n = C.c_int32()
+        self._check(lib().lb_batch_wait(self._h, C.byref(n)))
         return n.value
 
     def collect(self, max_n=1 << 16):
@@ -692,6 +720,13 @@ class Batch:
         self._check(lib().lb_batch_status(self._h, C.byref(a), C.byref(b), C.byref(c), C.byref(d), C.byref(e)))
         return {"resident": a.value, "resident_nodes": b.value, "bytes_used": c.value, "bytes_free": d.value,
                 "largest_free": e.value}
+
+    def set_precision(self, kind, precision):
+        """Precision of the solves of one kind admitted from now on: kind
+        "perm" (single-phase) or "relperm" (two-phase)."""
+        k = {"perm": 0, "relperm": 1}.get(kind, kind)
+        self._check(lib().lb_batch_set_precision(self._h, int(k), int(precision)))
+        self.prec[int(k)] = int(precision)
 
     def set_shape(self, wg_per_cu):
         self._check(lib().lb_batch_set_shape(self._h, int(wg_per_cu)))

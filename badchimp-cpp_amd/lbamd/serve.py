@@ -10,7 +10,7 @@ advanced together, and finished solves are returned with their latency
 import time
 from collections import deque
 
-from .ffi import D3Q19, FP32S, Batch, batch_footprint
+from .ffi import D3Q19, FP32S, FP64, Batch, batch_footprint
 
 
 class Request:
@@ -45,12 +45,23 @@ class Request:
 
 
 class Server:
+    """precision: storage of the permeability solves; relperm_precision: of the
+    relative-permeability solves (FP64 by default: over 1e4..1e5 steps the
+    colour-gradient runs in FP32S or FP16S drift from FP64 by 1e-4..5e-3 in
+    the phase permeabilities, while FP64 itself moves by 2e-12 under a 1e-13
+    perturbation of its start, so only FP64 storage meets a 1e-4 bar)."""
+
     def __init__(self, pool_bytes, lattice=D3Q19, precision=FP32S, check_every=100, device=0, lookahead=64,
-                 batch=None):
+                 batch=None, relperm_precision=FP64):
         # batch: an object with the Batch interface (add, add_twophase,
         # advance, collect, status, close); a device Batch by default
-        self.batch = batch if batch is not None else Batch(lattice, precision, pool_bytes, check_every, device)
+        if batch is None:
+            batch = Batch(lattice, precision, pool_bytes, check_every, device)
+            batch.set_precision("relperm", relperm_precision)
+        self.batch = batch
         self.lattice, self.precision = lattice, precision
+        prec = getattr(batch, "prec", None)  # per kind, when the batch keeps it
+        self.relperm_precision = prec[1] if prec is not None else relperm_precision
         self.pool_bytes = pool_bytes
         self.lookahead = lookahead
         self.waiting = deque()
@@ -62,8 +73,9 @@ class Server:
         if req.t_arrival is None:
             req.t_arrival = self.clock()
         if req.bytes is None:
-            nf = 2 if req.kind == "relperm" else 1
-            req.bytes = batch_footprint(self.lattice, self.precision, req.geo, req.periodic, nf)[0]
+            two = req.kind == "relperm"
+            prec = self.relperm_precision if two else self.precision
+            req.bytes = batch_footprint(self.lattice, prec, req.geo, req.periodic, 2 if two else 1)[0]
         if req.bytes > self.pool_bytes:
             raise ValueError("request %r needs %d bytes, more than the whole pool" % (req.tag, req.bytes))
         self.waiting.append(req)
@@ -86,11 +98,23 @@ class Server:
         self.waiting = kept
 
     def step(self, intervals=1):
-        """Admit, advance all resident solves, and collect finished ones."""
-        self.admit()
+        """Advance all resident solves by up to `intervals` check intervals and
+        collect the finished ones.  Waiting requests are admitted while the
+        intervals run on the device (their host-side set-up and uploads
+        overlap the steps) and join at the next call; with nothing resident
+        they are admitted first."""
         if not self.resident:
-            return []
-        self.batch.advance(intervals)
+            self.admit()
+            if not self.resident:
+                return []
+        launch = getattr(self.batch, "launch", None)
+        if launch is None:
+            self.admit()
+            self.batch.advance(intervals)
+        else:
+            launch(intervals)
+            self.admit()
+            self.batch.wait()
         now = self.clock()
         out = []
         for r in self.batch.collect():

@@ -231,10 +231,11 @@ int lb_reduce_flux(lb_ctx* ctx, int32_t dir, double* sum_fc,
                    double* sum_phi0_fc, double* sum_phi1_fc);
 
 /* Single-phase permeability solve on the device: step BGK until
- * |k_t - k_{t-M}| / |k_t| < tol, checked every M = check_every steps, or
- * max_steps.  k = nu * (sum_n j_x) / (n_box * F_x) with j = sum_q f c + F/2
- * and nu = (tau - 1/2)/3 (std_one_phase/main.cpp:603-624 uses the same
- * relative-change test).  Returns k and the steps taken. */
+ * |k_t - k_{t-M}| <= tol |k_t| at two consecutive checks, every
+ * M = check_every steps, or max_steps.  k = nu * (sum_n j_x) / (n_box * F_x)
+ * with j = sum_q f c + F/2 and nu = (tau - 1/2)/3 (std_one_phase/main.cpp:
+ * 603-624 prints the same relative change of its fluxes; it has no stop).
+ * Returns k and the steps taken. */
 int lb_solve_permeability(lb_ctx* ctx, double tau, const double* force,
                           int32_t check_every, double tol, int64_t max_steps,
                           double n_box, double* k_out, int64_t* steps_out);
@@ -352,6 +353,16 @@ int lb_vtu_write(lb_vtu* v, double time);                             /* Output:
 int32_t lb_vtu_num_points(const lb_vtu* v);
 int lb_vtu_destroy(lb_vtu* v);
 
+/* Relative-permeability solve (colour gradient, fixed body force on the
+ * total, LB_FORCE_FIXED): lb_step_cg until both phase permeabilities
+ * k_i = nu_i <phi_i sum_q f c_x> / F_x (nu_i = (tau_i - 1/2)/3,
+ * LBglobalforcing.h:61-78), taken every check_every steps from the state the
+ * last step of the interval starts from, change by at most
+ * tol (|k_0| + |k_1|) at two consecutive checks, or max_steps.  The stop of
+ * a two-phase solve in a batch; k_phase[2] and the steps taken. */
+int lb_solve_relperm(lb_ctx* ctx, double tau0, double tau1, double sigma, double beta,
+                     const double* force, int32_t check_every, double tol, int64_t max_steps,
+                     double n_box, double* k_phase, int64_t* steps_out);
 /* ---- batched and serving form -------------------------------------------
  * No counterpart in the reference: the reference runs one solve per process
  * (std_case/main.cpp).  A batch keeps many independent permeability
@@ -361,7 +372,7 @@ int lb_vtu_destroy(lb_vtu* v);
  * them with one launch per step.  A solve is admitted when its footprint
  * (populations x 2, coordinates, row-window table, block partials) fits into
  * the pool; it retires when |dk| <= tol |k| between checks every check_every
- * steps (or at max_steps) and frees its memory.  Each solve's per-node
+ * steps, at two consecutive checks (or at max_steps), and frees its memory.  Each solve's per-node
  * arithmetic and reduction order are those of a single context with row
  * windows, so its k and step count are the same alone, in any mix, and
  * through lb_solve_permeability. */
@@ -381,14 +392,15 @@ int lb_batch_footprint(int lattice, int precision, int32_t n_fields, int32_t nd,
 int lb_batch_add(lb_batch* b, int32_t nd, const int32_t* dims, const int32_t* geo,
                  int32_t periodic_mask, double tau, const double* force, double tol,
                  int64_t max_steps, int64_t tag);
-/* Admit a colour-gradient two-phase solve (FP32S / FP16S batches): rho0 per
+/* Admit a colour-gradient two-phase solve (in the batch's two-phase precision,
+ * lb_batch_set_precision): rho0 per
  * geometry cell (phase-0 density, rho1 = 1 - rho0, f_k = w rho_k at rest),
  * the solid-boundary colour from wettability (rho0 = wettability on every
  * solid neighbour, main_TWOPHASE.cpp:171-178,280-284), fixed body force on the
  * total (the LB_FORCE_FIXED mode of lb_step_cg).  It retires when both phase
  * permeabilities k_i = nu_i <phi_i sum_q f c_x> / F_x (nu_i = (tau_i - 1/2)/3,
  * LBglobalforcing.h:61-78) change by at most tol (|k_0| + |k_1|) between
- * checks. */
+ * checks, at two consecutive checks. */
 /* Admit count single-phase solves (geometry k: dims[k*nd..], geos[k]) with
  * the parameters of lb_batch_add, tagged tags[k].  The row layouts are built
  * on up to 16 host threads, then the solves are admitted in order until one
@@ -402,10 +414,19 @@ int lb_batch_add_twophase(lb_batch* b, int32_t nd, const int32_t* dims, const in
                           int32_t periodic_mask, const double* rho0, double wettability,
                           double tau0, double tau1, double sigma, double beta,
                           const double* force, double tol, int64_t max_steps, int64_t tag);
-/* Advance every resident solve by `intervals` check intervals; solves that
- * finish retire at the end of their interval.  Admissions take effect at the
- * next interval. */
+/* Advance every resident solve by up to `intervals` check intervals and wait
+ * (lb_batch_launch, then lb_batch_wait).  Each interval ends on the device:
+ * the per-solve sums and the stop test, so a solve stops at its own interval
+ * (it is skipped by the rest of the sequence) and the host synchronises once
+ * per call.  Admissions take effect at the next call. */
 int lb_batch_advance(lb_batch* b, int32_t intervals, int32_t* n_finished);
+/* The same in two halves: lb_batch_launch enqueues the intervals (HIP graphs
+ * of one interval each, replayed) and returns; lb_batch_wait waits for them,
+ * retires the solves whose stop test decided and frees their memory.  In
+ * between, the host may admit solves (lb_batch_add*): their uploads run on
+ * a stream of their own and overlap the intervals in flight. */
+int lb_batch_launch(lb_batch* b, int32_t intervals);
+int lb_batch_wait(lb_batch* b, int32_t* n_finished);
 /* Take up to max_n finished solves: tag, k (two-phase: k_0 + k_1),
  * k_phase[2] (two-phase k_0, k_1), steps, seconds resident, status
  * (0 converged, 1 max_steps reached, 2 k not finite). */
@@ -413,6 +434,12 @@ int lb_batch_collect(lb_batch* b, int32_t max_n, int64_t* tags, double* k, doubl
                      int64_t* steps, double* seconds, int32_t* status, int32_t* n_out);
 int lb_batch_status(const lb_batch* b, int32_t* n_resident, int64_t* resident_nodes,
                     int64_t* bytes_used, int64_t* bytes_free, int64_t* largest_free);
+/* Precision of the solves of one kind admitted from now on (kind 0
+ * single-phase, 1 two-phase; LB_FP64, LB_FP32S or LB_FP16S); both kinds start
+ * at the batch's precision.  LB_ERR_STATE while solves of that kind are
+ * resident.  A two-phase solve in FP64 has the bits of lb_step_cg (fixed
+ * force) on a context with row windows. */
+int lb_batch_set_precision(lb_batch* b, int32_t kind, int32_t precision);
 /* Launch shape: -1 autotune on live steps after each change of the resident
  * set (default), 0 one workgroup per 256-node block, k > 0 a persistent launch
  * of k workgroups per CU. */

@@ -225,13 +225,18 @@ def test_batch_d2q9_equals_single_context(gpu_available):
 
 
 def test_batch_rejections(gpu_available, geos):
-    """Two-phase solves need a reduced-precision batch; geometries the row
-    windows cannot address and bad arguments are refused with a message."""
+    """Geometries the row windows cannot address and bad arguments are
+    refused with a message; the precision of a kind cannot change while
+    solves of that kind are resident."""
     b = lbamd.Batch(lbamd.D3Q19, lbamd.FP64, 1 << 26, 50)
     rho0 = (geos[0] > 0).astype(float)
+    assert b.add_twophase(geos[0], "x", rho0, 1.0, 1.0, 1.0, 0.01, 0.7, FORCE, 1e-4, 1000, 0)
     with pytest.raises(lbamd.LBError) as e:
-        b.add_twophase(geos[0], "x", rho0, 1.0, 1.0, 1.0, 0.01, 0.7, FORCE, 1e-4, 1000, 0)
-    assert e.value.code == -4
+        b.set_precision("relperm", lbamd.FP32S)
+    assert e.value.code == -2
+    b.set_precision("perm", lbamd.FP16S)  # no single-phase solve resident
+    b.close()
+    b = lbamd.Batch(lbamd.D3Q19, lbamd.FP64, 1 << 26, 50)
     with pytest.raises(lbamd.LBError) as e:
         b.add(geos[0], "xyz", TAU, FORCE, 1e-6, 1000, 1)  # periodic along the row axis
     assert e.value.code == -4

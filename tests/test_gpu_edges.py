@@ -22,7 +22,9 @@ def test_empty_bulk_list(gpu_available):
         ctx.step_bgk(3, 1.0, np.array([1e-6, 0.0, 0.0]))
         assert ctx.reduce_flux(0) == (0.0, 0.0, 0.0)
         k, steps = ctx.solve_permeability(1.0, np.array([1e-6, 0.0, 0.0]), 10, 1e-6, 100, float(geo.size))
-        assert k == 0.0 and steps == 20
+        # k = 0 at every check: the test holds from the second check on and
+        # stops at the third (two consecutive checks)
+        assert k == 0.0 and steps == 30
         ctx.close()
 
 

@@ -68,7 +68,7 @@ def test_admission_order_lookahead_and_latency():
     pool = sizes[1] + 1
     assert sizes[0] + two <= pool < sizes[0] + sizes[1]
     fb = FakeBatch(pool, {0: 3, 1: 2, 2: 1, 3: 1})
-    srv = Server(pool, batch=fb)
+    srv = Server(pool, batch=fb, relperm_precision=lbamd.FP32S)
     reqs = [Request(0, gs[0]), Request(1, gs[1]),
             Request(2, gs[2], kind="relperm", rho0=(gs[2] > 0).astype(float)), Request(3, gs[3])]
     for r in reqs:
@@ -102,7 +102,7 @@ def test_stream_replay_admits_while_others_run():
     work = {i: 4 for i in range(6)}
     pool = 10 << 30
     fb = FakeBatch(pool, work)
-    srv = Server(pool, batch=fb)
+    srv = Server(pool, batch=fb, relperm_precision=lbamd.FP32S)
     reqs = [Request(i, g, t_arrival=0.0 if i < 3 else 0.01) for i, g in enumerate(gs)]
     done, wall = run_stream(srv, reqs)
     assert sorted(d["tag"] for d in done) == list(range(6))
