@@ -63,6 +63,12 @@ __device__ __forceinline__ int32_t node_block(int32_t b, int32_t nb) {
   return ((k / kXcdChunk) * 8 + xcd) * kXcdChunk + (k % kXcdChunk);
 }
 
+// The same mapping with the dispatch order reversed: a pass that follows
+// another one over the same nodes starts where the first one ended, so its
+// first reads find the lines that the first pass fetched last still in the
+// MALL.  The two-phase collision pass runs this way after the colour pass.
+__device__ __forceinline__ int32_t node_block_rev(int32_t b, int32_t nb) { return node_block(nb - 1 - b, nb); }
+
 // Element indices are 32-bit: a context keeps Q * NP * fields below 2^32.
 // kOff32: the layout is only used for population buffers below 4 GiB, so the
 // kernels may address them with 32-bit byte offsets from the buffer base.

@@ -189,6 +189,7 @@ struct lb_batch {
   // launch shape: wg_per_cu > 0 persistent, 0 one workgroup per item
   int32_t shape_mode = -1;  // -1 autotune, else fixed
   int32_t shape = 4;
+  int32_t ns = 1;           // 256-thread slices per workgroup (workgroups of 256 ns threads)
   int64_t tuned_nodes = -1;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   // results and counters
@@ -221,11 +222,12 @@ void drop_graphs(lb_batch* b) {
     }
 }
 
-// grid and items per workgroup of a shape
-void shape_dims(const lb_batch* b, int32_t shape, int32_t n_items, int32_t* grid, int32_t* per) {
-  if (shape <= 0 || n_items <= b->n_cu * shape) {
-    *grid = n_items;
-    *per = 1;
+// grid and items per workgroup of a shape (workgroups per CU; 0: one
+// workgroup per ns items) with ns items in flight per workgroup
+void shape_dims(const lb_batch* b, int32_t shape, int32_t ns, int32_t n_items, int32_t* grid, int32_t* per) {
+  if (shape <= 0 || n_items <= b->n_cu * shape * ns) {
+    *per = ns;
+    *grid = (n_items + ns - 1) / ns;
     return;
   }
   const int32_t g = b->n_cu * shape;
@@ -233,28 +235,30 @@ void shape_dims(const lb_batch* b, int32_t shape, int32_t n_items, int32_t* grid
   *grid = (n_items + *per - 1) / *per;
 }
 
-hipError_t launch_step(lb_batch* b, bool reduce, int32_t shape, int32_t koff) {
+hipError_t launch_step(lb_batch* b, bool reduce, int32_t shape, int32_t ns, int32_t koff) {
   const int32_t parity = static_cast<int32_t>(b->global_step & 1);
   int32_t grid, per;
   hipError_t e = hipSuccess;
   if (b->n_items_bgk > 0) {
+    // single-phase: j_x summed over every step of the interval (the first
+    // step of the interval overwrites the partials)
     const int32_t n = b->n_items_bgk;
-    shape_dims(b, shape, n, &grid, &per);
+    shape_dims(b, shape, ns, n, &grid, &per);
     if (b->prec[0] == LB_FP64)
-      e = batch_bgk_f64(b->lattice, reduce, b->d_solves, b->d_items, n, grid, per, parity, b->stream);
+      e = batch_bgk_f64(b->lattice, true, b->d_solves, b->d_items, n, grid, per, parity, ns, koff > 0, b->stream);
     else
-      e = batch_bgk_lowp(b->prec[0], b->lattice, reduce, b->d_solves, b->d_items, n, grid, per, parity,
-                         b->d_step, koff, b->stream);
+      e = batch_bgk_lowp(b->prec[0], b->lattice, true, b->d_solves, b->d_items, n, grid, per, parity,
+                         b->d_step, koff, ns, koff > 0, b->stream);
   }
   if (e == hipSuccess && b->n_items_cg > 0) {
     const int32_t n = b->n_items_cg;
     const uint32_t* it = b->d_items + b->n_items_bgk;
-    shape_dims(b, shape, n, &grid, &per);
+    shape_dims(b, shape, ns, n, &grid, &per);
     if (b->prec[1] == LB_FP64)
-      e = batch_cg_f64(b->lattice, reduce, b->d_solves, it, n, grid, per, parity, b->stream);
+      e = batch_cg_f64(b->lattice, reduce, b->d_solves, it, n, grid, per, parity, ns, b->stream);
     else
       e = batch_cg_lowp(b->prec[1], b->lattice, reduce, b->d_solves, it, n, grid, per, parity, b->d_step, koff,
-                        b->stream);
+                        ns, b->stream);
   }
   return e;
 }
@@ -312,16 +316,19 @@ hipError_t close_interval(lb_batch* b) {
 
 // One check interval as individual launches, timing each candidate launch
 // shape on live steps (the shape never changes a result); keeps the fastest.
+// A candidate is (workgroups per CU, 0 for one workgroup per ns items; ns,
+// the 256-node items a workgroup works on at a time, i.e. workgroups of 256
+// or 512 threads).
 int tune_interval(lb_batch* b) {
   const int M = b->check_every;
-  const int32_t cand[] = {0, 1, 2, 4, 8};
-  constexpr int kCand = 5;
+  const int32_t cand[][2] = {{0, 1}, {1, 1}, {2, 1}, {4, 1}, {8, 1}, {0, 2}, {1, 2}, {2, 2}, {4, 2}};
+  constexpr int kCand = 9;
   std::vector<hipEvent_t> ev(2 * M);
   for (auto& e : ev) LB_BHIP(b, hipEventCreate(&e));
   const int64_t gs = b->global_step;
   for (int s = 0; s < M; ++s) {
     hipEventRecord(ev[2 * s], b->stream);
-    hipError_t e = launch_step(b, s == M - 1, cand[s % kCand], s);
+    hipError_t e = launch_step(b, s == M - 1, cand[s % kCand][0], cand[s % kCand][1], s);
     hipEventRecord(ev[2 * s + 1], b->stream);
     if (e != hipSuccess) return bfail(b, LB_ERR_HIP, "batch step: %s", hipGetErrorString(e));
     ++b->global_step;
@@ -330,8 +337,8 @@ int tune_interval(lb_batch* b) {
   hipError_t e = close_interval(b);
   if (e != hipSuccess) return bfail(b, LB_ERR_HIP, "batch check: %s", hipGetErrorString(e));
   LB_BHIP(b, hipStreamSynchronize(b->stream));
-  double t[kCand] = {0, 0, 0, 0, 0};
-  int cnt[kCand] = {0, 0, 0, 0, 0};
+  double t[kCand] = {};
+  int cnt[kCand] = {};
   for (int s = 0; s < M; ++s) {
     float ms = 0.f;
     hipEventElapsedTime(&ms, ev[2 * s], ev[2 * s + 1]);
@@ -350,7 +357,8 @@ int tune_interval(lb_batch* b) {
       bt = t[c] / cnt[c];
       best = c;
     }
-  b->shape = cand[best];
+  b->shape = cand[best][0];
+  b->ns = cand[best][1];
   b->tuned_nodes = b->resident_nodes;
   b->launches += M + 3;
   return LB_OK;
@@ -369,7 +377,7 @@ int interval_graph(lb_batch* b, int parity, hipGraphExec_t* out) {
     const int64_t gs = b->global_step;
     b->global_step = parity;  // launch_step reads only the parity of the step
     for (int s = 0; s < M && e == hipSuccess; ++s) {
-      e = launch_step(b, s == M - 1, b->shape, s);
+      e = launch_step(b, s == M - 1, b->shape, b->ns, s);
       ++b->global_step;
     }
     b->global_step = gs;
@@ -402,7 +410,10 @@ int launch_sequence(lb_batch* b, int32_t intervals) {
   int32_t k = 0;
   const bool tune = b->shape_mode < 0 &&
                     (b->tuned_nodes < 0 || std::llabs(b->resident_nodes - b->tuned_nodes) * 5 > b->tuned_nodes);
-  if (b->shape_mode >= 0) b->shape = b->shape_mode;
+  if (b->shape_mode >= 0) {
+    b->shape = b->shape_mode;
+    b->ns = 1;
+  }
   if (tune) {
     const int rc = tune_interval(b);
     if (rc != LB_OK) return rc;
@@ -964,6 +975,14 @@ int lb_batch_set_shape(lb_batch* b, int32_t wg_per_cu) {
 int lb_batch_shape(const lb_batch* b, int32_t* wg_per_cu) {
   if (!b || !wg_per_cu) return LB_ERR_ARG;
   *wg_per_cu = b->shape;
+  return LB_OK;
+}
+
+int lb_batch_tuning(const lb_batch* b, int32_t* wg_per_cu, int32_t* wg_threads, int64_t* tuned_nodes) {
+  if (!b) return LB_ERR_ARG;
+  if (wg_per_cu) *wg_per_cu = b->shape;
+  if (wg_threads) *wg_threads = kBlock * b->ns;
+  if (tuned_nodes) *tuned_nodes = b->tuned_nodes;
   return LB_OK;
 }
 

@@ -1257,15 +1257,18 @@ int solve_permeability(lb_ctx* c, const char* name, double tau, double tau_anti,
   while (steps < max_steps) {
     const int64_t chunk = std::min<int64_t>(check_every, max_steps - steps);
     for (int64_t s = 0; s < chunk; ++s) {
+      // every step adds its block sums of j_x to the partials (the first
+      // step of the chunk overwrites them); the last one stores rho and u
       const bool last = s == chunk - 1;
       hipError_t e;
       if (c->precision == LB_FP64)
-        e = bgk_f64(c->lattice, !c->post, last, last, static_cast<const double*>(c->d_f[c->cur]),
+        e = bgk_f64(c->lattice, !c->post, last, true, static_cast<const double*>(c->d_f[c->cur]),
                     static_cast<double*>(c->d_f[1 - c->cur]), c->addr, c->n_bulk, c->NP, a, c->d_rho,
-                    c->d_vel, c->d_partial, c->stream);
+                    c->d_vel, c->d_partial, c->stream, s > 0);
       else
-        e = bgk_lowp(c->precision, c->lattice, !c->post, last, last, c->d_f[c->cur], c->d_f[1 - c->cur],
-                     c->addr, c->n_bulk, c->NP, with_step(a, c->step_count), c->d_rho, c->d_vel, c->d_partial, c->stream);
+        e = bgk_lowp(c->precision, c->lattice, !c->post, last, true, c->d_f[c->cur], c->d_f[1 - c->cur],
+                     c->addr, c->n_bulk, c->NP, with_step(a, c->step_count), c->d_rho, c->d_vel, c->d_partial,
+                     c->stream, s > 0);
       if (e != hipSuccess) return fail(c, LB_ERR_HIP, "bgk launch: %s", hipGetErrorString(e));
       c->cur = 1 - c->cur;
       c->post = true;
@@ -1278,8 +1281,9 @@ int solve_permeability(lb_ctx* c, const char* name, double tau, double tau_anti,
     if (e == hipSuccess) e = hipMemcpyAsync(&jx, c->d_scalar + 4, sizeof(double), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return fail(c, LB_ERR_HIP, "permeability reduction: %s", hipGetErrorString(e));
-    // j_x summed over the state the last step started from; k = nu <j_x> / F_x
-    k = nu * (jx / n_box) / a.F[0];
+    // j_x summed over the states the chunk's steps started from;
+    // k = nu <j_x> / F_x with <j_x> the chunk's average per node of the box
+    k = nu * ((jx / static_cast<double>(chunk)) / n_box) / a.F[0];
     if (!std::isfinite(k)) return fail(c, LB_ERR_STATE, "permeability is not finite after %lld steps", (long long)steps);
     held = have_prev && std::fabs(k - k_prev) <= tol * std::fabs(k) ? held + 1 : 0;
     if (held >= kStopHold) break;

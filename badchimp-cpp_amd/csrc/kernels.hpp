@@ -20,10 +20,11 @@ struct Addr {
 // ---- FP64 (reference arithmetic, -ffp-contract=off) ----------------------
 // One BGK + Guo collide-stream step in pull form (std_case/main.cpp:110-145).
 // first: buffer holds pre-collision f (identity read); store: write rho/vel;
-// reduce: write per-block partial sums of j_x = sum_q f c_x + F_x/2.
+// reduce: write per-block partial sums of j_x = sum_q f c_x + F_x/2 (accum:
+// add them to the partials, summing over the steps of a check interval).
 hipError_t bgk_f64(int lattice, bool first, bool store, bool reduce, const double* fin,
                    double* fout, const Addr& ad, int32_t n, int64_t NP, const BgkArgs& a,
-                   double* rho, double* vel, double* partial, hipStream_t s);
+                   double* rho, double* vel, double* partial, hipStream_t s, bool accum = false);
 
 // Two-phase pass 1 (+ pass 2 partial sums): phi = (rho0-rho1)/(rho0+rho1)
 // and sum_q (f0+f1) c_x per block (main_TWOPHASE.cpp:238-298).  rowwin: the
@@ -64,10 +65,12 @@ hipError_t tag_mass_f64(int lattice, bool first, const double* fin, const TableA
 
 // ---- batched form -----------------------------------------------------------
 // One FP64 step of every solve listed in items (see k_batch_bgk): grid
-// workgroups of 256 threads, each over a run of per_wg items; reduce writes
-// each solve's block partials of j_x.
+// workgroups of 256 ns threads (ns = 1 or 2 items at a time), each over a run
+// of per_wg items; reduce writes each solve's block partials of j_x (accum:
+// adds to them).
 hipError_t batch_bgk_f64(int lattice, bool reduce, const BatchSolve* solves, const uint32_t* items,
-                         int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, hipStream_t s);
+                         int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, int ns, bool accum,
+                         hipStream_t s);
 // Two-phase rest state (phase-0 density r0 per compact node) placed where the
 // first pull through ad reads it (FP64 form).
 hipError_t prestream_f64(int lattice, const RowWin& ad, const double* r0, int32_t n, int64_t NP, double* f,
@@ -75,7 +78,7 @@ hipError_t prestream_f64(int lattice, const RowWin& ad, const double* r0, int32_
 // Both passes of one FP64 colour-gradient step of every two-phase solve listed
 // in items (as batch_bgk_f64); reduce writes each solve's three flux partials.
 hipError_t batch_cg_f64(int lattice, bool reduce, const BatchSolve* solves, const uint32_t* items,
-                        int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, hipStream_t s);
+                        int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, int ns, hipStream_t s);
 // out[slot] = sum of the solve's block partials, for each slot listed.
 hipError_t batch_sum(const BatchSolve* solves, const int32_t* slots, int32_t n_slots, double* out,
                      hipStream_t s);

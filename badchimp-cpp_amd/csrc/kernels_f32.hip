@@ -101,6 +101,28 @@ __device__ __forceinline__ double block_sum(double v) {
   return r;
 }
 
+// block_sum for each 256-thread slice of a workgroup of NS slices: the same
+// shuffles and the same order over the slice's four waves, so a slice sums
+// exactly as a 256-thread workgroup does.  Result valid in the slice's
+// first thread.
+template <int NS>
+__device__ __forceinline__ double slice_sum(double v) {
+  __shared__ double red[NS * (kBlock / 64)];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double r = 0.0;
+  if ((threadIdx.x % kBlock) == 0) {
+    const int w0 = (threadIdx.x / kBlock) * (kBlock / 64);
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) r += red[w0 + w];
+  }
+  __syncthreads();
+  return r;
+}
+
 template <class L, typename S, bool FIRST, class A>
 __device__ __forceinline__ void load_g(const S* __restrict__ gin, const A& ad, int32_t i, int64_t NP,
                                        float (&g)[L::nQ]) {
@@ -275,7 +297,8 @@ __global__ __launch_bounds__(kBlock) void k_bgk_lp(const S* __restrict__ gin, S*
                                                    double* __restrict__ vel_o,
                                                    double* __restrict__ partial, float tauA_inv,
                                                    float tauA_factor, const float* __restrict__ sr,
-                                                   const float* __restrict__ nu, int tab_n, uint32_t step) {
+                                                   const float* __restrict__ nu, int tab_n, uint32_t step,
+                                                   int32_t accum) {
   const int32_t b = node_block(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
   const int32_t i = b * kBlock + threadIdx.x;
   double jx = 0.0;
@@ -286,31 +309,36 @@ __global__ __launch_bounds__(kBlock) void k_bgk_lp(const S* __restrict__ gin, S*
   }
   if (REDUCE) {
     const double s = block_sum(jx);
-    if (threadIdx.x == 0) partial[b] = s;
+    if (threadIdx.x == 0) partial[b] = accum ? partial[b] + s : s;
   }
 }
 
 // Batched form (see k_batch_bgk in the FP64 kernels): the parameters are
 // rounded to float exactly as the single-solve launcher rounds them.
-template <class L, typename S, bool REDUCE>
-__global__ __launch_bounds__(kBlock) void k_batch_bgk_lp(const BatchSolve* __restrict__ solves,
-                                                         const uint32_t* __restrict__ items, int32_t n_items,
-                                                         int32_t per_wg, int32_t parity,
-                                                         const uint32_t* __restrict__ step0, int32_t koff) {
+template <class L, typename S, bool REDUCE, int NS>
+__global__ __launch_bounds__(kBlock * NS) void k_batch_bgk_lp(const BatchSolve* __restrict__ solves,
+                                                              const uint32_t* __restrict__ items, int32_t n_items,
+                                                              int32_t per_wg, int32_t parity,
+                                                              const uint32_t* __restrict__ step0, int32_t koff,
+                                                              int32_t accum) {
   const uint32_t gstep = *step0 + static_cast<uint32_t>(koff);  // global step of this launch
   const int32_t g = node_block(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
+  const int32_t sub = static_cast<int32_t>(threadIdx.x) / kBlock, tid = static_cast<int32_t>(threadIdx.x) % kBlock;
   const int32_t beg = g * per_wg;
   const int32_t end = beg + per_wg < n_items ? beg + per_wg : n_items;
-  for (int32_t it = beg; it < end; ++it) {
-    const uint32_t item = items[it];
+  for (int32_t base = beg; base < end; base += NS) {
+    const int32_t it = base + sub;
+    const uint32_t item = items[it < end ? it : base];
     const BatchSolve& s = solves[item >> kItemShift];
-    if (!s.live) continue;  // stopped at an earlier check of this launch sequence
+    // a slice past the run, or a solve stopped at an earlier check of this
+    // launch sequence, computes nothing (but joins the slice sums)
+    const bool act = it < end && s.live;
     const int32_t lb = static_cast<int32_t>(item & kItemMask);
     const int p = (parity ^ s.parity) & 1;
     const RowWin ad{s.coord, s.win, s.NP, s.PY, s.W};
-    const int32_t i = lb * kBlock + static_cast<int32_t>(threadIdx.x);
+    const int32_t i = lb * kBlock + tid;
     double jx = 0.0;
-    if (i < s.n) {
+    if (act && i < s.n) {
       const float F[3] = {static_cast<float>(s.a.F[0]), static_cast<float>(s.a.F[1]), static_cast<float>(s.a.F[2])};
       jx = bgk_node_lp<L, S, RowWin, false, false, REDUCE>(
           static_cast<const S*>(s.f[p]), static_cast<S*>(s.f[p ^ 1]), ad, i, s.NP, static_cast<float>(s.a.tau_inv),
@@ -318,8 +346,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_bgk_lp(const BatchSolve* __res
           gstep - s.step_base);
     }
     if (REDUCE) {
-      const double t = block_sum(jx);
-      if (threadIdx.x == 0) s.partial[lb] = t;
+      const double t = slice_sum<NS>(jx);
+      if (act && tid == 0) s.partial[lb] = accum ? s.partial[lb] + t : t;
     }
   }
 }
@@ -577,7 +605,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_lp(const S* __restrict__ gin, S* 
                                                   double* __restrict__ rho_o, double* __restrict__ vel_o,
                                                   uint32_t step) {
   constexpr int Q = L::nQ;
-  const int32_t b = node_block(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
+  const int32_t b = node_block_rev(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
   const int32_t i = b * kBlock + threadIdx.x;
   if (i >= n) return;
   const uint32_t fs = static_cast<uint32_t>(Q * NP);
@@ -594,24 +622,26 @@ __global__ __launch_bounds__(kBlock) void k_cg_lp(const S* __restrict__ gin, S* 
 // Batched two-phase passes (items as k_batch_bgk_lp; fixed force; every solid
 // neighbour carries the solve's wall colour phi_w).  Pass 1 also writes the
 // three flux partials per block on the interval's last step.
-template <class L, typename S, bool REDUCE>
-__global__ __launch_bounds__(kBlock) void k_batch_cg_phi_lp(const BatchSolve* __restrict__ solves,
-                                                            const uint32_t* __restrict__ items, int32_t n_items,
-                                                            int32_t per_wg, int32_t parity) {
+template <class L, typename S, bool REDUCE, int NS>
+__global__ __launch_bounds__(kBlock * NS) void k_batch_cg_phi_lp(const BatchSolve* __restrict__ solves,
+                                                                 const uint32_t* __restrict__ items,
+                                                                 int32_t n_items, int32_t per_wg, int32_t parity) {
   constexpr int Q = L::nQ;
   const int32_t g = node_block(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
+  const int32_t sub = static_cast<int32_t>(threadIdx.x) / kBlock, tid = static_cast<int32_t>(threadIdx.x) % kBlock;
   const int32_t beg = g * per_wg;
   const int32_t end = beg + per_wg < n_items ? beg + per_wg : n_items;
-  for (int32_t it = beg; it < end; ++it) {
-    const uint32_t item = items[it];
+  for (int32_t base = beg; base < end; base += NS) {
+    const int32_t it = base + sub;
+    const uint32_t item = items[it < end ? it : base];
     const BatchSolve& s = solves[item >> kItemShift];
-    if (!s.live) continue;  // stopped at an earlier check of this launch sequence
+    const bool act = it < end && s.live;
     const int32_t lb = static_cast<int32_t>(item & kItemMask);
     const int p = (parity ^ s.parity) & 1;
     const RowWin ad{s.coord, s.win, s.NP, s.PY, s.W};
-    const int32_t i = lb * kBlock + static_cast<int32_t>(threadIdx.x);
+    const int32_t i = lb * kBlock + tid;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    if (i < s.n) {
+    if (act && i < s.n) {
       float t[Q], d[Q];
       load_td<L, S, false>(static_cast<const S*>(s.f[p]), Q * s.NP, ad, i, t, d);
       static_cast<float*>(s.phi)[i] = phi_td<L>(t, d);
@@ -619,10 +649,10 @@ __global__ __launch_bounds__(kBlock) void k_batch_cg_phi_lp(const BatchSolve* __
     }
     if (REDUCE) {
       const int64_t nb = (s.n + kBlock - 1) / kBlock;
-      s0 = block_sum(s0);
-      s1 = block_sum(s1);
-      s2 = block_sum(s2);
-      if (threadIdx.x == 0) {
+      s0 = slice_sum<NS>(s0);
+      s1 = slice_sum<NS>(s1);
+      s2 = slice_sum<NS>(s2);
+      if (act && tid == 0) {
         s.partial[lb] = s0;
         s.partial[nb + lb] = s1;
         s.partial[2 * nb + lb] = s2;
@@ -631,24 +661,27 @@ __global__ __launch_bounds__(kBlock) void k_batch_cg_phi_lp(const BatchSolve* __
   }
 }
 
-template <class L, typename S>
-__global__ __launch_bounds__(kBlock) void k_batch_cg_lp(const BatchSolve* __restrict__ solves,
-                                                        const uint32_t* __restrict__ items, int32_t n_items,
-                                                        int32_t per_wg, int32_t parity,
-                                                        const uint32_t* __restrict__ step0, int32_t koff) {
+template <class L, typename S, int NS>
+__global__ __launch_bounds__(kBlock * NS) void k_batch_cg_lp(const BatchSolve* __restrict__ solves,
+                                                             const uint32_t* __restrict__ items, int32_t n_items,
+                                                             int32_t per_wg, int32_t parity,
+                                                             const uint32_t* __restrict__ step0, int32_t koff) {
   constexpr int Q = L::nQ;
   const uint32_t gstep = *step0 + static_cast<uint32_t>(koff);  // global step of this launch
-  const int32_t g = node_block(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
+  const int32_t g = node_block_rev(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
+  const int32_t sub = static_cast<int32_t>(threadIdx.x) / kBlock, tid = static_cast<int32_t>(threadIdx.x) % kBlock;
   const int32_t beg = g * per_wg;
   const int32_t end = beg + per_wg < n_items ? beg + per_wg : n_items;
-  for (int32_t it = beg; it < end; ++it) {
+  for (int32_t base = beg; base < end; base += NS) {
+    const int32_t it = base + sub;
+    if (it >= end) continue;
     const uint32_t item = items[it];
     const BatchSolve& s = solves[item >> kItemShift];
     if (!s.live) continue;  // stopped at an earlier check of this launch sequence
     const int32_t lb = static_cast<int32_t>(item & kItemMask);
     const int p = (parity ^ s.parity) & 1;
     const RowWin ad{s.coord, s.win, s.NP, s.PY, s.W};
-    const int32_t i = lb * kBlock + static_cast<int32_t>(threadIdx.x);
+    const int32_t i = lb * kBlock + tid;
     if (i >= s.n) continue;
     const uint32_t fs = Q * s.NP;
     float t[Q], dd[Q];
@@ -795,13 +828,13 @@ hipError_t gather_lowp(int precision, int lattice, int nf, const void* gin, void
 
 hipError_t bgk_lowp(int precision, int lattice, bool first, bool store, bool reduce,
                     const void* gin, void* gout, const Addr& ad, int32_t n, int64_t NP,
-                    const BgkArgs& a, double* rho, double* vel, double* partial, hipStream_t s) {
+                    const BgkArgs& a, double* rho, double* vel, double* partial, hipStream_t s, bool accum) {
   if (n <= 0) return hipSuccess;
-  if (reduce && !store) return hipErrorInvalidValue;
+  const int32_t ac = accum ? 1 : 0;
   const float ti = static_cast<float>(a.tau_inv), tf = static_cast<float>(a.tau_factor);
   const float F0 = static_cast<float>(a.F[0]), F1 = static_cast<float>(a.F[1]), F2 = static_cast<float>(a.F[2]);
   const float tai = static_cast<float>(a.tauA_inv), taf = static_cast<float>(a.tauA_factor);  // TRT only
-  // (first, store, reduce); reduce comes with store
+  // (first, store, reduce)
   const int key = (first ? 4 : 0) | (store ? 2 : 0) | (reduce ? 1 : 0);
   return dispatch(precision, lattice, ad, [&](auto l, auto st, auto addr) {
     using L = decltype(l);
@@ -812,13 +845,15 @@ hipError_t bgk_lowp(int precision, int lattice, bool first, bool store, bool red
     const dim3 g = grid_for(n), b(kBlock);
 #define LB_K(F_, S_, R_, T_)                                                                             \
   k_bgk_lp<L, S, A, F_, S_, R_, T_><<<g, b, 0, s>>>(gi, go, addr, n, NP, ti, tf, F0, F1, F2, rho, vel, partial, \
-                                                    tai, taf, a.tab_srf, a.tab_nuf, a.tab_n, a.step)
+                                                    tai, taf, a.tab_srf, a.tab_nuf, a.tab_n, a.step, ac)
 #define LB_KEYS(T_)                                 \
   switch (key) {                                    \
     case 0: LB_K(false, false, false, T_); break;   \
+    case 1: LB_K(false, false, true, T_); break;    \
     case 2: LB_K(false, true, false, T_); break;    \
     case 3: LB_K(false, true, true, T_); break;     \
     case 4: LB_K(true, false, false, T_); break;    \
+    case 5: LB_K(true, false, true, T_); break;     \
     case 6: LB_K(true, true, false, T_); break;     \
     default: LB_K(true, true, true, T_); break;     \
   }
@@ -860,26 +895,39 @@ hipError_t flux_lowp(int precision, int lattice, int nf, bool first, const void*
 
 hipError_t batch_bgk_lowp(int precision, int lattice, bool reduce, const BatchSolve* solves, const uint32_t* items,
                           int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, const uint32_t* step0,
-                          int32_t koff, hipStream_t s) {
+                          int32_t koff, int ns, bool accum, hipStream_t s) {
   return dispatch(precision, lattice, Addr{}, [&](auto l, auto st, auto) {
     using L = decltype(l);
     using S = decltype(st);
     const dim3 g(grid), b(kBlock);
-    if (reduce) k_batch_bgk_lp<L, S, true><<<g, b, 0, s>>>(solves, items, n_items, per_wg, parity, step0, koff);
-    else k_batch_bgk_lp<L, S, false><<<g, b, 0, s>>>(solves, items, n_items, per_wg, parity, step0, koff);
+    if (ns == 2) {
+      const dim3 b2(2 * kBlock);
+      if (reduce) k_batch_bgk_lp<L, S, true, 2><<<g, b2, 0, s>>>(solves, items, n_items, per_wg, parity, step0, koff, accum ? 1 : 0);
+      else k_batch_bgk_lp<L, S, false, 2><<<g, b2, 0, s>>>(solves, items, n_items, per_wg, parity, step0, koff, accum ? 1 : 0);
+    } else {
+      if (reduce) k_batch_bgk_lp<L, S, true, 1><<<g, b, 0, s>>>(solves, items, n_items, per_wg, parity, step0, koff, accum ? 1 : 0);
+      else k_batch_bgk_lp<L, S, false, 1><<<g, b, 0, s>>>(solves, items, n_items, per_wg, parity, step0, koff, accum ? 1 : 0);
+    }
   });
 }
 
 hipError_t batch_cg_lowp(int precision, int lattice, bool reduce, const BatchSolve* solves, const uint32_t* items,
                          int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, const uint32_t* step0,
-                         int32_t koff, hipStream_t s) {
+                         int32_t koff, int ns, hipStream_t s) {
   return dispatch(precision, lattice, Addr{}, [&](auto l, auto st, auto) {
     using L = decltype(l);
     using S = decltype(st);
     const dim3 g(grid), b(kBlock);
-    if (reduce) k_batch_cg_phi_lp<L, S, true><<<g, b, 0, s>>>(solves, items, n_items, per_wg, parity);
-    else k_batch_cg_phi_lp<L, S, false><<<g, b, 0, s>>>(solves, items, n_items, per_wg, parity);
-    k_batch_cg_lp<L, S><<<g, b, 0, s>>>(solves, items, n_items, per_wg, parity, step0, koff);
+    if (ns == 2) {
+      const dim3 b2(2 * kBlock);
+      if (reduce) k_batch_cg_phi_lp<L, S, true, 2><<<g, b2, 0, s>>>(solves, items, n_items, per_wg, parity);
+      else k_batch_cg_phi_lp<L, S, false, 2><<<g, b2, 0, s>>>(solves, items, n_items, per_wg, parity);
+      k_batch_cg_lp<L, S, 2><<<g, b2, 0, s>>>(solves, items, n_items, per_wg, parity, step0, koff);
+    } else {
+      if (reduce) k_batch_cg_phi_lp<L, S, true, 1><<<g, b, 0, s>>>(solves, items, n_items, per_wg, parity);
+      else k_batch_cg_phi_lp<L, S, false, 1><<<g, b, 0, s>>>(solves, items, n_items, per_wg, parity);
+      k_batch_cg_lp<L, S, 1><<<g, b, 0, s>>>(solves, items, n_items, per_wg, parity, step0, koff);
+    }
   });
 }
 

@@ -31,7 +31,7 @@ hipError_t gather_lowp(int precision, int lattice, int nf, const void* gin, void
                        const Addr& ad, int32_t n, int64_t NP, hipStream_t s);
 hipError_t bgk_lowp(int precision, int lattice, bool first, bool store, bool reduce,
                     const void* gin, void* gout, const Addr& ad, int32_t n, int64_t NP,
-                    const BgkArgs& a, double* rho, double* vel, double* partial, hipStream_t s);
+                    const BgkArgs& a, double* rho, double* vel, double* partial, hipStream_t s, bool accum = false);
 hipError_t flux_lowp(int precision, int lattice, int nf, bool first, const void* gin,
                      const Addr& ad, int32_t n, int64_t NP, int dir, double* partial,
                      hipStream_t s);
@@ -41,11 +41,13 @@ hipError_t flux_lowp(int precision, int lattice, int nf, bool first, const void*
 // stochastic rounding).
 hipError_t batch_bgk_lowp(int precision, int lattice, bool reduce, const BatchSolve* solves, const uint32_t* items,
                           int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, const uint32_t* step0,
-                          int32_t koff, hipStream_t s);
-// Both passes of one batched two-phase step (see k_batch_cg_phi_lp).
+                          int32_t koff, int ns, bool accum, hipStream_t s);
+// Both passes of one batched two-phase step (see k_batch_cg_phi_lp).  ns:
+// 256-node items per workgroup at a time (1 or 2; workgroups of 256 ns
+// threads), as for the single-phase step.
 hipError_t batch_cg_lowp(int precision, int lattice, bool reduce, const BatchSolve* solves, const uint32_t* items,
                          int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, const uint32_t* step0,
-                         int32_t koff, hipStream_t s);
+                         int32_t koff, int ns, hipStream_t s);
 // Two-phase populations are stored as (t, d) = (f0 + f1 - w, f0 - f1).
 // rowwin: the row windows of ad with one wall colour, else the link table.
 hipError_t cg_phi_lowp(int precision, int lattice, bool first, bool momx, const void* gin, const Addr& ad,

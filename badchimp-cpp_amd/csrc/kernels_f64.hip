@@ -61,6 +61,26 @@ __device__ __forceinline__ double block_sum(double v) {
   return r;
 }
 
+// block_sum for each 256-thread slice of a workgroup of NS slices: the same
+// shuffles and the same order over the slice's four waves.
+template <int NS>
+__device__ __forceinline__ double slice_sum(double v) {
+  __shared__ double red[NS * (kBlock / 64)];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double r = 0.0;
+  if ((threadIdx.x % kBlock) == 0) {
+    const int w0 = (threadIdx.x / kBlock) * (kBlock / 64);
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) r += red[w0 + w];
+  }
+  __syncthreads();
+  return r;
+}
+
 // ------------------------------------------------------------------ BGK
 // GeneralizedNewtonian::omegaBGK's viscosity (LBrheology.h:436-462, source
 // 0): strain-rate tensor from the non-equilibrium moments and the Guo
@@ -192,19 +212,21 @@ __device__ __forceinline__ double bgk_node(const double* __restrict__ fin, doubl
   return jx;
 }
 
+// REDUCE: the block's sum of j_x goes to partial[b], or is added to it
+// (accum), so that a check interval's partials hold the sum over its steps.
 template <class L, class A, bool FIRST, bool STORE, bool REDUCE, int MODE>
 __global__ __launch_bounds__(kBlock) void k_bgk(const double* __restrict__ fin,
                                                 double* __restrict__ fout, A ad, int32_t n,
                                                 int64_t NP, BgkArgs a, double* __restrict__ rho_o,
                                                 double* __restrict__ vel_o,
-                                                double* __restrict__ partial) {
+                                                double* __restrict__ partial, int32_t accum) {
   const int32_t b = node_block(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
   const int32_t i = b * kBlock + threadIdx.x;
   double jx = 0.0;
   if (i < n) jx = bgk_node<L, A, FIRST, STORE, REDUCE, MODE>(fin, fout, ad, i, NP, a, rho_o, vel_o);
   if (REDUCE) {
     const double s = block_sum(jx);
-    if (threadIdx.x == 0) partial[b] = s;
+    if (threadIdx.x == 0) partial[b] = accum ? partial[b] + s : s;
   }
 }
 
@@ -214,29 +236,35 @@ __global__ __launch_bounds__(kBlock) void k_bgk(const double* __restrict__ fin,
 // workgroups sweep the items as one compact front.  A solve's buffers, address table and parameters come from its
 // descriptor (uniform per item), and the per-node arithmetic is the single
 // solve's (bgk_node), so a solve gets the same bits alone or in any mix.
-template <class L, bool REDUCE>
-__global__ __launch_bounds__(kBlock) void k_batch_bgk(const BatchSolve* __restrict__ solves,
-                                                      const uint32_t* __restrict__ items, int32_t n_items,
-                                                      int32_t per_wg, int32_t parity) {
+// NS: items a workgroup works on at a time, one per 256-thread slice; every
+// slice reduces its item exactly as a 256-thread workgroup would.
+template <class L, bool REDUCE, int NS>
+__global__ __launch_bounds__(kBlock * NS) void k_batch_bgk(const BatchSolve* __restrict__ solves,
+                                                           const uint32_t* __restrict__ items, int32_t n_items,
+                                                           int32_t per_wg, int32_t parity, int32_t accum) {
   const int32_t g = node_block(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
+  const int32_t sub = static_cast<int32_t>(threadIdx.x) / kBlock, tid = static_cast<int32_t>(threadIdx.x) % kBlock;
   const int32_t beg = g * per_wg;
   const int32_t end = beg + per_wg < n_items ? beg + per_wg : n_items;
-  for (int32_t it = beg; it < end; ++it) {
-    const uint32_t item = items[it];
+  for (int32_t base = beg; base < end; base += NS) {
+    const int32_t it = base + sub;
+    const uint32_t item = items[it < end ? it : base];
     const BatchSolve& s = solves[item >> kItemShift];
-    if (!s.live) continue;  // stopped at an earlier check of this launch sequence
+    // a slice past the run, or a solve stopped at an earlier check of this
+    // launch sequence, computes nothing (but joins the slice sums)
+    const bool act = it < end && s.live;
     const int32_t lb = static_cast<int32_t>(item & kItemMask);
     const int p = (parity ^ s.parity) & 1;
     const RowWin ad{s.coord, s.win, s.NP, s.PY, s.W};
-    const int32_t i = lb * kBlock + static_cast<int32_t>(threadIdx.x);
+    const int32_t i = lb * kBlock + tid;
     double jx = 0.0;
-    if (i < s.n)
+    if (act && i < s.n)
       jx = bgk_node<L, RowWin, false, false, REDUCE>(static_cast<const double*>(s.f[p]),
                                                      static_cast<double*>(s.f[p ^ 1]), ad, i, s.NP, s.a,
                                                      nullptr, nullptr);
     if (REDUCE) {
-      const double t = block_sum(jx);
-      if (threadIdx.x == 0) s.partial[lb] = t;
+      const double t = slice_sum<NS>(jx);
+      if (act && tid == 0) s.partial[lb] = accum ? s.partial[lb] + t : t;
     }
   }
 }
@@ -276,8 +304,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_check(BatchSolve* __restrict__
   const double* o = out + 3 * slot;
   bool conv;
   if (c.kind == 0) {
-    // k = nu <j_x> / F_x with j_x summed over the state the last step started from
-    c.k[0] = c.nu[0] * (o[0] / c.n_box) / c.F0;
+    // k = nu <j_x> / F_x with j_x averaged over the states the interval's
+    // steps started from
+    c.k[0] = c.nu[0] * ((o[0] / static_cast<double>(M)) / c.n_box) / c.F0;
     conv = c.have_prev && fabs(c.k[0] - c.kp[0]) <= c.tol * fabs(c.k[0]);
   } else {
     // phase permeabilities k_i = nu_i <phi_i sum_q f c_x> / F_x (LBglobalforcing.h:61-78)
@@ -544,7 +573,7 @@ __global__ __launch_bounds__(kBlock) void k_cg(const double* __restrict__ fin,
                                                double* __restrict__ rho_o,
                                                double* __restrict__ vel_o) {
   constexpr int D = L::nD;
-  const int32_t b = node_block(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
+  const int32_t b = node_block_rev(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
   const int32_t i = b * kBlock + threadIdx.x;
   if (i >= n) return;
   double F[D];
@@ -558,24 +587,26 @@ __global__ __launch_bounds__(kBlock) void k_cg(const double* __restrict__ fin,
 // every solid neighbour carries the solve's wall colour phi_w).  Pass 1
 // writes the colour field and, on the interval's last step, the three flux
 // partials per block with k_flux's arithmetic; pass 2 is cg_node.
-template <class L, bool REDUCE>
-__global__ __launch_bounds__(kBlock) void k_batch_cg_phi(const BatchSolve* __restrict__ solves,
-                                                         const uint32_t* __restrict__ items, int32_t n_items,
-                                                         int32_t per_wg, int32_t parity) {
+template <class L, bool REDUCE, int NS>
+__global__ __launch_bounds__(kBlock * NS) void k_batch_cg_phi(const BatchSolve* __restrict__ solves,
+                                                              const uint32_t* __restrict__ items, int32_t n_items,
+                                                              int32_t per_wg, int32_t parity) {
   constexpr int Q = L::nQ;
   const int32_t g = node_block(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
+  const int32_t sub = static_cast<int32_t>(threadIdx.x) / kBlock, tid = static_cast<int32_t>(threadIdx.x) % kBlock;
   const int32_t beg = g * per_wg;
   const int32_t end = beg + per_wg < n_items ? beg + per_wg : n_items;
-  for (int32_t it = beg; it < end; ++it) {
-    const uint32_t item = items[it];
+  for (int32_t base = beg; base < end; base += NS) {
+    const int32_t it = base + sub;
+    const uint32_t item = items[it < end ? it : base];
     const BatchSolve& s = solves[item >> kItemShift];
-    if (!s.live) continue;  // stopped at an earlier check of this launch sequence
+    const bool act = it < end && s.live;
     const int32_t lb = static_cast<int32_t>(item & kItemMask);
     const int p = (parity ^ s.parity) & 1;
     const RowWin ad{s.coord, s.win, s.NP, s.PY, s.W};
-    const int32_t i = lb * kBlock + static_cast<int32_t>(threadIdx.x);
+    const int32_t i = lb * kBlock + tid;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    if (i < s.n) {
+    if (act && i < s.n) {
       double f[2][Q];
       load_node<L, 2, false>(static_cast<const double*>(s.f[p]), ad, i, s.NP, f);
       const double r0 = qsum<L>(f[0]);
@@ -594,10 +625,10 @@ __global__ __launch_bounds__(kBlock) void k_batch_cg_phi(const BatchSolve* __res
     }
     if (REDUCE) {
       const int64_t nb = (s.n + kBlock - 1) / kBlock;
-      s0 = block_sum(s0);
-      s1 = block_sum(s1);
-      s2 = block_sum(s2);
-      if (threadIdx.x == 0) {
+      s0 = slice_sum<NS>(s0);
+      s1 = slice_sum<NS>(s1);
+      s2 = slice_sum<NS>(s2);
+      if (act && tid == 0) {
         s.partial[lb] = s0;
         s.partial[nb + lb] = s1;
         s.partial[2 * nb + lb] = s2;
@@ -606,22 +637,25 @@ __global__ __launch_bounds__(kBlock) void k_batch_cg_phi(const BatchSolve* __res
   }
 }
 
-template <class L>
-__global__ __launch_bounds__(kBlock) void k_batch_cg(const BatchSolve* __restrict__ solves,
-                                                     const uint32_t* __restrict__ items, int32_t n_items,
-                                                     int32_t per_wg, int32_t parity) {
+template <class L, int NS>
+__global__ __launch_bounds__(kBlock * NS) void k_batch_cg(const BatchSolve* __restrict__ solves,
+                                                          const uint32_t* __restrict__ items, int32_t n_items,
+                                                          int32_t per_wg, int32_t parity) {
   constexpr int D = L::nD;
-  const int32_t g = node_block(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
+  const int32_t g = node_block_rev(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
+  const int32_t sub = static_cast<int32_t>(threadIdx.x) / kBlock, tid = static_cast<int32_t>(threadIdx.x) % kBlock;
   const int32_t beg = g * per_wg;
   const int32_t end = beg + per_wg < n_items ? beg + per_wg : n_items;
-  for (int32_t it = beg; it < end; ++it) {
+  for (int32_t base = beg; base < end; base += NS) {
+    const int32_t it = base + sub;
+    if (it >= end) continue;
     const uint32_t item = items[it];
     const BatchSolve& s = solves[item >> kItemShift];
     if (!s.live) continue;  // stopped at an earlier check of this launch sequence
     const int32_t lb = static_cast<int32_t>(item & kItemMask);
     const int p = (parity ^ s.parity) & 1;
     const RowWin ad{s.coord, s.win, s.NP, s.PY, s.W};
-    const int32_t i = lb * kBlock + static_cast<int32_t>(threadIdx.x);
+    const int32_t i = lb * kBlock + tid;
     if (i >= s.n) continue;
     CgArgs a{};
     a.nu0Inv = s.nu0Inv;
@@ -768,19 +802,21 @@ inline dim3 grid_for(int32_t n) { return dim3(static_cast<unsigned>((n + kBlock 
 template <class L, class A>
 static void launch_bgk(bool first, bool store, bool reduce, dim3 g, hipStream_t s, const double* fin,
                        double* fout, const A& ad, int32_t n, int64_t NP, const BgkArgs& a, double* rho,
-                       double* vel, double* partial) {
+                       double* vel, double* partial, bool accum) {
   const dim3 b(kBlock);
-  // (first, store, reduce): reduce comes with store (the last step of a
-  // check interval), so (x, false, true) is not instantiated
+  // (first, store, reduce)
   const int key = (first ? 4 : 0) | (store ? 2 : 0) | (reduce ? 1 : 0);
+  const int32_t ac = accum ? 1 : 0;
 #define LB_K(F_, S_, R_, T_) \
-  k_bgk<L, A, F_, S_, R_, T_><<<g, b, 0, s>>>(fin, fout, ad, n, NP, a, rho, vel, partial)
+  k_bgk<L, A, F_, S_, R_, T_><<<g, b, 0, s>>>(fin, fout, ad, n, NP, a, rho, vel, partial, ac)
 #define LB_KEYS(T_)                            \
   switch (key) {                               \
     case 0: LB_K(false, false, false, T_); break; \
+    case 1: LB_K(false, false, true, T_); break;  \
     case 2: LB_K(false, true, false, T_); break;  \
     case 3: LB_K(false, true, true, T_); break;   \
     case 4: LB_K(true, false, false, T_); break;  \
+    case 5: LB_K(true, false, true, T_); break;   \
     case 6: LB_K(true, true, false, T_); break;   \
     default: LB_K(true, true, true, T_); break;   \
   }
@@ -797,14 +833,15 @@ static void launch_bgk(bool first, bool store, bool reduce, dim3 g, hipStream_t 
 
 hipError_t bgk_f64(int lattice, bool first, bool store, bool reduce, const double* fin,
                    double* fout, const Addr& ad, int32_t n, int64_t NP, const BgkArgs& a,
-                   double* rho, double* vel, double* partial, hipStream_t s) {
+                   double* rho, double* vel, double* partial, hipStream_t s, bool accum) {
   if (n <= 0) return hipSuccess;
-  if (reduce && !store) return hipErrorInvalidValue;
   const dim3 g = grid_for(n);
   LB_DISPATCH_LATTICE(lattice, L, {
-    if (ad.layout == 2) launch_bgk<L>(first, store, reduce, g, s, fin, fout, ad.win, n, NP, a, rho, vel, partial);
-    else if (ad.layout == 1) launch_bgk<L>(first, store, reduce, g, s, fin, fout, ad.rows, n, NP, a, rho, vel, partial);
-    else launch_bgk<L>(first, store, reduce, g, s, fin, fout, ad.table, n, NP, a, rho, vel, partial);
+    if (ad.layout == 2)
+      launch_bgk<L>(first, store, reduce, g, s, fin, fout, ad.win, n, NP, a, rho, vel, partial, accum);
+    else if (ad.layout == 1)
+      launch_bgk<L>(first, store, reduce, g, s, fin, fout, ad.rows, n, NP, a, rho, vel, partial, accum);
+    else launch_bgk<L>(first, store, reduce, g, s, fin, fout, ad.table, n, NP, a, rho, vel, partial, accum);
   });
   return hipGetLastError();
 }
@@ -921,21 +958,33 @@ hipError_t prestream_f64(int lattice, const RowWin& ad, const double* r0, int32_
 }
 
 hipError_t batch_cg_f64(int lattice, bool reduce, const BatchSolve* solves, const uint32_t* items,
-                        int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, hipStream_t s) {
+                        int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, int ns, hipStream_t s) {
   LB_DISPATCH_LATTICE(lattice, L, {
-    const dim3 g(grid), b(kBlock);
-    if (reduce) k_batch_cg_phi<L, true><<<g, b, 0, s>>>(solves, items, n_items, per_wg, parity);
-    else k_batch_cg_phi<L, false><<<g, b, 0, s>>>(solves, items, n_items, per_wg, parity);
-    k_batch_cg<L><<<g, b, 0, s>>>(solves, items, n_items, per_wg, parity);
+    const dim3 g(grid), b(kBlock), b2(2 * kBlock);
+    if (ns == 2) {
+      if (reduce) k_batch_cg_phi<L, true, 2><<<g, b2, 0, s>>>(solves, items, n_items, per_wg, parity);
+      else k_batch_cg_phi<L, false, 2><<<g, b2, 0, s>>>(solves, items, n_items, per_wg, parity);
+      k_batch_cg<L, 2><<<g, b2, 0, s>>>(solves, items, n_items, per_wg, parity);
+    } else {
+      if (reduce) k_batch_cg_phi<L, true, 1><<<g, b, 0, s>>>(solves, items, n_items, per_wg, parity);
+      else k_batch_cg_phi<L, false, 1><<<g, b, 0, s>>>(solves, items, n_items, per_wg, parity);
+      k_batch_cg<L, 1><<<g, b, 0, s>>>(solves, items, n_items, per_wg, parity);
+    }
   });
   return hipGetLastError();
 }
 
 hipError_t batch_bgk_f64(int lattice, bool reduce, const BatchSolve* solves, const uint32_t* items,
-                         int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, hipStream_t s) {
+                         int32_t n_items, int32_t grid, int32_t per_wg, int32_t parity, int ns, bool accum, hipStream_t s) {
   LB_DISPATCH_LATTICE(lattice, L, {
-    if (reduce) k_batch_bgk<L, true><<<dim3(grid), dim3(kBlock), 0, s>>>(solves, items, n_items, per_wg, parity);
-    else k_batch_bgk<L, false><<<dim3(grid), dim3(kBlock), 0, s>>>(solves, items, n_items, per_wg, parity);
+    const dim3 b(kBlock * (ns == 2 ? 2 : 1));
+    if (ns == 2) {
+      if (reduce) k_batch_bgk<L, true, 2><<<dim3(grid), b, 0, s>>>(solves, items, n_items, per_wg, parity, accum ? 1 : 0);
+      else k_batch_bgk<L, false, 2><<<dim3(grid), b, 0, s>>>(solves, items, n_items, per_wg, parity, accum ? 1 : 0);
+    } else {
+      if (reduce) k_batch_bgk<L, true, 1><<<dim3(grid), b, 0, s>>>(solves, items, n_items, per_wg, parity, accum ? 1 : 0);
+      else k_batch_bgk<L, false, 1><<<dim3(grid), b, 0, s>>>(solves, items, n_items, per_wg, parity, accum ? 1 : 0);
+    }
   });
   return hipGetLastError();
 }

@@ -142,6 +142,7 @@ SIGNATURES = {
     "lb_batch_set_precision": (C.c_int, [_vp, C.c_int32, C.c_int32]),
     "lb_batch_set_shape": (C.c_int, [_vp, C.c_int32]),
     "lb_batch_shape": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
+    "lb_batch_tuning": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
     "lb_batch_kernel_time": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
 }
 
@@ -735,6 +736,13 @@ class Batch:
         v = C.c_int32()
         self._check(lib().lb_batch_shape(self._h, C.byref(v)))
         return v.value
+
+    def tuning(self):
+        """The launch parameters the tuner chose: workgroups per CU, threads
+        per workgroup, and the resident nodes they were tuned at."""
+        a, t, n = C.c_int32(), C.c_int32(), C.c_int64()
+        self._check(lib().lb_batch_tuning(self._h, C.byref(a), C.byref(t), C.byref(n)))
+        return {"wg_per_cu": a.value, "wg_threads": t.value, "tuned_nodes": n.value}
 
     def kernel_time(self):
         ms, n, l = C.c_double(), C.c_int64(), C.c_int64()

@@ -5,22 +5,28 @@ single-phase BGK, tau = 1, body force g_x = 1e-6, 256^3 box periodic in x
 with solid walls on the y and z faces, filled with a seeded sphere pack of
 radius 8 at porosity 0.40, solids as half-way bounce-back.
 
-value    = MLUPS of the FP32-shifted form (shifted populations in float,
-           float arithmetic, FP64 flux reductions), whole run, inputs resident.
-fp64     = the same for the plain FP64 form (the reference's arithmetic).
-roofline = the dominant kernel (the collide-stream step) against 8 TB/s,
+value    = MLUPS of the plain FP64 form (the reference's arithmetic), whole
+           run, inputs resident.
+fp32s, fp16s = the same for the precision forms (shifted populations in
+           float or half storage, float arithmetic, FP64 flux reductions),
+           each with its permeability error against FP64.
+roofline = the dominant kernel (the FP64 collide-stream step) against 8 TB/s,
            with algorithmic bytes per lattice update = 2 Q s
-           (populations read once and written once; s = 4 B FP32S, 8 B FP64).
-permeability = k of both forms run to |dk|/k < 1e-7 (checked every 100
-           steps), and the relative error of FP32S against FP64.
+           (populations read once and written once; s = 8 B FP64, 4 B FP32S,
+           2 B FP16S).
+permeability = k of the three forms run to |dk|/k <= 1e-7 at two consecutive
+           checks 100 steps apart, and the relative errors against FP64.
 cpu_baseline = the compiled reference std_case (oracle/_ref, built from the
            reference's sources with MPICH) on the host's cores, one MPI rank
-           per core (at most 16), on a 128^3 pack of the same generator, from
-           two differenced runs; "port" beside it is the oracle's OpenMP
-           restatement of the same loop on the full 256^3 node list.
+           per core of the process's CPU share (16 on a one-GPU box; nproc
+           and the affinity set are recorded beside it), on a 128^3 pack of
+           the same generator, from two differenced runs; "port" beside it is
+           the oracle's OpenMP restatement of the same loop on the full 256^3
+           node list.
 batched  = the ensemble of independent sphere-pack permeability solves (cubic
            sizes log-uniform in 32..128, porosity U(0.30, 0.50), radius
-           U(3, 8), one seed; |dk|/k < 1e-6 checked every 100 steps), all
+           U(3, 8), one seed; |dk|/k <= 1e-6 at two consecutive checks 100
+           steps apart), all
            resident in one pool and advanced together in FP32S: solves/s over
            the whole ensemble (wall clock from the first admission to the last
            result, host set-up of each solve included), against the plain FP64
@@ -40,15 +46,18 @@ serving  = a stream of requests, 70 % permeability (single-phase) and 30 %
            relative permeability (colour-gradient, saturation U(0.2, 0.8),
            phases drawn per pore cell), cubic sizes U(48, 160), seeded sphere
            packs; admitted by HBM footprint into one pool, advanced together
-           (continuous batching), each resident until it converges.  In FP32S
-           (the top-level figures) and in FP16S ("fp16s"): first a burst (all
-           requests at t = 0) gives the capacity in solves/s; then Poisson
-           arrivals at 0.7 x that capacity give p50/p99 latency.  The same
-           stop rules on the plain FP64 path one request at a time (every
-           relative-permeability request and a size-stratified subset of the
-           permeability requests, scaled) give the baseline solves/s and the
-           k error of both forms, the relative-permeability error split by
-           whether the FP64 run converged or stopped at its step limit.
+           (continuous batching), each resident until it converges (relative
+           change of k, or of both phase permeabilities, at most 1e-6 per 100
+           steps at two consecutive checks) or reaches its step limit.
+           Permeability requests are stored in FP32S (top-level figures) or
+           FP16S ("fp16s"), relative-permeability requests in FP64 (the
+           reduced forms drift from FP64 by more than 1e-4 over the two-phase
+           transient).  FP32S: a burst (all requests at t = 0) gives the
+           capacity in solves/s, then Poisson arrivals at 0.7 x that capacity
+           give p50/p99 latency; FP16S: the burst.  The same stop rules on the
+           plain FP64 path one request at a time (every relative-permeability
+           request and a size-stratified subset of the permeability requests,
+           scaled) give the baseline solves/s and the k errors.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256]
 """
@@ -66,6 +75,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "badchimp-cpp_amd"))
 
 import lbamd  # noqa: E402
+from lbamd.workloads import TWO_PHASE, ensemble, request_mix  # noqa: E402
 
 METRIC = "D3Q19 MLUPS & %HBM roofline; batched perm solves/s + p99 at ≤1e-4 k err"
 HBM_PEAK_GBS = 8000.0
@@ -246,9 +256,25 @@ def trt_leg(args, geo, topo, nb, types, bulk, pos, force, n_bulk, k_bgk):
     return out
 
 
+def host_cores():
+    """Cores this process may use: the affinity set, at most 16 (the host
+    CPU share of one GPU on the boxes this runs on), with what nproc and
+    os.cpu_count() report beside it."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = os.cpu_count() or 1
+    try:
+        nproc = int(subprocess.run(["nproc"], capture_output=True, text=True, timeout=10).stdout.strip())
+    except (OSError, ValueError, subprocess.SubprocessError):
+        nproc = None
+    return min(16, aff), {"nproc": nproc, "os_cpu_count": os.cpu_count(), "affinity": aff,
+                          "used": min(16, aff)}
+
+
 def cpu_baseline(nb, types, bulk, n_nodes, tau, force, budget_s=15.0):
     import oracle
-    threads = min(16, os.cpu_count() or 1)
+    threads, _ = host_cores()
     f0 = rest_state(n_nodes, bulk)
     # one step to size the sample, then as many as fit the budget (at most 50)
     t0 = time.perf_counter()
@@ -271,32 +297,15 @@ def cpu_reference(tau, force, size=128, n_a=20, n_b=520):
     from oracle import ref_run
     if not ref_run.available():
         raise RuntimeError("reference build or MPICH runtime absent")
-    ranks = min(16, os.cpu_count() or 1)
+    ranks, cores = host_cores()
     geo, info = lbamd.sphere_pack((size,) * 3, 8.0, 0.40, 1, periodic="x", walls=True)
     dt, n_bulk, wall = ref_run.time_std_case(geo, "x", tau, force, ranks, n_a, n_b)
     return {"value": n_bulk * (n_b - n_a) / dt / 1e6, "unit": "MLUPS", "cores": ranks, "kind": "reference",
+            "host_cores": cores,
             "sample": "reference std_case (src/std_case/main.cpp, D3Q19, -O3, MPICH) on a %d^3 pack of the same "
                       "generator (%d bulk nodes, porosity %.3f), %d MPI ranks (x slabs), %d steps: runs of %d and "
                       "%d steps differenced (%.1f s, %.1f s)" % (size, n_bulk, info["porosity"], ranks, n_b - n_a,
                                                                n_a, n_b, wall[n_a], wall[n_b])}
-
-
-def ensemble(n, seed=2024):
-    """Parameters and geometries of the batched ensemble."""
-    from concurrent.futures import ThreadPoolExecutor
-    rng = np.random.default_rng(seed)
-    L = np.rint(np.exp(rng.uniform(np.log(32), np.log(128), n))).astype(int)
-    phi = rng.uniform(0.30, 0.50, n)
-    rad = rng.uniform(3.0, 8.0, n)
-    seeds = rng.integers(1, 2**62, n)
-
-    def one(i):
-        geo, info = lbamd.sphere_pack((L[i],) * 3, rad[i], phi[i], int(seeds[i]), periodic="x", walls=True)
-        return geo
-
-    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
-        geos = list(ex.map(one, range(n)))
-    return geos, {"L": L, "porosity": phi, "radius": rad}
 
 
 def batched_leg(n, every, tau, force):
@@ -310,7 +319,7 @@ def batched_leg(n, every, tau, force):
     def run(precision):
         b = lbamd.Batch(lbamd.D3Q19, precision, pool, 100)
         t0 = time.perf_counter()
-        assert b.add_many(geos, "x", tau, force, 1e-6, 200000, list(range(n))) == n
+        assert b.add_many(geos, "x", tau, force, 1e-6, 400000, list(range(n))) == n
         t_adm = time.perf_counter() - t0
         res = {}
         while len(res) < n:
@@ -319,20 +328,20 @@ def batched_leg(n, every, tau, force):
                 res[r["tag"]] = r
         wall = time.perf_counter() - t0
         kms, launches, lups = b.kernel_time()
-        shape = b.shape()
+        shape = b.tuning()
         b.close()
         return res, wall, t_adm, kms, lups, shape
 
     res, wall, t_adm, kms, lups, shape = run(lbamd.FP32S)
-    res16, wall16, _, kms16, lups16, _ = run(lbamd.FP16S)
+    res16, wall16, _, kms16, lups16, shape16 = run(lbamd.FP16S)
     log("batched FP16S: %d solves in %.2f s, %.1f GLUPS device" % (n, wall16, lups16 / (kms16 * 1e-3) / 1e9))
-    log("batched FP32S: %d solves in %.2f s (admission %.2f s), %.1f GLUPS device, shape %d" %
+    log("batched FP32S: %d solves in %.2f s (admission %.2f s), %.1f GLUPS device, tuned %s" %
         (n, wall, t_adm, lups / (kms * 1e-3) / 1e9, shape))
     # plain FP64 one at a time on a size-stratified subset
     order = np.argsort(nfluid, kind="stable")
     subset = order[every // 2::every]
     t_one = 0.0
-    errs, errs16 = [], []
+    errs, errs16, steps64 = [], [], []
     for i in subset:
         g = geos[i]
         topo = lbamd.Topology.from_geometry(g, lbamd.D3Q19, "x")
@@ -344,16 +353,23 @@ def batched_leg(n, every, tau, force):
         ctx.upload_f(f0)
         ctx.sync()
         t1 = time.perf_counter()
-        k64, s64 = ctx.solve_permeability(tau, force, 100, 1e-6, 200000, float(g.size))
+        k64, s64 = ctx.solve_permeability(tau, force, 100, 1e-6, 400000, float(g.size))
         t_one += time.perf_counter() - t1
         ctx.close()
+        steps64.append(s64)
         errs.append(abs(res[int(i)]["k"] - k64) / abs(k64))
         errs16.append(abs(res16[int(i)]["k"] - k64) / abs(k64))
     est_one = t_one * n / len(subset)
     steps = np.array([res[i]["steps"] for i in range(n)])
+    steps16 = np.array([res16[i]["steps"] for i in range(n)])
+    sub = [int(i) for i in subset]
     out = {"n_solves": n, "solves_per_s": n / wall, "wall_s": wall, "admission_s": t_adm,
-           "device_GLUPS": lups / (kms * 1e-3) / 1e9, "lups": int(lups), "launch_shape_wg_per_cu": shape,
+           "device_GLUPS": lups / (kms * 1e-3) / 1e9, "lups": int(lups), "tuned_launch": shape, "fp16s_tuned_launch": shape16,
            "steps_median": int(np.median(steps)), "steps_max": int(steps.max()),
+           "fp16s_steps_median": int(np.median(steps16)), "fp16s_steps_max": int(steps16.max()),
+           "steps_total": {"fp32s": int(steps.sum()), "fp16s": int(steps16.sum())},
+           "steps_on_fp64_subset": {"fp64": int(np.sum(steps64)), "fp32s": int(steps[sub].sum()),
+                                    "fp16s": int(steps16[sub].sum())},
            "fluid_nodes": int(nfluid.sum()), "pool_GB": pool / 1e9,
            "fp64_one_at_a_time_solves_per_s": n / est_one, "fp64_subset": int(len(subset)),
            "speedup_vs_fp64_one_at_a_time": est_one / wall,
@@ -361,7 +377,8 @@ def batched_leg(n, every, tau, force):
            "fp16s_solves_per_s": n / wall16, "fp16s_device_GLUPS": lups16 / (kms16 * 1e-3) / 1e9,
            "fp16s_speedup_vs_fp64_one_at_a_time": est_one / wall16,
            "k_rel_err_fp16s_vs_fp64_max": float(max(errs16)),
-           "converged": int(sum(1 for r in res.values() if r["status"] == 0))}
+           "converged": int(sum(1 for r in res.values() if r["status"] == 0)),
+           "fp16s_converged": int(sum(1 for r in res16.values() if r["status"] == 0))}
     log("FP64 one at a time (subset of %d, scaled): %.2f solves/s; batched %.2f solves/s (x%.2f); "
         "max k err %.2e" % (len(subset), out["fp64_one_at_a_time_solves_per_s"], out["solves_per_s"],
                             out["speedup_vs_fp64_one_at_a_time"], out["k_rel_err_fp32s_vs_fp64_max"]))
@@ -369,35 +386,7 @@ def batched_leg(n, every, tau, force):
 
 
 W19 = np.array([2.0] * 3 + [1.0] * 6 + [2.0] * 3 + [1.0] * 6 + [12.0]) / 36.0
-TP = {"tau0": 1.0, "tau1": 1.0, "sigma": 0.01, "beta": 0.7, "wett": 1.0}
-
-
-def request_mix(n, seed, t_rate=None):
-    """Serving requests: sizes U(48, 160), 30 % two-phase with S_w U(0.2, 0.8)."""
-    from concurrent.futures import ThreadPoolExecutor
-    from lbamd.serve import Request
-    rng = np.random.default_rng(seed)
-    L = rng.integers(48, 161, n)
-    two = rng.random(n) < 0.3
-    phi = rng.uniform(0.30, 0.50, n)
-    rad = rng.uniform(3.0, 8.0, n)
-    sw = rng.uniform(0.2, 0.8, n)
-    seeds = rng.integers(1, 2**62, n)
-    arrivals = np.cumsum(rng.exponential(1.0 / t_rate, n)) if t_rate else np.zeros(n)
-
-    def one(i):
-        geo, _ = lbamd.sphere_pack((L[i],) * 3, rad[i], phi[i], int(seeds[i]), periodic="x", walls=True)
-        if not two[i]:
-            return Request(i, geo, "x", tau=1.0, force=(1e-6, 0.0, 0.0), tol=1e-6, max_steps=200000,
-                           t_arrival=float(arrivals[i]))
-        r = np.random.default_rng(int(seeds[i]) + 1)
-        rho0 = np.where(geo > 0, (r.random(geo.shape) < sw[i]).astype(float), 0.0)
-        return Request(i, geo, "x", force=(1e-6, 0.0, 0.0), tol=1e-4, max_steps=20000, t_arrival=float(arrivals[i]),
-                       kind="relperm", rho0=rho0, wett=TP["wett"], tau0=TP["tau0"], tau1=TP["tau1"],
-                       sigma=TP["sigma"], beta=TP["beta"])
-
-    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
-        return list(ex.map(one, range(n)))
+TP = TWO_PHASE
 
 
 def fp64_one(req):
@@ -431,23 +420,9 @@ def fp64_one(req):
     ctx.upload_f(f0)
     ctx.set_wall_densities(np.stack([r0, r1], 1))
     ctx.sync()
-    nu0, nu1 = (req.tau0 - 0.5) / 3.0, (req.tau1 - 0.5) / 3.0
-    F = np.asarray(req.force, dtype=float)
     t0 = time.perf_counter()
-    prev, steps = None, 0
-    while True:
-        # flux of the state the interval's last step starts from, as the batch reports it
-        ctx.step_cg(99, req.tau0, req.tau1, req.sigma, req.beta, lbamd.FORCE_FIXED, F)
-        _, s1, s2 = ctx.reduce_flux(0)
-        ctx.step_cg(1, req.tau0, req.tau1, req.sigma, req.beta, lbamd.FORCE_FIXED, F)
-        steps += 100
-        k = (nu0 * (s1 / nbox) / F[0], nu1 * (s2 / nbox) / F[0])
-        scale = abs(k[0]) + abs(k[1])
-        if prev is not None and abs(k[0] - prev[0]) <= req.tol * scale and abs(k[1] - prev[1]) <= req.tol * scale:
-            break
-        prev = k
-        if steps >= req.max_steps:
-            break
+    k, steps = ctx.solve_relperm(req.tau0, req.tau1, req.sigma, req.beta, np.asarray(req.force, dtype=float), 100,
+                                 req.tol, req.max_steps, nbox)
     dt = time.perf_counter() - t0
     ctx.close()
     return k, steps, dt
@@ -543,11 +518,24 @@ def twophase_leg(size, steps, seed=3):
     return out
 
 
+SERVE_TOL = 1e-6
+SERVE_MAX_STEPS = 400000
+RELPERM_MAX_STEPS = 20000
+
+
+def serve_requests(n_req, seed, t_rate=None):
+    reqs = request_mix(n_req, seed, t_rate=t_rate, tol=SERVE_TOL, max_steps=SERVE_MAX_STEPS)
+    for r in reqs:
+        if r.kind == "relperm":
+            r.max_steps = RELPERM_MAX_STEPS
+    return reqs
+
+
 def serve_run(pool, precision, n_req, seed, t_rate=None):
     """One serving stream: (results, wall seconds, kernel ms, LUPs)."""
     from lbamd.serve import Server, run_stream
-    srv = Server(pool, precision=precision, check_every=100)
-    done, wall = run_stream(srv, request_mix(n_req, seed, t_rate=t_rate))
+    srv = Server(pool, precision=precision, check_every=100, relperm_precision=lbamd.FP64)
+    done, wall = run_stream(srv, serve_requests(n_req, seed, t_rate=t_rate))
     kms, _, lups = srv.batch.kernel_time()
     srv.close()
     return done, wall, kms, lups
@@ -555,7 +543,7 @@ def serve_run(pool, precision, n_req, seed, t_rate=None):
 
 def serving_leg(n_req, pool_gb, every):
     pool = int(pool_gb * 1e9)
-    base_reqs = {r.tag: r for r in request_mix(n_req, 7)}
+    base_reqs = {r.tag: r for r in serve_requests(n_req, 7)}
     # 3) the plain FP64 path one request at a time: every relative-permeability
     #    request, and a subset of the permeability requests stratified by size
     #    (every m-th by cell count); each kind's time scaled by its own count
@@ -571,6 +559,8 @@ def serving_leg(n_req, pool_gb, every):
             k, steps, dt = fp64_one(base_reqs[t])
             ref[t] = (kind, k, steps)
             t_kind += dt
+            log("  FP64 one at a time: request %d (%s, %d^3) %d steps, %.2f s" % (
+                t, kind, base_reqs[t].geo.shape[0], steps, dt))
         est += t_kind * len(tags) / len(sub)
         n_sub += len(sub)
     fp64_rate = n_req / est
@@ -598,14 +588,19 @@ def serving_leg(n_req, pool_gb, every):
 
     out = {"n_requests": n_req, "mix": "70% permeability / 30% relative permeability, L U(48,160)",
            "pool_GB": pool / 1e9, "fp64_one_at_a_time_solves_per_s": fp64_rate, "fp64_subset": n_sub,
-           "two_phase_stop": "|dk_i| <= 1e-4 (|k_0|+|k_1|) per 100 steps, at most 20000 steps"}
+           "stop": "|dk| <= %g |k| (two-phase: |dk_i| <= %g (|k_0|+|k_1|)) per 100 steps at two consecutive "
+                   "checks; at most %d steps (two-phase %d)" % (SERVE_TOL, SERVE_TOL, SERVE_MAX_STEPS,
+                                                                  RELPERM_MAX_STEPS),
+           "relperm_storage": "FP64"}
     for precision, name in ((lbamd.FP32S, "fp32s"), (lbamd.FP16S, "fp16s")):
         # 1) burst: capacity
         done, wall, kms, lups = serve_run(pool, precision, n_req, 7)
         cap = len(done) / wall
-        # 2) Poisson arrivals at 0.7 x capacity
-        done2, wall2, _, _ = serve_run(pool, precision, n_req, 8, t_rate=0.7 * cap)
-        lat = np.array([d["latency"] for d in done2])
+        log("serving %s burst: %d requests in %.1f s" % (name, len(done), wall))
+        # 2) Poisson arrivals at 0.7 x capacity (FP32S)
+        if precision == lbamd.FP32S:
+            done2, wall2, _, _ = serve_run(pool, precision, n_req, 8, t_rate=0.7 * cap)
+            lat = np.array([d["latency"] for d in done2])
         err = errors(done)
         per_kind = {}
         for kind in ("perm", "relperm"):
@@ -617,14 +612,19 @@ def serving_leg(n_req, pool_gb, every):
         leg = {"burst_solves_per_s": cap, "burst_wall_s": wall,
                "burst_device_GLUPS": lups / (kms * 1e-3) / 1e9 if kms > 0 else None,
                "burst_device_busy_frac": kms * 1e-3 / wall,
-               "poisson_rate_per_s": 0.7 * cap, "poisson_wall_s": wall2, "poisson_solves_per_s": len(done2) / wall2,
-               "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
                "speedup_vs_fp64_one_at_a_time": cap / fp64_rate,
                "steps_median": int(np.median([d["steps"] for d in done])), "per_kind_burst": per_kind}
+        if precision == lbamd.FP32S:
+            leg.update({"poisson_rate_per_s": 0.7 * cap, "poisson_wall_s": wall2,
+                        "poisson_solves_per_s": len(done2) / wall2,
+                        "p50_latency_s": float(np.percentile(lat, 50)),
+                        "p99_latency_s": float(np.percentile(lat, 99))})
         leg.update(err)
-        log("serving %s: burst %.2f solves/s (x%.2f vs FP64 one at a time %.3f/s), Poisson %.2f/s p50 %.2f s "
-            "p99 %.2f s; k err perm %s, relperm converged %s (%d), at max steps %s (%d)" % (
-                name, cap, cap / fp64_rate, fp64_rate, 0.7 * cap, leg["p50_latency_s"], leg["p99_latency_s"],
+        log("serving %s: burst %.2f solves/s (x%.2f vs FP64 one at a time %.3f/s)%s; k err perm %s, relperm "
+            "converged %s (%d), at max steps %s (%d)" % (
+                name, cap, cap / fp64_rate, fp64_rate,
+                ", Poisson %.2f/s p50 %.2f s p99 %.2f s" % (0.7 * cap, leg["p50_latency_s"], leg["p99_latency_s"])
+                if "p50_latency_s" in leg else "",
                 err["k_rel_err_max_perm"], err["k_rel_err_max_relperm_converged"], err["n_relperm_converged"],
                 err["k_rel_err_max_relperm_at_max_steps"], err["n_relperm_at_max_steps"]))
         if precision == lbamd.FP32S:
@@ -645,7 +645,7 @@ def main():
     ap.add_argument("--no-perm", action="store_true")
     ap.add_argument("--batch", type=int, default=512, help="ensemble size (0: skip)")
     ap.add_argument("--baseline-every", type=int, default=16)
-    ap.add_argument("--serve", type=int, default=48, help="requests per serving run (0: skip)")
+    ap.add_argument("--serve", type=int, default=32, help="requests per serving run (0: skip)")
     ap.add_argument("--no-channel", action="store_true")
     ap.add_argument("--no-trt", action="store_true")
     ap.add_argument("--no-setup", action="store_true")
@@ -725,35 +725,41 @@ def main():
             cpu = dict(port, reference_error=str(e)[:300])
             log("CPU reference not timed (%s); the port stands in" % e)
 
+    k_err = {}
+    if perm:
+        k_err = {"fp32s": perm["k_rel_err_fp32s"], "fp16s": perm["k_rel_err_fp16s"]}
     line = {
         "metric": METRIC,
-        "value": round(mlups32, 2),
+        "value": round(mlups64, 2),
         "unit": "MLUPS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": t32 / args.steps * 1e3,
+        "ms_per_step": t64 / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (shifted populations, FP64 reductions)",
+        "dtype": "f64",
         "data": "synthetic (seeded sphere pack)",
         "config": {"workload": "D3Q19 BGK+Guo collide-stream, %d^3 sphere pack r=8 porosity %.3f, "
                                "periodic x, walls y/z, tau=1, F=1e-6" % (args.size, info["porosity"]),
-                   "bulk_nodes": n_bulk, "form": "FP32-shifted", "layout": lay32,
+                   "bulk_nodes": n_bulk, "form": "FP64 (plain)", "layout": lay64,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": traffic32,
-                     "kernel": "k_bgk_lp<D3Q19,float>", "avg_kernel_ms": k32_ms,
-                     "bytes_per_lup": bytes32, "plain_copy_GBs": round(copy_gbs, 1),
+        "roofline": {"bound": "hbm", "achieved": round(ach64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach64 / HBM_PEAK_GBS, 4), "traffic": traffic64,
+                     "kernel": "k_bgk<D3Q19,RowWin>", "avg_kernel_ms": k64_ms,
+                     "bytes_per_lup": bytes64, "plain_copy_GBs": round(copy_gbs, 1),
                      "aosoa_stream_GBs": round(stream_gbs, 1)},
-        "fp16s": {"value": round(mlups16, 2), "unit": "MLUPS", "avg_kernel_ms": k16_ms,
+        "fp32s": {"value": round(mlups32, 2), "unit": "MLUPS", "ms_per_step": t32 / args.steps * 1e3,
+                  "avg_kernel_ms": k32_ms, "kernel": "k_bgk_lp<D3Q19,float,RowWin>",
+                  "achieved_GBs": round(ach32, 1), "frac": round(ach32 / HBM_PEAK_GBS, 4),
+                  "bytes_per_lup": bytes32, "traffic": traffic32, "k_rel_err_vs_fp64": k_err.get("fp32s")},
+        "fp16s": {"value": round(mlups16, 2), "unit": "MLUPS", "ms_per_step": t16 / args.steps * 1e3,
+                  "avg_kernel_ms": k16_ms, "kernel": "k_bgk_lp<D3Q19,half,RowWin>",
                   "achieved_GBs": round(n_bulk * 76 / (k16_ms * 1e-3) / 1e9, 1),
                   "frac": round(n_bulk * 76 / (k16_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "bytes_per_lup": 76,
-                  "traffic": profiled_traffic("FP16S") if args.size == 256 else None},
-        "fp64": {"value": round(mlups64, 2), "unit": "MLUPS", "avg_kernel_ms": k64_ms,
-                 "achieved_GBs": round(ach64, 1), "frac": round(ach64 / HBM_PEAK_GBS, 4),
-                 "bytes_per_lup": bytes64, "traffic": traffic64},
+                  "traffic": profiled_traffic("FP16S") if args.size == 256 else None,
+                  "k_rel_err_vs_fp64": k_err.get("fp16s")},
         "permeability": perm,
         "setup": setup,
         "trt": trt,

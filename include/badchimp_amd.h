@@ -230,12 +230,14 @@ int lb_last_force(lb_ctx* ctx, double* force);
 int lb_reduce_flux(lb_ctx* ctx, int32_t dir, double* sum_fc,
                    double* sum_phi0_fc, double* sum_phi1_fc);
 
-/* Single-phase permeability solve on the device: step BGK until
- * |k_t - k_{t-M}| <= tol |k_t| at two consecutive checks, every
- * M = check_every steps, or max_steps.  k = nu * (sum_n j_x) / (n_box * F_x)
- * with j = sum_q f c + F/2 and nu = (tau - 1/2)/3 (std_one_phase/main.cpp:
- * 603-624 prints the same relative change of its fluxes; it has no stop).
- * Returns k and the steps taken. */
+/* Single-phase permeability solve on the device: step BGK in check
+ * intervals of M = check_every steps until |k_t - k_{t-M}| <= tol |k_t| at two
+ * consecutive checks, or max_steps.  k = nu <j_x> / F_x, where <j_x> is
+ * sum_n j_x / n_box averaged over the states the interval's steps started
+ * from (j = sum_q f c + F/2, nu = (tau - 1/2)/3); the average over an
+ * interval keeps the rounding noise of FP16 storage out of the stop test.
+ * std_one_phase/main.cpp:603-624 prints the same relative change of its
+ * fluxes (it has no stop).  Returns k and the steps taken. */
 int lb_solve_permeability(lb_ctx* ctx, double tau, const double* force,
                           int32_t check_every, double tol, int64_t max_steps,
                           double n_box, double* k_out, int64_t* steps_out);
@@ -441,10 +443,17 @@ int lb_batch_status(const lb_batch* b, int32_t* n_resident, int64_t* resident_no
  * force) on a context with row windows. */
 int lb_batch_set_precision(lb_batch* b, int32_t kind, int32_t precision);
 /* Launch shape: -1 autotune on live steps after each change of the resident
- * set (default), 0 one workgroup per 256-node block, k > 0 a persistent launch
- * of k workgroups per CU. */
+ * set by 20 % or more (default), 0 one workgroup per 256-node block, k > 0 a
+ * persistent launch of k workgroups per CU (a fixed shape uses 256-thread
+ * workgroups).  The tuner times nine candidates, workgroups per CU
+ * {0, 1, 2, 4, 8} with 256-thread workgroups and {0, 1, 2, 4} with 512-thread
+ * workgroups that work on two 256-node blocks at a time, on the live steps
+ * of one interval.  No candidate changes a result. */
 int lb_batch_set_shape(lb_batch* b, int32_t wg_per_cu);
 int lb_batch_shape(const lb_batch* b, int32_t* wg_per_cu);
+/* The tuner's choice: workgroups per CU, threads per workgroup, and the
+ * resident node count it was tuned at (-1: not tuned yet). */
+int lb_batch_tuning(const lb_batch* b, int32_t* wg_per_cu, int32_t* wg_threads, int64_t* tuned_nodes);
 /* Device time of the step launches and sums (HIP events), launches, and
  * lattice updates done. */
 int lb_batch_kernel_time(const lb_batch* b, double* ms, int64_t* launches, int64_t* lups);
