@@ -301,6 +301,8 @@ __global__ __launch_bounds__(kBlock) void k_bgk_lp(const S* __restrict__ gin, S*
                                                    int32_t accum) {
   const int32_t b = node_block(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
   const int32_t i = b * kBlock + threadIdx.x;
+  // the block's running sum, read before the node work (latency overlapped)
+  const double prev = REDUCE && accum && threadIdx.x == 0 ? partial[b] : 0.0;
   double jx = 0.0;
   if (i < n) {
     const float F[3] = {F0, F1, F2};
@@ -309,7 +311,7 @@ __global__ __launch_bounds__(kBlock) void k_bgk_lp(const S* __restrict__ gin, S*
   }
   if (REDUCE) {
     const double s = block_sum(jx);
-    if (threadIdx.x == 0) partial[b] = accum ? partial[b] + s : s;
+    if (threadIdx.x == 0) partial[b] = accum ? prev + s : s;
   }
 }
 
@@ -337,6 +339,9 @@ __global__ __launch_bounds__(kBlock * NS) void k_batch_bgk_lp(const BatchSolve* 
     const int p = (parity ^ s.parity) & 1;
     const RowWin ad{s.coord, s.win, s.NP, s.PY, s.W};
     const int32_t i = lb * kBlock + tid;
+    // the running sum of this block over the interval, read before the node
+    // work so that its latency overlaps the gathers
+    const double prev = REDUCE && accum && tid == 0 && act ? s.partial[lb] : 0.0;
     double jx = 0.0;
     if (act && i < s.n) {
       const float F[3] = {static_cast<float>(s.a.F[0]), static_cast<float>(s.a.F[1]), static_cast<float>(s.a.F[2])};
@@ -347,7 +352,7 @@ __global__ __launch_bounds__(kBlock * NS) void k_batch_bgk_lp(const BatchSolve* 
     }
     if (REDUCE) {
       const double t = slice_sum<NS>(jx);
-      if (act && tid == 0) s.partial[lb] = accum ? s.partial[lb] + t : t;
+      if (act && tid == 0) s.partial[lb] = accum ? prev + t : t;
     }
   }
 }

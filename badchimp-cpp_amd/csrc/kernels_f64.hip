@@ -222,11 +222,13 @@ __global__ __launch_bounds__(kBlock) void k_bgk(const double* __restrict__ fin,
                                                 double* __restrict__ partial, int32_t accum) {
   const int32_t b = node_block(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
   const int32_t i = b * kBlock + threadIdx.x;
+  // the block's running sum, read before the node work (latency overlapped)
+  const double prev = REDUCE && accum && threadIdx.x == 0 ? partial[b] : 0.0;
   double jx = 0.0;
   if (i < n) jx = bgk_node<L, A, FIRST, STORE, REDUCE, MODE>(fin, fout, ad, i, NP, a, rho_o, vel_o);
   if (REDUCE) {
     const double s = block_sum(jx);
-    if (threadIdx.x == 0) partial[b] = accum ? partial[b] + s : s;
+    if (threadIdx.x == 0) partial[b] = accum ? prev + s : s;
   }
 }
 
@@ -257,6 +259,9 @@ __global__ __launch_bounds__(kBlock * NS) void k_batch_bgk(const BatchSolve* __r
     const int p = (parity ^ s.parity) & 1;
     const RowWin ad{s.coord, s.win, s.NP, s.PY, s.W};
     const int32_t i = lb * kBlock + tid;
+    // the running sum of this block over the interval, read before the node
+    // work so that its latency overlaps the gathers
+    const double prev = REDUCE && accum && tid == 0 && act ? s.partial[lb] : 0.0;
     double jx = 0.0;
     if (act && i < s.n)
       jx = bgk_node<L, RowWin, false, false, REDUCE>(static_cast<const double*>(s.f[p]),
@@ -264,7 +269,7 @@ __global__ __launch_bounds__(kBlock * NS) void k_batch_bgk(const BatchSolve* __r
                                                      nullptr, nullptr);
     if (REDUCE) {
       const double t = slice_sum<NS>(jx);
-      if (act && tid == 0) s.partial[lb] = accum ? s.partial[lb] + t : t;
+      if (act && tid == 0) s.partial[lb] = accum ? prev + t : t;
     }
   }
 }
