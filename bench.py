@@ -589,8 +589,9 @@ def serving_leg(n_req, pool_gb, every):
     out = {"n_requests": n_req, "mix": "70% permeability / 30% relative permeability, L U(48,160)",
            "pool_GB": pool / 1e9, "fp64_one_at_a_time_solves_per_s": fp64_rate, "fp64_subset": n_sub,
            "stop": "|dk| <= %g |k| (two-phase: |dk_i| <= %g (|k_0|+|k_1|)) per 100 steps at two consecutive "
-                   "checks; at most %d steps (two-phase %d)" % (SERVE_TOL, SERVE_TOL, SERVE_MAX_STEPS,
-                                                                  RELPERM_MAX_STEPS),
+                   "checks; at most %d steps; two-phase requests run a fixed %d steps (the reference's twophase "
+                   "loop runs nIterations with no stop test) unless they settle first" % (
+                       SERVE_TOL, SERVE_TOL, SERVE_MAX_STEPS, RELPERM_MAX_STEPS),
            "relperm_storage": "FP64"}
     for precision, name in ((lbamd.FP32S, "fp32s"), (lbamd.FP16S, "fp16s")):
         # 1) burst: capacity
