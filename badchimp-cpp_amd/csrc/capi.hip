@@ -576,7 +576,7 @@ int lb_upload_f(lb_ctx* c, int32_t nf, const double* f) {
     for (int32_t i = 0; i < c->n_bulk; ++i) {
       const double* src = f + static_cast<size_t>(c->node_of[i]) * nf * Q;
       for (int k = 0; k < nf; ++k)
-        for (int q = 0; q < Q; ++q) h[static_cast<size_t>(k) * Q * NP + pidx_host(Q, q, i)] = src[k * Q + q];
+        for (int q = 0; q < Q; ++q) h[static_cast<size_t>(nf) * pidx_host(Q, q, i) + k] = src[k * Q + q];
     }
     LB_HIP(c, hipMemcpy(c->d_f[0], h.data(), count * sizeof(double), hipMemcpyHostToDevice));
   } else {
@@ -584,7 +584,7 @@ int lb_upload_f(lb_ctx* c, int32_t nf, const double* f) {
     for (int32_t i = 0; i < c->n_bulk; ++i) {
       const double* src = f + static_cast<size_t>(c->node_of[i]) * nf * Q;
       for (int k = 0; k < nf; ++k)
-        for (int q = 0; q < Q; ++q) h[static_cast<size_t>(k) * Q * NP + pidx_host(Q, q, i)] = src[k * Q + q];
+        for (int q = 0; q < Q; ++q) h[static_cast<size_t>(nf) * pidx_host(Q, q, i) + k] = src[k * Q + q];
     }
     double* tmp = nullptr;
     LB_HIP(c, hipMalloc(&tmp, count * sizeof(double)));
@@ -629,7 +629,7 @@ int lb_download_f(lb_ctx* c, int32_t nf, double* f) {
   for (int32_t i = 0; i < c->n_bulk; ++i) {
     double* dst = f + static_cast<size_t>(c->node_of[i]) * nf * Q;
     for (int k = 0; k < nf; ++k)
-      for (int q = 0; q < Q; ++q) dst[k * Q + q] = h[static_cast<size_t>(k) * Q * NP + pidx_host(Q, q, i)];
+      for (int q = 0; q < Q; ++q) dst[k * Q + q] = h[static_cast<size_t>(nf) * pidx_host(Q, q, i) + k];
   }
   return LB_OK;
 }
@@ -700,13 +700,14 @@ int lb_init_equilibrium(lb_ctx* c, int32_t n_fields, int32_t field, int32_t rho_
   if (e == hipSuccess) e = hipMemcpy(dr, hr.data(), NP * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(dv, hv.data(), c->nD * NP * sizeof(double), hipMemcpyHostToDevice);
   if (c->precision == LB_FP64) {
-    if (e == hipSuccess) e = init_eq_f64(c->lattice, static_cast<double*>(c->d_f[c->cur]), field, c->n_bulk, NP, dr, dv, c->stream);
+    if (e == hipSuccess) e = init_eq_f64(c->lattice, static_cast<double*>(c->d_f[c->cur]), c->n_fields, field, c->n_bulk, NP, dr, dv,
+                                         c->stream);
   } else {
     double* tmp = nullptr;
     const size_t cnt = static_cast<size_t>(c->n_fields) * c->Q * NP;
     if (e == hipSuccess) e = hipMalloc(&tmp, cnt * sizeof(double));
     if (e == hipSuccess) e = load_lowp(c->precision, c->lattice, c->n_fields, c->d_f[c->cur], tmp, c->n_bulk, NP, c->stream);
-    if (e == hipSuccess) e = init_eq_f64(c->lattice, tmp, field, c->n_bulk, NP, dr, dv, c->stream);
+    if (e == hipSuccess) e = init_eq_f64(c->lattice, tmp, c->n_fields, field, c->n_bulk, NP, dr, dv, c->stream);
     if (e == hipSuccess) e = store_lowp(c->precision, c->lattice, c->n_fields, tmp, c->d_f[c->cur], c->n_bulk, NP, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (tmp) hipFree(tmp);

@@ -37,8 +37,10 @@ constexpr int kStopHold = 2;
 // Population element (direction q, node i) of one field: blocks of kPB
 // consecutive nodes keep their Q directions one after another (AoSoA), so a
 // wavefront's loads and stores of one node block fall into one DRAM region
-// instead of Q separate planes.  Field k starts at k * Q * NP (NP is a
-// multiple of kPB).
+// instead of Q separate planes (NP is a multiple of kPB).  With nf fields
+// (two for the two-phase populations) the fields of one element are
+// adjacent: field k of element e = pidx(q, i) is at nf * e + k, so one
+// vector access moves both fields of a direction.
 constexpr int kPLog = 6;
 constexpr uint32_t kPB = 1u << kPLog;
 // Written as i + 64 (Q - 1) (i >> 6) + 64 q (the same value), so that the

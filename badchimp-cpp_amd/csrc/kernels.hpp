@@ -48,7 +48,7 @@ hipError_t flux_f64(int lattice, int nfields, bool first, const double* fin, con
 // nodes in ascending node order (rank[i]: position of compact node i).
 hipError_t unstage_f64(int lattice, int nf, const double* g, double* f, int32_t n, int64_t NP,
                        const int32_t* rank, hipStream_t s);
-hipError_t init_eq_f64(int lattice, double* f, int field, int32_t n, int64_t NP,
+hipError_t init_eq_f64(int lattice, double* f, int nf, int field, int32_t n, int64_t NP,
                        const double* rho, const double* vel, hipStream_t s);
 
 // ---- tagged boundaries (std_one_phase/main.cpp:27-204, 513-575) -------------

@@ -59,6 +59,22 @@ struct Store<uint16_t> {
   }
 };
 
+// The two stored values (t, d) of one two-phase element, adjacent in memory
+// (field k of element e at 2 e + k): one 8-byte (float) or 4-byte (half)
+// access moves both.
+template <typename S>
+struct PairOf;
+template <>
+struct PairOf<float> {
+  typedef float T __attribute__((ext_vector_type(2)));
+};
+template <>
+struct PairOf<uint16_t> {
+  typedef uint16_t T __attribute__((ext_vector_type(2)));
+};
+template <typename S>
+using Pair = typename PairOf<S>::T;
+
 // Random words of the stochastic rounding: two hashes of the node (its index
 // in the solve) and the solve's own step count, then one word per stored
 // value k: a rotation of the first hash, xor the second, plus a per-k
@@ -377,13 +393,12 @@ __global__ __launch_bounds__(kBlock) void k_prestream(RowWin ad, const double* _
   uint32_t src[Q - 1];
   ad.template sources<L>(i, src);
   const double a = r0[i], b = 1.0 - a;
-  const int64_t fs = static_cast<int64_t>(Q) * NP;
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int64_t o = q < Q - 1 ? static_cast<int64_t>(src[q]) : static_cast<int64_t>(pidx<Q>(Q - 1, i));
     const double f0 = L::w(q) * a, f1 = L::w(q) * b;
-    g[o] = Store<S>::put(static_cast<float>((f0 + f1) - L::w(q)));
-    g[fs + o] = Store<S>::put(static_cast<float>(f0 - f1));
+    reinterpret_cast<Pair<S>*>(g)[o] =
+        Pair<S>{Store<S>::put(static_cast<float>((f0 + f1) - L::w(q))), Store<S>::put(static_cast<float>(f0 - f1))};
   }
 }
 
@@ -392,20 +407,21 @@ __global__ __launch_bounds__(kBlock) void k_store(const double* __restrict__ f, 
                                                   int32_t n, int64_t NP, int nf) {
   const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  const int64_t fs = static_cast<int64_t>(L::nQ) * NP;
 #pragma unroll
   for (int q = 0; q < L::nQ; ++q) {
     const int64_t o = pidx<L::nQ>(q, i);
     if (nf == 2) {
-      g[o] = Store<S>::put(static_cast<float>((f[o] + f[fs + o]) - L::w(q)));
-      g[fs + o] = Store<S>::put(static_cast<float>(f[o] - f[fs + o]));
+      const double f0 = f[2 * o], f1 = f[2 * o + 1];
+      reinterpret_cast<Pair<S>*>(g)[o] =
+          Pair<S>{Store<S>::put(static_cast<float>((f0 + f1) - L::w(q))), Store<S>::put(static_cast<float>(f0 - f1))};
     } else {
       g[o] = Store<S>::put(static_cast<float>(f[o] - L::w(q)));
     }
   }
 }
 
-// rank == null: f in the same AoSoA layout as g.  Otherwise node i goes to
+// rank == null: f in the same AoSoA layout as g (two fields: element e at
+// 2 e, 2 e + 1).  Otherwise node i goes to
 // f[(rank[i] nf + k) Q + q], the LbField layout over the bulk nodes in
 // ascending node order (rank[i]: the position of node i among them).
 template <class L, typename S>
@@ -413,64 +429,67 @@ __global__ __launch_bounds__(kBlock) void k_load(const S* __restrict__ g, double
                                                  int32_t n, int64_t NP, int nf, const int32_t* __restrict__ rank) {
   const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  const int64_t fs = static_cast<int64_t>(L::nQ) * NP;
   const int64_t r = rank ? static_cast<int64_t>(rank[i]) * nf * L::nQ : 0;
 #pragma unroll
   for (int q = 0; q < L::nQ; ++q) {
     const int64_t o = pidx<L::nQ>(q, i);
-    const int64_t o0 = rank ? r + q : o, o1 = rank ? r + L::nQ + q : fs + o;
-    const double t = static_cast<double>(Store<S>::get(g[o])) + L::w(q);
     if (nf == 2) {
-      const double d = static_cast<double>(Store<S>::get(g[fs + o]));
-      f[o0] = 0.5 * (t + d);
-      f[o1] = 0.5 * (t - d);
+      const Pair<S> v = reinterpret_cast<const Pair<S>*>(g)[o];
+      const double t = static_cast<double>(Store<S>::get(v.x)) + L::w(q);
+      const double d = static_cast<double>(Store<S>::get(v.y));
+      f[rank ? r + q : 2 * o] = 0.5 * (t + d);
+      f[rank ? r + L::nQ + q : 2 * o + 1] = 0.5 * (t - d);
     } else {
-      f[o0] = t;
+      f[rank ? r + q : o] = static_cast<double>(Store<S>::get(g[o])) + L::w(q);
     }
   }
 }
 
-// Both planes (t, d) of a two-phase node, one address computation.
+// Both values (t, d) of each direction of a two-phase node: one pair load
+// per direction.
 template <class L, typename S, bool FIRST, class A>
-__device__ __forceinline__ void load_td(const S* __restrict__ gin, uint32_t fs, const A& ad, int32_t i,
-                                        double (&t)[L::nQ], double (&d)[L::nQ]) {
+__device__ __forceinline__ void load_td(const S* __restrict__ gin, const A& ad, int32_t i, double (&t)[L::nQ],
+                                        double (&d)[L::nQ]) {
   constexpr bool O = A::kOff32;
+  const Pair<S>* __restrict__ g2 = reinterpret_cast<const Pair<S>*>(gin);
+  uint32_t src[L::nQ - 1];
   if (FIRST) {
 #pragma unroll
-    for (int q = 0; q < L::nQ - 1; ++q) {
-      t[q] = Store<S>::get(ld_elem<O>(gin, pidx<L::nQ>(q, i)));
-      d[q] = Store<S>::get(ld_elem<O>(gin, fs + pidx<L::nQ>(q, i)));
-    }
+    for (int q = 0; q < L::nQ - 1; ++q) src[q] = pidx<L::nQ>(q, i);
   } else {
-    uint32_t src[L::nQ - 1];
     ad.template sources<L>(i, src);
-#pragma unroll
-    for (int q = 0; q < L::nQ - 1; ++q) {
-      t[q] = Store<S>::get(ld_elem<O>(gin, src[q]));
-      d[q] = Store<S>::get(ld_elem<O>(gin, fs + src[q]));
-    }
   }
-  t[L::nQ - 1] = Store<S>::get(ld_elem<O>(gin, pidx<L::nQ>(L::nQ - 1, i)));
-  d[L::nQ - 1] = Store<S>::get(ld_elem<O>(gin, fs + pidx<L::nQ>(L::nQ - 1, i)));
+#pragma unroll
+  for (int q = 0; q < L::nQ - 1; ++q) {
+    const Pair<S> v = ld_elem<O>(g2, src[q]);
+    t[q] = Store<S>::get(v.x);
+    d[q] = Store<S>::get(v.y);
+  }
+  const Pair<S> v = ld_elem<O>(g2, pidx<L::nQ>(L::nQ - 1, i));
+  t[L::nQ - 1] = Store<S>::get(v.x);
+  d[L::nQ - 1] = Store<S>::get(v.y);
 }
 
 // (t, d) of a node from row-window links, and the links themselves for the
 // neighbour phi lookup: bit q of fluid set when the source of direction q is
 // the fluid node j[q], otherwise it is a solid (bounce-back).
 template <class L, typename S>
-__device__ __forceinline__ void load_td_links(const S* __restrict__ gin, uint32_t fs, const RowWin& ad, int32_t i,
+__device__ __forceinline__ void load_td_links(const S* __restrict__ gin, const RowWin& ad, int32_t i,
                                               double (&t)[L::nQ], double (&d)[L::nQ], uint32_t (&j)[L::nQ - 1],
                                               uint32_t& fluid) {
+  const Pair<S>* __restrict__ g2 = reinterpret_cast<const Pair<S>*>(gin);
   rowwin_links<L>(ad, i, j, fluid);
 #pragma unroll
   for (int q = 0; q < L::nQ - 1; ++q) {
     const uint32_t m = 0u - ((fluid >> q) & 1u);
     const uint32_t e = (pidx<L::nQ>(q, j[q]) & m) | (pidx<L::nQ>(L::rev(q), i) & ~m);
-    t[q] = Store<S>::get(ld_elem<true>(gin, e));
-    d[q] = Store<S>::get(ld_elem<true>(gin, fs + e));
+    const Pair<S> v = ld_elem<true>(g2, e);
+    t[q] = Store<S>::get(v.x);
+    d[q] = Store<S>::get(v.y);
   }
-  t[L::nQ - 1] = Store<S>::get(ld_elem<true>(gin, pidx<L::nQ>(L::nQ - 1, i)));
-  d[L::nQ - 1] = Store<S>::get(ld_elem<true>(gin, fs + pidx<L::nQ>(L::nQ - 1, i)));
+  const Pair<S> v = ld_elem<true>(g2, pidx<L::nQ>(L::nQ - 1, i));
+  t[L::nQ - 1] = Store<S>::get(v.x);
+  d[L::nQ - 1] = Store<S>::get(v.y);
 }
 
 // phi = (r0 - r1) / (r0 + r1) = sum d / (1 + sum t)  (main_TWOPHASE.cpp:246-262)
@@ -521,7 +540,7 @@ template <class L, typename S, bool O, bool STORE>
 __device__ __forceinline__ void cg_collide(const double (&t)[L::nQ], const double (&dd)[L::nQ],
                                            const float (&pn)[L::nQ], const double (&F)[3], double nu0Inv,
                                            double nu1Inv, double sigma, double beta, S* __restrict__ gout,
-                                           uint32_t fs, int32_t i, int64_t NP, double* __restrict__ rho_o,
+                                           int32_t i, int64_t NP, double* __restrict__ rho_o,
                                            double* __restrict__ vel_o, uint32_t step) {
   constexpr int Q = L::nQ, D = L::nD;
   const SrSeed hn = sr_node(i, step);  // unused (removed) for float storage
@@ -564,8 +583,7 @@ __device__ __forceinline__ void cg_collide(const double (&t)[L::nQ], const doubl
   const double m3uF = -3.0 * dot<L>(u, F);
   const double omt = 1.0 - ti;
   const double r45 = 4.5 * rho, r3 = 3.0 * rho;
-  S* __restrict__ out = elem_ptr<O>(gout, pidx<Q>(0, i));
-  S* __restrict__ outd = elem_ptr<O>(gout, fs + pidx<Q>(0, i));
+  Pair<S>* __restrict__ out = elem_ptr<O>(reinterpret_cast<Pair<S>*>(gout), pidx<Q>(0, i));
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const double w = L::w(q);
@@ -585,8 +603,8 @@ __device__ __forceinline__ void cg_collide(const double (&t)[L::nQ], const doubl
     const double bs = omt * t[q] + ti * geq + dF + st;
     const float bt = static_cast<float>(bs);
     const float bd = static_cast<float>(phi_n * (w + bs) + 2.0 * rc);
-    __builtin_nontemporal_store(Store<S>::put_sr(bt, sr_bits(hn, q)), out + (q << kPLog));
-    __builtin_nontemporal_store(Store<S>::put_sr(bd, sr_bits(hn, q + 32)), outd + (q << kPLog));
+    const Pair<S> v{Store<S>::put_sr(bt, sr_bits(hn, q)), Store<S>::put_sr(bd, sr_bits(hn, q + 32))};
+    __builtin_nontemporal_store(v, out + (q << kPLog));
   }
 }
 
@@ -602,7 +620,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_phi_lp(const S* __restrict__ gin,
   double mx = 0.0;
   if (i < n) {
     double t[Q], d[Q];
-    load_td<L, S, FIRST>(gin, static_cast<uint32_t>(Q * NP), ad, i, t, d);
+    load_td<L, S, FIRST>(gin, ad, i, t, d);
     phi[i] = static_cast<float>(phi_td<L>(t, d));
     if (MOMX) mx = qsumc<L>(t, 0);
   }
@@ -625,14 +643,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_lp(const S* __restrict__ gin, S* 
   const int32_t b = node_block_rev(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
   const int32_t i = b * kBlock + threadIdx.x;
   if (i >= n) return;
-  const uint32_t fs = static_cast<uint32_t>(Q * NP);
   double t[Q], dd[Q];
-  load_td<L, S, FIRST>(gin, fs, ad, i, t, dd);
+  load_td<L, S, FIRST>(gin, ad, i, t, dd);
   double F[3] = {F0, F1, F2};
   if (momx_mode) F[0] = dscalar[0];
   float pn[Q];
   neighbour_phi<L>(ad, i, phi, wp, pn);
-  cg_collide<L, S, A::kOff32, STORE>(t, dd, pn, F, nu0Inv, nu1Inv, sigma, beta, gout, fs, i, NP, rho_o, vel_o,
+  cg_collide<L, S, A::kOff32, STORE>(t, dd, pn, F, nu0Inv, nu1Inv, sigma, beta, gout, i, NP, rho_o, vel_o,
                                      step);
 }
 
@@ -660,7 +677,7 @@ __global__ __launch_bounds__(kBlock * NS) void k_batch_cg_phi_lp(const BatchSolv
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     if (act && i < s.n) {
       double t[Q], d[Q];
-      load_td<L, S, false>(static_cast<const S*>(s.f[p]), Q * s.NP, ad, i, t, d);
+      load_td<L, S, false>(static_cast<const S*>(s.f[p]), ad, i, t, d);
       static_cast<float*>(s.phi)[i] = static_cast<float>(phi_td<L>(t, d));
       if (REDUCE) flux_td<L>(t, d, 0, s0, s1, s2);
     }
@@ -700,10 +717,9 @@ __global__ __launch_bounds__(kBlock * NS) void k_batch_cg_lp(const BatchSolve* _
     const RowWin ad{s.coord, s.win, s.NP, s.PY, s.W};
     const int32_t i = lb * kBlock + tid;
     if (i >= s.n) continue;
-    const uint32_t fs = Q * s.NP;
     double t[Q], dd[Q];
     uint32_t j[Q - 1], fluid;
-    load_td_links<L, S>(static_cast<const S*>(s.f[p]), fs, ad, i, t, dd, j, fluid);
+    load_td_links<L, S>(static_cast<const S*>(s.f[p]), ad, i, t, dd, j, fluid);
     const float pw = static_cast<float>(s.phi_w);
     float pn[Q];
 #pragma unroll
@@ -715,7 +731,7 @@ __global__ __launch_bounds__(kBlock * NS) void k_batch_cg_lp(const BatchSolve* _
     }
     const double F[3] = {s.a.F[0], s.a.F[1], s.a.F[2]};
     cg_collide<L, S, true, false>(t, dd, pn, F, s.nu0Inv, s.nu1Inv, s.sigma, s.beta, static_cast<S*>(s.f[p ^ 1]),
-                                  fs, i, s.NP, nullptr, nullptr, gstep - s.step_base);
+                                  i, s.NP, nullptr, nullptr, gstep - s.step_base);
   }
 }
 
@@ -725,13 +741,13 @@ __global__ __launch_bounds__(kBlock) void k_gather_lp(const S* __restrict__ gin,
   constexpr int Q = L::nQ;
   const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  const int64_t fs = static_cast<int64_t>(Q) * NP;
   uint32_t src[Q - 1];
   ad.template sources<L>(i, src);
+  const int64_t m = nf;  // fields per element
   for (int k = 0; k < nf; ++k) {
 #pragma unroll
-    for (int q = 0; q < Q - 1; ++q) gout[k * fs + pidx<Q>(q, i)] = gin[k * fs + src[q]];
-    gout[k * fs + pidx<Q>(Q - 1, i)] = gin[k * fs + pidx<Q>(Q - 1, i)];
+    for (int q = 0; q < Q - 1; ++q) gout[m * pidx<Q>(q, i) + k] = gin[m * src[q] + k];
+    gout[m * pidx<Q>(Q - 1, i) + k] = gin[m * pidx<Q>(Q - 1, i) + k];
   }
 }
 
@@ -764,7 +780,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_td(const S* __restrict__ gin, T
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
   if (i < n) {
     double t[Q], d[Q];
-    load_td<L, S, FIRST>(gin, static_cast<uint32_t>(Q * NP), ad, i, t, d);
+    load_td<L, S, FIRST>(gin, ad, i, t, d);
     flux_td<L>(t, d, dir, s0, s1, s2);
   }
   s0 = block_sum(s0);

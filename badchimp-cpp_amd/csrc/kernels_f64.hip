@@ -20,28 +20,35 @@
 namespace lbamd {
 namespace {
 
-// Load the pre-collision populations of node i for NF fields.
+// Both fields of one element (two-phase populations, adjacent in memory).
+typedef double Double2 __attribute__((ext_vector_type(2)));
+
+// Load the pre-collision populations of node i for NF fields (two fields:
+// one 16-byte load per direction).
 template <class L, int NF, bool FIRST, class A>
 __device__ __forceinline__ void load_node(const double* __restrict__ fin, const A& ad, int32_t i,
-                                          int64_t NP, double (&f)[NF][L::nQ]) {
+                                          int64_t /*NP*/, double (&f)[NF][L::nQ]) {
+  static_assert(NF == 1 || NF == 2, "one or two fields");
   constexpr bool O = A::kOff32;
-  const uint32_t np = static_cast<uint32_t>(NP);
-  const uint32_t fstride = static_cast<uint32_t>(L::nQ) * np;
+  uint32_t src[L::nQ - 1];
   if (FIRST) {
 #pragma unroll
-    for (int q = 0; q < L::nQ - 1; ++q)
-#pragma unroll
-      for (int k = 0; k < NF; ++k) f[k][q] = ld_elem<O>(fin, k * fstride + pidx<L::nQ>(q, i));
+    for (int q = 0; q < L::nQ - 1; ++q) src[q] = pidx<L::nQ>(q, i);
   } else {
-    uint32_t src[L::nQ - 1];
     ad.template sources<L>(i, src);
-#pragma unroll
-    for (int q = 0; q < L::nQ - 1; ++q)
-#pragma unroll
-      for (int k = 0; k < NF; ++k) f[k][q] = ld_elem<O>(fin, k * fstride + src[q]);
   }
+  auto load = [&](int q, uint32_t e) {
+    if constexpr (NF == 1) {
+      f[0][q] = ld_elem<O>(fin, e);
+    } else {
+      const Double2 v = ld_elem<O>(reinterpret_cast<const Double2*>(fin), e);
+      f[0][q] = v.x;
+      f[NF - 1][q] = v.y;
+    }
+  };
 #pragma unroll
-  for (int k = 0; k < NF; ++k) f[k][L::nQ - 1] = ld_elem<O>(fin, k * fstride + pidx<L::nQ>(L::nQ - 1, i));
+  for (int q = 0; q < L::nQ - 1; ++q) load(q, src[q]);
+  load(L::nQ - 1, pidx<L::nQ>(L::nQ - 1, i));
 }
 
 // Deterministic block sum of 256 values; result valid in thread 0.
@@ -353,12 +360,10 @@ __global__ __launch_bounds__(kBlock) void k_prestream(RowWin ad, const double* _
   uint32_t src[Q - 1];
   ad.template sources<L>(i, src);
   const double a = r0[i], b = 1.0 - a;
-  const int64_t fs = static_cast<int64_t>(Q) * NP;
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int64_t o = q < Q - 1 ? static_cast<int64_t>(src[q]) : static_cast<int64_t>(pidx<Q>(Q - 1, i));
-    f[o] = L::w(q) * a;
-    f[fs + o] = L::w(q) * b;
+    reinterpret_cast<Double2*>(f)[o] = Double2{L::w(q) * a, L::w(q) * b};
   }
 }
 
@@ -542,9 +547,7 @@ __device__ __forceinline__ void cg_node(const double* __restrict__ fin, double* 
   const double rfb = ((a.beta * r0) * r1) / rho;
   const double c0 = r0 / rho;
   const double c1 = r1 / rho;
-  const int64_t fstride = static_cast<int64_t>(Q) * NP;
-  double* __restrict__ out0 = elem_ptr<A::kOff32>(fout, pidx<Q>(0, i));
-  double* __restrict__ out1 = elem_ptr<A::kOff32>(fout, static_cast<uint32_t>(fstride) + pidx<Q>(0, i));
+  Double2* __restrict__ out = elem_ptr<A::kOff32>(reinterpret_cast<Double2*>(fout), pidx<Q>(0, i));
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const double cu = cdot<L>(q, u);
@@ -564,8 +567,8 @@ __device__ __forceinline__ void cg_node(const double* __restrict__ fin, double* 
       rc = 0.0;
     }
     const double base = ((ft[q] + om) + dF) + st;
-    __builtin_nontemporal_store(((c0 * base) + rc) + dQ0, out0 + (q << kPLog));
-    __builtin_nontemporal_store(((c1 * base) - rc) + dQ1, out1 + (q << kPLog));
+    const Double2 v{((c0 * base) + rc) + dQ0, ((c1 * base) - rc) + dQ1};
+    __builtin_nontemporal_store(v, out + (q << kPLog));
   }
 }
 
@@ -684,13 +687,13 @@ __global__ __launch_bounds__(kBlock) void k_gather(const double* __restrict__ fi
   constexpr int Q = L::nQ;
   const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  const int64_t fstride = static_cast<int64_t>(Q) * NP;
   uint32_t src[Q - 1];
   ad.template sources<L>(i, src);
+  const int64_t m = nf;  // fields per element
   for (int k = 0; k < nf; ++k) {
 #pragma unroll
-    for (int q = 0; q < Q - 1; ++q) fout[k * fstride + pidx<Q>(q, i)] = fin[k * fstride + src[q]];
-    fout[k * fstride + pidx<Q>(Q - 1, i)] = fin[k * fstride + pidx<Q>(Q - 1, i)];
+    for (int q = 0; q < Q - 1; ++q) fout[m * pidx<Q>(q, i) + k] = fin[m * src[q] + k];
+    fout[m * pidx<Q>(Q - 1, i) + k] = fin[m * pidx<Q>(Q - 1, i) + k];
   }
 }
 
@@ -728,7 +731,7 @@ __global__ __launch_bounds__(kBlock) void k_flux(const double* __restrict__ fin,
 }
 
 template <class L>
-__global__ __launch_bounds__(kBlock) void k_init_eq(double* __restrict__ f, int field,
+__global__ __launch_bounds__(kBlock) void k_init_eq(double* __restrict__ f, int nf, int field,
                                                     int32_t n, int64_t NP,
                                                     const double* __restrict__ rho,
                                                     const double* __restrict__ vel) {
@@ -740,16 +743,16 @@ __global__ __launch_bounds__(kBlock) void k_init_eq(double* __restrict__ f, int 
   for (int d = 0; d < D; ++d) u[d] = vel[d * NP + i];
   const double uu = dot<L>(u, u);
   const double r = rho[i];
-  const int64_t base = static_cast<int64_t>(field) * Q * NP;
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const double cu = cdot<L>(q, u);
-    f[base + pidx<Q>(q, i)] = (L::w(q) * r) * ((1.0 + L::c2Inv * cu) + L::c4Inv0_5 * (cu * cu - L::c2 * uu));
+    f[static_cast<int64_t>(nf) * pidx<Q>(q, i) + field] = (L::w(q) * r) * ((1.0 + L::c2Inv * cu) + L::c4Inv0_5 * (cu * cu - L::c2 * uu));
   }
 }
 
 // Populations of nf fields in the LbField layout over the bulk nodes in
-// ascending node order: node i goes to f[(rank[i] nf + k) Q + q].
+// ascending node order: node i goes to f[(rank[i] nf + k) Q + q] (from
+// g[nf pidx(q, i) + k]).
 template <class L>
 __global__ __launch_bounds__(kBlock) void k_unstage(const double* __restrict__ g, double* __restrict__ f,
                                                     int32_t n, int64_t NP, int nf,
@@ -760,7 +763,7 @@ __global__ __launch_bounds__(kBlock) void k_unstage(const double* __restrict__ g
   const int64_t r = static_cast<int64_t>(rank[i]) * nf * Q;
   for (int k = 0; k < nf; ++k)
 #pragma unroll
-    for (int q = 0; q < Q; ++q) f[r + k * Q + q] = g[static_cast<int64_t>(k) * Q * NP + pidx<Q>(q, i)];
+    for (int q = 0; q < Q; ++q) f[r + k * Q + q] = g[static_cast<int64_t>(nf) * pidx<Q>(q, i) + k];
 }
 
 __global__ __launch_bounds__(kBlock) void k_sum_partials(const double* __restrict__ partial,
@@ -926,11 +929,11 @@ hipError_t flux_f64(int lattice, int nfields, bool first, const double* fin, con
   return hipGetLastError();
 }
 
-hipError_t init_eq_f64(int lattice, double* f, int field, int32_t n, int64_t NP,
+hipError_t init_eq_f64(int lattice, double* f, int nf, int field, int32_t n, int64_t NP,
                        const double* rho, const double* vel, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   LB_DISPATCH_LATTICE(lattice, L, {
-    k_init_eq<L><<<grid_for(n), dim3(kBlock), 0, s>>>(f, field, n, NP, rho, vel);
+    k_init_eq<L><<<grid_for(n), dim3(kBlock), 0, s>>>(f, nf, field, n, NP, rho, vel);
   });
   return hipGetLastError();
 }
