@@ -695,8 +695,11 @@ __global__ __launch_bounds__(kBlock * NS) void k_batch_cg_phi_lp(const BatchSolv
   }
 }
 
+// At least 4 waves per SIMD (at most 128 VGPRs, no spills): left to
+// itself the compiler takes 132 VGPRs (3 waves) for the FP64 two-phase
+// arithmetic, and the pass ran 6 % slower (192^3 pack, FP32S).
 template <class L, typename S, int NS>
-__global__ __launch_bounds__(kBlock * NS) void k_batch_cg_lp(const BatchSolve* __restrict__ solves,
+__global__ __launch_bounds__(kBlock * NS, 4) void k_batch_cg_lp(const BatchSolve* __restrict__ solves,
                                                              const uint32_t* __restrict__ items, int32_t n_items,
                                                              int32_t per_wg, int32_t parity,
                                                              const uint32_t* __restrict__ step0, int32_t koff) {
