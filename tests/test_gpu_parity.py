@@ -172,6 +172,49 @@ def test_twophase_fixed_force_bitexact(gpu_available, name, layout):
     assert np.array_equal(phi[b], cgo[b])
 
 
+@pytest.mark.parametrize("precision,layout", [(lbamd.FP64, "table"), (lbamd.FP64, "grid"),
+                                              (lbamd.FP32S, "grid"), (lbamd.FP16S, "grid")])
+def test_twophase_d2q9_vs_oracle(gpu_available, precision, layout):
+    """The colour-gradient step on D2Q9 (random 33 x 17 geometry, periodic x,
+    walls on y; phases drawn per node; wall colour 1): FP64 bit for bit
+    against the oracle, both layouts, in chunks; FP32S within phi 1e-4
+    (absolute) and velocity 1e-4 (norm-wise), FP16S within 2e-3 and 2e-2."""
+    rng = np.random.default_rng(7)
+    geo = (rng.random((33, 17)) > 0.3).astype(np.int64)
+    geo[:, 0] = 0
+    geo[:, -1] = 0
+    T = OracleTopo(geo, "D2Q9", "x")
+    r0 = np.zeros(T.n_nodes)
+    r0[T.bulk] = (rng.random(T.bulk.size) < 0.5).astype(float)
+    r1 = np.zeros(T.n_nodes)
+    r1[T.bulk] = 1.0 - r0[T.bulk]
+    sb = T.node_types == 1
+    r0[sb], r1[sb] = 1.0, 0.0
+    W = oracle.WEIGHTS[T.lattice]
+    f0 = np.zeros((T.n_nodes, 2, 9))
+    f0[T.bulk, 0] = W[None] * r0[T.bulk, None]
+    f0[T.bulk, 1] = W[None] * r1[T.bulk, None]
+    rho = np.stack([r0, r1], 1)
+    p = {"tau0": 0.9, "tau1": 1.3, "sigma": 0.01, "beta": 0.7}
+    force = [4e-6, -1e-6]
+    fo, ro, vo, cgo, _ = oracle.run_twophase(T.lattice, T.neighbors, T.node_types, T.bulk, f0, rho, p["tau0"],
+                                             p["tau1"], p["sigma"], p["beta"], 0, force, 30)
+    f, r, v, phi, _ = gpu_twophase(T, f0, rho, p, lbamd.FORCE_FIXED, force, 30, chunks=[1, 29], layout=layout,
+                                   precision=precision)
+    b = T.bulk
+    if precision == lbamd.FP64:
+        assert np.array_equal(f[b], fo[b])
+        assert np.array_equal(r[b], ro[b])
+        assert np.array_equal(v[b], vo[b])
+        assert np.array_equal(phi[b], cgo[b])
+    else:
+        ptol, vtol = (1e-4, 1e-4) if precision == lbamd.FP32S else (2e-3, 2e-2)
+        ep = np.abs(phi[b] - cgo[b]).max()
+        ev = np.abs(v[b] - vo[b]).max() / np.abs(vo[b]).max()
+        print("phi %.3e  vel %.3e" % (ep, ev))
+        assert ep < ptol and ev < vtol
+
+
 def test_flux_reduction(gpu_available):
     d = load_ref("tp_pack_16")
     p = d["params"]
