@@ -33,6 +33,7 @@ SOURCES = {
     "batch.hip": HIP + ["-O2", "-ffp-contract=off"],
     "calibrate.hip": HIP + ["-O3"],
     "topo_device.hip": HIP + ["-O3"],
+    "halo.hip": HIP + ["-O3"],
     "topology.cpp": ["-O2", "-ffp-contract=off", "-x", "c++"],
     "vtu.cpp": ["-O2", "-x", "c++"],
 }

@@ -117,6 +117,13 @@ void free_tags(lb_ctx* c) {
   c->tag_pressure.clear();
 }
 
+void free_halos(lb_ctx* c) {
+  for (auto& h : c->halos)
+    for (void* p : {static_cast<void*>(h.d_send), static_cast<void*>(h.d_recv), static_cast<void*>(h.d_buf)})
+      if (p) hipFree(p);
+  c->halos.clear();
+}
+
 void free_fields(lb_ctx* c) {
   for (auto& p : c->d_f) {
     if (p) hipFree(p);
@@ -312,15 +319,29 @@ int setup_topology(lb_ctx* c, int32_t n_nodes, const int32_t* neighbor, const in
   }
   for (size_t k = 0; k < static_cast<size_t>(n_nodes) * Q; ++k)
     if (neighbor[k] < 0 || neighbor[k] >= n_nodes) return fail(c, LB_ERR_ARG, "neighbour index out of range");
+  // other-rank fluid nodes (the halo): a population slot each, after the
+  // bulk nodes, in ascending node order; the bulk nodes gather from them
+  // through the link table (lb_halo_unpack fills them between steps)
+  std::vector<int32_t> ghosts;
+  for (int32_t n = 1; n < n_nodes; ++n)
+    if (node_type[n] == 4) ghosts.push_back(n);
+  const int32_t n_ghost = static_cast<int32_t>(ghosts.size());
+  if (static_cast<int64_t>(n_bulk) + n_ghost >= static_cast<int64_t>(kLinkIndexMask))
+    return fail(c, LB_ERR_ARG, "too many bulk and halo nodes");
   // compact order: the bulk order, or the tiled row order of the row layout
   RowBuild rb;
   std::string why = "no positions given";
-  bool rows_ok = pos && !(flags & LB_TOPO_TABLE_ONLY) &&
-                 build_rows(c->lattice, n_nodes, neighbor, bulk, n_bulk, bidx, pos, rb, why);
+  bool rows_ok = false;
+  if (n_ghost > 0)
+    why = "other-rank neighbours (the halo slots are addressed through the link table)";
+  else
+    rows_ok = pos && !(flags & LB_TOPO_TABLE_ONLY) &&
+              build_rows(c->lattice, n_nodes, neighbor, bulk, n_bulk, bidx, pos, rb, why);
   std::vector<int32_t> order(n_bulk);
   for (int32_t k = 0; k < n_bulk; ++k) order[k] = rows_ok ? rb.order[k] : k;
   std::vector<int32_t> compact(n_nodes, -1);
   for (int32_t i = 0; i < n_bulk; ++i) compact[bulk[order[i]]] = i;
+  for (int32_t k = 0; k < n_ghost; ++k) compact[ghosts[k]] = n_bulk + k;
   // The push of the reference writes f_post(p, m) into slot (p, neighbor(p,m)).
   // For node 0 the slot is shared: the last bulk node (ascending) wins.
   std::vector<int32_t> winner(Q, -1);
@@ -329,7 +350,8 @@ int setup_topology(lb_ctx* c, int32_t n_nodes, const int32_t* neighbor, const in
     for (int p = 0; p < Q - 1; ++p)
       if (nb[p] == 0) winner[p] = compact[bulk[k]];
   }
-  const int64_t NP = ((static_cast<int64_t>(n_bulk) + kBlock - 1) / kBlock) * kBlock + (n_bulk == 0 ? kBlock : 0);
+  const int64_t n_slots = static_cast<int64_t>(n_bulk) + n_ghost;
+  const int64_t NP = ((n_slots + kBlock - 1) / kBlock) * kBlock + (n_slots == 0 ? kBlock : 0);
   // element indices of both fields of a buffer are 32-bit
   if (static_cast<uint64_t>(Q) * static_cast<uint64_t>(NP) * 2u >= (1ull << 32))
     return fail(c, LB_ERR_UNSUPPORTED, "%d bulk nodes exceed one context (limit %llu); split the domain", n_bulk,
@@ -404,6 +426,9 @@ int setup_topology(lb_ctx* c, int32_t n_nodes, const int32_t* neighbor, const in
   c->NP = 0;
   free_fields(c);
   free_tags(c);
+  free_halos(c);
+  c->n_ghost = 0;
+  c->slot_of.clear();
   for (void* p : {static_cast<void*>(c->d_coord), static_cast<void*>(c->d_rows), static_cast<void*>(c->d_win)})
     if (p) hipFree(p);
   c->d_coord = nullptr;
@@ -476,6 +501,8 @@ int setup_topology(lb_ctx* c, int32_t n_nodes, const int32_t* neighbor, const in
   }
   c->wall_nodes = walls;
   c->node_type.assign(node_type, node_type + n_nodes);
+  c->n_ghost = n_ghost;
+  c->slot_of = std::move(compact);
   c->has_topo = true;
   c->have_macro = false;
   return LB_OK;
@@ -523,6 +550,7 @@ int lb_ctx_destroy(lb_ctx* c) {
   hipStreamSynchronize(c->stream);
   free_fields(c);
   free_tags(c);
+  free_halos(c);
   for (void* p : {static_cast<void*>(c->d_link), static_cast<void*>(c->d_rho), static_cast<void*>(c->d_vel),
                   static_cast<void*>(c->d_phi), static_cast<void*>(c->d_phi_wall), static_cast<void*>(c->d_partial),
                   static_cast<void*>(c->d_scalar), static_cast<void*>(c->d_coord),
@@ -917,6 +945,8 @@ int lb_set_node_tags(lb_ctx* c, int32_t n_nodes, const int32_t* neighbor, const 
   if (!c->has_topo) return fail(c, LB_ERR_STATE, "no topology: call lb_set_topology first");
   if (c->precision != LB_FP64)
     return fail(c, LB_ERR_UNSUPPORTED, "tagged boundaries are implemented for LB_FP64 contexts");
+  if (c->n_ghost > 0)
+    return fail(c, LB_ERR_UNSUPPORTED, "tagged boundaries are implemented for one-rank node lists (no halo)");
   if (!neighbor || !tags || !interior_domains || !force_on || n_nodes != c->n_nodes)
     return fail(c, LB_ERR_ARG, "lb_set_node_tags: bad argument (arrays over the %d nodes of the topology)",
                 c->n_nodes);
@@ -1088,6 +1118,10 @@ int lb_step_cg(lb_ctx* c, int32_t nsteps, double tau0, double tau1, double sigma
   if (!c) return LB_ERR_ARG;
   int rc = check_ready(c, 2);
   if (rc != LB_OK) return rc;
+  if (c->n_ghost > 0)
+    return fail(c, LB_ERR_UNSUPPORTED,
+                "colour-gradient steps with other-rank neighbours would need the colour of the halo nodes "
+                "between the two passes; the library exchanges populations only");
   if (nsteps < 0 || !force || !(tau0 > 0.5) || !(tau1 > 0.5) ||
       (force_mode != LB_FORCE_FIXED && force_mode != LB_FORCE_MOMX))
     return fail(c, LB_ERR_ARG, "lb_step_cg: bad argument");
@@ -1166,6 +1200,10 @@ int lb_solve_relperm(lb_ctx* c, double tau0, double tau1, double sigma, double b
   if (!c) return LB_ERR_ARG;
   int rc = check_ready(c, 2);
   if (rc != LB_OK) return rc;
+  if (c->n_ghost > 0)
+    return fail(c, LB_ERR_UNSUPPORTED,
+                "colour-gradient steps with other-rank neighbours would need the colour of the halo nodes "
+                "between the two passes; the library exchanges populations only");
   if (!force || !(tau0 > 0.5) || !(tau1 > 0.5) || check_every < 1 || !(tol > 0.0) || max_steps < 1 ||
       !(n_box > 0.0) || force[0] == 0.0)
     return fail(c, LB_ERR_ARG, "lb_solve_relperm: bad argument");
@@ -1417,5 +1455,131 @@ int lb_write_vector(lb_ctx* c, int32_t which, const char* prefix) {
   if (rc != LB_OK) return rc;
   return write_file(c, std::string(prefix) + ".lbvec", {1, c->nD, c->n_nodes}, v);
 }
+
+// ------------------------------------------------------------ halo exchange
+int lb_halo_add(lb_ctx* c, int32_t neighbor_rank, int32_t n_send, const int32_t* nodes_to_send,
+                const int32_t* n_dir_send, const int32_t* dir_send, int32_t n_recv, const int32_t* nodes_received,
+                const int32_t* n_dir_recv, const int32_t* dir_recv, int32_t* index) {
+  if (!c) return LB_ERR_ARG;
+  if (!c->has_topo) return fail(c, LB_ERR_STATE, "no topology: call lb_set_topology first");
+  if (n_send < 0 || n_recv < 0 || (n_send > 0 && (!nodes_to_send || !n_dir_send)) ||
+      (n_recv > 0 && (!nodes_received || !n_dir_recv)) || !index)
+    return fail(c, LB_ERR_ARG, "lb_halo_add: bad argument");
+  const int Q = c->Q;
+  // element index of each value: the node's own slot of the direction
+  auto build = [&](int32_t n, const int32_t* nodes, const int32_t* ndir, const int32_t* dirs, bool own,
+                   std::vector<uint32_t>& out) -> int {
+    int64_t total = 0;
+    for (int32_t k = 0; k < n; ++k) {
+      if (ndir[k] < 0 || ndir[k] > Q - 1) return fail(c, LB_ERR_ARG, "lb_halo_add: %d directions for one node", ndir[k]);
+      total += ndir[k];
+    }
+    if (total > 0 && !dirs) return fail(c, LB_ERR_ARG, "lb_halo_add: null direction list");
+    out.clear();
+    out.reserve(static_cast<size_t>(total));
+    int64_t at = 0;
+    for (int32_t k = 0; k < n; ++k) {
+      const int32_t node = nodes[k];
+      if (node <= 0 || node >= c->n_nodes)
+        return fail(c, LB_ERR_ARG, "lb_halo_add: node %d out of range", node);
+      const int32_t slot = c->slot_of[node];
+      if (own ? !(slot >= 0 && slot < c->n_bulk) : !(slot >= c->n_bulk))
+        return fail(c, LB_ERR_ARG, own ? "lb_halo_add: node %d to send is not a bulk node of this rank"
+                                       : "lb_halo_add: node %d to receive is not an other-rank (halo) node",
+                    node);
+      for (int32_t m = 0; m < ndir[k]; ++m, ++at) {
+        const int32_t q = dirs[at];
+        if (q < 0 || q >= Q - 1) return fail(c, LB_ERR_ARG, "lb_halo_add: direction %d", q);
+        out.push_back(pidx_host(Q, static_cast<uint32_t>(q), static_cast<uint32_t>(slot)));
+      }
+    }
+    return LB_OK;
+  };
+  std::vector<uint32_t> es, er;
+  int rc = build(n_send, nodes_to_send, n_dir_send, dir_send, true, es);
+  if (rc != LB_OK) return rc;
+  rc = build(n_recv, nodes_received, n_dir_recv, dir_recv, false, er);
+  if (rc != LB_OK) return rc;
+  hipSetDevice(c->device);
+  lb_ctx::Halo h;
+  h.rank = neighbor_rank;
+  h.n_send = static_cast<int64_t>(es.size());
+  h.n_recv = static_cast<int64_t>(er.size());
+  const size_t nbuf = 2 * static_cast<size_t>(std::max<int64_t>(std::max(h.n_send, h.n_recv), 1));
+  hipError_t e = hipMalloc(&h.d_send, std::max<size_t>(es.size(), 1) * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&h.d_recv, std::max<size_t>(er.size(), 1) * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&h.d_buf, nbuf * sizeof(double));
+  if (e == hipSuccess && !es.empty())
+    e = hipMemcpy(h.d_send, es.data(), es.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !er.empty())
+    e = hipMemcpy(h.d_recv, er.data(), er.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    for (void* p : {static_cast<void*>(h.d_send), static_cast<void*>(h.d_recv), static_cast<void*>(h.d_buf)})
+      if (p) hipFree(p);
+    return fail(c, LB_ERR_NOMEM, "lb_halo_add: %s", hipGetErrorString(e));
+  }
+  c->halos.push_back(h);
+  *index = static_cast<int32_t>(c->halos.size()) - 1;
+  return LB_OK;
+}
+
+int lb_halo_sizes(const lb_ctx* c, int32_t index, int32_t* neighbor_rank, int64_t* n_send, int64_t* n_recv) {
+  if (!c || index < 0 || index >= static_cast<int32_t>(c->halos.size())) return LB_ERR_ARG;
+  const auto& h = c->halos[index];
+  if (neighbor_rank) *neighbor_rank = h.rank;
+  if (n_send) *n_send = h.n_send;
+  if (n_recv) *n_recv = h.n_recv;
+  return LB_OK;
+}
+
+namespace {
+int halo_check(lb_ctx* c, int32_t index, int32_t nf, const void* buf) {
+  if (index < 0 || index >= static_cast<int32_t>(c->halos.size()))
+    return fail(c, LB_ERR_ARG, "no halo exchange %d", index);
+  if (!buf) return fail(c, LB_ERR_ARG, "null buffer");
+  if (c->n_fields == 0) return fail(c, LB_ERR_STATE, "no populations");
+  if (nf < 1 || nf > c->n_fields) return fail(c, LB_ERR_ARG, "%d fields of %d", nf, c->n_fields);
+  if (!c->post)
+    return fail(c, LB_ERR_STATE,
+                "the halo carries post-collision values: exchange after a step (the first step after an upload "
+                "reads no halo)");
+  return LB_OK;
+}
+}  // namespace
+
+int lb_halo_pack(lb_ctx* c, int32_t index, int32_t nf, double* send) {
+  if (!c) return LB_ERR_ARG;
+  int rc = halo_check(c, index, nf, send);
+  if (rc != LB_OK) return rc;
+  hipSetDevice(c->device);
+  const auto& h = c->halos[index];
+  hipError_t e = halo_pack(c->precision, c->n_fields, nf, c->d_f[c->cur], h.d_send, h.n_send, h.d_buf, c->stream);
+  if (e == hipSuccess && h.n_send > 0)
+    e = hipMemcpyAsync(send, h.d_buf, static_cast<size_t>(nf) * h.n_send * sizeof(double), hipMemcpyDefault,
+                       c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return fail(c, LB_ERR_HIP, "lb_halo_pack: %s", hipGetErrorString(e));
+  return LB_OK;
+}
+
+int lb_halo_unpack(lb_ctx* c, int32_t index, int32_t nf, const double* recv) {
+  if (!c) return LB_ERR_ARG;
+  int rc = halo_check(c, index, nf, recv);
+  if (rc != LB_OK) return rc;
+  hipSetDevice(c->device);
+  const auto& h = c->halos[index];
+  hipError_t e = hipSuccess;
+  if (h.n_recv > 0)
+    e = hipMemcpyAsync(h.d_buf, recv, static_cast<size_t>(nf) * h.n_recv * sizeof(double), hipMemcpyDefault,
+                       c->stream);
+  if (e == hipSuccess)
+    e = halo_unpack(c->precision, c->n_fields, nf, c->d_f[c->cur], h.d_recv, h.n_recv, h.d_buf, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return fail(c, LB_ERR_HIP, "lb_halo_unpack: %s", hipGetErrorString(e));
+  return LB_OK;
+}
+
+int32_t lb_halo_count(const lb_ctx* c) { return c ? static_cast<int32_t>(c->halos.size()) : -1; }
+int32_t lb_halo_num_nodes(const lb_ctx* c) { return c && c->has_topo ? c->n_ghost : -1; }
 
 }  // extern "C"

@@ -30,6 +30,21 @@ struct lb_ctx {
   int32_t* d_rank = nullptr;
   std::vector<std::array<int32_t, 3>> bulk_runs;
   std::vector<int32_t> wall_nodes;  // wall slot -> node number
+  // other-rank fluid nodes (type 4, the halo): population slots after the
+  // bulk nodes in compact order; slot_of[node] = compact index (bulk, then
+  // halo) or -1
+  int32_t n_ghost = 0;
+  std::vector<int32_t> slot_of;
+  // exchanges with neighbouring ranks (lb_halo_add): element index per value
+  // sent / received, and a device staging buffer of both fields
+  struct Halo {
+    int32_t rank = -1;
+    int64_t n_send = 0, n_recv = 0;
+    uint32_t* d_send = nullptr;
+    uint32_t* d_recv = nullptr;
+    double* d_buf = nullptr;
+  };
+  std::vector<Halo> halos;
   std::vector<int16_t> node_type;
   uint32_t* d_link = nullptr;       // pull table in compact order (both layouts)
   uint32_t* d_coord = nullptr;      // row layout: packed cell coordinates

@@ -98,4 +98,11 @@ hipError_t sum_partials(const double* partial, int64_t nblocks, int ncomp, doubl
 hipError_t finish_momx(const double* partial, int64_t nblocks, double momx,
                        int64_t n_global, double* dscalar, hipStream_t s);
 
+// Halo exchange (halo.hip): out[k n + j] = field k (< nf of nf_total) of the
+// element elem[j], as the double of the stored value; unpack writes back.
+hipError_t halo_pack(int precision, int nf_total, int nf, const void* f, const uint32_t* elem, int64_t n,
+                     double* out, hipStream_t s);
+hipError_t halo_unpack(int precision, int nf_total, int nf, void* f, const uint32_t* elem, int64_t n,
+                       const double* in, hipStream_t s);
+
 }  // namespace lbamd

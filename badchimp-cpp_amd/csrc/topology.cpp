@@ -638,6 +638,22 @@ int lb_topo_node_ranks(const lb_topo* t, int32_t* out) {
   std::copy(t->node_rank.begin(), t->node_rank.end(), out);
   return LB_OK;
 }
+int32_t lb_topo_num_neighbor_ranks(const lb_topo* t) { return t ? static_cast<int32_t>(t->processors.size()) : -1; }
+
+int lb_topo_neighbor_rank(const lb_topo* t, int32_t k, int32_t* rank, int32_t* n_nodes, int32_t* local,
+                          int32_t* remote) {
+  if (!t || k < 0 || k >= static_cast<int32_t>(t->processors.size()))
+    return fail_global(LB_ERR_ARG, "lb_topo_neighbor_rank: no neighbouring rank %d", k);
+  const auto& blk = t->processors[static_cast<size_t>(k)];
+  if (rank) *rank = blk.first;
+  if (n_nodes) *n_nodes = static_cast<int32_t>(blk.second.size());
+  for (size_t i = 0; i < blk.second.size(); ++i) {
+    if (local) local[i] = blk.second[i].first;
+    if (remote) remote[i] = blk.second[i].second;
+  }
+  return LB_OK;
+}
+
 int lb_topo_bulk(const lb_topo* t, int32_t* out) {
   if (!t || !out) return fail_global(LB_ERR_ARG, "null argument");
   std::copy(t->bulk.begin(), t->bulk.end(), out);

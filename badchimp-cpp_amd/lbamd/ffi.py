@@ -52,6 +52,14 @@ SIGNATURES = {
     "lb_layout": (C.c_int32, [_vp]),
     "lb_layout_note": (C.c_char_p, [_vp]),
     "lb_set_global_bulk_count": (C.c_int, [_vp, C.c_int64]),
+    "lb_halo_add": (C.c_int, [_vp, C.c_int32, C.c_int32, _i32p, _i32p, _i32p, C.c_int32, _i32p, _i32p, _i32p,
+                              C.POINTER(C.c_int32)]),
+    "lb_halo_sizes": (C.c_int, [_vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
+                                C.POINTER(C.c_int64)]),
+    "lb_halo_pack": (C.c_int, [_vp, C.c_int32, C.c_int32, _f64p]),
+    "lb_halo_unpack": (C.c_int, [_vp, C.c_int32, C.c_int32, _f64p]),
+    "lb_halo_count": (C.c_int32, [_vp]),
+    "lb_halo_num_nodes": (C.c_int32, [_vp]),
     "lb_upload_f": (C.c_int, [_vp, C.c_int32, _f64p]),
     "lb_download_f": (C.c_int, [_vp, C.c_int32, _f64p]),
     "lb_download_f_noalloc": (C.c_int, [_vp, C.c_int32, _f64p]),
@@ -109,6 +117,8 @@ SIGNATURES = {
     "lb_topo_positions": (C.c_int, [_vp, _i32p]),
     "lb_topo_node_types": (C.c_int, [_vp, _i16p]),
     "lb_topo_node_ranks": (C.c_int, [_vp, _i32p]),
+    "lb_topo_num_neighbor_ranks": (C.c_int32, [_vp]),
+    "lb_topo_neighbor_rank": (C.c_int, [_vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _vp, _vp]),
     "lb_topo_bulk": (C.c_int, [_vp, _i32p]),
     "lb_topo_attribute": (C.c_int, [_vp, C.c_char_p, C.c_int32, _f64p]),
     "lb_topo_write_vtklb": (C.c_int, [_vp, C.c_char_p, C.c_int32, C.POINTER(C.c_char_p),
@@ -295,6 +305,20 @@ class Topology:
         _check_global(lib().lb_topo_node_ranks(self._h, out))
         return out
 
+    def neighbor_ranks(self):
+        """PROCESSOR blocks: [(rank, local node numbers, node numbers on that
+        rank)] (LBvtk::getNeigRank / getNeigNodesNo / getNeigNodesNeigNo)."""
+        out = []
+        for k in range(lib().lb_topo_num_neighbor_ranks(self._h)):
+            r, n = C.c_int32(), C.c_int32()
+            _check_global(lib().lb_topo_neighbor_rank(self._h, k, C.byref(r), C.byref(n), None, None))
+            loc = np.zeros(max(n.value, 1), dtype=np.int32)
+            rem = np.zeros(max(n.value, 1), dtype=np.int32)
+            _check_global(lib().lb_topo_neighbor_rank(self._h, k, None, None, loc.ctypes.data_as(C.c_void_p),
+                                                      rem.ctypes.data_as(C.c_void_p)))
+            out.append((r.value, loc[: n.value], rem[: n.value]))
+        return out
+
     def bulk(self):
         out = np.zeros(max(self.n_bulk, 1), dtype=np.int32)
         _check_global(lib().lb_topo_bulk(self._h, out))
@@ -391,6 +415,40 @@ class Context:
 
     def set_global_bulk_count(self, n):
         self._check(lib().lb_set_global_bulk_count(self._h, int(n)))
+
+    # halo exchange between ranks
+    @property
+    def halo_nodes(self):
+        return lib().lb_halo_num_nodes(self._h)
+
+    def halo_add(self, neighbor_rank, nodes_to_send, n_dir_send, dir_send, nodes_received, n_dir_recv, dir_recv):
+        """Register the exchange with one neighbouring rank (the MonLatMpi
+        lists); returns its index."""
+        a = [np.ascontiguousarray(x, dtype=np.int32) for x in (nodes_to_send, n_dir_send, dir_send,
+                                                                nodes_received, n_dir_recv, dir_recv)]
+        a = [x if x.size else np.zeros(1, dtype=np.int32) for x in a]
+        k = C.c_int32()
+        self._check(lib().lb_halo_add(self._h, int(neighbor_rank), len(nodes_to_send), a[0], a[1], a[2],
+                                      len(nodes_received), a[3], a[4], a[5], C.byref(k)))
+        return k.value
+
+    def halo_sizes(self, k):
+        r, ns, nr = C.c_int32(), C.c_int64(), C.c_int64()
+        self._check(lib().lb_halo_sizes(self._h, int(k), C.byref(r), C.byref(ns), C.byref(nr)))
+        return r.value, ns.value, nr.value
+
+    def halo_pack(self, k, nf=1):
+        """The values this rank sends to exchange k: array (nf, n_send)."""
+        _, ns, _ = self.halo_sizes(k)
+        out = np.zeros(max(nf * ns, 1))
+        self._check(lib().lb_halo_pack(self._h, int(k), int(nf), out))
+        return out[: nf * ns].reshape(nf, ns)
+
+    def halo_unpack(self, k, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        nf = v.shape[0] if v.ndim == 2 else 1
+        flat = v.ravel() if v.size else np.zeros(1)
+        self._check(lib().lb_halo_unpack(self._h, int(k), int(nf), flat))
 
     # fields
     def upload_f(self, f):

@@ -49,7 +49,7 @@ enum lb_status {
   LB_ERR_ARG = -1,           /* invalid argument */
   LB_ERR_STATE = -2,         /* call out of order (e.g. step before topology) */
   LB_ERR_GHOST0_LINK = -3,   /* a bulk node links to the ghost node 0 */
-  LB_ERR_UNSUPPORTED = -4,   /* e.g. other-rank fluid neighbours on one GPU */
+  LB_ERR_UNSUPPORTED = -4,   /* e.g. a two-phase step on a context with halo nodes */
   LB_ERR_HIP = -5,           /* HIP runtime error */
   LB_ERR_IO = -6,            /* file could not be read or parsed */
   LB_ERR_NOMEM = -7          /* device or host allocation failed */
@@ -112,6 +112,42 @@ const char* lb_layout_note(const lb_ctx* ctx);
 /* Total bulk-node count over all ranks, the divisor of the flux mean
  * (main_TWOPHASE.cpp:197-199,301).  Defaults to n_bulk. */
 int lb_set_global_bulk_count(lb_ctx* ctx, int64_t n);
+
+/* ---- Halo exchange between ranks (one subdomain per context) ------------
+ * A node list split over ranks (lb_topo_from_geometry of a geometry whose
+ * fluid cells carry rank + 1, or a rank's .vtklb) holds other-rank fluid
+ * nodes (type 4).  lb_set_topology
+ * gives each of them a population slot after the bulk nodes, and the bulk
+ * nodes gather from those slots through the link table (such a context
+ * uses the link-table layout).  Between two steps every rank fills the halo
+ * of its neighbours: as communicateLbField (LBmonlatmpi.h:236-300), it sends
+ * the post-collision value of every listed (node, direction) and writes
+ * what it receives into its halo slots.  The transport (MPI, RCCL, a device
+ * copy) is the caller's.  Single-phase steps (BGK, TRT, rheology) read the
+ * halo; colour-gradient steps and tagged boundaries refuse a context with
+ * halo nodes (LB_ERR_UNSUPPORTED).
+ *
+ * lb_halo_add registers the exchange with one neighbouring rank from the
+ * lists MonLatMpi holds after BndMpi::setup (LBbndmpi.h:206-315, the
+ * addMpiBnd arguments): nodes_to_send (bulk nodes of this rank), n_dir_send
+ * per node and dir_send (the directions, concatenated); nodes_received (this
+ * rank's halo nodes of that rank), n_dir_recv and dir_recv.  *index is the
+ * exchange's number.  lb_halo_pack writes the n_send values of the nf first
+ * fields into send[f * n_send + j] (j in list order), lb_halo_unpack reads
+ * recv[f * n_recv + j]; buffers are host or device memory.  The values are
+ * the stored ones as double: f in LB_FP64, f - w_q in LB_FP32S and
+ * LB_FP16S, so contexts of one precision exchange them exactly.  Both need a
+ * post-collision state (after a step; LB_ERR_STATE otherwise), and the
+ * first step after an upload reads no halo. */
+int lb_halo_add(lb_ctx* ctx, int32_t neighbor_rank, int32_t n_send, const int32_t* nodes_to_send,
+                const int32_t* n_dir_send, const int32_t* dir_send, int32_t n_recv,
+                const int32_t* nodes_received, const int32_t* n_dir_recv, const int32_t* dir_recv,
+                int32_t* index);
+int lb_halo_sizes(const lb_ctx* ctx, int32_t index, int32_t* neighbor_rank, int64_t* n_send, int64_t* n_recv);
+int lb_halo_pack(lb_ctx* ctx, int32_t index, int32_t nf, double* send);
+int lb_halo_unpack(lb_ctx* ctx, int32_t index, int32_t nf, const double* recv);
+int32_t lb_halo_count(const lb_ctx* ctx);      /* exchanges registered */
+int32_t lb_halo_num_nodes(const lb_ctx* ctx);  /* halo (other-rank) nodes with a slot */
 
 /* ------------------------------------------------------------ fields */
 /* Populations of n_fields fields (1: single phase, 2: two phase) in the
@@ -310,6 +346,14 @@ int lb_topo_positions(const lb_topo* t, int32_t* pos);        /* [n_nodes*nD], n
 int lb_topo_node_types(const lb_topo* t, int16_t* types);     /* [n_nodes] */
 int lb_topo_node_ranks(const lb_topo* t, int32_t* ranks);     /* [n_nodes] */
 int lb_topo_bulk(const lb_topo* t, int32_t* bulk);            /* [n_bulk] */
+/* PROCESSOR blocks of a node list split over ranks: neighbouring rank k's
+ * rank, and its nodes held here as (local node number, node number on that
+ * rank) pairs.  Replaces LBvtk::getNumNeigProc / getNeigRank /
+ * getNeigNodesNo / getNeigNodesNeigNo (LBvtk.h:152-173).  local / remote may
+ * be null (n_nodes alone gives the sizes). */
+int32_t lb_topo_num_neighbor_ranks(const lb_topo* t);
+int lb_topo_neighbor_rank(const lb_topo* t, int32_t k, int32_t* rank, int32_t* n_nodes, int32_t* local,
+                          int32_t* remote);
 /* Point attribute of a .vtklb file (SCALARS block), as double over n_nodes
  * (node 0 = 0); as_float rounds each value through float first, the way
  * main_TWOPHASE.cpp:158,166,173 reads rho0/rho1/wettability. */

@@ -1,0 +1,59 @@
+"""Halo exchange lists of a domain split over ranks (lbamd.halo.exchange_lists,
+the lists BndMpi::setup builds, LBbndmpi.h:206-315).  For every pair of
+ranks, what one rank sends is what the other receives: the k-th value sent
+is the value of the same cell (up to the periodic wrap of the halo) and the
+same direction as the k-th value received, and every direction listed
+points from the halo node into the receiving rank."""
+import numpy as np
+import pytest
+
+import lbamd
+from lbamd.halo import exchange_lists
+
+
+def split(shape, n_ranks, periodic, seed, lattice):
+    if lattice == lbamd.D3Q19:
+        geo, _ = lbamd.sphere_pack(shape, 3.0, 0.45, seed, periodic=periodic, walls=True)
+    else:
+        rng = np.random.default_rng(seed)
+        geo = (rng.random(shape) > 0.3).astype(np.int32)
+        geo[:, 0] = 0
+        geo[:, -1] = 0
+    g = np.asarray(geo, np.int32).copy()
+    cut = np.linspace(0, shape[0], n_ranks + 1).astype(int)
+    for r in range(n_ranks):
+        sl = g[cut[r]:cut[r + 1]]
+        sl[sl > 0] = r + 1
+    return g
+
+
+@pytest.mark.parametrize("lattice,shape,n_ranks", [(lbamd.D3Q19, (24, 16, 12), 2), (lbamd.D3Q19, (30, 14, 12), 3),
+                                                   (lbamd.D2Q9, (40, 18), 2)])
+def test_exchange_lists_pair_up(lattice, shape, n_ranks):
+    g = split(shape, n_ranks, "x", 5, lattice)
+    topos = [lbamd.Topology.from_geometry(g, lattice, "x", rank=r + 1) for r in range(n_ranks)]
+    lists = exchange_lists(topos)
+    pos = [t.positions() for t in topos]
+    ranks = [t.node_ranks() for t in topos]
+    nbs = [t.neighbors() for t in topos]
+    period = shape[0]
+    n_pairs = 0
+    for r in range(n_ranks):
+        assert lists[r], "rank %d has no neighbour" % r
+        for (s, ns, nds, ds, nr, ndr, dr) in lists[r]:
+            # r's receive side: halo nodes of rank s, directions into rank r
+            assert np.all(ranks[r][nr] == s)
+            recv_nodes = np.repeat(nr, ndr)
+            assert np.all(ranks[r][nbs[r][recv_nodes, dr]] == r)
+            # r's send side: its own bulk nodes
+            assert np.all(np.isin(ns, topos[r].bulk()))
+            # s receives exactly what r sends, value by value
+            other = [x for x in lists[s] if x[0] == r]
+            assert len(other) == 1
+            _, _, _, _, nr2, ndr2, dr2 = other[0]
+            a, b = np.repeat(ns, nds), np.repeat(nr2, ndr2)
+            assert len(a) == len(b) == len(ds) and np.array_equal(ds, dr2)
+            d = pos[r][a] - pos[s][b]
+            assert np.all(d[:, 1:] == 0) and np.all(d[:, 0] % period == 0)
+            n_pairs += 1
+    assert n_pairs == sum(len(x) for x in lists)
