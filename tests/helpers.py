@@ -62,9 +62,9 @@ def ref_names(prefix=""):
 
 
 def std_ref_names():
-    """The BGK std_case fixtures (not two-phase, two-relaxation-time, rheology
-    or std_one_phase)."""
-    return [n for n in ref_names() if not n.startswith(("tp_", "trt_", "rheo_", "op_"))]
+    """The one-rank BGK std_case fixtures (not two-phase, two-relaxation-time,
+    rheology, std_one_phase or the multi-rank runs)."""
+    return [n for n in ref_names() if not n.startswith(("tp_", "trt_", "rheo_", "op_", "mpi_"))]
 
 
 def float_round(a):

@@ -8,12 +8,16 @@ D2Q9, two and three ranks over the periodic axis.  The per-node arithmetic
 is the same in both runs; only where a gather finds a neighbour's value
 differs (a halo slot filled by the exchange instead of the neighbour's own
 slot).  The whole-domain context is pinned to the compiled reference
-elsewhere (test_gpu_parity.py), so the split run inherits that pin.
+elsewhere (test_gpu_parity.py).  The split run is also pinned directly: the
+compiled reference std_case run on 2 and 3 MPI ranks (its own halo exchange;
+fixtures from tests/golden/make_mpi_fixtures.py) gives every rank's rho and
+u bit for bit.
 """
 import numpy as np
 import pytest
 
 import lbamd
+from helpers import load_ref, ref_names
 from lbamd.halo import exchange, exchange_lists
 
 pytestmark = pytest.mark.gpu
@@ -118,6 +122,43 @@ def test_split_equals_whole(gpu_available, precision, lattice, shape, n_ranks, o
         assert np.array_equal(f, wf[k])
         assert np.array_equal(rho, wr[k])
         assert np.array_equal(vel, wv[k])
+
+
+@pytest.mark.parametrize("name", ref_names("mpi_"))
+def test_split_vs_reference_mpi_run(gpu_available, name):
+    """The compiled reference std_case run on R MPI ranks (its own halo
+    exchange, tests/golden/make_mpi_fixtures.py) against R contexts on one
+    GPU exchanging through lb_halo_pack / lb_halo_unpack: every rank's rho
+    and u of the last iteration, bit for bit."""
+    d = load_ref(name)
+    p = d["params"]
+    lattice = {"D2Q9": lbamd.D2Q9, "D3Q19": lbamd.D3Q19}[p["lattice"]]
+    ranks = p["ranks"]
+    topos = [lbamd.Topology.from_geometry(d["geo"], lattice, p["periodic"], rank=r + 1) for r in range(ranks)]
+    lists = exchange_lists(topos)
+    W = {lbamd.D3Q19: np.array([2.0] * 3 + [1.0] * 6 + [2.0] * 3 + [1.0] * 6 + [12.0]) / 36.0,
+         lbamd.D2Q9: np.array([4.0, 1.0] * 4 + [16.0]) / 36.0}[lattice]
+    ctxs, reg = [], []
+    for r, t in enumerate(topos):
+        ctx = lbamd.Context(lattice, lbamd.FP64)
+        ctx.set_topology(t.neighbors(), t.node_types(), t.bulk())
+        f0 = np.zeros((t.n_nodes, W.size))
+        f0[t.bulk()] = W  # std_case/main.cpp:67-96 with init_rho = 1
+        ctx.upload_f(f0)
+        reg.append({x[0]: ctx.halo_add(*x) for x in lists[r]})
+        ctxs.append(ctx)
+    force = np.asarray(p["force"], dtype=float)
+    for _ in range(p["steps"] + 1):  # the last iteration's stored fields (std_case/main.cpp:109-155)
+        for ctx in ctxs:
+            ctx.step_bgk(1, p["tau"], force)
+        exchange(ctxs, reg)
+    for r, (t, ctx) in enumerate(zip(topos, ctxs)):
+        b = t.bulk()
+        rho, vel = ctx.download_scalar(lbamd.RHO)[b], ctx.download_vel()[b]
+        assert np.array_equal(rho, d["rho_%d" % r]), (name, r)
+        assert np.array_equal(vel, d["vel_%d" % r][:, : vel.shape[1]]), (name, r)
+        ctx.close()
+        t.close()
 
 
 def test_halo_refusals(gpu_available):
