@@ -119,7 +119,8 @@ void free_tags(lb_ctx* c) {
 
 void free_halos(lb_ctx* c) {
   for (auto& h : c->halos)
-    for (void* p : {static_cast<void*>(h.d_send), static_cast<void*>(h.d_recv), static_cast<void*>(h.d_buf)})
+    for (void* p : {static_cast<void*>(h.d_send), static_cast<void*>(h.d_recv), static_cast<void*>(h.d_send_nodes),
+                    static_cast<void*>(h.d_recv_nodes), static_cast<void*>(h.d_buf)})
       if (p) hipFree(p);
   c->halos.clear();
 }
@@ -145,6 +146,7 @@ int alloc_fields(lb_ctx* c, int nf) {
   c->cur = 0;
   c->post = false;
   c->step_count = 0;
+  c->cg_colour_done = false;
   return LB_OK;
 }
 
@@ -625,6 +627,7 @@ int lb_upload_f(lb_ctx* c, int32_t nf, const double* f) {
   c->cur = 0;
   c->post = false;
   c->step_count = 0;
+  c->cg_colour_done = false;
   c->have_macro = false;
   c->have_visc = false;  // the viscosity slot holds no rheology result now
   return LB_OK;
@@ -705,6 +708,7 @@ int lb_init_equilibrium(lb_ctx* c, int32_t n_fields, int32_t field, int32_t rho_
       rho_field < 0 || rho_field >= rho_nfields || !rho || !vel)
     return fail(c, LB_ERR_ARG, "lb_init_equilibrium: bad argument");
   hipSetDevice(c->device);
+  c->cg_colour_done = false;
   if (c->n_fields != n_fields || c->post) {
     // (re)start from an unset state: all fields are to be initialised; the
     // fresh buffer starts at zero, so that a field not yet initialised holds
@@ -866,6 +870,7 @@ int step_single(lb_ctx* c, int32_t nsteps, const BgkArgs& a) {
     c->cur = 1 - c->cur;
     c->post = true;
     ++c->step_count;
+    c->cg_colour_done = false;
   }
   const int rc = t.stop(nsteps);
   if (rc != LB_OK) return rc;
@@ -1067,6 +1072,7 @@ int lb_step_tagged(lb_ctx* c, int32_t nsteps, double tau, const double* force, d
     c->cur = 1 - c->cur;
     c->post = true;
     ++c->step_count;
+    c->cg_colour_done = false;
     ++launches;
   }
   rc = t.stop(launches);
@@ -1113,20 +1119,16 @@ int lb_step_trt(lb_ctx* c, int32_t nsteps, double tau_sym, double tau_anti, cons
   return step_single(c, nsteps, single_phase_args(c, tau_sym, tau_anti, force));
 }
 
-int lb_step_cg(lb_ctx* c, int32_t nsteps, double tau0, double tau1, double sigma, double beta,
-               int32_t force_mode, const double* force) {
-  if (!c) return LB_ERR_ARG;
-  int rc = check_ready(c, 2);
-  if (rc != LB_OK) return rc;
-  if (c->n_ghost > 0)
-    return fail(c, LB_ERR_UNSUPPORTED,
-                "colour-gradient steps with other-rank neighbours would need the colour of the halo nodes "
-                "between the two passes; the library exchanges populations only");
-  if (nsteps < 0 || !force || !(tau0 > 0.5) || !(tau1 > 0.5) ||
-      (force_mode != LB_FORCE_FIXED && force_mode != LB_FORCE_MOMX))
-    return fail(c, LB_ERR_ARG, "lb_step_cg: bad argument");
-  if (nsteps == 0) return LB_OK;
-  hipSetDevice(c->device);
+}  // extern "C"
+
+namespace {
+
+bool cg_args_ok(double tau0, double tau1, int32_t force_mode, const double* force) {
+  return force && tau0 > 0.5 && tau1 > 0.5 && (force_mode == LB_FORCE_FIXED || force_mode == LB_FORCE_MOMX);
+}
+
+CgArgs cg_args(const lb_ctx* c, double tau0, double tau1, double sigma, double beta, int32_t force_mode,
+               const double* force) {
   CgArgs a{};
   const double c2 = c->lattice == LB_D2Q9 ? D2Q9::c2 : D3Q19::c2;
   a.nu0Inv = 1.0 / (c2 * (tau0 - 0.5));
@@ -1138,59 +1140,140 @@ int lb_step_cg(lb_ctx* c, int32_t nsteps, double tau0, double tau1, double sigma
   for (int d = 0; d < 3; ++d) a.F[d] = d < c->nD ? force[d] : 0.0;
   a.momx_mode = force_mode == LB_FORCE_MOMX;
   a.momx = force[0];
-  const bool momx = a.momx_mode != 0;
-  // row windows when every solid neighbour has one colour and both fields fit
-  // the 32-bit byte offsets of that layout; else the link table
-  const bool rowwin = c->addr.layout == 2 && c->wall_uniform &&
-                      rowwin_fits(c->Q, c->NP, 2, static_cast<int64_t>(elem_size(c->precision)));
+  return a;
+}
+
+// Row windows when every solid neighbour has one colour and both fields fit
+// the 32-bit byte offsets of that layout; else the link table.
+bool cg_rowwin(const lb_ctx* c) {
+  return c->addr.layout == 2 && c->wall_uniform &&
+         rowwin_fits(c->Q, c->NP, 2, static_cast<int64_t>(elem_size(c->precision)));
+}
+
+// The colour pass of the iteration at step_count: phi of every bulk node,
+// and with momx the per-block partial sums of the x-momentum.  The phi
+// buffer holds float values in the reduced-precision forms.
+hipError_t colour_pass(lb_ctx* c, bool rowwin, bool momx) {
+  if (c->precision == LB_FP64)
+    return cg_phi_f64(c->lattice, !c->post, momx, static_cast<const double*>(c->d_f[c->cur]), c->addr, rowwin,
+                      c->n_bulk, c->NP, c->d_phi, c->d_partial, c->stream);
+  return cg_phi_lowp(c->precision, c->lattice, !c->post, momx, c->d_f[c->cur], c->addr, rowwin, c->n_bulk, c->NP,
+                     reinterpret_cast<float*>(c->d_phi), c->d_partial, c->stream);
+}
+
+// The collision pass of the same iteration; macroscopic fields stored when
+// `store`.  The caller then advances the state (advance_cg).
+hipError_t collision_pass(lb_ctx* c, bool rowwin, bool store, CgArgs a) {
   const WallPhi wp{c->d_phi_wall, c->wall_phi};
-  Timer t(c);
-  int64_t launches = 0;
-  for (int32_t s = 0; s < nsteps; ++s) {
-    const bool last = s == nsteps - 1;
-    hipError_t e;
-    if (c->precision == LB_FP64) {
-      const double* fin = static_cast<const double*>(c->d_f[c->cur]);
-      double* fout = static_cast<double*>(c->d_f[1 - c->cur]);
-      e = cg_phi_f64(c->lattice, !c->post, momx, fin, c->addr, rowwin, c->n_bulk, c->NP, c->d_phi, c->d_partial,
-                     c->stream);
-      if (e == hipSuccess && momx)
-        e = finish_momx(c->d_partial, c->n_blocks, a.momx, c->n_global, c->d_scalar, c->stream);
-      if (e == hipSuccess)
-        e = cg_f64(c->lattice, !c->post, last, fin, fout, c->addr, rowwin, c->n_bulk, c->NP, a, c->d_phi, wp,
-                   c->d_scalar, c->d_rho, c->d_vel, c->stream);
-    } else {
-      // the phi buffer holds float values in the reduced-precision forms
-      float* phi = reinterpret_cast<float*>(c->d_phi);
-      e = cg_phi_lowp(c->precision, c->lattice, !c->post, momx, c->d_f[c->cur], c->addr, rowwin, c->n_bulk,
-                      c->NP, phi, c->d_partial, c->stream);
-      if (e == hipSuccess && momx)
-        e = finish_momx(c->d_partial, c->n_blocks, a.momx, c->n_global, c->d_scalar, c->stream);
-      if (e == hipSuccess)
-      a.step = c->step_count;
-      if (e == hipSuccess)
-        e = cg_lowp(c->precision, c->lattice, !c->post, last, c->d_f[c->cur], c->d_f[1 - c->cur], c->addr, rowwin,
-                    c->n_bulk, c->NP, a, phi, wp, c->d_scalar, c->d_rho, c->d_vel, c->stream);
-    }
-    if (e != hipSuccess) return fail(c, LB_ERR_HIP, "two-phase launch: %s", hipGetErrorString(e));
-    launches += momx ? 3 : 2;
-    c->cur = 1 - c->cur;
-    c->post = true;
-    ++c->step_count;
-  }
-  rc = t.stop(launches);
-  if (rc != LB_OK) return rc;
-  if (momx) {
-    double h[2];
-    LB_HIP(c, hipMemcpy(h, c->d_scalar, 2 * sizeof(double), hipMemcpyDeviceToHost));
-    c->last_force[0] = h[0];
-  } else {
-    c->last_force[0] = a.F[0];
-  }
+  if (c->precision == LB_FP64)
+    return cg_f64(c->lattice, !c->post, store, static_cast<const double*>(c->d_f[c->cur]),
+                  static_cast<double*>(c->d_f[1 - c->cur]), c->addr, rowwin, c->n_bulk, c->NP, a, c->d_phi, wp,
+                  c->d_scalar, c->d_rho, c->d_vel, c->stream);
+  a.step = c->step_count;
+  return cg_lowp(c->precision, c->lattice, !c->post, store, c->d_f[c->cur], c->d_f[1 - c->cur], c->addr, rowwin,
+                 c->n_bulk, c->NP, a, reinterpret_cast<const float*>(c->d_phi), wp, c->d_scalar, c->d_rho, c->d_vel,
+                 c->stream);
+}
+
+void advance_cg(lb_ctx* c) {
+  c->cur = 1 - c->cur;
+  c->post = true;
+  ++c->step_count;
+  c->cg_colour_done = false;
+}
+
+void finish_cg(lb_ctx* c, double force_x, const CgArgs& a) {
+  c->last_force[0] = force_x;
   c->last_force[1] = a.F[1];
   c->last_force[2] = a.F[2];
   c->have_macro = true;
   c->have_visc = false;  // the viscosity slot holds no rheology result now
+}
+
+}  // namespace
+
+extern "C" {
+
+int lb_step_cg(lb_ctx* c, int32_t nsteps, double tau0, double tau1, double sigma, double beta,
+               int32_t force_mode, const double* force) {
+  if (!c) return LB_ERR_ARG;
+  int rc = check_ready(c, 2);
+  if (rc != LB_OK) return rc;
+  if (c->n_ghost > 0)
+    return fail(c, LB_ERR_UNSUPPORTED,
+                "colour-gradient steps with other-rank neighbours exchange the colour of the halo nodes between "
+                "the two passes: use lb_step_cg_colour, lb_halo_pack_colour / lb_halo_unpack_colour and "
+                "lb_step_cg_collide");
+  if (nsteps < 0 || !cg_args_ok(tau0, tau1, force_mode, force)) return fail(c, LB_ERR_ARG, "lb_step_cg: bad argument");
+  if (nsteps == 0) return LB_OK;
+  hipSetDevice(c->device);
+  const CgArgs a = cg_args(c, tau0, tau1, sigma, beta, force_mode, force);
+  const bool momx = a.momx_mode != 0;
+  const bool rowwin = cg_rowwin(c);
+  Timer t(c);
+  int64_t launches = 0;
+  for (int32_t s = 0; s < nsteps; ++s) {
+    hipError_t e = colour_pass(c, rowwin, momx);
+    if (e == hipSuccess && momx)
+      e = finish_momx(c->d_partial, c->n_blocks, a.momx, c->n_global, c->d_scalar, c->stream);
+    if (e == hipSuccess) e = collision_pass(c, rowwin, s == nsteps - 1, a);
+    if (e != hipSuccess) return fail(c, LB_ERR_HIP, "two-phase launch: %s", hipGetErrorString(e));
+    launches += momx ? 3 : 2;
+    advance_cg(c);
+  }
+  rc = t.stop(launches);
+  if (rc != LB_OK) return rc;
+  double fx = a.F[0];
+  if (momx) {
+    double h[2];
+    LB_HIP(c, hipMemcpy(h, c->d_scalar, 2 * sizeof(double), hipMemcpyDeviceToHost));
+    fx = h[0];
+  }
+  finish_cg(c, fx, a);
+  return LB_OK;
+}
+
+int lb_step_cg_colour(lb_ctx* c, double* flux_x) {
+  if (!c) return LB_ERR_ARG;
+  int rc = check_ready(c, 2);
+  if (rc != LB_OK) return rc;
+  hipSetDevice(c->device);
+  const bool momx = flux_x != nullptr;
+  hipError_t e = colour_pass(c, cg_rowwin(c), momx);
+  // the block partials summed as the momentum-controlled step sums them
+  // (slot 1 of the scalar buffer; slot 0 receives -2 sum, unused)
+  if (e == hipSuccess && momx) e = finish_momx(c->d_partial, c->n_blocks, 0.0, 1, c->d_scalar, c->stream);
+  if (e == hipSuccess && momx)
+    e = hipMemcpyAsync(flux_x, c->d_scalar + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess && momx) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return fail(c, LB_ERR_HIP, "colour pass: %s", hipGetErrorString(e));
+  c->cg_colour_done = true;
+  return LB_OK;
+}
+
+int lb_step_cg_collide(lb_ctx* c, double tau0, double tau1, double sigma, double beta, int32_t force_mode,
+                       const double* force, double flux_x_total) {
+  if (!c) return LB_ERR_ARG;
+  int rc = check_ready(c, 2);
+  if (rc != LB_OK) return rc;
+  if (!cg_args_ok(tau0, tau1, force_mode, force)) return fail(c, LB_ERR_ARG, "lb_step_cg_collide: bad argument");
+  if (!c->cg_colour_done)
+    return fail(c, LB_ERR_STATE, "no colour pass for this iteration: call lb_step_cg_colour first");
+  hipSetDevice(c->device);
+  CgArgs a = cg_args(c, tau0, tau1, sigma, beta, force_mode, force);
+  if (a.momx_mode) {
+    // main_TWOPHASE.cpp:300-307, the arithmetic of k_finish_momx
+    const double mean = flux_x_total / static_cast<double>(c->n_global);
+    a.F[0] = 2.0 * (a.momx - mean);
+    a.momx_mode = 0;
+  }
+  Timer t(c);
+  hipError_t e = collision_pass(c, cg_rowwin(c), true, a);
+  if (e != hipSuccess) return fail(c, LB_ERR_HIP, "two-phase launch: %s", hipGetErrorString(e));
+  advance_cg(c);
+  rc = t.stop(1);
+  if (rc != LB_OK) return rc;
+  finish_cg(c, a.F[0], a);
   return LB_OK;
 }
 
@@ -1312,6 +1395,7 @@ int solve_permeability(lb_ctx* c, const char* name, double tau, double tau_anti,
       c->cur = 1 - c->cur;
       c->post = true;
       ++c->step_count;
+      c->cg_colour_done = false;
       ++launches;
     }
     steps += chunk;
@@ -1466,9 +1550,10 @@ int lb_halo_add(lb_ctx* c, int32_t neighbor_rank, int32_t n_send, const int32_t*
       (n_recv > 0 && (!nodes_received || !n_dir_recv)) || !index)
     return fail(c, LB_ERR_ARG, "lb_halo_add: bad argument");
   const int Q = c->Q;
-  // element index of each value: the node's own slot of the direction
+  // element index of each value: the node's own slot of the direction; and
+  // the compact index of each node (its colour)
   auto build = [&](int32_t n, const int32_t* nodes, const int32_t* ndir, const int32_t* dirs, bool own,
-                   std::vector<uint32_t>& out) -> int {
+                   std::vector<uint32_t>& out, std::vector<uint32_t>& slots) -> int {
     int64_t total = 0;
     for (int32_t k = 0; k < n; ++k) {
       if (ndir[k] < 0 || ndir[k] > Q - 1) return fail(c, LB_ERR_ARG, "lb_halo_add: %d directions for one node", ndir[k]);
@@ -1477,6 +1562,8 @@ int lb_halo_add(lb_ctx* c, int32_t neighbor_rank, int32_t n_send, const int32_t*
     if (total > 0 && !dirs) return fail(c, LB_ERR_ARG, "lb_halo_add: null direction list");
     out.clear();
     out.reserve(static_cast<size_t>(total));
+    slots.clear();
+    slots.reserve(static_cast<size_t>(n));
     int64_t at = 0;
     for (int32_t k = 0; k < n; ++k) {
       const int32_t node = nodes[k];
@@ -1487,6 +1574,7 @@ int lb_halo_add(lb_ctx* c, int32_t neighbor_rank, int32_t n_send, const int32_t*
         return fail(c, LB_ERR_ARG, own ? "lb_halo_add: node %d to send is not a bulk node of this rank"
                                        : "lb_halo_add: node %d to receive is not an other-rank (halo) node",
                     node);
+      slots.push_back(static_cast<uint32_t>(slot));
       for (int32_t m = 0; m < ndir[k]; ++m, ++at) {
         const int32_t q = dirs[at];
         if (q < 0 || q >= Q - 1) return fail(c, LB_ERR_ARG, "lb_halo_add: direction %d", q);
@@ -1495,26 +1583,34 @@ int lb_halo_add(lb_ctx* c, int32_t neighbor_rank, int32_t n_send, const int32_t*
     }
     return LB_OK;
   };
-  std::vector<uint32_t> es, er;
-  int rc = build(n_send, nodes_to_send, n_dir_send, dir_send, true, es);
+  std::vector<uint32_t> es, er, ns, nr;
+  int rc = build(n_send, nodes_to_send, n_dir_send, dir_send, true, es, ns);
   if (rc != LB_OK) return rc;
-  rc = build(n_recv, nodes_received, n_dir_recv, dir_recv, false, er);
+  rc = build(n_recv, nodes_received, n_dir_recv, dir_recv, false, er, nr);
   if (rc != LB_OK) return rc;
   hipSetDevice(c->device);
   lb_ctx::Halo h;
   h.rank = neighbor_rank;
   h.n_send = static_cast<int64_t>(es.size());
   h.n_recv = static_cast<int64_t>(er.size());
-  const size_t nbuf = 2 * static_cast<size_t>(std::max<int64_t>(std::max(h.n_send, h.n_recv), 1));
-  hipError_t e = hipMalloc(&h.d_send, std::max<size_t>(es.size(), 1) * sizeof(uint32_t));
-  if (e == hipSuccess) e = hipMalloc(&h.d_recv, std::max<size_t>(er.size(), 1) * sizeof(uint32_t));
+  h.n_send_nodes = n_send;
+  h.n_recv_nodes = n_recv;
+  // both fields of the populations, or one colour per node
+  const size_t nbuf = static_cast<size_t>(std::max<int64_t>(
+      {2 * h.n_send, 2 * h.n_recv, h.n_send_nodes, h.n_recv_nodes, int64_t{1}}));
+  auto upload = [](uint32_t** d, const std::vector<uint32_t>& v) {
+    hipError_t e = hipMalloc(d, std::max<size_t>(v.size(), 1) * sizeof(uint32_t));
+    if (e == hipSuccess && !v.empty()) e = hipMemcpy(*d, v.data(), v.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+    return e;
+  };
+  hipError_t e = upload(&h.d_send, es);
+  if (e == hipSuccess) e = upload(&h.d_recv, er);
+  if (e == hipSuccess) e = upload(&h.d_send_nodes, ns);
+  if (e == hipSuccess) e = upload(&h.d_recv_nodes, nr);
   if (e == hipSuccess) e = hipMalloc(&h.d_buf, nbuf * sizeof(double));
-  if (e == hipSuccess && !es.empty())
-    e = hipMemcpy(h.d_send, es.data(), es.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
-  if (e == hipSuccess && !er.empty())
-    e = hipMemcpy(h.d_recv, er.data(), er.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
-    for (void* p : {static_cast<void*>(h.d_send), static_cast<void*>(h.d_recv), static_cast<void*>(h.d_buf)})
+    for (void* p : {static_cast<void*>(h.d_send), static_cast<void*>(h.d_recv), static_cast<void*>(h.d_send_nodes),
+                    static_cast<void*>(h.d_recv_nodes), static_cast<void*>(h.d_buf)})
       if (p) hipFree(p);
     return fail(c, LB_ERR_NOMEM, "lb_halo_add: %s", hipGetErrorString(e));
   }
@@ -1576,6 +1672,53 @@ int lb_halo_unpack(lb_ctx* c, int32_t index, int32_t nf, const double* recv) {
     e = halo_unpack(c->precision, c->n_fields, nf, c->d_f[c->cur], h.d_recv, h.n_recv, h.d_buf, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return fail(c, LB_ERR_HIP, "lb_halo_unpack: %s", hipGetErrorString(e));
+  return LB_OK;
+}
+
+}  // extern "C"
+
+namespace {
+int colour_check(lb_ctx* c, int32_t index, const void* buf) {
+  if (index < 0 || index >= static_cast<int32_t>(c->halos.size()))
+    return fail(c, LB_ERR_ARG, "no halo exchange %d", index);
+  if (!buf) return fail(c, LB_ERR_ARG, "null buffer");
+  if (!c->cg_colour_done)
+    return fail(c, LB_ERR_STATE, "the colour is exchanged between lb_step_cg_colour and lb_step_cg_collide");
+  return LB_OK;
+}
+// the colour buffer holds double in LB_FP64 and float in the other forms
+int colour_precision(const lb_ctx* c) { return c->precision == LB_FP64 ? LB_FP64 : LB_FP32S; }
+}  // namespace
+
+extern "C" {
+
+int lb_halo_pack_colour(lb_ctx* c, int32_t index, double* send) {
+  if (!c) return LB_ERR_ARG;
+  int rc = colour_check(c, index, send);
+  if (rc != LB_OK) return rc;
+  hipSetDevice(c->device);
+  const auto& h = c->halos[index];
+  hipError_t e = halo_pack(colour_precision(c), 1, 1, c->d_phi, h.d_send_nodes, h.n_send_nodes, h.d_buf, c->stream);
+  if (e == hipSuccess && h.n_send_nodes > 0)
+    e = hipMemcpyAsync(send, h.d_buf, h.n_send_nodes * sizeof(double), hipMemcpyDefault, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return fail(c, LB_ERR_HIP, "lb_halo_pack_colour: %s", hipGetErrorString(e));
+  return LB_OK;
+}
+
+int lb_halo_unpack_colour(lb_ctx* c, int32_t index, const double* recv) {
+  if (!c) return LB_ERR_ARG;
+  int rc = colour_check(c, index, recv);
+  if (rc != LB_OK) return rc;
+  hipSetDevice(c->device);
+  const auto& h = c->halos[index];
+  hipError_t e = hipSuccess;
+  if (h.n_recv_nodes > 0)
+    e = hipMemcpyAsync(h.d_buf, recv, h.n_recv_nodes * sizeof(double), hipMemcpyDefault, c->stream);
+  if (e == hipSuccess)
+    e = halo_unpack(colour_precision(c), 1, 1, c->d_phi, h.d_recv_nodes, h.n_recv_nodes, h.d_buf, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return fail(c, LB_ERR_HIP, "lb_halo_unpack_colour: %s", hipGetErrorString(e));
   return LB_OK;
 }
 

@@ -36,12 +36,16 @@ struct lb_ctx {
   int32_t n_ghost = 0;
   std::vector<int32_t> slot_of;
   // exchanges with neighbouring ranks (lb_halo_add): element index per value
-  // sent / received, and a device staging buffer of both fields
+  // sent / received, compact index per node sent / received (the colour),
+  // and a device staging buffer of both fields
   struct Halo {
     int32_t rank = -1;
     int64_t n_send = 0, n_recv = 0;
+    int64_t n_send_nodes = 0, n_recv_nodes = 0;
     uint32_t* d_send = nullptr;
     uint32_t* d_recv = nullptr;
+    uint32_t* d_send_nodes = nullptr;
+    uint32_t* d_recv_nodes = nullptr;
     double* d_buf = nullptr;
   };
   std::vector<Halo> halos;
@@ -68,6 +72,7 @@ struct lb_ctx {
   int64_t n_blocks = 0;
   bool have_macro = false;
   uint32_t step_count = 0;       // steps since the populations were set (seeds FP16 stochastic rounding)
+  bool cg_colour_done = false;   // lb_step_cg_colour ran for the iteration at step_count
   bool have_visc = false;        // the last stored step was a rheology step
   // tagged boundaries (std_one_phase): pull table, anti-bounce-back masks,
   // domain labels with mass-source flags, force attribute, per-domain

@@ -58,6 +58,8 @@ SIGNATURES = {
                                 C.POINTER(C.c_int64)]),
     "lb_halo_pack": (C.c_int, [_vp, C.c_int32, C.c_int32, _f64p]),
     "lb_halo_unpack": (C.c_int, [_vp, C.c_int32, C.c_int32, _f64p]),
+    "lb_halo_pack_colour": (C.c_int, [_vp, C.c_int32, _f64p]),
+    "lb_halo_unpack_colour": (C.c_int, [_vp, C.c_int32, _f64p]),
     "lb_halo_count": (C.c_int32, [_vp]),
     "lb_halo_num_nodes": (C.c_int32, [_vp]),
     "lb_upload_f": (C.c_int, [_vp, C.c_int32, _f64p]),
@@ -76,6 +78,9 @@ SIGNATURES = {
     "lb_step_rheology": (C.c_int, [_vp, C.c_int32, _f64p]),
     "lb_step_cg": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double,
                              C.c_int32, _f64p]),
+    "lb_step_cg_colour": (C.c_int, [_vp, C.POINTER(C.c_double)]),
+    "lb_step_cg_collide": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32, _f64p,
+                                     C.c_double]),
     "lb_last_force": (C.c_int, [_vp, _f64p]),
     "lb_reduce_flux": (C.c_int, [_vp, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                  C.POINTER(C.c_double)]),
@@ -354,6 +359,7 @@ class Context:
         self.Q = Q_OF[lattice]
         self.nD = ND_OF[lattice]
         self.n_nodes = 0
+        self._halo_node_counts = []  # per exchange: (nodes to send, nodes received)
 
     def _check(self, rc):
         if rc != 0:
@@ -387,6 +393,7 @@ class Context:
                                           bulk if bulk.size else np.zeros(1, np.int32),
                                           bulk.size, TOPO_ALLOW_GHOST0 if allow_ghost0 else 0))
         self.n_nodes = n_nodes
+        self._halo_node_counts = []  # a new topology drops the exchanges
 
     def set_topology_grid(self, neighbor, node_type, bulk, pos, allow_ghost0=False, table_only=False):
         neighbor = np.ascontiguousarray(neighbor, dtype=np.int32)
@@ -400,6 +407,7 @@ class Context:
                                                bulk if bulk.size else np.zeros(1, np.int32),
                                                bulk.size, pos.ravel(), flags))
         self.n_nodes = n_nodes
+        self._halo_node_counts = []
 
     def set_topology_from(self, topo, allow_ghost0=False, table_only=False):
         self.set_topology_grid(topo.neighbors(), topo.node_types(), topo.bulk(), topo.positions(),
@@ -430,6 +438,8 @@ class Context:
         k = C.c_int32()
         self._check(lib().lb_halo_add(self._h, int(neighbor_rank), len(nodes_to_send), a[0], a[1], a[2],
                                       len(nodes_received), a[3], a[4], a[5], C.byref(k)))
+        assert k.value == len(self._halo_node_counts)
+        self._halo_node_counts.append((len(nodes_to_send), len(nodes_received)))
         return k.value
 
     def halo_sizes(self, k):
@@ -449,6 +459,19 @@ class Context:
         nf = v.shape[0] if v.ndim == 2 else 1
         flat = v.ravel() if v.size else np.zeros(1)
         self._check(lib().lb_halo_unpack(self._h, int(k), int(nf), flat))
+
+    def halo_pack_colour(self, k):
+        """The colour of the nodes this rank sends to exchange k (one per node)."""
+        if not 0 <= k < len(self._halo_node_counts):
+            raise LBError(-1, "no halo exchange %d" % k)
+        n = self._halo_node_counts[k][0]
+        out = np.zeros(max(n, 1))
+        self._check(lib().lb_halo_pack_colour(self._h, int(k), out))
+        return out[:n]
+
+    def halo_unpack_colour(self, k, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        self._check(lib().lb_halo_unpack_colour(self._h, int(k), v if v.size else np.zeros(1)))
 
     # fields
     def upload_f(self, f):
@@ -541,6 +564,22 @@ class Context:
         F[: len(force)] = force
         self._check(lib().lb_step_cg(self._h, int(nsteps), float(tau0), float(tau1), float(sigma),
                                      float(beta), int(force_mode), F))
+
+    def step_cg_colour(self, flux=False):
+        """The colour pass of one colour-gradient iteration (lb_step_cg_colour);
+        with flux=True returns this rank's sum of sum_q (f0 + f1) c_x over its
+        bulk nodes."""
+        s = C.c_double()
+        self._check(lib().lb_step_cg_colour(self._h, C.byref(s) if flux else None))
+        return s.value if flux else None
+
+    def step_cg_collide(self, tau0, tau1, sigma, beta, force_mode, force, flux_x_total=0.0):
+        """The collision pass of the iteration begun by step_cg_colour; in
+        FORCE_MOMX mode flux_x_total is the sum of every rank's flux."""
+        F = np.zeros(3)
+        F[: len(force)] = force
+        self._check(lib().lb_step_cg_collide(self._h, float(tau0), float(tau1), float(sigma), float(beta),
+                                             int(force_mode), F, float(flux_x_total)))
 
     def last_force(self):
         out = np.zeros(3)

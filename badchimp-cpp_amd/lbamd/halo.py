@@ -14,6 +14,8 @@ k-th value of the other.
 """
 import numpy as np
 
+from .ffi import FORCE_MOMX
+
 
 def dir_lists(topo, rank, ghosts):
     """makeDirList: per halo node, the directions (rest excluded) whose
@@ -53,14 +55,45 @@ def exchange_lists(topos):
     return out
 
 
-def exchange(contexts, registered):
+def exchange(contexts, registered, nf=1):
     """One halo exchange between contexts of one process: every rank packs
-    for each neighbour and the neighbour unpacks.  registered[r] maps a
-    neighbour rank to the exchange index of contexts[r] (Context.halo_add)."""
+    for each neighbour and the neighbour unpacks (the first nf population
+    fields).  registered[r] maps a neighbour rank to the exchange index of
+    contexts[r] (Context.halo_add)."""
     packed = {}
     for r, ctx in enumerate(contexts):
         for s, k in registered[r].items():
-            packed[(r, s)] = ctx.halo_pack(k)
+            packed[(r, s)] = ctx.halo_pack(k, nf)
     for r, ctx in enumerate(contexts):
         for s, k in registered[r].items():
             ctx.halo_unpack(k, packed[(s, r)])
+
+
+def exchange_colour(contexts, registered):
+    """The colour exchange of a colour-gradient iteration
+    (communciateScalarField of cgField, main_TWOPHASE.cpp:287)."""
+    packed = {}
+    for r, ctx in enumerate(contexts):
+        for s, k in registered[r].items():
+            packed[(r, s)] = ctx.halo_pack_colour(k)
+    for r, ctx in enumerate(contexts):
+        for s, k in registered[r].items():
+            ctx.halo_unpack_colour(k, packed[(s, r)])
+
+
+def step_twophase(contexts, registered, tau0, tau1, sigma, beta, force_mode, force):
+    """One iteration of the colour-gradient loop over a split domain, every
+    rank in this process (main_TWOPHASE.cpp:238-387 on each rank): the colour
+    pass, the colour exchange, the flux summed over the ranks in rank order
+    (MPI_Allreduce), the collision pass, the exchange of both population
+    fields.  Every context needs lb_set_global_bulk_count for FORCE_MOMX."""
+    momx = force_mode == FORCE_MOMX
+    fluxes = [ctx.step_cg_colour(flux=momx) for ctx in contexts]
+    exchange_colour(contexts, registered)
+    total = 0.0
+    if momx:
+        for s in fluxes:
+            total += s
+    for ctx in contexts:
+        ctx.step_cg_collide(tau0, tau1, sigma, beta, force_mode, force, total)
+    exchange(contexts, registered, nf=2)

@@ -124,8 +124,11 @@ int lb_set_global_bulk_count(lb_ctx* ctx, int64_t n);
  * the post-collision value of every listed (node, direction) and writes
  * what it receives into its halo slots.  The transport (MPI, RCCL, a device
  * copy) is the caller's.  Single-phase steps (BGK, TRT, rheology) read the
- * halo; colour-gradient steps and tagged boundaries refuse a context with
- * halo nodes (LB_ERR_UNSUPPORTED).
+ * halo.  The colour-gradient iteration also needs the colour of the halo
+ * nodes between its two passes: it runs as lb_step_cg_colour, the colour
+ * exchange (lb_halo_pack_colour / lb_halo_unpack_colour below), then
+ * lb_step_cg_collide; lb_step_cg refuses a context with halo nodes
+ * (LB_ERR_UNSUPPORTED), and so do tagged boundaries.
  *
  * lb_halo_add registers the exchange with one neighbouring rank from the
  * lists MonLatMpi holds after BndMpi::setup (LBbndmpi.h:206-315, the
@@ -146,6 +149,18 @@ int lb_halo_add(lb_ctx* ctx, int32_t neighbor_rank, int32_t n_send, const int32_
 int lb_halo_sizes(const lb_ctx* ctx, int32_t index, int32_t* neighbor_rank, int64_t* n_send, int64_t* n_recv);
 int lb_halo_pack(lb_ctx* ctx, int32_t index, int32_t nf, double* send);
 int lb_halo_unpack(lb_ctx* ctx, int32_t index, int32_t nf, const double* recv);
+/* The colour exchange of the colour-gradient iteration,
+ * communciateScalarField of cgField (main_TWOPHASE.cpp:287;
+ * MonLatMpi::communicateScalarField, LBmonlatmpi.h:181-205):
+ * lb_halo_pack_colour writes the colour of the n_send nodes_to_send of
+ * exchange `index` (as given to lb_halo_add) into send[j], and
+ * lb_halo_unpack_colour writes recv[j] into the colour of its n_recv
+ * nodes_received.  One double per node, the stored colour (double in
+ * LB_FP64, float in the other forms, so exact).  Both run between
+ * lb_step_cg_colour and lb_step_cg_collide of one iteration (LB_ERR_STATE
+ * otherwise). */
+int lb_halo_pack_colour(lb_ctx* ctx, int32_t index, double* send);
+int lb_halo_unpack_colour(lb_ctx* ctx, int32_t index, const double* recv);
 int32_t lb_halo_count(const lb_ctx* ctx);      /* exchanges registered */
 int32_t lb_halo_num_nodes(const lb_ctx* ctx);  /* halo (other-rank) nodes with a slot */
 
@@ -255,6 +270,27 @@ int lb_step_trt(lb_ctx* ctx, int32_t nsteps, double tau_sym, double tau_anti,
 int lb_step_cg(lb_ctx* ctx, int32_t nsteps, double tau0, double tau1,
                double sigma, double beta, int32_t force_mode,
                const double* force);
+
+/* One iteration of the same loop body in two calls, for a rank of a split
+ * domain (main_TWOPHASE.cpp:238-308 on each rank):
+ * lb_step_cg_colour computes cgField = (rho0 - rho1)/(rho0 + rho1) of every
+ * bulk node from its gathered populations; with flux_x non-NULL it also
+ * writes this rank's sum over its bulk nodes of sum_q (f0 + f1) c_q,x
+ * (meanfcX before MPI_Allreduce, main_TWOPHASE.cpp:290-298).
+ * The caller then exchanges the colour (lb_halo_pack_colour /
+ * lb_halo_unpack_colour) and, in LB_FORCE_MOMX mode, sums flux_x over the
+ * ranks.  lb_step_cg_collide runs the collision with the arguments of
+ * lb_step_cg; in LB_FORCE_MOMX mode the force is 2 (force[0] -
+ * flux_x_total / N), N the global bulk count (lb_set_global_bulk_count),
+ * as main_TWOPHASE.cpp:300-307 (flux_x_total is ignored in
+ * LB_FORCE_FIXED mode).  It stores rho and u of every iteration, and
+ * refuses to run without the colour pass of the same iteration
+ * (LB_ERR_STATE).  On a context without halo nodes the two calls give the
+ * bits of lb_step_cg(ctx, 1, ...). */
+int lb_step_cg_colour(lb_ctx* ctx, double* flux_x);
+int lb_step_cg_collide(lb_ctx* ctx, double tau0, double tau1, double sigma,
+                       double beta, int32_t force_mode, const double* force,
+                       double flux_x_total);
 
 /* Body force used by the last two-phase step (force.dat of main_TWOPHASE). */
 int lb_last_force(lb_ctx* ctx, double* force);

@@ -1,5 +1,5 @@
-"""Run the compiled reference std_case with its domain split over MPI ranks and
-store every rank's fields, the fixtures of the halo exchange
+"""Run the compiled reference std_case and twophase with the domain split over
+MPI ranks and store every rank's fields, the fixtures of the halo exchange
 (tests/test_gpu_halo.py).
 
 Needs the reference binaries built by oracle/build_ref.sh and MPICH's
@@ -8,17 +8,23 @@ elsewhere).  For every case:
 
   1. the fluid cells of the geometry are labelled 1..R by x slab, the
      reference's own multi-rank layout (oracle.ref_run.rank_slabs), and each
-     rank's node list is written as input/mpi/tmp<rank>.vtklb with the
-     library's writer (byte-identical to PythonScripts/vtklb.py,
-     tests/test_topology.py), init_rho = 1;
-  2. std_case (D2Q9 as shipped, or its D3Q19 build) runs on R ranks under
-     mpiexec, so the ranks exchange their halo through BndMpi /
-     MonLatMpi::communicateLbField (LBbndmpi.h:206-315, LBmonlatmpi.h:236-300)
-     every step (std_case/main.cpp:147);
-  3. each rank's VTU of the last iteration is parsed: rho and vel on its bulk
-     nodes, in its bulk order;
+     rank's node list is written as input/mpi/tmp<rank>.vtklb: for std_case
+     with the library's writer (byte-identical to PythonScripts/vtklb.py,
+     tests/test_topology.py), init_rho = 1; for twophase with the oracle's
+     restatement of that writer, as make_ref_fixtures.py writes the one-rank
+     twophase cases, with the attributes rho0, rho1, wettability and source;
+  2. std_case (D2Q9 as shipped, or its D3Q19 build) or twophase (D3Q19)
+     runs on R ranks under mpiexec, so the ranks exchange their halo through
+     BndMpi / MonLatMpi::communicateLbField (LBbndmpi.h:206-315,
+     LBmonlatmpi.h:236-300) every step (std_case/main.cpp:147,
+     main_TWOPHASE.cpp:386-387), twophase also the colour field
+     (communciateScalarField, main_TWOPHASE.cpp:287) and the flux sum
+     (MPI_Allreduce, main_TWOPHASE.cpp:297);
+  3. each rank's VTU of the last iteration is parsed: rho (rho0, rho1) and
+     vel on its bulk nodes, in its bulk order; for twophase also the body
+     force of force.dat (written by rank 0);
   4. the rank-labelled geometry, the parameters and the fields of every rank
-     are stored in ref_mpi_<case>.npz.
+     are stored in ref_mpi_<case>.npz (ref_mpi_tp_<case>.npz for twophase).
 
 Run:  python tests/golden/make_mpi_fixtures.py
 """
@@ -36,8 +42,9 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path[:0] = [ROOT, os.path.join(ROOT, "badchimp-cpp_amd"), HERE]
 
 import lbamd  # noqa: E402
-from make_ref_fixtures import last_vtu, parse_vtu  # noqa: E402
+from make_ref_fixtures import fmt, input_dat, last_vtu, parse_vtu  # noqa: E402
 from oracle.ref_run import MPIEXEC, REF, rank_slabs  # noqa: E402
+from oracle.vtklb_oracle import Geometry  # noqa: E402
 
 LATTICE = {"D2Q9": lbamd.D2Q9, "D3Q19": lbamd.D3Q19}
 
@@ -88,6 +95,61 @@ def run(name, lat, fluid, periodic, ranks, tau, force, steps):
         shutil.rmtree(root, ignore_errors=True)
 
 
+def tp_cases():
+    """(name, fluid geometry, periodic, ranks, rho0, wettability, tau0, tau1,
+    sigma, beta, momx, force, steps)"""
+    rng = np.random.default_rng(31)
+    geo, _ = lbamd.sphere_pack((20, 12, 10), 3.0, 0.5, 3, periodic="xyz", walls=False)
+    fluid = (geo > 0).astype(np.int32)
+    r0 = rng.choice([0.0, 0.3, 1.0], size=fluid.shape)
+    yield ("pack_2ranks", fluid, "xyz", 2, r0, np.full(fluid.shape, 0.8), 0.8, 1.2, 0.01, 0.7, 1e-4,
+           [0.0, 1e-6, -2e-6], 60)
+    geo, _ = lbamd.sphere_pack((24, 12, 10), 3.0, 0.55, 9, periodic="x", walls=True)
+    fluid = (geo > 0).astype(np.int32)
+    r0 = (rng.random(fluid.shape) > 0.5).astype(float)
+    yield ("walls_3ranks", fluid, "x", 3, r0, np.full(fluid.shape, 1.0), 1.0, 1.0, 0.01, 0.7, 5e-5,
+           [0.0, 0.0, 0.0], 40)
+
+
+def run_tp(name, fluid, periodic, ranks, r0, wett, tau0, tau1, sigma, beta, momx, force, steps):
+    g = rank_slabs(fluid, ranks)
+    geo = Geometry(g.astype(np.int64), "D3Q19", periodic)
+    attrs = [("rho0", r0), ("rho1", 1.0 - r0), ("wettability", wett),
+             ("source", np.zeros(g.shape, dtype=np.int64))]
+    root = tempfile.mkdtemp(prefix="lbmpi_tp_")
+    try:
+        os.makedirs(os.path.join(root, "input", "mpi"))
+        os.makedirs(os.path.join(root, "output", "tp"))
+        for r in range(1, ranks + 1):
+            with open(os.path.join(root, "input", "mpi", "tmp%d.vtklb" % (r - 1)), "w") as fh:
+                fh.write(geo.vtklb_text(r, attrs))
+        with open(os.path.join(root, "input", "input.dat"), "w") as fh:
+            fh.write(input_dat(steps, ["tau %s" % fmt([tau0, tau1]), "bodyforce %s" % fmt(force),
+                                       "sigma %s" % fmt(sigma), "beta %s" % fmt(beta), "momx %s" % fmt(momx)],
+                               extra="outdir tp/\n"))
+        env = dict(os.environ, LD_LIBRARY_PATH=os.path.join(REF, "mpilib"), OMP_NUM_THREADS="1")
+        res = subprocess.run([MPIEXEC, "-launcher", "fork", "-n", str(ranks), os.path.join(REF, "twophase")],
+                             cwd=root, env=env, capture_output=True, text=True, timeout=600)
+        if res.returncode != 0:
+            raise RuntimeError("%s failed: %s %s" % (name, res.stdout[-2000:], res.stderr[-2000:]))
+        od = os.path.join(root, "output", "tp")
+        params = dict(case="twophase", lattice="D3Q19", periodic=periodic, ranks=ranks, tau0=tau0, tau1=tau1,
+                      sigma=sigma, beta=beta, momx=momx, force=force, steps=steps)
+        out = {"geo": g.astype(np.int8), "params": json.dumps(params), "rho0_attr": r0, "wett_attr": wett,
+               "force_x": np.loadtxt(os.path.join(od, "force.dat"), ndmin=2)}
+        for r in range(ranks):
+            data = parse_vtu(last_vtu(od, "fluid").replace("/0000_fluid_", "/%04d_fluid_" % r))
+            for k in ("rho0", "rho1", "vel"):
+                out["%s_%d" % (k, r)] = data[k]
+        np.savez_compressed(os.path.join(HERE, "ref_mpi_tp_%s.npz" % name), **out)
+        print("tp_" + name, ranks, "ranks:", [len(out["rho0_%d" % r]) for r in range(ranks)], "bulk nodes,",
+              "force", out["force_x"][-1])
+    finally:
+        shutil.rmtree(root, ignore_errors=True)
+
+
 if __name__ == "__main__":
     for c in cases():
         run(*c)
+    for c in tp_cases():
+        run_tp(*c)

@@ -82,8 +82,8 @@ def tp_start(d, T):
     sb = T.node_types == 1
     r0[sb] = w[sb]
     r1[sb] = 1 - w[sb]
-    f0 = np.zeros((T.n_nodes, 2, 19))
-    W = oracle.WEIGHTS[1]
+    W = oracle.WEIGHTS[T.lattice]
+    f0 = np.zeros((T.n_nodes, 2, W.size))
     f0[T.bulk, 0] = W[None] * r0[T.bulk, None]
     f0[T.bulk, 1] = W[None] * r1[T.bulk, None]
     return f0, np.stack([r0, r1], 1)

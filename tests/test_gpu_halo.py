@@ -17,8 +17,9 @@ import numpy as np
 import pytest
 
 import lbamd
-from helpers import load_ref, ref_names
-from lbamd.halo import exchange, exchange_lists
+from helpers import OracleTopo, load_ref, ref_names, tp_start
+from lbamd.halo import exchange, exchange_lists, step_twophase
+from oracle.ref_run import rank_slabs
 
 pytestmark = pytest.mark.gpu
 
@@ -124,7 +125,7 @@ def test_split_equals_whole(gpu_available, precision, lattice, shape, n_ranks, o
         assert np.array_equal(vel, wv[k])
 
 
-@pytest.mark.parametrize("name", ref_names("mpi_"))
+@pytest.mark.parametrize("name", [n for n in ref_names("mpi_") if not n.startswith("mpi_tp_")])
 def test_split_vs_reference_mpi_run(gpu_available, name):
     """The compiled reference std_case run on R MPI ranks (its own halo
     exchange, tests/golden/make_mpi_fixtures.py) against R contexts on one
@@ -186,9 +187,195 @@ def test_halo_refusals(gpu_available):
     assert e.value.code == -2
     ctx.step_bgk(1, 1.0, [1e-6, 0.0, 0.0])
     assert ctx.halo_pack(k).shape == (1, int(nds.sum()))
-    # colour-gradient steps need the halo's colour too: refused
-    ctx.upload_f(np.stack([f0, 0.0 * f0], 1))
-    with pytest.raises(lbamd.LBError) as e:
-        ctx.step_cg(1, 1.0, 1.0, 0.01, 0.7, lbamd.FORCE_FIXED, [1e-6, 0.0, 0.0])
-    assert e.value.code == -4
     ctx.close()
+
+
+# ---- the colour-gradient iteration over a split domain
+
+def tp_contexts(ranked, lattice, periodic, ranks, precision, rho0, wett):
+    """One context per rank (link table), the two-phase start state and
+    wall colours of tp_start, the exchanges registered."""
+    topos = [lbamd.Topology.from_geometry(ranked, lattice, periodic, rank=r + 1) for r in range(ranks)]
+    lists = exchange_lists(topos)
+    ctxs, reg, where = [], [], []
+    d = {"rho0_attr": rho0, "wett_attr": wett}
+    for r, t in enumerate(topos):
+        T = OracleTopo(ranked, lattice, periodic, rank=r + 1)
+        assert np.array_equal(T.neighbors, t.neighbors())
+        f0, rho = tp_start(d, T)
+        ctx = lbamd.Context(lattice, precision)
+        ctx.set_topology(T.neighbors, T.node_types, T.bulk)
+        assert ctx.halo_nodes > 0
+        ctx.upload_f(f0)
+        ctx.set_wall_densities(rho)
+        reg.append({x[0]: ctx.halo_add(*x) for x in lists[r]})
+        ctxs.append(ctx)
+        where.append(T)
+        t.close()
+    n_global = sum(T.bulk.size for T in where)
+    for ctx in ctxs:
+        ctx.set_global_bulk_count(n_global)
+    return ctxs, reg, where
+
+
+def tp_whole(fluid, lattice, periodic, precision, rho0, wett):
+    T = OracleTopo(fluid, lattice, periodic)
+    f0, rho = tp_start({"rho0_attr": rho0, "wett_attr": wett}, T)
+    ctx = lbamd.Context(lattice, precision)
+    ctx.set_topology(T.neighbors, T.node_types, T.bulk)
+    ctx.upload_f(f0)
+    ctx.set_wall_densities(rho)
+    return ctx, T
+
+
+def tp_fields(ctx, T):
+    b = T.bulk
+    return (T.positions[b], ctx.download_f(2)[b], ctx.download_scalar(lbamd.RHO, 2)[b], ctx.download_vel()[b],
+            ctx.download_scalar(lbamd.PHI)[b])
+
+
+def tp_geometry(lattice, shape, periodic, ranks, seed):
+    if lattice == lbamd.D3Q19:
+        geo, _ = lbamd.sphere_pack(shape, 3.0, 0.5, seed, periodic=periodic, walls=periodic != "xyz")
+        fluid = (np.asarray(geo) > 0).astype(np.int64)
+    else:
+        rng = np.random.default_rng(seed)
+        fluid = (rng.random(shape) > 0.3).astype(np.int64)
+        fluid[:, 0] = 0
+        fluid[:, -1] = 0
+    rng = np.random.default_rng(seed + 1)
+    rho0 = rng.choice([0.0, 0.3, 1.0], size=shape)
+    wett = np.full(shape, 0.8)
+    return fluid, rank_slabs(fluid, ranks).astype(np.int64), rho0, wett
+
+
+TP = dict(tau0=0.8, tau1=1.2, sigma=0.01, beta=0.7)
+
+
+@pytest.mark.parametrize("precision", [lbamd.FP64, lbamd.FP32S])
+@pytest.mark.parametrize("lattice,shape,periodic,ranks,mode", [
+    (lbamd.D3Q19, (20, 12, 10), "xyz", 2, lbamd.FORCE_FIXED),
+    (lbamd.D3Q19, (24, 12, 10), "x", 3, lbamd.FORCE_FIXED),
+    (lbamd.D2Q9, (40, 18), "x", 2, lbamd.FORCE_FIXED),
+    (lbamd.D3Q19, (20, 12, 10), "xyz", 2, lbamd.FORCE_MOMX),
+])
+def test_twophase_split_equals_whole(gpu_available, precision, lattice, shape, periodic, ranks, mode):
+    """The colour-gradient loop on R contexts (colour pass, colour exchange,
+    collision pass, population exchange per iteration) against the whole
+    domain in one context: bit for bit with a fixed force (the per-node
+    arithmetic is the same; only where a gather or the gradient finds a
+    neighbour's value differs); with the momentum-controlled force to 1e-12
+    of each field's scale (the flux is summed per rank, then over the
+    ranks)."""
+    fluid, ranked, rho0, wett = tp_geometry(lattice, shape, periodic, ranks, 7)
+    force = [1e-4 if mode == lbamd.FORCE_MOMX else 3e-6, 1e-6, -2e-6][: 3 if lattice == lbamd.D3Q19 else 2]
+    nsteps = 30
+    ctx, T = tp_whole(fluid, lattice, periodic, precision, rho0, wett)
+    ctx.step_cg(nsteps, force_mode=mode, force=force, **TP)
+    wp, wf, wr, wv, wphi = tp_fields(ctx, T)
+    wF = ctx.last_force()
+    ctx.close()
+    ctxs, reg, topos = tp_contexts(ranked, lattice, periodic, ranks, precision, rho0, wett)
+    for _ in range(nsteps):
+        step_twophase(ctxs, reg, TP["tau0"], TP["tau1"], TP["sigma"], TP["beta"], mode, force)
+    where = {tuple(p): k for k, p in enumerate(wp)}
+    seen = 0
+    for ctx, Tr in zip(ctxs, topos):
+        pos, f, r, v, phi = tp_fields(ctx, Tr)
+        k = np.array([where[tuple(p)] for p in pos])
+        seen += k.size
+        if mode == lbamd.FORCE_FIXED:
+            assert np.array_equal(f, wf[k])
+            assert np.array_equal(r, wr[k])
+            assert np.array_equal(v, wv[k])
+            assert np.array_equal(phi, wphi[k])
+            assert np.array_equal(ctx.last_force(), wF)
+        else:
+            for got, ref in ((f, wf[k]), (r, wr[k]), (v, wv[k]), (phi, wphi[k])):
+                assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+            assert abs(ctx.last_force()[0] - wF[0]) <= 1e-12 * abs(wF[0])
+        ctx.close()
+    assert seen == len(wp)
+
+
+@pytest.mark.parametrize("name", ref_names("mpi_tp_"))
+def test_twophase_split_vs_reference_mpi_run(gpu_available, name):
+    """The compiled reference twophase run on R MPI ranks (its own exchanges
+    of populations and colour, its MPI_Allreduce of the flux;
+    tests/golden/make_mpi_fixtures.py) against R contexts on one GPU: every
+    rank's rho0, rho1 and u of the last iteration, and the body force, to
+    1e-12 of their scale (the flux sums run in another order)."""
+    d = load_ref(name)
+    p = d["params"]
+    ranked = d["geo"].astype(np.int64)
+    ctxs, reg, topos = tp_contexts(ranked, lbamd.D3Q19, p["periodic"], p["ranks"], lbamd.FP64, d["rho0_attr"],
+                                   d["wett_attr"])
+    force = [p["momx"]] + list(p["force"][1:])
+    for _ in range(p["steps"] + 1):
+        step_twophase(ctxs, reg, p["tau0"], p["tau1"], p["sigma"], p["beta"], lbamd.FORCE_MOMX, force)
+    for r, (ctx, T) in enumerate(zip(ctxs, topos)):
+        _, _, rho, vel, _ = tp_fields(ctx, T)
+        for got, ref in ((rho[:, 0], d["rho0_%d" % r]), (rho[:, 1], d["rho1_%d" % r]), (vel, d["vel_%d" % r])):
+            assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max(), (name, r)
+        F = ctx.last_force()[0]
+        assert abs(F - d["force_x"][-1, 1]) <= 1e-12 * abs(d["force_x"][-1, 1])
+        ctx.close()
+
+
+@pytest.mark.parametrize("precision,layout,mode", [
+    (lbamd.FP64, "table", lbamd.FORCE_FIXED), (lbamd.FP64, "table", lbamd.FORCE_MOMX),
+    (lbamd.FP64, "grid", lbamd.FORCE_MOMX), (lbamd.FP32S, "grid", lbamd.FORCE_MOMX),
+    (lbamd.FP16S, "grid", lbamd.FORCE_FIXED)])
+def test_twophase_colour_then_collide_equals_step_cg(gpu_available, precision, layout, mode):
+    """On one context without a halo, lb_step_cg_colour + lb_step_cg_collide
+    per iteration (the flux handed back unchanged) give the bits of
+    lb_step_cg."""
+    fluid, _, rho0, wett = tp_geometry(lbamd.D3Q19, (18, 14, 12), "x", 1, 3)
+    T = OracleTopo(fluid, lbamd.D3Q19, "x")
+    f0, rho = tp_start({"rho0_attr": rho0, "wett_attr": wett}, T)
+    force = [1e-4 if mode == lbamd.FORCE_MOMX else 3e-6, 1e-6, -2e-6]
+    out = []
+    for split in (False, True):
+        ctx = lbamd.Context(lbamd.D3Q19, precision)
+        if layout == "table":
+            ctx.set_topology(T.neighbors, T.node_types, T.bulk)
+        else:
+            ctx.set_topology_grid(T.neighbors, T.node_types, T.bulk, T.positions)
+        ctx.upload_f(f0)
+        ctx.set_wall_densities(rho)
+        for _ in range(12):
+            if split:
+                s = ctx.step_cg_colour(flux=mode == lbamd.FORCE_MOMX)
+                ctx.step_cg_collide(force_mode=mode, force=force, flux_x_total=s or 0.0, **TP)
+            else:
+                ctx.step_cg(1, force_mode=mode, force=force, **TP)
+        out.append(tp_fields(ctx, T) + (ctx.last_force(),))
+        ctx.close()
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
+def test_twophase_split_refusals(gpu_available):
+    fluid, ranked, rho0, wett = tp_geometry(lbamd.D3Q19, (16, 10, 8), "x", 2, 5)
+    ctxs, reg, _ = tp_contexts(ranked, lbamd.D3Q19, "x", 2, lbamd.FP64, rho0, wett)
+    ctx, k = ctxs[0], reg[0][1]
+    # lb_step_cg cannot exchange the colour between its passes
+    with pytest.raises(lbamd.LBError) as e:
+        ctx.step_cg(1, force_mode=lbamd.FORCE_FIXED, force=[1e-6, 0.0, 0.0], **TP)
+    assert e.value.code == -4
+    # the collision and the colour exchange need the colour pass of the iteration
+    with pytest.raises(lbamd.LBError) as e:
+        ctx.step_cg_collide(force_mode=lbamd.FORCE_FIXED, force=[1e-6, 0.0, 0.0], **TP)
+    assert e.value.code == -2
+    with pytest.raises(lbamd.LBError) as e:
+        ctx.halo_pack_colour(k)
+    assert e.value.code == -2
+    ctx.step_cg_colour()
+    assert ctx.halo_pack_colour(k).shape == (ctx._halo_node_counts[k][0],)
+    ctx.step_cg_collide(force_mode=lbamd.FORCE_FIXED, force=[1e-6, 0.0, 0.0], **TP)
+    # one colour pass per iteration: the next collision needs a new one
+    with pytest.raises(lbamd.LBError) as e:
+        ctx.step_cg_collide(force_mode=lbamd.FORCE_FIXED, force=[1e-6, 0.0, 0.0], **TP)
+    assert e.value.code == -2
+    for c in ctxs:
+        c.close()

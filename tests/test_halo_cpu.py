@@ -8,6 +8,8 @@ import numpy as np
 import pytest
 
 import lbamd
+import oracle
+from helpers import OracleTopo, load_ref, ref_names, tp_start
 from lbamd.halo import exchange_lists
 
 
@@ -57,3 +59,31 @@ def test_exchange_lists_pair_up(lattice, shape, n_ranks):
             assert np.all(d[:, 1:] == 0) and np.all(d[:, 0] % period == 0)
             n_pairs += 1
     assert n_pairs == sum(len(x) for x in lists)
+
+
+@pytest.mark.parametrize("name", ref_names("mpi_tp_"))
+def test_reference_mpi_twophase_equals_oracle_whole_domain(name):
+    """The fixtures of the split two-phase run (the compiled reference's
+    twophase on R MPI ranks, exchanging populations, the colour field and
+    the flux sum) against the oracle on the undivided domain: every rank's
+    rho0, rho1 and u at the same cells, and the body force, to 1e-12 of
+    their scale (the flux of the force controller is summed per rank, then
+    over the ranks)."""
+    d = load_ref(name)
+    p = d["params"]
+    ranked = d["geo"].astype(np.int64)
+    T = OracleTopo((ranked > 0).astype(np.int64), "D3Q19", p["periodic"])
+    f0, rho = tp_start(d, T)
+    _, r, v, _, F = oracle.run_twophase(1, T.neighbors, T.node_types, T.bulk, f0, rho, p["tau0"], p["tau1"],
+                                        p["sigma"], p["beta"], 1, [p["momx"]] + p["force"][1:], p["steps"] + 1)
+    where = {tuple(x): n for n, x in enumerate(T.positions)}
+    seen = 0
+    for rank in range(p["ranks"]):
+        Tr = OracleTopo(ranked, "D3Q19", p["periodic"], rank=rank + 1)
+        at = np.array([where[tuple(x)] for x in Tr.positions[Tr.bulk]])
+        seen += at.size
+        for got, ref in ((r[at, 0], d["rho0_%d" % rank]), (r[at, 1], d["rho1_%d" % rank]),
+                         (v[at], d["vel_%d" % rank])):
+            assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max(), (name, rank)
+    assert seen == T.bulk.size
+    assert abs(F[0] - d["force_x"][-1, 1]) <= 1e-12 * abs(d["force_x"][-1, 1])
