@@ -87,3 +87,28 @@ def test_reference_mpi_twophase_equals_oracle_whole_domain(name):
             assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max(), (name, rank)
     assert seen == T.bulk.size
     assert abs(F[0] - d["force_x"][-1, 1]) <= 1e-12 * abs(d["force_x"][-1, 1])
+
+
+@pytest.mark.parametrize("name", [n for n in ref_names("mpi_") if not n.startswith("mpi_tp_")])
+def test_reference_mpi_std_case_equals_oracle_whole_domain(name):
+    """The compiled reference's std_case on R MPI ranks against the oracle on
+    the undivided domain: with no global sum in the loop, every rank's rho
+    and u equal the whole domain's at the same cells bit for bit."""
+    d = load_ref(name)
+    p = d["params"]
+    ranked = d["geo"].astype(np.int64)
+    T = OracleTopo((ranked > 0).astype(np.int64), p["lattice"], p["periodic"])
+    W = oracle.WEIGHTS[T.lattice]
+    f0 = np.zeros((T.n_nodes, W.size))
+    f0[T.bulk] = W  # init_rho = 1
+    _, r, v = oracle.run_std_case(T.lattice, T.neighbors, T.node_types, T.bulk, f0, p["tau"],
+                                  np.asarray(p["force"], dtype=float), p["steps"] + 1)
+    where = {tuple(x): n for n, x in enumerate(T.positions)}
+    seen = 0
+    for rank in range(p["ranks"]):
+        Tr = OracleTopo(ranked, p["lattice"], p["periodic"], rank=rank + 1)
+        at = np.array([where[tuple(x)] for x in Tr.positions[Tr.bulk]])
+        seen += at.size
+        assert np.array_equal(r[at], d["rho_%d" % rank]), (name, rank)
+        assert np.array_equal(v[at], d["vel_%d" % rank][:, : v.shape[1]]), (name, rank)
+    assert seen == T.bulk.size
