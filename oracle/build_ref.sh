@@ -7,6 +7,8 @@
 #                   (std_case/main.cpp:14-15 offers the switch); the edited
 #                   copy is written to oracle/_ref/ and never committed
 #   twophase        src/twophase/main_TWOPHASE.cpp as shipped (D3Q19)
+#   twophase_d2q9   the same file with its lattice lines switched to D2Q9
+#                   (main_TWOPHASE.cpp:50-51 offers the switch)
 #   std_case_trt_d2q9, std_case_trt_d3q19
 #                   std_case/main.cpp with its collision switched to the
 #                   reference's two-relaxation-time operator: calcOmegaBGK ->
@@ -30,7 +32,7 @@
 #                   (LBglobalforcing.h:8-98) on fields read from a file
 #   init_ref        oracle/init_harness.cpp: the reference's initiateLbField
 #                   (LBinitiatefield.h:33-57) on fields read from a file
-#   std_case_gpu_d2q9, std_case_gpu_d3q19, twophase_gpu
+#   std_case_gpu_d2q9, std_case_gpu_d3q19, twophase_gpu_d2q9, twophase_gpu_d3q19
 #                   tests/integration/{std_case,twophase}_gpu.cpp: the
 #                   reference's set-up classes driving this repository's
 #                   library through the C++ shim (needs
@@ -103,6 +105,15 @@ for lat in D2Q9 D3Q19; do
   if cmp -s "$f.new" "$f"; then rm "$f.new"; else mv "$f.new" "$f"; fi
 done
 
+f="$OUT/twophase_d2q9.cpp"
+sed -e 's|^//#define LT D2Q9|#define LT D2Q9|' -e 's|^#define LT D3Q19|//#define LT D3Q19|' \
+    "$REF/twophase/main_TWOPHASE.cpp" > "$f.new"
+if ! grep -q '^#define LT D2Q9' "$f.new" || grep -q '^#define LT D3Q19' "$f.new"; then
+  echo "build_ref: the lattice switch did not apply to twophase/main_TWOPHASE.cpp" >&2
+  exit 1
+fi
+if cmp -s "$f.new" "$f"; then rm "$f.new"; else mv "$f.new" "$f"; fi
+
 f="$OUT/one_phase.cpp"
 sed -e 's|std::string chimpDir = "/home/[^"]*";|std::string chimpDir = "./../";|' \
     "$REF/std_one_phase/main.cpp" > "$f.new"
@@ -122,6 +133,7 @@ build() {  # name, main source, extra include dir
 build std_case_d2q9 "$REF/std_case/main.cpp" "$REF/std_case"
 build std_case_d3q19 "$OUT/std_case_d3q19.cpp" "$REF/std_case"
 build twophase "$REF/twophase/main_TWOPHASE.cpp" "$REF/twophase"
+build twophase_d2q9 "$OUT/twophase_d2q9.cpp" "$REF/twophase"
 build std_case_trt_d2q9 "$OUT/std_case_trt_d2q9.cpp" "$REF/std_case"
 build std_case_trt_d3q19 "$OUT/std_case_trt_d3q19.cpp" "$REF/std_case"
 build rheology_d2q9 "$OUT/rheology_d2q9.cpp" "$REF/polymer_viscosity_tabulation"
@@ -154,10 +166,13 @@ if [ -e "$LIBDIR/libbadchimp_amd.so" ]; then
     fi
   done
   TPDEMO="$REPO/tests/integration/twophase_gpu.cpp"
-  if [ ! -x "$OUT/twophase_gpu" ] || [ "$TPDEMO" -nt "$OUT/twophase_gpu" ] || [ "$SHIM" -nt "$OUT/twophase_gpu" ]; then
-    g++ -std=c++17 -O2 -w -I/opt/conda/include -I"$REF" -I"$REPO/badchimp-cpp_amd/include" \
-      "$TPDEMO" "$REF/lbsolver/LBd2q9.cpp" "$REF/lbsolver/LBd3q19.cpp" /opt/conda/lib/libmpi.so \
-      -L"$LIBDIR" -lbadchimp_amd -Wl,-rpath,'$ORIGIN/../../badchimp-cpp_amd/lib' -o "$OUT/twophase_gpu"
-    echo "build_ref: built twophase_gpu"
-  fi
+  for lat in D2Q9 D3Q19; do
+    name=twophase_gpu_$(echo $lat | tr 'A-Z' 'a-z')
+    if [ ! -x "$OUT/$name" ] || [ "$TPDEMO" -nt "$OUT/$name" ] || [ "$SHIM" -nt "$OUT/$name" ]; then
+      g++ -std=c++17 -O2 -w -I/opt/conda/include -I"$REF" -I"$REPO/badchimp-cpp_amd/include" -DLB_LATTICE=$lat \
+        "$TPDEMO" "$REF/lbsolver/LBd2q9.cpp" "$REF/lbsolver/LBd3q19.cpp" /opt/conda/lib/libmpi.so \
+        -L"$LIBDIR" -lbadchimp_amd -Wl,-rpath,'$ORIGIN/../../badchimp-cpp_amd/lib' -o "$OUT/$name"
+      echo "build_ref: built $name"
+    fi
+  done
 fi

@@ -302,6 +302,20 @@ def tp_cases():
     yield "tp_walls_12x14x10", g, "x", r0, np.full(g.shape, 1.0), 1.0, 1.0, 0.01, 0.7, 5e-5, [0.0, 0.0, 0.0], 60
 
 
+def tp_d2q9_cases():
+    """The twophase case built for D2Q9 (oracle/_ref/twophase_d2q9), same
+    fields as tp_cases."""
+    rng = np.random.default_rng(23)
+    g = load_geo("d2q9_random_7x5_pxy")
+    r0 = rng.choice([0.0, 0.4, 1.0], size=g.shape)
+    yield "tp_d2q9_random_7x5", g, "xy", r0, np.ones(g.shape), 1.0, 1.0, 0.01, 0.7, 1e-4, [0.0, 1e-6], 40
+    g = (rng.random((40, 18)) > 0.3).astype(np.int64)
+    g[:, 0] = 0
+    g[:, -1] = 0
+    r0 = (rng.random(g.shape) > 0.5).astype(float)
+    yield "tp_d2q9_walls_40x18", g, "x", r0, np.full(g.shape, 0.8), 0.8, 1.2, 0.01, 0.7, 5e-5, [0.0, -1e-6], 100
+
+
 VTU_KEEP = {"d2q9_random_7x5", "d3q19_pack_6x5x4_pxyz", "trt_d3q19_pack_6x5x4_px_walls", "tp_pack_6x5x4"}
 
 
@@ -369,7 +383,8 @@ def main(selected):
                             label=label, force_on=fon, params=json.dumps(params), rho=data["rho"],
                             vel=data["vel"], flux=data["flux"])
         print("wrote ref_%s.npz" % name, data["rho"].shape, data["flux"])
-    for name, g, per, r0, wett, tau0, tau1, sigma, beta, momx, force, max_it in tp_cases():
+    tp_all = [("D3Q19", "twophase") + c for c in tp_cases()] + [("D2Q9", "twophase_d2q9") + c for c in tp_d2q9_cases()]
+    for lat, binary, name, g, per, r0, wett, tau0, tau1, sigma, beta, momx, force, max_it in tp_all:
         if selected and name not in selected:
             continue
         attrs = [("rho0", r0), ("rho1", 1.0 - r0), ("wettability", wett),
@@ -377,9 +392,9 @@ def main(selected):
         dat = input_dat(max_it, ["tau %s" % fmt([tau0, tau1]), "bodyforce %s" % fmt(force),
                                  "sigma %s" % fmt(sigma), "beta %s" % fmt(beta), "momx %s" % fmt(momx)],
                         extra="outdir tp/\n")
-        data = run_case("twophase", g, "D3Q19", per, attrs, dat, twophase=True,
+        data = run_case(binary, g, lat, per, attrs, dat, twophase=True,
                         keep=name if name in VTU_KEEP else None)
-        params = dict(case="twophase", lattice="D3Q19", periodic=per, tau0=tau0, tau1=tau1, sigma=sigma,
+        params = dict(case="twophase", lattice=lat, periodic=per, tau0=tau0, tau1=tau1, sigma=sigma,
                       beta=beta, momx=momx, force=force, steps=max_it)
         out = dict(geo=g.astype(np.int8), params=json.dumps(params), rho0_attr=r0, wett_attr=wett,
                    rho0=data["rho0"], rho1=data["rho1"], vel=data["vel"], force_x=data["force_x"])

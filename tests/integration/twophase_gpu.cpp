@@ -7,8 +7,9 @@
 //   twophase_gpu <case.vtklb> <tau0> <tau1> <sigma> <beta> <momx> <F_y> <F_z> <iterations> <precision> <out.bin>
 //
 // Runs iterations + 1 steps and writes the last force F_x, then for each bulk
-// node in findBulkNodes order rho0, rho1 and the three velocity components
-// (doubles).  Built by oracle/build_ref.sh where the reference tree is present.
+// node in findBulkNodes order rho0, rho1 and the nD velocity components
+// (doubles).  LB_LATTICE selects D2Q9 or D3Q19 (F_z is ignored in 2-D).
+// Built by oracle/build_ref.sh where the reference tree is present.
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -18,7 +19,10 @@
 #include "LBSOLVER.h"
 #include "lbamd/lbsolver_gpu.hpp"
 
-using LT = D3Q19;
+#ifndef LB_LATTICE
+#define LB_LATTICE D3Q19
+#endif
+using LT = LB_LATTICE;
 
 int main(int argc, char** argv) {
   if (argc != 12) {
@@ -77,7 +81,8 @@ int main(int argc, char** argv) {
   std::ofstream os(out, std::ios::binary);
   os.write(reinterpret_cast<const char*>(&fx), sizeof(double));
   for (int n : bulk) {
-    const double v[5] = {rho(0, n), rho(1, n), vel(0, 0, n), vel(0, 1, n), vel(0, 2, n)};
+    double v[2 + LT::nD] = {rho(0, n), rho(1, n)};
+    for (int d = 0; d < LT::nD; ++d) v[2 + d] = vel(0, d, n);
     os.write(reinterpret_cast<const char*>(v), sizeof(v));
   }
   return os ? 0 : 1;

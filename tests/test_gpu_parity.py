@@ -10,7 +10,7 @@ import pytest
 
 import lbamd
 import oracle
-from helpers import OracleTopo, float_round, load_ref, ref_names, std_ref_names
+from helpers import OracleTopo, float_round, load_ref, ref_names, std_ref_names, tp_start
 
 pytestmark = pytest.mark.gpu
 
@@ -107,17 +107,7 @@ def test_ghost0_rejected_without_flag(gpu_available):
 
 
 def tp_initial(T, d):
-    r0 = float_round(T.attribute(d["rho0_attr"]))
-    r1 = float_round(T.attribute(1.0 - d["rho0_attr"]))
-    w = float_round(T.attribute(d["wett_attr"]))
-    sb = T.node_types == 1
-    r0[sb] = w[sb]
-    r1[sb] = 1 - w[sb]
-    f0 = np.zeros((T.n_nodes, 2, 19))
-    W = oracle.WEIGHTS[1]
-    f0[T.bulk, 0] = W[None] * r0[T.bulk, None]
-    f0[T.bulk, 1] = W[None] * r1[T.bulk, None]
-    return f0, np.stack([r0, r1], 1)
+    return tp_start(d, T)
 
 
 def gpu_twophase(T, f0, rho, p, mode, force, nsteps, chunks=None, layout="table", precision=lbamd.FP64):
@@ -146,7 +136,7 @@ def test_twophase_vs_reference_fixture(gpu_available, name):
     f, r, v, phi, F = gpu_twophase(T, f0, rho, p, lbamd.FORCE_MOMX, force, p["steps"] + 1)
     b = T.bulk
     # the flux controller sums over nodes in a different order: 1e-12 relative
-    for got, ref in ((r[b, 0], d["rho0"]), (r[b, 1], d["rho1"]), (v[b], d["vel"])):
+    for got, ref in ((r[b, 0], d["rho0"]), (r[b, 1], d["rho1"]), (v[b], d["vel"][:, : v.shape[1]])):
         scale = np.abs(ref).max()
         assert np.abs(got - ref).max() <= 1e-12 * scale
     assert abs(F[0] - d["force_x"][-1, 1]) <= 1e-12 * abs(d["force_x"][-1, 1])
