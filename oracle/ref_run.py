@@ -41,6 +41,26 @@ def rank_slabs(geo, ranks):
     return g
 
 
+def rank_blocks(geo, counts):
+    """Fluid cells of geo labelled 1..prod(counts) by block: counts[a] blocks
+    along axis a (axes beyond len(counts) are not cut), numbered in C order of
+    the block index."""
+    g = np.ascontiguousarray(geo, dtype=np.int32).copy()
+    fluid = g > 0
+    label = np.zeros(g.shape, dtype=np.int32)
+    stride = 1
+    for a in reversed(range(len(counts))):
+        idx = np.zeros(g.shape[a], dtype=np.int32)
+        for b, xs in enumerate(np.array_split(np.arange(g.shape[a]), counts[a])):
+            idx[xs] = b
+        shape = [1] * g.ndim
+        shape[a] = g.shape[a]
+        label += stride * idx.reshape(shape)
+        stride *= counts[a]
+    g[fluid] = label[fluid] + 1
+    return g
+
+
 def _input_dat(steps, tau, force):
     f = " ".join(repr(float(x)) for x in force)
     return ("<iterations>\n    max %d\n    write %d\n<end>\n<fluid>\n    tau %r\n    bodyforce %s\n<end>\n"

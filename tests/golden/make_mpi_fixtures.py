@@ -43,27 +43,36 @@ sys.path[:0] = [ROOT, os.path.join(ROOT, "badchimp-cpp_amd"), HERE]
 
 import lbamd  # noqa: E402
 from make_ref_fixtures import fmt, input_dat, last_vtu, parse_vtu  # noqa: E402
-from oracle.ref_run import MPIEXEC, REF, rank_slabs  # noqa: E402
+from oracle.ref_run import MPIEXEC, REF, rank_blocks, rank_slabs  # noqa: E402
 from oracle.vtklb_oracle import Geometry  # noqa: E402
 
 LATTICE = {"D2Q9": lbamd.D2Q9, "D3Q19": lbamd.D3Q19}
 
 
 def cases():
-    """(name, lattice, fluid geometry, periodic, ranks, tau, force, steps)"""
+    """(name, lattice, fluid geometry, periodic, ranks, tau, force, steps,
+    blocks per axis or None for x slabs)"""
     geo, _ = lbamd.sphere_pack((24, 16, 12), 3.0, 0.45, 5, periodic="x", walls=True)
-    yield "d3q19_pack_2ranks", "D3Q19", (geo > 0).astype(np.int32), "x", 2, 0.87, [2e-5, -1e-5, 5e-6], 40
+    yield "d3q19_pack_2ranks", "D3Q19", (geo > 0).astype(np.int32), "x", 2, 0.87, [2e-5, -1e-5, 5e-6], 40, None
     geo, _ = lbamd.sphere_pack((30, 14, 12), 3.0, 0.45, 7, periodic="x", walls=True)
-    yield "d3q19_pack_3ranks", "D3Q19", (geo > 0).astype(np.int32), "x", 3, 1.0, [1e-5, 0.0, 0.0], 30
+    yield "d3q19_pack_3ranks", "D3Q19", (geo > 0).astype(np.int32), "x", 3, 1.0, [1e-5, 0.0, 0.0], 30, None
     rng = np.random.default_rng(5)
     geo = (rng.random((40, 18)) > 0.3).astype(np.int32)
     geo[:, 0] = 0
     geo[:, -1] = 0
-    yield "d2q9_random_2ranks", "D2Q9", geo, "x", 2, 0.8, [1e-5, -3e-6], 40
+    yield "d2q9_random_2ranks", "D2Q9", geo, "x", 2, 0.8, [1e-5, -3e-6], 40, None
+    # 2 x 2 blocks over x and y, periodic along both: ranks that share only an
+    # edge exchange the diagonal directions
+    geo, _ = lbamd.sphere_pack((20, 16, 10), 3.0, 0.5, 11, periodic="xy", walls=True)
+    yield ("d3q19_pack_4ranks_2x2", "D3Q19", (geo > 0).astype(np.int32), "xy", 4, 0.9, [1e-5, 4e-6, -2e-6], 30,
+           (2, 2))
+    rng = np.random.default_rng(9)
+    geo = (rng.random((24, 20)) > 0.3).astype(np.int32)
+    yield "d2q9_random_4ranks_2x2", "D2Q9", geo, "xy", 4, 0.8, [1e-5, -3e-6], 40, (2, 2)
 
 
-def run(name, lat, fluid, periodic, ranks, tau, force, steps):
-    g = rank_slabs(fluid, ranks)
+def run(name, lat, fluid, periodic, ranks, tau, force, steps, blocks=None):
+    g = rank_slabs(fluid, ranks) if blocks is None else rank_blocks(fluid, blocks)
     root = tempfile.mkdtemp(prefix="lbmpi_")
     try:
         for d in ("input/mpi", "output", "run"):
@@ -83,7 +92,7 @@ def run(name, lat, fluid, periodic, ranks, tau, force, steps):
         if res.returncode != 0:
             raise RuntimeError("%s failed: %s %s" % (name, res.stdout[-2000:], res.stderr[-2000:]))
         out = {"geo": g, "params": json.dumps({"lattice": lat, "periodic": periodic, "ranks": ranks, "tau": tau,
-                                               "force": force, "steps": steps})}
+                                               "force": force, "steps": steps, "blocks": blocks})}
         od = os.path.join(root, "output")
         for r in range(ranks):
             data = parse_vtu(last_vtu(od, "lb_run").replace("/0000_lb_run_", "/%04d_lb_run_" % r))

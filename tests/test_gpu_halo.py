@@ -19,26 +19,22 @@ import pytest
 import lbamd
 from helpers import OracleTopo, load_ref, ref_names, tp_start
 from lbamd.halo import exchange, exchange_lists, step_twophase
-from oracle.ref_run import rank_slabs
+from oracle.ref_run import rank_blocks
 
 pytestmark = pytest.mark.gpu
 
 
-def split_geometry(lattice, shape, n_ranks, seed):
+def split_geometry(lattice, shape, periodic, blocks, seed):
     if lattice == lbamd.D3Q19:
-        geo, _ = lbamd.sphere_pack(shape, 3.0, 0.45, seed, periodic="x", walls=True)
+        geo, _ = lbamd.sphere_pack(shape, 3.0, 0.45, seed, periodic=periodic, walls=True)
     else:
         rng = np.random.default_rng(seed)
         geo = (rng.random(shape) > 0.3).astype(np.int32)
-        geo[:, 0] = 0
-        geo[:, -1] = 0
+        if "y" not in periodic:
+            geo[:, 0] = 0
+            geo[:, -1] = 0
     fluid = (np.asarray(geo) > 0).astype(np.int32)
-    g = fluid.copy()
-    cut = np.linspace(0, shape[0], n_ranks + 1).astype(int)
-    for r in range(n_ranks):
-        sl = g[cut[r]:cut[r + 1]]
-        sl[sl > 0] = r + 1
-    return fluid, g
+    return fluid, rank_blocks(fluid, blocks)
 
 
 def initial(topo, cell_f):
@@ -53,8 +49,8 @@ def initial(topo, cell_f):
     return f
 
 
-def run_whole(lattice, fluid, precision, cell_f, nsteps, op):
-    t = lbamd.Topology.from_geometry(fluid, lattice, "x")
+def run_whole(lattice, fluid, periodic, precision, cell_f, nsteps, op):
+    t = lbamd.Topology.from_geometry(fluid, lattice, periodic)
     ctx = lbamd.Context(lattice, precision)
     ctx.set_topology(t.neighbors(), t.node_types(), t.bulk())
     ctx.upload_f(initial(t, cell_f))
@@ -66,8 +62,8 @@ def run_whole(lattice, fluid, precision, cell_f, nsteps, op):
     return out
 
 
-def run_split(lattice, g, n_ranks, precision, cell_f, nsteps, op):
-    topos = [lbamd.Topology.from_geometry(g, lattice, "x", rank=r + 1) for r in range(n_ranks)]
+def run_split(lattice, g, periodic, n_ranks, precision, cell_f, nsteps, op):
+    topos = [lbamd.Topology.from_geometry(g, lattice, periodic, rank=r + 1) for r in range(n_ranks)]
     lists = exchange_lists(topos)
     ctxs, reg = [], []
     for r, t in enumerate(topos):
@@ -100,23 +96,28 @@ def trt(ctx, n):
 
 
 @pytest.mark.parametrize("precision", [lbamd.FP64, lbamd.FP32S])
-@pytest.mark.parametrize("lattice,shape,n_ranks,op", [
-    (lbamd.D3Q19, (24, 16, 12), 2, bgk),
-    (lbamd.D3Q19, (30, 14, 12), 3, bgk),
-    (lbamd.D3Q19, (24, 16, 12), 2, trt),
-    (lbamd.D2Q9, (40, 18), 2, bgk),
+@pytest.mark.parametrize("lattice,shape,periodic,blocks,op", [
+    (lbamd.D3Q19, (24, 16, 12), "x", (2,), bgk),
+    (lbamd.D3Q19, (30, 14, 12), "x", (3,), bgk),
+    (lbamd.D3Q19, (24, 16, 12), "x", (2,), trt),
+    (lbamd.D2Q9, (40, 18), "x", (2,), bgk),
+    (lbamd.D3Q19, (20, 16, 10), "xy", (2, 2), bgk),
+    (lbamd.D3Q19, (18, 18, 12), "xyz", (3, 2), trt),
+    (lbamd.D2Q9, (24, 20), "xy", (2, 2), bgk),
 ])
-def test_split_equals_whole(gpu_available, precision, lattice, shape, n_ranks, op):
-    fluid, g = split_geometry(lattice, shape, n_ranks, 5)
+def test_split_equals_whole(gpu_available, precision, lattice, shape, periodic, blocks, op):
+    """Slabs, and blocks whose ranks also meet along edges and corners."""
+    fluid, g = split_geometry(lattice, shape, periodic, blocks, 5)
+    n_ranks = int(np.prod(blocks))
     Q = 19 if lattice == lbamd.D3Q19 else 9
     rng = np.random.default_rng(11)
     w = {19: np.array([2.0] * 3 + [1.0] * 6 + [2.0] * 3 + [1.0] * 6 + [12.0]) / 36.0,
          9: np.array([4.0, 1.0] * 4 + [16.0]) / 36.0}[Q]
     cell_f = w * (1.0 + 1e-3 * rng.standard_normal(shape + (Q,)))
     nsteps = 25
-    wp, wf, wr, wv = run_whole(lattice, fluid, precision, cell_f, nsteps, op)
+    wp, wf, wr, wv = run_whole(lattice, fluid, periodic, precision, cell_f, nsteps, op)
     where = {tuple(p): k for k, p in enumerate(wp)}
-    parts = run_split(lattice, g, n_ranks, precision, cell_f, nsteps, op)
+    parts = run_split(lattice, g, periodic, n_ranks, precision, cell_f, nsteps, op)
     assert sum(len(p[0]) for p in parts) == len(wp)
     for pos, f, rho, vel in parts:
         k = np.array([where[tuple(p)] for p in pos])
@@ -163,7 +164,7 @@ def test_split_vs_reference_mpi_run(gpu_available, name):
 
 
 def test_halo_refusals(gpu_available):
-    fluid, g = split_geometry(lbamd.D3Q19, (16, 10, 8), 2, 3)
+    fluid, g = split_geometry(lbamd.D3Q19, (16, 10, 8), "x", (2,), 3)
     t = lbamd.Topology.from_geometry(g, lbamd.D3Q19, "x", rank=1)
     lists = exchange_lists([t, lbamd.Topology.from_geometry(g, lbamd.D3Q19, "x", rank=2)])
     ctx = lbamd.Context(lbamd.D3Q19, lbamd.FP64)
@@ -234,7 +235,7 @@ def tp_fields(ctx, T):
             ctx.download_scalar(lbamd.PHI)[b])
 
 
-def tp_geometry(lattice, shape, periodic, ranks, seed):
+def tp_geometry(lattice, shape, periodic, blocks, seed):
     if lattice == lbamd.D3Q19:
         geo, _ = lbamd.sphere_pack(shape, 3.0, 0.5, seed, periodic=periodic, walls=periodic != "xyz")
         fluid = (np.asarray(geo) > 0).astype(np.int64)
@@ -246,20 +247,22 @@ def tp_geometry(lattice, shape, periodic, ranks, seed):
     rng = np.random.default_rng(seed + 1)
     rho0 = rng.choice([0.0, 0.3, 1.0], size=shape)
     wett = np.full(shape, 0.8)
-    return fluid, rank_slabs(fluid, ranks).astype(np.int64), rho0, wett
+    return fluid, rank_blocks(fluid, blocks).astype(np.int64), rho0, wett
 
 
 TP = dict(tau0=0.8, tau1=1.2, sigma=0.01, beta=0.7)
 
 
 @pytest.mark.parametrize("precision", [lbamd.FP64, lbamd.FP32S])
-@pytest.mark.parametrize("lattice,shape,periodic,ranks,mode", [
-    (lbamd.D3Q19, (20, 12, 10), "xyz", 2, lbamd.FORCE_FIXED),
-    (lbamd.D3Q19, (24, 12, 10), "x", 3, lbamd.FORCE_FIXED),
-    (lbamd.D2Q9, (40, 18), "x", 2, lbamd.FORCE_FIXED),
-    (lbamd.D3Q19, (20, 12, 10), "xyz", 2, lbamd.FORCE_MOMX),
+@pytest.mark.parametrize("lattice,shape,periodic,blocks,mode", [
+    (lbamd.D3Q19, (20, 12, 10), "xyz", (2,), lbamd.FORCE_FIXED),
+    (lbamd.D3Q19, (24, 12, 10), "x", (3,), lbamd.FORCE_FIXED),
+    (lbamd.D2Q9, (40, 18), "x", (2,), lbamd.FORCE_FIXED),
+    (lbamd.D3Q19, (20, 12, 10), "xyz", (2,), lbamd.FORCE_MOMX),
+    (lbamd.D3Q19, (20, 16, 10), "xyz", (2, 2), lbamd.FORCE_FIXED),
+    (lbamd.D2Q9, (24, 20), "x", (2, 2), lbamd.FORCE_FIXED),
 ])
-def test_twophase_split_equals_whole(gpu_available, precision, lattice, shape, periodic, ranks, mode):
+def test_twophase_split_equals_whole(gpu_available, precision, lattice, shape, periodic, blocks, mode):
     """The colour-gradient loop on R contexts (colour pass, colour exchange,
     collision pass, population exchange per iteration) against the whole
     domain in one context: bit for bit with a fixed force (the per-node
@@ -267,7 +270,8 @@ def test_twophase_split_equals_whole(gpu_available, precision, lattice, shape, p
     neighbour's value differs); with the momentum-controlled force to 1e-12
     of each field's scale (the flux is summed per rank, then over the
     ranks)."""
-    fluid, ranked, rho0, wett = tp_geometry(lattice, shape, periodic, ranks, 7)
+    ranks = int(np.prod(blocks))
+    fluid, ranked, rho0, wett = tp_geometry(lattice, shape, periodic, blocks, 7)
     force = [1e-4 if mode == lbamd.FORCE_MOMX else 3e-6, 1e-6, -2e-6][: 3 if lattice == lbamd.D3Q19 else 2]
     nsteps = 30
     ctx, T = tp_whole(fluid, lattice, periodic, precision, rho0, wett)
@@ -330,7 +334,7 @@ def test_twophase_colour_then_collide_equals_step_cg(gpu_available, precision, l
     """On one context without a halo, lb_step_cg_colour + lb_step_cg_collide
     per iteration (the flux handed back unchanged) give the bits of
     lb_step_cg."""
-    fluid, _, rho0, wett = tp_geometry(lbamd.D3Q19, (18, 14, 12), "x", 1, 3)
+    fluid, _, rho0, wett = tp_geometry(lbamd.D3Q19, (18, 14, 12), "x", (1,), 3)
     T = OracleTopo(fluid, lbamd.D3Q19, "x")
     f0, rho = tp_start({"rho0_attr": rho0, "wett_attr": wett}, T)
     force = [1e-4 if mode == lbamd.FORCE_MOMX else 3e-6, 1e-6, -2e-6]
@@ -356,7 +360,7 @@ def test_twophase_colour_then_collide_equals_step_cg(gpu_available, precision, l
 
 
 def test_twophase_split_refusals(gpu_available):
-    fluid, ranked, rho0, wett = tp_geometry(lbamd.D3Q19, (16, 10, 8), "x", 2, 5)
+    fluid, ranked, rho0, wett = tp_geometry(lbamd.D3Q19, (16, 10, 8), "x", (2,), 5)
     ctxs, reg, _ = tp_contexts(ranked, lbamd.D3Q19, "x", 2, lbamd.FP64, rho0, wett)
     ctx, k = ctxs[0], reg[0][1]
     # lb_step_cg cannot exchange the colour between its passes

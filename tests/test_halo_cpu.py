@@ -11,34 +11,35 @@ import lbamd
 import oracle
 from helpers import OracleTopo, load_ref, ref_names, tp_start
 from lbamd.halo import exchange_lists
+from oracle.ref_run import rank_blocks
 
 
-def split(shape, n_ranks, periodic, seed, lattice):
+def split(shape, blocks, periodic, seed, lattice):
     if lattice == lbamd.D3Q19:
         geo, _ = lbamd.sphere_pack(shape, 3.0, 0.45, seed, periodic=periodic, walls=True)
     else:
         rng = np.random.default_rng(seed)
         geo = (rng.random(shape) > 0.3).astype(np.int32)
-        geo[:, 0] = 0
-        geo[:, -1] = 0
-    g = np.asarray(geo, np.int32).copy()
-    cut = np.linspace(0, shape[0], n_ranks + 1).astype(int)
-    for r in range(n_ranks):
-        sl = g[cut[r]:cut[r + 1]]
-        sl[sl > 0] = r + 1
-    return g
+        if periodic == "x":
+            geo[:, 0] = 0
+            geo[:, -1] = 0
+    return rank_blocks(np.asarray(geo), blocks)
 
 
-@pytest.mark.parametrize("lattice,shape,n_ranks", [(lbamd.D3Q19, (24, 16, 12), 2), (lbamd.D3Q19, (30, 14, 12), 3),
-                                                   (lbamd.D2Q9, (40, 18), 2)])
-def test_exchange_lists_pair_up(lattice, shape, n_ranks):
-    g = split(shape, n_ranks, "x", 5, lattice)
-    topos = [lbamd.Topology.from_geometry(g, lattice, "x", rank=r + 1) for r in range(n_ranks)]
+@pytest.mark.parametrize("lattice,shape,periodic,blocks", [
+    (lbamd.D3Q19, (24, 16, 12), "x", (2,)), (lbamd.D3Q19, (30, 14, 12), "x", (3,)), (lbamd.D2Q9, (40, 18), "x", (2,)),
+    (lbamd.D3Q19, (20, 16, 10), "xy", (2, 2)), (lbamd.D3Q19, (18, 18, 12), "xyz", (3, 2)),
+    (lbamd.D2Q9, (24, 20), "xy", (2, 2))])
+def test_exchange_lists_pair_up(lattice, shape, periodic, blocks):
+    """Slabs and blocks (ranks that share only an edge or a corner exchange
+    the diagonal directions alone)."""
+    g = split(shape, blocks, periodic, 5, lattice)
+    n_ranks = int(np.prod(blocks))
+    topos = [lbamd.Topology.from_geometry(g, lattice, periodic, rank=r + 1) for r in range(n_ranks)]
     lists = exchange_lists(topos)
     pos = [t.positions() for t in topos]
     ranks = [t.node_ranks() for t in topos]
     nbs = [t.neighbors() for t in topos]
-    period = shape[0]
     n_pairs = 0
     for r in range(n_ranks):
         assert lists[r], "rank %d has no neighbour" % r
@@ -49,16 +50,24 @@ def test_exchange_lists_pair_up(lattice, shape, n_ranks):
             assert np.all(ranks[r][nbs[r][recv_nodes, dr]] == r)
             # r's send side: its own bulk nodes
             assert np.all(np.isin(ns, topos[r].bulk()))
-            # s receives exactly what r sends, value by value
+            # s receives exactly what r sends, value by value: the same cell
+            # up to the periodic wrap, the same direction
             other = [x for x in lists[s] if x[0] == r]
             assert len(other) == 1
             _, _, _, _, nr2, ndr2, dr2 = other[0]
             a, b = np.repeat(ns, nds), np.repeat(nr2, ndr2)
             assert len(a) == len(b) == len(ds) and np.array_equal(ds, dr2)
             d = pos[r][a] - pos[s][b]
-            assert np.all(d[:, 1:] == 0) and np.all(d[:, 0] % period == 0)
+            for ax in range(len(shape)):
+                if "xyz"[ax] in periodic:
+                    assert np.all(d[:, ax] % shape[ax] == 0)
+                else:
+                    assert np.all(d[:, ax] == 0)
             n_pairs += 1
     assert n_pairs == sum(len(x) for x in lists)
+    if len(blocks) == 2:
+        # the block of rank 1 touches the diagonal block along an edge only
+        assert any(x[0] == blocks[1] + 1 for x in lists[0])
 
 
 @pytest.mark.parametrize("name", ref_names("mpi_tp_"))
