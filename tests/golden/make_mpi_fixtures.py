@@ -13,9 +13,10 @@ elsewhere).  For every case:
      tests/test_topology.py), init_rho = 1; for twophase with the oracle's
      restatement of that writer, as make_ref_fixtures.py writes the one-rank
      twophase cases, with the attributes rho0, rho1, wettability and source;
-  2. std_case (D2Q9 as shipped, or its D3Q19 build) or twophase (D3Q19)
-     runs on R ranks under mpiexec, so the ranks exchange their halo through
-     BndMpi / MonLatMpi::communicateLbField (LBbndmpi.h:206-315,
+  2. std_case (D2Q9 as shipped, or its D3Q19 build) or twophase (D3Q19 as
+     shipped, or its D2Q9 build) runs on R ranks under mpiexec, so the
+     ranks exchange their halo through BndMpi /
+     MonLatMpi::communicateLbField (LBbndmpi.h:206-315,
      LBmonlatmpi.h:236-300) every step (std_case/main.cpp:147,
      main_TWOPHASE.cpp:386-387), twophase also the colour field
      (communciateScalarField, main_TWOPHASE.cpp:287) and the flux sum
@@ -105,24 +106,30 @@ def run(name, lat, fluid, periodic, ranks, tau, force, steps, blocks=None):
 
 
 def tp_cases():
-    """(name, fluid geometry, periodic, ranks, rho0, wettability, tau0, tau1,
-    sigma, beta, momx, force, steps)"""
+    """(name, lattice, fluid geometry, periodic, ranks, rho0, wettability,
+    tau0, tau1, sigma, beta, momx, force, steps)"""
     rng = np.random.default_rng(31)
     geo, _ = lbamd.sphere_pack((20, 12, 10), 3.0, 0.5, 3, periodic="xyz", walls=False)
     fluid = (geo > 0).astype(np.int32)
     r0 = rng.choice([0.0, 0.3, 1.0], size=fluid.shape)
-    yield ("pack_2ranks", fluid, "xyz", 2, r0, np.full(fluid.shape, 0.8), 0.8, 1.2, 0.01, 0.7, 1e-4,
+    yield ("pack_2ranks", "D3Q19", fluid, "xyz", 2, r0, np.full(fluid.shape, 0.8), 0.8, 1.2, 0.01, 0.7, 1e-4,
            [0.0, 1e-6, -2e-6], 60)
     geo, _ = lbamd.sphere_pack((24, 12, 10), 3.0, 0.55, 9, periodic="x", walls=True)
     fluid = (geo > 0).astype(np.int32)
     r0 = (rng.random(fluid.shape) > 0.5).astype(float)
-    yield ("walls_3ranks", fluid, "x", 3, r0, np.full(fluid.shape, 1.0), 1.0, 1.0, 0.01, 0.7, 5e-5,
+    yield ("walls_3ranks", "D3Q19", fluid, "x", 3, r0, np.full(fluid.shape, 1.0), 1.0, 1.0, 0.01, 0.7, 5e-5,
            [0.0, 0.0, 0.0], 40)
+    fluid = (rng.random((36, 16)) > 0.3).astype(np.int32)
+    fluid[:, 0] = 0
+    fluid[:, -1] = 0
+    r0 = (rng.random(fluid.shape) > 0.5).astype(float)
+    yield ("d2q9_walls_2ranks", "D2Q9", fluid, "x", 2, r0, np.full(fluid.shape, 0.8), 0.8, 1.2, 0.01, 0.7, 5e-5,
+           [0.0, -1e-6], 60)
 
 
-def run_tp(name, fluid, periodic, ranks, r0, wett, tau0, tau1, sigma, beta, momx, force, steps):
+def run_tp(name, lat, fluid, periodic, ranks, r0, wett, tau0, tau1, sigma, beta, momx, force, steps):
     g = rank_slabs(fluid, ranks)
-    geo = Geometry(g.astype(np.int64), "D3Q19", periodic)
+    geo = Geometry(g.astype(np.int64), lat, periodic)
     attrs = [("rho0", r0), ("rho1", 1.0 - r0), ("wettability", wett),
              ("source", np.zeros(g.shape, dtype=np.int64))]
     root = tempfile.mkdtemp(prefix="lbmpi_tp_")
@@ -137,12 +144,13 @@ def run_tp(name, fluid, periodic, ranks, r0, wett, tau0, tau1, sigma, beta, momx
                                        "sigma %s" % fmt(sigma), "beta %s" % fmt(beta), "momx %s" % fmt(momx)],
                                extra="outdir tp/\n"))
         env = dict(os.environ, LD_LIBRARY_PATH=os.path.join(REF, "mpilib"), OMP_NUM_THREADS="1")
-        res = subprocess.run([MPIEXEC, "-launcher", "fork", "-n", str(ranks), os.path.join(REF, "twophase")],
+        binary = os.path.join(REF, "twophase" if lat == "D3Q19" else "twophase_d2q9")
+        res = subprocess.run([MPIEXEC, "-launcher", "fork", "-n", str(ranks), binary],
                              cwd=root, env=env, capture_output=True, text=True, timeout=600)
         if res.returncode != 0:
             raise RuntimeError("%s failed: %s %s" % (name, res.stdout[-2000:], res.stderr[-2000:]))
         od = os.path.join(root, "output", "tp")
-        params = dict(case="twophase", lattice="D3Q19", periodic=periodic, ranks=ranks, tau0=tau0, tau1=tau1,
+        params = dict(case="twophase", lattice=lat, periodic=periodic, ranks=ranks, tau0=tau0, tau1=tau1,
                       sigma=sigma, beta=beta, momx=momx, force=force, steps=steps)
         out = {"geo": g.astype(np.int8), "params": json.dumps(params), "rho0_attr": r0, "wett_attr": wett,
                "force_x": np.loadtxt(os.path.join(od, "force.dat"), ndmin=2)}

@@ -312,14 +312,16 @@ def test_twophase_split_vs_reference_mpi_run(gpu_available, name):
     d = load_ref(name)
     p = d["params"]
     ranked = d["geo"].astype(np.int64)
-    ctxs, reg, topos = tp_contexts(ranked, lbamd.D3Q19, p["periodic"], p["ranks"], lbamd.FP64, d["rho0_attr"],
+    lattice = {"D2Q9": lbamd.D2Q9, "D3Q19": lbamd.D3Q19}[p["lattice"]]
+    ctxs, reg, topos = tp_contexts(ranked, lattice, p["periodic"], p["ranks"], lbamd.FP64, d["rho0_attr"],
                                    d["wett_attr"])
     force = [p["momx"]] + list(p["force"][1:])
     for _ in range(p["steps"] + 1):
         step_twophase(ctxs, reg, p["tau0"], p["tau1"], p["sigma"], p["beta"], lbamd.FORCE_MOMX, force)
     for r, (ctx, T) in enumerate(zip(ctxs, topos)):
         _, _, rho, vel, _ = tp_fields(ctx, T)
-        for got, ref in ((rho[:, 0], d["rho0_%d" % r]), (rho[:, 1], d["rho1_%d" % r]), (vel, d["vel_%d" % r])):
+        for got, ref in ((rho[:, 0], d["rho0_%d" % r]), (rho[:, 1], d["rho1_%d" % r]),
+                         (vel, d["vel_%d" % r][:, : vel.shape[1]])):
             assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max(), (name, r)
         F = ctx.last_force()[0]
         assert abs(F - d["force_x"][-1, 1]) <= 1e-12 * abs(d["force_x"][-1, 1])

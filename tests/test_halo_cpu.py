@@ -81,18 +81,19 @@ def test_reference_mpi_twophase_equals_oracle_whole_domain(name):
     d = load_ref(name)
     p = d["params"]
     ranked = d["geo"].astype(np.int64)
-    T = OracleTopo((ranked > 0).astype(np.int64), "D3Q19", p["periodic"])
+    T = OracleTopo((ranked > 0).astype(np.int64), p["lattice"], p["periodic"])
     f0, rho = tp_start(d, T)
-    _, r, v, _, F = oracle.run_twophase(1, T.neighbors, T.node_types, T.bulk, f0, rho, p["tau0"], p["tau1"],
-                                        p["sigma"], p["beta"], 1, [p["momx"]] + p["force"][1:], p["steps"] + 1)
+    _, r, v, _, F = oracle.run_twophase(T.lattice, T.neighbors, T.node_types, T.bulk, f0, rho, p["tau0"],
+                                        p["tau1"], p["sigma"], p["beta"], 1, [p["momx"]] + p["force"][1:],
+                                        p["steps"] + 1)
     where = {tuple(x): n for n, x in enumerate(T.positions)}
     seen = 0
     for rank in range(p["ranks"]):
-        Tr = OracleTopo(ranked, "D3Q19", p["periodic"], rank=rank + 1)
+        Tr = OracleTopo(ranked, p["lattice"], p["periodic"], rank=rank + 1)
         at = np.array([where[tuple(x)] for x in Tr.positions[Tr.bulk]])
         seen += at.size
         for got, ref in ((r[at, 0], d["rho0_%d" % rank]), (r[at, 1], d["rho1_%d" % rank]),
-                         (v[at], d["vel_%d" % rank])):
+                         (v[at], d["vel_%d" % rank][:, : v.shape[1]])):
             assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max(), (name, rank)
     assert seen == T.bulk.size
     assert abs(F[0] - d["force_x"][-1, 1]) <= 1e-12 * abs(d["force_x"][-1, 1])
