@@ -375,7 +375,7 @@ int lb_topo_destroy(lb_topo* t);
 int32_t lb_topo_num_nodes(const lb_topo* t);  /* incl. node 0 */
 int32_t lb_topo_num_bulk(const lb_topo* t);
 int32_t lb_topo_num_dims(const lb_topo* t);
-int32_t lb_topo_rank(const lb_topo* t);
+int32_t lb_topo_rank(const lb_topo* t);      /* the MPI rank of this node list, 0-based (the file's process) */
 int lb_topo_global_dims(const lb_topo* t, int32_t* dims);  /* padded dims */
 int lb_topo_neighbors(const lb_topo* t, int32_t* neighbor);   /* [n_nodes*Q] */
 int lb_topo_positions(const lb_topo* t, int32_t* pos);        /* [n_nodes*nD], node 0 = -1 */
