@@ -888,7 +888,8 @@ int step_single(lb_ctx* c, int32_t nsteps, const BgkArgs& a) {
   };
   int32_t s = 0;
   hipGraphExec_t run = nullptr;
-  if (nsteps >= kGraphMinSteps && c->precision != LB_FP16S) {
+  // (no bulk node: the launches enqueue nothing, so there is nothing to capture)
+  if (nsteps >= kGraphMinSteps && c->precision != LB_FP16S && c->n_bulk > 0) {
     if (!c->post) {
       hipError_t e = launch_single(c, a, c->cur, true, false);
       if (e != hipSuccess) return fail(c, LB_ERR_HIP, "bgk launch: %s", hipGetErrorString(e));
