@@ -1298,14 +1298,15 @@ int lb_step_cg_colour(lb_ctx* c, double* flux_x) {
   if (rc != LB_OK) return rc;
   hipSetDevice(c->device);
   const bool momx = flux_x != nullptr;
+  Timer t(c);
   hipError_t e = colour_pass(c, cg_rowwin(c), momx);
   // the block partials summed as the momentum-controlled step sums them
   // (slot 1 of the scalar buffer; slot 0 receives -2 sum, unused)
   if (e == hipSuccess && momx) e = finish_momx(c->d_partial, c->n_blocks, 0.0, 1, c->d_scalar, c->stream);
-  if (e == hipSuccess && momx)
-    e = hipMemcpyAsync(flux_x, c->d_scalar + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess && momx) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return fail(c, LB_ERR_HIP, "colour pass: %s", hipGetErrorString(e));
+  rc = t.stop(momx ? 2 : 1);
+  if (rc != LB_OK) return rc;
+  if (momx) LB_HIP(c, hipMemcpy(flux_x, c->d_scalar + 1, sizeof(double), hipMemcpyDeviceToHost));
   c->cg_colour_done = true;
   return LB_OK;
 }

@@ -31,8 +31,9 @@ batched  = the ensemble of independent sphere-pack permeability solves (cubic
            the whole ensemble (wall clock from the first admission to the last
            result, host set-up of each solve included), against the plain FP64
            path run one solve at a time (lb_solve_permeability on its own
-           context), timed on a size-stratified subset (every m-th solve by
-           fluid count) and scaled by m; k error of FP32S vs FP64 on that subset.
+           context), timed on every solve (--baseline-every m: every m-th by
+           fluid count, scaled by m); k error of FP32S and FP16S vs FP64 on
+           the solves timed.
 channel  = the D2Q9 256x128 channel (solid rows y = 0, 127, periodic x, tau 1,
            F 1e-6, 5000 steps from rest) in FP64: MLUPS and the velocity
            profile against the Poiseuille start-up series.
@@ -57,9 +58,9 @@ serving  = a stream of requests, 70 % permeability (single-phase) and 30 %
            transient).  FP32S: a burst (all requests at t = 0) gives the
            capacity in solves/s, then Poisson arrivals at 0.7 x that capacity
            give p50/p99 latency; FP16S: the burst.  The same stop rules on the
-           plain FP64 path one request at a time (every relative-permeability
-           request and a size-stratified subset of the permeability requests,
-           scaled) give the baseline solves/s and the k errors.
+           plain FP64 path one request at a time (every request;
+           --serve-baseline-every m times every m-th permeability request by
+           size and scales) give the baseline solves/s and the k errors.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256]
 """
@@ -339,9 +340,9 @@ def batched_leg(n, every, tau, force):
     log("batched FP16S: %d solves in %.2f s, %.1f GLUPS device" % (n, wall16, lups16 / (kms16 * 1e-3) / 1e9))
     log("batched FP32S: %d solves in %.2f s (admission %.2f s), %.1f GLUPS device, tuned %s" %
         (n, wall, t_adm, lups / (kms * 1e-3) / 1e9, shape))
-    # plain FP64 one at a time on a size-stratified subset
+    # plain FP64 one at a time: every solve, or a size-stratified subset
     order = np.argsort(nfluid, kind="stable")
-    subset = order[every // 2::every]
+    subset = order[every // 2::every] if every > 1 else order
     t_one = 0.0
     errs, errs16, steps64 = [], [], []
     for i in subset:
@@ -381,8 +382,9 @@ def batched_leg(n, every, tau, force):
            "k_rel_err_fp16s_vs_fp64_max": float(max(errs16)),
            "converged": int(sum(1 for r in res.values() if r["status"] == 0)),
            "fp16s_converged": int(sum(1 for r in res16.values() if r["status"] == 0))}
-    log("FP64 one at a time (subset of %d, scaled): %.2f solves/s; batched %.2f solves/s (x%.2f); "
-        "max k err %.2e" % (len(subset), out["fp64_one_at_a_time_solves_per_s"], out["solves_per_s"],
+    log("FP64 one at a time (%d of %d solves timed%s): %.2f solves/s; batched %.2f solves/s (x%.2f); "
+        "max k err %.2e" % (len(subset), n, "" if len(subset) == n else ", scaled",
+                            out["fp64_one_at_a_time_solves_per_s"], out["solves_per_s"],
                             out["speedup_vs_fp64_one_at_a_time"], out["k_rel_err_fp32s_vs_fp64_max"]))
     return out
 
@@ -547,8 +549,9 @@ def serving_leg(n_req, pool_gb, every):
     pool = int(pool_gb * 1e9)
     base_reqs = {r.tag: r for r in serve_requests(n_req, 7)}
     # 3) the plain FP64 path one request at a time: every relative-permeability
-    #    request, and a subset of the permeability requests stratified by size
-    #    (every m-th by cell count); each kind's time scaled by its own count
+    #    request, and every permeability request (or, with every > 1, a subset
+    #    stratified by size, every m-th by cell count, its time scaled by the
+    #    kind's count)
     ref, est, n_sub = {}, 0.0, 0
     for kind in ("perm", "relperm"):
         tags = sorted((t for t in range(n_req) if base_reqs[t].kind == kind), key=lambda t: base_reqs[t].geo.size)
@@ -647,7 +650,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-perm", action="store_true")
     ap.add_argument("--batch", type=int, default=512, help="ensemble size (0: skip)")
-    ap.add_argument("--baseline-every", type=int, default=16)
+    ap.add_argument("--baseline-every", type=int, default=1,
+                    help="time the FP64 one-at-a-time baseline on every m-th solve (1: all)")
+    ap.add_argument("--serve-baseline-every", type=int, default=1,
+                    help="time the serving baseline on every m-th permeability request (1: all)")
     ap.add_argument("--serve", type=int, default=32, help="requests per serving run (0: skip)")
     ap.add_argument("--no-channel", action="store_true")
     ap.add_argument("--no-trt", action="store_true")
@@ -714,7 +720,7 @@ def main():
         log("channel: %.1f MLUPS, profile error vs start-up series %.2e" % (channel["MLUPS"],
                                                                           channel["profile_err_vs_startup_series"]))
     twophase = twophase_leg(args.tp_size, args.tp_steps) if args.tp_steps > 0 else None
-    serving = serving_leg(args.serve, args.serve_pool_gb, 8) if args.serve > 0 else None
+    serving = serving_leg(args.serve, args.serve_pool_gb, args.serve_baseline_every) if args.serve > 0 else None
 
     cpu = None
     if not args.no_cpu:
