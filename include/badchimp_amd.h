@@ -100,7 +100,11 @@ int lb_set_topology(lb_ctx* ctx, int32_t n_nodes, const int32_t* neighbor,
  * cell coordinate per node and a cached per-row bit mask instead of one
  * 4-byte link per direction.  The layout is used only after every link it
  * yields has been checked against the neighbour index; otherwise the table
- * layout stays (lb_layout_note says why).  Results are identical in both. */
+ * layout stays (lb_layout_note says why).  Results are identical in both in
+ * LB_FP64 and LB_FP32S.  LB_FP16S seeds its stochastic rounding by a node's
+ * place in the layout's node order, which the row layout changes when the
+ * box has more than 32 rows along y: the two layouts then agree to rounding
+ * noise. */
 int lb_set_topology_grid(lb_ctx* ctx, int32_t n_nodes, const int32_t* neighbor,
                          const int16_t* node_type, const int32_t* bulk,
                          int32_t n_bulk, const int32_t* pos, int32_t flags);

@@ -66,6 +66,9 @@ def test_reduced_precision_vs_oracle(gpu_available, precision, vtol):
 
 @pytest.mark.parametrize("precision", [lbamd.FP64, lbamd.FP32S, lbamd.FP16S])
 def test_layouts_give_identical_results(gpu_available, precision):
+    # 20 rows along y fit one 32-row tile, so the row layout keeps the bulk
+    # order and FP16S, which seeds its rounding by a node's place in that
+    # order, gives the same bits too
     geo, T = pack((24, 20, 18), 3.0, 0.5, 5)
     rng = np.random.default_rng(2)
     f0 = oracle.rest_state(1, T.n_nodes, T.bulk, np.ones(T.n_nodes))
