@@ -14,6 +14,7 @@
 #include "../../include/badchimp_amd.h"
 #include "ctx.hpp"
 #include "errors.hpp"
+#include "host_threads.hpp"
 #include "kernels.hpp"
 #include "kernels_f32.hpp"
 #include "lattice.hpp"
@@ -210,22 +211,43 @@ bool build_rows(int lattice, int32_t n_nodes, const int32_t* nb, const int32_t* 
     return lattice == LB_D2Q9 ? D2Q9::c(q, d) : D3Q19::c(q, d);
   };
   if (n_bulk == 0) { why = "no bulk nodes"; return false; }
-  int32_t lo[3], hi[3], period[3] = {0, 0, 0};
-  for (int a = 0; a < 3; ++a) { lo[a] = P(bulk[0], a); hi[a] = lo[a]; }
-  for (int32_t k = 0; k < n_bulk; ++k)
-    for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], P(bulk[k], a)); hi[a] = std::max(hi[a], P(bulk[k], a)); }
-  // periods from the fluid links that wrap
-  for (int32_t k = 0; k < n_bulk; ++k) {
-    const int32_t n = bulk[k];
-    for (int q = 0; q < Q - 1; ++q) {
-      const int32_t m = nb[static_cast<size_t>(n) * Q + q];
-      if (m <= 0 || bidx[m] < 0) continue;
+  // extent and periods over chunks of the bulk list (one slot per thread):
+  // the periods come from the fluid links that wrap
+  int32_t tlo[kMaxHostThreads][3], thi[kMaxHostThreads][3], tper[kMaxHostThreads][3];
+  bool tbad[kMaxHostThreads];
+  for (int t = 0; t < kMaxHostThreads; ++t) {
+    tbad[t] = false;
+    for (int a = 0; a < 3; ++a) { tlo[t][a] = P(bulk[0], a); thi[t][a] = tlo[t][a]; tper[t][a] = 0; }
+  }
+  parallel_ranges(n_bulk, [&](int64_t b, int64_t e, int t) {
+    for (int64_t k = b; k < e; ++k) {
+      const int32_t n = bulk[k];
       for (int a = 0; a < 3; ++a) {
-        const int32_t d = std::abs(P(m, a) - P(n, a) - C(q, a));
-        if (d == 0) continue;
-        if (period[a] == 0) period[a] = d;
-        else if (period[a] != d) { why = "inconsistent periodic wrap"; return false; }
+        tlo[t][a] = std::min(tlo[t][a], P(n, a));
+        thi[t][a] = std::max(thi[t][a], P(n, a));
       }
+      for (int q = 0; q < Q - 1; ++q) {
+        const int32_t m = nb[static_cast<size_t>(n) * Q + q];
+        if (m <= 0 || bidx[m] < 0) continue;
+        for (int a = 0; a < 3; ++a) {
+          const int32_t d = std::abs(P(m, a) - P(n, a) - C(q, a));
+          if (d == 0) continue;
+          if (tper[t][a] == 0) tper[t][a] = d;
+          else if (tper[t][a] != d) tbad[t] = true;
+        }
+      }
+    }
+  });
+  int32_t lo[3], hi[3], period[3] = {0, 0, 0};
+  for (int a = 0; a < 3; ++a) { lo[a] = tlo[0][a]; hi[a] = thi[0][a]; }
+  for (int t = 0; t < kMaxHostThreads; ++t) {
+    if (tbad[t]) { why = "inconsistent periodic wrap"; return false; }
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::min(lo[a], tlo[t][a]);
+      hi[a] = std::max(hi[a], thi[t][a]);
+      if (tper[t][a] == 0) continue;
+      if (period[a] == 0) period[a] = tper[t][a];
+      else if (period[a] != tper[t][a]) { why = "inconsistent periodic wrap"; return false; }
     }
   }
   int32_t E[3];
@@ -251,26 +273,51 @@ bool build_rows(int lattice, int32_t n_nodes, const int32_t* nb, const int32_t* 
       X[a] = v;
     }
   };
-  std::vector<int64_t> key(n_bulk);
-  for (int32_t k = 0; k < n_bulk; ++k) {
-    int32_t X[3];
-    cell(bulk[k], X);
-    key[k] = static_cast<int64_t>(g.row(X[0], X[1])) * (static_cast<int64_t>(g.W) * 32) + X[2];
-  }
+  // compact order: by (row, cell along the row), i.e. bucketed by row, each
+  // row's nodes then ordered by their cell
+  std::vector<int32_t> row_of(n_bulk), cz(n_bulk);
+  parallel_ranges(n_bulk, [&](int64_t b, int64_t e, int) {
+    for (int64_t k = b; k < e; ++k) {
+      int32_t X[3];
+      cell(bulk[k], X);
+      row_of[k] = g.row(X[0], X[1]);
+      cz[k] = X[2];
+    }
+  });
+  std::vector<int64_t> start(static_cast<size_t>(R) + 1, 0);
+  for (int32_t k = 0; k < n_bulk; ++k) ++start[static_cast<size_t>(row_of[k]) + 1];
+  for (int64_t r = 0; r < R; ++r) start[r + 1] += start[r];
   rb.order.resize(n_bulk);
-  for (int32_t k = 0; k < n_bulk; ++k) rb.order[k] = k;
-  std::stable_sort(rb.order.begin(), rb.order.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
-  for (int32_t i = 1; i < n_bulk; ++i)
-    if (key[rb.order[i]] == key[rb.order[i - 1]]) { why = "two bulk nodes share a cell"; return false; }
+  {
+    std::vector<int64_t> fill(start.begin(), start.end() - 1);
+    for (int32_t k = 0; k < n_bulk; ++k) rb.order[fill[row_of[k]]++] = k;
+  }
+  bool dup[kMaxHostThreads] = {};
+  parallel_ranges(R, [&](int64_t b, int64_t e, int t) {
+    for (int64_t r = b; r < e; ++r) {
+      int32_t* o0 = rb.order.data() + start[r];
+      int32_t* o1 = rb.order.data() + start[r + 1];
+      std::sort(o0, o1, [&](int32_t x, int32_t y) { return cz[x] < cz[y]; });
+      for (int32_t* o = o0 + 1; o < o1; ++o)
+        if (cz[*o] == cz[*(o - 1)]) dup[t] = true;
+    }
+  });
+  for (int t = 0; t < kMaxHostThreads; ++t)
+    if (dup[t]) { why = "two bulk nodes share a cell"; return false; }
   rb.coord.resize(n_bulk);
   rb.rows.assign(static_cast<size_t>(R) * g.W, uint2{0u, 0u});
-  for (int32_t i = 0; i < n_bulk; ++i) {
-    int32_t X[3];
-    cell(bulk[rb.order[i]], X);
-    rb.coord[i] = (static_cast<uint32_t>(X[0]) << 22) | (static_cast<uint32_t>(X[1]) << 12) | static_cast<uint32_t>(X[2]);
-    uint2& e = rb.rows[static_cast<size_t>(g.row(X[0], X[1])) * g.W + (X[2] >> 5)];
-    e.y |= 1u << (X[2] & 31);
-  }
+  // packed cells per compact node; the row bit masks row by row (a row's
+  // nodes are consecutive in the compact order)
+  parallel_ranges(R, [&](int64_t b, int64_t e, int) {
+    for (int64_t r = b; r < e; ++r)
+      for (int64_t i = start[r]; i < start[r + 1]; ++i) {
+        int32_t X[3];
+        cell(bulk[rb.order[i]], X);
+        rb.coord[i] = (static_cast<uint32_t>(X[0]) << 22) | (static_cast<uint32_t>(X[1]) << 12) |
+                      static_cast<uint32_t>(X[2]);
+        rb.rows[static_cast<size_t>(r) * g.W + (X[2] >> 5)].y |= 1u << (X[2] & 31);
+      }
+  });
   uint32_t run = 0;
   for (auto& e : rb.rows) {
     e.x = run;
@@ -392,31 +439,38 @@ int setup_topology(lb_ctx* c, int32_t n_nodes, const int32_t* neighbor, const in
         l = (kLinkBounce << 30) | static_cast<uint32_t>(wall_slot[s]);
       }
       link[static_cast<size_t>(q) * NP + i] = l;
-      if (rows_ok) {
-        const auto r = row_source(c->lattice, rb, q, i);
-        const bool stream = (l >> 30) == kLinkStream;
-        if (r.first != stream || (stream && r.second != static_cast<int64_t>(l & kLinkIndexMask))) {
-          rows_ok = false;
-          why = "row layout disagrees with the neighbour index";
-        }
-      }
     }
+  }
+  // a layout is used only when every link it yields is the table's (checked
+  // over chunks of the compact nodes on host threads)
+  auto agrees = [&](auto&& source) {
+    bool bad[kMaxHostThreads] = {};
+    parallel_ranges(n_bulk, [&](int64_t b, int64_t e, int t) {
+      for (int64_t i = b; i < e && !bad[t]; ++i)
+        for (int q = 0; q < Q - 1; ++q) {
+          const uint32_t l = link[static_cast<size_t>(q) * NP + i];
+          const auto r = source(q, static_cast<int32_t>(i));
+          const bool stream = (l >> 30) == kLinkStream;
+          if (r.first != stream || (stream && r.second != static_cast<int64_t>(l & kLinkIndexMask))) {
+            bad[t] = true;
+            break;
+          }
+        }
+    });
+    return std::none_of(bad, bad + kMaxHostThreads, [](bool x) { return x; });
+  };
+  if (rows_ok && !agrees([&](int q, int32_t i) { return row_source(c->lattice, rb, q, i); })) {
+    rows_ok = false;
+    why = "row layout disagrees with the neighbour index";
   }
   // row windows when the row axis is not periodic, checked link by link
   std::vector<uint4> win;
   int32_t PY = 0;
   bool win_ok = rows_ok && win_allowed && build_windows(rb, win, PY, why);
-  for (int32_t i = 0; win_ok && i < n_bulk; ++i)
-    for (int q = 0; q < Q - 1; ++q) {
-      const uint32_t l = link[static_cast<size_t>(q) * NP + i];
-      const auto r = win_source(c->lattice, rb, win, PY, q, i);
-      const bool stream = (l >> 30) == kLinkStream;
-      if (r.first != stream || (stream && r.second != static_cast<int64_t>(l & kLinkIndexMask))) {
-        win_ok = false;
-        why = "row windows disagree with the neighbour index";
-        break;
-      }
-    }
+  if (win_ok && !agrees([&](int q, int32_t i) { return win_source(c->lattice, rb, win, PY, q, i); })) {
+    win_ok = false;
+    why = "row windows disagree with the neighbour index";
+  }
   // device allocation: the context holds no topology until every buffer
   // below is in place, so a failure leaves no pointer to freed memory
   hipStreamSynchronize(c->stream);

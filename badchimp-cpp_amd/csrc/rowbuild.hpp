@@ -128,10 +128,6 @@ inline bool build_windows(const RowBuild& rb, std::vector<uint4>& win, int32_t& 
   const int32_t PX = g.E[0] + 2;
   PY = g.E[1] + 2;
   win.assign(static_cast<size_t>(PX) * PY * g.W, uint4{0u, 0u, 0u, 0u});
-  auto fluid = [&](size_t r, int32_t z) -> bool {
-    if (z < 0 || z >= g.E[2]) return false;
-    return (rb.rows[r * g.W + (z >> 5)].y >> (z & 31)) & 1u;
-  };
   auto first_at_or_after = [&](size_t r, int32_t z) -> uint32_t {
     // compact index of the first fluid cell of row r at or after z (or the
     // index one past the row's last fluid cell)
@@ -150,13 +146,17 @@ inline bool build_windows(const RowBuild& rb, std::vector<uint4>& win, int32_t& 
       inside = RowGeom::wrap(Y, g.E[1], g.periodic & 2) && inside;
       if (!inside) continue;
       const size_t r = static_cast<size_t>(g.row(X, Y));
+      const uint2* words = &rb.rows[r * g.W];
       for (int32_t w = 0; w < g.W; ++w) {
-        const int32_t s0 = 32 * w - 16;
-        uint64_t m = 0;
-        for (int k = 0; k < 64; ++k)
-          if (fluid(r, s0 + k)) m |= 1ull << k;
+        // cells 32w - 16 .. 32w + 47 of the row: the upper half of word
+        // w - 1, word w, the lower half of word w + 1 (no cells before z = 0
+        // or after the row's last word; a word holds no bits beyond E_Z)
+        const uint64_t prev = w > 0 ? words[w - 1].y : 0u;
+        const uint64_t cur = words[w].y;
+        const uint64_t next = w + 1 < g.W ? words[w + 1].y : 0u;
+        const uint64_t m = (prev >> 16) | (cur << 16) | (next << 48);
         win[(static_cast<size_t>(px) * PY + py) * g.W + w] =
-            uint4{first_at_or_after(r, s0), static_cast<uint32_t>(m), static_cast<uint32_t>(m >> 32), 0u};
+            uint4{first_at_or_after(r, 32 * w - 16), static_cast<uint32_t>(m), static_cast<uint32_t>(m >> 32), 0u};
       }
     }
   return true;

@@ -125,6 +125,16 @@ def setup_leg(geo, info, topo, text_size=128):
     dev.close()
     out = {"nodes": int(topo.n_nodes), "host_builder_s": info.get("host_builder_s"), "device_builder_s": t_dev,
            "device_equals_host": bool(same)}
+    # the context's layout from the node list: link table, row windows, every
+    # link of both checked against the table (lb_set_topology_grid)
+    nb, types, bulk, pos = topo.neighbors(), topo.node_types(), topo.bulk(), topo.positions()
+    ctx = lbamd.Context(lbamd.D3Q19, lbamd.FP64)
+    t0 = time.perf_counter()
+    ctx.set_topology_grid(nb, types, bulk, pos)
+    ctx.sync()
+    out["layout_s"] = time.perf_counter() - t0
+    out["layout"] = ctx.layout_note
+    ctx.close()
     with tempfile.TemporaryDirectory() as tmp:
         path = os.path.join(tmp, "t.vtklbb")
         t0 = time.perf_counter()
@@ -152,9 +162,10 @@ def setup_leg(geo, info, topo, text_size=128):
                     "text_bytes": os.path.getsize(tp), "binary_read_same_case_s": t_bin,
                     "text_equals_binary": bool(all(np.array_equal(getattr(a, m)(), getattr(b, m)())
                                                    for m in ("neighbors", "positions", "node_types", "bulk")))})
-    log("set-up: host builder %.2f s, device builder %.3f s (same: %s); binary twin write %.2f s, read %.2f s; "
-        "text %d^3 read %.2f s vs binary %.3f s" % (out["host_builder_s"], t_dev, same, out["binary_write_s"],
-                                                    out["binary_read_s"], text_size, t_text, t_bin))
+    log("set-up: host builder %.2f s, device builder %.3f s (same: %s); context layout %.2f s; binary twin "
+        "write %.2f s, read %.2f s; text %d^3 read %.2f s vs binary %.3f s" % (
+            out["host_builder_s"], t_dev, same, out["layout_s"], out["binary_write_s"], out["binary_read_s"],
+            text_size, t_text, t_bin))
     return out
 
 
