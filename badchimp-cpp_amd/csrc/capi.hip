@@ -1399,8 +1399,9 @@ int lb_solve_relperm(lb_ctx* c, double tau0, double tau1, double sigma, double b
   if (rc != LB_OK) return rc;
   if (c->n_ghost > 0)
     return fail(c, LB_ERR_UNSUPPORTED,
-                "colour-gradient steps with other-rank neighbours would need the colour of the halo nodes "
-                "between the two passes; the library exchanges populations only");
+                "lb_solve_relperm: a rank of a split domain exchanges colour and populations between the passes "
+                "and sums the flux over the ranks; step with lb_step_cg_colour / lb_step_cg_collide and the "
+                "lb_halo_* calls, and reduce with lb_reduce_flux");
   if (!force || !(tau0 > 0.5) || !(tau1 > 0.5) || check_every < 1 || !(tol > 0.0) || max_steps < 1 ||
       !(n_box > 0.0) || force[0] == 0.0)
     return fail(c, LB_ERR_ARG, "lb_solve_relperm: bad argument");
@@ -1478,6 +1479,11 @@ int solve_permeability(lb_ctx* c, const char* name, double tau, double tau_anti,
                        int64_t* steps_out) {
   int rc = check_ready(c, 1);
   if (rc != LB_OK) return rc;
+  if (c->n_ghost > 0)
+    return fail(c, LB_ERR_UNSUPPORTED,
+                "%s: a rank of a split domain exchanges its halo between steps and sums the flux over the ranks; "
+                "step with lb_step_bgk / lb_step_trt and the lb_halo_* calls, and reduce with lb_reduce_flux",
+                name);
   if (!force || !(tau > 0.5) || check_every < 1 || !(tol > 0.0) || max_steps < 1 || !(n_box > 0.0) ||
       force[0] == 0.0)
     return fail(c, LB_ERR_ARG, "%s: bad argument", name);

@@ -132,7 +132,9 @@ int lb_set_global_bulk_count(lb_ctx* ctx, int64_t n);
  * nodes between its two passes: it runs as lb_step_cg_colour, the colour
  * exchange (lb_halo_pack_colour / lb_halo_unpack_colour below), then
  * lb_step_cg_collide; lb_step_cg refuses a context with halo nodes
- * (LB_ERR_UNSUPPORTED), and so do tagged boundaries.
+ * (LB_ERR_UNSUPPORTED), and so do tagged boundaries and the solves
+ * (lb_solve_permeability, lb_solve_permeability_trt, lb_solve_relperm: their
+ * stop tests sum over one context; a split domain sums over the ranks).
  *
  * lb_halo_add registers the exchange with one neighbouring rank from the
  * lists MonLatMpi holds after BndMpi::setup (LBbndmpi.h:206-315, the

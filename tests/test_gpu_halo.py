@@ -188,6 +188,10 @@ def test_halo_refusals(gpu_available):
     assert e.value.code == -2
     ctx.step_bgk(1, 1.0, [1e-6, 0.0, 0.0])
     assert ctx.halo_pack(k).shape == (1, int(nds.sum()))
+    # a solve's stop test sums over one context: refused on a rank of a split domain
+    with pytest.raises(lbamd.LBError) as e:
+        ctx.solve_permeability(1.0, np.array([1e-6, 0.0, 0.0]), 10, 1e-6, 100, 1000.0)
+    assert e.value.code == -4
     ctx.close()
 
 
@@ -379,6 +383,9 @@ def test_twophase_split_refusals(gpu_available):
     ctx.step_cg_colour()
     assert ctx.halo_pack_colour(k).shape == (ctx._halo_node_counts[k][0],)
     ctx.step_cg_collide(force_mode=lbamd.FORCE_FIXED, force=[1e-6, 0.0, 0.0], **TP)
+    with pytest.raises(lbamd.LBError) as e:
+        ctx.solve_relperm(TP["tau0"], TP["tau1"], TP["sigma"], TP["beta"], [1e-6, 0.0, 0.0], 10, 1e-6, 100, 1000.0)
+    assert e.value.code == -4
     # one colour pass per iteration: the next collision needs a new one
     with pytest.raises(lbamd.LBError) as e:
         ctx.step_cg_collide(force_mode=lbamd.FORCE_FIXED, force=[1e-6, 0.0, 0.0], **TP)
