@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -409,38 +410,75 @@ int setup_topology(lb_ctx* c, int32_t n_nodes, const int32_t* neighbor, const in
   if (rows_ok && static_cast<uint64_t>(Q) * static_cast<uint64_t>(NP) * elem_size(c->precision) >= (1ull << 32))
     why = "population buffer above 4 GiB";
   const bool win_allowed = static_cast<uint64_t>(Q) * static_cast<uint64_t>(NP) * elem_size(c->precision) < (1ull << 32);
-  std::vector<uint32_t> link(static_cast<size_t>(Q - 1) * NP, 0u);
+  // The pull table, built over chunks of the compact nodes on host threads.
+  // A wall link first holds the wall's node number; the walls then get their
+  // slots in the order of first use (compact node, then direction), which
+  // one sequential pass over the nodes with wall links assigns.
+  if (n_nodes > static_cast<int32_t>(kLinkIndexMask))
+    return fail(c, LB_ERR_UNSUPPORTED, "node list of %d nodes: at most %u", n_nodes, kLinkIndexMask);
+  std::vector<uint32_t> link(static_cast<size_t>(Q - 1) * NP, 0u);  // padding nodes link nowhere
+  std::vector<uint32_t> wall_dirs(n_bulk);  // bit q: direction q links to a wall
+  struct LinkError {
+    int64_t at = INT64_MAX;  // i (Q - 1) + q of the first refused link of a chunk
+    int32_t node = 0, src = 0, q = 0;
+  };
+  LinkError err[kMaxHostThreads];
+  bool ghost0[kMaxHostThreads] = {};
+  const bool allow0 = (flags & LB_TOPO_ALLOW_GHOST0) != 0;
+  parallel_ranges(n_bulk, [&](int64_t b, int64_t e, int t) {
+    for (int64_t i = b; i < e; ++i) {
+      const int32_t n = bulk[order[i]];
+      const int32_t* nb = neighbor + static_cast<size_t>(n) * Q;
+      uint32_t wd = 0u;
+      for (int q = 0; q < Q - 1; ++q) {
+        const int32_t s = nb[rev_of(c->lattice, q)];
+        uint32_t l = 0u;
+        if (compact[s] >= 0) {
+          l = (kLinkStream << 30) | static_cast<uint32_t>(compact[s]);
+        } else if (node_type[s] > 1 || (s == 0 && !allow0)) {
+          if (err[t].at == INT64_MAX) err[t] = LinkError{i * (Q - 1) + q, n, s, q};
+        } else if (s == 0) {
+          l = (kLinkGhost0 << 30) | static_cast<uint32_t>(winner[rev_of(c->lattice, q)]);
+          ghost0[t] = true;
+        } else {
+          l = (kLinkBounce << 30) | static_cast<uint32_t>(s);
+          wd |= 1u << q;
+        }
+        link[static_cast<size_t>(q) * NP + i] = l;
+      }
+      wall_dirs[i] = wd;
+    }
+  });
+  // the first refused link in (compact node, direction) order, as one pass
+  // over the nodes would find it
+  const LinkError* first = std::min_element(err, err + kMaxHostThreads,
+                                            [](const LinkError& x, const LinkError& y) { return x.at < y.at; });
+  if (first->at != INT64_MAX) {
+    if (first->src != 0)
+      return fail(c, LB_ERR_UNSUPPORTED,
+                  "bulk node %d has a fluid neighbour %d on another rank: halo exchange is not available on one GPU",
+                  first->node, first->src);
+    return fail(c, LB_ERR_GHOST0_LINK,
+                "bulk node %d links to the ghost node 0 (direction %d); make the geometry periodic or walled, "
+                "or pass LB_TOPO_ALLOW_GHOST0 to reproduce the reference's shared-slot result", first->node, first->q);
+  }
+  if (std::any_of(ghost0, ghost0 + kMaxHostThreads, [](bool x) { return x; })) {
+    rows_ok = false;
+    why = "fluid links to the ghost node 0";
+  }
   std::vector<int32_t> wall_slot(n_nodes, -1);
   std::vector<int32_t> walls;
-  for (int32_t i = 0; i < n_bulk; ++i) {
-    const int32_t n = bulk[order[i]];
-    const int32_t* nb = neighbor + static_cast<size_t>(n) * Q;
-    for (int q = 0; q < Q - 1; ++q) {
-      const int32_t s = nb[rev_of(c->lattice, q)];
-      uint32_t l;
-      if (compact[s] >= 0) {
-        l = (kLinkStream << 30) | static_cast<uint32_t>(compact[s]);
-      } else if (node_type[s] > 1) {
-        return fail(c, LB_ERR_UNSUPPORTED,
-                    "bulk node %d has a fluid neighbour %d on another rank: halo exchange is not available on one GPU", n, s);
-      } else if (s == 0) {
-        if (!(flags & LB_TOPO_ALLOW_GHOST0))
-          return fail(c, LB_ERR_GHOST0_LINK,
-                      "bulk node %d links to the ghost node 0 (direction %d); make the geometry periodic or walled, "
-                      "or pass LB_TOPO_ALLOW_GHOST0 to reproduce the reference's shared-slot result", n, q);
-        l = (kLinkGhost0 << 30) | static_cast<uint32_t>(winner[rev_of(c->lattice, q)]);
-        rows_ok = false;
-        why = "fluid links to the ghost node 0";
-      } else {
-        if (wall_slot[s] < 0) {
-          wall_slot[s] = static_cast<int32_t>(walls.size());
-          walls.push_back(s);
-        }
-        l = (kLinkBounce << 30) | static_cast<uint32_t>(wall_slot[s]);
+  for (int32_t i = 0; i < n_bulk; ++i)
+    for (uint32_t wd = wall_dirs[i]; wd != 0u; wd &= wd - 1u) {
+      const int q = __builtin_ctz(wd);
+      uint32_t& l = link[static_cast<size_t>(q) * NP + i];
+      const int32_t s = static_cast<int32_t>(l & kLinkIndexMask);
+      if (wall_slot[s] < 0) {
+        wall_slot[s] = static_cast<int32_t>(walls.size());
+        walls.push_back(s);
       }
-      link[static_cast<size_t>(q) * NP + i] = l;
+      l = (kLinkBounce << 30) | static_cast<uint32_t>(wall_slot[s]);
     }
-  }
   // a layout is used only when every link it yields is the table's (checked
   // over chunks of the compact nodes on host threads)
   auto agrees = [&](auto&& source) {
