@@ -16,6 +16,7 @@
 
 #include "../../include/badchimp_amd.h"
 #include "errors.hpp"
+#include "host_threads.hpp"
 #include "lattice.hpp"
 
 using lbamd::fail_global;
@@ -290,78 +291,90 @@ int build_from_geometry(int lattice, int nd, const int32_t* dims, const int32_t*
   Padded P(pd);
   const int64_t M = P.size;
   std::vector<int32_t> g(M, -1), label(M, 0);
-  // interior copy + per-rank labels in C order (vtklb.py:84-94)
+  // Interior cells in C order, one run along the last axis at a time: the
+  // padded index of the first cell of interior run r (the last axis has
+  // stride 1).
+  int64_t n_in = 1;
+  for (int d = 0; d < nd; ++d) n_in *= dims[d];
+  const int32_t nlast = dims[nd - 1];
+  const int64_t n_runs = n_in / nlast;
+  auto run_start = [&](int64_t r) {
+    int64_t flat = 1;
+    for (int d = nd - 2; d >= 0; --d) {
+      flat += (r % dims[d] + 1) * P.stride[d];
+      r /= dims[d];
+    }
+    return flat;
+  };
+  // interior copy + per-rank labels in C order (vtklb.py:84-94); the own
+  // fluid cells in C order (== label order) come out of the same pass
+  // (vtklb.py:184-190), since the rim copies below touch no interior cell
   int max_rank = 0;
+  std::vector<int64_t> nodes;
   {
-    std::vector<int32_t> idx(nd, 0);
-    int64_t n_in = 1;
-    for (int d = 0; d < nd; ++d) n_in *= dims[d];
     std::vector<int32_t> counter(1, 0);
-    for (int64_t i = 0; i < n_in; ++i) {
-      int64_t rem = i, flat = 0;
-      for (int d = nd - 1; d >= 0; --d) {
-        idx[d] = static_cast<int32_t>(rem % dims[d]);
-        rem /= dims[d];
-      }
-      for (int d = 0; d < nd; ++d) flat += (idx[d] + 1) * P.stride[d];
-      const int32_t v = geo[i];
-      if (v < 0) return fail_global(LB_ERR_ARG, "geometry values must be >= 0");
-      g[flat] = v;
-      if (v > 0) {
-        if (v >= static_cast<int32_t>(counter.size())) counter.resize(v + 1, 0);
-        label[flat] = ++counter[v];
-        max_rank = std::max(max_rank, v);
+    const int32_t* src = geo;
+    for (int64_t r = 0; r < n_runs; ++r) {
+      const int64_t flat0 = run_start(r);
+      for (int32_t k = 0; k < nlast; ++k) {
+        const int32_t v = *src++;
+        if (v < 0) return fail_global(LB_ERR_ARG, "geometry values must be >= 0");
+        const int64_t flat = flat0 + k;
+        g[flat] = v;
+        if (v > 0) {
+          if (v >= static_cast<int32_t>(counter.size())) counter.resize(v + 1, 0);
+          label[flat] = ++counter[v];
+          max_rank = std::max(max_rank, v);
+          if (v == rank) nodes.push_back(flat);
+        }
       }
     }
   }
   if (rank < 1 || rank > max_rank) return fail_global(LB_ERR_ARG, "rank %d not present in the geometry", rank);
-  // periodic rim copies, axis by axis (vtklb.py:97-114)
+  // periodic rim copies, axis by axis (vtklb.py:97-114): the layer k = 0 of
+  // the axis takes layer n - 2, then layer n - 1 takes layer 1; each layer is
+  // a run of stride[ax] cells per index of the axes before it
   for (int ax = 0; ax < nd; ++ax) {
     if (!(periodic & (1 << ax))) continue;
-    const int32_t n = pd[ax];
-    for (int64_t i = 0; i < M; ++i) {
-      const int32_t k = static_cast<int32_t>((i / P.stride[ax]) % n);
-      if (k == 0) {
-        const int64_t src = i + (n - 2) * P.stride[ax];
-        g[i] = g[src];
-        label[i] = label[src];
-      }
-    }
-    for (int64_t i = 0; i < M; ++i) {
-      const int32_t k = static_cast<int32_t>((i / P.stride[ax]) % n);
-      if (k == n - 1) {
-        const int64_t src = i - (n - 2) * P.stride[ax];
-        g[i] = g[src];
-        label[i] = label[src];
+    const int64_t n = pd[ax], inner = P.stride[ax], outer = M / (n * inner);
+    for (int pass = 0; pass < 2; ++pass) {
+      const int64_t dst_k = pass == 0 ? 0 : n - 1, src_k = pass == 0 ? n - 2 : 1;
+      for (int64_t o = 0; o < outer; ++o) {
+        const int64_t dst = (o * n + dst_k) * inner, src = (o * n + src_k) * inner;
+        std::copy(g.begin() + src, g.begin() + src + inner, g.begin() + dst);
+        std::copy(label.begin() + src, label.begin() + src + inner, label.begin() + dst);
       }
     }
   }
-  auto interior = [&](int64_t i) {
-    for (int d = 0; d < nd; ++d) {
-      const int32_t k = static_cast<int32_t>((i / P.stride[d]) % pd[d]);
-      if (k == 0 || k == pd[d] - 1) return false;
-    }
-    return true;
-  };
   std::vector<int64_t> shift(Q);
   for (int q = 0; q < Q; ++q) {
     shift[q] = 0;
     for (int d = 0; d < nd; ++d) shift[q] += lattice_c(lattice, q, d) * P.stride[d];
   }
-  // own fluid in C order (== label order), vtklb.py:184-190
-  std::vector<int64_t> nodes;
-  for (int64_t i = 0; i < M; ++i)
-    if (g[i] == rank && interior(i)) nodes.push_back(i);
   const size_t n_fluid = nodes.size();
   // solid cells reached from own fluid, C order of the padded array (vtklb.py:192-197)
+  // (host threads mark with relaxed byte stores; each notes the other ranks it meets)
   std::vector<uint8_t> mark(M, 0);
   std::set<int> nb_ranks;
-  for (size_t k = 0; k < n_fluid; ++k)
-    for (int q = 0; q < Q; ++q) {
-      const int64_t j = nodes[k] + shift[q];
-      if (g[j] == 0) mark[j] = 1;
-      if (g[j] > 0 && g[j] != rank) nb_ranks.insert(g[j]);
-    }
+  {
+    std::vector<std::set<int>> seen(lbamd::kMaxHostThreads);
+    uint8_t* mk = mark.data();
+    lbamd::parallel_ranges(static_cast<int64_t>(n_fluid), [&](int64_t b, int64_t e, int th) {
+      std::set<int>& sn = seen[th];
+      int32_t last = 0;
+      for (int64_t k = b; k < e; ++k)
+        for (int q = 0; q < Q; ++q) {
+          const int64_t j = nodes[k] + shift[q];
+          const int32_t v = g[j];
+          if (v == 0) __atomic_store_n(mk + j, uint8_t{1}, __ATOMIC_RELAXED);
+          else if (v > 0 && v != rank && v != last) {
+            sn.insert(v);
+            last = v;
+          }
+        }
+    });
+    for (const auto& sn : seen) nb_ranks.insert(sn.begin(), sn.end());
+  }
   for (int64_t i = 0; i < M; ++i)
     if (mark[i]) nodes.push_back(i);
   // other-rank cells, one block per neighbouring rank (vtklb.py:199-215);
@@ -390,30 +403,32 @@ int build_from_geometry(int lattice, int nd, const int32_t* dims, const int32_t*
   t.pos.assign(static_cast<size_t>(t.n_nodes) * nd, -1);
   t.neighbor.assign(static_cast<size_t>(t.n_nodes) * Q, 0);
   t.nodetype_attr.assign(t.n_nodes, 0);
-  for (int32_t n = 0; n < npts; ++n) {
-    const int64_t i = nodes[n];
-    std::vector<int32_t> idx(nd);
-    for (int d = 0; d < nd; ++d) {
-      idx[d] = static_cast<int32_t>((i / P.stride[d]) % pd[d]);
-      t.pos[static_cast<size_t>(n + 1) * nd + d] = idx[d] - 1;
-    }
-    // neighbour index (vtklb.py:23-39)
-    for (int q = 0; q < Q; ++q) {
-      bool inside = true;
+  // positions and neighbour index (vtklb.py:23-39), node by node on host threads
+  lbamd::parallel_ranges(npts, [&](int64_t b, int64_t e, int) {
+    int32_t idx[3];
+    for (int64_t n = b; n < e; ++n) {
+      const int64_t i = nodes[n];
       for (int d = 0; d < nd; ++d) {
-        const int32_t k = idx[d] + lattice_c(lattice, q, d);
-        inside = inside && k >= 0 && k < pd[d];
+        idx[d] = static_cast<int32_t>((i / P.stride[d]) % pd[d]);
+        t.pos[static_cast<size_t>(n + 1) * nd + d] = idx[d] - 1;
       }
-      int32_t v = 0;
-      if (inside) {
-        const int64_t j = i + shift[q];
-        v = local[j];
-        if (v == 0 && g[j] == rank) v = label[j];
+      for (int q = 0; q < Q; ++q) {
+        bool inside = true;
+        for (int d = 0; d < nd; ++d) {
+          const int32_t k = idx[d] + lattice_c(lattice, q, d);
+          inside = inside && k >= 0 && k < pd[d];
+        }
+        int32_t v = 0;
+        if (inside) {
+          const int64_t j = i + shift[q];
+          v = local[j];
+          if (v == 0 && g[j] == rank) v = label[j];
+        }
+        t.neighbor[static_cast<size_t>(n + 1) * Q + q] = v;
       }
-      t.neighbor[static_cast<size_t>(n + 1) * Q + q] = v;
+      t.nodetype_attr[n + 1] = g[i] > 0 ? 1 : 0;
     }
-    t.nodetype_attr[n + 1] = g[i] > 0 ? 1 : 0;
-  }
+  });
   // PROCESSOR blocks (vtklb.py:271-281)
   for (int r : nb_ranks) {
     std::vector<std::pair<int32_t, int32_t>> pairs;
