@@ -241,3 +241,33 @@ def test_binary_twin_of_built_topology_and_rejections(tmp_path):
     open(bad, "wb").write(b"XXXXXXXX" + raw[8:])
     with pytest.raises(lbamd.LBError):  # not a binary twin
         lbamd.Topology.from_binary(bad, lbamd.D3Q19)
+
+
+@pytest.mark.parametrize("lattice,shape,periodic", [
+    ("D3Q19", (144, 64, 64), "xz"), ("D2Q9", (960, 640), "xy"),
+])
+def test_native_builder_large_multirank_matches_oracle(lattice, shape, periodic):
+    """Geometries large enough that the native builder's passes run on host
+    threads (over 64 k fluid nodes per rank), three ranks in blocks along the
+    first two axes so that every rank meets two others: every rank's node
+    list, positions, owner ranks, node types and bulk list equal the oracle's
+    restatement of the reference writer."""
+    rng = np.random.default_rng(shape[0])
+    fluid = rng.random(shape) > 0.3
+    x = np.arange(shape[0])[:, None] * 3 // shape[0]
+    y = np.arange(shape[1])[None, :] * 2 // shape[1]
+    owner = np.where(x < 2, x + 1, 3) + 0 * y
+    owner = np.where((x == 1) & (y == 1), 3, owner)
+    if len(shape) == 3:
+        owner = owner[:, :, None]
+    geo = np.where(fluid, owner, 0).astype(np.int32)
+    L = lbamd.LATTICE_NAME[lattice]
+    for r in (1, 2, 3):
+        built = lbamd.Topology.from_geometry(geo, L, periodic, rank=r)
+        T = OracleTopo(geo, lattice, periodic, rank=r)
+        assert (T.node_types > 0).sum() > 65536
+        assert np.array_equal(built.neighbors(), T.neighbors)
+        assert np.array_equal(built.positions(), T.positions)
+        assert np.array_equal(built.node_ranks(), T.ranks)
+        assert np.array_equal(built.node_types(), T.node_types)
+        assert np.array_equal(built.bulk(), T.bulk)
