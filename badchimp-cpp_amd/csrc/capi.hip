@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <string>
@@ -693,24 +694,29 @@ int lb_upload_f(lb_ctx* c, int32_t nf, const double* f) {
   const int Q = c->Q;
   const int64_t NP = c->NP;
   const size_t count = static_cast<size_t>(nf) * Q * NP;
+  // the device layout of the populations staged on the host: every slot of
+  // the NP padded nodes written once, on host threads (padding nodes zero)
+  std::unique_ptr<double[]> h(new (std::nothrow) double[count]);
+  if (!h) return fail(c, LB_ERR_NOMEM, "lb_upload_f: no host staging memory");
+  double* hp = h.get();
+  parallel_ranges(NP, [&](int64_t b, int64_t e, int) {
+    for (int64_t i = b; i < e; ++i) {
+      if (i < c->n_bulk) {
+        const double* src = f + static_cast<size_t>(c->node_of[i]) * nf * Q;
+        for (int k = 0; k < nf; ++k)
+          for (int q = 0; q < Q; ++q) hp[static_cast<size_t>(nf) * pidx_host(Q, q, i) + k] = src[k * Q + q];
+      } else {
+        for (int k = 0; k < nf; ++k)
+          for (int q = 0; q < Q; ++q) hp[static_cast<size_t>(nf) * pidx_host(Q, q, i) + k] = 0.0;
+      }
+    }
+  });
   if (c->precision == LB_FP64) {
-    std::vector<double> h(count, 0.0);
-    for (int32_t i = 0; i < c->n_bulk; ++i) {
-      const double* src = f + static_cast<size_t>(c->node_of[i]) * nf * Q;
-      for (int k = 0; k < nf; ++k)
-        for (int q = 0; q < Q; ++q) h[static_cast<size_t>(nf) * pidx_host(Q, q, i) + k] = src[k * Q + q];
-    }
-    LB_HIP(c, hipMemcpy(c->d_f[0], h.data(), count * sizeof(double), hipMemcpyHostToDevice));
+    LB_HIP(c, hipMemcpy(c->d_f[0], hp, count * sizeof(double), hipMemcpyHostToDevice));
   } else {
-    std::vector<double> h(count, 0.0);
-    for (int32_t i = 0; i < c->n_bulk; ++i) {
-      const double* src = f + static_cast<size_t>(c->node_of[i]) * nf * Q;
-      for (int k = 0; k < nf; ++k)
-        for (int q = 0; q < Q; ++q) h[static_cast<size_t>(nf) * pidx_host(Q, q, i) + k] = src[k * Q + q];
-    }
     double* tmp = nullptr;
     LB_HIP(c, hipMalloc(&tmp, count * sizeof(double)));
-    hipError_t e = hipMemcpy(tmp, h.data(), count * sizeof(double), hipMemcpyHostToDevice);
+    hipError_t e = hipMemcpy(tmp, hp, count * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = store_lowp(c->precision, c->lattice, nf, tmp, c->d_f[0], c->n_bulk, NP, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     hipFree(tmp);
@@ -736,24 +742,36 @@ int lb_download_f(lb_ctx* c, int32_t nf, double* f) {
   const int Q = c->Q;
   const int64_t NP = c->NP;
   const size_t count = static_cast<size_t>(nf) * Q * NP;
-  std::vector<double> h(count);
+  std::unique_ptr<double[]> h(new (std::nothrow) double[count]);  // every element copied in below
+  if (!h) return fail(c, LB_ERR_NOMEM, "lb_download_f: no host staging memory");
+  double* hp = h.get();
+  {
+    // fault the staging pages in on host threads, not inside the copy
+    const size_t bytes = count * sizeof(double);
+    char* base = reinterpret_cast<char*>(hp);
+    parallel_ranges(static_cast<int64_t>((bytes + 4095) / 4096), [&](int64_t b, int64_t e, int) {
+      for (int64_t p = b; p < e; ++p) base[std::min(static_cast<size_t>(p) * 4096, bytes - 1)] = 0;
+    });
+  }
   if (c->precision == LB_FP64) {
-    LB_HIP(c, hipMemcpyAsync(h.data(), src, count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    LB_HIP(c, hipMemcpyAsync(hp, src, count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     LB_HIP(c, hipStreamSynchronize(c->stream));
   } else {
     double* tmp = nullptr;
     LB_HIP(c, hipMalloc(&tmp, count * sizeof(double)));
     hipError_t e = load_lowp(c->precision, c->lattice, nf, src, tmp, c->n_bulk, NP, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), tmp, count * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(hp, tmp, count * sizeof(double), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     hipFree(tmp);
     if (e != hipSuccess) return fail(c, LB_ERR_HIP, "download conversion: %s", hipGetErrorString(e));
   }
-  for (int32_t i = 0; i < c->n_bulk; ++i) {
-    double* dst = f + static_cast<size_t>(c->node_of[i]) * nf * Q;
-    for (int k = 0; k < nf; ++k)
-      for (int q = 0; q < Q; ++q) dst[k * Q + q] = h[static_cast<size_t>(nf) * pidx_host(Q, q, i) + k];
-  }
+  parallel_ranges(c->n_bulk, [&](int64_t b, int64_t e, int) {
+    for (int64_t i = b; i < e; ++i) {
+      double* dst = f + static_cast<size_t>(c->node_of[i]) * nf * Q;
+      for (int k = 0; k < nf; ++k)
+        for (int q = 0; q < Q; ++q) dst[k * Q + q] = hp[static_cast<size_t>(nf) * pidx_host(Q, q, i) + k];
+    }
+  });
   return LB_OK;
 }
 

@@ -23,6 +23,10 @@ cpu_baseline = the compiled reference std_case (oracle/_ref, built from the
            the same generator, from two differenced runs; "port" beside it is
            the oracle's OpenMP restatement of the same loop on the full 256^3
            node list.
+host_io  = the FP64 step with the populations handed over from host memory:
+           one lb_upload_f and one lb_download_f of every node's populations,
+           timed after the headline's timed region, and the rate of the same
+           K steps with those two transfers added (never the value).
 batched  = the ensemble of independent sphere-pack permeability solves (cubic
            sizes log-uniform in 32..128, porosity U(0.30, 0.50), radius
            U(3, 8), one seed; |dk|/k <= 1e-6 at two consecutive checks 100
@@ -180,7 +184,12 @@ def make_ctx(precision, nb, types, bulk, pos, layout):
     return ctx
 
 
-def gpu_leg(topo, nb, types, bulk, pos, layout, precision, steps, warmup, tau, force, tau_anti=None, table=None):
+def gpu_leg(topo, nb, types, bulk, pos, layout, precision, steps, warmup, tau, force, tau_anti=None, table=None,
+            host_io=None):
+    """Time `steps` steps with the populations resident.  With a dict in
+    host_io, also time (after the timed region) one lb_upload_f and one
+    lb_download_f of every node's FP64 populations, the host buffers a caller
+    hands over and gets back."""
     ctx = make_ctx(precision, nb, types, bulk, pos, layout)
     got_layout = {2: "row windows", 1: "rows"}.get(ctx.layout, "table")
     if ctx.layout_note and ctx.layout_note != got_layout:
@@ -209,6 +218,17 @@ def gpu_leg(topo, nb, types, bulk, pos, layout, precision, steps, warmup, tau, f
     ctx.sync()
     t1 = time.perf_counter()
     kms, launches = ctx.kernel_time()
+    if host_io is not None:
+        out = np.zeros(f0.size)
+        ctx.download_f(out=out)  # first call outside the timing
+        ctx.sync()
+        t2 = time.perf_counter()
+        ctx.upload_f(f0)
+        ctx.sync()
+        t3 = time.perf_counter()
+        ctx.download_f(out=out)
+        t4 = time.perf_counter()
+        host_io.update(bytes_each_way=int(f0.nbytes), upload_s=t3 - t2, download_s=t4 - t3)
     ctx.close()
     return t1 - t0, kms / max(launches, 1), got_layout
 
@@ -688,8 +708,17 @@ def main():
     t32, k32_ms, lay32 = gpu_leg(topo, nb, types, bulk, pos, args.layout, lbamd.FP32S, args.steps, args.warmup, tau, force)
     mlups32 = n_bulk * args.steps / t32 / 1e6
     log("FP32S: %.1f MLUPS, kernel %.3f ms, layout %s" % (mlups32, k32_ms, lay32))
-    t64, k64_ms, lay64 = gpu_leg(topo, nb, types, bulk, pos, args.layout, lbamd.FP64, args.steps, args.warmup, tau, force)
+    host_io = {}
+    t64, k64_ms, lay64 = gpu_leg(topo, nb, types, bulk, pos, args.layout, lbamd.FP64, args.steps, args.warmup, tau, force,
+                                 host_io=host_io)
     mlups64 = n_bulk * args.steps / t64 / 1e6
+    # the same K steps with the populations handed over from host memory and back
+    host_io["steps"] = args.steps
+    host_io["MLUPS_host_buffers"] = round(n_bulk * args.steps / (t64 + host_io["upload_s"] + host_io["download_s"]) / 1e6, 2)
+    host_io["GBs_upload"] = round(host_io["bytes_each_way"] / host_io["upload_s"] / 1e9, 2)
+    host_io["GBs_download"] = round(host_io["bytes_each_way"] / host_io["download_s"] / 1e9, 2)
+    log("FP64 with host buffers: %.1f MLUPS over %d steps (upload %.3f s, download %.3f s)" % (
+        host_io["MLUPS_host_buffers"], args.steps, host_io["upload_s"], host_io["download_s"]))
     log("FP64 : %.1f MLUPS, kernel %.3f ms, layout %s" % (mlups64, k64_ms, lay64))
     t16, k16_ms, _ = gpu_leg(topo, nb, types, bulk, pos, args.layout, lbamd.FP16S, args.steps, args.warmup, tau, force)
     mlups16 = n_bulk * args.steps / t16 / 1e6
@@ -780,6 +809,7 @@ def main():
                   "frac": round(n_bulk * 76 / (k16_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "bytes_per_lup": 76,
                   "traffic": profiled_traffic("FP16S") if args.size == 256 else None,
                   "k_rel_err_vs_fp64": k_err.get("fp16s")},
+        "host_io": host_io,
         "permeability": perm,
         "setup": setup,
         "trt": trt,
